@@ -1,0 +1,82 @@
+"""Build the native libraries in-tree.
+
+  libesdg_setup.so  host setup (basis, meshes, geometry, states); g++ only
+  libesdg.so        gfx950 HIP kernels + C ABI + deviceclass mirror; hipcc, links libesdg_setup
+  oracle/liboracle.so and oracle/_ref/ (test oracle; built by oracle/Makefile)
+
+Run as ``python -m esdgsem_mpiocca_amd.build`` or through ``__graft_entry__.build()``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+ORACLE = os.path.join(ROOT, "oracle")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("ESDG_OFFLOAD_ARCH", "gfx950")
+
+# gfx950 code only.  Packed FP32 instructions are disabled (a stability rule on this hardware
+# pool); FP contraction stays at the compiler default for device code.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+    "-munsafe-fp-atomics",
+]
+
+HIP_SOURCES = [
+    "esdg_runtime.hip",
+    "esdg_kernels.hip",
+    "esdg_volume.hip",
+    "esdg_fused.hip",
+    "esdg_deviceclass.hip",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_setup() -> str:
+    out = os.path.join(PKG, "libesdg_setup.so")
+    src = os.path.join(CSRC, "host", "esdg_setup.cpp")
+    _run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-Wall",
+          f"-I{INCLUDE}", src, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_hip() -> str:
+    out = os.path.join(PKG, "libesdg.so")
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    if not srcs:
+        raise RuntimeError("no HIP sources found")
+    cmd = [HIPCC] + HIP_FLAGS + [f"-I{INCLUDE}", f"-I{CSRC}"] + srcs + [
+        "-o", out + ".tmp", f"-L{PKG}", "-lesdg_setup", "-Wl,-rpath,$ORIGIN"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", ORACLE, "all"])
+
+
+def build_all() -> None:
+    build_setup()
+    build_hip()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "setup": build_setup, "hip": build_hip, "oracle": build_oracle}[w]()

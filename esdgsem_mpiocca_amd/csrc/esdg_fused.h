@@ -1,0 +1,40 @@
+// esdg_fused.h -- fused per-stage kernels (a faster schedule of the same per-stage arithmetic).
+#pragma once
+
+#include <string>
+
+#include "esdg_internal.h"
+
+namespace esdg {
+
+// Device pointers the fused stage reads and writes (owned by the deviceclass mirror).
+struct FusedArgs {
+    int Nelem, Nfaces, artificial_viscosity, positivity;
+    double *q, *gRK;
+    const double *Jac, *Yxi, *Yeta, *Xxi, *Xeta, *B, *Bx, *By, *D, *Dhat, *Dstrong, *VdmInv, *GLw, *EleSizes;
+    const double *nx, *ny, *scal, *bL, *bR;
+    const int *EdgeData, *EdgeReversed, *ElemToEdge, *EMS, *EOR;
+    double* ViscPara;   // element viscosity of the stage (written by the fused kernels)
+};
+
+class FusedStage {
+  public:
+    int init(esdg_ctx* ctx, int Nelem, int Nfaces, bool av);
+    int setup(esdg_ctx* ctx, const int* EdgeData, const int* ElemToEdge, const int* EMS, const int* EOR,
+              const double* Jac, const double* nx, const double* ny, const double* scal);
+    // One RK stage: residual, low-storage accumulate and update (gRK = rkA gRK + Qt; q += cdt gRK).
+    int stage(esdg_ctx* ctx, int s, double rkA, double cdt, const FusedArgs& a);
+    void release(esdg_ctx* ctx);
+    void* buffer(const std::string& name);
+
+    bool ready = false;
+    int Nelem = 0, Nfaces = 0;
+    bool av = false;
+    double* qn = nullptr;       // second state buffer (the fused stage writes q^{s+1} here)
+    double* fstar = nullptr;    // face fluxes
+    double* fgrad = nullptr;    // face gradient fluxes (AV)
+    double* gtr = nullptr;      // element-side traces of the scaled gradients (AV)
+    double* fvisc = nullptr;    // face viscous fluxes (AV)
+};
+
+}  // namespace esdg

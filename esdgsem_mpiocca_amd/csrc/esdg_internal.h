@@ -1,0 +1,76 @@
+// esdg_internal.h -- shared definitions of the gfx950 implementation behind include/esdg.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "esdg.h"
+
+#define ESDG_NEQ 3
+#define ESDG_MAX_NGL 10
+
+struct esdg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    esdg_params p{};
+    std::string err;
+    size_t bytes = 0;
+    std::map<void*, size_t> allocs;
+    double* d_scratch = nullptr;      // small device scratch (reduction partials)
+    size_t scratch_count = 0;
+    // profiling: per kernel name, (start,stop) event pairs not yet summed, and sums
+    bool profile = false;
+    struct Prof { std::vector<std::pair<hipEvent_t, hipEvent_t>> pending; int64_t launches = 0; double ms = 0; };
+    std::map<std::string, Prof> prof;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace esdg {
+
+int set_error(esdg_ctx* ctx, int code, const std::string& msg);
+int hip_check(esdg_ctx* ctx, hipError_t e, const char* what);
+
+// Records HIP events around one kernel launch when profiling is on.
+class KernelScope {
+  public:
+    KernelScope(esdg_ctx* ctx, const char* name);
+    ~KernelScope();
+  private:
+    esdg_ctx* ctx_;
+    const char* name_;
+    hipEvent_t start_ = nullptr;
+};
+
+// Checks the launch and reports through the context.
+int launch_check(esdg_ctx* ctx, const char* name);
+
+// Element / face counts are int32 (indices fit); byte offsets are 64-bit.
+inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace esdg
+
+#define ESDG_REQUIRE(ctx, cond, msg)                                        \
+    do {                                                                    \
+        if (!(cond)) return esdg::set_error((ctx), ESDG_ERR_ARG, (msg));    \
+    } while (0)
+
+// Dispatch on the runtime ngl to an ahead-of-time compiled template instance.
+#define ESDG_NGL_DISPATCH(ngl, FN, ...)              \
+    switch (ngl) {                                   \
+        case 2: FN<2>(__VA_ARGS__); break;           \
+        case 3: FN<3>(__VA_ARGS__); break;           \
+        case 4: FN<4>(__VA_ARGS__); break;           \
+        case 5: FN<5>(__VA_ARGS__); break;           \
+        case 6: FN<6>(__VA_ARGS__); break;           \
+        case 7: FN<7>(__VA_ARGS__); break;           \
+        case 8: FN<8>(__VA_ARGS__); break;           \
+        case 9: FN<9>(__VA_ARGS__); break;           \
+        case 10: FN<10>(__VA_ARGS__); break;         \
+        default: break;                              \
+    }

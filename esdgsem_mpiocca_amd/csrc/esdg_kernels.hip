@@ -1,0 +1,1026 @@
+// esdg_kernels.hip -- gfx950 kernels for every per-stage reference kernel except the volume
+// integrals (esdg_volume.hip).  One C-ABI entry per deviceclass kernel call, same argument order.
+//
+// Launch shapes (MI355X: wave64, 256 CUs):
+//  * face kernels: one thread per face node, 256-thread blocks; the face arrays are contiguous
+//    per face (f*ngl + k), so a wave reads/writes whole faces;
+//  * element kernels: a block holds EPB = 256 / ngl^2 elements, one thread per node, so loads of
+//    the node-major state (e*3*ngl^2 + v*ngl^2 + node) are contiguous per element;
+//  * RK kernels: flat element-wise streams with 16-byte accesses.
+// Per-element reductions (lambda max, sensor energies, limiter means) run in LDS in the order of
+// the reference's serial loops, so their rounding matches it.
+#include "esdg_internal.h"
+
+namespace esdg {
+
+// node index of trace point k of element side s (okl/GatherEdgeData/SolidWalls.okl:74-89)
+__device__ __forceinline__ int side_node(int ngl, int side, int k) {
+    switch (side) {
+        case 0: return k;
+        case 2: return ngl * ngl - ngl + k;
+        case 3: return k * ngl;
+        default: return (k + 1) * ngl - 1;
+    }
+}
+
+template <int NGL>
+struct ElemBlock {
+    static constexpr int NODES = NGL * NGL;
+    static constexpr int EPB = (256 / NODES) > 0 ? (256 / NODES) : 1;
+    static constexpr int THREADS = EPB * NODES;
+};
+
+// ---------------------------------------------------------------------------------------------
+// CollectEdgeData_Bottom: okl/GatherEdgeData/CollectEdgeData_Bottom.okl:16-134
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_collect_bottom(int Nfaces, int ngl, const int* __restrict__ ED,
+                                                        const double* __restrict__ B, double* __restrict__ bL,
+                                                        double* __restrict__ bR) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nfaces * ngl) return;
+    const int f = (int)(t / ngl), i = (int)(t % ngl);
+    const int* E = ED + 8 * (int64_t)f;
+    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+    const int ngl2 = ngl * ngl;
+    double l = 0.0, r = 0.0;
+    if (cpuL == 0) l = B[(int64_t)iel * ngl2 + side_node(ngl, isl, i)];
+    if (cpuR == 0) r = B[(int64_t)ier * ngl2 + side_node(ngl, isr, i)];
+    if (cpuL == -1) l = r;
+    if (cpuR == -1) r = l;
+    bL[t] = l;
+    bR[t] = r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// CollectEdgeData: okl/GatherEdgeData/SolidWalls.okl:16-207 (walls), InnerOuter.okl,
+// OceanBoundary.okl (exterior state variants)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void wall_mirror(double nx, double ny, double vx, double vy, double& ox, double& oy) {
+    const double v_n = nx * vx + ny * vy;
+    const double v_t = -ny * vx + nx * vy;
+    ox = -nx * v_n - ny * v_t;
+    oy = -ny * v_n + nx * v_t;
+}
+
+__global__ void __launch_bounds__(256) k_collect(int Nfaces, int ngl, int bc, double h_zero, const int* __restrict__ ED,
+                                                 const double* __restrict__ q, const double* __restrict__ nxa,
+                                                 const double* __restrict__ nya, const double* __restrict__ bLa,
+                                                 const double* __restrict__ bRa, double* __restrict__ qL,
+                                                 double* __restrict__ qR) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nfaces * ngl) return;
+    const int f = (int)(t / ngl), i = (int)(t % ngl);
+    const int* E = ED + 8 * (int64_t)f;
+    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+    const int ngl2 = ngl * ngl;
+    double L0 = 0.0, L1 = 0.0, L2 = 0.0, R0 = 0.0, R1 = 0.0, R2 = 0.0;
+    if (cpuL == 0) {
+        const int64_t id = (int64_t)iel * ngl2 * ESDG_NEQ + side_node(ngl, isl, i);
+        L0 = q[id]; L1 = q[id + ngl2]; L2 = q[id + 2 * ngl2];
+    }
+    if (cpuR == 0) {
+        const int64_t id = (int64_t)ier * ngl2 * ESDG_NEQ + side_node(ngl, isr, i);
+        R0 = q[id]; R1 = q[id + ngl2]; R2 = q[id + 2 * ngl2];
+    }
+    const double nx = nxa[t], ny = nya[t];
+    if (bc == ESDG_BC_WALLS) {
+        if (cpuL == -1) { L0 = R0; wall_mirror(nx, ny, R1, R2, L1, L2); }
+        if (cpuR == -1) { R0 = L0; wall_mirror(nx, ny, L1, L2, R1, R2); }
+    } else if (bc == ESDG_BC_INNER_OUTER) {
+        // okl/GatherEdgeData/InnerOuter.okl: exterior keeps h, momentum zero
+        if (cpuL == -1) { L0 = R0; L1 = 0.0; L2 = 0.0; }
+        if (cpuR == -1) { R0 = L0; R1 = 0.0; R2 = 0.0; }
+    } else {
+        // okl/GatherEdgeData/OceanBoundary.okl: open (still water h_zero - b) where b <= 6.7,
+        // reflecting wall elsewhere
+        if (cpuL == -1) {
+            const double b = bRa[t];
+            if (b <= 6.7) { L0 = h_zero - b; L1 = 0.0; L2 = 0.0; }
+            else { L0 = R0; wall_mirror(nx, ny, R1, R2, L1, L2); }
+        }
+        if (cpuR == -1) {
+            const double b = bLa[t];
+            if (b <= 6.7) { R0 = h_zero - b; R1 = 0.0; R2 = 0.0; }
+            else { R0 = L0; wall_mirror(nx, ny, L1, L2, R1, R2); }
+        }
+    }
+    const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
+    qL[eid] = L0; qL[eid + ngl] = L1; qL[eid + 2 * ngl] = L2;
+    qR[eid] = R0; qR[eid + ngl] = R1; qR[eid + 2 * ngl] = R2;
+}
+
+// ---------------------------------------------------------------------------------------------
+// calcNumFluxes.  ES: okl/RiemannSolvers/calcESFlux.okl:17-215; rotated ES:
+// okl/RiemannSolvers/calcESFluxRotated.okl:17-214.  The dissipation matrix M = R |S| R^T is
+// formed in the reference's order.
+// ---------------------------------------------------------------------------------------------
+struct FluxConsts {
+    double g, geomFace, tol;
+    int ES;
+};
+
+__device__ __forceinline__ void es_dissipation(double R21, double R22, double R23, double R31, double R32, double R33,
+                                               double SL11, double SL22, double SL33, double j1, double j2, double j3,
+                                               double& NF0, double& NF1, double& NF2) {
+    const double R11 = 1.0, R12 = 0.0, R13 = 1.0;
+    const double S11 = R11 * SL11, S12 = 0.0, S13 = R13 * SL33;
+    const double S21 = R21 * SL11, S22 = R22 * SL22, S23 = R23 * SL33;
+    const double S31 = R31 * SL11, S32 = R32 * SL22, S33 = R33 * SL33;
+    const double M11 = S11 * R11 + S12 * R12 + S13 * R13;
+    const double M12 = S11 * R21 + S12 * R22 + S13 * R23;
+    const double M13 = S11 * R31 + S12 * R32 + S13 * R33;
+    const double M21 = S21 * R11 + S22 * R12 + S23 * R13;
+    const double M22 = S21 * R21 + S22 * R22 + S23 * R23;
+    const double M23 = S21 * R31 + S22 * R32 + S23 * R33;
+    const double M31 = S31 * R11 + S32 * R12 + S33 * R13;
+    const double M32 = S31 * R21 + S32 * R22 + S33 * R23;
+    const double M33 = S31 * R31 + S32 * R32 + S33 * R33;
+    NF0 += -0.5 * (M11 * j1 + M12 * j2 + M13 * j3);
+    NF1 += -0.5 * (M21 * j1 + M22 * j2 + M23 * j3);
+    NF2 += -0.5 * (M31 * j1 + M32 * j2 + M33 * j3);
+}
+
+// One face node of the ES flux in Cartesian momentum; returns F* (unscaled).
+__device__ __forceinline__ void flux_es(const FluxConsts& c, double nx, double ny, double hL, double huL, double hvL,
+                                        double hR, double huR, double hvR, double bL, double bR, double& NF0,
+                                        double& NF1, double& NF2) {
+    const double g = c.g;
+    double uL, vL, uR, vR;
+    if (hL >= c.tol) { uL = huL / hL; vL = hvL / hL; } else { hL = 0.0; uL = 0.0; vL = 0.0; }
+    if (hR >= c.tol) { uR = huR / hR; vR = hvR / hR; } else { hR = 0.0; uR = 0.0; vR = 0.0; }
+    const double avgH = 0.5 * (hL + hR);
+    const double avgH2 = 0.5 * (hL * hL + hR * hR);
+    const double avgU = 0.5 * (uL + uR);
+    const double avgV = 0.5 * (vL + vR);
+    const double normV = avgU * nx + avgV * ny;
+    NF0 = avgH * normV;
+    NF1 = avgH * avgU * normV + (0.5 * g * avgH2) * nx;
+    NF2 = avgH * avgV * normV + (0.5 * g * avgH2) * ny;
+    if (c.ES) {
+        const double cL = g * hL, cR = g * hR;
+        const double avgC = sqrt(fabs(0.5 * (cL + cR)));
+        const double q1L = g * (hL + bL) - 0.5 * uL * uL - 0.5 * vL * vL;
+        const double q1R = g * (hR + bR) - 0.5 * uR * uR - 0.5 * vR * vR;
+        const double SL11 = 0.5 / g * fabs(nx * avgU + ny * avgV + avgC);
+        const double SL22 = fabs(avgH * (nx * avgU + ny * avgV));
+        const double SL33 = 0.5 / g * fabs(nx * avgU + ny * avgV - avgC);
+        es_dissipation(avgU + nx * avgC, ny, avgU - nx * avgC, avgV + ny * avgC, nx, avgV - ny * avgC, SL11, SL22, SL33,
+                       q1R - q1L, uR - uL, vR - vL, NF0, NF1, NF2);
+    }
+}
+
+// One face node of the rotated ES flux; returns F* with momentum rotated back.
+__device__ __forceinline__ void flux_es_rotated(const FluxConsts& c, double nx, double ny, double hL, double huL,
+                                                double hvL, double hR, double huR, double hvR, double bL, double bR,
+                                                double& NF0, double& NF1, double& NF2) {
+    const double g = c.g;
+    double uL, vL, uR, vR;
+    if (hL > c.tol) { uL = huL / hL; vL = hvL / hL; } else { uL = 0.0; vL = 0.0; }
+    if (hR > c.tol) { uR = huR / hR; vR = hvR / hR; } else { uR = 0.0; vR = 0.0; }
+    const double vLn = nx * uL + ny * vL, vLt = -ny * uL + nx * vL;
+    const double vRn = nx * uR + ny * vR, vRt = -ny * uR + nx * vR;
+    const double avgH = 0.5 * (hL + hR);
+    const double avgH2 = 0.5 * (hL * hL + hR * hR);
+    const double avgU = 0.5 * (vLn + vRn);
+    const double avgV = 0.5 * (vLt + vRt);
+    double F0 = avgH * avgU;
+    double F1 = avgH * avgU * avgU + (0.5 * g * avgH2);
+    double F2 = avgH * avgV * avgU;
+    if (c.ES) {
+        const double cL = g * hL, cR = g * hR;
+        const double avgC = sqrt(fabs(0.5 * (cL + cR)));
+        const double q1L = g * (hL + bL) - 0.5 * vLn * vLn - 0.5 * vLt * vLt;
+        const double q1R = g * (hR + bR) - 0.5 * vRn * vRn - 0.5 * vRt * vRt;
+        const double SL11 = 0.5 / g * fabs(avgU + avgC);
+        const double SL22 = fabs(avgH * avgU);
+        const double SL33 = 0.5 / g * fabs(avgU - avgC);
+        es_dissipation(avgU + avgC, 0.0, avgU - avgC, avgV, 1.0, avgV, SL11, SL22, SL33, q1R - q1L, vRn - vLn,
+                       vRt - vLt, F0, F1, F2);
+    }
+    NF0 = F0;
+    NF1 = nx * F1 - ny * F2;
+    NF2 = ny * F1 + nx * F2;
+}
+
+// Local Lax-Friedrichs: okl/RiemannSolvers/calcLaxFriedrich.okl:17-170.  Dry sides keep h but
+// zero momentum and wave speed; the dissipation uses max(c+|u|, c+|v|) over both sides.
+__device__ __forceinline__ void flux_lf(const FluxConsts& c, double nx, double ny, double hL, double huL, double hvL,
+                                        double hR, double huR, double hvR, double& NF0, double& NF1, double& NF2) {
+    const double g = c.g;
+    double uL = 0.0, vL = 0.0, cL = 0.0, uR = 0.0, vR = 0.0, cR = 0.0;
+    if (hL > c.tol) { uL = huL / hL; vL = hvL / hL; cL = sqrt(g * hL); } else { huL = 0.0; hvL = 0.0; }
+    if (hR > c.tol) { uR = huR / hR; vR = hvR / hR; cR = sqrt(g * hR); } else { huR = 0.0; hvR = 0.0; }
+    const double F2L = huL * uL + 0.5 * g * hL * hL, F2R = huR * uR + 0.5 * g * hR * hR;
+    const double G2L = huL * vL, G2R = huR * vR;
+    const double F3L = huL * vL, F3R = huR * vR;
+    const double G3L = hvL * vL + 0.5 * g * hL * hL, G3R = hvR * vR + 0.5 * g * hR * hR;
+    const double maxL1 = fmax(cL + fabs(uL), cR + fabs(uR));
+    const double maxL2 = fmax(cL + fabs(vL), cR + fabs(vR));
+    const double lam = fmax(maxL1, maxL2);
+    const double FF0 = 0.5 * (huL + huR), FF1 = 0.5 * (F2L + F2R), FF2 = 0.5 * (F3L + F3R);
+    const double GG0 = 0.5 * (hvL + hvR), GG1 = 0.5 * (G2L + G2R), GG2 = 0.5 * (G3L + G3R);
+    NF0 = (nx * FF0 + ny * GG0) - 0.5 * lam * (hR - hL);
+    NF1 = (nx * FF1 + ny * GG1) - 0.5 * lam * (huR - huL);
+    NF2 = (nx * FF2 + ny * GG2) - 0.5 * lam * (hvR - hvL);
+}
+
+__global__ void __launch_bounds__(256) k_num_fluxes(int Nfaces, int ngl, int kind, FluxConsts c,
+                                                    const int* __restrict__ ER, const double* __restrict__ nxa,
+                                                    const double* __restrict__ nya, const double* __restrict__ scal,
+                                                    const double* __restrict__ qL, const double* __restrict__ qR,
+                                                    const double* __restrict__ BL, const double* __restrict__ BR,
+                                                    double* __restrict__ SP) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nfaces * ngl) return;
+    const int f = (int)(t / ngl), i = (int)(t % ngl);
+    const int rv = ER[f];
+    const int ir = (1 - rv) * (ngl - 1 - i) + rv * i;
+    const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
+    const int64_t eidR = (int64_t)f * ngl * ESDG_NEQ + ir;
+    const int64_t xidR = (int64_t)f * ngl + ir;
+    const double nx = nxa[t], ny = nya[t], sc = scal[t];
+    double F0, F1, F2;
+    const double hL = qL[eid], huL = qL[eid + ngl], hvL = qL[eid + 2 * ngl];
+    const double hR = qR[eidR], huR = qR[eidR + ngl], hvR = qR[eidR + 2 * ngl];
+    if (kind == ESDG_FLUX_ES_ROTATED)
+        flux_es_rotated(c, nx, ny, hL, huL, hvL, hR, huR, hvR, BL[t], BR[xidR], F0, F1, F2);
+    else if (kind == ESDG_FLUX_LF)
+        flux_lf(c, nx, ny, hL, huL, hvL, hR, huR, hvR, F0, F1, F2);
+    else
+        flux_es(c, nx, ny, hL, huL, hvL, hR, huR, hvR, BL[t], BR[xidR], F0, F1, F2);
+    SP[eid] = sc * c.geomFace * F0;
+    SP[eid + ngl] = sc * c.geomFace * F1;
+    SP[eid + 2 * ngl] = sc * c.geomFace * F2;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Surface lifts.  okl/DG/SurfaceKernel.okl:17-142, okl/BR1_Gradient/SurfaceKernelGradient.okl,
+// okl/ViscoseParts/SurfaceKernelVisc.okl.  Thread (element, k) applies side 0 and 2 (phase 1), then
+// after a block barrier sides 1 and 3 (phase 2): the corner nodes receive the side-0/2 term first,
+// as in the reference.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int surf_node(int ngl, int side, int k) {
+    switch (side) {
+        case 0: return k;
+        case 2: return ngl * ngl - ngl + k;
+        case 1: return k * ngl + ngl - 1;
+        default: return k * ngl;
+    }
+}
+
+enum SurfKind { SURF_MAIN = 0, SURF_GRAD = 1, SURF_VISC = 2 };
+
+template <int KIND>
+__global__ void __launch_bounds__(256) k_surface(int Nelem, int ngl, const double* __restrict__ Jac,
+                                                 const int* __restrict__ EMS, const int* __restrict__ EOR,
+                                                 const int* __restrict__ E2E, const double* __restrict__ F,
+                                                 const double* __restrict__ G, const double* __restrict__ DB1,
+                                                 const double* __restrict__ DB2, double* __restrict__ out0,
+                                                 double* __restrict__ out1) {
+    const int epb = blockDim.x / ngl;
+    const int el = threadIdx.x / ngl, k = threadIdx.x % ngl;
+    const int e = blockIdx.x * epb + el;
+    const bool active = el < epb && e < Nelem;
+    const int ngl2 = ngl * ngl;
+    for (int phase = 0; phase < 2; ++phase) {
+        if (active) {
+            for (int ss = 0; ss < 2; ++ss) {
+                const int s = phase == 0 ? 2 * ss : 2 * ss + 1;   // phase 0: sides 0,2; phase 1: sides 1,3
+                const int ifa = E2E[e * 4 + s] - 1;
+                const int rev = EOR[e * 4 + s];
+                const int ms = EMS[e * 4 + s];
+                const int inc = rev * k + (1 - rev) * (ngl - 1 - k);
+                const int nd = surf_node(ngl, s, k);
+                const double rJ = Jac[(int64_t)e * ngl2 + nd];
+                if (KIND == SURF_MAIN) {
+                    const int64_t fl = (int64_t)ifa * ngl * ESDG_NEQ + inc;
+                    const int64_t ex = (int64_t)ifa * ngl + inc;
+                    double* o = out0 + (int64_t)e * ngl2 * ESDG_NEQ + nd;
+                    o[0] += -ms * F[fl] * rJ;
+                    o[ngl2] += -ms * rJ * F[fl + ngl] + rJ * DB1[ex];
+                    o[2 * ngl2] += -ms * rJ * F[fl + 2 * ngl] + rJ * DB2[ex];
+                } else if (KIND == SURF_GRAD) {
+                    const int64_t fl = (int64_t)ifa * ngl * 2 + inc;
+                    const int64_t id = (int64_t)e * ngl2 * 2 + nd;
+                    if (phase == 0) out0[id] += ms * F[fl] * rJ;
+                    else out0[id] += ms * rJ * F[fl];
+                    out0[id + ngl2] += ms * rJ * F[fl + ngl];
+                    out1[id] += ms * rJ * G[fl];
+                    out1[id + ngl2] += ms * rJ * G[fl + ngl];
+                } else {
+                    const int64_t fl = (int64_t)ifa * ngl * 2 + inc;
+                    double* o = out0 + (int64_t)e * ngl2 * ESDG_NEQ + nd;
+                    o[ngl2] += -ms * rJ * F[fl];
+                    o[2 * ngl2] += -ms * rJ * F[fl + ngl];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FindLambdaMax: okl/DG/FindLambdaMax.okl:17-100
+// ---------------------------------------------------------------------------------------------
+template <int NGL>
+__global__ void __launch_bounds__(256) k_lambda_max(int Nelem, double g, double tol, const double* __restrict__ q,
+                                                    double* __restrict__ LM) {
+    using EB = ElemBlock<NGL>;
+    __shared__ double s[EB::THREADS];
+    const int el = threadIdx.x / EB::NODES, n = threadIdx.x % EB::NODES;
+    const int e = blockIdx.x * EB::EPB + el;
+    double lam = 0.0;
+    if (e < Nelem) {
+        const int64_t id = (int64_t)e * EB::NODES * ESDG_NEQ + n;
+        const double h = q[id];
+        const double Qinv = (h > tol) ? 1.0 / h : 0.0;
+        const double c = sqrt(g * h);
+        const double u = q[id + EB::NODES] * Qinv;
+        const double v = q[id + 2 * EB::NODES] * Qinv;
+        lam = c + fmax(fabs(u), fabs(v));
+    }
+    s[threadIdx.x] = lam;
+    __syncthreads();
+    if (threadIdx.x < EB::EPB) {
+        const int ee = blockIdx.x * EB::EPB + threadIdx.x;
+        if (ee < Nelem) {
+            double m = 0.0;
+            for (int k = 0; k < EB::NODES; ++k) m = fmax(m, s[threadIdx.x * EB::NODES + k]);
+            LM[ee] = m;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Low-storage and SSP Runge-Kutta updates: okl/RungeKutta/rkLS.okl:17-89,
+// UpdateKernelLS.okl:17-78, UpdateKernelSSP_V1.okl:17-98
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_calc_rk(int64_t n2, const double2* __restrict__ Qt, double rkA,
+                                                 double2* __restrict__ gRK) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = Qt[i];
+        double2 r = gRK[i];
+        r.x = rkA * r.x + a.x;
+        r.y = rkA * r.y + a.y;
+        gRK[i] = r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_update_ls(int64_t n2, double cdt, const double2* __restrict__ gRK,
+                                                   double2* __restrict__ q) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = gRK[i];
+        double2 r = q[i];
+        r.x += cdt * a.x;
+        r.y += cdt * a.y;
+        q[i] = r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_update_ssp(int64_t n2, double A, double Bc, double cdt,
+                                                    const double2* __restrict__ Qt, const double2* __restrict__ qT,
+                                                    double2* __restrict__ q) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = Qt[i], t = qT[i];
+        double2 r = q[i];
+        r.x = A * t.x + Bc * r.x + cdt * a.x;
+        r.y = A * t.y + Bc * r.y + cdt * a.y;
+        q[i] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// ShockCapturing: okl/ShockCapturing/ShockCapturing.okl:17-234.  Node (a = xi, c = eta).
+// ---------------------------------------------------------------------------------------------
+struct SensorConsts {
+    double ZeroTOL, eps0, sigmaMin, sigmaMax;
+};
+
+template <int NGL>
+__device__ __forceinline__ double sensor_from_modes(const double* m, const SensorConsts& c) {
+    constexpr int N = NGL - 1;
+    double z1 = 0.0, n1 = 0.0, z2 = 0.0, n2 = 0.0;
+    for (int i = 0; i < N; ++i) z1 += m[i * NGL + N] * m[i * NGL + N] + m[N * NGL + i] * m[N * NGL + i];
+    z1 += m[N * NGL + N] * m[N * NGL + N];
+    for (int i = 0; i < N - 1; ++i)
+        z2 += m[i * NGL + N - 1] * m[i * NGL + N - 1] + m[(N - 1) * NGL + i] * m[(N - 1) * NGL + i];
+    z2 += m[(N - 1) * NGL + N - 1] * m[(N - 1) * NGL + N - 1];
+    for (int i = 0; i < NGL; ++i)
+        for (int j = 0; j < NGL; ++j) n1 += m[i * NGL + j] * m[i * NGL + j];
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) n2 += m[i * NGL + j] * m[i * NGL + j];
+    const double t1 = (n1 < c.ZeroTOL) ? 0.0 : z1 / n1;
+    const double t2 = (n1 < c.ZeroTOL) ? 0.0 : z2 / n2;   // the reference guards term 2 with nenner1
+    const double sigma = log10(fmax(t1, t2));
+    if (c.sigmaMin >= sigma) return 0.0;
+    if (c.sigmaMax >= sigma) {
+        const double PI = 3.141592653589793;
+        const double delta = 1.0 + sin(PI * (sigma - 0.5 * (c.sigmaMax + c.sigmaMin)) / (c.sigmaMax - c.sigmaMin));
+        return c.eps0 * 0.5 * delta;
+    }
+    return c.eps0;
+}
+
+template <int NGL>
+__global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts sc, const double* __restrict__ q,
+                                                         const double* __restrict__ b, const double* __restrict__ VdmInv,
+                                                         double* __restrict__ visc, double* __restrict__ visc_plot) {
+    using EB = ElemBlock<NGL>;
+    __shared__ double sV[NGL * NGL];
+    __shared__ double sU[EB::THREADS];
+    __shared__ double sM[EB::THREADS];
+    const int el = threadIdx.x / EB::NODES, n = threadIdx.x % EB::NODES;
+    const int e = blockIdx.x * EB::EPB + el;
+    const int a = n / NGL, cc = n % NGL;   // thread's (xi, eta) slot in the modal arrays
+    if (threadIdx.x < NGL * NGL) sV[threadIdx.x] = VdmInv[threadIdx.x];
+    if (e < Nelem) {
+        // slot (a, cc) holds node (xi = a, eta = cc), i.e. node index cc*NGL + a
+        const int node = cc * NGL + a;
+        sU[threadIdx.x] = b[(int64_t)e * EB::NODES + node] + q[(int64_t)e * EB::NODES * ESDG_NEQ + node];
+    }
+    __syncthreads();
+    double* U = sU + el * EB::NODES;
+    double* M = sM + el * EB::NODES;
+    double s = 0.0;
+    if (e < Nelem) {
+#pragma unroll
+        for (int l = 0; l < NGL; ++l) s += sV[a * NGL + l] * U[l * NGL + cc];
+    }
+    __syncthreads();
+    if (e < Nelem) M[n] = s;
+    __syncthreads();
+    s = 0.0;
+    if (e < Nelem) {
+#pragma unroll
+        for (int l = 0; l < NGL; ++l) s += M[a * NGL + l] * sV[cc * NGL + l];
+    }
+    __syncthreads();
+    if (e < Nelem) U[n] = s;   // modal coefficients m[a][cc]
+    __syncthreads();
+    if (threadIdx.x < EB::EPB) {
+        const int ee = blockIdx.x * EB::EPB + threadIdx.x;
+        if (ee < Nelem) {
+            const double v = sensor_from_modes<NGL>(sU + threadIdx.x * EB::NODES, sc);
+            visc[ee] = v;
+            if (visc_plot) visc_plot[ee] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// PosPres: okl/Positivity/PosPres.okl:17-223
+// ---------------------------------------------------------------------------------------------
+template <int NGL>
+__global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double zero_tol,
+                                                  const double* __restrict__ EleSizes, const double* __restrict__ GLw,
+                                                  const double* __restrict__ Jac, double* __restrict__ q) {
+    using EB = ElemBlock<NGL>;
+    __shared__ double sH[EB::THREADS], sHU[EB::THREADS], sHV[EB::THREADS], sJ[EB::THREADS];
+    __shared__ double sAvg[EB::EPB][5];   // hAvg, huAvg, hvAvg, theta, hMin
+    const int el = threadIdx.x / EB::NODES, n = threadIdx.x % EB::NODES;
+    const int e = blockIdx.x * EB::EPB + el;
+    const int64_t id = (int64_t)e * EB::NODES * ESDG_NEQ + n;
+    if (e < Nelem) {
+        sH[threadIdx.x] = q[id];
+        sHU[threadIdx.x] = q[id + EB::NODES];
+        sHV[threadIdx.x] = q[id + 2 * EB::NODES];
+        sJ[threadIdx.x] = Jac[(int64_t)e * EB::NODES + n];
+    }
+    __syncthreads();
+    if (threadIdx.x < EB::EPB) {
+        const int ee = blockIdx.x * EB::EPB + threadIdx.x;
+        if (ee < Nelem) {
+            const double* H = sH + threadIdx.x * EB::NODES;
+            const double* HU = sHU + threadIdx.x * EB::NODES;
+            const double* HV = sHV + threadIdx.x * EB::NODES;
+            const double* JJ = sJ + threadIdx.x * EB::NODES;
+            double hmin = 0.0, ha = 0.0, hua = 0.0, hva = 0.0;
+            for (int l = 0; l < NGL; ++l)
+                for (int k = 0; k < NGL; ++k) {
+                    const int nn = k * NGL + l;
+                    const double w = GLw[l] * GLw[k] / JJ[nn];
+                    ha += H[nn] * w;
+                    hua += HU[nn] * w;
+                    hva += HV[nn] * w;
+                    hmin = fmin(hmin, H[nn]);
+                }
+            const double size = EleSizes[ee];
+            ha = ha / size;
+            if (fabs(ha) < zero_tol) ha = 0.0;
+            sAvg[threadIdx.x][0] = ha;
+            sAvg[threadIdx.x][1] = hua / size;
+            sAvg[threadIdx.x][2] = hva / size;
+            sAvg[threadIdx.x][3] = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
+            sAvg[threadIdx.x][4] = hmin;
+        }
+    }
+    __syncthreads();
+    if (e < Nelem) {
+        const double ha = sAvg[el][0], hua = sAvg[el][1], hva = sAvg[el][2], th = sAvg[el][3], hmin = sAvg[el][4];
+        const double h = sH[threadIdx.x];
+        if (hmin < 0.0) {
+            double hn = 0.0, hun = 0.0, hvn = 0.0;
+            if (ha > 0.0) {
+                hn = fabs(th * (h - ha) + ha);
+                if (hn > tol) {
+                    hun = th * (sHU[threadIdx.x] - hua) + hua;
+                    hvn = th * (sHV[threadIdx.x] - hva) + hva;
+                }
+            }
+            if (ha < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
+            q[id] = hn;
+            q[id + EB::NODES] = hun;
+            q[id + 2 * EB::NODES] = hvn;
+        } else if (h < tol) {
+            q[id + EB::NODES] = 0.0;
+            q[id + 2 * EB::NODES] = 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// BR1 gradient chain: okl/BR1_Gradient/*.okl
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_num_fluxes_gradient(int Nfaces, int ngl, double tol, double geomFace,
+                                                             const int* __restrict__ ED, const double* __restrict__ nxa,
+                                                             const double* __restrict__ nya,
+                                                             const double* __restrict__ scal,
+                                                             const double* __restrict__ qL,
+                                                             const double* __restrict__ qR, double* __restrict__ SGX,
+                                                             double* __restrict__ SGY) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nfaces * ngl) return;
+    const int f = (int)(t / ngl), i = (int)(t % ngl);
+    const int cpuL = ED[8 * (int64_t)f + 5], cpuR = ED[8 * (int64_t)f + 6];
+    const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
+    const int64_t gid = (int64_t)f * ngl * 2 + i;
+    const double nx = nxa[t], ny = nya[t], sc = scal[t];
+    const double* LL = (cpuL == -1) ? qR : qL;
+    const double* RR = (cpuR == -1) ? qL : qR;
+    const double hL = LL[eid], huL = LL[eid + ngl], hvL = LL[eid + 2 * ngl];
+    const double hR = RR[eid], huR = RR[eid + ngl], hvR = RR[eid + 2 * ngl];
+    const double iL = (hL > tol) ? 1.0 / hL : 0.0;
+    const double iR = (hR > tol) ? 1.0 / hR : 0.0;
+    const double uL = huL * iL, uR = huR * iR, vL = hvL * iL, vR = hvR * iR;
+    SGX[gid] = nx * sc * geomFace * 0.5 * (uL + uR);
+    SGX[gid + ngl] = nx * sc * geomFace * 0.5 * (vL + vR);
+    SGY[gid] = ny * sc * geomFace * 0.5 * (uL + uR);
+    SGY[gid + ngl] = ny * sc * geomFace * 0.5 * (vL + vR);
+}
+
+// okl/BR1_Gradient/calcGradient.okl:18-156; thread slot n = c*NGL + a is node (xi=a, eta=c)
+template <int NGL>
+__global__ void __launch_bounds__(256) k_calc_gradient(int Nelem, double tol, const double* __restrict__ Jac,
+                                                       const double* __restrict__ Yxi, const double* __restrict__ Yeta,
+                                                       const double* __restrict__ Xxi, const double* __restrict__ Xeta,
+                                                       const double* __restrict__ q, const double* __restrict__ Dh,
+                                                       double* __restrict__ gX, double* __restrict__ gY) {
+    using EB = ElemBlock<NGL>;
+    __shared__ double sD[NGL * NGL];
+    __shared__ double su[EB::THREADS], sv[EB::THREADS];
+    const int el = threadIdx.x / EB::NODES, n = threadIdx.x % EB::NODES;
+    const int e = blockIdx.x * EB::EPB + el;
+    if (threadIdx.x < NGL * NGL) sD[threadIdx.x] = Dh[threadIdx.x];
+    if (e < Nelem) {
+        const int64_t id = (int64_t)e * EB::NODES * ESDG_NEQ + n;
+        const double h = q[id];
+        const double Qinv = (h > tol) ? 1.0 / h : 0.0;
+        su[threadIdx.x] = q[id + EB::NODES] * Qinv;
+        sv[threadIdx.x] = q[id + 2 * EB::NODES] * Qinv;
+    }
+    __syncthreads();
+    if (e >= Nelem) return;
+    const int a = n % NGL, c = n / NGL;
+    const double* u = su + el * EB::NODES;
+    const double* v = sv + el * EB::NODES;
+    double u_xi = 0.0, u_eta = 0.0, v_xi = 0.0, v_eta = 0.0;
+#pragma unroll
+    for (int l = 0; l < NGL; ++l) {
+        u_xi += sD[a * NGL + l] * u[c * NGL + l];
+        u_eta += sD[c * NGL + l] * u[l * NGL + a];
+        v_xi += sD[a * NGL + l] * v[c * NGL + l];
+        v_eta += sD[c * NGL + l] * v[l * NGL + a];
+    }
+    const int64_t xid = (int64_t)e * EB::NODES + n;
+    const double rJ = Jac[xid];
+    const double u_x = Yeta[xid] * u_xi - Yxi[xid] * u_eta;
+    const double v_x = Yeta[xid] * v_xi - Yxi[xid] * v_eta;
+    const double u_y = -Xeta[xid] * u_xi + Xxi[xid] * u_eta;
+    const double v_y = -Xeta[xid] * v_xi + Xxi[xid] * v_eta;
+    const int64_t gid = (int64_t)e * EB::NODES * 2 + n;
+    gX[gid] = rJ * u_x;
+    gX[gid + EB::NODES] = rJ * v_x;
+    gY[gid] = rJ * u_y;
+    gY[gid + EB::NODES] = rJ * v_y;
+}
+
+__global__ void __launch_bounds__(256) k_scale_gradient(int64_t nodes, int ngl2, const double* __restrict__ q,
+                                                        double* __restrict__ gX, double* __restrict__ gY) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nodes) return;
+    const int64_t e = t / ngl2, n = t % ngl2;
+    const double h = q[e * ngl2 * ESDG_NEQ + n];
+    const int64_t id = e * ngl2 * 2 + n;
+    gX[id] = h * gX[id];
+    gX[id + ngl2] = h * gX[id + ngl2];
+    gY[id] = h * gY[id];
+    gY[id + ngl2] = h * gY[id + ngl2];
+}
+
+// okl/BR1_Gradient/CollectEdgeDataGradient.okl:17-194.  The right-side traces of boundary faces
+// (unset in the reference, never used because the viscous flux vanishes there) are written as 0.
+__global__ void __launch_bounds__(256) k_collect_gradient(int Nfaces, int ngl, const int* __restrict__ ED,
+                                                          const double* __restrict__ gX, const double* __restrict__ gY,
+                                                          const double* __restrict__ visc, double* __restrict__ VL,
+                                                          double* __restrict__ VR, double* __restrict__ gXL,
+                                                          double* __restrict__ gXR, double* __restrict__ gYL,
+                                                          double* __restrict__ gYR) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nfaces * ngl) return;
+    const int f = (int)(t / ngl), i = (int)(t % ngl);
+    const int* E = ED + 8 * (int64_t)f;
+    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], rv = E[4], cpuL = E[5], cpuR = E[6];
+    const int ngl2 = ngl * ngl;
+    const int inc = (rv == 0) ? (ngl - 1) - i : i;
+    double XL0 = 0, XL1 = 0, XR0 = 0, XR1 = 0, YL0 = 0, YL1 = 0, YR0 = 0, YR1 = 0;
+    double vl = 0.0, vr = 0.0;
+    if (cpuL == 0) {
+        const int64_t id = (int64_t)iel * ngl2 * 2 + side_node(ngl, isl, i);
+        XL0 = gX[id]; XL1 = gX[id + ngl2]; YL0 = gY[id]; YL1 = gY[id + ngl2];
+        vl = visc[iel];
+    }
+    if (cpuR == 0) {
+        const int64_t id = (int64_t)ier * ngl2 * 2 + side_node(ngl, isr, inc);
+        XR0 = gX[id]; XR1 = gX[id + ngl2]; YR0 = gY[id]; YR1 = gY[id + ngl2];
+        vr = visc[ier];
+    }
+    if (cpuL == -1) { vl = visc[ier]; vr = visc[ier]; XL0 = 0.0; XL1 = 0.0; }
+    if (cpuR == -1) { vl = visc[iel]; vr = visc[iel]; XL0 = 0.0; XL1 = 0.0; }
+    if (i == 0) { VL[f] = vl; VR[f] = vr; }
+    const int64_t eid = (int64_t)f * ngl * 2 + i;
+    gXL[eid] = XL0; gXL[eid + ngl] = XL1;
+    gXR[eid] = XR0; gXR[eid + ngl] = XR1;
+    gYL[eid] = YL0; gYL[eid + ngl] = YL1;
+    gYR[eid] = YR0; gYR[eid + ngl] = YR1;
+}
+
+// okl/ViscoseParts/VolumeKernelViscose.okl:17-150; slot n = c*NGL + a is node (xi=a, eta=c)
+template <int NGL>
+__global__ void __launch_bounds__(256) k_volume_viscose(int Nelem, const double* __restrict__ Jac,
+                                                        const double* __restrict__ Yxi, const double* __restrict__ Yeta,
+                                                        const double* __restrict__ Xxi, const double* __restrict__ Xeta,
+                                                        const double* __restrict__ gX, const double* __restrict__ gY,
+                                                        const double* __restrict__ Ds, const double* __restrict__ visc,
+                                                        double* __restrict__ Qt) {
+    using EB = ElemBlock<NGL>;
+    __shared__ double sD[NGL * NGL];
+    __shared__ double sF2[EB::THREADS], sF3[EB::THREADS], sG2[EB::THREADS], sG3[EB::THREADS];
+    const int el = threadIdx.x / EB::NODES, n = threadIdx.x % EB::NODES;
+    const int e = blockIdx.x * EB::EPB + el;
+    if (threadIdx.x < NGL * NGL) sD[threadIdx.x] = Ds[threadIdx.x];
+    const int64_t xid = (int64_t)e * EB::NODES + n;
+    if (e < Nelem) {
+        const double eps = visc[e];
+        const int64_t id = (int64_t)e * EB::NODES * 2 + n;
+        const double f2 = eps * gX[id], f3 = eps * gX[id + EB::NODES];
+        const double g2 = eps * gY[id], g3 = eps * gY[id + EB::NODES];
+        const double ye = Yeta[xid], yx = Yxi[xid], xe = Xeta[xid], xx = Xxi[xid];
+        sF2[threadIdx.x] = ye * f2 - xe * g2;
+        sG2[threadIdx.x] = -yx * f2 + xx * g2;
+        sF3[threadIdx.x] = ye * f3 - xe * g3;
+        sG3[threadIdx.x] = -yx * f3 + xx * g3;
+    }
+    __syncthreads();
+    if (e >= Nelem) return;
+    const int a = n % NGL, c = n / NGL;
+    const double* F2 = sF2 + el * EB::NODES;
+    const double* F3 = sF3 + el * EB::NODES;
+    const double* G2 = sG2 + el * EB::NODES;
+    const double* G3 = sG3 + el * EB::NODES;
+    double d2 = 0.0, d3 = 0.0;
+#pragma unroll
+    for (int l = 0; l < NGL; ++l) {
+        d2 += sD[a * NGL + l] * F2[c * NGL + l];
+        d2 += sD[c * NGL + l] * G2[l * NGL + a];
+        d3 += sD[a * NGL + l] * F3[c * NGL + l];
+        d3 += sD[c * NGL + l] * G3[l * NGL + a];
+    }
+    const double rJ = Jac[xid];
+    double* o = Qt + (int64_t)e * EB::NODES * ESDG_NEQ + n;
+    o[EB::NODES] += rJ * d2;
+    o[2 * EB::NODES] += rJ * d3;
+}
+
+// okl/ViscoseParts/calcNumFluxesViscose.okl:17-107
+__global__ void __launch_bounds__(256) k_num_fluxes_viscose(int Nfaces, int ngl, double geomFace,
+                                                            const int* __restrict__ ED, const double* __restrict__ nxa,
+                                                            const double* __restrict__ nya,
+                                                            const double* __restrict__ scal,
+                                                            const double* __restrict__ VL, const double* __restrict__ VR,
+                                                            const double* __restrict__ gXL,
+                                                            const double* __restrict__ gXR,
+                                                            const double* __restrict__ gYL,
+                                                            const double* __restrict__ gYR, double* __restrict__ SPV) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nfaces * ngl) return;
+    const int f = (int)(t / ngl), i = (int)(t % ngl);
+    const int cpuL = ED[8 * (int64_t)f + 5], cpuR = ED[8 * (int64_t)f + 6];
+    const int64_t eid = (int64_t)f * ngl * 2 + i;
+    const double nx = nxa[t], ny = nya[t], sc = scal[t];
+    const double f2L = nx * gXL[eid] + ny * gYL[eid];
+    const double f2R = nx * gXR[eid] + ny * gYR[eid];
+    const double f3L = nx * gXL[eid + ngl] + ny * gYL[eid + ngl];
+    const double f3R = nx * gXR[eid + ngl] + ny * gYR[eid + ngl];
+    double N0 = 0.0, N1 = 0.0;
+    if (!(cpuL == -1 || cpuR == -1)) {
+        const double m = fmax(VL[f], VR[f]);
+        N0 = 0.5 * m * (f2L + f2R);
+        N1 = 0.5 * m * (f3L + f3R);
+    }
+    SPV[eid] = -sc * geomFace * N0;
+    SPV[eid + ngl] = -sc * geomFace * N1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------------------------
+template <int NGL>
+void launch_lambda(esdg_ctx* ctx, int Nelem, const double* q, double* LM) {
+    using EB = ElemBlock<NGL>;
+    k_lambda_max<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.g, ctx->p.PosPresTOL, q, LM);
+}
+template <int NGL>
+void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* V, double* visc, double* vp) {
+    using EB = ElemBlock<NGL>;
+    SensorConsts sc{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
+    k_shock_capturing<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, sc, q, B, V, visc, vp);
+}
+template <int NGL>
+void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q) {
+    using EB = ElemBlock<NGL>;
+    k_pos_pres<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.PosPresTOL, ctx->p.ZeroTOL,
+                                                                                  sizes, GLw, Jac, q);
+}
+template <int NGL>
+void launch_gradient(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta, const double* Xxi,
+                     const double* Xeta, const double* q, const double* Dh, double* gX, double* gY) {
+    using EB = ElemBlock<NGL>;
+    k_calc_gradient<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.PosPresTOL, Jac, Yxi, Yeta,
+                                                                                       Xxi, Xeta, q, Dh, gX, gY);
+}
+template <int NGL>
+void launch_viscose(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta, const double* Xxi,
+                    const double* Xeta, const double* gX, const double* gY, const double* Ds, const double* visc,
+                    double* Qt) {
+    using EB = ElemBlock<NGL>;
+    k_volume_viscose<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, Jac, Yxi, Yeta, Xxi, Xeta, gX,
+                                                                                        gY, Ds, visc, Qt);
+}
+
+}  // namespace esdg
+
+using namespace esdg;
+
+#define ESDG_ENTRY(name) KernelScope ks__(ctx, name)
+
+extern "C" {
+
+int esdg_collect_edge_data_bottom(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const double* B, double* bL, double* bR) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeData && B && bL && bR, "esdg_collect_edge_data_bottom: bad argument");
+    if (Nfaces == 0) return ESDG_OK;
+    ESDG_ENTRY("collect_edge_data_bottom");
+    const int ngl = ctx->p.ngl;
+    k_collect_bottom<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(Nfaces, ngl, EdgeData, B, bL, bR);
+    return launch_check(ctx, "collect_edge_data_bottom");
+}
+
+int esdg_collect_edge_data(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const double* q, const double* x,
+                           const double* y, const double* nx, const double* ny, const double* bL, const double* bR,
+                           double* qL, double* qR, double t) {
+    (void)x; (void)y; (void)t;
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeData && q && nx && ny && qL && qR, "esdg_collect_edge_data: bad argument");
+    ESDG_REQUIRE(ctx, ctx->p.bc_kind != ESDG_BC_OCEAN || (bL && bR), "esdg_collect_edge_data: ocean boundary needs bL, bR");
+    ESDG_REQUIRE(ctx, ctx->p.bc_kind >= 0 && ctx->p.bc_kind <= 2, "esdg_collect_edge_data: unknown boundary kind");
+    if (Nfaces == 0) return ESDG_OK;
+    ESDG_ENTRY("collect_edge_data");
+    const int ngl = ctx->p.ngl;
+    k_collect<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(Nfaces, ngl, ctx->p.bc_kind, ctx->p.h_zero,
+                                                                                EdgeData, q, nx, ny, bL, bR, qL, qR);
+    return launch_check(ctx, "collect_edge_data");
+}
+
+int esdg_calc_num_fluxes(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const int* EdgeReversed, const double* nx,
+                         const double* ny, const double* scal, const double* qL, const double* qR, const double* bL,
+                         const double* bR, double* SurfaceParts) {
+    (void)EdgeData;
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeReversed && nx && ny && scal && qL && qR && bL && bR && SurfaceParts,
+                 "esdg_calc_num_fluxes: bad argument");
+    const int kind = ctx->p.num_flux;
+    ESDG_REQUIRE(ctx, kind == ESDG_FLUX_ES || kind == ESDG_FLUX_ES_ROTATED || kind == ESDG_FLUX_LF,
+                 "esdg_calc_num_fluxes: num_flux must be 0 (ES), 1 (LF) or 3 (rotated ES)");
+    if (Nfaces == 0) return ESDG_OK;
+    ESDG_ENTRY("calc_num_fluxes");
+    const int ngl = ctx->p.ngl;
+    FluxConsts c{ctx->p.g, ctx->p.geomFace, ctx->p.PosPresTOL, ctx->p.ES};
+    k_num_fluxes<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(Nfaces, ngl, kind, c, EdgeReversed, nx, ny,
+                                                                                   scal, qL, qR, bL, bR, SurfaceParts);
+    return launch_check(ctx, "calc_num_fluxes");
+}
+
+static int surface_common(esdg_ctx* ctx, int kind, const char* name, int Nelem, const double* Jac, const int* EMS,
+                          const int* EOR, const int* E2E, const double* F, const double* G, const double* DB1,
+                          const double* DB2, double* o0, double* o1) {
+    if (Nelem == 0) return ESDG_OK;
+    KernelScope ks(ctx, name);
+    const int ngl = ctx->p.ngl;
+    const int epb = 256 / ngl;
+    const unsigned nb = blocks_for(Nelem, epb);
+    if (kind == SURF_MAIN)
+        k_surface<SURF_MAIN><<<nb, epb * ngl, 0, ctx->stream>>>(Nelem, ngl, Jac, EMS, EOR, E2E, F, G, DB1, DB2, o0, o1);
+    else if (kind == SURF_GRAD)
+        k_surface<SURF_GRAD><<<nb, epb * ngl, 0, ctx->stream>>>(Nelem, ngl, Jac, EMS, EOR, E2E, F, G, DB1, DB2, o0, o1);
+    else
+        k_surface<SURF_VISC><<<nb, epb * ngl, 0, ctx->stream>>>(Nelem, ngl, Jac, EMS, EOR, E2E, F, G, DB1, DB2, o0, o1);
+    return launch_check(ctx, name);
+}
+
+int esdg_surface_kernel(esdg_ctx* ctx, int Nelem, const double* Jac, const int* EMS, const int* EOR, const int* E2E,
+                        const double* FStar, const double* DBSurf1, const double* DBSurf2, double* Qt) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Jac && EMS && EOR && E2E && FStar && DBSurf1 && DBSurf2 && Qt,
+                 "esdg_surface_kernel: bad argument");
+    return surface_common(ctx, SURF_MAIN, "surface_kernel", Nelem, Jac, EMS, EOR, E2E, FStar, nullptr, DBSurf1, DBSurf2, Qt,
+                          nullptr);
+}
+
+int esdg_find_lambda_max(esdg_ctx* ctx, int Nelem, const double* q, double* LambdaMax) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && q && LambdaMax, "esdg_find_lambda_max: bad argument");
+    if (Nelem == 0) return ESDG_OK;
+    ESDG_ENTRY("find_lambda_max");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_lambda, ctx, Nelem, q, LambdaMax);
+    return launch_check(ctx, "find_lambda_max");
+}
+
+static unsigned stream_blocks(int64_t n2) { return (unsigned)std::min<int64_t>((n2 + 255) / 256, 256 * 64); }
+
+int esdg_calc_rk(esdg_ctx* ctx, int Nelem, const double* Qt, double rkA, double rkB, double* gRK) {
+    (void)rkB;
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Qt && gRK, "esdg_calc_rk: bad argument");
+    const int64_t n = (int64_t)Nelem * ctx->p.ngl * ctx->p.ngl * ESDG_NEQ;
+    if (n == 0) return ESDG_OK;
+    ESDG_REQUIRE(ctx, n % 2 == 0, "esdg_calc_rk: odd state length");
+    ESDG_REQUIRE(ctx, ((uintptr_t)Qt % 16) == 0 && ((uintptr_t)gRK % 16) == 0, "esdg_calc_rk: arrays must be 16-byte aligned");
+    ESDG_ENTRY("calc_rk");
+    k_calc_rk<<<stream_blocks(n / 2), 256, 0, ctx->stream>>>(n / 2, (const double2*)Qt, rkA, (double2*)gRK);
+    return launch_check(ctx, "calc_rk");
+}
+
+int esdg_update_kernel_ls(esdg_ctx* ctx, int Nelem, double rkC, double dt, const double* gRK, double* q) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && gRK && q, "esdg_update_kernel_ls: bad argument");
+    const int64_t n = (int64_t)Nelem * ctx->p.ngl * ctx->p.ngl * ESDG_NEQ;
+    if (n == 0) return ESDG_OK;
+    ESDG_REQUIRE(ctx, n % 2 == 0, "esdg_update_kernel_ls: odd state length");
+    ESDG_REQUIRE(ctx, ((uintptr_t)q % 16) == 0 && ((uintptr_t)gRK % 16) == 0, "esdg_update_kernel_ls: arrays must be 16-byte aligned");
+    ESDG_ENTRY("update_kernel_ls");
+    k_update_ls<<<stream_blocks(n / 2), 256, 0, ctx->stream>>>(n / 2, rkC * dt, (const double2*)gRK, (double2*)q);
+    return launch_check(ctx, "update_kernel_ls");
+}
+
+int esdg_update_kernel_ssp(esdg_ctx* ctx, int Nelem, double rkA, double rkB, double rkC, double dt, const double* Qt,
+                           const double* qTmp, double* q) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Qt && qTmp && q, "esdg_update_kernel_ssp: bad argument");
+    const int64_t n = (int64_t)Nelem * ctx->p.ngl * ctx->p.ngl * ESDG_NEQ;
+    if (n == 0) return ESDG_OK;
+    ESDG_REQUIRE(ctx, n % 2 == 0, "esdg_update_kernel_ssp: odd state length");
+    ESDG_REQUIRE(ctx, ((uintptr_t)q % 16) == 0 && ((uintptr_t)Qt % 16) == 0 && ((uintptr_t)qTmp % 16) == 0,
+                 "esdg_update_kernel_ssp: arrays must be 16-byte aligned");
+    ESDG_ENTRY("update_kernel_ssp");
+    k_update_ssp<<<stream_blocks(n / 2), 256, 0, ctx->stream>>>(n / 2, rkA, rkB, rkC * dt, (const double2*)Qt,
+                                                                 (const double2*)qTmp, (double2*)q);
+    return launch_check(ctx, "update_kernel_ssp");
+}
+
+int esdg_shock_capturing(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* VdmInv,
+                         const double* EleSizes, double* ViscPara, double* ViscForPlot) {
+    (void)EleSizes;   // passed but unused by the reference kernel as well
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && q && B && VdmInv && ViscPara, "esdg_shock_capturing: bad argument");
+    ESDG_REQUIRE(ctx, ctx->p.ngl >= 3, "esdg_shock_capturing: needs ngl >= 3");
+    if (Nelem == 0) return ESDG_OK;
+    ESDG_ENTRY("shock_capturing");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_sensor, ctx, Nelem, q, B, VdmInv, ViscPara, ViscForPlot);
+    return launch_check(ctx, "shock_capturing");
+}
+
+int esdg_pos_pres(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && EleSizes && GLw && Jac && q, "esdg_pos_pres: bad argument");
+    if (Nelem == 0) return ESDG_OK;
+    ESDG_ENTRY("pos_pres");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q);
+    return launch_check(ctx, "pos_pres");
+}
+
+int esdg_calc_num_fluxes_gradient(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const double* nx, const double* ny,
+                                  const double* scal, const double* qL, const double* qR, double* SGX, double* SGY) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeData && nx && ny && scal && qL && qR && SGX && SGY,
+                 "esdg_calc_num_fluxes_gradient: bad argument");
+    if (Nfaces == 0) return ESDG_OK;
+    ESDG_ENTRY("calc_num_fluxes_gradient");
+    const int ngl = ctx->p.ngl;
+    k_num_fluxes_gradient<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(
+        Nfaces, ngl, ctx->p.PosPresTOL, ctx->p.geomFace, EdgeData, nx, ny, scal, qL, qR, SGX, SGY);
+    return launch_check(ctx, "calc_num_fluxes_gradient");
+}
+
+int esdg_calc_gradient(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta,
+                       const double* Xxi, const double* Xeta, const double* q, const double* B, const double* Dhat,
+                       double* gX, double* gY) {
+    (void)B;   // passed but unused by the reference kernel as well
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Jac && Yxi && Yeta && Xxi && Xeta && q && Dhat && gX && gY,
+                 "esdg_calc_gradient: bad argument");
+    if (Nelem == 0) return ESDG_OK;
+    ESDG_ENTRY("calc_gradient");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_gradient, ctx, Nelem, Jac, Yxi, Yeta, Xxi, Xeta, q, Dhat, gX, gY);
+    return launch_check(ctx, "calc_gradient");
+}
+
+int esdg_surface_kernel_gradient(esdg_ctx* ctx, int Nelem, const double* Jac, const int* EMS, const int* EOR,
+                                 const int* E2E, const double* SGX, const double* SGY, double* gX, double* gY) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Jac && EMS && EOR && E2E && SGX && SGY && gX && gY,
+                 "esdg_surface_kernel_gradient: bad argument");
+    return surface_common(ctx, SURF_GRAD, "surface_kernel_gradient", Nelem, Jac, EMS, EOR, E2E, SGX, SGY, nullptr,
+                          nullptr, gX, gY);
+}
+
+int esdg_scale_gradient(esdg_ctx* ctx, int Nelem, const double* q, double* gX, double* gY) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && q && gX && gY, "esdg_scale_gradient: bad argument");
+    const int ngl2 = ctx->p.ngl * ctx->p.ngl;
+    const int64_t nodes = (int64_t)Nelem * ngl2;
+    if (nodes == 0) return ESDG_OK;
+    ESDG_ENTRY("scale_gradient");
+    k_scale_gradient<<<blocks_for(nodes, 256), 256, 0, ctx->stream>>>(nodes, ngl2, q, gX, gY);
+    return launch_check(ctx, "scale_gradient");
+}
+
+int esdg_collect_edge_data_gradient(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const double* gX, const double* gY,
+                                    const double* ViscPara, double* VL, double* VR, double* gXL, double* gXR,
+                                    double* gYL, double* gYR) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeData && gX && gY && ViscPara && VL && VR && gXL && gXR && gYL && gYR,
+                 "esdg_collect_edge_data_gradient: bad argument");
+    if (Nfaces == 0) return ESDG_OK;
+    ESDG_ENTRY("collect_edge_data_gradient");
+    const int ngl = ctx->p.ngl;
+    k_collect_gradient<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(Nfaces, ngl, EdgeData, gX, gY,
+                                                                                         ViscPara, VL, VR, gXL, gXR, gYL,
+                                                                                         gYR);
+    return launch_check(ctx, "collect_edge_data_gradient");
+}
+
+int esdg_volume_kernel_viscose(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta,
+                               const double* Xxi, const double* Xeta, const double* gX, const double* gY,
+                               const double* Dstrong, const double* ViscPara, double* Qt) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Jac && Yxi && Yeta && Xxi && Xeta && gX && gY && Dstrong && ViscPara && Qt,
+                 "esdg_volume_kernel_viscose: bad argument");
+    if (Nelem == 0) return ESDG_OK;
+    ESDG_ENTRY("volume_kernel_viscose");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_viscose, ctx, Nelem, Jac, Yxi, Yeta, Xxi, Xeta, gX, gY, Dstrong, ViscPara, Qt);
+    return launch_check(ctx, "volume_kernel_viscose");
+}
+
+int esdg_calc_num_fluxes_viscose(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const double* nx, const double* ny,
+                                 const double* scal, const double* VL, const double* VR, const double* gXL,
+                                 const double* gXR, const double* gYL, const double* gYR, double* SPV) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeData && nx && ny && scal && VL && VR && gXL && gXR && gYL && gYR && SPV,
+                 "esdg_calc_num_fluxes_viscose: bad argument");
+    if (Nfaces == 0) return ESDG_OK;
+    ESDG_ENTRY("calc_num_fluxes_viscose");
+    const int ngl = ctx->p.ngl;
+    k_num_fluxes_viscose<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(
+        Nfaces, ngl, ctx->p.geomFace, EdgeData, nx, ny, scal, VL, VR, gXL, gXR, gYL, gYR, SPV);
+    return launch_check(ctx, "calc_num_fluxes_viscose");
+}
+
+int esdg_surface_kernel_visc(esdg_ctx* ctx, int Nelem, const double* Jac, const int* EMS, const int* EOR, const int* E2E,
+                             const double* SPV, double* Qt) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Jac && EMS && EOR && E2E && SPV && Qt, "esdg_surface_kernel_visc: bad argument");
+    return surface_common(ctx, SURF_VISC, "surface_kernel_visc", Nelem, Jac, EMS, EOR, E2E, SPV, nullptr, nullptr, nullptr,
+                          Qt, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+// esdg_volume.hip -- volume integrals on gfx950.
+//
+// Flux-differencing volume term (okl/DG/VolumeKernelFluxDiffV99.okl:17-218):
+//   Qt(i,j) = -J^-1 [ sum_l D(j,l) F#(u_ij, u_il) + sum_l D(i,l) G#(u_ij, u_lj) ] - g h (0, Bx, By)
+// with i the eta row and j the xi column.  The two-point fluxes are symmetric in their two
+// states, so each unordered pair of nodes on a line is evaluated once and added to both nodes'
+// sums.  The pairs are visited so that every node still accumulates its terms in increasing l,
+// as in the reference's serial loop.
+//
+// Work mapping (wave64, one wave per block): lane = (element el, line r), EPW = 64 / ngl elements
+// per wave.  Row phase: the lane owns eta-row r of its element, holds the row's 7 node fields in
+// registers (loaded straight from HBM: 8 contiguous doubles per array) and performs all xi-pairs.
+// It parks h, hu, hv, u, v in LDS.  Column phase: the same lane takes xi-column r, reads the column
+// from LDS, loads the xi-metrics of that column from HBM (one 64-byte segment per element and row),
+// continues the sums of its column nodes with the eta-pairs, and stores Qt.  The row sums cross
+// over through LDS.  Per element the kernel reads q, the four metrics, 1/J, Bx, By once and writes
+// Qt once: 13 doubles per node, the compulsory traffic.
+#include "esdg_internal.h"
+
+namespace esdg {
+
+template <int NGL>
+struct VolCfg {
+    static constexpr int EPW = 64 / NGL;        // elements per wave
+    static constexpr int NN = NGL * NGL;
+    static constexpr int RS = NGL + 1;          // LDS row stride (padding against bank conflicts)
+    static constexpr int PLANE = NGL * RS;      // one field of one element in LDS
+    static constexpr int ESZ = 5 * PLANE;       // h, hu, hv, u, v
+};
+
+// Symmetric two-point EC flux along one line (F# of the reference for metrics (Y, X) = (Yeta, Xeta);
+// G# = -F# evaluated with (Yxi, Xxi)).  Operands per node: h, hu, hv, u = hu/h, v = hv/h, h^2.
+struct PairOut {
+    double f1, f2, f3;
+};
+
+__device__ __forceinline__ PairOut ec_pair(double fourth_g, double half_g, double hL, double huL, double hvL,
+                                           double uL, double vL, double h2L, double YL, double XL, double hR,
+                                           double huR, double hvR, double uR, double vR, double h2R, double YR,
+                                           double XR) {
+    const double s = hL + hR;
+    const double h_A = fourth_g * s * s;
+    const double h2_A = 0.5 * (h2L + h2R);
+    const double hu_A = 0.5 * (huL + huR);
+    const double hv_A = 0.5 * (hvL + hvR);
+    const double u_A = 0.5 * (uL + uR);
+    const double v_A = 0.5 * (vL + vR);
+    const double Y_A = 0.5 * (YL + YR);
+    const double X_A = 0.5 * (XL + XR);
+    const double pr = h_A - half_g * h2_A;
+    PairOut o;
+    o.f1 = Y_A * hu_A - X_A * hv_A;
+    o.f2 = Y_A * (hu_A * u_A + pr) - X_A * hv_A * u_A;
+    o.f3 = Y_A * (hu_A * v_A) - X_A * (hv_A * v_A + pr);
+    return o;
+}
+
+// All pairs of one line of NGL nodes; acc[k] += sgn * D(k,l) F(k,l), l ascending for every k.
+template <int NGL, bool NEG>
+__device__ __forceinline__ void line_sweep(const double* __restrict__ D, double fourth_g, double half_g,
+                                           const double (&h)[NGL], const double (&hu)[NGL], const double (&hv)[NGL],
+                                           const double (&u)[NGL], const double (&v)[NGL], const double (&h2)[NGL],
+                                           const double (&Y)[NGL], const double (&X)[NGL], double (&a1)[NGL],
+                                           double (&a2)[NGL], double (&a3)[NGL]) {
+#pragma unroll
+    for (int c = 0; c < NGL; ++c) {
+#pragma unroll
+        for (int l = c; l < NGL; ++l) {
+            const PairOut F = ec_pair(fourth_g, half_g, h[c], hu[c], hv[c], u[c], v[c], h2[c], Y[c], X[c], h[l], hu[l],
+                                      hv[l], u[l], v[l], h2[l], Y[l], X[l]);
+            const double dcl = NEG ? -D[c * NGL + l] : D[c * NGL + l];
+            a1[c] += dcl * F.f1;
+            a2[c] += dcl * F.f2;
+            a3[c] += dcl * F.f3;
+            if (l != c) {
+                const double dlc = NEG ? -D[l * NGL + c] : D[l * NGL + c];
+                a1[l] += dlc * F.f1;
+                a2[l] += dlc * F.f2;
+                a3[l] += dlc * F.f3;
+            }
+        }
+    }
+}
+
+template <int NGL>
+__global__ void __launch_bounds__(64) k_volume_fluxdiff(int Nelem, double g, const double* __restrict__ Jac,
+                                                        const double* __restrict__ Yxi, const double* __restrict__ Yeta,
+                                                        const double* __restrict__ Xxi, const double* __restrict__ Xeta,
+                                                        const double* __restrict__ q, const double* __restrict__ D,
+                                                        const double* __restrict__ Bx, const double* __restrict__ By,
+                                                        double* __restrict__ Qt) {
+    using C = VolCfg<NGL>;
+    __shared__ double s[C::EPW * C::ESZ];
+    const int lane = threadIdx.x;
+    const int el = lane / NGL, r = lane % NGL;
+    const int e = blockIdx.x * C::EPW + el;
+    const bool act = (el < C::EPW) && (e < Nelem);
+    double* S = s + (act ? el : 0) * C::ESZ;
+    const double half_g = 0.5 * g, fourth_g = 0.25 * g;
+
+    double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], h2[NGL], Y[NGL], X[NGL];
+    double a1[NGL], a2[NGL], a3[NGL];
+#pragma unroll
+    for (int c = 0; c < NGL; ++c) { a1[c] = 0.0; a2[c] = 0.0; a3[c] = 0.0; }
+
+    // ---- row phase: eta-row r, nodes (r, c) -------------------------------------------------
+    if (act) {
+        const int64_t qb = (int64_t)e * C::NN * ESDG_NEQ + r * NGL;
+        const int64_t xb = (int64_t)e * C::NN + r * NGL;
+#pragma unroll
+        for (int c = 0; c < NGL; ++c) {
+            h[c] = q[qb + c];
+            hu[c] = q[qb + C::NN + c];
+            hv[c] = q[qb + 2 * C::NN + c];
+            Y[c] = Yeta[xb + c];
+            X[c] = Xeta[xb + c];
+        }
+#pragma unroll
+        for (int c = 0; c < NGL; ++c) {
+            const double inv = (h[c] > 0) ? 1.0 / h[c] : 0.0;
+            u[c] = hu[c] * inv;
+            v[c] = hv[c] * inv;
+            h2[c] = h[c] * h[c];
+            S[0 * C::PLANE + r * C::RS + c] = h[c];
+            S[1 * C::PLANE + r * C::RS + c] = hu[c];
+            S[2 * C::PLANE + r * C::RS + c] = hv[c];
+            S[3 * C::PLANE + r * C::RS + c] = u[c];
+            S[4 * C::PLANE + r * C::RS + c] = v[c];
+        }
+        line_sweep<NGL, false>(D, fourth_g, half_g, h, hu, hv, u, v, h2, Y, X, a1, a2, a3);
+    }
+    __syncthreads();
+
+    // ---- column phase: xi-column r, nodes (k, r) -------------------------------------------
+    double b1[NGL], b2[NGL], b3[NGL];
+    if (act) {
+        const int64_t xb = (int64_t)e * C::NN + r;
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) {
+            h[k] = S[0 * C::PLANE + k * C::RS + r];
+            hu[k] = S[1 * C::PLANE + k * C::RS + r];
+            hv[k] = S[2 * C::PLANE + k * C::RS + r];
+            u[k] = S[3 * C::PLANE + k * C::RS + r];
+            v[k] = S[4 * C::PLANE + k * C::RS + r];
+            h2[k] = h[k] * h[k];
+            Y[k] = Yxi[xb + k * NGL];
+            X[k] = Xxi[xb + k * NGL];
+        }
+    }
+    __syncthreads();
+    if (act) {
+        // hand the row sums over: node (r, c) of the row phase -> LDS plane slot [r][c]
+#pragma unroll
+        for (int c = 0; c < NGL; ++c) {
+            S[0 * C::PLANE + r * C::RS + c] = a1[c];
+            S[1 * C::PLANE + r * C::RS + c] = a2[c];
+            S[2 * C::PLANE + r * C::RS + c] = a3[c];
+        }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) {
+            b1[k] = S[0 * C::PLANE + k * C::RS + r];
+            b2[k] = S[1 * C::PLANE + k * C::RS + r];
+            b3[k] = S[2 * C::PLANE + k * C::RS + r];
+        }
+        line_sweep<NGL, true>(D, fourth_g, half_g, h, hu, hv, u, v, h2, Y, X, b1, b2, b3);
+        const int64_t xb = (int64_t)e * C::NN + r;
+        const int64_t ob = (int64_t)e * C::NN * ESDG_NEQ + r;
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) {
+            const double rJ = Jac[xb + k * NGL];
+            Qt[ob + k * NGL] = -rJ * b1[k];
+            Qt[ob + C::NN + k * NGL] = -rJ * b2[k] - g * h[k] * Bx[xb + k * NGL];
+            Qt[ob + 2 * C::NN + k * NGL] = -rJ * b3[k] - g * h[k] * By[xb + k * NGL];
+        }
+    }
+}
+
+// Standard strong-form volume term (okl/DG/VolumeKernelV99.okl:17-167), FluxDifferencing = 0.
+// Thread slot n = i*NGL + j is node (eta row i, xi column j) as in the okl.
+template <int NGL>
+__global__ void __launch_bounds__(256) k_volume_std(int Nelem, double g, double tol, const double* __restrict__ Jac,
+                                                    const double* __restrict__ Yxi, const double* __restrict__ Yeta,
+                                                    const double* __restrict__ Xxi, const double* __restrict__ Xeta,
+                                                    const double* __restrict__ q, const double* __restrict__ D,
+                                                    const double* __restrict__ Bx, const double* __restrict__ By,
+                                                    double* __restrict__ Qt) {
+    constexpr int NN = NGL * NGL;
+    constexpr int EPB = (256 / NN) > 0 ? 256 / NN : 1;
+    __shared__ double sD[NN];
+    __shared__ double sF[3][EPB * NN], sG[3][EPB * NN];
+    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
+    const int e = blockIdx.x * EPB + el;
+    if (threadIdx.x < NN) sD[threadIdx.x] = D[threadIdx.x];
+    const int64_t xid = (int64_t)e * NN + n;
+    const int64_t fid = (int64_t)e * NN * ESDG_NEQ + n;
+    double h0 = 0.0;
+    if (e < Nelem) {
+        h0 = q[fid];
+        double h = h0, hu = 0.0, hv = 0.0, inv = 0.0;
+        if (h > tol) { hu = q[fid + NN]; hv = q[fid + 2 * NN]; inv = 1.0 / h; }
+        else h = 0.0;
+        const double half_g = 0.5 * g;
+        const double F1 = hu, F2 = hu * hu * inv + half_g * h * h, F3 = hv * hu * inv;
+        const double G1 = hv, G2 = hu * hv * inv, G3 = hv * hv * inv + half_g * h * h;
+        const double ye = Yeta[xid], yx = Yxi[xid], xe = Xeta[xid], xx = Xxi[xid];
+        sF[0][threadIdx.x] = ye * F1 - xe * G1;
+        sG[0][threadIdx.x] = -yx * F1 + xx * G1;
+        sF[1][threadIdx.x] = ye * F2 - xe * G2;
+        sG[1][threadIdx.x] = -yx * F2 + xx * G2;
+        sF[2][threadIdx.x] = ye * F3 - xe * G3;
+        sG[2][threadIdx.x] = -yx * F3 + xx * G3;
+    }
+    __syncthreads();
+    if (e >= Nelem) return;
+    const int i = n / NGL, j = n % NGL;
+    const int base = el * NN;
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+#pragma unroll
+    for (int l = 0; l < NGL; ++l) {
+        const double dil = sD[i * NGL + l], djl = sD[j * NGL + l];
+        d1 += djl * sF[0][base + i * NGL + l] + dil * sG[0][base + l * NGL + j];
+        d2 += djl * sF[1][base + i * NGL + l] + dil * sG[1][base + l * NGL + j];
+        d3 += djl * sF[2][base + i * NGL + l] + dil * sG[2][base + l * NGL + j];
+    }
+    const double rJ = Jac[xid];
+    Qt[fid] = -rJ * d1;
+    Qt[fid + NN] = -rJ * d2 - g * h0 * Bx[xid];
+    Qt[fid + 2 * NN] = -rJ * d3 - g * h0 * By[xid];
+}
+
+template <int NGL>
+void launch_volume_fluxdiff(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta,
+                            const double* Xxi, const double* Xeta, const double* q, const double* D, const double* Bx,
+                            const double* By, double* Qt) {
+    using C = VolCfg<NGL>;
+    k_volume_fluxdiff<NGL><<<blocks_for(Nelem, C::EPW), 64, 0, ctx->stream>>>(Nelem, ctx->p.g, Jac, Yxi, Yeta, Xxi, Xeta, q,
+                                                                              D, Bx, By, Qt);
+}
+
+template <int NGL>
+void launch_volume_std(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta,
+                       const double* Xxi, const double* Xeta, const double* q, const double* D, const double* Bx,
+                       const double* By, double* Qt) {
+    constexpr int NN = NGL * NGL;
+    constexpr int EPB = (256 / NN) > 0 ? 256 / NN : 1;
+    k_volume_std<NGL><<<blocks_for(Nelem, EPB), EPB * NN, 0, ctx->stream>>>(Nelem, ctx->p.g, ctx->p.PosPresTOL, Jac, Yxi,
+                                                                            Yeta, Xxi, Xeta, q, D, Bx, By, Qt);
+}
+
+}  // namespace esdg
+
+using namespace esdg;
+
+extern "C" {
+
+int esdg_volume_kernel(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta,
+                       const double* Xxi, const double* Xeta, const double* q, const double* D, const double* Bx,
+                       const double* By, double* Qt) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Jac && Yxi && Yeta && Xxi && Xeta && q && D && Bx && By && Qt,
+                 "esdg_volume_kernel: bad argument");
+    if (Nelem == 0) return ESDG_OK;
+    KernelScope ks(ctx, "volume_kernel");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fluxdiff, ctx, Nelem, Jac, Yxi, Yeta, Xxi, Xeta, q, D, Bx, By, Qt);
+    return launch_check(ctx, "volume_kernel");
+}
+
+int esdg_volume_kernel_std(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta,
+                           const double* Xxi, const double* Xeta, const double* q, const double* D, const double* Bx,
+                           const double* By, double* Qt) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Jac && Yxi && Yeta && Xxi && Xeta && q && D && Bx && By && Qt,
+                 "esdg_volume_kernel_std: bad argument");
+    if (Nelem == 0) return ESDG_OK;
+    KernelScope ks(ctx, "volume_kernel_std");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_std, ctx, Nelem, Jac, Yxi, Yeta, Xxi, Xeta, q, D, Bx, By, Qt);
+    return launch_check(ctx, "volume_kernel_std");
+}
+
+}  // extern "C"

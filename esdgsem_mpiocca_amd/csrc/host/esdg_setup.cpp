@@ -1,0 +1,761 @@
+// esdg_setup.cpp -- host-side FP64 problem setup (basis, meshes, geometry, bathymetry, states).
+//
+// The operator and geometry arrays here are the constant inputs of the per-stage kernels.  They
+// follow the reference's own setup arithmetic operation by operation, so that the same mesh and
+// degree give the same bits as the reference host code:
+//   LGL/Gauss nodes, weights, Legendre recursion   src/basis.cpp:566-776
+//   barycentric weights, derivative matrix         src/basis.cpp:826-868
+//   flux-differencing D, Dstrong, Dhat             src/basis.cpp:108-133
+//   modal transform VdmInv                         src/basis.cpp:495-564, 778-824
+//   element sizes                                  src/basis.cpp:939-966
+//   Cartesian mesh and connectivity                src/Mesh.cpp:316-696, 92-113, 282-307
+//   mapped geometry (metrics, normals, scal)       src/Mesh.cpp:1667-1808
+//   single-rank EdgeData layout                    src/MeshPartitioning.cpp:65-132, main.cpp:650-716
+//   split-form bathymetry gradient Bx, By          src/SW2D.cpp:1048-1105
+//   RK coefficients                                src/RungeKutta.cpp:18-90
+// The initial states are new (they define the benchmark configurations) and are documented at
+// make_state() below.
+#include "esdg_setup.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg, int code = -1) {
+    g_err = msg;
+    return code;
+}
+
+// ------------------------------------------------------------------------------------------
+// Legendre polynomials and the LGL / Gauss point sets
+// ------------------------------------------------------------------------------------------
+
+// L_n(x) and L_n'(x) by the three-term recursion (basis.cpp:566-602).
+void legendre(int n, double x, double* L, double* dL) {
+    if (n == 0) { *L = 1.0; *dL = 0.0; return; }
+    if (n == 1) { *L = x; *dL = 1.0; return; }
+    double Lm2 = 1.0, Lm1 = x, dLm2 = 0.0, dLm1 = 1.0, Ln = 0.0, dLn = 0.0;
+    for (int k = 2; k <= n; k++) {
+        Ln = ((2.0 * k - 1.0) / k) * x * Lm1 - (k - 1.0) / k * Lm2;
+        dLn = dLm2 + (2.0 * k - 1.0) * Lm1;
+        Lm2 = Lm1; Lm1 = Ln; dLm2 = dLm1; dLm1 = dLn;
+    }
+    *L = Ln; *dL = dLn;
+}
+
+// q = L_{N+1} - L_{N-1}, q' and L_N at x: the LGL interior nodes are the roots of q
+// (basis.cpp:623-653).
+void q_and_L(int N, double x, double* q, double* dq, double* LN) {
+    double Lm2 = 1.0, Lm1 = x, dLm2 = 0.0, dLm1 = 1.0, dLN = 0.0, Ln = 0.0;
+    for (int k = 2; k <= N; k++) {
+        Ln = (2.0 * k - 1.0) / k * x * Lm1 - (k - 1.0) / k * Lm2;
+        dLN = dLm2 + (2.0 * k - 1.0) * Lm1;
+        if (k < N) { Lm2 = Lm1; Lm1 = Ln; dLm2 = dLm1; dLm1 = dLN; }
+    }
+    const int k = N + 1;
+    const double Lp1 = (2.0 * k - 1.0) / k * x * Ln - (k - 1.0) / k * Lm1;
+    const double dLp1 = dLm1 + (2.0 * k - 1.0) * Ln;
+    *q = Lp1 - Lm1;
+    *dq = dLp1 - dLm1;
+    *LN = Ln;
+    (void)dLN;
+}
+
+struct Basis {
+    int N = 0, ngl = 0;
+    std::vector<double> xGL, wGL, xG, wG, wbary, D, D0, Dstrong, Dhat, Vdm, VdmInv;
+};
+
+void lgl_nodes(Basis& B) {
+    const int N = B.N, ngl = B.ngl;
+    const int newton_it = 4;
+    const double TOL = 4. * std::pow(10., -16);
+    double q = 0, dq = 0, LN = 0;
+    if (N == 1) {
+        B.xGL[0] = -1.0; B.wGL[0] = 1.0; B.xGL[1] = 1.0; B.wGL[1] = B.wGL[0];
+    } else if (N == 2) {
+        B.xGL[0] = -1.0; B.wGL[0] = 2.0 / (N * (N + 1.0));
+        B.xGL[ngl - 1] = 1.0; B.wGL[ngl - 1] = B.wGL[0];
+        q_and_L(N, 0.0, &q, &dq, &LN);
+        B.xGL[1] = 0.0; B.wGL[1] = 2.0 / (N * (N + 1) * std::pow(LN, 2));
+    } else {
+        B.xGL[0] = -1.0; B.wGL[0] = 2.0 / (N * (N + 1.0));
+        B.xGL[ngl - 1] = 1.0; B.wGL[ngl - 1] = B.wGL[0];
+        for (int j = 1; j <= (int)std::floor(N / 2.0); j++) {
+            double x = -std::cos((j + 1.0 / 4.0) * M_PI / N - 3.0 / (8.0 * N * M_PI) * 1.0 / (j + 1.0 / 4.0));
+            for (int k = 0; k <= newton_it; k++) {  // Newton on q
+                q_and_L(N, x, &q, &dq, &LN);
+                const double delta = -q / dq;
+                x = x + delta;
+                if (std::fabs(delta) <= TOL * std::fabs(x)) break;
+            }
+            q_and_L(N, x, &q, &dq, &LN);
+            B.xGL[j] = x;
+            B.xGL[ngl - j - 1] = -x;
+            B.wGL[j] = 2.0 / (N * (N + 1) * std::pow(LN, 2));
+            B.wGL[ngl - j - 1] = B.wGL[j];
+        }
+        if (N % 2 == 0) {
+            q_and_L(N, 0.0, &q, &dq, &LN);
+            B.xGL[N / 2] = 0.0;
+            B.wGL[N / 2] = 2.0 / (N * (N + 1) * std::pow(LN, 2));
+        }
+    }
+}
+
+void gauss_nodes(Basis& B) {
+    const int N = B.N, ngl = B.ngl;
+    const int newton_it = 4;
+    const double TOL = 4. * std::pow(10., -16);
+    double L1 = 0, L2 = 0;
+    if (N == 0) { B.xG[0] = 0.0; B.wG[0] = 2.0; return; }
+    if (N == 1) {
+        B.xG[0] = -std::sqrt(1.0 / 3.0); B.wG[0] = 1.0; B.xG[1] = -B.xG[0]; B.wG[1] = B.wG[0];
+        return;
+    }
+    for (int j = 0; j < (int)std::floor((N + 1.0) / 2.0); j++) {
+        double x = -std::cos((2.0 * j + 1.0) / (2.0 * N + 2.0) * M_PI);
+        for (int k = 0; k <= newton_it; k++) {
+            legendre(N + 1, x, &L1, &L2);
+            const double delta = -L1 / L2;
+            x = x + delta;
+            if (std::fabs(delta) <= TOL * std::fabs(x)) break;
+        }
+        legendre(N + 1, x, &L1, &L2);
+        B.xG[j] = x;
+        B.xG[ngl - j - 1] = -x;
+        B.wG[j] = 2.0 / ((1.0 - std::pow(x, 2)) * std::pow(L2, 2));
+        B.wG[ngl - j - 1] = B.wG[j];
+    }
+    if (N % 2 == 0) {
+        legendre(N + 1, 0.0, &L1, &L2);
+        B.xG[N / 2] = 0.0;
+        B.wG[N / 2] = 2.0 / std::pow(L2, 2);
+    }
+}
+
+// Lagrange basis through nodes x (barycentric weights w) evaluated at x0 (basis.cpp:778-809).
+void lagrange_at(double x0, int N, const double* x, const double* w, double* P) {
+    const double TOL = 4. * std::pow(10., -16);
+    bool match = false;
+    for (int j = 0; j < N + 1; j++) {
+        P[j] = 0.0;
+        if (std::fabs(x0 - x[j]) < TOL) { match = true; P[j] = 1; }
+    }
+    if (!match) {
+        double s = 0.0;
+        for (int j = 0; j < N + 1; j++) {
+            const double t = w[j] / (x0 - x[j]);
+            P[j] = t;
+            s = s + t;
+        }
+        for (int j = 0; j < N + 1; j++) P[j] = P[j] / s;
+    }
+}
+
+Basis make_basis(int N, int flux_differencing) {
+    if (N < 1 || N > 15) throw std::invalid_argument("polynomial degree N must be in [1,15]");
+    Basis B;
+    B.N = N; B.ngl = N + 1;
+    const int ngl = B.ngl, ngl2 = ngl * ngl;
+    B.xGL.assign(ngl, 0.0); B.wGL.assign(ngl, 0.0); B.xG.assign(ngl, 0.0); B.wG.assign(ngl, 0.0);
+    B.wbary.assign(ngl, 1.0);
+    B.D.assign(ngl2, 0.0); B.D0.assign(ngl2, 0.0); B.Dstrong.assign(ngl2, 0.0); B.Dhat.assign(ngl2, 0.0);
+    B.Vdm.assign(ngl2, 0.0); B.VdmInv.assign(ngl2, 0.0);
+    lgl_nodes(B);
+    gauss_nodes(B);
+    // barycentric weights (basis.cpp:826-849)
+    for (int j = 1; j < ngl; j++)
+        for (int k = 0; k < j; k++) {
+            B.wbary[k] = B.wbary[k] * (B.xGL[k] - B.xGL[j]);
+            B.wbary[j] = B.wbary[j] * (B.xGL[j] - B.xGL[k]);
+        }
+    for (int j = 0; j < ngl; j++) B.wbary[j] = 1.0 / B.wbary[j];
+    // derivative matrix, diagonal by the negative-sum trick in column order (basis.cpp:851-868)
+    for (int i = 0; i < ngl; i++)
+        for (int j = 0; j < ngl; j++)
+            if (i != j) {
+                const double dij = (B.wbary[j] / B.wbary[i]) * (1.0 / (B.xGL[i] - B.xGL[j]));
+                B.D[i * ngl + j] = dij;
+                B.D[i * ngl + i] = B.D[i * ngl + i] - dij;
+            }
+    for (int i = 0; i < ngl2; i++) {
+        B.D0[i] = B.D[i];
+        B.Dstrong[i] = B.D[i];
+        if (flux_differencing) B.D[i] = 2.0 * B.D[i];
+    }
+    for (int i = 0; i < ngl; i++)
+        for (int j = 0; j < ngl; j++)
+            B.Dhat[i * ngl + j] = -B.wGL[j] * B.D0[j * ngl + i] / B.wGL[i];
+    B.D[0] = B.D[0] + 1.0 / B.wGL[0];
+    B.D[ngl2 - 1] = B.D[ngl2 - 1] - 1.0 / B.wGL[ngl - 1];
+    B.Dstrong[0] = B.D0[0] + 1.0 / B.wGL[0];
+    B.Dstrong[ngl2 - 1] = B.D0[ngl2 - 1] - 1.0 / B.wGL[ngl - 1];
+    // modal transform: L2 projection onto orthonormal Legendre modes by Gauss quadrature
+    std::vector<double> vt(ngl2, 0.0), interp(ngl2, 0.0), row(ngl, 0.0);
+    for (int i = 0; i < ngl; i++)
+        for (int j = 0; j < ngl; j++) {
+            double L, dL;
+            legendre(j, B.xG[i], &L, &dL);
+            vt[j * ngl + i] = L * std::sqrt(j + 0.5) * B.wG[i];
+            legendre(j, B.xGL[i], &L, &dL);
+            B.Vdm[i * ngl + j] = L * std::sqrt(j + 0.5);
+        }
+    for (int i = 0; i <= N; i++) {
+        lagrange_at(B.xG[i], N, B.xGL.data(), B.wbary.data(), row.data());
+        for (int j = 0; j < ngl; j++) interp[i * ngl + j] = row[j];
+    }
+    for (int i = 0; i < ngl; i++)
+        for (int j = 0; j < ngl; j++)
+            for (int l = 0; l < ngl; l++)
+                B.VdmInv[i * ngl + j] += vt[i * ngl + l] * interp[l * ngl + j];
+    return B;
+}
+
+// ------------------------------------------------------------------------------------------
+// Meshes
+// ------------------------------------------------------------------------------------------
+
+struct Geometry {
+    int ngl = 0, Nelem = 0, Nfaces = 0;
+    std::vector<double> x, y, xXi, xEta, yXi, yEta, J;       // per node; J is the Jacobian (not inverted)
+    std::vector<double> enx, eny, escal;                     // per element side node [e][side][k]
+    std::vector<int> edge_info;                              // reference Mesh::EdgeInfo [f][7]
+    std::vector<int> elem_to_edge;                           // [e][4], 1-based
+};
+
+// Structured quad topology and node placement of Mesh::GenerateMesh (Mesh.cpp:316-696).  For
+// the Cartesian kind the metrics, J and side normals are the generator's constants; for the
+// curved kind the nodes are displaced and the metrics/normals are recomputed the way
+// Mesh::ConstructMappedGeometry does.
+Geometry make_mesh(const esdg_case_spec& s, const Basis& B) {
+    Geometry G;
+    const int ngl = B.ngl, ngl2 = ngl * ngl;
+    const int NelemX = s.nelx, NelemY = s.nely;
+    if (NelemX < 1 || NelemY < 1) throw std::invalid_argument("nelx and nely must be >= 1");
+    if ((int64_t)NelemX * NelemY * ngl2 * 3 > (int64_t)INT32_MAX)
+        throw std::invalid_argument("mesh too large for 32-bit node indices");
+    const bool PX = s.periodic_x != 0, PY = s.periodic_y != 0;
+    G.ngl = ngl;
+    G.Nelem = NelemX * NelemY;
+    int nedges = (NelemX + 1) * NelemY + (NelemY + 1) * NelemX;
+    if (PX) nedges -= NelemX;
+    if (PY) nedges -= NelemY;
+    G.Nfaces = nedges;
+    const size_t nn = (size_t)G.Nelem * ngl2;
+    G.x.assign(nn, 0.0); G.y.assign(nn, 0.0);
+    G.xXi.assign(nn, 0.0); G.xEta.assign(nn, 0.0); G.yXi.assign(nn, 0.0); G.yEta.assign(nn, 0.0);
+    G.J.assign(nn, 0.0);
+    G.enx.assign((size_t)G.Nelem * 4 * ngl, 0.0);
+    G.eny.assign((size_t)G.Nelem * 4 * ngl, 0.0);
+    G.escal.assign((size_t)G.Nelem * 4 * ngl, 0.0);
+    G.edge_info.assign((size_t)nedges * 7, 0);
+    G.elem_to_edge.assign((size_t)G.Nelem * 4, 0);
+
+    const double deltaX = (s.xR - s.xL) / NelemX;
+    const double deltaY = (s.yR - s.yL) / NelemY;
+    const double x_xi = 0.5 * deltaX;
+    const double y_eta = 0.5 * deltaY;
+    const double Jac = x_xi * y_eta;
+    for (int ieY = 0; ieY < NelemY; ++ieY)
+        for (int ieX = 0; ieX < NelemX; ++ieX)
+            for (int j = 0; j < ngl; ++j)
+                for (int i = 0; i < ngl; ++i) {
+                    const size_t id = ((size_t)ieY * NelemX + ieX) * ngl2 + j * ngl + i;
+                    G.x[id] = s.xL + (B.xGL[i] + 1.0) / 2.0 * deltaX + ieX * deltaX;
+                    G.y[id] = s.yL + (B.xGL[j] + 1.0) / 2.0 * deltaY + ieY * deltaY;
+                }
+    for (size_t id = 0; id < nn; ++id) {
+        G.xXi[id] = x_xi; G.xEta[id] = 0.0; G.yXi[id] = 0.0; G.yEta[id] = y_eta; G.J[id] = Jac;
+    }
+
+    // Face numbering: horizontal faces first, then vertical ones (Mesh.cpp:412-655).
+    auto side_set = [&](int e, int side, double nxv, double nyv, double sc) {
+        for (int k = 0; k < ngl; ++k) {
+            const size_t id = (size_t)e * 4 * ngl + side * ngl + k;
+            G.enx[id] = nxv; G.eny[id] = nyv; G.escal[id] = sc;
+        }
+    };
+    auto finish_edge = [&](int is) {
+        int* E = &G.edge_info[(size_t)(is - 1) * 7];
+        E[0] = 0; E[1] = 0; E[6] = 1;
+        G.elem_to_edge[(size_t)E[2] * 4 + E[4]] = is;
+        if (E[3] != -1) G.elem_to_edge[(size_t)E[3] * 4 + E[5]] = is;
+    };
+    int jLoopBound = NelemY;
+    if (PX) jLoopBound -= 1;
+    for (int j = 0; j <= jLoopBound; j++)
+        for (int i = 0; i < NelemX; i++) {
+            const int is = (j * NelemX + i) + 1;
+            int* E = &G.edge_info[(size_t)(is - 1) * 7];
+            if (j == 0) {
+                E[2] = i; E[4] = 0; E[3] = -1; E[5] = -1;
+                side_set(E[2], E[4], 0.0, -1.0, x_xi);
+                if (PX) {
+                    E[3] = (NelemY - 1) * NelemX + i; E[5] = 2;
+                    side_set(E[3], E[5], 0.0, 1.0, x_xi);
+                }
+            } else if (j == NelemY) {
+                if (!PX) {
+                    E[2] = (NelemY - 1) * NelemX + i; E[4] = 2; E[3] = -1; E[5] = -1;
+                    side_set(E[2], E[4], 0.0, 1.0, x_xi);
+                }
+            } else {
+                E[2] = (j - 1) * NelemX + i; E[3] = j * NelemX + i; E[4] = 2; E[5] = 0;
+                side_set(E[2], E[4], 0.0, 1.0, x_xi);
+                side_set(E[3], E[5], 0.0, -1.0, x_xi);
+            }
+            finish_edge(is);
+        }
+    int iLoopBound = NelemX;
+    if (PY) iLoopBound -= 1;
+    for (int j = 0; j < NelemY; j++)
+        for (int i = 0; i <= iLoopBound; i++) {
+            const int is = ((jLoopBound + 1) * NelemX) + (j * (iLoopBound + 1) + i) + 1;
+            int* E = &G.edge_info[(size_t)(is - 1) * 7];
+            if (i == 0) {
+                E[2] = j * NelemX; E[4] = 3; E[3] = -1; E[5] = -1;
+                side_set(E[2], E[4], -1.0, 0.0, y_eta);
+                if (PY) {
+                    E[3] = (j + 1) * NelemX - 1; E[5] = 1;
+                    side_set(E[3], E[5], 1.0, 0.0, y_eta);
+                }
+            } else if (i == NelemX) {
+                if (!PY) {
+                    E[2] = (j + 1) * NelemX - 1; E[4] = 1; E[3] = -1; E[5] = -1;
+                    side_set(E[2], E[4], 1.0, 0.0, y_eta);
+                }
+            } else {
+                E[2] = j * NelemX + i - 1; E[3] = j * NelemX + i; E[4] = 1; E[5] = 3;
+                side_set(E[2], E[4], 1.0, 0.0, y_eta);
+                side_set(E[3], E[5], -1.0, 0.0, y_eta);
+            }
+            finish_edge(is);
+        }
+
+    if (s.mesh_kind == ESDG_MESH_CURVED_SINE) {
+        if (PX || PY) throw std::invalid_argument("the curved mesh kind is not periodic");
+        const double a = s.perturb_amp, w = s.perturb_len;
+        if (!(w > 0.0)) throw std::invalid_argument("perturb_len must be > 0");
+        std::vector<double> xe(ngl2), ye(ngl2), xxi(ngl2), xeta(ngl2), yxi(ngl2), yeta(ngl2), Je(ngl2);
+        const double* D = B.D0.data();
+        for (int e = 0; e < G.Nelem; ++e) {
+            const size_t base = (size_t)e * ngl2;
+            for (int k = 0; k < ngl2; ++k) {
+                const double xc = G.x[base + k], yc = G.y[base + k];
+                const double d = a * std::sin(M_PI * xc / w) * std::sin(M_PI * yc / w);
+                xe[k] = xc + d;
+                ye[k] = yc + d;
+            }
+            // metric terms by the nodal derivative matrix (Mesh.cpp:1722-1742)
+            for (int j = 0; j < ngl; j++)
+                for (int i = 0; i < ngl; i++) {
+                    const int ij = j * ngl + i;
+                    double ax = 0.0, bx = 0.0, ay = 0.0, by = 0.0;
+                    for (int k = 0; k < ngl; k++) {
+                        ax += D[i * ngl + k] * xe[j * ngl + k];
+                        bx += D[j * ngl + k] * xe[k * ngl + i];
+                        ay += D[i * ngl + k] * ye[j * ngl + k];
+                        by += D[j * ngl + k] * ye[k * ngl + i];
+                    }
+                    xxi[ij] = ax; xeta[ij] = bx; yxi[ij] = ay; yeta[ij] = by;
+                    Je[ij] = xxi[ij] * yeta[ij] - xeta[ij] * yxi[ij];
+                }
+            for (int k = 0; k < ngl2; ++k) {
+                G.x[base + k] = xe[k]; G.y[base + k] = ye[k];
+                G.xXi[base + k] = xxi[k]; G.xEta[base + k] = xeta[k];
+                G.yXi[base + k] = yxi[k]; G.yEta[base + k] = yeta[k];
+                G.J[base + k] = Je[k];
+            }
+            // outward normals and surface Jacobians per side (Mesh.cpp:1744-1806)
+            double* nxs = &G.enx[(size_t)e * 4 * ngl];
+            double* nys = &G.eny[(size_t)e * 4 * ngl];
+            double* scs = &G.escal[(size_t)e * 4 * ngl];
+            for (int j = 0; j < ngl; j++) {
+                const int s1 = ngl + j, s3 = 3 * ngl + j;
+                const int Nj = j * ngl + ngl - 1, Zj = j * ngl;
+                double ye_ = yeta[Nj], xe_ = xeta[Nj], Jt = Je[Nj];
+                scs[s1] = std::sqrt(ye_ * ye_ + xe_ * xe_);
+                nxs[s1] = std::copysign(1.0, Jt) * (ye_ / scs[s1]);
+                nys[s1] = std::copysign(1.0, Jt) * (-xe_ / scs[s1]);
+                ye_ = yeta[Zj]; xe_ = xeta[Zj]; Jt = Je[Zj];
+                scs[s3] = std::sqrt(ye_ * ye_ + xe_ * xe_);
+                nxs[s3] = -std::copysign(1.0, Jt) * (ye_ / scs[s3]);
+                nys[s3] = -std::copysign(1.0, Jt) * (-xe_ / scs[s3]);
+            }
+            for (int i = 0; i < ngl; i++) {
+                const int s0 = i, s2 = 2 * ngl + i;
+                const int iZ = i, iN = (ngl - 1) * ngl + i;
+                double yx = yxi[iZ], xx = xxi[iZ], Jt = Je[iZ];
+                scs[s0] = std::sqrt(yx * yx + xx * xx);
+                nxs[s0] = -std::copysign(1.0, Jt) * (-yx / scs[s0]);
+                nys[s0] = -std::copysign(1.0, Jt) * (xx / scs[s0]);
+                yx = yxi[iN]; xx = xxi[iN]; Jt = Je[iN];
+                scs[s2] = std::sqrt(yx * yx + xx * xx);
+                nxs[s2] = std::copysign(1.0, Jt) * (-yx / scs[s2]);
+                nys[s2] = std::copysign(1.0, Jt) * (xx / scs[s2]);
+            }
+        }
+    } else if (s.mesh_kind != ESDG_MESH_CARTESIAN) {
+        throw std::invalid_argument("unknown mesh kind");
+    }
+    return G;
+}
+
+// ------------------------------------------------------------------------------------------
+// Counter-based random numbers: SplitMix64 of (seed, index), uniform in [0,1).
+// ------------------------------------------------------------------------------------------
+inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double uniform01(uint64_t seed, uint64_t idx) {
+    return (double)(splitmix64(seed ^ splitmix64(idx)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Case object
+// ------------------------------------------------------------------------------------------
+struct esdg_case {
+    esdg_case_spec spec{};
+    Basis B;
+    int Nelem = 0, Nfaces = 0;
+    double min_ele_size = 0.0;
+    std::map<std::string, std::vector<double>> d;
+    std::map<std::string, std::vector<int>> iv;
+};
+
+namespace {
+
+// Bathymetry and initial state at every node (the benchmark configurations).
+//   GAUSSIAN_BUMP: b = 0.8 exp(-5((x-1)^2+(y-1)^2)); h = 1 - b + amp*exp(-20((x-0.7)^2+(y-1)^2)).
+//   DAM_BREAK:     b = 0.1 sin(x) cos(y); h = 2 for x < p0 (default 5) else 1, plus amp*(2U-1).
+//   FLOOD:         b = (0.3/4) sum_{m<16} cos(2 pi (k_m x + l_m y)/Lx + phi_m), integers
+//                  k_m, l_m in [0,3] (not both 0), phi_m in [0, 2 pi), all seeded; water level
+//                  p0 for x < p1, level p2 elsewhere; h = max(0, level - b).  Wet/dry fronts.
+//   OCEAN_UPLIFT:  depth 4000 m over the first 80% of x, linear shelf to 50 m over the last 20%;
+//                  b = 4000 - depth, h = depth + eta, eta = amp*exp(-(xr/a)^2-(yr/c)^2) with
+//                  rotation p2 (radians), axes a = p0, c = p1 (metres), centre at 40% / 50%.
+//   CONSTANT:      h = p0, hu = p1, hv = p2, b = 0.
+void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<double>& y) {
+    const esdg_case_spec& s = C.spec;
+    const int ngl2 = C.B.ngl * C.B.ngl;
+    const size_t nn = (size_t)C.Nelem * ngl2;
+    std::vector<double>& b = C.d["b"];
+    std::vector<double>& q = C.d["q"];
+    b.assign(nn, 0.0);
+    q.assign(nn * 3, 0.0);
+    // 16 seeded Fourier modes for FLOOD
+    double km[16], lm[16], ph[16];
+    for (int m = 0; m < 16; ++m) {
+        int k, l;
+        uint64_t t = 0;
+        do {
+            k = (int)(uniform01(s.seed + 7, 3 * m + 0 + 1000 * t) * 4.0);
+            l = (int)(uniform01(s.seed + 7, 3 * m + 1 + 1000 * t) * 4.0);
+            ++t;
+        } while (k == 0 && l == 0);
+        km[m] = k; lm[m] = l;
+        ph[m] = 2.0 * M_PI * uniform01(s.seed + 7, 3 * m + 2);
+    }
+    const double Lx = s.xR - s.xL, Ly = s.yR - s.yL;
+    for (int e = 0; e < C.Nelem; ++e)
+        for (int k = 0; k < ngl2; ++k) {
+            const size_t id = (size_t)e * ngl2 + k;
+            const size_t qid = (size_t)e * ngl2 * 3 + k;
+            const double X = x[id], Y = y[id];
+            double bb = 0.0, h = 0.0, hu = 0.0, hv = 0.0;
+            switch (s.case_kind) {
+                case ESDG_CASE_GAUSSIAN_BUMP:
+                    bb = 0.8 * std::exp(-5.0 * ((X - 1.0) * (X - 1.0) + (Y - 1.0) * (Y - 1.0)));
+                    h = 1.0 - bb + s.amp * std::exp(-20.0 * ((X - 0.7) * (X - 0.7) + (Y - 1.0) * (Y - 1.0)));
+                    break;
+                case ESDG_CASE_DAM_BREAK: {
+                    const double xd = (s.p0 != 0.0) ? s.p0 : 5.0;
+                    bb = 0.1 * std::sin(X) * std::cos(Y);
+                    h = (X < xd) ? 2.0 : 1.0;
+                    h += s.amp * (2.0 * uniform01(s.seed, qid) - 1.0);
+                    break;
+                }
+                case ESDG_CASE_FLOOD: {
+                    double acc = 0.0;
+                    for (int m = 0; m < 16; ++m)
+                        acc += std::cos(2.0 * M_PI * (km[m] * (X - s.xL) + lm[m] * (Y - s.yL)) / Lx + ph[m]);
+                    bb = 0.3 / 4.0 * acc;
+                    const double level = (X < s.p1) ? s.p0 : s.p2;
+                    h = level - bb;
+                    if (h < 0.0) h = 0.0;
+                    break;
+                }
+                case ESDG_CASE_OCEAN_UPLIFT: {
+                    const double xs = s.xL + 0.8 * Lx;
+                    double depth = 4000.0;
+                    if (X > xs) depth = 4000.0 + (50.0 - 4000.0) * (X - xs) / (0.2 * Lx);
+                    bb = 4000.0 - depth;
+                    const double cx = s.xL + 0.4 * Lx, cy = s.yL + 0.5 * Ly;
+                    const double c = std::cos(s.p2), sn = std::sin(s.p2);
+                    const double xr = c * (X - cx) + sn * (Y - cy);
+                    const double yr = -sn * (X - cx) + c * (Y - cy);
+                    h = depth + s.amp * std::exp(-(xr / s.p0) * (xr / s.p0) - (yr / s.p1) * (yr / s.p1));
+                    break;
+                }
+                case ESDG_CASE_CONSTANT:
+                    bb = 0.0; h = s.p0; hu = s.p1; hv = s.p2;
+                    break;
+                default:
+                    throw std::invalid_argument("unknown case kind");
+            }
+            b[id] = bb;
+            q[qid] = h;
+            q[qid + ngl2] = hu;
+            q[qid + 2 * ngl2] = hv;
+        }
+}
+
+void build_case(esdg_case& C) {
+    const esdg_case_spec& s = C.spec;
+    C.B = make_basis(s.N, 1);
+    const Basis& B = C.B;
+    const int ngl = B.ngl, ngl2 = ngl * ngl;
+    Geometry G = make_mesh(s, B);
+    C.Nelem = G.Nelem;
+    C.Nfaces = G.Nfaces;
+    const size_t nn = (size_t)G.Nelem * ngl2;
+
+    C.d["x_GL"] = B.xGL; C.d["w_GL"] = B.wGL;
+    C.d["D"] = B.D; C.d["D0"] = B.D0; C.d["Dstrong"] = B.Dstrong; C.d["Dhat"] = B.Dhat;
+    C.d["VdmInv"] = B.VdmInv;
+
+    // per-face normals and scal from the left element's side (Mesh.cpp:282-307)
+    std::vector<double> fnx((size_t)G.Nfaces * ngl), fny((size_t)G.Nfaces * ngl), fsc((size_t)G.Nfaces * ngl);
+    for (int is = 0; is < G.Nfaces; is++)
+        for (int i = 0; i < ngl; i++) {
+            const size_t id = (size_t)is * ngl + i;
+            const size_t idLoc = (size_t)G.edge_info[(size_t)is * 7 + 2] * 4 * ngl + (size_t)G.edge_info[(size_t)is * 7 + 4] * ngl + i;
+            fnx[id] = G.enx[idLoc]; fny[id] = G.eny[idLoc]; fsc[id] = G.escal[idLoc];
+        }
+    // EdgeData (MeshPartitioning.cpp:104-131 then main.cpp:653-663) and per-element face maps
+    std::vector<int> ED((size_t)G.Nfaces * 8), ER(G.Nfaces), E2E((size_t)G.Nelem * 4), EMS((size_t)G.Nelem * 4), EOR((size_t)G.Nelem * 4);
+    for (int is = 0; is < G.Nfaces; is++) {
+        const int* E = &G.edge_info[(size_t)is * 7];
+        int* o = &ED[(size_t)is * 8];
+        o[0] = E[2]; o[1] = E[3]; o[2] = E[4]; o[3] = E[5]; o[4] = E[6];
+        o[5] = 0; o[6] = (E[3] == -1) ? -1 : 0; o[7] = is;
+        ER[is] = o[4];
+    }
+    for (int ie = 0; ie < G.Nelem; ++ie)
+        for (int is = 0; is < 4; is++) {
+            const size_t id = (size_t)ie * 4 + is;
+            const int ifa = G.elem_to_edge[id];
+            if (ifa < 1) throw std::runtime_error("mesh element side without a face");
+            E2E[id] = ifa;
+            const int* E = &G.edge_info[(size_t)(ifa - 1) * 7];
+            if (E[2] == ie) { EMS[id] = +1; EOR[id] = 1; }
+            else { EMS[id] = -1; EOR[id] = E[6]; }
+        }
+    // 1/J and element sizes (main.cpp:700, basis.cpp:939-966)
+    std::vector<double> Jinv(nn);
+    for (size_t i = 0; i < nn; ++i) Jinv[i] = 1.0 / G.J[i];
+    std::vector<double> sizes(G.Nelem, 0.0);
+    double tmin = 0.0;
+    for (int ie = 0; ie < G.Nelem; ie++) {
+        for (int j = 0; j < ngl; ++j)
+            for (int i = 0; i < ngl; ++i) {
+                const size_t xid = (size_t)ie * ngl2 + j * ngl + i;
+                sizes[ie] += 1.0 / Jinv[xid] * B.wGL[i] * B.wGL[j];
+            }
+        tmin = (ie == 0) ? sizes[ie] : std::min(tmin, sizes[ie]);
+    }
+    C.min_ele_size = std::sqrt(tmin);
+
+    make_state(C, G.x, G.y);
+
+    // split-form bathymetry gradient (SW2D.cpp:1048-1105)
+    const std::vector<double>& b = C.d["b"];
+    std::vector<double> Bx(nn, 0.0), By(nn, 0.0);
+    const double* D = B.D0.data();
+    for (int ie = 0; ie < G.Nelem; ie++)
+        for (int j = 0; j < ngl; ++j)
+            for (int i = 0; i < ngl; ++i) {
+                const size_t id = (size_t)ie * ngl2;
+                double B_xi = 0.0, BYeta_xi = 0.0, BXxi_eta = 0.0;
+                double B_eta = 0.0, BYxi_eta = 0.0, BXeta_xi = 0.0;
+                for (int l = 0; l < ngl; ++l) {
+                    const int il = i * ngl + l, jl = j * ngl + l, li = l * ngl + i;
+                    B_xi += D[il] * b[id + jl];
+                    BYeta_xi += D[il] * b[id + jl] * G.yEta[id + jl];
+                    BXeta_xi += D[il] * b[id + jl] * G.xEta[id + jl];
+                    B_eta += D[jl] * b[id + li];
+                    BYxi_eta += D[jl] * b[id + li] * G.yXi[id + li];
+                    BXxi_eta += D[jl] * b[id + li] * G.xXi[id + li];
+                }
+                const size_t xid = id + j * ngl + i;
+                Bx[xid] = 0.5 * Jinv[xid] * (G.yEta[xid] * B_xi + BYeta_xi - G.yXi[xid] * B_eta - BYxi_eta);
+                By[xid] = 0.5 * Jinv[xid] * (-G.xEta[xid] * B_xi - BXeta_xi + G.xXi[xid] * B_eta + BXxi_eta);
+            }
+
+    C.d["x"] = std::move(G.x); C.d["y"] = std::move(G.y);
+    C.d["xXi"] = std::move(G.xXi); C.d["xEta"] = std::move(G.xEta);
+    C.d["yXi"] = std::move(G.yXi); C.d["yEta"] = std::move(G.yEta);
+    C.d["Jac"] = std::move(Jinv);
+    C.d["Bx"] = std::move(Bx); C.d["By"] = std::move(By);
+    C.d["nx"] = std::move(fnx); C.d["ny"] = std::move(fny); C.d["scal"] = std::move(fsc);
+    C.d["EleSizes"] = std::move(sizes);
+    C.iv["EdgeData"] = std::move(ED); C.iv["EdgeReversed"] = std::move(ER);
+    C.iv["ElemToEdge"] = std::move(E2E); C.iv["ElemEdgeMasterSlave"] = std::move(EMS);
+    C.iv["ElemEdgeOrientation"] = std::move(EOR);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+const char* esdg_setup_last_error(void) { return g_err.c_str(); }
+
+int esdg_basis_build(int N, int flux_differencing, double* x_GL, double* w_GL, double* D, double* D0,
+                     double* Dstrong, double* Dhat, double* VdmInv) {
+    try {
+        Basis B = make_basis(N, flux_differencing);
+        const size_t ngl = B.ngl, ngl2 = ngl * ngl;
+        auto cp = [](double* dst, const std::vector<double>& v, size_t n) {
+            if (dst) std::memcpy(dst, v.data(), n * sizeof(double));
+        };
+        cp(x_GL, B.xGL, ngl); cp(w_GL, B.wGL, ngl);
+        cp(D, B.D, ngl2); cp(D0, B.D0, ngl2); cp(Dstrong, B.Dstrong, ngl2); cp(Dhat, B.Dhat, ngl2);
+        cp(VdmInv, B.VdmInv, ngl2);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+}
+
+int esdg_rk_coefficients(int order, int ssp, int* stages, double* A, double* B, double* C) {
+    if (!stages || !A || !B || !C) return fail("null argument");
+    for (int i = 0; i < 5; ++i) { A[i] = 0.0; B[i] = 0.0; C[i] = 0.0; }
+    if (order == 3) {
+        *stages = 3;
+        if (ssp) {  // Shu-Osher SSP-RK3
+            A[0] = 1.0; A[1] = 3.0 / 4.0; A[2] = 1.0 / 3.0;
+            B[0] = 0.0; B[1] = 1.0 / 4.0; B[2] = 2.0 / 3.0;
+            C[0] = 1.0; C[1] = 1.0 / 4.0; C[2] = 2.0 / 3.0;
+        } else {    // Williamson low-storage RK3
+            A[0] = 0.0; A[1] = -5.0 / 9.0; A[2] = -153.0 / 128.0;
+            B[0] = 0.0; B[1] = 1.0 / 3.0; B[2] = 3.0 / 4.0;
+            C[0] = 1.0 / 3.0; C[1] = 15.0 / 16.0; C[2] = 8.0 / 15.0;
+        }
+        return 0;
+    }
+    if (order == 4) {  // Carpenter-Kennedy 5-stage 4th-order 2N-storage
+        *stages = 5;
+        A[0] = 0.0;
+        A[1] = -(567301805773.0 / 1357537059087.0);
+        A[2] = -(2404267990393.0 / 2016746695238.0);
+        A[3] = -(3550918686646.0 / 2091501179385.0);
+        A[4] = -(1275806237668.0 / 842570457699.0);
+        B[0] = 0.0;
+        B[1] = (1432997174477.0 / 9575080441755.0);
+        B[2] = (2526269341429.0 / 6820363962896.0);
+        B[3] = (2006345519317.0 / 3224310063776.0);
+        B[4] = (2802321613138.0 / 2924317926251.0);
+        C[0] = (1432997174477.0 / 9575080441755.0);
+        C[1] = (5161836677717.0 / 13612068292357.0);
+        C[2] = (1720146321549.0 / 2090206949498.0);
+        C[3] = (3134564353537.0 / 4481467310338.0);
+        C[4] = (2277821191437.0 / 14882151754819.0);
+        return 0;
+    }
+    return fail("unsupported RK order (3 or 4)");
+}
+
+int esdg_case_create(const esdg_case_spec* spec, esdg_case** out) {
+    if (!spec || !out) return fail("null argument");
+    *out = nullptr;
+    esdg_case* C = new (std::nothrow) esdg_case();
+    if (!C) return fail("out of host memory");
+    try {
+        C->spec = *spec;
+        build_case(*C);
+    } catch (const std::exception& e) {
+        delete C;
+        return fail(e.what());
+    }
+    *out = C;
+    return 0;
+}
+
+int esdg_case_get_info(const esdg_case* c, esdg_case_info* info) {
+    if (!c || !info) return fail("null argument");
+    info->N = c->B.N; info->ngl = c->B.ngl; info->Nelem = c->Nelem; info->Nfaces = c->Nfaces;
+    info->min_ele_size = c->min_ele_size;
+    return 0;
+}
+
+int esdg_case_array(const esdg_case* c, const char* name, void** data, int64_t* count, int* is_int) {
+    if (!c || !name || !data || !count || !is_int) return fail("null argument");
+    auto it = c->d.find(name);
+    if (it != c->d.end()) {
+        *data = (void*)it->second.data(); *count = (int64_t)it->second.size(); *is_int = 0;
+        return 0;
+    }
+    auto jt = c->iv.find(name);
+    if (jt != c->iv.end()) {
+        *data = (void*)jt->second.data(); *count = (int64_t)jt->second.size(); *is_int = 1;
+        return 0;
+    }
+    return fail(std::string("no case array named ") + name);
+}
+
+void esdg_case_destroy(esdg_case* c) { delete c; }
+
+int esdg_total_entropy(int ngl, int Nelem, double g, const double* w_GL, const double* q, const double* b,
+                       const double* Jac, double* out) {
+    if (!w_GL || !q || !b || !Jac || !out) return fail("null argument");
+    const int ngl2 = ngl * ngl;
+    double acc = 0.0;
+    for (int ie = 0; ie < Nelem; ie++)
+        for (int j = 0; j < ngl; ++j)
+            for (int i = 0; i < ngl; ++i) {
+                const size_t id = (size_t)ie * ngl2 * 3 + j * ngl + i;
+                const size_t xid = (size_t)ie * ngl2 + j * ngl + i;
+                const double h = q[id], hu = q[id + ngl2], hv = q[id + 2 * ngl2];
+                double u = 0.0, v = 0.0;
+                if (h > std::pow(10, -12)) { u = hu / h; v = hv / h; }
+                const double E = 0.5 * h * (u * u + v * v) + 0.5 * g * h * h + g * h * b[xid];
+                acc += E / Jac[xid] * w_GL[i] * w_GL[j];
+            }
+    *out = acc;
+    return 0;
+}
+
+int esdg_total_mass(int ngl, int Nelem, const double* w_GL, const double* q, const double* Jac, double* out) {
+    if (!w_GL || !q || !Jac || !out) return fail("null argument");
+    const int ngl2 = ngl * ngl;
+    double acc = 0.0;
+    for (int ie = 0; ie < Nelem; ie++)
+        for (int j = 0; j < ngl; ++j)
+            for (int i = 0; i < ngl; ++i) {
+                const size_t id = (size_t)ie * ngl2 * 3 + j * ngl + i;
+                const size_t xid = (size_t)ie * ngl2 + j * ngl + i;
+                acc += q[id] / Jac[xid] * w_GL[i] * w_GL[j];
+            }
+    *out = acc;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,259 @@
+"""ctypes binding of libesdg.so (include/esdg.h): the gfx950 hot path.
+
+There is no CPU fallback: every call goes to the HIP library, and a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .setup_api import Case, RunParams
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ESDG_H2D, ESDG_D2H, ESDG_D2D = 0, 1, 2
+
+
+class EsdgDeviceError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("ngl", C.c_int), ("ES", C.c_int), ("num_flux", C.c_int), ("bc_kind", C.c_int),
+                ("g", C.c_double), ("geomFace", C.c_double), ("PosPresTOL", C.c_double),
+                ("ZeroTOL", C.c_double), ("eps0", C.c_double), ("sigmaMin", C.c_double),
+                ("sigmaMax", C.c_double), ("h_zero", C.c_double)]
+
+
+class RunParamsC(C.Structure):
+    _fields_ = [("CFL", C.c_double), ("DFL", C.c_double), ("T", C.c_double),
+                ("min_ele_size", C.c_double), ("artificial_viscosity", C.c_int),
+                ("positivity", C.c_int), ("rk_order", C.c_int), ("rk_ssp", C.c_int),
+                ("flux_differencing", C.c_int), ("fused", C.c_int)]
+
+
+_INPUT_DOUBLES = ("GLw", "nx", "ny", "scal", "Yxi", "Yeta", "Xxi", "Xeta", "B", "Bx", "By", "D",
+                  "Dstrong", "Dhat", "Jac", "x", "y", "q", "EleSizes", "VdmInv")
+_INPUT_INTS = ("ElemEdgeMasterSlave", "ElemEdgeOrientation", "ElemToEdge", "EdgeData",
+               "EdgeReversed")
+
+
+class Inputs(C.Structure):
+    _fields_ = ([("Nelem", C.c_int), ("Nfaces", C.c_int)]
+                + [(n, C.POINTER(C.c_double)) for n in _INPUT_DOUBLES]
+                + [(n, C.POINTER(C.c_int)) for n in _INPUT_INTS])
+
+
+def library_path() -> str:
+    return os.path.join(_PKG, "libesdg.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise EsdgDeviceError(f"{path} is missing: the HIP library must be built "
+                                  "(__graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(path)
+        L.esdg_last_error.restype = C.c_char_p
+        L.esdg_last_error.argtypes = [C.c_void_p]
+        L.esdg_bytes_allocated.restype = C.c_size_t
+        L.esdg_bytes_allocated.argtypes = [C.c_void_p]
+        L.esdg_stream.restype = C.c_void_p
+        L.esdg_stream.argtypes = [C.c_void_p]
+        L.esdg_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Params)]
+        L.esdg_malloc.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.c_size_t]
+        L.esdg_free.argtypes = [C.c_void_p, C.c_void_p]
+        L.esdg_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.esdg_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_size_t]
+        L.esdg_profile_query.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_double)]
+        L.esdg_solver_create.argtypes = [C.c_void_p, C.POINTER(Inputs), C.POINTER(RunParamsC),
+                                         C.POINTER(C.c_void_p)]
+        L.esdg_solver_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.esdg_solver_get_q.argtypes = [C.c_void_p, C.c_void_p]
+        L.esdg_solver_set_q.argtypes = [C.c_void_p, C.c_void_p]
+        L.esdg_solver_time.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.esdg_solver_last_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.esdg_solver_buffer.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.esdg_solver_destroy.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def make_params(case: Case, rp: RunParams, bc_kind: int = 0, h_zero: float = 0.0) -> Params:
+    return Params(case.ngl, rp.ES, rp.num_flux, bc_kind, rp.g, case.geom_face, rp.pos_pres_tol,
+                  rp.zero_tol, rp.eps0, rp.sigma_min, rp.sigma_max, h_zero)
+
+
+class DeviceArray:
+    """A device allocation of the context, shaped like a numpy array."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx.check(lib().esdg_malloc(ctx.handle, C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @property
+    def _as_parameter_(self):
+        return C.c_void_p(self.ptr)
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.nbytes == self.nbytes, (host.nbytes, self.nbytes)
+        self.ctx.check(lib().esdg_memcpy(self.ctx.handle, C.c_void_p(self.ptr),
+                                         host.ctypes.data_as(C.c_void_p), self.nbytes, ESDG_H2D))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.ctx.check(lib().esdg_memcpy(self.ctx.handle, out.ctypes.data_as(C.c_void_p),
+                                         C.c_void_p(self.ptr), self.nbytes, ESDG_D2H))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().esdg_free(self.ctx.handle, C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """One HIP device + stream (esdg_ctx).  Kernel entries are called as ``ctx.<name>(...)``."""
+
+    def __init__(self, params: Params, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().esdg_create(C.byref(h), device, C.byref(params))
+        if rc != 0:
+            raise EsdgDeviceError(lib().esdg_last_error(None).decode())
+        self.handle = h.value
+        self.params = params
+
+    def check(self, rc):
+        if rc != 0:
+            raise EsdgDeviceError(lib().esdg_last_error(C.c_void_p(self.handle)).decode())
+
+    def array(self, host: np.ndarray) -> DeviceArray:
+        host = np.ascontiguousarray(host)
+        return DeviceArray(self, host.shape, host.dtype).upload(host)
+
+    def empty(self, shape, dtype=np.float64) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def call(self, name: str, *args):
+        fn = getattr(lib(), "esdg_" + name)
+        cargs = [C.c_void_p(self.handle)]
+        for a in args:
+            if isinstance(a, DeviceArray):
+                cargs.append(C.c_void_p(a.ptr))
+            elif a is None:
+                cargs.append(C.c_void_p(0))
+            elif isinstance(a, (bool, int, np.integer)):
+                cargs.append(C.c_int(int(a)))
+            elif isinstance(a, (float, np.floating)):
+                cargs.append(C.c_double(float(a)))
+            else:
+                raise TypeError(f"unsupported argument {a!r}")
+        self.check(fn(*cargs))
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        return lambda *args: self.call(name, *args)
+
+    def finish(self):
+        self.check(lib().esdg_finish(C.c_void_p(self.handle)))
+
+    def profile(self, on=True):
+        self.check(lib().esdg_profile_enable(C.c_void_p(self.handle), int(on)))
+
+    def profile_reset(self):
+        self.check(lib().esdg_profile_reset(C.c_void_p(self.handle)))
+
+    def profile_query(self, name):
+        n, ms = C.c_int64(), C.c_double()
+        self.check(lib().esdg_profile_query(C.c_void_p(self.handle), name.encode(), C.byref(n),
+                                            C.byref(ms)))
+        return n.value, ms.value
+
+    def bytes_allocated(self):
+        return lib().esdg_bytes_allocated(C.c_void_p(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().esdg_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+
+class Solver:
+    """deviceclass mirror: buffers + DGtimeloop on the device for one Case."""
+
+    def __init__(self, ctx: Context, case: Case, rp: RunParams, fused: bool = False):
+        self.ctx, self.case, self.rp = ctx, case, rp
+        a = case.arrays
+        self._keep = {
+            "GLw": a["w_GL"], "nx": a["nx"], "ny": a["ny"], "scal": a["scal"], "Yxi": a["yXi"],
+            "Yeta": a["yEta"], "Xxi": a["xXi"], "Xeta": a["xEta"], "B": a["b"], "Bx": a["Bx"],
+            "By": a["By"], "D": a["D"], "Dstrong": a["Dstrong"], "Dhat": a["Dhat"],
+            "Jac": a["Jac"], "x": a["x"], "y": a["y"], "q": a["q"], "EleSizes": a["EleSizes"],
+            "VdmInv": a["VdmInv"], "ElemEdgeMasterSlave": a["ElemEdgeMasterSlave"],
+            "ElemEdgeOrientation": a["ElemEdgeOrientation"], "ElemToEdge": a["ElemToEdge"],
+            "EdgeData": a["EdgeData"], "EdgeReversed": a["EdgeReversed"]}
+        if not rp.flux_differencing:
+            # FluxDifferencing = 0 builds D without the factor 2 (src/basis.cpp:108-119)
+            self._keep["D"] = a["Dstrong"]
+        inp = Inputs()
+        inp.Nelem, inp.Nfaces = case.Nelem, case.Nfaces
+        for n in _INPUT_DOUBLES:
+            arr = np.ascontiguousarray(self._keep[n], dtype=np.float64)
+            self._keep[n] = arr
+            setattr(inp, n, arr.ctypes.data_as(C.POINTER(C.c_double)))
+        for n in _INPUT_INTS:
+            arr = np.ascontiguousarray(self._keep[n], dtype=np.int32)
+            self._keep[n] = arr
+            setattr(inp, n, arr.ctypes.data_as(C.POINTER(C.c_int)))
+        rpc = RunParamsC(rp.CFL, rp.DFL, rp.T, case.min_ele_size, rp.artificial_viscosity,
+                         rp.positivity, rp.rk_order, rp.rk_ssp, rp.flux_differencing, int(fused))
+        h = C.c_void_p()
+        ctx.check(lib().esdg_solver_create(C.c_void_p(ctx.handle), C.byref(inp), C.byref(rpc),
+                                           C.byref(h)))
+        self.handle = h.value
+
+    def run(self, max_steps: int):
+        n, t = C.c_int(), C.c_double()
+        self.ctx.check(lib().esdg_solver_run(C.c_void_p(self.handle), max_steps, C.byref(n),
+                                             C.byref(t)))
+        return n.value, t.value
+
+    def get_q(self) -> np.ndarray:
+        out = np.empty(self.case.Nelem * self.case.ngl ** 2 * 3)
+        self.ctx.check(lib().esdg_solver_get_q(C.c_void_p(self.handle),
+                                               out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_q(self, q: np.ndarray):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self.ctx.check(lib().esdg_solver_set_q(C.c_void_p(self.handle),
+                                               q.ctypes.data_as(C.c_void_p)))
+
+    def last_dt(self) -> float:
+        d = C.c_double()
+        self.ctx.check(lib().esdg_solver_last_dt(C.c_void_p(self.handle), C.byref(d)))
+        return d.value
+
+    def buffer_ptr(self, name: str) -> int:
+        p = C.c_void_p()
+        self.ctx.check(lib().esdg_solver_buffer(C.c_void_p(self.handle), name.encode(), C.byref(p)))
+        return p.value
+
+    def close(self):
+        if self.handle:
+            lib().esdg_solver_destroy(C.c_void_p(self.handle))
+            self.handle = None

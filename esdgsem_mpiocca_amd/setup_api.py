@@ -1,0 +1,189 @@
+"""ctypes binding of libesdg_setup.so (include/esdg_setup.h): basis, RK tables and test cases.
+
+A :class:`Case` owns the host arrays that the reference builds in main.cpp before
+``deviceclass::copyDeviceVariables`` (geometry, connectivity, bathymetry, initial state), and a
+:class:`RunParams` holds the run flags of input.dat that the hot path reads.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class EsdgError(RuntimeError):
+    pass
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_PKG, "libesdg_setup.so")
+        if not os.path.exists(path):
+            raise EsdgError(f"{path} is missing: run __graft_entry__.build() first")
+        lib = C.CDLL(path)
+        lib.esdg_setup_last_error.restype = C.c_char_p
+        lib.esdg_case_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        lib.esdg_case_create.argtypes = [C.POINTER(CaseSpec), C.POINTER(C.c_void_p)]
+        lib.esdg_case_get_info.argtypes = [C.c_void_p, C.POINTER(CaseInfo)]
+        lib.esdg_case_destroy.argtypes = [C.c_void_p]
+        lib.esdg_case_destroy.restype = None
+        _LIB = lib
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise EsdgError(_lib().esdg_setup_last_error().decode())
+
+
+class CaseSpec(C.Structure):
+    _fields_ = [
+        ("case_kind", C.c_int), ("mesh_kind", C.c_int), ("N", C.c_int),
+        ("nelx", C.c_int), ("nely", C.c_int),
+        ("xL", C.c_double), ("xR", C.c_double), ("yL", C.c_double), ("yR", C.c_double),
+        ("periodic_x", C.c_int), ("periodic_y", C.c_int),
+        ("perturb_amp", C.c_double), ("perturb_len", C.c_double), ("g", C.c_double),
+        ("amp", C.c_double), ("p0", C.c_double), ("p1", C.c_double), ("p2", C.c_double),
+        ("p3", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+class CaseInfo(C.Structure):
+    _fields_ = [("N", C.c_int), ("ngl", C.c_int), ("Nelem", C.c_int), ("Nfaces", C.c_int),
+                ("min_ele_size", C.c_double)]
+
+
+CASE_GAUSSIAN_BUMP, CASE_DAM_BREAK, CASE_FLOOD, CASE_OCEAN_UPLIFT, CASE_CONSTANT = range(5)
+MESH_CARTESIAN, MESH_CURVED_SINE = 0, 1
+
+DOUBLE_ARRAYS = ("x", "y", "xXi", "xEta", "yXi", "yEta", "Jac", "b", "Bx", "By", "q", "nx", "ny",
+                 "scal", "EleSizes", "x_GL", "w_GL", "D", "D0", "Dstrong", "Dhat", "VdmInv")
+INT_ARRAYS = ("EdgeData", "EdgeReversed", "ElemToEdge", "ElemEdgeMasterSlave",
+              "ElemEdgeOrientation")
+
+
+@dataclasses.dataclass
+class RunParams:
+    """Run flags and constants of the hot path (input.dat lines 2-30; src/deviceclass.cpp:131-218)."""
+    g: float = 9.81
+    CFL: float = 0.5
+    DFL: float = 0.1
+    T: float = 1.0e30
+    ES: int = 1
+    num_flux: int = 0           # 0 ES (calcESFlux), 3 rotated ES (calcESFluxRotated)
+    flux_differencing: int = 1
+    artificial_viscosity: int = 0
+    positivity: int = 0
+    pos_pres_tol: float = 1.0e-8
+    eps0: float = 0.0
+    sigma_min: float = -4.0
+    sigma_max: float = -2.0
+    rk_order: int = 4
+    rk_ssp: int = 0
+    zero_tol: float = 1.0e-5    # hard-coded in the reference (src/deviceclass.cpp:148)
+
+
+class Case:
+    """A mesh + state built by libesdg_setup.  Arrays are numpy copies (safe to keep)."""
+
+    def __init__(self, case_kind, mesh_kind, N, nelx, nely, xL, xR, yL, yR, periodic_x=0,
+                 periodic_y=0, perturb_amp=0.0, perturb_len=1.0, g=9.81, amp=0.0, p=(0, 0, 0, 0),
+                 seed=0):
+        spec = CaseSpec(case_kind, mesh_kind, N, nelx, nely, xL, xR, yL, yR, periodic_x,
+                        periodic_y, perturb_amp, perturb_len, g, amp, p[0], p[1], p[2], p[3], seed)
+        self.spec = spec
+        h = C.c_void_p()
+        _check(_lib().esdg_case_create(C.byref(spec), C.byref(h)))
+        try:
+            info = CaseInfo()
+            _check(_lib().esdg_case_get_info(h, C.byref(info)))
+            self.N, self.ngl, self.Nelem, self.Nfaces = info.N, info.ngl, info.Nelem, info.Nfaces
+            self.min_ele_size = info.min_ele_size
+            self.arrays = {}
+            for name in DOUBLE_ARRAYS + INT_ARRAYS:
+                ptr, cnt, isint = C.c_void_p(), C.c_int64(), C.c_int()
+                _check(_lib().esdg_case_array(h, name.encode(), C.byref(ptr), C.byref(cnt),
+                                              C.byref(isint)))
+                ct = C.c_int32 if isint.value else C.c_double
+                buf = (ct * cnt.value).from_address(ptr.value) if cnt.value else (ct * 0)()
+                self.arrays[name] = np.array(np.ctypeslib.as_array(buf), copy=True)
+        finally:
+            _lib().esdg_case_destroy(h)
+
+    def __getitem__(self, name):
+        return self.arrays[name]
+
+    @property
+    def geom_face(self) -> float:
+        return 1.0 / float(self.arrays["w_GL"][0])
+
+    # --- the benchmark configurations -------------------------------------------------
+    @classmethod
+    def gaussian_bump(cls, N=4, nel=16, lake_at_rest=False):
+        """16x16 periodic Cartesian mesh on [0,2]^2, b = 0.8 exp(-5 r^2), h = 1-b (+0.01 bump)."""
+        return cls(CASE_GAUSSIAN_BUMP, MESH_CARTESIAN, N, nel, nel, 0.0, 2.0, 0.0, 2.0, 1, 1,
+                   amp=0.0 if lake_at_rest else 0.01)
+
+    @classmethod
+    def dam_break(cls, N=7, nel=512, noise=1e-3, seed=0, perturb=0.05):
+        """nel^2 curved quads on [0,10]^2 (x,y += 0.05 sin(pi x/5) sin(pi y/5)), walls, dam break."""
+        return cls(CASE_DAM_BREAK, MESH_CURVED_SINE, N, nel, nel, 0.0, 10.0, 0.0, 10.0, 0, 0,
+                   perturb_amp=perturb, perturb_len=5.0, amp=noise, p=(5.0, 0, 0, 0), seed=seed)
+
+    @classmethod
+    def flood(cls, N=7, nel=2048, seed=0, level_left=0.8, x_dam=3.0, level_right=-0.2,
+              perturb=0.05):
+        """nel^2 curved quads on [0,10]^2, random Fourier bathymetry, reservoir over dry land."""
+        return cls(CASE_FLOOD, MESH_CURVED_SINE, N, nel, nel, 0.0, 10.0, 0.0, 10.0, 0, 0,
+                   perturb_amp=perturb, perturb_len=5.0, p=(level_left, x_dam, level_right, 0),
+                   seed=seed)
+
+    @classmethod
+    def constant(cls, N, nel, h=1.0, hu=0.0, hv=0.0, curved=True, perturb=0.05):
+        """Constant state on flat bottom (free-stream preservation)."""
+        return cls(CASE_CONSTANT, MESH_CURVED_SINE if curved else MESH_CARTESIAN, N, nel, nel,
+                   0.0, 10.0, 0.0, 10.0, 0, 0, perturb_amp=perturb if curved else 0.0,
+                   perturb_len=5.0, p=(h, hu, hv, 0))
+
+
+def basis(N: int, flux_differencing: int = 1):
+    ngl = N + 1
+    out = {k: np.zeros(ngl if k in ("x_GL", "w_GL") else ngl * ngl)
+           for k in ("x_GL", "w_GL", "D", "D0", "Dstrong", "Dhat", "VdmInv")}
+    ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    _check(_lib().esdg_basis_build(N, flux_differencing, ptr(out["x_GL"]), ptr(out["w_GL"]),
+                                   ptr(out["D"]), ptr(out["D0"]), ptr(out["Dstrong"]),
+                                   ptr(out["Dhat"]), ptr(out["VdmInv"])))
+    return out
+
+
+def rk_coefficients(order: int, ssp: int = 0):
+    A, B, Cc = (np.zeros(5) for _ in range(3))
+    st = C.c_int()
+    ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    _check(_lib().esdg_rk_coefficients(order, ssp, C.byref(st), ptr(A), ptr(B), ptr(Cc)))
+    n = st.value
+    return A[:n].copy(), B[:n].copy(), Cc[:n].copy()
+
+
+def total_entropy(case: Case, q: np.ndarray, g: float) -> float:
+    out = C.c_double()
+    ptr = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    _check(_lib().esdg_total_entropy(case.ngl, case.Nelem, C.c_double(g), ptr(case["w_GL"]),
+                                     ptr(q), ptr(case["b"]), ptr(case["Jac"]), C.byref(out)))
+    return out.value
+
+
+def total_mass(case: Case, q: np.ndarray) -> float:
+    out = C.c_double()
+    ptr = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    _check(_lib().esdg_total_mass(case.ngl, case.Nelem, ptr(case["w_GL"]), ptr(q),
+                                  ptr(case["Jac"]), C.byref(out)))
+    return out.value

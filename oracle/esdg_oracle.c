@@ -1,0 +1,847 @@
+/*
+ * esdg_oracle.c -- TEST ORACLE.  Test infrastructure only: loaded by tests/, the smoke check and
+ * bench.py's CPU-baseline leg as the checker; never part of the product path.
+ *
+ * Plain-C restatement of the reference's per-stage kernels.  Each function states the okl file
+ * and lines it follows.  Operation order inside every expression follows the cited source, so
+ * that with -ffp-contract=off on x86-64 the results equal an OCCA Serial-mode build of the
+ * reference bit for bit (same parse tree, same IEEE rounding, no FMA).
+ *
+ * Pinning status of each function is recorded in tests/ (see the test names) and summarised in
+ * the repository's design notes; nothing here is "parity unpinned".
+ */
+#include "esdg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NEQ 3
+
+/* The reference's kernels call max/min on doubles; NaN never reaches them on the tested paths.
+ * fmax/fmin ignore a NaN operand, which is also what the device code does. */
+#define OMAX(a, b) fmax((a), (b))
+#define OMIN(a, b) fmin((a), (b))
+
+/* side -> node index of trace point k (okl/GatherEdgeData/SolidWalls.okl:74-89) */
+static int side_node(int ngl, int side, int k) {
+    switch (side) {
+        case 0: return k;                         /* bottom: j=0 */
+        case 2: return ngl * ngl - ngl + k;       /* top:    j=N */
+        case 3: return k * ngl;                   /* left:   i=0 */
+        default: return (k + 1) * ngl - 1;        /* right:  i=N */
+    }
+}
+
+/* okl/GatherEdgeData/CollectEdgeData_Bottom.okl:16-134 */
+void oracle_collect_edge_data_bottom(const oracle_params* p, int Nfaces, const int* EdgeData,
+                                     const double* B, double* bL, double* bR) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int* E = EdgeData + 8 * f;
+        const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+        for (int i = 0; i < ngl; ++i) {
+            const long id = (long)f * ngl + i;
+            bL[id] = 0.0;
+            bR[id] = 0.0;
+            if (cpuL == 0) bL[id] = B[(long)iel * ngl2 + side_node(ngl, isl, i)];
+            if (cpuR == 0) bR[id] = B[(long)ier * ngl2 + side_node(ngl, isr, i)];
+            if (cpuL == -1) bL[id] = bR[id];
+            if (cpuR == -1) bR[id] = bL[id];
+        }
+    }
+}
+
+/* wall mirror of the momentum: reflect the normal component (SolidWalls.okl:153-173) */
+static void wall_mirror(double nx, double ny, double vx, double vy, double* ox, double* oy) {
+    const double v_n = nx * vx + ny * vy;
+    const double v_t = -ny * vx + nx * vy;
+    *ox = -nx * v_n - ny * v_t;
+    *oy = -ny * v_n + nx * v_t;
+}
+
+/* okl/GatherEdgeData/SolidWalls.okl:16-207 (procID = 0, single rank) */
+void oracle_collect_edge_data_walls(const oracle_params* p, int Nfaces, const int* EdgeData,
+                                    const double* q, const double* nx, const double* ny,
+                                    double* qL, double* qR) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int* E = EdgeData + 8 * f;
+        const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+        for (int i = 0; i < ngl; ++i) {
+            const long eid = (long)f * ngl * NEQ + i;
+            const long xid = (long)f * ngl + i;
+            double L[3] = {0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0};
+            if (cpuL == 0) {
+                const long id = (long)iel * ngl2 * NEQ + side_node(ngl, isl, i);
+                L[0] = q[id]; L[1] = q[id + ngl2]; L[2] = q[id + 2 * ngl2];
+            }
+            if (cpuR == 0) {
+                const long id = (long)ier * ngl2 * NEQ + side_node(ngl, isr, i);
+                R[0] = q[id]; R[1] = q[id + ngl2]; R[2] = q[id + 2 * ngl2];
+            }
+            if (cpuL == -1) {
+                L[0] = R[0];
+                wall_mirror(nx[xid], ny[xid], R[1], R[2], &L[1], &L[2]);
+            }
+            if (cpuR == -1) {
+                R[0] = L[0];
+                wall_mirror(nx[xid], ny[xid], L[1], L[2], &R[1], &R[2]);
+            }
+            for (int v = 0; v < 3; ++v) {
+                qL[eid + v * ngl] = L[v];
+                qR[eid + v * ngl] = R[v];
+            }
+        }
+    }
+}
+
+/* two-point EC volume fluxes F# (xi direction, eta metrics) and G# (eta direction, xi metrics):
+ * okl/DG/VolumeKernelFluxDiffV99.okl:17-57 */
+static void fvol(double half_g, double fourth_g, double q1L, double q2L, double q3L, double iL,
+                 double q1R, double q2R, double q3R, double iR, double YL, double YR, double XL,
+                 double XR, double* F1, double* F2, double* F3) {
+    const double h_A = fourth_g * (q1L + q1R) * (q1L + q1R);
+    const double h2_A = 0.5 * (q1L * q1L + q1R * q1R);
+    const double hu_A = 0.5 * (q2L + q2R);
+    const double hv_A = 0.5 * (q3L + q3R);
+    const double u_A = 0.5 * (q2L * iL + q2R * iR);
+    const double v_A = 0.5 * (q3L * iL + q3R * iR);
+    const double Y_A = 0.5 * (YL + YR);
+    const double X_A = 0.5 * (XL + XR);
+    const double pr = h_A - half_g * h2_A;
+    *F1 = Y_A * hu_A - X_A * hv_A;
+    *F2 = Y_A * (hu_A * u_A + pr) - X_A * hv_A * u_A;
+    *F3 = Y_A * (hu_A * v_A) - X_A * (hv_A * v_A + pr);
+}
+static void gvol(double half_g, double fourth_g, double q1L, double q2L, double q3L, double iL,
+                 double q1R, double q2R, double q3R, double iR, double YL, double YR, double XL,
+                 double XR, double* G1, double* G2, double* G3) {
+    const double h_A = fourth_g * (q1L + q1R) * (q1L + q1R);
+    const double h2_A = 0.5 * (q1L * q1L + q1R * q1R);
+    const double hu_A = 0.5 * (q2L + q2R);
+    const double hv_A = 0.5 * (q3L + q3R);
+    const double u_A = 0.5 * (q2L * iL + q2R * iR);
+    const double v_A = 0.5 * (q3L * iL + q3R * iR);
+    const double Y_A = 0.5 * (YL + YR);
+    const double X_A = 0.5 * (XL + XR);
+    const double pr = h_A - half_g * h2_A;
+    *G1 = -Y_A * hu_A + X_A * hv_A;
+    *G2 = -Y_A * (hu_A * u_A + pr) + X_A * hv_A * u_A;
+    *G3 = -Y_A * (hu_A * v_A) + X_A * (hv_A * v_A + pr);
+}
+
+/* okl/DG/VolumeKernelFluxDiffV99.okl:60-218.  Node (i = eta row, j = xi column) as in the okl. */
+void oracle_volume_fluxdiff(const oracle_params* p, int Nelem, const double* Jac, const double* Yxi,
+                            const double* Yeta, const double* Xxi, const double* Xeta, const double* q,
+                            const double* D, const double* Bx, const double* By, double* Qt) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    const double g = p->g, half_g = 0.5 * g, fourth_g = 0.25 * g;
+    double* inv = (double*)malloc(sizeof(double) * ngl2);
+    for (int e = 0; e < Nelem; ++e) {
+        const double* Q1 = q + (long)e * ngl2 * NEQ;
+        const double* Q2 = Q1 + ngl2;
+        const double* Q3 = Q2 + ngl2;
+        const long x0 = (long)e * ngl2;
+        for (int n = 0; n < ngl2; ++n) inv[n] = (Q1[n] > 0) ? 1.0 / Q1[n] : 0.0;
+        for (int i = 0; i < ngl; ++i)
+            for (int j = 0; j < ngl; ++j) {
+                const int n = i * ngl + j;
+                double FD1 = 0.0, FD2 = 0.0, FD3 = 0.0;
+                for (int l = 0; l < ngl; ++l) {
+                    const int m = i * ngl + l;
+                    const double djl = D[j * ngl + l];
+                    double F1, F2, F3;
+                    fvol(half_g, fourth_g, Q1[n], Q2[n], Q3[n], inv[n], Q1[m], Q2[m], Q3[m], inv[m],
+                         Yeta[x0 + n], Yeta[x0 + m], Xeta[x0 + n], Xeta[x0 + m], &F1, &F2, &F3);
+                    FD1 += djl * F1;
+                    FD2 += djl * F2;
+                    FD3 += djl * F3;
+                }
+                for (int l = 0; l < ngl; ++l) {
+                    const int m = l * ngl + j;
+                    const double dil = D[i * ngl + l];
+                    double G1, G2, G3;
+                    gvol(half_g, fourth_g, Q1[n], Q2[n], Q3[n], inv[n], Q1[m], Q2[m], Q3[m], inv[m],
+                         Yxi[x0 + n], Yxi[x0 + m], Xxi[x0 + n], Xxi[x0 + m], &G1, &G2, &G3);
+                    FD1 += dil * G1;
+                    FD2 += dil * G2;
+                    FD3 += dil * G3;
+                }
+                const double rJ = Jac[x0 + n];
+                double* out = Qt + (long)e * ngl2 * NEQ + n;
+                out[0] = -rJ * FD1;
+                out[ngl2] = -rJ * FD2 - g * Q1[n] * Bx[x0 + n];
+                out[2 * ngl2] = -rJ * FD3 - g * Q1[n] * By[x0 + n];
+            }
+    }
+    free(inv);
+}
+
+/* index of the right state for left-ordered face node i (calcESFlux.okl:24-26) */
+static int rev_idx(int ngl, int reversed, int i) {
+    return (1 - reversed) * (ngl - 1 - i) + reversed * i;
+}
+
+/* Entropy-stable flux in Cartesian momentum: okl/RiemannSolvers/calcESFlux.okl:17-215 */
+void oracle_num_flux_es(const oracle_params* p, int Nfaces, const int* EdgeReversed,
+                        const double* nxa, const double* nya, const double* scal, const double* qL,
+                        const double* qR, const double* BL, const double* BR, double* SP) {
+    const int ngl = p->ngl;
+    const double g = p->g, tol = p->PosPresTOL;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int rv = EdgeReversed[f];
+        for (int i = 0; i < ngl; ++i) {
+            const long xid = (long)f * ngl + i;
+            const long eid = (long)f * ngl * NEQ + i;
+            const long eidR = (long)f * ngl * NEQ + rev_idx(ngl, rv, i);
+            const long xidR = (long)f * ngl + rev_idx(ngl, rv, i);
+            const double nx = nxa[xid], ny = nya[xid], sc = scal[xid];
+            double hL = qL[eid], uL, vL;
+            if (hL >= tol) { uL = qL[eid + ngl] / hL; vL = qL[eid + 2 * ngl] / hL; }
+            else { hL = 0.0; uL = 0.0; vL = 0.0; }
+            double hR = qR[eidR], uR, vR;
+            if (hR >= tol) { uR = qR[eidR + ngl] / hR; vR = qR[eidR + 2 * ngl] / hR; }
+            else { hR = 0.0; uR = 0.0; vR = 0.0; }
+            const double avgH = 0.5 * (hL + hR);
+            const double avgH2 = 0.5 * (hL * hL + hR * hR);
+            const double avgU = 0.5 * (uL + uR);
+            const double avgV = 0.5 * (vL + vR);
+            const double normV = avgU * nx + avgV * ny;
+            double NF0 = avgH * normV;
+            double NF1 = avgH * avgU * normV + (0.5 * g * avgH2) * nx;
+            double NF2 = avgH * avgV * normV + (0.5 * g * avgH2) * ny;
+            if (p->ES) {
+                const double bL = BL[xid], bR = BR[xidR];
+                const double cL = g * hL, cR = g * hR;
+                const double avgC = sqrt(fabs(0.5 * (cL + cR)));
+                const double q1L = g * (hL + bL) - 0.5 * uL * uL - 0.5 * vL * vL;
+                const double q1R = g * (hR + bR) - 0.5 * uR * uR - 0.5 * vR * vR;
+                const double j1 = q1R - q1L, j2 = uR - uL, j3 = vR - vL;
+                const double R11 = 1.0, R12 = 0.0, R13 = 1.0;
+                const double R21 = avgU + nx * avgC, R22 = ny, R23 = avgU - nx * avgC;
+                const double R31 = avgV + ny * avgC, R32 = nx, R33 = avgV - ny * avgC;
+                const double SL11 = 0.5 / g * fabs(nx * avgU + ny * avgV + avgC);
+                const double SL22 = fabs(avgH * (nx * avgU + ny * avgV));
+                const double SL33 = 0.5 / g * fabs(nx * avgU + ny * avgV - avgC);
+                const double S11 = R11 * SL11, S12 = 0.0, S13 = R13 * SL33;
+                const double S21 = R21 * SL11, S22 = R22 * SL22, S23 = R23 * SL33;
+                const double S31 = R31 * SL11, S32 = R32 * SL22, S33 = R33 * SL33;
+                const double M11 = S11 * R11 + S12 * R12 + S13 * R13;
+                const double M12 = S11 * R21 + S12 * R22 + S13 * R23;
+                const double M13 = S11 * R31 + S12 * R32 + S13 * R33;
+                const double M21 = S21 * R11 + S22 * R12 + S23 * R13;
+                const double M22 = S21 * R21 + S22 * R22 + S23 * R23;
+                const double M23 = S21 * R31 + S22 * R32 + S23 * R33;
+                const double M31 = S31 * R11 + S32 * R12 + S33 * R13;
+                const double M32 = S31 * R21 + S32 * R22 + S33 * R23;
+                const double M33 = S31 * R31 + S32 * R32 + S33 * R33;
+                NF0 += -0.5 * (M11 * j1 + M12 * j2 + M13 * j3);
+                NF1 += -0.5 * (M21 * j1 + M22 * j2 + M23 * j3);
+                NF2 += -0.5 * (M31 * j1 + M32 * j2 + M33 * j3);
+            }
+            SP[eid] = sc * p->geomFace * NF0;
+            SP[eid + ngl] = sc * p->geomFace * NF1;
+            SP[eid + 2 * ngl] = sc * p->geomFace * NF2;
+        }
+    }
+}
+
+/* Entropy-stable flux in the face (normal, tangential) frame, rotated back:
+ * okl/RiemannSolvers/calcESFluxRotated.okl:17-214 */
+void oracle_num_flux_es_rotated(const oracle_params* p, int Nfaces, const int* EdgeReversed,
+                                const double* nxa, const double* nya, const double* scal,
+                                const double* qL, const double* qR, const double* BL,
+                                const double* BR, double* SP) {
+    const int ngl = p->ngl;
+    const double g = p->g, tol = p->PosPresTOL;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int rv = EdgeReversed[f];
+        for (int i = 0; i < ngl; ++i) {
+            const long xid = (long)f * ngl + i;
+            const long eid = (long)f * ngl * NEQ + i;
+            const long eidR = (long)f * ngl * NEQ + rev_idx(ngl, rv, i);
+            const long xidR = (long)f * ngl + rev_idx(ngl, rv, i);
+            const double nx = nxa[xid], ny = nya[xid], sc = scal[xid];
+            const double hL = qL[eid];
+            double uL, vL;
+            if (hL > tol) { uL = qL[eid + ngl] / hL; vL = qL[eid + 2 * ngl] / hL; }
+            else { uL = 0.0; vL = 0.0; }
+            const double hR = qR[eidR];
+            double uR, vR;
+            if (hR > tol) { uR = qR[eidR + ngl] / hR; vR = qR[eidR + 2 * ngl] / hR; }
+            else { uR = 0.0; vR = 0.0; }
+            const double vLn = nx * uL + ny * vL, vLt = -ny * uL + nx * vL;
+            const double vRn = nx * uR + ny * vR, vRt = -ny * uR + nx * vR;
+            const double avgH = 0.5 * (hL + hR);
+            const double avgH2 = 0.5 * (hL * hL + hR * hR);
+            const double avgU = 0.5 * (vLn + vRn);
+            const double avgV = 0.5 * (vLt + vRt);
+            double NF0 = avgH * avgU;
+            double NF1 = avgH * avgU * avgU + (0.5 * g * avgH2);
+            double NF2 = avgH * avgV * avgU;
+            if (p->ES) {
+                const double bL = BL[xid], bR = BR[xidR];
+                const double cL = g * hL, cR = g * hR;
+                const double avgC = sqrt(fabs(0.5 * (cL + cR)));
+                const double q1L = g * (hL + bL) - 0.5 * vLn * vLn - 0.5 * vLt * vLt;
+                const double q1R = g * (hR + bR) - 0.5 * vRn * vRn - 0.5 * vRt * vRt;
+                const double j1 = q1R - q1L, j2 = vRn - vLn, j3 = vRt - vLt;
+                const double R11 = 1.0, R12 = 0.0, R13 = 1.0;
+                const double R21 = avgU + avgC, R22 = 0.0, R23 = avgU - avgC;
+                const double R31 = avgV, R32 = 1.0, R33 = avgV;
+                const double SL11 = 0.5 / g * fabs(avgU + avgC);
+                const double SL22 = fabs(avgH * avgU);
+                const double SL33 = 0.5 / g * fabs(avgU - avgC);
+                const double S11 = R11 * SL11, S12 = 0.0, S13 = R13 * SL33;
+                const double S21 = R21 * SL11, S22 = R22 * SL22, S23 = R23 * SL33;
+                const double S31 = R31 * SL11, S32 = R32 * SL22, S33 = R33 * SL33;
+                const double M11 = S11 * R11 + S12 * R12 + S13 * R13;
+                const double M12 = S11 * R21 + S12 * R22 + S13 * R23;
+                const double M13 = S11 * R31 + S12 * R32 + S13 * R33;
+                const double M21 = S21 * R11 + S22 * R12 + S23 * R13;
+                const double M22 = S21 * R21 + S22 * R22 + S23 * R23;
+                const double M23 = S21 * R31 + S22 * R32 + S23 * R33;
+                const double M31 = S31 * R11 + S32 * R12 + S33 * R13;
+                const double M32 = S31 * R21 + S32 * R22 + S33 * R23;
+                const double M33 = S31 * R31 + S32 * R32 + S33 * R33;
+                NF0 += -0.5 * (M11 * j1 + M12 * j2 + M13 * j3);
+                NF1 += -0.5 * (M21 * j1 + M22 * j2 + M23 * j3);
+                NF2 += -0.5 * (M31 * j1 + M32 * j2 + M33 * j3);
+            }
+            const double r1 = nx * NF1 - ny * NF2;
+            const double r2 = ny * NF1 + nx * NF2;
+            SP[eid] = sc * p->geomFace * NF0;
+            SP[eid + ngl] = sc * p->geomFace * r1;
+            SP[eid + 2 * ngl] = sc * p->geomFace * r2;
+        }
+    }
+}
+
+/* node index and face-node index of trace point k of side s, for the surface kernels
+ * (okl/DG/SurfaceKernel.okl:34-72 sides 0/2, :92-130 sides 1/3) */
+static int surf_node(int ngl, int side, int k) {
+    switch (side) {
+        case 0: return k;
+        case 2: return ngl * ngl - ngl + k;
+        case 1: return k * ngl + ngl - 1;
+        default: return k * ngl;
+    }
+}
+
+/* okl/DG/SurfaceKernel.okl:17-142 */
+void oracle_surface(const oracle_params* p, int Nelem, const double* Jac, const int* EMS,
+                    const int* EOR, const int* E2E, const double* FStar, const double* DB1,
+                    const double* DB2, double* Qt) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    static const int order[4] = {0, 2, 1, 3};
+    for (int e = 0; e < Nelem; ++e)
+        for (int o = 0; o < 4; ++o) {
+            const int s = order[o];
+            const int ifa = E2E[e * 4 + s] - 1;
+            const int rev = EOR[e * 4 + s];
+            const int ms = EMS[e * 4 + s];
+            for (int k = 0; k < ngl; ++k) {
+                const int inc = rev * k + (1 - rev) * (ngl - 1 - k);
+                const long fl = (long)ifa * ngl * NEQ + inc;
+                const long ex = (long)ifa * ngl + inc;
+                const int nd = surf_node(ngl, s, k);
+                const double rJ = Jac[(long)e * ngl2 + nd];
+                double* out = Qt + (long)e * ngl2 * NEQ + nd;
+                out[0] += -ms * FStar[fl] * rJ;
+                out[ngl2] += -ms * rJ * FStar[fl + ngl] + rJ * DB1[ex];
+                out[2 * ngl2] += -ms * rJ * FStar[fl + 2 * ngl] + rJ * DB2[ex];
+            }
+        }
+}
+
+/* okl/DG/FindLambdaMax.okl:17-100 */
+void oracle_find_lambda_max(const oracle_params* p, int Nelem, const double* q, double* LambdaMax) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    for (int e = 0; e < Nelem; ++e) {
+        double m = 0.0;
+        for (int n = 0; n < ngl2; ++n) {
+            const long id = (long)e * ngl2 * NEQ + n;
+            const double h = q[id];
+            const double Qinv = (h > p->PosPresTOL) ? 1.0 / h : 0.0;
+            const double c = sqrt(p->g * h);
+            const double u = q[id + ngl2] * Qinv;
+            const double v = q[id + 2 * ngl2] * Qinv;
+            m = OMAX(m, c + OMAX(fabs(u), fabs(v)));
+        }
+        LambdaMax[e] = m;
+    }
+}
+
+/* okl/RungeKutta/rkLS.okl:17-89 */
+void oracle_rk_ls_calc(const oracle_params* p, int Nelem, const double* Qt, double rkA, double* gRK) {
+    const long n = (long)Nelem * p->ngl * p->ngl * NEQ;
+    for (long i = 0; i < n; ++i) gRK[i] = rkA * gRK[i] + Qt[i];
+}
+
+/* okl/RungeKutta/UpdateKernelLS.okl:17-78 */
+void oracle_rk_ls_update(const oracle_params* p, int Nelem, double rkC, double dt, const double* gRK,
+                         double* q) {
+    const long n = (long)Nelem * p->ngl * p->ngl * NEQ;
+    for (long i = 0; i < n; ++i) q[i] += rkC * dt * gRK[i];
+}
+
+/* okl/RungeKutta/UpdateKernelSSP_V1.okl:17-98 */
+void oracle_rk_ssp_update(const oracle_params* p, int Nelem, double a, double b, double c, double dt,
+                          const double* Qt, const double* qTmp, double* q) {
+    const long n = (long)Nelem * p->ngl * p->ngl * NEQ;
+    for (long i = 0; i < n; ++i) q[i] = a * qTmp[i] + b * q[i] + c * dt * Qt[i];
+}
+
+/* okl/ShockCapturing/ShockCapturing.okl:17-234.  U[a][c] is h+b at (xi = a, eta = c). */
+void oracle_shock_capturing(const oracle_params* p, int Nelem, const double* q, const double* b,
+                            const double* VdmInv, double* ViscPara, double* ViscForPlot) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl, N = ngl - 1;
+    double* U = (double*)malloc(sizeof(double) * ngl2);
+    double* M1 = (double*)malloc(sizeof(double) * ngl2);
+    double* m = (double*)malloc(sizeof(double) * ngl2);
+    for (int e = 0; e < Nelem; ++e) {
+        for (int a = 0; a < ngl; ++a)
+            for (int c = 0; c < ngl; ++c)
+                U[a * ngl + c] = b[(long)e * ngl2 + c * ngl + a] + q[(long)e * ngl2 * NEQ + c * ngl + a];
+        for (int a = 0; a < ngl; ++a)
+            for (int c = 0; c < ngl; ++c) {
+                double s = 0.0;
+                for (int l = 0; l < ngl; ++l) s += VdmInv[a * ngl + l] * U[l * ngl + c];
+                M1[a * ngl + c] = s;
+            }
+        for (int a = 0; a < ngl; ++a)
+            for (int c = 0; c < ngl; ++c) {
+                double s = 0.0;
+                for (int l = 0; l < ngl; ++l) s += M1[a * ngl + l] * VdmInv[c * ngl + l];
+                m[a * ngl + c] = s;
+            }
+        double z1 = 0.0, n1 = 0.0, z2 = 0.0, n2 = 0.0, t1, t2, visc;
+        for (int i = 0; i < N; ++i)
+            z1 += m[i * ngl + N] * m[i * ngl + N] + m[N * ngl + i] * m[N * ngl + i];
+        z1 += m[N * ngl + N] * m[N * ngl + N];
+        for (int i = 0; i < N - 1; ++i)
+            z2 += m[i * ngl + N - 1] * m[i * ngl + N - 1] + m[(N - 1) * ngl + i] * m[(N - 1) * ngl + i];
+        z2 += m[(N - 1) * ngl + N - 1] * m[(N - 1) * ngl + N - 1];
+        for (int i = 0; i < ngl; ++i)
+            for (int j = 0; j < ngl; ++j) n1 += m[i * ngl + j] * m[i * ngl + j];
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) n2 += m[i * ngl + j] * m[i * ngl + j];
+        t1 = (n1 < p->ZeroTOL) ? 0.0 : z1 / n1;
+        t2 = (n1 < p->ZeroTOL) ? 0.0 : z2 / n2;   /* the reference tests nenner1 for both terms */
+        double sigma = OMAX(t1, t2);
+        sigma = log10(sigma);
+        if (p->sigmaMin >= sigma) visc = 0.0;
+        else if (p->sigmaMax >= sigma) {
+            const double PI = atan(1.0) * 4;
+            const double delta = 1.0 + sin(PI * (sigma - 0.5 * (p->sigmaMax + p->sigmaMin)) / (p->sigmaMax - p->sigmaMin));
+            visc = p->eps0 * 0.5 * delta;
+        } else visc = p->eps0;
+        ViscPara[e] = visc;
+        if (ViscForPlot) ViscForPlot[e] = visc;
+    }
+    free(U); free(M1); free(m);
+}
+
+/* okl/Positivity/PosPres.okl:17-223 */
+void oracle_pos_pres(const oracle_params* p, int Nelem, const double* EleSizes, const double* GLw,
+                     const double* Jac, double* q) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    const double tol = p->PosPresTOL;
+    for (int e = 0; e < Nelem; ++e) {
+        double* Q1 = q + (long)e * ngl2 * NEQ;
+        double* Q2 = Q1 + ngl2;
+        double* Q3 = Q2 + ngl2;
+        double Hmin = 0.0, Havg = 0.0, HUavg = 0.0, HVavg = 0.0;
+        for (int l = 0; l < ngl; ++l)
+            for (int k = 0; k < ngl; ++k) {
+                const int n = k * ngl + l;
+                const double w = GLw[l] * GLw[k] / Jac[(long)e * ngl2 + n];
+                Havg += Q1[n] * w;
+                HUavg += Q2[n] * w;
+                HVavg += Q3[n] * w;
+                Hmin = OMIN(Hmin, Q1[n]);
+            }
+        Havg = Havg / EleSizes[e];
+        if (fabs(Havg) < p->ZeroTOL) Havg = 0.0;
+        const double theta = (Havg == Hmin) ? 1.0 : OMIN(1.0, Havg / (Havg - Hmin));
+        const double huA = HUavg / EleSizes[e];
+        const double hvA = HVavg / EleSizes[e];
+        for (int n = 0; n < ngl2; ++n) {
+            if (Hmin < 0.0) {
+                double hn = 0.0, hun = 0.0, hvn = 0.0;
+                if (Havg > 0.0) {
+                    hn = fabs(theta * (Q1[n] - Havg) + Havg);
+                    if (hn > tol) {
+                        hun = theta * (Q2[n] - huA) + huA;
+                        hvn = theta * (Q3[n] - hvA) + hvA;
+                    }
+                }
+                if (Havg < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
+                Q1[n] = hn; Q2[n] = hun; Q3[n] = hvn;
+            } else if (Q1[n] < tol) {
+                Q2[n] = 0.0; Q3[n] = 0.0;
+            }
+        }
+    }
+}
+
+/* okl/BR1_Gradient/calcNumFluxesGradient.okl:16-135 (values read unreversed, as there) */
+void oracle_num_fluxes_gradient(const oracle_params* p, int Nfaces, const int* EdgeData,
+                                const double* nxa, const double* nya, const double* scal,
+                                const double* qL, const double* qR, double* SGX, double* SGY) {
+    const int ngl = p->ngl;
+    const double tol = p->PosPresTOL;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int cpuL = EdgeData[8 * f + 5], cpuR = EdgeData[8 * f + 6];
+        for (int i = 0; i < ngl; ++i) {
+            const long eid = (long)f * ngl * NEQ + i;
+            const long gid = (long)f * ngl * 2 + i;
+            const long xid = (long)f * ngl + i;
+            const double nx = nxa[xid], ny = nya[xid], sc = scal[xid];
+            const double* LL = (cpuL == -1) ? qR : qL;
+            const double* RR = (cpuR == -1) ? qL : qR;
+            const double hL = LL[eid], huL = LL[eid + ngl], hvL = LL[eid + 2 * ngl];
+            const double hR = RR[eid], huR = RR[eid + ngl], hvR = RR[eid + 2 * ngl];
+            const double iL = (hL > tol) ? 1.0 / hL : 0.0;
+            const double iR = (hR > tol) ? 1.0 / hR : 0.0;
+            const double uL = huL * iL, uR = huR * iR, vL = hvL * iL, vR = hvR * iR;
+            SGX[gid] = nx * sc * p->geomFace * 0.5 * (uL + uR);
+            SGX[gid + ngl] = nx * sc * p->geomFace * 0.5 * (vL + vR);
+            SGY[gid] = ny * sc * p->geomFace * 0.5 * (uL + uR);
+            SGY[gid + ngl] = ny * sc * p->geomFace * 0.5 * (vL + vR);
+        }
+    }
+}
+
+/* okl/BR1_Gradient/calcGradient.okl:18-156; node (xi = a, eta = c) */
+void oracle_calc_gradient(const oracle_params* p, int Nelem, const double* Jac, const double* Yxi,
+                          const double* Yeta, const double* Xxi, const double* Xeta, const double* q,
+                          const double* Dh, double* gX, double* gY) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    double* u = (double*)malloc(sizeof(double) * ngl2);
+    double* v = (double*)malloc(sizeof(double) * ngl2);
+    for (int e = 0; e < Nelem; ++e) {
+        const double* Q = q + (long)e * ngl2 * NEQ;
+        for (int n = 0; n < ngl2; ++n) {
+            const double h = Q[n];
+            const double Qinv = (h > p->PosPresTOL) ? 1.0 / h : 0.0;
+            u[n] = Q[n + ngl2] * Qinv;
+            v[n] = Q[n + 2 * ngl2] * Qinv;
+        }
+        for (int c = 0; c < ngl; ++c)
+            for (int a = 0; a < ngl; ++a) {
+                const int n = c * ngl + a;
+                const long xid = (long)e * ngl2 + n;
+                double u_xi = 0.0, u_eta = 0.0, v_xi = 0.0, v_eta = 0.0;
+                for (int l = 0; l < ngl; ++l) {
+                    u_xi += Dh[a * ngl + l] * u[c * ngl + l];
+                    u_eta += Dh[c * ngl + l] * u[l * ngl + a];
+                    v_xi += Dh[a * ngl + l] * v[c * ngl + l];
+                    v_eta += Dh[c * ngl + l] * v[l * ngl + a];
+                }
+                const double rJ = Jac[xid];
+                const double u_x = Yeta[xid] * u_xi - Yxi[xid] * u_eta;
+                const double v_x = Yeta[xid] * v_xi - Yxi[xid] * v_eta;
+                const double u_y = -Xeta[xid] * u_xi + Xxi[xid] * u_eta;
+                const double v_y = -Xeta[xid] * v_xi + Xxi[xid] * v_eta;
+                const long id = (long)e * ngl2 * 2 + n;
+                gX[id] = rJ * u_x;
+                gX[id + ngl2] = rJ * v_x;
+                gY[id] = rJ * u_y;
+                gY[id + ngl2] = rJ * v_y;
+            }
+    }
+    free(u); free(v);
+}
+
+/* okl/BR1_Gradient/SurfaceKernelGradient.okl:16-148 */
+void oracle_surface_gradient(const oracle_params* p, int Nelem, const double* Jac, const int* EMS,
+                             const int* EOR, const int* E2E, const double* SGX, const double* SGY,
+                             double* gX, double* gY) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    static const int order[4] = {0, 2, 1, 3};
+    for (int e = 0; e < Nelem; ++e)
+        for (int o = 0; o < 4; ++o) {
+            const int s = order[o];
+            const int ifa = E2E[e * 4 + s] - 1;
+            const int rev = EOR[e * 4 + s];
+            const int ms = EMS[e * 4 + s];
+            for (int k = 0; k < ngl; ++k) {
+                const int inc = rev * k + (1 - rev) * (ngl - 1 - k);
+                const long fl = (long)ifa * ngl * 2 + inc;
+                const int nd = surf_node(ngl, s, k);
+                const double rJ = Jac[(long)e * ngl2 + nd];
+                const long id = (long)e * ngl2 * 2 + nd;
+                if (s == 0 || s == 2) gX[id] += ms * SGX[fl] * rJ;
+                else gX[id] += ms * rJ * SGX[fl];
+                gX[id + ngl2] += ms * rJ * SGX[fl + ngl];
+                gY[id] += ms * rJ * SGY[fl];
+                gY[id + ngl2] += ms * rJ * SGY[fl + ngl];
+            }
+        }
+}
+
+/* okl/BR1_Gradient/scaleGradient.okl:17-60 */
+void oracle_scale_gradient(const oracle_params* p, int Nelem, const double* q, double* gX, double* gY) {
+    const int ngl2 = p->ngl * p->ngl;
+    for (int e = 0; e < Nelem; ++e)
+        for (int n = 0; n < ngl2; ++n) {
+            const double h = q[(long)e * ngl2 * NEQ + n];
+            const long id = (long)e * ngl2 * 2 + n;
+            gX[id] = h * gX[id];
+            gX[id + ngl2] = h * gX[id + ngl2];
+            gY[id] = h * gY[id];
+            gY[id + ngl2] = h * gY[id + ngl2];
+        }
+}
+
+/* okl/BR1_Gradient/CollectEdgeDataGradient.okl:17-194.  The reference leaves the right-side
+ * traces of boundary faces unset; they are never used (the viscous flux is zero on boundary
+ * faces), and the oracle writes zero there. */
+void oracle_collect_edge_data_gradient(const oracle_params* p, int Nfaces, const int* EdgeData,
+                                       const double* gX, const double* gY, const double* ViscPara,
+                                       double* VL, double* VR, double* gXL, double* gXR,
+                                       double* gYL, double* gYR) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int* E = EdgeData + 8 * f;
+        const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], rv = E[4], cpuL = E[5], cpuR = E[6];
+        for (int i = 0; i < ngl; ++i) {
+            const int inc = (rv == 0) ? (ngl - 1) - i : i;
+            const long eid = (long)f * ngl * 2 + i;
+            double XL[2] = {0, 0}, XR[2] = {0, 0}, YL[2] = {0, 0}, YR[2] = {0, 0};
+            if (i == 0) { VL[f] = 0.0; VR[f] = 0.0; }
+            if (cpuL == 0) {
+                const long id = (long)iel * ngl2 * 2 + side_node(ngl, isl, i);
+                XL[0] = gX[id]; XL[1] = gX[id + ngl2]; YL[0] = gY[id]; YL[1] = gY[id + ngl2];
+                if (i == 0) VL[f] = ViscPara[iel];
+            }
+            if (cpuR == 0) {
+                const long id = (long)ier * ngl2 * 2 + side_node(ngl, isr, inc);
+                XR[0] = gX[id]; XR[1] = gX[id + ngl2]; YR[0] = gY[id]; YR[1] = gY[id + ngl2];
+                if (i == 0) VR[f] = ViscPara[ier];
+            }
+            if (cpuL == -1) {
+                if (i == 0) { VL[f] = ViscPara[ier]; VR[f] = ViscPara[ier]; }
+                XL[0] = 0.0; XL[1] = 0.0;
+            }
+            if (cpuR == -1) {
+                if (i == 0) { VL[f] = ViscPara[iel]; VR[f] = ViscPara[iel]; }
+                XL[0] = 0.0; XL[1] = 0.0;
+            }
+            gXL[eid] = XL[0]; gXL[eid + ngl] = XL[1];
+            gXR[eid] = XR[0]; gXR[eid + ngl] = XR[1];
+            gYL[eid] = YL[0]; gYL[eid + ngl] = YL[1];
+            gYR[eid] = YR[0]; gYR[eid + ngl] = YR[1];
+        }
+    }
+}
+
+/* okl/ViscoseParts/VolumeKernelViscose.okl:17-150; node (xi = a, eta = c) */
+void oracle_volume_viscose(const oracle_params* p, int Nelem, const double* Jac, const double* Yxi,
+                           const double* Yeta, const double* Xxi, const double* Xeta,
+                           const double* gX, const double* gY, const double* Ds,
+                           const double* ViscPara, double* Qt) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    double* F2 = (double*)malloc(sizeof(double) * ngl2);
+    double* F3 = (double*)malloc(sizeof(double) * ngl2);
+    double* G2 = (double*)malloc(sizeof(double) * ngl2);
+    double* G3 = (double*)malloc(sizeof(double) * ngl2);
+    for (int e = 0; e < Nelem; ++e) {
+        const double eps = ViscPara[e];
+        for (int n = 0; n < ngl2; ++n) {
+            const long xid = (long)e * ngl2 + n;
+            const long id = (long)e * ngl2 * 2 + n;
+            const double f2 = eps * gX[id], f3 = eps * gX[id + ngl2];
+            const double g2 = eps * gY[id], g3 = eps * gY[id + ngl2];
+            F2[n] = Yeta[xid] * f2 - Xeta[xid] * g2;
+            G2[n] = -Yxi[xid] * f2 + Xxi[xid] * g2;
+            F3[n] = Yeta[xid] * f3 - Xeta[xid] * g3;
+            G3[n] = -Yxi[xid] * f3 + Xxi[xid] * g3;
+        }
+        for (int c = 0; c < ngl; ++c)
+            for (int a = 0; a < ngl; ++a) {
+                double d2 = 0.0, d3 = 0.0;
+                for (int l = 0; l < ngl; ++l) {
+                    d2 += Ds[a * ngl + l] * F2[c * ngl + l];
+                    d2 += Ds[c * ngl + l] * G2[l * ngl + a];
+                    d3 += Ds[a * ngl + l] * F3[c * ngl + l];
+                    d3 += Ds[c * ngl + l] * G3[l * ngl + a];
+                }
+                const int n = c * ngl + a;
+                const double rJ = Jac[(long)e * ngl2 + n];
+                double* out = Qt + (long)e * ngl2 * NEQ + n;
+                out[ngl2] += rJ * d2;
+                out[2 * ngl2] += rJ * d3;
+            }
+    }
+    free(F2); free(F3); free(G2); free(G3);
+}
+
+/* okl/ViscoseParts/calcNumFluxesViscose.okl:17-107 */
+void oracle_num_fluxes_viscose(const oracle_params* p, int Nfaces, const int* EdgeData,
+                               const double* nxa, const double* nya, const double* scal,
+                               const double* VL, const double* VR, const double* gXL,
+                               const double* gXR, const double* gYL, const double* gYR,
+                               double* SPV) {
+    const int ngl = p->ngl;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int cpuL = EdgeData[8 * f + 5], cpuR = EdgeData[8 * f + 6];
+        for (int i = 0; i < ngl; ++i) {
+            const long xid = (long)f * ngl + i;
+            const long eid = (long)f * ngl * 2 + i;
+            const double nx = nxa[xid], ny = nya[xid], sc = scal[xid];
+            const double f2L = nx * gXL[eid] + ny * gYL[eid];
+            const double f2R = nx * gXR[eid] + ny * gYR[eid];
+            const double f3L = nx * gXL[eid + ngl] + ny * gYL[eid + ngl];
+            const double f3R = nx * gXR[eid + ngl] + ny * gYR[eid + ngl];
+            double NF0, NF1;
+            if (cpuL == -1 || cpuR == -1) { NF0 = 0.0; NF1 = 0.0; }
+            else {
+                NF0 = 0.5 * OMAX(VL[f], VR[f]) * (f2L + f2R);
+                NF1 = 0.5 * OMAX(VL[f], VR[f]) * (f3L + f3R);
+            }
+            SPV[eid] = -sc * p->geomFace * NF0;
+            SPV[eid + ngl] = -sc * p->geomFace * NF1;
+        }
+    }
+}
+
+/* okl/ViscoseParts/SurfaceKernelVisc.okl:17-144 */
+void oracle_surface_visc(const oracle_params* p, int Nelem, const double* Jac, const int* EMS,
+                         const int* EOR, const int* E2E, const double* SPV, double* Qt) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    static const int order[4] = {0, 2, 1, 3};
+    for (int e = 0; e < Nelem; ++e)
+        for (int o = 0; o < 4; ++o) {
+            const int s = order[o];
+            const int ifa = E2E[e * 4 + s] - 1;
+            const int rev = EOR[e * 4 + s];
+            const int ms = EMS[e * 4 + s];
+            for (int k = 0; k < ngl; ++k) {
+                const int inc = rev * k + (1 - rev) * (ngl - 1 - k);
+                const long fl = (long)ifa * ngl * 2 + inc;
+                const int nd = surf_node(ngl, s, k);
+                const double rJ = Jac[(long)e * ngl2 + nd];
+                double* out = Qt + (long)e * ngl2 * NEQ + nd;
+                out[ngl2] += -ms * rJ * SPV[fl];
+                out[2 * ngl2] += -ms * rJ * SPV[fl + ngl];
+            }
+        }
+}
+
+/* deviceclass::DGtimeloop, src/deviceclass.cpp:868-1130, single rank, walls, with a step cap. */
+int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, double* t_out,
+                    double* dt_hist) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    const int Ne = r->Nelem, Nf = r->Nfaces;
+    const long nq = (long)Ne * ngl2 * NEQ;
+    const long nf = (long)Nf * ngl;
+    double* Qt = (double*)calloc(nq, sizeof(double));
+    double* gRK = (double*)calloc(nq, sizeof(double));
+    double* Qtmp = (double*)calloc(nq, sizeof(double));
+    double* qL = (double*)calloc(nf * NEQ, sizeof(double));
+    double* qR = (double*)calloc(nf * NEQ, sizeof(double));
+    double* bL = (double*)calloc(nf, sizeof(double));
+    double* bR = (double*)calloc(nf, sizeof(double));
+    double* SP = (double*)calloc(nf * NEQ, sizeof(double));
+    double* DB = (double*)calloc(nf, sizeof(double));   /* DBSurf1/2: zero (DiscBottom = 0) */
+    double* lam = (double*)calloc(Ne, sizeof(double));
+    double* visc = (double*)calloc(Ne, sizeof(double));
+    double *gX = NULL, *gY = NULL, *SGX = NULL, *SGY = NULL, *VL = NULL, *VR = NULL;
+    double *gXL = NULL, *gXR = NULL, *gYL = NULL, *gYR = NULL, *SPV = NULL;
+    if (r->artificial_viscosity) {
+        gX = (double*)calloc((long)Ne * ngl2 * 2, sizeof(double));
+        gY = (double*)calloc((long)Ne * ngl2 * 2, sizeof(double));
+        SGX = (double*)calloc(nf * 2, sizeof(double));
+        SGY = (double*)calloc(nf * 2, sizeof(double));
+        VL = (double*)calloc(Nf, sizeof(double));
+        VR = (double*)calloc(Nf, sizeof(double));
+        gXL = (double*)calloc(nf * 2, sizeof(double));
+        gXR = (double*)calloc(nf * 2, sizeof(double));
+        gYL = (double*)calloc(nf * 2, sizeof(double));
+        gYR = (double*)calloc(nf * 2, sizeof(double));
+        SPV = (double*)calloc(nf * 2, sizeof(double));
+    }
+    /* RK coefficients (src/RungeKutta.cpp:18-90) */
+    double A[5] = {0}, Bc[5] = {0}, C[5] = {0};
+    int stages;
+    if (r->rk_order == 4) {
+        stages = 5;
+        A[1] = -(567301805773.0 / 1357537059087.0); A[2] = -(2404267990393.0 / 2016746695238.0);
+        A[3] = -(3550918686646.0 / 2091501179385.0); A[4] = -(1275806237668.0 / 842570457699.0);
+        Bc[1] = (1432997174477.0 / 9575080441755.0); Bc[2] = (2526269341429.0 / 6820363962896.0);
+        Bc[3] = (2006345519317.0 / 3224310063776.0); Bc[4] = (2802321613138.0 / 2924317926251.0);
+        C[0] = (1432997174477.0 / 9575080441755.0); C[1] = (5161836677717.0 / 13612068292357.0);
+        C[2] = (1720146321549.0 / 2090206949498.0); C[3] = (3134564353537.0 / 4481467310338.0);
+        C[4] = (2277821191437.0 / 14882151754819.0);
+    } else {
+        stages = 3;
+        if (r->rk_ssp) {
+            A[0] = 1.0; A[1] = 3.0 / 4.0; A[2] = 1.0 / 3.0;
+            Bc[0] = 0.0; Bc[1] = 1.0 / 4.0; Bc[2] = 2.0 / 3.0;
+            C[0] = 1.0; C[1] = 1.0 / 4.0; C[2] = 2.0 / 3.0;
+        } else {
+            A[1] = -5.0 / 9.0; A[2] = -153.0 / 128.0;
+            Bc[1] = 1.0 / 3.0; Bc[2] = 3.0 / 4.0;
+            C[0] = 1.0 / 3.0; C[1] = 15.0 / 16.0; C[2] = 8.0 / 15.0;
+        }
+    }
+    oracle_collect_edge_data_bottom(p, Nf, r->EdgeData, r->B, bL, bR);
+    double t = 0.0;
+    int step = 0;
+    while (t < r->T && step < r->max_steps) {
+        oracle_find_lambda_max(p, Ne, q, lam);
+        double lmax = 0.0;
+        for (int e = 0; e < Ne; ++e) lmax = fmax(lmax, lam[e]);
+        const double dt_i = r->min_ele_size / (ngl) * r->CFL / lmax;
+        double dt;
+        if (r->artificial_viscosity) {
+            oracle_shock_capturing(p, Ne, q, r->B, r->VdmInv, visc, NULL);
+            double vmax = 0.0;
+            for (int e = 0; e < Ne; ++e) vmax = fmax(vmax, visc[e]);
+            const double dt_v = r->DFL / (pow(ngl, 2)) * pow(r->min_ele_size, 2) / vmax;
+            dt = fmin(r->T - t, fmin(dt_i, dt_v));
+        } else {
+            dt = fmin(r->T - t, dt_i);
+        }
+        if (dt_hist) dt_hist[step] = dt;
+        if (r->rk_ssp) memcpy(Qtmp, q, sizeof(double) * nq);
+        for (int s = 0; s < stages; ++s) {
+            oracle_collect_edge_data_walls(p, Nf, r->EdgeData, q, r->nx, r->ny, qL, qR);
+            oracle_volume_fluxdiff(p, Ne, r->Jac, r->Yxi, r->Yeta, r->Xxi, r->Xeta, q, r->D, r->Bx, r->By, Qt);
+            if (r->num_flux == 3)
+                oracle_num_flux_es_rotated(p, Nf, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);
+            else
+                oracle_num_flux_es(p, Nf, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);
+            oracle_surface(p, Ne, r->Jac, r->EMS, r->EOR, r->ElemToEdge, SP, DB, DB, Qt);
+            if (r->artificial_viscosity) {
+                if (s > 0) oracle_shock_capturing(p, Ne, q, r->B, r->VdmInv, visc, NULL);
+                oracle_num_fluxes_gradient(p, Nf, r->EdgeData, r->nx, r->ny, r->scal, qL, qR, SGX, SGY);
+                oracle_calc_gradient(p, Ne, r->Jac, r->Yxi, r->Yeta, r->Xxi, r->Xeta, q, r->Dhat, gX, gY);
+                oracle_surface_gradient(p, Ne, r->Jac, r->EMS, r->EOR, r->ElemToEdge, SGX, SGY, gX, gY);
+                oracle_scale_gradient(p, Ne, q, gX, gY);
+                oracle_collect_edge_data_gradient(p, Nf, r->EdgeData, gX, gY, visc, VL, VR, gXL, gXR, gYL, gYR);
+                oracle_volume_viscose(p, Ne, r->Jac, r->Yxi, r->Yeta, r->Xxi, r->Xeta, gX, gY, r->Dstrong, visc, Qt);
+                oracle_num_fluxes_viscose(p, Nf, r->EdgeData, r->nx, r->ny, r->scal, VL, VR, gXL, gXR, gYL, gYR, SPV);
+                oracle_surface_visc(p, Ne, r->Jac, r->EMS, r->EOR, r->ElemToEdge, SPV, Qt);
+            }
+            if (r->rk_ssp) {
+                oracle_rk_ssp_update(p, Ne, A[s], Bc[s], C[s], dt, Qt, Qtmp, q);
+            } else {
+                oracle_rk_ls_calc(p, Ne, Qt, A[s], gRK);
+                oracle_rk_ls_update(p, Ne, C[s], dt, gRK, q);
+            }
+            if (r->positivity) oracle_pos_pres(p, Ne, r->EleSizes, r->GLw, r->Jac, q);
+        }
+        t += dt;
+        ++step;
+    }
+    if (t_out) *t_out = t;
+    free(Qt); free(gRK); free(Qtmp); free(qL); free(qR); free(bL); free(bR); free(SP); free(DB);
+    free(lam); free(visc);
+    free(gX); free(gY); free(SGX); free(SGY); free(VL); free(VR);
+    free(gXL); free(gXR); free(gYL); free(gYR); free(SPV);
+    return step;
+}
