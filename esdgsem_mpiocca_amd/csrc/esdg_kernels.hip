@@ -355,38 +355,23 @@ __global__ void __launch_bounds__(256) k_lambda_max(int Nelem, double g, double 
 // Low-storage and SSP Runge-Kutta updates: okl/RungeKutta/rkLS.okl:17-89,
 // UpdateKernelLS.okl:17-78, UpdateKernelSSP_V1.okl:17-98
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_calc_rk(int64_t n2, const double2* __restrict__ Qt, double rkA,
-                                                 double2* __restrict__ gRK) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 a = Qt[i];
-        double2 r = gRK[i];
-        r.x = rkA * r.x + a.x;
-        r.y = rkA * r.y + a.y;
-        gRK[i] = r;
-    }
+__global__ void __launch_bounds__(256) k_calc_rk(int64_t n, const double* __restrict__ Qt, double rkA,
+                                                 double* __restrict__ gRK) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        gRK[i] = rkA * gRK[i] + Qt[i];
 }
 
-__global__ void __launch_bounds__(256) k_update_ls(int64_t n2, double cdt, const double2* __restrict__ gRK,
-                                                   double2* __restrict__ q) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 a = gRK[i];
-        double2 r = q[i];
-        r.x += cdt * a.x;
-        r.y += cdt * a.y;
-        q[i] = r;
-    }
+__global__ void __launch_bounds__(256) k_update_ls(int64_t n, double cdt, const double* __restrict__ gRK,
+                                                   double* __restrict__ q) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        q[i] += cdt * gRK[i];
 }
 
-__global__ void __launch_bounds__(256) k_update_ssp(int64_t n2, double A, double Bc, double cdt,
-                                                    const double2* __restrict__ Qt, const double2* __restrict__ qT,
-                                                    double2* __restrict__ q) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 a = Qt[i], t = qT[i];
-        double2 r = q[i];
-        r.x = A * t.x + Bc * r.x + cdt * a.x;
-        r.y = A * t.y + Bc * r.y + cdt * a.y;
-        q[i] = r;
-    }
+__global__ void __launch_bounds__(256) k_update_ssp(int64_t n, double A, double Bc, double cdt,
+                                                    const double* __restrict__ Qt, const double* __restrict__ qT,
+                                                    double* __restrict__ q) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        q[i] = A * qT[i] + Bc * q[i] + cdt * Qt[i];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -865,7 +850,7 @@ int esdg_find_lambda_max(esdg_ctx* ctx, int Nelem, const double* q, double* Lamb
     return launch_check(ctx, "find_lambda_max");
 }
 
-static unsigned stream_blocks(int64_t n2) { return (unsigned)std::min<int64_t>((n2 + 255) / 256, 256 * 64); }
+static unsigned stream_blocks(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 256 * 64); }
 
 int esdg_calc_rk(esdg_ctx* ctx, int Nelem, const double* Qt, double rkA, double rkB, double* gRK) {
     (void)rkB;
@@ -873,10 +858,8 @@ int esdg_calc_rk(esdg_ctx* ctx, int Nelem, const double* Qt, double rkA, double 
     ESDG_REQUIRE(ctx, Nelem >= 0 && Qt && gRK, "esdg_calc_rk: bad argument");
     const int64_t n = (int64_t)Nelem * ctx->p.ngl * ctx->p.ngl * ESDG_NEQ;
     if (n == 0) return ESDG_OK;
-    ESDG_REQUIRE(ctx, n % 2 == 0, "esdg_calc_rk: odd state length");
-    ESDG_REQUIRE(ctx, ((uintptr_t)Qt % 16) == 0 && ((uintptr_t)gRK % 16) == 0, "esdg_calc_rk: arrays must be 16-byte aligned");
     ESDG_ENTRY("calc_rk");
-    k_calc_rk<<<stream_blocks(n / 2), 256, 0, ctx->stream>>>(n / 2, (const double2*)Qt, rkA, (double2*)gRK);
+    k_calc_rk<<<stream_blocks(n), 256, 0, ctx->stream>>>(n, Qt, rkA, gRK);
     return launch_check(ctx, "calc_rk");
 }
 
@@ -885,10 +868,8 @@ int esdg_update_kernel_ls(esdg_ctx* ctx, int Nelem, double rkC, double dt, const
     ESDG_REQUIRE(ctx, Nelem >= 0 && gRK && q, "esdg_update_kernel_ls: bad argument");
     const int64_t n = (int64_t)Nelem * ctx->p.ngl * ctx->p.ngl * ESDG_NEQ;
     if (n == 0) return ESDG_OK;
-    ESDG_REQUIRE(ctx, n % 2 == 0, "esdg_update_kernel_ls: odd state length");
-    ESDG_REQUIRE(ctx, ((uintptr_t)q % 16) == 0 && ((uintptr_t)gRK % 16) == 0, "esdg_update_kernel_ls: arrays must be 16-byte aligned");
     ESDG_ENTRY("update_kernel_ls");
-    k_update_ls<<<stream_blocks(n / 2), 256, 0, ctx->stream>>>(n / 2, rkC * dt, (const double2*)gRK, (double2*)q);
+    k_update_ls<<<stream_blocks(n), 256, 0, ctx->stream>>>(n, rkC * dt, gRK, q);
     return launch_check(ctx, "update_kernel_ls");
 }
 
@@ -898,12 +879,8 @@ int esdg_update_kernel_ssp(esdg_ctx* ctx, int Nelem, double rkA, double rkB, dou
     ESDG_REQUIRE(ctx, Nelem >= 0 && Qt && qTmp && q, "esdg_update_kernel_ssp: bad argument");
     const int64_t n = (int64_t)Nelem * ctx->p.ngl * ctx->p.ngl * ESDG_NEQ;
     if (n == 0) return ESDG_OK;
-    ESDG_REQUIRE(ctx, n % 2 == 0, "esdg_update_kernel_ssp: odd state length");
-    ESDG_REQUIRE(ctx, ((uintptr_t)q % 16) == 0 && ((uintptr_t)Qt % 16) == 0 && ((uintptr_t)qTmp % 16) == 0,
-                 "esdg_update_kernel_ssp: arrays must be 16-byte aligned");
     ESDG_ENTRY("update_kernel_ssp");
-    k_update_ssp<<<stream_blocks(n / 2), 256, 0, ctx->stream>>>(n / 2, rkA, rkB, rkC * dt, (const double2*)Qt,
-                                                                 (const double2*)qTmp, (double2*)q);
+    k_update_ssp<<<stream_blocks(n), 256, 0, ctx->stream>>>(n, rkA, rkB, rkC * dt, Qt, qTmp, q);
     return launch_check(ctx, "update_kernel_ssp");
 }
 

@@ -10,7 +10,7 @@
 // Work mapping (wave64, one wave per block): lane = (element el, line r), EPW = 64 / ngl elements
 // per wave.  Row phase: the lane owns eta-row r of its element, holds the row's 7 node fields in
 // registers (loaded straight from HBM: 8 contiguous doubles per array) and performs all xi-pairs.
-// It parks h, hu, hv, u, v in LDS.  Column phase: the same lane takes xi-column r, reads the column
+// It parks h, hu, hv and 1/h in LDS (u = hu/h and v = hv/h are re-derived bit-identically).  Column phase: the same lane takes xi-column r, reads the column
 // from LDS, loads the xi-metrics of that column from HBM (one 64-byte segment per element and row),
 // continues the sums of its column nodes with the eta-pairs, and stores Qt.  The row sums cross
 // over through LDS.  Per element the kernel reads q, the four metrics, 1/J, Bx, By once and writes
@@ -25,22 +25,21 @@ struct VolCfg {
     static constexpr int NN = NGL * NGL;
     static constexpr int RS = NGL + 1;          // LDS row stride (padding against bank conflicts)
     static constexpr int PLANE = NGL * RS;      // one field of one element in LDS
-    static constexpr int ESZ = 5 * PLANE;       // h, hu, hv, u, v
+    static constexpr int ESZ = 4 * PLANE;       // h, hu, hv, 1/h (the row sums reuse three planes)
 };
 
 // Symmetric two-point EC flux along one line (F# of the reference for metrics (Y, X) = (Yeta, Xeta);
-// G# = -F# evaluated with (Yxi, Xxi)).  Operands per node: h, hu, hv, u = hu/h, v = hv/h, h^2.
+// G# = -F# evaluated with (Yxi, Xxi)).  Operands per node: h, hu, hv, u = hu/h, v = hv/h.
 struct PairOut {
     double f1, f2, f3;
 };
 
 __device__ __forceinline__ PairOut ec_pair(double fourth_g, double half_g, double hL, double huL, double hvL,
-                                           double uL, double vL, double h2L, double YL, double XL, double hR,
-                                           double huR, double hvR, double uR, double vR, double h2R, double YR,
-                                           double XR) {
+                                           double uL, double vL, double YL, double XL, double hR, double huR,
+                                           double hvR, double uR, double vR, double YR, double XR) {
     const double s = hL + hR;
     const double h_A = fourth_g * s * s;
-    const double h2_A = 0.5 * (h2L + h2R);
+    const double h2_A = 0.5 * (hL * hL + hR * hR);
     const double hu_A = 0.5 * (huL + huR);
     const double hv_A = 0.5 * (hvL + hvR);
     const double u_A = 0.5 * (uL + uR);
@@ -59,15 +58,15 @@ __device__ __forceinline__ PairOut ec_pair(double fourth_g, double half_g, doubl
 template <int NGL, bool NEG>
 __device__ __forceinline__ void line_sweep(const double* __restrict__ D, double fourth_g, double half_g,
                                            const double (&h)[NGL], const double (&hu)[NGL], const double (&hv)[NGL],
-                                           const double (&u)[NGL], const double (&v)[NGL], const double (&h2)[NGL],
+                                           const double (&u)[NGL], const double (&v)[NGL],
                                            const double (&Y)[NGL], const double (&X)[NGL], double (&a1)[NGL],
                                            double (&a2)[NGL], double (&a3)[NGL]) {
 #pragma unroll
     for (int c = 0; c < NGL; ++c) {
 #pragma unroll
         for (int l = c; l < NGL; ++l) {
-            const PairOut F = ec_pair(fourth_g, half_g, h[c], hu[c], hv[c], u[c], v[c], h2[c], Y[c], X[c], h[l], hu[l],
-                                      hv[l], u[l], v[l], h2[l], Y[l], X[l]);
+            const PairOut F = ec_pair(fourth_g, half_g, h[c], hu[c], hv[c], u[c], v[c], Y[c], X[c], h[l], hu[l],
+                                      hv[l], u[l], v[l], Y[l], X[l]);
             const double dcl = NEG ? -D[c * NGL + l] : D[c * NGL + l];
             a1[c] += dcl * F.f1;
             a2[c] += dcl * F.f2;
@@ -83,7 +82,7 @@ __device__ __forceinline__ void line_sweep(const double* __restrict__ D, double 
 }
 
 template <int NGL>
-__global__ void __launch_bounds__(64) k_volume_fluxdiff(int Nelem, double g, const double* __restrict__ Jac,
+__global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, const double* __restrict__ Jac,
                                                         const double* __restrict__ Yxi, const double* __restrict__ Yeta,
                                                         const double* __restrict__ Xxi, const double* __restrict__ Xeta,
                                                         const double* __restrict__ q, const double* __restrict__ D,
@@ -98,7 +97,7 @@ __global__ void __launch_bounds__(64) k_volume_fluxdiff(int Nelem, double g, con
     double* S = s + (act ? el : 0) * C::ESZ;
     const double half_g = 0.5 * g, fourth_g = 0.25 * g;
 
-    double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], h2[NGL], Y[NGL], X[NGL];
+    double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], Y[NGL], X[NGL];
     double a1[NGL], a2[NGL], a3[NGL];
 #pragma unroll
     for (int c = 0; c < NGL; ++c) { a1[c] = 0.0; a2[c] = 0.0; a3[c] = 0.0; }
@@ -120,14 +119,12 @@ __global__ void __launch_bounds__(64) k_volume_fluxdiff(int Nelem, double g, con
             const double inv = (h[c] > 0) ? 1.0 / h[c] : 0.0;
             u[c] = hu[c] * inv;
             v[c] = hv[c] * inv;
-            h2[c] = h[c] * h[c];
             S[0 * C::PLANE + r * C::RS + c] = h[c];
             S[1 * C::PLANE + r * C::RS + c] = hu[c];
             S[2 * C::PLANE + r * C::RS + c] = hv[c];
-            S[3 * C::PLANE + r * C::RS + c] = u[c];
-            S[4 * C::PLANE + r * C::RS + c] = v[c];
+            S[3 * C::PLANE + r * C::RS + c] = inv;
         }
-        line_sweep<NGL, false>(D, fourth_g, half_g, h, hu, hv, u, v, h2, Y, X, a1, a2, a3);
+        line_sweep<NGL, false>(D, fourth_g, half_g, h, hu, hv, u, v, Y, X, a1, a2, a3);
     }
     __syncthreads();
 
@@ -140,9 +137,9 @@ __global__ void __launch_bounds__(64) k_volume_fluxdiff(int Nelem, double g, con
             h[k] = S[0 * C::PLANE + k * C::RS + r];
             hu[k] = S[1 * C::PLANE + k * C::RS + r];
             hv[k] = S[2 * C::PLANE + k * C::RS + r];
-            u[k] = S[3 * C::PLANE + k * C::RS + r];
-            v[k] = S[4 * C::PLANE + k * C::RS + r];
-            h2[k] = h[k] * h[k];
+            const double inv = S[3 * C::PLANE + k * C::RS + r];
+            u[k] = hu[k] * inv;
+            v[k] = hv[k] * inv;
             Y[k] = Yxi[xb + k * NGL];
             X[k] = Xxi[xb + k * NGL];
         }
@@ -165,7 +162,7 @@ __global__ void __launch_bounds__(64) k_volume_fluxdiff(int Nelem, double g, con
             b2[k] = S[1 * C::PLANE + k * C::RS + r];
             b3[k] = S[2 * C::PLANE + k * C::RS + r];
         }
-        line_sweep<NGL, true>(D, fourth_g, half_g, h, hu, hv, u, v, h2, Y, X, b1, b2, b3);
+        line_sweep<NGL, true>(D, fourth_g, half_g, h, hu, hv, u, v, Y, X, b1, b2, b3);
         const int64_t xb = (int64_t)e * C::NN + r;
         const int64_t ob = (int64_t)e * C::NN * ESDG_NEQ + r;
 #pragma unroll
