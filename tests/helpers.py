@@ -44,3 +44,52 @@ def small_cases():
     c = Case.dam_break(N=5, nel=5, noise=1e-3, seed=4)
     out.append(("curved_N5_es", c, RunParams(num_flux=0)))
     return out
+
+
+def oracle_residual(case, rp, q, av_visc=None):
+    """Qt of one stage (gather, volume, numerical flux, surface lift [, BR1 + viscous terms]) built
+    from the oracle kernels in the DGtimeloop order.  Returns Qt."""
+    from oracle import binding as O
+    p = O.params(case, rp)
+    a = case.arrays
+    Ne, Nf, ngl = case.Nelem, case.Nfaces, case.ngl
+    fn, nn = Nf * ngl, Ne * ngl * ngl
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    qL, qR, bL, bR = np.zeros(fn * 3), np.zeros(fn * 3), np.zeros(fn), np.zeros(fn)
+    O.call("collect_edge_data_bottom", p, Nf, a["EdgeData"], a["b"], bL, bR)
+    O.call("collect_edge_data_walls", p, Nf, a["EdgeData"], q, a["nx"], a["ny"], qL, qR)
+    Qt = np.zeros(nn * 3)
+    O.call("volume_fluxdiff", p, Ne, a["Jac"], a["yXi"], a["yEta"], a["xXi"], a["xEta"], q, a["D"], a["Bx"],
+           a["By"], Qt)
+    SP = np.zeros(fn * 3)
+    O.call("num_flux_es_rotated" if rp.num_flux == 3 else "num_flux_es", p, Nf, a["EdgeReversed"], a["nx"],
+           a["ny"], a["scal"], qL, qR, bL, bR, SP)
+    DB = np.zeros(fn)
+    O.call("surface", p, Ne, a["Jac"], a["ElemEdgeMasterSlave"], a["ElemEdgeOrientation"], a["ElemToEdge"], SP,
+           DB, DB, Qt)
+    if av_visc is not None:
+        visc = np.ascontiguousarray(av_visc, dtype=np.float64)
+        SGX, SGY = np.zeros(fn * 2), np.zeros(fn * 2)
+        O.call("num_fluxes_gradient", p, Nf, a["EdgeData"], a["nx"], a["ny"], a["scal"], qL, qR, SGX, SGY)
+        gX, gY = np.zeros(nn * 2), np.zeros(nn * 2)
+        O.call("calc_gradient", p, Ne, a["Jac"], a["yXi"], a["yEta"], a["xXi"], a["xEta"], q, a["Dhat"], gX, gY)
+        O.call("surface_gradient", p, Ne, a["Jac"], a["ElemEdgeMasterSlave"], a["ElemEdgeOrientation"],
+               a["ElemToEdge"], SGX, SGY, gX, gY)
+        O.call("scale_gradient", p, Ne, q, gX, gY)
+        VL, VR = np.zeros(Nf), np.zeros(Nf)
+        XL, XR, YL, YR = (np.zeros(fn * 2) for _ in range(4))
+        O.call("collect_edge_data_gradient", p, Nf, a["EdgeData"], gX, gY, visc, VL, VR, XL, XR, YL, YR)
+        O.call("volume_viscose", p, Ne, a["Jac"], a["yXi"], a["yEta"], a["xXi"], a["xEta"], gX, gY, a["Dstrong"],
+               visc, Qt)
+        SPV = np.zeros(fn * 2)
+        O.call("num_fluxes_viscose", p, Nf, a["EdgeData"], a["nx"], a["ny"], a["scal"], VL, VR, XL, XR, YL, YR, SPV)
+        O.call("surface_visc", p, Ne, a["Jac"], a["ElemEdgeMasterSlave"], a["ElemEdgeOrientation"],
+               a["ElemToEdge"], SPV, Qt)
+    return Qt
+
+
+def quad_weights(case):
+    """w_i w_j J at every node (J = 1/Jac): the discrete volume measure."""
+    w = case["w_GL"]
+    ww = np.outer(w, w).ravel()            # index j*ngl + i  ->  w_j w_i
+    return (ww[None, :] / case["Jac"].reshape(case.Nelem, -1))

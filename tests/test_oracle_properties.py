@@ -1,0 +1,412 @@
+"""Pins of the oracle's kernels by what the mathematics fixes.
+
+The reference ships no tests or golden vectors for these kernels and its okl path cannot be built
+here, so each oracle function is checked against properties a plausible restatement mistake
+(transposed operator, swapped metric, wrong sign or side index, dropped term) would break:
+independent vectorised formulations, consistency and entropy identities of the two-point and
+surface fluxes, free-stream preservation and well-balancing on curved meshes, discrete
+conservation and semi-discrete entropy conservation of the whole residual, exactness of the BR1
+gradient, the modal sensor's closed form, the limiter's invariants and the RK order.
+"""
+import numpy as np
+import pytest
+from numpy.polynomial import legendre as L
+
+from esdgsem_mpiocca_amd import Case, RunParams
+from esdgsem_mpiocca_amd import setup_api as S
+from helpers import oracle_residual, perturbed_state, quad_weights
+from oracle import binding as O
+
+G = 9.81
+
+
+# --------------------------------------------------------------------------------------------
+# volume term: an independent vectorised restatement with p# = g/2 hL hR
+# --------------------------------------------------------------------------------------------
+def numpy_volume(case, q):
+    ngl, E = case.ngl, case.Nelem
+    D = case["D"].reshape(ngl, ngl)
+    Q = q.reshape(E, 3, ngl, ngl)                  # [e, v, row(eta), col(xi)]
+    h, hu, hv = Q[:, 0], Q[:, 1], Q[:, 2]
+    inv = np.where(h > 0, 1.0 / np.where(h > 0, h, 1.0), 0.0)
+    u, v = hu * inv, hv * inv
+    met = {k: case[k].reshape(E, ngl, ngl) for k in ("yXi", "yEta", "xXi", "xEta", "Jac", "Bx", "By")}
+
+    def two_point(axis, Y, X):
+        # pairs along `axis` (2 = xi within a row, 1 = eta within a column); returns [e, a, b, comp]
+        def pair(f):
+            a = np.expand_dims(f, axis + 1)         # node a
+            b = np.expand_dims(f, axis)             # node b
+            return a, b
+        hA, hB = pair(h); huA, huB = pair(hu); hvA, hvB = pair(hv)
+        uA, uB = pair(u); vA, vB = pair(v); YA, YB = pair(Y); XA, XB = pair(X)
+        p = 0.5 * G * hA * hB
+        f = np.stack([0.5 * (huA + huB), 0.25 * (huA + huB) * (uA + uB) + p, 0.25 * (huA + huB) * (vA + vB)])
+        g = np.stack([0.5 * (hvA + hvB), 0.25 * (hvA + hvB) * (uA + uB), 0.25 * (hvA + hvB) * (vA + vB) + p])
+        return 0.5 * (YA + YB) * f - 0.5 * (XA + XB) * g
+
+    Fx = two_point(2, met["yEta"], met["xEta"])        # [3, e, i, j, l]
+    Fy = two_point(1, met["yXi"], met["xXi"])          # [3, e, i, l, j]
+    div = np.einsum("jl,ceijl->ceij", D, Fx) - np.einsum("il,ceilj->ceij", D, Fy)
+    out = -met["Jac"][None] * div
+    out[1] -= G * h * met["Bx"]
+    out[2] -= G * h * met["By"]
+    return np.moveaxis(out, 0, 1).reshape(-1)
+
+
+@pytest.mark.parametrize("mk", ["cart", "curved"])
+def test_volume_matches_independent_form(mk):
+    c = Case.gaussian_bump(N=4, nel=4) if mk == "cart" else Case.dam_break(N=6, nel=3, seed=1)
+    rp = RunParams()
+    q = perturbed_state(c, seed=2)
+    Qt = np.zeros_like(q)
+    a = c.arrays
+    O.call("volume_fluxdiff", O.params(c, rp), c.Nelem, a["Jac"], a["yXi"], a["yEta"], a["xXi"], a["xEta"], q,
+           a["D"], a["Bx"], a["By"], Qt)
+    ref = numpy_volume(c, q)
+    assert np.abs(Qt - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+# --------------------------------------------------------------------------------------------
+# surface fluxes on a synthetic set of face nodes (ngl = 1: one node per "face")
+# --------------------------------------------------------------------------------------------
+def flux(name, qL, qR, n, bL=None, bR=None, ES=1, tol=1e-12):
+    M = len(qL)
+    p = O.OracleParams(1, ES, G, 1.0, tol, 1e-5, 0.0, 0.0, 0.0)
+    QL = np.ascontiguousarray(np.stack([qL[:, 0], qL[:, 1], qL[:, 2]], axis=1).reshape(-1))
+    QR = np.ascontiguousarray(np.stack([qR[:, 0], qR[:, 1], qR[:, 2]], axis=1).reshape(-1))
+    bL = np.zeros(M) if bL is None else np.ascontiguousarray(bL)
+    bR = np.zeros(M) if bR is None else np.ascontiguousarray(bR)
+    SP = np.zeros(3 * M)
+    O.call(name, p, M, np.ones(M, np.int32), np.ascontiguousarray(n[:, 0]), np.ascontiguousarray(n[:, 1]),
+           np.ones(M), QL, QR, bL, bR, SP)
+    return SP.reshape(M, 3)
+
+
+def random_faces(M, seed):
+    rng = np.random.default_rng(seed)
+    h = rng.uniform(0.5, 2.0, (M, 2))
+    u = rng.uniform(-1, 1, (M, 2, 2))
+    th = rng.uniform(0, 2 * np.pi, M)
+    n = np.stack([np.cos(th), np.sin(th)], 1)
+    qL = np.stack([h[:, 0], h[:, 0] * u[:, 0, 0], h[:, 0] * u[:, 0, 1]], 1)
+    qR = np.stack([h[:, 1], h[:, 1] * u[:, 1, 0], h[:, 1] * u[:, 1, 1]], 1)
+    return qL, qR, n
+
+
+def phys_flux(q, n):
+    h, hu, hv = q[:, 0], q[:, 1], q[:, 2]
+    u, v = hu / h, hv / h
+    un = u * n[:, 0] + v * n[:, 1]
+    return np.stack([h * un, hu * un + 0.5 * G * h * h * n[:, 0], hv * un + 0.5 * G * h * h * n[:, 1]], 1)
+
+
+def ent_vars(q, b):
+    h, u, v = q[:, 0], q[:, 1] / q[:, 0], q[:, 2] / q[:, 0]
+    return np.stack([G * (h + b) - 0.5 * (u * u + v * v), u, v], 1)
+
+
+@pytest.mark.parametrize("name", ["num_flux_es", "num_flux_es_rotated"])
+def test_surface_flux_consistency_and_entropy(name):
+    qL, qR, n = random_faces(400, 3)
+    # consistency: F*(u, u) = n . f(u) (with and without dissipation)
+    for ES in (0, 1):
+        F = flux(name, qL, qL, n, ES=ES)
+        assert np.abs(F - phys_flux(qL, n)).max() < 1e-13
+    # entropy conservation of the EC part (flat bottom): [v] . F_EC = [psi], psi = g/2 h^2 u_n
+    Fec = flux(name, qL, qR, n, ES=0)
+    b0 = np.zeros(len(qL))
+    jv = ent_vars(qR, b0) - ent_vars(qL, b0)
+    psi = lambda q: 0.5 * G * q[:, 0] ** 2 * (q[:, 1] * n[:, 0] + q[:, 2] * n[:, 1]) / q[:, 0]
+    assert np.abs((jv * Fec).sum(1) - (psi(qR) - psi(qL))).max() < 1e-12
+    # entropy dissipation of the ES flux, with bathymetry jumps in v1
+    rng = np.random.default_rng(9)
+    bL, bR = rng.uniform(-0.2, 0.2, len(qL)), rng.uniform(-0.2, 0.2, len(qL))
+    Fes = flux(name, qL, qR, n, bL, bR, ES=1)
+    Fec = flux(name, qL, qR, n, bL, bR, ES=0)
+    jv = ent_vars(qR, bR) - ent_vars(qL, bL)
+    assert ((jv * (Fes - Fec)).sum(1) <= 1e-13).all()
+    assert ((jv * (Fes - Fec)).sum(1) < -1e-6).mean() > 0.9
+
+
+def test_es_dissipation_matrix_independent():
+    """F_ES - F_EC = -1/2 R |Lambda| R^T [v] with the eigen-structure of calcESFlux.okl."""
+    qL, qR, n = random_faces(200, 5)
+    Fes = flux("num_flux_es", qL, qR, n, ES=1)
+    Fec = flux("num_flux_es", qL, qR, n, ES=0)
+    jv = ent_vars(qR, 0 * qR[:, 0]) - ent_vars(qL, 0 * qL[:, 0])
+    for k in range(len(qL)):
+        hL, hR = qL[k, 0], qR[k, 0]
+        uA = 0.5 * (qL[k, 1] / hL + qR[k, 1] / hR)
+        vA = 0.5 * (qL[k, 2] / hL + qR[k, 2] / hR)
+        c = np.sqrt(0.5 * G * (hL + hR))
+        nx, ny = n[k]
+        un = uA * nx + vA * ny
+        R = np.array([[1, 0, 1], [uA + nx * c, ny, uA - nx * c], [vA + ny * c, nx, vA - ny * c]])
+        Lam = np.diag([abs(un + c) / (2 * G), 0.5 * (hL + hR) * abs(un), abs(un - c) / (2 * G)])
+        d = -0.5 * R @ Lam @ R.T @ jv[k]
+        assert np.allclose(Fes[k] - Fec[k], d, rtol=1e-12, atol=1e-14)
+
+
+def test_rotated_equals_cartesian_on_axis_aligned_faces():
+    qL, qR, n = random_faces(100, 7)
+    for nn in ((1, 0), (0, 1), (-1, 0), (0, -1)):
+        nrm = np.tile(np.array(nn, float), (len(qL), 1))
+        a = flux("num_flux_es", qL, qR, nrm)
+        b = flux("num_flux_es_rotated", qL, qR, nrm)
+        assert np.abs(a - b).max() < 1e-12
+
+
+def test_wall_gather_mirror_is_involution():
+    c = Case.dam_break(N=3, nel=3, seed=5)
+    q = perturbed_state(c, seed=5)
+    p = O.params(c, RunParams())
+    fn = c.Nfaces * c.ngl
+    qL, qR = np.zeros(fn * 3), np.zeros(fn * 3)
+    O.call("collect_edge_data_walls", p, c.Nfaces, c["EdgeData"], q, c["nx"], c["ny"], qL, qR)
+    ED = c["EdgeData"].reshape(-1, 8)
+    nx, ny = c["nx"].reshape(-1, c.ngl), c["ny"].reshape(-1, c.ngl)
+    L3, R3 = qL.reshape(-1, 3, c.ngl), qR.reshape(-1, 3, c.ngl)
+    bnd = ED[:, 1] == -1
+    # the mirror keeps h, flips the normal momentum, keeps the tangential one
+    assert np.array_equal(L3[bnd, 0], R3[bnd, 0])
+    mnL = L3[bnd, 1] * nx[bnd] + L3[bnd, 2] * ny[bnd]
+    mnR = R3[bnd, 1] * nx[bnd] + R3[bnd, 2] * ny[bnd]
+    mtL = -L3[bnd, 1] * ny[bnd] + L3[bnd, 2] * nx[bnd]
+    mtR = -R3[bnd, 1] * ny[bnd] + R3[bnd, 2] * nx[bnd]
+    assert np.abs(mnL + mnR).max() < 1e-15 and np.abs(mtL - mtR).max() < 1e-15
+    # interior faces: both traces sample the same physical points (x, y agree)
+    x, y = c["x"].reshape(c.Nelem, -1), c["y"].reshape(c.Nelem, -1)
+    side = {0: lambda k: k, 2: lambda k: c.ngl ** 2 - c.ngl + k, 3: lambda k: k * c.ngl,
+            1: lambda k: (k + 1) * c.ngl - 1}
+    for f in np.nonzero(~bnd)[0]:
+        eL, eR, sL, sR = ED[f, :4]
+        for k in range(c.ngl):
+            assert abs(x[eL, side[sL](k)] - x[eR, side[sR](k)]) < 1e-13
+            assert abs(y[eL, side[sL](k)] - y[eR, side[sR](k)]) < 1e-13
+
+
+# --------------------------------------------------------------------------------------------
+# whole residual invariants
+# --------------------------------------------------------------------------------------------
+def term_scale(c, q):
+    """Magnitude of the individual volume terms (g h^2 |D| |J^-1| |metric|): roundoff reference."""
+    h = np.abs(q.reshape(c.Nelem, 3, -1)[:, 0]).max()
+    met = max(np.abs(c[k]).max() for k in ("xXi", "xEta", "yXi", "yEta"))
+    return G * h * h * np.abs(c["D"]).max() * np.abs(c["Jac"]).max() * met
+
+
+def test_free_stream_and_well_balanced_curved():
+    c = Case.dam_break(N=5, nel=5, seed=1)
+    E = c.Nelem
+    rp = RunParams(num_flux=3)
+    # lake at rest over non-flat bathymetry: h + b = 2, u = v = 0
+    q = np.zeros((E, 3, c.ngl ** 2))
+    q[:, 0] = 2.0 - c["b"].reshape(E, -1)
+    sc = term_scale(c, q.ravel())
+    for nf in (0, 3):
+        Qt = oracle_residual(c, RunParams(num_flux=nf), q.ravel())
+        assert np.abs(Qt).max() < 1e-14 * sc
+    # the same state with a dropped bathymetry source term is far from balanced
+    c2 = Case.dam_break(N=5, nel=5, seed=1)
+    c2.arrays["Bx"] = np.zeros_like(c2["Bx"])
+    assert np.abs(oracle_residual(c2, rp, q.ravel())).max() > 1e-6 * sc
+
+
+def test_free_stream_with_velocity_periodic():
+    c = Case.gaussian_bump(N=4, nel=4)
+    E = c.Nelem
+    # constant state on flat bottom: every volume/surface term must cancel
+    flat = Case(S.CASE_CONSTANT, S.MESH_CARTESIAN, 4, 4, 4, 0.0, 2.0, 0.0, 2.0, 1, 1, p=(1.3, 0.4, -0.2, 0))
+    Qt = oracle_residual(flat, RunParams(num_flux=0), flat["q"])
+    assert np.abs(Qt).max() < 1e-14 * term_scale(flat, flat["q"])
+    # curved walls, flat bottom, fluid at rest: pressure terms cancel through the metric identities
+    cv = Case.constant(N=6, nel=4, h=1.7)
+    Qt = oracle_residual(cv, RunParams(num_flux=3), cv["q"])
+    assert np.abs(Qt).max() < 1e-14 * term_scale(cv, cv["q"])
+
+
+@pytest.mark.parametrize("mk", ["periodic", "walls_curved"])
+def test_mass_conservation(mk):
+    c = Case.gaussian_bump(N=4, nel=6) if mk == "periodic" else Case.dam_break(N=5, nel=4, seed=3)
+    q = perturbed_state(c, seed=4)
+    Qt = oracle_residual(c, RunParams(num_flux=0 if mk == "periodic" else 3), q).reshape(c.Nelem, 3, -1)
+    wJ = quad_weights(c)
+    assert abs((wJ * Qt[:, 0]).sum()) < 1e-12 * np.abs(wJ * Qt[:, 0]).sum()
+
+
+def test_semi_discrete_entropy_conservation_periodic():
+    """Periodic, EC volume + EC surface flux: sum w J v . Qt = 0; with ES dissipation it is < 0."""
+    c = Case.gaussian_bump(N=4, nel=6)
+    q = perturbed_state(c, seed=6, amp_u=0.5)
+    E = c.Nelem
+    Q = q.reshape(E, 3, -1)
+    b = c["b"].reshape(E, -1)
+    v = np.stack([G * (Q[:, 0] + b) - 0.5 * ((Q[:, 1] / Q[:, 0]) ** 2 + (Q[:, 2] / Q[:, 0]) ** 2),
+                  Q[:, 1] / Q[:, 0], Q[:, 2] / Q[:, 0]], 1)
+    wJ = quad_weights(c)
+    for ES, sign in ((0, 0), (1, -1)):
+        Qt = oracle_residual(c, RunParams(num_flux=0, ES=ES), q).reshape(E, 3, -1)
+        dS = (wJ[:, None] * v * Qt).sum()
+        scale = np.abs(wJ[:, None] * v * Qt).sum()
+        if sign == 0:
+            assert abs(dS) < 1e-12 * scale
+        else:
+            assert dS < -1e-8 * scale
+
+
+# --------------------------------------------------------------------------------------------
+# artificial viscosity chain
+# --------------------------------------------------------------------------------------------
+def gradient_of(c, q, rp):
+    p = O.params(c, rp)
+    a = c.arrays
+    fn, nn = c.Nfaces * c.ngl, c.Nelem * c.ngl ** 2
+    qL, qR = np.zeros(fn * 3), np.zeros(fn * 3)
+    O.call("collect_edge_data_walls", p, c.Nfaces, a["EdgeData"], q, a["nx"], a["ny"], qL, qR)
+    SGX, SGY = np.zeros(fn * 2), np.zeros(fn * 2)
+    O.call("num_fluxes_gradient", p, c.Nfaces, a["EdgeData"], a["nx"], a["ny"], a["scal"], qL, qR, SGX, SGY)
+    gX, gY = np.zeros(nn * 2), np.zeros(nn * 2)
+    O.call("calc_gradient", p, c.Nelem, a["Jac"], a["yXi"], a["yEta"], a["xXi"], a["xEta"], q, a["Dhat"], gX, gY)
+    O.call("surface_gradient", p, c.Nelem, a["Jac"], a["ElemEdgeMasterSlave"], a["ElemEdgeOrientation"],
+           a["ElemToEdge"], SGX, SGY, gX, gY)
+    O.call("scale_gradient", p, c.Nelem, q, gX, gY)
+    return gX.reshape(c.Nelem, 2, -1), gY.reshape(c.Nelem, 2, -1)
+
+
+@pytest.mark.parametrize("mk", ["cart", "curved"])
+def test_br1_gradient_exact_for_linear_velocity(mk):
+    c = (Case(S.CASE_CONSTANT, S.MESH_CARTESIAN, 4, 4, 3, 0.0, 10.0, 0.0, 10.0, p=(1.5, 0, 0, 0))
+         if mk == "cart" else Case.constant(N=6, nel=4, h=1.5))
+    E = c.Nelem
+    x, y = c["x"].reshape(E, -1), c["y"].reshape(E, -1)
+    h = 1.5
+    u, v = 0.3 + 0.2 * x - 0.1 * y, -0.4 + 0.05 * x + 0.15 * y
+    q = np.stack([np.full_like(x, h), h * u, h * v], 1).ravel()
+    gX, gY = gradient_of(c, q, RunParams())
+    tol = 1e-12 if mk == "cart" else 1e-10
+    assert np.abs(gX[:, 0] - h * 0.2).max() < tol and np.abs(gY[:, 0] + h * 0.1).max() < tol
+    assert np.abs(gX[:, 1] - h * 0.05).max() < tol and np.abs(gY[:, 1] - h * 0.15).max() < tol
+
+
+def test_viscous_terms_vanish_with_zero_viscosity_and_are_dissipative():
+    c = Case.dam_break(N=4, nel=4, seed=2)
+    q = perturbed_state(c, seed=8)
+    rp = RunParams(num_flux=3)
+    base = oracle_residual(c, rp, q)
+    zero = oracle_residual(c, rp, q, av_visc=np.zeros(c.Nelem))
+    assert np.array_equal(base, zero)
+    # constant viscosity: the added term dissipates kinetic energy  sum w J (u, v) . dQt <= 0
+    eps = np.full(c.Nelem, 0.01)
+    visc = oracle_residual(c, rp, q, av_visc=eps) - base
+    E = c.Nelem
+    Q = q.reshape(E, 3, -1)
+    dV = visc.reshape(E, 3, -1)
+    wJ = quad_weights(c)
+    rate = (wJ * (Q[:, 1] / Q[:, 0] * dV[:, 1] + Q[:, 2] / Q[:, 0] * dV[:, 2])).sum()
+    assert rate < 0
+
+
+def test_shock_sensor_closed_form():
+    """h + b = c0 + a * phi_k(xi) on one element: only one non-constant mode, so the energy ratios
+    and the viscosity ramp follow in closed form."""
+    N = 6
+    c = Case(S.CASE_CONSTANT, S.MESH_CARTESIAN, N, 1, 1, -1.0, 1.0, -1.0, 1.0, p=(2.0, 0, 0, 0))
+    xi = c["x_GL"]
+    ngl = N + 1
+    phi = lambda k, x: L.legval(x, np.eye(ngl)[k]) * np.sqrt(k + 0.5)
+    rp = RunParams(eps0=0.02, sigma_min=-4.0, sigma_max=-2.0)
+    p = O.params(c, rp)
+    for k, a in ((N, 0.05), (N, 0.002), (N - 1, 0.01), (N - 2, 0.3), (N, 1e-4)):
+        h = (2.0 + a * phi(k, xi))[None, :].repeat(ngl, 0)     # [eta, xi]
+        q = np.stack([h.ravel(), 0 * h.ravel(), 0 * h.ravel()])[None].ravel()
+        visc = np.zeros(1)
+        O.call("shock_capturing", p, 1, q, np.zeros(ngl * ngl), c["VdmInv"], visc, None)
+        m00 = 2.0 * np.sqrt(2.0) * np.sqrt(2.0)               # constant 2 -> mode (0,0) coefficient
+        e_all = m00 ** 2 + a ** 2 * 2.0
+        if k == N:
+            sigma = np.log10(a * a * 2.0 / e_all)
+        elif k == N - 1:
+            sigma = np.log10(a * a * 2.0 / (e_all))           # term 2 over the modes below N
+        else:
+            sigma = -np.inf
+        if sigma <= rp.sigma_min:
+            expect = 0.0
+        elif sigma <= rp.sigma_max:
+            expect = 0.5 * rp.eps0 * (1 + np.sin(np.pi * (sigma - 0.5 * (rp.sigma_max + rp.sigma_min)) /
+                                                 (rp.sigma_max - rp.sigma_min)))
+        else:
+            expect = rp.eps0
+        assert abs(visc[0] - expect) < 1e-10 * rp.eps0, (k, a, visc[0], expect)
+
+
+# --------------------------------------------------------------------------------------------
+# limiter and RK
+# --------------------------------------------------------------------------------------------
+def test_positivity_limiter_invariants():
+    c = Case.flood(N=4, nel=6, seed=2)
+    E = c.Nelem
+    rng = np.random.default_rng(3)
+    Q = c["q"].reshape(E, 3, -1).copy() + 0.0
+    Q[:, 0] += rng.uniform(-0.05, 0.05, Q[:, 0].shape)
+    Q[:, 1] = rng.uniform(-0.1, 0.1, Q[:, 0].shape)
+    q = Q.ravel().copy()
+    rp = RunParams(pos_pres_tol=1e-3)
+    O.call("pos_pres", O.params(c, rp), E, c["EleSizes"], c["w_GL"], c["Jac"], q)
+    Q2 = q.reshape(E, 3, -1)
+    wJ = quad_weights(c)
+    mean0 = (wJ * Q[:, 0]).sum(1) / c["EleSizes"]
+    mean1 = (wJ * Q2[:, 0]).sum(1) / c["EleSizes"]
+    limited = (Q[:, 0] < 0).any(1) & (mean0 > 1e-5)
+    assert limited.any()
+    assert np.abs(mean1[limited] - mean0[limited]).max() < 1e-13
+    assert (Q2[limited, 0] >= 0).all()
+    untouched = (Q[:, 0] >= 1e-3).all(1)
+    assert np.array_equal(Q2[untouched], Q[untouched])
+    neg = mean0 < -1e-5
+    assert (np.isneginf(Q2[neg, 0])).all()
+
+
+def test_lsrk4_is_fourth_order():
+    lam = -1.3 + 0.7j
+    A, B, Cc = (np.array(x) for x in zip(*[(a, b, c) for a, b, c in zip(*__import__("esdgsem_mpiocca_amd").rk_coefficients(4))]))
+    p = O.OracleParams(1, 1, G, 1.0, 0.0, 1e-5, 0.0, 0.0, 0.0)
+    errs = []
+    for nsteps in (20, 40, 80):
+        dt = 1.0 / nsteps
+        y = np.array([1.0, 0.0, 0.0])      # (Re, Im, pad): one "element" of ngl=1 holds 3 values
+        g = np.zeros(3)
+        for _ in range(nsteps):
+            for s in range(5):
+                z = y[0] + 1j * y[1]
+                f = lam * z
+                Qt = np.array([f.real, f.imag, 0.0])
+                O.call("rk_ls_calc", p, 1, Qt, float(A[s]), g)
+                O.call("rk_ls_update", p, 1, float(Cc[s]), dt, g, y)
+        errs.append(abs(y[0] + 1j * y[1] - np.exp(lam)))
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert (orders > 3.8).all(), orders
+
+
+# --------------------------------------------------------------------------------------------
+# time loop
+# --------------------------------------------------------------------------------------------
+def test_timeloop_lake_at_rest_and_mass():
+    c = Case.gaussian_bump(N=4, nel=16, lake_at_rest=True)
+    rp = RunParams(num_flux=0)
+    q, n, t, dts = O.timeloop(c, rp, 200)
+    Q = q.reshape(c.Nelem, 3, -1)
+    assert n == 200
+    assert np.abs(Q[:, 1:]).max() <= 1e-13
+    # dt rule of DGtimeloop: sqrt(min area)/ngl * CFL / lambda_max
+    lam0 = np.sqrt(G * c["q"].reshape(c.Nelem, 3, -1)[:, 0].max())
+    assert abs(dts[0] - c.min_ele_size / c.ngl * rp.CFL / lam0) < 1e-15 * dts[0]
+
+
+def test_timeloop_mass_conservation_bump():
+    c = Case.gaussian_bump(N=4, nel=8)
+    q, n, t, _ = O.timeloop(c, RunParams(num_flux=0), 50)
+    wJ = quad_weights(c)
+    m0 = (wJ * c["q"].reshape(c.Nelem, 3, -1)[:, 0]).sum()
+    m1 = (wJ * q.reshape(c.Nelem, 3, -1)[:, 0]).sum()
+    assert abs(m1 - m0) < 1e-13 * m0
