@@ -45,8 +45,8 @@ KERNELS = ["volume_kernel", "collect_edge_data", "calc_num_fluxes", "surface_ker
            "update_kernel_ls", "shock_capturing", "calc_num_fluxes_gradient", "calc_gradient",
            "surface_kernel_gradient", "scale_gradient", "collect_edge_data_gradient",
            "volume_kernel_viscose", "calc_num_fluxes_viscose", "surface_kernel_visc",
-           "find_lambda_max", "reduce_max", "pos_pres", "fused_face", "fused_volume",
-           "fused_viscous_face", "fused_update"]
+           "find_lambda_max", "reduce_max", "pos_pres", "fused_face", "fused_volume", "fused_av",
+           "fused_viscous_face", "fused_update", "fused_timestep"]
 
 
 def c1_params(RunParams):
@@ -86,7 +86,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nel", type=int, default=512)
     ap.add_argument("--N", type=int, default=7)
-    ap.add_argument("--fused", type=int, default=0)
+    ap.add_argument("--fused", type=int, default=1,
+                    help="1: fused stage schedule (default); 0: the reference's kernel-by-kernel sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nel", type=int, default=128)
     ap.add_argument("--cpu-sample-steps", type=int, default=15)

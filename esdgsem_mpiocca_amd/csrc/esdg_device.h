@@ -1,0 +1,185 @@
+// esdg_device.h -- device functions shared by the kernels that follow the reference kernel by
+// kernel (esdg_kernels.hip) and the fused stage kernels (esdg_fused.hip).  Each states the okl
+// source it restates; the operation order inside every expression is that source's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "esdg.h"
+
+namespace esdg {
+
+// node index of trace point k of element side s (okl/GatherEdgeData/SolidWalls.okl:74-89)
+__device__ __forceinline__ int side_node(int ngl, int side, int k) {
+    switch (side) {
+        case 0: return k;
+        case 2: return ngl * ngl - ngl + k;
+        case 3: return k * ngl;
+        default: return (k + 1) * ngl - 1;
+    }
+}
+
+template <int NGL>
+struct ElemBlock {
+    static constexpr int NODES = NGL * NGL;
+    static constexpr int EPB = (256 / NODES) > 0 ? (256 / NODES) : 1;
+    static constexpr int THREADS = EPB * NODES;
+};
+
+__device__ __forceinline__ void wall_mirror(double nx, double ny, double vx, double vy, double& ox, double& oy) {
+    const double v_n = nx * vx + ny * vy;
+    const double v_t = -ny * vx + nx * vy;
+    ox = -nx * v_n - ny * v_t;
+    oy = -ny * v_n + nx * v_t;
+}
+
+struct FluxConsts {
+    double g, geomFace, tol;
+    int ES;
+};
+
+__device__ __forceinline__ void es_dissipation(double R21, double R22, double R23, double R31, double R32, double R33,
+                                               double SL11, double SL22, double SL33, double j1, double j2, double j3,
+                                               double& NF0, double& NF1, double& NF2) {
+    const double R11 = 1.0, R12 = 0.0, R13 = 1.0;
+    const double S11 = R11 * SL11, S12 = 0.0, S13 = R13 * SL33;
+    const double S21 = R21 * SL11, S22 = R22 * SL22, S23 = R23 * SL33;
+    const double S31 = R31 * SL11, S32 = R32 * SL22, S33 = R33 * SL33;
+    const double M11 = S11 * R11 + S12 * R12 + S13 * R13;
+    const double M12 = S11 * R21 + S12 * R22 + S13 * R23;
+    const double M13 = S11 * R31 + S12 * R32 + S13 * R33;
+    const double M21 = S21 * R11 + S22 * R12 + S23 * R13;
+    const double M22 = S21 * R21 + S22 * R22 + S23 * R23;
+    const double M23 = S21 * R31 + S22 * R32 + S23 * R33;
+    const double M31 = S31 * R11 + S32 * R12 + S33 * R13;
+    const double M32 = S31 * R21 + S32 * R22 + S33 * R23;
+    const double M33 = S31 * R31 + S32 * R32 + S33 * R33;
+    NF0 += -0.5 * (M11 * j1 + M12 * j2 + M13 * j3);
+    NF1 += -0.5 * (M21 * j1 + M22 * j2 + M23 * j3);
+    NF2 += -0.5 * (M31 * j1 + M32 * j2 + M33 * j3);
+}
+
+// One face node of the ES flux in Cartesian momentum; returns F* (unscaled).
+__device__ __forceinline__ void flux_es(const FluxConsts& c, double nx, double ny, double hL, double huL, double hvL,
+                                        double hR, double huR, double hvR, double bL, double bR, double& NF0,
+                                        double& NF1, double& NF2) {
+    const double g = c.g;
+    double uL, vL, uR, vR;
+    if (hL >= c.tol) { uL = huL / hL; vL = hvL / hL; } else { hL = 0.0; uL = 0.0; vL = 0.0; }
+    if (hR >= c.tol) { uR = huR / hR; vR = hvR / hR; } else { hR = 0.0; uR = 0.0; vR = 0.0; }
+    const double avgH = 0.5 * (hL + hR);
+    const double avgH2 = 0.5 * (hL * hL + hR * hR);
+    const double avgU = 0.5 * (uL + uR);
+    const double avgV = 0.5 * (vL + vR);
+    const double normV = avgU * nx + avgV * ny;
+    NF0 = avgH * normV;
+    NF1 = avgH * avgU * normV + (0.5 * g * avgH2) * nx;
+    NF2 = avgH * avgV * normV + (0.5 * g * avgH2) * ny;
+    if (c.ES) {
+        const double cL = g * hL, cR = g * hR;
+        const double avgC = sqrt(fabs(0.5 * (cL + cR)));
+        const double q1L = g * (hL + bL) - 0.5 * uL * uL - 0.5 * vL * vL;
+        const double q1R = g * (hR + bR) - 0.5 * uR * uR - 0.5 * vR * vR;
+        const double SL11 = 0.5 / g * fabs(nx * avgU + ny * avgV + avgC);
+        const double SL22 = fabs(avgH * (nx * avgU + ny * avgV));
+        const double SL33 = 0.5 / g * fabs(nx * avgU + ny * avgV - avgC);
+        es_dissipation(avgU + nx * avgC, ny, avgU - nx * avgC, avgV + ny * avgC, nx, avgV - ny * avgC, SL11, SL22, SL33,
+                       q1R - q1L, uR - uL, vR - vL, NF0, NF1, NF2);
+    }
+}
+
+// One face node of the rotated ES flux; returns F* with momentum rotated back.
+__device__ __forceinline__ void flux_es_rotated(const FluxConsts& c, double nx, double ny, double hL, double huL,
+                                                double hvL, double hR, double huR, double hvR, double bL, double bR,
+                                                double& NF0, double& NF1, double& NF2) {
+    const double g = c.g;
+    double uL, vL, uR, vR;
+    if (hL > c.tol) { uL = huL / hL; vL = hvL / hL; } else { uL = 0.0; vL = 0.0; }
+    if (hR > c.tol) { uR = huR / hR; vR = hvR / hR; } else { uR = 0.0; vR = 0.0; }
+    const double vLn = nx * uL + ny * vL, vLt = -ny * uL + nx * vL;
+    const double vRn = nx * uR + ny * vR, vRt = -ny * uR + nx * vR;
+    const double avgH = 0.5 * (hL + hR);
+    const double avgH2 = 0.5 * (hL * hL + hR * hR);
+    const double avgU = 0.5 * (vLn + vRn);
+    const double avgV = 0.5 * (vLt + vRt);
+    double F0 = avgH * avgU;
+    double F1 = avgH * avgU * avgU + (0.5 * g * avgH2);
+    double F2 = avgH * avgV * avgU;
+    if (c.ES) {
+        const double cL = g * hL, cR = g * hR;
+        const double avgC = sqrt(fabs(0.5 * (cL + cR)));
+        const double q1L = g * (hL + bL) - 0.5 * vLn * vLn - 0.5 * vLt * vLt;
+        const double q1R = g * (hR + bR) - 0.5 * vRn * vRn - 0.5 * vRt * vRt;
+        const double SL11 = 0.5 / g * fabs(avgU + avgC);
+        const double SL22 = fabs(avgH * avgU);
+        const double SL33 = 0.5 / g * fabs(avgU - avgC);
+        es_dissipation(avgU + avgC, 0.0, avgU - avgC, avgV, 1.0, avgV, SL11, SL22, SL33, q1R - q1L, vRn - vLn,
+                       vRt - vLt, F0, F1, F2);
+    }
+    NF0 = F0;
+    NF1 = nx * F1 - ny * F2;
+    NF2 = ny * F1 + nx * F2;
+}
+
+// Local Lax-Friedrichs: okl/RiemannSolvers/calcLaxFriedrich.okl:17-170.  Dry sides keep h but
+// zero momentum and wave speed; the dissipation uses max(c+|u|, c+|v|) over both sides.
+__device__ __forceinline__ void flux_lf(const FluxConsts& c, double nx, double ny, double hL, double huL, double hvL,
+                                        double hR, double huR, double hvR, double& NF0, double& NF1, double& NF2) {
+    const double g = c.g;
+    double uL = 0.0, vL = 0.0, cL = 0.0, uR = 0.0, vR = 0.0, cR = 0.0;
+    if (hL > c.tol) { uL = huL / hL; vL = hvL / hL; cL = sqrt(g * hL); } else { huL = 0.0; hvL = 0.0; }
+    if (hR > c.tol) { uR = huR / hR; vR = hvR / hR; cR = sqrt(g * hR); } else { huR = 0.0; hvR = 0.0; }
+    const double F2L = huL * uL + 0.5 * g * hL * hL, F2R = huR * uR + 0.5 * g * hR * hR;
+    const double G2L = huL * vL, G2R = huR * vR;
+    const double F3L = huL * vL, F3R = huR * vR;
+    const double G3L = hvL * vL + 0.5 * g * hL * hL, G3R = hvR * vR + 0.5 * g * hR * hR;
+    const double maxL1 = fmax(cL + fabs(uL), cR + fabs(uR));
+    const double maxL2 = fmax(cL + fabs(vL), cR + fabs(vR));
+    const double lam = fmax(maxL1, maxL2);
+    const double FF0 = 0.5 * (huL + huR), FF1 = 0.5 * (F2L + F2R), FF2 = 0.5 * (F3L + F3R);
+    const double GG0 = 0.5 * (hvL + hvR), GG1 = 0.5 * (G2L + G2R), GG2 = 0.5 * (G3L + G3R);
+    NF0 = (nx * FF0 + ny * GG0) - 0.5 * lam * (hR - hL);
+    NF1 = (nx * FF1 + ny * GG1) - 0.5 * lam * (huR - huL);
+    NF2 = (nx * FF2 + ny * GG2) - 0.5 * lam * (hvR - hvL);
+}
+
+__device__ __forceinline__ int surf_node(int ngl, int side, int k) {
+    switch (side) {
+        case 0: return k;
+        case 2: return ngl * ngl - ngl + k;
+        case 1: return k * ngl + ngl - 1;
+        default: return k * ngl;
+    }
+}
+
+struct SensorConsts {
+    double ZeroTOL, eps0, sigmaMin, sigmaMax;
+};
+
+template <int NGL>
+__device__ __forceinline__ double sensor_from_modes(const double* m, const SensorConsts& c) {
+    constexpr int N = NGL - 1;
+    double z1 = 0.0, n1 = 0.0, z2 = 0.0, n2 = 0.0;
+    for (int i = 0; i < N; ++i) z1 += m[i * NGL + N] * m[i * NGL + N] + m[N * NGL + i] * m[N * NGL + i];
+    z1 += m[N * NGL + N] * m[N * NGL + N];
+    for (int i = 0; i < N - 1; ++i)
+        z2 += m[i * NGL + N - 1] * m[i * NGL + N - 1] + m[(N - 1) * NGL + i] * m[(N - 1) * NGL + i];
+    z2 += m[(N - 1) * NGL + N - 1] * m[(N - 1) * NGL + N - 1];
+    for (int i = 0; i < NGL; ++i)
+        for (int j = 0; j < NGL; ++j) n1 += m[i * NGL + j] * m[i * NGL + j];
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) n2 += m[i * NGL + j] * m[i * NGL + j];
+    const double t1 = (n1 < c.ZeroTOL) ? 0.0 : z1 / n1;
+    const double t2 = (n1 < c.ZeroTOL) ? 0.0 : z2 / n2;   // the reference guards term 2 with nenner1
+    const double sigma = log10(fmax(t1, t2));
+    if (c.sigmaMin >= sigma) return 0.0;
+    if (c.sigmaMax >= sigma) {
+        const double PI = 3.141592653589793;
+        const double delta = 1.0 + sin(PI * (sigma - 0.5 * (c.sigmaMax + c.sigmaMin)) / (c.sigmaMax - c.sigmaMin));
+        return c.eps0 * 0.5 * delta;
+    }
+    return c.eps0;
+}
+
+
+}  // namespace esdg

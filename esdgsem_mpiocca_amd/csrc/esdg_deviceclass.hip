@@ -250,7 +250,7 @@ class deviceclass {
         FusedArgs a;
         a.Nelem = Nelem; a.Nfaces = Nfaces;
         a.artificial_viscosity = rp_.artificial_viscosity; a.positivity = rp_.positivity;
-        a.q = o_q; a.gRK = o_gRK;
+        a.q = o_q; a.gRK = o_gRK; a.Qt = o_Qt;
         a.Jac = o_Jac; a.Yxi = o_Yxi; a.Yeta = o_Yeta; a.Xxi = o_Xxi; a.Xeta = o_Xeta;
         a.B = o_B; a.Bx = o_Bx; a.By = o_By; a.D = o_D; a.Dhat = o_Dhat; a.Dstrong = o_Dstrong;
         a.VdmInv = o_VdmInv; a.GLw = o_GLw; a.EleSizes = o_EleSizes;
@@ -274,6 +274,7 @@ class deviceclass {
         if (n == "ViscPara") return o_ViscPara;
         if (n == "qGradientX") return o_qGradientX;
         if (n == "qGradientY") return o_qGradientY;
+        if (n == "bL") return o_bL;
         return fused_.buffer(n);
     }
 

@@ -10,7 +10,7 @@ namespace esdg {
 // Device pointers the fused stage reads and writes (owned by the deviceclass mirror).
 struct FusedArgs {
     int Nelem, Nfaces, artificial_viscosity, positivity;
-    double *q, *gRK;
+    double *q, *gRK, *Qt;
     const double *Jac, *Yxi, *Yeta, *Xxi, *Xeta, *B, *Bx, *By, *D, *Dhat, *Dstrong, *VdmInv, *GLw, *EleSizes;
     const double *nx, *ny, *scal, *bL, *bR;
     const int *EdgeData, *EdgeReversed, *ElemToEdge, *EMS, *EOR;
@@ -30,11 +30,11 @@ class FusedStage {
     bool ready = false;
     int Nelem = 0, Nfaces = 0;
     bool av = false;
-    double* qn = nullptr;       // second state buffer (the fused stage writes q^{s+1} here)
-    double* fstar = nullptr;    // face fluxes
-    double* fgrad = nullptr;    // face gradient fluxes (AV)
-    double* gtr = nullptr;      // element-side traces of the scaled gradients (AV)
-    double* fvisc = nullptr;    // face viscous fluxes (AV)
+    double* qn = nullptr;       // reserved
+    double* fstar = nullptr;    // face fluxes F* [f][3][ngl]
+    double* fgrad = nullptr;    // BR1 face fluxes [X|Y][f][2][ngl] (AV)
+    double* grad = nullptr;     // scaled velocity gradients [X|Y][e][2][ngl^2] (AV)
+    double* fvisc = nullptr;    // viscous face fluxes [f][2][ngl] (AV)
 };
 
 }  // namespace esdg

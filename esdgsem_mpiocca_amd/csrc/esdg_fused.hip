@@ -1,11 +1,440 @@
-// esdg_fused.hip -- fused per-stage kernels.
+// esdg_fused.hip -- the fused stage: the reference's per-stage arithmetic in fewer HBM passes.
+//
+// Reference sequence per RK stage (src/deviceclass.cpp:957-1122) and what the fused schedule
+// does with it:
+//   CollectEdgeData + calcNumFluxes [+ calcNumFluxesGradient]      -> fused_face (one pass over faces,
+//                                                                     traces read straight from q)
+//   VolumeKernel + SurfaceKernel                                    -> fused_volume (esdg_volume.hip)
+//   [ShockCapturing + calcGradient + SurfaceKernelGradient + scaleGradient + VolumeKernelViscose]
+//                                                                   -> fused_av (one element pass)
+//   [CollectEdgeDataGradient + calcNumFluxesViscose]                -> fused_viscous_face
+//   [SurfaceKernelVisc] + calcRK + UpdateKernel [+ PosPres]         -> fused_update, or folded into
+//                                                                     fused_volume when AV is off
+// Every value is computed with the reference kernel's expressions in the reference's order, and Qt
+// receives its contributions in the reference's order (volume, surface, viscous volume, viscous
+// surface), so the fused stage reproduces the kernel-by-kernel results.
+#include "esdg_device.h"
 #include "esdg_fused.h"
 
 namespace esdg {
 
+int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt);
+
+// ---------------------------------------------------------------------------------------------
+// fused_face: CollectEdgeData (SolidWalls.okl:16-207) + calcNumFluxes (calcESFlux.okl /
+// calcESFluxRotated.okl) [+ calcNumFluxesGradient.okl:16-135], one thread per face node.
+// ---------------------------------------------------------------------------------------------
+struct FaceState {
+    double h, hu, hv;
+};
+
+// Trace of node k as CollectEdgeData writes it into qL (left = true) or qR.
+__device__ __forceinline__ FaceState gathered(const double* __restrict__ q, int ngl, const int* E, bool left, int k,
+                                              double nx, double ny) {
+    const int ngl2 = ngl * ngl;
+    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+    FaceState L{0.0, 0.0, 0.0}, R{0.0, 0.0, 0.0};
+    if (cpuL == 0) {
+        const int64_t id = (int64_t)iel * ngl2 * ESDG_NEQ + side_node(ngl, isl, k);
+        L = FaceState{q[id], q[id + ngl2], q[id + 2 * ngl2]};
+    }
+    if (cpuR == 0) {
+        const int64_t id = (int64_t)ier * ngl2 * ESDG_NEQ + side_node(ngl, isr, k);
+        R = FaceState{q[id], q[id + ngl2], q[id + 2 * ngl2]};
+    }
+    if (cpuL == -1) { L.h = R.h; wall_mirror(nx, ny, R.hu, R.hv, L.hu, L.hv); }
+    if (cpuR == -1) { R.h = L.h; wall_mirror(nx, ny, L.hu, L.hv, R.hu, R.hv); }
+    return left ? L : R;
+}
+
+template <bool AV>
+__global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kind, FluxConsts c,
+                                                    const int* __restrict__ ED, const double* __restrict__ q,
+                                                    const double* __restrict__ nxa, const double* __restrict__ nya,
+                                                    const double* __restrict__ scal, const double* __restrict__ BL,
+                                                    const double* __restrict__ BR, double* __restrict__ SP,
+                                                    double* __restrict__ SGX, double* __restrict__ SGY) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nfaces * ngl) return;
+    const int f = (int)(t / ngl), i = (int)(t % ngl);
+    const int* E = ED + 8 * (int64_t)f;
+    const int rv = E[4];
+    const int ir = (1 - rv) * (ngl - 1 - i) + rv * i;
+    const int64_t xidR = (int64_t)f * ngl + ir;
+    const double nx = nxa[t], ny = nya[t], sc = scal[t];
+    // qL at node i and qR at node ir, exactly as the gather kernel stores them
+    const FaceState L = gathered(q, ngl, E, true, i, nx, ny);
+    const FaceState R = gathered(q, ngl, E, false, ir, nxa[xidR], nya[xidR]);
+    double F0, F1, F2;
+    if (kind == ESDG_FLUX_ES_ROTATED)
+        flux_es_rotated(c, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
+    else
+        flux_es(c, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
+    const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
+    SP[eid] = sc * c.geomFace * F0;
+    SP[eid + ngl] = sc * c.geomFace * F1;
+    SP[eid + 2 * ngl] = sc * c.geomFace * F2;
+    if (AV) {
+        // calcNumFluxesGradient reads qL and qR unreversed at node i
+        const int cpuL = E[5], cpuR = E[6];
+        const FaceState Ri = (rv == 1) ? R : gathered(q, ngl, E, false, i, nx, ny);
+        const FaceState LL = (cpuL == -1) ? Ri : L;
+        const FaceState RR = (cpuR == -1) ? L : Ri;
+        const double iL = (LL.h > c.tol) ? 1.0 / LL.h : 0.0;
+        const double iR = (RR.h > c.tol) ? 1.0 / RR.h : 0.0;
+        const double uL = LL.hu * iL, uR = RR.hu * iR, vL = LL.hv * iL, vR = RR.hv * iR;
+        const int64_t gid = (int64_t)f * ngl * 2 + i;
+        SGX[gid] = nx * sc * c.geomFace * 0.5 * (uL + uR);
+        SGX[gid + ngl] = nx * sc * c.geomFace * 0.5 * (vL + vR);
+        SGY[gid] = ny * sc * c.geomFace * 0.5 * (uL + uR);
+        SGY[gid + ngl] = ny * sc * c.geomFace * 0.5 * (vL + vR);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// fused_av: ShockCapturing.okl:17-234 + calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148
+// + scaleGradient.okl:17-60 + VolumeKernelViscose.okl:17-150 for one block of elements, one thread
+// per node, all intermediate fields in LDS.  Writes the scaled gradients (read by the viscous face
+// kernel), the element viscosity and Qt(hu, hv) += viscous volume term.
+// ---------------------------------------------------------------------------------------------
+template <int NGL>
+__global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorConsts sc,
+                                                  const double* __restrict__ q, const double* __restrict__ b,
+                                                  const double* __restrict__ VdmInv, const double* __restrict__ Dh,
+                                                  const double* __restrict__ Ds, const double* __restrict__ Jac,
+                                                  const double* __restrict__ Yxi, const double* __restrict__ Yeta,
+                                                  const double* __restrict__ Xxi, const double* __restrict__ Xeta,
+                                                  const int* __restrict__ E2E, const int* __restrict__ EMS,
+                                                  const int* __restrict__ EOR, const double* __restrict__ SGX,
+                                                  const double* __restrict__ SGY, double* __restrict__ gX,
+                                                  double* __restrict__ gY, double* __restrict__ visc,
+                                                  double* __restrict__ Qt) {
+    using EB = ElemBlock<NGL>;
+    constexpr int NN = NGL * NGL;
+    __shared__ double sV[NN], sDh[NN], sDs[NN];
+    __shared__ double sA[EB::THREADS], sB[EB::THREADS], sC[EB::THREADS], sE[EB::THREADS];
+    __shared__ double sEps[EB::EPB];
+    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
+    const int e = blockIdx.x * EB::EPB + el;
+    const bool act = e < Nelem;
+    if (threadIdx.x < NN) {
+        sV[threadIdx.x] = VdmInv[threadIdx.x];
+        sDh[threadIdx.x] = Dh[threadIdx.x];
+        sDs[threadIdx.x] = Ds[threadIdx.x];
+    }
+    const int a = n % NGL, c = n / NGL;           // node n = (xi = a, eta = c)
+    const int64_t xid = (int64_t)e * NN + n;
+    double h = 0.0, hu = 0.0, hv = 0.0;
+    if (act) {
+        const int64_t id = (int64_t)e * NN * ESDG_NEQ + n;
+        h = q[id]; hu = q[id + NN]; hv = q[id + 2 * NN];
+        // sensor input: slot (a', c') = n' holds h+b at node (xi = a', eta = c'), i.e. node c'*NGL + a'
+        const int aa = n / NGL, cc = n % NGL;
+        const int node = cc * NGL + aa;
+        sA[threadIdx.x] = b[(int64_t)e * NN + node] + q[(int64_t)e * NN * ESDG_NEQ + node];
+        const double Qinv = (h > tol) ? 1.0 / h : 0.0;
+        sB[threadIdx.x] = hu * Qinv;                // u at node n
+        sC[threadIdx.x] = hv * Qinv;                // v at node n
+    }
+    __syncthreads();
+    // ---- sensor: modal transform of h + b ----
+    double* U = sA + el * NN;
+    {
+        const int aa = n / NGL, cc = n % NGL;
+        double s = 0.0;
+        if (act) {
+#pragma unroll
+            for (int l = 0; l < NGL; ++l) s += sV[aa * NGL + l] * U[l * NGL + cc];
+        }
+        __syncthreads();
+        if (act) sE[threadIdx.x] = s;
+        __syncthreads();
+        double* M = sE + el * NN;
+        s = 0.0;
+        if (act) {
+#pragma unroll
+            for (int l = 0; l < NGL; ++l) s += M[aa * NGL + l] * sV[cc * NGL + l];
+        }
+        __syncthreads();
+        if (act) U[n] = s;
+        __syncthreads();
+        if (threadIdx.x < EB::EPB) {
+            const int ee = blockIdx.x * EB::EPB + threadIdx.x;
+            double v = 0.0;
+            if (ee < Nelem) {
+                v = sensor_from_modes<NGL>(sA + threadIdx.x * NN, sc);
+                visc[ee] = v;
+            }
+            sEps[threadIdx.x] = v;
+        }
+    }
+    // ---- BR1 gradient (weak form volume part) ----
+    const double* u = sB + el * NN;
+    const double* v = sC + el * NN;
+    double gx0 = 0.0, gx1 = 0.0, gy0 = 0.0, gy1 = 0.0;
+    double ye = 0.0, yx = 0.0, xe = 0.0, xx = 0.0, rJ = 0.0;
+    if (act) {
+        double u_xi = 0.0, u_eta = 0.0, v_xi = 0.0, v_eta = 0.0;
+#pragma unroll
+        for (int l = 0; l < NGL; ++l) {
+            u_xi += sDh[a * NGL + l] * u[c * NGL + l];
+            u_eta += sDh[c * NGL + l] * u[l * NGL + a];
+            v_xi += sDh[a * NGL + l] * v[c * NGL + l];
+            v_eta += sDh[c * NGL + l] * v[l * NGL + a];
+        }
+        ye = Yeta[xid]; yx = Yxi[xid]; xe = Xeta[xid]; xx = Xxi[xid]; rJ = Jac[xid];
+        const double u_x = ye * u_xi - yx * u_eta;
+        const double v_x = ye * v_xi - yx * v_eta;
+        const double u_y = -xe * u_xi + xx * u_eta;
+        const double v_y = -xe * v_xi + xx * v_eta;
+        gx0 = rJ * u_x; gx1 = rJ * v_x; gy0 = rJ * u_y; gy1 = rJ * v_y;
+        // surface lift of the BR1 face fluxes: sides 0/2 first, then 1/3 (SurfaceKernelGradient.okl)
+#pragma unroll
+        for (int ph = 0; ph < 2; ++ph) {
+            int side = -1, t = 0;
+            if (ph == 0) {
+                if (c == 0) { side = 0; t = a; }
+                else if (c == NGL - 1) { side = 2; t = a; }
+            } else {
+                if (a == NGL - 1) { side = 1; t = c; }
+                else if (a == 0) { side = 3; t = c; }
+            }
+            if (side < 0) continue;
+            const int ifa = E2E[e * 4 + side] - 1;
+            const int rev = EOR[e * 4 + side];
+            const int ms = EMS[e * 4 + side];
+            const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
+            const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
+            if (ph == 0) gx0 += ms * SGX[fl] * rJ;
+            else gx0 += ms * rJ * SGX[fl];
+            gx1 += ms * rJ * SGX[fl + NGL];
+            gy0 += ms * rJ * SGY[fl];
+            gy1 += ms * rJ * SGY[fl + NGL];
+        }
+        // scaleGradient: multiply by h
+        gx0 = h * gx0; gx1 = h * gx1; gy0 = h * gy0; gy1 = h * gy1;
+        const int64_t gid = (int64_t)e * NN * 2 + n;
+        gX[gid] = gx0; gX[gid + NN] = gx1;
+        gY[gid] = gy0; gY[gid + NN] = gy1;
+    }
+    __syncthreads();   // sEps ready; u, v no longer needed
+    // ---- viscous volume term: contravariant fluxes, then Dstrong ----
+    if (act) {
+        const double eps = sEps[el];
+        const double f2 = eps * gx0, f3 = eps * gx1, g2 = eps * gy0, g3 = eps * gy1;
+        sA[threadIdx.x] = ye * f2 - xe * g2;      // F2
+        sB[threadIdx.x] = -yx * f2 + xx * g2;     // G2
+        sC[threadIdx.x] = ye * f3 - xe * g3;      // F3
+        sE[threadIdx.x] = -yx * f3 + xx * g3;     // G3
+    }
+    __syncthreads();
+    if (act) {
+        const double* F2 = sA + el * NN;
+        const double* G2 = sB + el * NN;
+        const double* F3 = sC + el * NN;
+        const double* G3 = sE + el * NN;
+        double d2 = 0.0, d3 = 0.0;
+#pragma unroll
+        for (int l = 0; l < NGL; ++l) {
+            d2 += sDs[a * NGL + l] * F2[c * NGL + l];
+            d2 += sDs[c * NGL + l] * G2[l * NGL + a];
+            d3 += sDs[a * NGL + l] * F3[c * NGL + l];
+            d3 += sDs[c * NGL + l] * G3[l * NGL + a];
+        }
+        double* o = Qt + (int64_t)e * NN * ESDG_NEQ + n;
+        o[NN] += rJ * d2;
+        o[2 * NN] += rJ * d3;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// fused_viscous_face: CollectEdgeDataGradient.okl:17-194 + calcNumFluxesViscose.okl:17-107
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl, double geomFace,
+                                                            const int* __restrict__ ED,
+                                                            const double* __restrict__ gX,
+                                                            const double* __restrict__ gY,
+                                                            const double* __restrict__ visc,
+                                                            const double* __restrict__ nxa,
+                                                            const double* __restrict__ nya,
+                                                            const double* __restrict__ scal, double* __restrict__ SPV) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nfaces * ngl) return;
+    const int f = (int)(t / ngl), i = (int)(t % ngl);
+    const int* E = ED + 8 * (int64_t)f;
+    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], rv = E[4], cpuL = E[5], cpuR = E[6];
+    const int64_t eid = (int64_t)f * ngl * 2 + i;
+    double N0 = 0.0, N1 = 0.0;
+    if (!(cpuL == -1 || cpuR == -1)) {
+        // both sides present (procL = procR = 0): the boundary-face branches of the gather never apply
+        const int ngl2 = ngl * ngl;
+        const int inc = (rv == 0) ? (ngl - 1) - i : i;
+        const int64_t idL = (int64_t)iel * ngl2 * 2 + side_node(ngl, isl, i);
+        const int64_t idR = (int64_t)ier * ngl2 * 2 + side_node(ngl, isr, inc);
+        const double nx = nxa[t], ny = nya[t];
+        const double f2L = nx * gX[idL] + ny * gY[idL];
+        const double f2R = nx * gX[idR] + ny * gY[idR];
+        const double f3L = nx * gX[idL + ngl2] + ny * gY[idL + ngl2];
+        const double f3R = nx * gX[idR + ngl2] + ny * gY[idR + ngl2];
+        const double m = fmax(visc[iel], visc[ier]);
+        N0 = 0.5 * m * (f2L + f2R);
+        N1 = 0.5 * m * (f3L + f3R);
+    }
+    const double sc = scal[t];
+    SPV[eid] = -sc * geomFace * N0;
+    SPV[eid + ngl] = -sc * geomFace * N1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// fused_update: SurfaceKernelVisc.okl:17-144 + rkLS.okl + UpdateKernelLS.okl [+ PosPres.okl]
+// ---------------------------------------------------------------------------------------------
+template <int NGL, bool VISC, bool POS>
+__global__ void __launch_bounds__(256) k_fused_update(int Nelem, double rkA, double cdt, double tol, double zero_tol,
+                                                      const double* __restrict__ Jac, const int* __restrict__ E2E,
+                                                      const int* __restrict__ EMS, const int* __restrict__ EOR,
+                                                      const double* __restrict__ SPV, const double* __restrict__ Qt,
+                                                      double* __restrict__ gRK, double* __restrict__ q,
+                                                      const double* __restrict__ EleSizes,
+                                                      const double* __restrict__ GLw) {
+    using EB = ElemBlock<NGL>;
+    constexpr int NN = NGL * NGL;
+    __shared__ double sH[POS ? EB::THREADS : 1], sHU[POS ? EB::THREADS : 1], sHV[POS ? EB::THREADS : 1];
+    __shared__ double sJ[POS ? EB::THREADS : 1];
+    __shared__ double sAvg[EB::EPB][5];
+    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
+    const int e = blockIdx.x * EB::EPB + el;
+    const bool act = e < Nelem;
+    const int a = n % NGL, c = n / NGL;
+    const int64_t id = (int64_t)e * NN * ESDG_NEQ + n;
+    double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+    if (act) {
+        double t1 = Qt[id], t2 = Qt[id + NN], t3 = Qt[id + 2 * NN];
+        if (VISC) {
+            const double rJ = Jac[(int64_t)e * NN + n];
+#pragma unroll
+            for (int ph = 0; ph < 2; ++ph) {
+                int side = -1, t = 0;
+                if (ph == 0) {
+                    if (c == 0) { side = 0; t = a; }
+                    else if (c == NGL - 1) { side = 2; t = a; }
+                } else {
+                    if (a == NGL - 1) { side = 1; t = c; }
+                    else if (a == 0) { side = 3; t = c; }
+                }
+                if (side < 0) continue;
+                const int ifa = E2E[e * 4 + side] - 1;
+                const int rev = EOR[e * 4 + side];
+                const int ms = EMS[e * 4 + side];
+                const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
+                const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
+                t2 += -ms * rJ * SPV[fl];
+                t3 += -ms * rJ * SPV[fl + NGL];
+            }
+        }
+        double g1 = gRK[id], g2 = gRK[id + NN], g3 = gRK[id + 2 * NN];
+        g1 = rkA * g1 + t1;
+        g2 = rkA * g2 + t2;
+        g3 = rkA * g3 + t3;
+        gRK[id] = g1; gRK[id + NN] = g2; gRK[id + 2 * NN] = g3;
+        q1 = q[id]; q2 = q[id + NN]; q3 = q[id + 2 * NN];
+        q1 += cdt * g1; q2 += cdt * g2; q3 += cdt * g3;
+        if (!POS) { q[id] = q1; q[id + NN] = q2; q[id + 2 * NN] = q3; }
+    }
+    if (!POS) return;
+    // positivity limiter on the updated state (PosPres.okl:17-223); node n is (xi = a, eta = c)
+    if (act) {
+        sH[threadIdx.x] = q1; sHU[threadIdx.x] = q2; sHV[threadIdx.x] = q3;
+        sJ[threadIdx.x] = Jac[(int64_t)e * NN + n];
+    }
+    __syncthreads();
+    if (threadIdx.x < EB::EPB) {
+        const int ee = blockIdx.x * EB::EPB + threadIdx.x;
+        if (ee < Nelem) {
+            const double* H = sH + threadIdx.x * NN;
+            const double* HU = sHU + threadIdx.x * NN;
+            const double* HV = sHV + threadIdx.x * NN;
+            const double* JJ = sJ + threadIdx.x * NN;
+            double hmin = 0.0, ha = 0.0, hua = 0.0, hva = 0.0;
+            for (int l = 0; l < NGL; ++l)
+                for (int k = 0; k < NGL; ++k) {
+                    const int nn = k * NGL + l;
+                    const double w = GLw[l] * GLw[k] / JJ[nn];
+                    ha += H[nn] * w;
+                    hua += HU[nn] * w;
+                    hva += HV[nn] * w;
+                    hmin = fmin(hmin, H[nn]);
+                }
+            const double size = EleSizes[ee];
+            ha = ha / size;
+            if (fabs(ha) < zero_tol) ha = 0.0;
+            sAvg[threadIdx.x][0] = ha;
+            sAvg[threadIdx.x][1] = hua / size;
+            sAvg[threadIdx.x][2] = hva / size;
+            sAvg[threadIdx.x][3] = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
+            sAvg[threadIdx.x][4] = hmin;
+        }
+    }
+    __syncthreads();
+    if (act) {
+        const double ha = sAvg[el][0], hua = sAvg[el][1], hva = sAvg[el][2], th = sAvg[el][3], hmin = sAvg[el][4];
+        if (hmin < 0.0) {
+            double hn = 0.0, hun = 0.0, hvn = 0.0;
+            if (ha > 0.0) {
+                hn = fabs(th * (q1 - ha) + ha);
+                if (hn > tol) {
+                    hun = th * (q2 - hua) + hua;
+                    hvn = th * (q3 - hva) + hva;
+                }
+            }
+            if (ha < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
+            q1 = hn; q2 = hun; q3 = hvn;
+        } else if (q1 < tol) {
+            q2 = 0.0; q3 = 0.0;
+        }
+        q[id] = q1; q[id + NN] = q2; q[id + 2 * NN] = q3;
+    }
+}
+
+template <int NGL>
+void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
+    using EB = ElemBlock<NGL>;
+    SensorConsts sc{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
+    const double* SGX = F.fgrad;
+    const double* SGY = F.fgrad + (size_t)A.Nfaces * NGL * 2;
+    double* gX = F.grad;
+    double* gY = F.grad + (size_t)A.Nelem * NGL * NGL * 2;
+    k_fused_av<NGL><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
+        A.Nelem, ctx->p.PosPresTOL, sc, A.q, A.B, A.VdmInv, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
+        A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, gX, gY, A.ViscPara, A.Qt);
+}
+
+template <int NGL>
+void launch_fused_update(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F, double rkA, double cdt, bool visc,
+                         bool pos) {
+    using EB = ElemBlock<NGL>;
+    const unsigned nb = blocks_for(A.Nelem, EB::EPB);
+    const double tol = ctx->p.PosPresTOL, zt = ctx->p.ZeroTOL;
+#define ESDG_UPD(V, P)                                                                                             \
+    k_fused_update<NGL, V, P><<<nb, EB::THREADS, 0, ctx->stream>>>(A.Nelem, rkA, cdt, tol, zt, A.Jac, A.ElemToEdge,  \
+                                                                   A.EMS, A.EOR, F.fvisc, A.Qt, A.gRK, A.q,          \
+                                                                   A.EleSizes, A.GLw)
+    if (visc && pos) ESDG_UPD(true, true);
+    else if (visc) ESDG_UPD(true, false);
+    else if (pos) ESDG_UPD(false, true);
+    else ESDG_UPD(false, false);
+#undef ESDG_UPD
+}
+
 int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     Nelem = Ne; Nfaces = Nf; av = with_av;
-    return set_error(ctx, ESDG_ERR_ARG, "fused stage is not available in this build");
+    const int ngl = ctx->p.ngl;
+    const size_t fn = (size_t)Nf * ngl;
+    int rc = esdg_malloc(ctx, (void**)&fstar, fn * 3 * sizeof(double));
+    if (!rc && av) rc = esdg_malloc(ctx, (void**)&fgrad, fn * 4 * sizeof(double));
+    if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * ngl * ngl * 4 * sizeof(double));
+    if (!rc && av) rc = esdg_malloc(ctx, (void**)&fvisc, fn * 2 * sizeof(double));
+    ready = (rc == ESDG_OK);
+    return rc;
 }
 
 int FusedStage::setup(esdg_ctx*, const int*, const int*, const int*, const int*, const double*, const double*,
@@ -13,15 +442,65 @@ int FusedStage::setup(esdg_ctx*, const int*, const int*, const int*, const int*,
     return ESDG_OK;
 }
 
-int FusedStage::stage(esdg_ctx* ctx, int, double, double, const FusedArgs&) {
-    return set_error(ctx, ESDG_ERR_STATE, "fused stage is not available in this build");
+int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArgs& A) {
+    if (!ready) return set_error(ctx, ESDG_ERR_STATE, "fused stage used before init");
+    const int ngl = ctx->p.ngl;
+    const int kind = ctx->p.num_flux;
+    FluxConsts fc{ctx->p.g, ctx->p.geomFace, ctx->p.PosPresTOL, ctx->p.ES};
+    const unsigned fb = blocks_for((int64_t)A.Nfaces * ngl, 256);
+    {
+        KernelScope ks(ctx, "fused_face");
+        if (av)
+            k_fused_face<true><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, A.EdgeData, A.q, A.nx, A.ny, A.scal,
+                                                            A.bL, A.bR, fstar, fgrad, fgrad + (size_t)A.Nfaces * ngl * 2);
+        else
+            k_fused_face<false><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, A.EdgeData, A.q, A.nx, A.ny,
+                                                             A.scal, A.bL, A.bR, fstar, nullptr, nullptr);
+        int rc = launch_check(ctx, "fused_face");
+        if (rc) return rc;
+    }
+    if (!av) {
+        // volume + surface + RK in one element pass; the limiter (if on) follows as its own pass
+        int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr);
+        if (!rc && A.positivity) rc = esdg_pos_pres(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q);
+        return rc;
+    }
+    int rc = fused_volume(ctx, A, false, rkA, cdt, fstar, A.Qt);
+    if (rc) return rc;
+    {
+        KernelScope ks(ctx, "fused_av");
+        ESDG_NGL_DISPATCH(ngl, launch_fused_av, ctx, A, *this);
+        rc = launch_check(ctx, "fused_av");
+        if (rc) return rc;
+    }
+    {
+        KernelScope ks(ctx, "fused_viscous_face");
+        k_fused_viscous_face<<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, ctx->p.geomFace, A.EdgeData, grad,
+                                                          grad + (size_t)A.Nelem * ngl * ngl * 2, A.ViscPara, A.nx,
+                                                          A.ny, A.scal, fvisc);
+        rc = launch_check(ctx, "fused_viscous_face");
+        if (rc) return rc;
+    }
+    {
+        KernelScope ks(ctx, "fused_update");
+        ESDG_NGL_DISPATCH(ngl, launch_fused_update, ctx, A, *this, rkA, cdt, true, A.positivity != 0);
+        rc = launch_check(ctx, "fused_update");
+    }
+    return rc;
 }
 
 void FusedStage::release(esdg_ctx* ctx) {
-    for (double** p : {&qn, &fstar, &fgrad, &gtr, &fvisc})
+    for (double** p : {&qn, &fstar, &fgrad, &grad, &fvisc})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
+    ready = false;
 }
 
-void* FusedStage::buffer(const std::string&) { return nullptr; }
+void* FusedStage::buffer(const std::string& name) {
+    if (name == "FStar") return fstar;
+    if (name == "FGrad") return fgrad;
+    if (name == "Grad") return grad;
+    if (name == "FVisc") return fvisc;
+    return nullptr;
+}
 
 }  // namespace esdg

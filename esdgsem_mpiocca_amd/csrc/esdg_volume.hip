@@ -15,9 +15,25 @@
 // continues the sums of its column nodes with the eta-pairs, and stores Qt.  The row sums cross
 // over through LDS.  Per element the kernel reads q, the four metrics, 1/J, Bx, By once and writes
 // Qt once: 13 doubles per node, the compulsory traffic.
+#include "esdg_device.h"
+#include "esdg_fused.h"
 #include "esdg_internal.h"
 
 namespace esdg {
+
+// What the flux-differencing kernel does after the volume term of its nodes:
+//   VOL_ONLY      store Qt (the reference's VolumeKernel)
+//   VOL_SURF      add the surface lift of the face fluxes (okl/DG/SurfaceKernel.okl, DBSurf = 0), store Qt
+//   VOL_SURF_RK   as VOL_SURF, then the low-storage RK step gRK = A gRK + Qt, q += cdt gRK in place
+//                 (okl/RungeKutta/rkLS.okl + UpdateKernelLS.okl); Qt never reaches HBM
+enum VolMode { VOL_ONLY = 0, VOL_SURF = 1, VOL_SURF_RK = 2 };
+
+struct SurfIn {
+    const int *E2E, *EMS, *EOR;
+    const double* FStar;
+    double rkA, cdt;
+    double* gRK;
+};
 
 template <int NGL>
 struct VolCfg {
@@ -81,13 +97,13 @@ __device__ __forceinline__ void line_sweep(const double* __restrict__ D, double 
     }
 }
 
-template <int NGL>
+template <int NGL, int MODE>
 __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, const double* __restrict__ Jac,
                                                         const double* __restrict__ Yxi, const double* __restrict__ Yeta,
                                                         const double* __restrict__ Xxi, const double* __restrict__ Xeta,
-                                                        const double* __restrict__ q, const double* __restrict__ D,
+                                                        double* q, const double* __restrict__ D,
                                                         const double* __restrict__ Bx, const double* __restrict__ By,
-                                                        double* __restrict__ Qt) {
+                                                        double* __restrict__ Qt, SurfIn si) {
     using C = VolCfg<NGL>;
     __shared__ double s[C::EPW * C::ESZ];
     const int lane = threadIdx.x;
@@ -165,12 +181,63 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         line_sweep<NGL, true>(D, fourth_g, half_g, h, hu, hv, u, v, Y, X, b1, b2, b3);
         const int64_t xb = (int64_t)e * C::NN + r;
         const int64_t ob = (int64_t)e * C::NN * ESDG_NEQ + r;
+        double rJ[NGL];
 #pragma unroll
         for (int k = 0; k < NGL; ++k) {
-            const double rJ = Jac[xb + k * NGL];
-            Qt[ob + k * NGL] = -rJ * b1[k];
-            Qt[ob + C::NN + k * NGL] = -rJ * b2[k] - g * h[k] * Bx[xb + k * NGL];
-            Qt[ob + 2 * C::NN + k * NGL] = -rJ * b3[k] - g * h[k] * By[xb + k * NGL];
+            rJ[k] = Jac[xb + k * NGL];
+            b1[k] = -rJ[k] * b1[k];
+            b2[k] = -rJ[k] * b2[k] - g * h[k] * Bx[xb + k * NGL];
+            b3[k] = -rJ[k] * b3[k] - g * h[k] * By[xb + k * NGL];
+        }
+        if (MODE != VOL_ONLY) {
+            // surface lift: bottom/top sides first (nodes k = 0, N of this column, trace index r),
+            // then right/left (the whole column r = N or 0, trace index k)
+#pragma unroll
+            for (int ph = 0; ph < 2; ++ph) {
+#pragma unroll
+                for (int k = 0; k < NGL; ++k) {
+                    int side = -1, t = 0;
+                    if (ph == 0) {
+                        if (k == 0) { side = 0; t = r; }
+                        else if (k == NGL - 1) { side = 2; t = r; }
+                    } else {
+                        if (r == NGL - 1) { side = 1; t = k; }
+                        else if (r == 0) { side = 3; t = k; }
+                    }
+                    if (side < 0) continue;
+                    const int ifa = si.E2E[e * 4 + side] - 1;
+                    const int rev = si.EOR[e * 4 + side];
+                    const int ms = si.EMS[e * 4 + side];
+                    const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
+                    const int64_t fl = (int64_t)ifa * NGL * ESDG_NEQ + inc;
+                    b1[k] += -ms * si.FStar[fl] * rJ[k];
+                    b2[k] += -ms * rJ[k] * si.FStar[fl + NGL];
+                    b3[k] += -ms * rJ[k] * si.FStar[fl + 2 * NGL];
+                }
+            }
+        }
+        if (MODE == VOL_SURF_RK) {
+#pragma unroll
+            for (int k = 0; k < NGL; ++k) {
+                const int64_t id = ob + k * NGL;
+                double g1 = si.gRK[id], g2 = si.gRK[id + C::NN], g3 = si.gRK[id + 2 * C::NN];
+                g1 = si.rkA * g1 + b1[k];
+                g2 = si.rkA * g2 + b2[k];
+                g3 = si.rkA * g3 + b3[k];
+                si.gRK[id] = g1;
+                si.gRK[id + C::NN] = g2;
+                si.gRK[id + 2 * C::NN] = g3;
+                q[id] = q[id] + si.cdt * g1;
+                q[id + C::NN] = q[id + C::NN] + si.cdt * g2;
+                q[id + 2 * C::NN] = q[id + 2 * C::NN] + si.cdt * g3;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NGL; ++k) {
+                Qt[ob + k * NGL] = b1[k];
+                Qt[ob + C::NN + k * NGL] = b2[k];
+                Qt[ob + 2 * C::NN + k * NGL] = b3[k];
+            }
         }
     }
 }
@@ -233,8 +300,29 @@ void launch_volume_fluxdiff(esdg_ctx* ctx, int Nelem, const double* Jac, const d
                             const double* Xxi, const double* Xeta, const double* q, const double* D, const double* Bx,
                             const double* By, double* Qt) {
     using C = VolCfg<NGL>;
-    k_volume_fluxdiff<NGL><<<blocks_for(Nelem, C::EPW), 64, 0, ctx->stream>>>(Nelem, ctx->p.g, Jac, Yxi, Yeta, Xxi, Xeta, q,
-                                                                              D, Bx, By, Qt);
+    SurfIn si{};
+    k_volume_fluxdiff<NGL, VOL_ONLY><<<blocks_for(Nelem, C::EPW), 64, 0, ctx->stream>>>(
+        Nelem, ctx->p.g, Jac, Yxi, Yeta, Xxi, Xeta, const_cast<double*>(q), D, Bx, By, Qt, si);
+}
+
+template <int NGL>
+void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar,
+                         double* Qt) {
+    using C = VolCfg<NGL>;
+    SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK};
+    if (rk)
+        k_volume_fluxdiff<NGL, VOL_SURF_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
+            a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
+    else
+        k_volume_fluxdiff<NGL, VOL_SURF><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
+            a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, Qt, si);
+}
+
+// Fused volume + surface lift (+ RK update in place when rk): called by the fused stage.
+int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt) {
+    KernelScope ks(ctx, "fused_volume");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt);
+    return launch_check(ctx, "fused_volume");
 }
 
 template <int NGL>

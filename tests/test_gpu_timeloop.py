@@ -38,22 +38,24 @@ def fields_rel_l2(case, a, b):
             for v in range(3)]
 
 
-def test_c0_bump_200_steps(dev):
+@pytest.mark.parametrize("fused", [False, True], ids=["reference_sequence", "fused"])
+def test_c0_bump_200_steps(dev, fused):
     """Periodic Cartesian 16x16, N=4, EC volume + ES surface, LSRK4, CFL 0.5, 200 steps."""
     case = Case.gaussian_bump(N=4, nel=16)
     rp = RunParams(num_flux=0)
     qo, no, to, _ = O.timeloop(case, rp, 200)
-    qg, ng, tg = run_gpu(dev, case, rp, 200)
+    qg, ng, tg = run_gpu(dev, case, rp, 200, fused=fused)
     assert ng == no == 200
     assert abs(tg - to) <= 1e-14 * to
     h, hu, hv = fields_rel_l2(case, qg, qo)
     assert h <= 1e-12 and hu <= 1e-12 and hv <= 1e-12, (h, hu, hv)
 
 
-def test_c0_lake_at_rest(dev):
+@pytest.mark.parametrize("fused", [False, True], ids=["reference_sequence", "fused"])
+def test_c0_lake_at_rest(dev, fused):
     case = Case.gaussian_bump(N=4, nel=16, lake_at_rest=True)
     rp = RunParams(num_flux=0)
-    qg, ng, _ = run_gpu(dev, case, rp, 200)
+    qg, ng, _ = run_gpu(dev, case, rp, 200, fused=fused)
     Q = qg.reshape(case.Nelem, 3, -1)
     assert np.abs(Q[:, 1]).max() <= 1e-13 and np.abs(Q[:, 2]).max() <= 1e-13
     eta = Q[:, 0] + case["b"].reshape(case.Nelem, -1)
@@ -61,25 +63,27 @@ def test_c0_lake_at_rest(dev):
     assert abs(total_mass(case, qg) - total_mass(case, case["q"])) <= 1e-13
 
 
-def test_dam_break_curved_av(dev):
+@pytest.mark.parametrize("fused", [False, True], ids=["reference_sequence", "fused"])
+def test_dam_break_curved_av(dev, fused):
     """Curved 12x12 N=7 dam break with modal sensor + artificial viscosity, rotated ES flux."""
     case = Case.dam_break(N=7, nel=12, noise=1e-3, seed=0)
     rp = RunParams(num_flux=3, artificial_viscosity=1, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5,
                    DFL=0.1)
     qo, no, to, dto = O.timeloop(case, rp, 20)
-    qg, ng, tg = run_gpu(dev, case, rp, 20)
+    qg, ng, tg = run_gpu(dev, case, rp, 20, fused=fused)
     assert ng == no
     assert abs(tg - to) <= 1e-12 * to
     errs = fields_rel_l2(case, qg, qo)
     assert max(errs) <= 1e-12, errs
 
 
-def test_flood_positivity(dev):
+@pytest.mark.parametrize("fused", [False, True], ids=["reference_sequence", "fused"])
+def test_flood_positivity(dev, fused):
     """Wet/dry flood on a curved 10x10 N=4 mesh with the positivity limiter."""
     case = Case.flood(N=4, nel=10, seed=1)
     rp = RunParams(num_flux=3, positivity=1, pos_pres_tol=1e-3, CFL=0.25)
     qo, no, to, _ = O.timeloop(case, rp, 20)
-    qg, ng, tg = run_gpu(dev, case, rp, 20)
+    qg, ng, tg = run_gpu(dev, case, rp, 20, fused=fused)
     assert ng == no
     errs = fields_rel_l2(case, qg, qo)
     assert max(errs) <= 1e-12, errs
@@ -98,3 +102,15 @@ def test_entropy_conservative_flux(dev):
     dS1 = total_entropy(case, qg1, rp.g) - total_entropy(case, case["q"], rp.g)
     assert abs(dS) < 1e-12
     assert dS1 < -1e-10
+
+
+@pytest.mark.parametrize("av,pos", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_fused_equals_reference_sequence_bitwise(dev, av, pos):
+    """The fused schedule performs the same arithmetic: identical state after 10 steps."""
+    case = Case.dam_break(N=7, nel=8, noise=1e-3, seed=5) if not pos else Case.flood(N=5, nel=8, seed=2)
+    rp = RunParams(num_flux=3, artificial_viscosity=av, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5,
+                   positivity=pos, pos_pres_tol=1e-3, CFL=0.25 if pos else 0.5)
+    qa, na, ta = run_gpu(dev, case, rp, 10, fused=False)
+    qb, nb, tb = run_gpu(dev, case, rp, 10, fused=True)
+    assert na == nb and ta == tb
+    assert np.array_equal(qa, qb) or np.abs(qa - qb).max() <= 1e-15 * np.abs(qa).max()
