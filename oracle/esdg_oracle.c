@@ -845,3 +845,91 @@ int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, doub
     free(gXL); free(gXR); free(gYL); free(gYR); free(SPV);
     return step;
 }
+
+/* Local Lax-Friedrichs flux: okl/RiemannSolvers/calcLaxFriedrich.okl:17-170 */
+void oracle_num_flux_lf(const oracle_params* p, int Nfaces, const int* EdgeReversed,
+                        const double* nxa, const double* nya, const double* scal, const double* qL,
+                        const double* qR, const double* BL, const double* BR, double* SP) {
+    (void)BL; (void)BR;
+    const int ngl = p->ngl;
+    const double g = p->g, tol = p->PosPresTOL;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int rv = EdgeReversed[f];
+        for (int i = 0; i < ngl; ++i) {
+            const long xid = (long)f * ngl + i;
+            const long eid = (long)f * ngl * NEQ + i;
+            const long eidR = (long)f * ngl * NEQ + rev_idx(ngl, rv, i);
+            const double nx = nxa[xid], ny = nya[xid], sc = scal[xid];
+            const double hL = qL[eid], hR = qR[eidR];
+            double uL, vL, huL, hvL, cL, uR, vR, huR, hvR, cR;
+            if (hL > tol) {
+                uL = qL[eid + ngl] / hL; vL = qL[eid + 2 * ngl] / hL;
+                huL = qL[eid + ngl]; hvL = qL[eid + 2 * ngl]; cL = sqrt(g * hL);
+            } else { uL = 0.0; vL = 0.0; huL = 0.0; hvL = 0.0; cL = 0.0; }
+            if (hR > tol) {
+                uR = qR[eidR + ngl] / hR; vR = qR[eidR + 2 * ngl] / hR;
+                huR = qR[eidR + ngl]; hvR = qR[eidR + 2 * ngl]; cR = sqrt(g * hR);
+            } else { uR = 0.0; vR = 0.0; huR = 0.0; hvR = 0.0; cR = 0.0; }
+            const double F1L = huL, F1R = huR, G1L = hvL, G1R = hvR;
+            const double F2L = huL * uL + 0.5 * g * hL * hL;
+            const double F2R = (huR * uR + 0.5 * g * hR * hR);
+            const double G2L = huL * vL, G2R = huR * vR;
+            const double F3L = huL * vL, F3R = huR * vR;
+            const double G3L = (hvL * vL + 0.5 * g * hL * hL);
+            const double G3R = (hvR * vR + 0.5 * g * hR * hR);
+            const double l1 = OMAX(cL + fabs(uL), cR + fabs(uR));
+            const double l2 = OMAX(cL + fabs(vL), cR + fabs(vR));
+            const double lm = OMAX(l1, l2);
+            const double N0 = (nx * (0.5 * (F1L + F1R)) + ny * (0.5 * (G1L + G1R))) - 0.5 * lm * (hR - hL);
+            const double N1 = (nx * (0.5 * (F2L + F2R)) + ny * (0.5 * (G2L + G2R))) - 0.5 * lm * (huR - huL);
+            const double N2 = (nx * (0.5 * (F3L + F3R)) + ny * (0.5 * (G3L + G3R))) - 0.5 * lm * (hvR - hvL);
+            SP[eid] = sc * p->geomFace * N0;
+            SP[eid + ngl] = sc * p->geomFace * N1;
+            SP[eid + 2 * ngl] = sc * p->geomFace * N2;
+        }
+    }
+}
+
+/* Standard strong-form volume term: okl/DG/VolumeKernelV99.okl:17-167 (FluxDifferencing = 0).
+ * Node (i = eta row, j = xi column). */
+void oracle_volume_std(const oracle_params* p, int Nelem, const double* Jac, const double* Yxi,
+                       const double* Yeta, const double* Xxi, const double* Xeta, const double* q,
+                       const double* D, const double* Bx, const double* By, double* Qt) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    const double g = p->g, half_g = 0.5 * g;
+    double* F = (double*)malloc(sizeof(double) * ngl2 * 3);
+    double* Gf = (double*)malloc(sizeof(double) * ngl2 * 3);
+    for (int e = 0; e < Nelem; ++e) {
+        const double* Q = q + (long)e * ngl2 * NEQ;
+        for (int n = 0; n < ngl2; ++n) {
+            const long xid = (long)e * ngl2 + n;
+            double h = Q[n], hu = 0.0, hv = 0.0, inv = 0.0;
+            if (h > p->PosPresTOL) { hu = Q[n + ngl2]; hv = Q[n + 2 * ngl2]; inv = 1.0 / h; }
+            else h = 0.0;
+            const double F1 = hu, F2 = hu * hu * inv + half_g * h * h, F3 = hv * hu * inv;
+            const double G1 = hv, G2 = hu * hv * inv, G3 = hv * hv * inv + half_g * h * h;
+            F[n] = Yeta[xid] * F1 - Xeta[xid] * G1;
+            Gf[n] = -Yxi[xid] * F1 + Xxi[xid] * G1;
+            F[ngl2 + n] = Yeta[xid] * F2 - Xeta[xid] * G2;
+            Gf[ngl2 + n] = -Yxi[xid] * F2 + Xxi[xid] * G2;
+            F[2 * ngl2 + n] = Yeta[xid] * F3 - Xeta[xid] * G3;
+            Gf[2 * ngl2 + n] = -Yxi[xid] * F3 + Xxi[xid] * G3;
+        }
+        for (int i = 0; i < ngl; ++i)
+            for (int j = 0; j < ngl; ++j) {
+                double d[3] = {0.0, 0.0, 0.0};
+                for (int l = 0; l < ngl; ++l) {
+                    const double dil = D[i * ngl + l], djl = D[j * ngl + l];
+                    for (int v = 0; v < 3; ++v)
+                        d[v] += djl * F[v * ngl2 + i * ngl + l] + dil * Gf[v * ngl2 + l * ngl + j];
+                }
+                const int n = i * ngl + j;
+                const long xid = (long)e * ngl2 + n;
+                double* out = Qt + (long)e * ngl2 * NEQ + n;
+                out[0] = -Jac[xid] * d[0];
+                out[ngl2] = -Jac[xid] * d[1] - g * Q[n] * Bx[xid];
+                out[2 * ngl2] = -Jac[xid] * d[2] - g * Q[n] * By[xid];
+            }
+    }
+    free(F); free(Gf);
+}

@@ -40,6 +40,12 @@ void oracle_num_flux_es_rotated(const oracle_params* p, int Nfaces, const int* E
                                 const double* nx, const double* ny, const double* scal,
                                 const double* qL, const double* qR, const double* bL,
                                 const double* bR, double* SurfaceParts);
+void oracle_num_flux_lf(const oracle_params* p, int Nfaces, const int* EdgeReversed,
+                        const double* nx, const double* ny, const double* scal, const double* qL,
+                        const double* qR, const double* bL, const double* bR, double* SurfaceParts);
+void oracle_volume_std(const oracle_params* p, int Nelem, const double* Jac, const double* Yxi,
+                       const double* Yeta, const double* Xxi, const double* Xeta, const double* q,
+                       const double* D, const double* Bx, const double* By, double* Qt);
 void oracle_surface(const oracle_params* p, int Nelem, const double* Jac, const int* EMS,
                     const int* EOR, const int* E2E, const double* FStar, const double* DBSurf1,
                     const double* DBSurf2, double* Qt);

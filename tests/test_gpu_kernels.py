@@ -109,9 +109,7 @@ def test_calc_num_fluxes(dev, flux):
         bL, bR = np.zeros(fn), np.zeros(fn)
         O.call("collect_edge_data_bottom", S.p, S.Nf, S.h["EdgeData"], S.h["B"], bL, bR)
         SP = np.zeros(fn * 3)
-        if flux == 1:
-            pytest.skip("LF flux has no oracle function yet")
-        name = "num_flux_es_rotated" if flux == 3 else "num_flux_es"
+        name = {0: "num_flux_es", 1: "num_flux_lf", 3: "num_flux_es_rotated"}[flux]
         O.call(name, S.p, S.Nf, S.h["ER"], S.h["nx"], S.h["ny"], S.h["scal"], qL, qR, bL, bR, SP)
         dqL, dqR, dbL, dbR = (S.ctx.array(a) for a in (qL, qR, bL, bR))
         dSP = S.ctx.empty(fn * 3)
@@ -120,6 +118,21 @@ def test_calc_num_fluxes(dev, flux):
         assert rel_l2(dSP.download(), SP) <= 1e-14
     finally:
         S.close()
+
+
+def test_volume_kernel_std(S):
+    """FluxDifferencing = 0: standard strong-form volume term with D = Dstrong."""
+    nq = S.Ne * S.ngl ** 2 * 3
+    Qt = np.zeros(nq)
+    h = S.h
+    Ds = np.ascontiguousarray(h["Dstrong"])
+    O.call("volume_std", S.p, S.Ne, h["Jac"], h["Yxi"], h["Yeta"], h["Xxi"], h["Xeta"], S.q, Ds, h["Bx"],
+           h["By"], Qt)
+    d = S.d
+    dQt = S.ctx.empty(nq)
+    S.ctx.volume_kernel_std(S.Ne, d["Jac"], d["Yxi"], d["Yeta"], d["Xxi"], d["Xeta"], S.ctx.array(S.q),
+                            d["Dstrong"], d["Bx"], d["By"], dQt)
+    assert rel_l2(dQt.download(), Qt) <= 1e-14
 
 
 def test_surface_kernel(S):
