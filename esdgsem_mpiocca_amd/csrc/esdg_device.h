@@ -156,19 +156,31 @@ struct SensorConsts {
     double ZeroTOL, eps0, sigmaMin, sigmaMax;
 };
 
+// One of the four modal energy sums of ShockCapturing.okl:145-196, in the okl's serial order:
+// which = 0: zaehler1 (top shell), 1: zaehler2 (second shell), 2: nenner1 (all), 3: nenner2 (below N).
 template <int NGL>
-__device__ __forceinline__ double sensor_from_modes(const double* m, const SensorConsts& c) {
+__device__ __forceinline__ double sensor_sum(const double* m, int which) {
     constexpr int N = NGL - 1;
-    double z1 = 0.0, n1 = 0.0, z2 = 0.0, n2 = 0.0;
-    for (int i = 0; i < N; ++i) z1 += m[i * NGL + N] * m[i * NGL + N] + m[N * NGL + i] * m[N * NGL + i];
-    z1 += m[N * NGL + N] * m[N * NGL + N];
-    for (int i = 0; i < N - 1; ++i)
-        z2 += m[i * NGL + N - 1] * m[i * NGL + N - 1] + m[(N - 1) * NGL + i] * m[(N - 1) * NGL + i];
-    z2 += m[(N - 1) * NGL + N - 1] * m[(N - 1) * NGL + N - 1];
-    for (int i = 0; i < NGL; ++i)
-        for (int j = 0; j < NGL; ++j) n1 += m[i * NGL + j] * m[i * NGL + j];
-    for (int i = 0; i < N; ++i)
-        for (int j = 0; j < N; ++j) n2 += m[i * NGL + j] * m[i * NGL + j];
+    double s = 0.0;
+    if (which == 0) {
+        for (int i = 0; i < N; ++i) s += m[i * NGL + N] * m[i * NGL + N] + m[N * NGL + i] * m[N * NGL + i];
+        s += m[N * NGL + N] * m[N * NGL + N];
+    } else if (which == 1) {
+        for (int i = 0; i < N - 1; ++i)
+            s += m[i * NGL + N - 1] * m[i * NGL + N - 1] + m[(N - 1) * NGL + i] * m[(N - 1) * NGL + i];
+        s += m[(N - 1) * NGL + N - 1] * m[(N - 1) * NGL + N - 1];
+    } else if (which == 2) {
+        for (int i = 0; i < NGL; ++i)
+            for (int j = 0; j < NGL; ++j) s += m[i * NGL + j] * m[i * NGL + j];
+    } else {
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) s += m[i * NGL + j] * m[i * NGL + j];
+    }
+    return s;
+}
+
+// sigma = log10(max(z1/n1, z2/n2)) and the sine ramp to eps0 (ShockCapturing.okl:198-217)
+__device__ __forceinline__ double sensor_ramp(double z1, double z2, double n1, double n2, const SensorConsts& c) {
     const double t1 = (n1 < c.ZeroTOL) ? 0.0 : z1 / n1;
     const double t2 = (n1 < c.ZeroTOL) ? 0.0 : z2 / n2;   // the reference guards term 2 with nenner1
     const double sigma = log10(fmax(t1, t2));
@@ -181,5 +193,9 @@ __device__ __forceinline__ double sensor_from_modes(const double* m, const Senso
     return c.eps0;
 }
 
+template <int NGL>
+__device__ __forceinline__ double sensor_from_modes(const double* m, const SensorConsts& c) {
+    return sensor_ramp(sensor_sum<NGL>(m, 0), sensor_sum<NGL>(m, 1), sensor_sum<NGL>(m, 2), sensor_sum<NGL>(m, 3), c);
+}
 
 }  // namespace esdg

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
 // fused_av: ShockCapturing.okl:17-234 + calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148
 // + scaleGradient.okl:17-60 + VolumeKernelViscose.okl:17-150 for one block of elements, one thread
 // per node, all intermediate fields in LDS.  Writes the scaled gradients (read by the viscous face
-// kernel), the element viscosity and Qt(hu, hv) += viscous volume term.
+// kernel) as element-side traces, the element viscosity and Qt(hu, hv) += viscous volume term.
 // ---------------------------------------------------------------------------------------------
 template <int NGL>
 __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorConsts sc,
@@ -106,14 +106,14 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
                                                   const double* __restrict__ Xxi, const double* __restrict__ Xeta,
                                                   const int* __restrict__ E2E, const int* __restrict__ EMS,
                                                   const int* __restrict__ EOR, const double* __restrict__ SGX,
-                                                  const double* __restrict__ SGY, double* __restrict__ gX,
-                                                  double* __restrict__ gY, double* __restrict__ visc,
-                                                  double* __restrict__ Qt) {
+                                                  const double* __restrict__ SGY, double* __restrict__ gtr,
+                                                  double* __restrict__ visc, double* __restrict__ Qt) {
     using EB = ElemBlock<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sV[NN], sDh[NN], sDs[NN];
     __shared__ double sA[EB::THREADS], sB[EB::THREADS], sC[EB::THREADS], sE[EB::THREADS];
     __shared__ double sEps[EB::EPB];
+    __shared__ double sSum[EB::EPB][4];
     const int el = threadIdx.x / NN, n = threadIdx.x % NN;
     const int e = blockIdx.x * EB::EPB + el;
     const bool act = e < Nelem;
@@ -158,11 +158,18 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
         __syncthreads();
         if (act) U[n] = s;
         __syncthreads();
+        // the four energy sums of an element run on four threads, each in the okl's serial order
+        if (threadIdx.x < 4 * EB::EPB) {
+            const int ee = threadIdx.x >> 2, which = threadIdx.x & 3;
+            if (blockIdx.x * EB::EPB + ee < Nelem) sSum[ee][which] = sensor_sum<NGL>(sA + ee * NN, which);
+        }
+        __syncthreads();
         if (threadIdx.x < EB::EPB) {
             const int ee = blockIdx.x * EB::EPB + threadIdx.x;
             double v = 0.0;
             if (ee < Nelem) {
-                v = sensor_from_modes<NGL>(sA + threadIdx.x * NN, sc);
+                const double* S4 = sSum[threadIdx.x];
+                v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
                 visc[ee] = v;
             }
             sEps[threadIdx.x] = v;
@@ -213,9 +220,16 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
         }
         // scaleGradient: multiply by h
         gx0 = h * gx0; gx1 = h * gx1; gy0 = h * gy0; gy1 = h * gy1;
-        const int64_t gid = (int64_t)e * NN * 2 + n;
-        gX[gid] = gx0; gX[gid + NN] = gx1;
-        gY[gid] = gy0; gY[gid + NN] = gy1;
+        // side traces of the scaled gradients, [e][side][gX0, gX1, gY0, gY1][k] (k in the side's order)
+        double* T = gtr + (int64_t)e * 16 * NGL;
+        auto put = [&](int side, int k) {
+            double* t = T + side * 4 * NGL + k;
+            t[0] = gx0; t[NGL] = gx1; t[2 * NGL] = gy0; t[3 * NGL] = gy1;
+        };
+        if (c == 0) put(0, a);
+        if (c == NGL - 1) put(2, a);
+        if (a == NGL - 1) put(1, c);
+        if (a == 0) put(3, c);
     }
     __syncthreads();   // sEps ready; u, v no longer needed
     // ---- viscous volume term: contravariant fluxes, then Dstrong ----
@@ -252,8 +266,7 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl, double geomFace,
                                                             const int* __restrict__ ED,
-                                                            const double* __restrict__ gX,
-                                                            const double* __restrict__ gY,
+                                                            const double* __restrict__ gtr,
                                                             const double* __restrict__ visc,
                                                             const double* __restrict__ nxa,
                                                             const double* __restrict__ nya,
@@ -266,16 +279,16 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl,
     const int64_t eid = (int64_t)f * ngl * 2 + i;
     double N0 = 0.0, N1 = 0.0;
     if (!(cpuL == -1 || cpuR == -1)) {
-        // both sides present (procL = procR = 0): the boundary-face branches of the gather never apply
-        const int ngl2 = ngl * ngl;
+        // both sides present (procL = procR = 0): the boundary-face branches of the gather never apply;
+        // the right element's trace is read at the reversed index, as CollectEdgeDataGradient does
         const int inc = (rv == 0) ? (ngl - 1) - i : i;
-        const int64_t idL = (int64_t)iel * ngl2 * 2 + side_node(ngl, isl, i);
-        const int64_t idR = (int64_t)ier * ngl2 * 2 + side_node(ngl, isr, inc);
+        const double* TL = gtr + ((int64_t)iel * 4 + isl) * 4 * ngl + i;
+        const double* TR = gtr + ((int64_t)ier * 4 + isr) * 4 * ngl + inc;
         const double nx = nxa[t], ny = nya[t];
-        const double f2L = nx * gX[idL] + ny * gY[idL];
-        const double f2R = nx * gX[idR] + ny * gY[idR];
-        const double f3L = nx * gX[idL + ngl2] + ny * gY[idL + ngl2];
-        const double f3R = nx * gX[idR + ngl2] + ny * gY[idR + ngl2];
+        const double f2L = nx * TL[0] + ny * TL[2 * ngl];
+        const double f2R = nx * TR[0] + ny * TR[2 * ngl];
+        const double f3L = nx * TL[ngl] + ny * TL[3 * ngl];
+        const double f3R = nx * TR[ngl] + ny * TR[3 * ngl];
         const double m = fmax(visc[iel], visc[ier]);
         N0 = 0.5 * m * (f2L + f2R);
         N1 = 0.5 * m * (f3L + f3R);
@@ -401,11 +414,9 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     SensorConsts sc{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
     const double* SGX = F.fgrad;
     const double* SGY = F.fgrad + (size_t)A.Nfaces * NGL * 2;
-    double* gX = F.grad;
-    double* gY = F.grad + (size_t)A.Nelem * NGL * NGL * 2;
     k_fused_av<NGL><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
         A.Nelem, ctx->p.PosPresTOL, sc, A.q, A.B, A.VdmInv, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
-        A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, gX, gY, A.ViscPara, A.Qt);
+        A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, A.Qt);
 }
 
 template <int NGL>
@@ -431,7 +442,7 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     const size_t fn = (size_t)Nf * ngl;
     int rc = esdg_malloc(ctx, (void**)&fstar, fn * 3 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fgrad, fn * 4 * sizeof(double));
-    if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * ngl * ngl * 4 * sizeof(double));
+    if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * 16 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fvisc, fn * 2 * sizeof(double));
     ready = (rc == ESDG_OK);
     return rc;
@@ -476,8 +487,7 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     {
         KernelScope ks(ctx, "fused_viscous_face");
         k_fused_viscous_face<<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, ctx->p.geomFace, A.EdgeData, grad,
-                                                          grad + (size_t)A.Nelem * ngl * ngl * 2, A.ViscPara, A.nx,
-                                                          A.ny, A.scal, fvisc);
+                                                          A.ViscPara, A.nx, A.ny, A.scal, fvisc);
         rc = launch_check(ctx, "fused_viscous_face");
         if (rc) return rc;
     }
