@@ -37,6 +37,7 @@ HIP_SOURCES = [
     "esdg_kernels.hip",
     "esdg_volume.hip",
     "esdg_fused.hip",
+    "esdg_stage.hip",
     "esdg_deviceclass.hip",
 ]
 

@@ -33,6 +33,52 @@ __device__ __forceinline__ void wall_mirror(double nx, double ny, double vx, dou
     oy = -ny * v_n + nx * v_t;
 }
 
+// Exterior-state variants of CollectEdgeData (src/deviceclass.cpp:363-415).
+struct GatherConsts {
+    int bc;          // esdg_bc
+    double h_zero;   // still-water level of the ocean boundary
+};
+
+struct FaceState {
+    double h, hu, hv;
+};
+
+// Trace of face node k as CollectEdgeData stores it into qL (left = true) or qR: the element side
+// traces (SolidWalls.okl:74-132) with the exterior state of a boundary face from the wall mirror
+// (SolidWalls.okl:133-175), zero momentum (InnerOuter.okl) or the ocean rule (OceanBoundary.okl:
+// still water h_zero - b where b <= 6.7, wall otherwise).  bl, br: the face's bL, bR at node k.
+__device__ __forceinline__ FaceState gathered(const double* __restrict__ q, int ngl, const int* E, bool left, int k,
+                                              double nx, double ny, const GatherConsts& gc, double bl, double br) {
+    const int ngl2 = ngl * ngl;
+    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+    FaceState L{0.0, 0.0, 0.0}, R{0.0, 0.0, 0.0};
+    if (cpuL == 0) {
+        const int64_t id = (int64_t)iel * ngl2 * 3 + side_node(ngl, isl, k);
+        L = FaceState{q[id], q[id + ngl2], q[id + 2 * ngl2]};
+    }
+    if (cpuR == 0) {
+        const int64_t id = (int64_t)ier * ngl2 * 3 + side_node(ngl, isr, k);
+        R = FaceState{q[id], q[id + ngl2], q[id + 2 * ngl2]};
+    }
+    if (gc.bc == ESDG_BC_INNER_OUTER) {
+        if (cpuL == -1) { L.h = R.h; L.hu = 0.0; L.hv = 0.0; }
+        if (cpuR == -1) { R.h = L.h; R.hu = 0.0; R.hv = 0.0; }
+    } else if (gc.bc == ESDG_BC_OCEAN) {
+        if (cpuL == -1) {
+            if (br > 6.7) { L.h = R.h; wall_mirror(nx, ny, R.hu, R.hv, L.hu, L.hv); }
+            else { L.h = gc.h_zero - br; L.hu = 0.0; L.hv = 0.0; }
+        }
+        if (cpuR == -1) {
+            if (bl > 6.7) { R.h = L.h; wall_mirror(nx, ny, L.hu, L.hv, R.hu, R.hv); }
+            else { R.h = gc.h_zero - bl; R.hu = 0.0; R.hv = 0.0; }
+        }
+    } else {
+        if (cpuL == -1) { L.h = R.h; wall_mirror(nx, ny, R.hu, R.hv, L.hu, L.hv); }
+        if (cpuR == -1) { R.h = L.h; wall_mirror(nx, ny, L.hu, L.hv, R.hu, R.hv); }
+    }
+    return left ? L : R;
+}
+
 struct FluxConsts {
     double g, geomFace, tol;
     int ES;
@@ -197,5 +243,71 @@ template <int NGL>
 __device__ __forceinline__ double sensor_from_modes(const double* m, const SensorConsts& c) {
     return sensor_ramp(sensor_sum<NGL>(m, 0), sensor_sum<NGL>(m, 1), sensor_sum<NGL>(m, 2), sensor_sum<NGL>(m, 3), c);
 }
+
+// ---------------------------------------------------------------------------------------------
+// Flux-differencing volume term building blocks (okl/DG/VolumeKernelFluxDiffV99.okl:17-57)
+// ---------------------------------------------------------------------------------------------
+template <int NGL>
+struct VolCfg {
+    static constexpr int EPW = 64 / NGL;        // elements per wave
+    static constexpr int NN = NGL * NGL;
+    static constexpr int RS = NGL + 1;          // LDS row stride (padding against bank conflicts)
+    static constexpr int PLANE = NGL * RS;      // one field of one element in LDS
+    static constexpr int ESZ = 4 * PLANE;       // h, hu, hv, 1/h (the row sums reuse three planes)
+};
+
+// Symmetric two-point EC flux along one line (F# of the reference for metrics (Y, X) = (Yeta, Xeta);
+// G# = -F# evaluated with (Yxi, Xxi)).  Operands per node: h, hu, hv, u = hu/h, v = hv/h.
+struct PairOut {
+    double f1, f2, f3;
+};
+
+__device__ __forceinline__ PairOut ec_pair(double fourth_g, double half_g, double hL, double huL, double hvL,
+                                           double uL, double vL, double YL, double XL, double hR, double huR,
+                                           double hvR, double uR, double vR, double YR, double XR) {
+    const double s = hL + hR;
+    const double h_A = fourth_g * s * s;
+    const double h2_A = 0.5 * (hL * hL + hR * hR);
+    const double hu_A = 0.5 * (huL + huR);
+    const double hv_A = 0.5 * (hvL + hvR);
+    const double u_A = 0.5 * (uL + uR);
+    const double v_A = 0.5 * (vL + vR);
+    const double Y_A = 0.5 * (YL + YR);
+    const double X_A = 0.5 * (XL + XR);
+    const double pr = h_A - half_g * h2_A;
+    PairOut o;
+    o.f1 = Y_A * hu_A - X_A * hv_A;
+    o.f2 = Y_A * (hu_A * u_A + pr) - X_A * hv_A * u_A;
+    o.f3 = Y_A * (hu_A * v_A) - X_A * (hv_A * v_A + pr);
+    return o;
+}
+
+// All pairs of one line of NGL nodes; acc[k] += sgn * D(k,l) F(k,l), l ascending for every k.
+template <int NGL, bool NEG>
+__device__ __forceinline__ void line_sweep(const double* __restrict__ D, double fourth_g, double half_g,
+                                           const double (&h)[NGL], const double (&hu)[NGL], const double (&hv)[NGL],
+                                           const double (&u)[NGL], const double (&v)[NGL],
+                                           const double (&Y)[NGL], const double (&X)[NGL], double (&a1)[NGL],
+                                           double (&a2)[NGL], double (&a3)[NGL]) {
+#pragma unroll
+    for (int c = 0; c < NGL; ++c) {
+#pragma unroll
+        for (int l = c; l < NGL; ++l) {
+            const PairOut F = ec_pair(fourth_g, half_g, h[c], hu[c], hv[c], u[c], v[c], Y[c], X[c], h[l], hu[l],
+                                      hv[l], u[l], v[l], Y[l], X[l]);
+            const double dcl = NEG ? -D[c * NGL + l] : D[c * NGL + l];
+            a1[c] += dcl * F.f1;
+            a2[c] += dcl * F.f2;
+            a3[c] += dcl * F.f3;
+            if (l != c) {
+                const double dlc = NEG ? -D[l * NGL + c] : D[l * NGL + c];
+                a1[l] += dlc * F.f1;
+                a2[l] += dlc * F.f2;
+                a3[l] += dlc * F.f3;
+            }
+        }
+    }
+}
+
 
 }  // namespace esdg

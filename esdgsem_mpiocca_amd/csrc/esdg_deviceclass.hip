@@ -11,6 +11,7 @@
 
 #include "esdg_fused.h"
 #include "esdg_internal.h"
+#include "esdg_stage.h"
 
 namespace esdg {
 
@@ -51,7 +52,11 @@ class deviceclass {
             A(&o_SurfacePartsVisc, fn * 2);
             A(&o_ViscParaL, Nfaces); A(&o_ViscParaR, Nfaces);
         }
-        if (rc == ESDG_OK && rp.fused) rc = fused_.init(ctx_, Nelem, Nfaces, rp.artificial_viscosity != 0);
+        if (rc == ESDG_OK && rp.fused == 1) rc = fused_.init(ctx_, Nelem, Nfaces, rp.artificial_viscosity != 0);
+        if (rp.fused == 2) {
+            // single-kernel stage: second state buffer (ping-pong), device clock, lambda_max cell
+            A(&o_q2, nodes * 3); A(&o_clock, 8); A(reinterpret_cast<double**>(&o_lmax), 1);
+        }
         return rc;
     }
 
@@ -78,7 +83,7 @@ class deviceclass {
         C(o_ElemToEdge, in.ElemToEdge, Nelem * 4 * sizeof(int));
         C(o_EdgeData, in.EdgeData, (size_t)Nfaces * 8 * sizeof(int));
         C(o_EdgeReversed, in.EdgeReversed, Nfaces * sizeof(int));
-        if (rc == ESDG_OK && rp_.fused)
+        if (rc == ESDG_OK && rp_.fused == 1)
             rc = fused_.setup(ctx_, o_EdgeData, o_ElemToEdge, o_ElemEdgeMasterSlave, o_ElemEdgeOrientation, o_Jac, o_nx, o_ny,
                               o_scal);
         return rc;
@@ -90,8 +95,13 @@ class deviceclass {
         if (rp_.rk_order != 3 && rp_.rk_order != 4) return set_error(ctx_, ESDG_ERR_ARG, "rk_order must be 3 or 4");
         if (rp_.rk_order == 4 && rp_.rk_ssp) return set_error(ctx_, ESDG_ERR_ARG, "rk_order 4 has no SSP variant (rkSSP must be 0)");
         if (rp_.artificial_viscosity && ngl < 3) return set_error(ctx_, ESDG_ERR_ARG, "artificial viscosity needs N >= 2");
+        if (rp_.fused < 0 || rp_.fused > 2) return set_error(ctx_, ESDG_ERR_ARG, "fused must be 0, 1 or 2");
         if (rp_.fused && (rp_.rk_ssp || !rp_.flux_differencing))
             return set_error(ctx_, ESDG_ERR_ARG, "the fused stage supports flux differencing with low-storage RK only");
+        if (rp_.fused == 2 && rp_.artificial_viscosity)
+            return set_error(ctx_, ESDG_ERR_ARG, "the single-kernel stage runs without artificial viscosity");
+        if (rp_.fused == 2 && (ngl < 3 || ngl > 8))
+            return set_error(ctx_, ESDG_ERR_ARG, "the single-kernel stage supports 2 <= N <= 7");
         const int nf = ctx_->p.num_flux;
         if (nf != ESDG_FLUX_ES && nf != ESDG_FLUX_ES_ROTATED && nf != ESDG_FLUX_LF)
             return set_error(ctx_, ESDG_ERR_ARG, "num_flux must be 0, 1 or 3");
@@ -207,6 +217,7 @@ class deviceclass {
             if (rc) return rc;
             bottom_done = true;
         }
+        if (rp_.fused == 2) return stage_timeloop(max_steps, steps_taken);
         int n = 0;
         while (t < rp_.T && n < max_steps) {
             double dt = 0.0;
@@ -232,6 +243,52 @@ class deviceclass {
         return esdg_finish(ctx_);
     }
 
+    // The same time loop with the single-kernel stage: dt and t live on the device, steps run in
+    // chunks (pairs replayed from a hipGraph) and the clock is read back after each chunk.  Steps
+    // issued after t reaches T leave the state unchanged and are not counted.
+    int stage_timeloop(int max_steps, int* steps_taken) {
+        int rc = ESDG_OK;
+        if (!stage_ready) {
+            StageScn S{};
+            S.ngl = ngl; S.Nelem = Nelem; S.Nfaces = Nfaces; S.ES = ctx_->p.ES; S.num_flux = ctx_->p.num_flux;
+            S.positivity = rp_.positivity;
+            S.g = ctx_->p.g; S.geomFace = ctx_->p.geomFace; S.tol = ctx_->p.PosPresTOL; S.zero_tol = ctx_->p.ZeroTOL;
+            S.dti = rp_.min_ele_size / (ngl) * rp_.CFL;
+            S.T = rp_.T;
+            S.Jac = o_Jac; S.Yxi = o_Yxi; S.Yeta = o_Yeta; S.Xxi = o_Xxi; S.Xeta = o_Xeta; S.Bx = o_Bx; S.By = o_By;
+            S.D = o_D; S.nx = o_nx; S.ny = o_ny; S.scal = o_scal; S.bL = o_bL; S.bR = o_bR;
+            S.EleSizes = o_EleSizes; S.GLw = o_GLw;
+            S.EdgeData = o_EdgeData; S.E2E = o_ElemToEdge; S.EMS = o_ElemEdgeMasterSlave; S.EOR = o_ElemEdgeOrientation;
+            S.q[0] = o_q; S.q[1] = o_q2; S.gRK = o_gRK; S.clock = o_clock; S.lmax = o_lmax;
+            rc = stage_.init(ctx_, std::vector<StageScn>{S}, ctx_->p.bc_kind, ctx_->p.h_zero);
+            if (rc) return rc;
+            stage_ready = true;
+        }
+        // the state is in o_q at every entry; the clock restarts from the host's t
+        double ck[8] = {0.0, t, 0.0, last_dt, 0.0, 0.0, 0.0, 0.0};
+        rc = esdg_memcpy(ctx_, o_clock, ck, sizeof ck, ESDG_H2D);
+        if (!rc) rc = esdg_fill(ctx_, reinterpret_cast<double*>(o_lmax), 0.0, 1);
+        int src = 0;
+        if (!rc) rc = stage_.lambda(ctx_, src);
+        int issued = 0;
+        while (!rc && issued < max_steps && ck[1] < rp_.T && ck[4] == 0.0) {
+            const int chunk = std::min(max_steps - issued, 64);
+            rc = stage_.run(ctx_, A, Cc, stages, chunk, true, src);
+            if (!rc) rc = esdg_memcpy(ctx_, ck, o_clock, sizeof ck, ESDG_D2H);
+            issued += chunk;
+        }
+        if (!rc && src == 1)
+            rc = hip_check(ctx_, hipMemcpyAsync(o_q, o_q2, (size_t)Nelem * ngl2 * 3 * sizeof(double), hipMemcpyDeviceToDevice,
+                                                ctx_->stream), "state copy");
+        if (!rc) rc = esdg_finish(ctx_);
+        if (rc) return rc;
+        t = ck[1];
+        last_dt = ck[3];
+        if (steps_taken) *steps_taken = (int)ck[2];
+        if (ck[4] != 0.0) return set_error(ctx_, ESDG_ERR_STATE, "time step is not positive and finite");
+        return ESDG_OK;
+    }
+
     int freeOccaVars() {
         void* ptrs[] = {o_q, o_Qt, o_gRK, o_Qtmp, o_D, o_Dstrong, o_Dhat, o_VdmInv, o_GLw, o_Jac, o_Yxi, o_Yeta, o_Xxi, o_Xeta,
                         o_B, o_Bx, o_By, o_x, o_y, o_nx, o_ny, o_scal, o_qL, o_qR, o_bL, o_bR, o_SurfaceParts, o_DBSurf1,
@@ -242,6 +299,11 @@ class deviceclass {
         for (void* p : ptrs)
             if (p) esdg_free(ctx_, p);
         fused_.release(ctx_);
+        stage_.release(ctx_);
+        for (void* p : {(void*)o_q2, (void*)o_clock, (void*)o_lmax})
+            if (p) esdg_free(ctx_, p);
+        o_q2 = o_clock = nullptr;
+        o_lmax = nullptr;
         o_q = o_Qt = o_gRK = o_Qtmp = nullptr;
         return ESDG_OK;
     }
@@ -286,6 +348,10 @@ class deviceclass {
     double t = 0.0, last_dt = 0.0;
     bool bottom_done = false;
     FusedStage fused_;
+    StageBatch stage_;
+    bool stage_ready = false;
+    double *o_q2 = nullptr, *o_clock = nullptr;
+    unsigned long long* o_lmax = nullptr;
     double *o_q = nullptr, *o_Qt = nullptr, *o_gRK = nullptr, *o_Qtmp = nullptr;
     double *o_D = nullptr, *o_Dstrong = nullptr, *o_Dhat = nullptr, *o_VdmInv = nullptr, *o_GLw = nullptr;
     double *o_Jac = nullptr, *o_Yxi = nullptr, *o_Yeta = nullptr, *o_Xxi = nullptr, *o_Xeta = nullptr;

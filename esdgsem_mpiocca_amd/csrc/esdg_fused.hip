@@ -24,31 +24,8 @@ int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double 
 // fused_face: CollectEdgeData (SolidWalls.okl:16-207) + calcNumFluxes (calcESFlux.okl /
 // calcESFluxRotated.okl) [+ calcNumFluxesGradient.okl:16-135], one thread per face node.
 // ---------------------------------------------------------------------------------------------
-struct FaceState {
-    double h, hu, hv;
-};
-
-// Trace of node k as CollectEdgeData writes it into qL (left = true) or qR.
-__device__ __forceinline__ FaceState gathered(const double* __restrict__ q, int ngl, const int* E, bool left, int k,
-                                              double nx, double ny) {
-    const int ngl2 = ngl * ngl;
-    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
-    FaceState L{0.0, 0.0, 0.0}, R{0.0, 0.0, 0.0};
-    if (cpuL == 0) {
-        const int64_t id = (int64_t)iel * ngl2 * ESDG_NEQ + side_node(ngl, isl, k);
-        L = FaceState{q[id], q[id + ngl2], q[id + 2 * ngl2]};
-    }
-    if (cpuR == 0) {
-        const int64_t id = (int64_t)ier * ngl2 * ESDG_NEQ + side_node(ngl, isr, k);
-        R = FaceState{q[id], q[id + ngl2], q[id + 2 * ngl2]};
-    }
-    if (cpuL == -1) { L.h = R.h; wall_mirror(nx, ny, R.hu, R.hv, L.hu, L.hv); }
-    if (cpuR == -1) { R.h = L.h; wall_mirror(nx, ny, L.hu, L.hv, R.hu, R.hv); }
-    return left ? L : R;
-}
-
 template <bool AV>
-__global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kind, FluxConsts c,
+__global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kind, FluxConsts c, GatherConsts gc,
                                                     const int* __restrict__ ED, const double* __restrict__ q,
                                                     const double* __restrict__ nxa, const double* __restrict__ nya,
                                                     const double* __restrict__ scal, const double* __restrict__ BL,
@@ -63,8 +40,8 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
     const int64_t xidR = (int64_t)f * ngl + ir;
     const double nx = nxa[t], ny = nya[t], sc = scal[t];
     // qL at node i and qR at node ir, exactly as the gather kernel stores them
-    const FaceState L = gathered(q, ngl, E, true, i, nx, ny);
-    const FaceState R = gathered(q, ngl, E, false, ir, nxa[xidR], nya[xidR]);
+    const FaceState L = gathered(q, ngl, E, true, i, nx, ny, gc, BL[t], BR[t]);
+    const FaceState R = gathered(q, ngl, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
     double F0, F1, F2;
     if (kind == ESDG_FLUX_ES_ROTATED)
         flux_es_rotated(c, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
@@ -77,7 +54,7 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
     if (AV) {
         // calcNumFluxesGradient reads qL and qR unreversed at node i
         const int cpuL = E[5], cpuR = E[6];
-        const FaceState Ri = (rv == 1) ? R : gathered(q, ngl, E, false, i, nx, ny);
+        const FaceState Ri = (rv == 1) ? R : gathered(q, ngl, E, false, i, nx, ny, gc, BL[t], BR[t]);
         const FaceState LL = (cpuL == -1) ? Ri : L;
         const FaceState RR = (cpuR == -1) ? L : Ri;
         const double iL = (LL.h > c.tol) ? 1.0 / LL.h : 0.0;
@@ -458,14 +435,15 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     const int ngl = ctx->p.ngl;
     const int kind = ctx->p.num_flux;
     FluxConsts fc{ctx->p.g, ctx->p.geomFace, ctx->p.PosPresTOL, ctx->p.ES};
+    GatherConsts gc{ctx->p.bc_kind, ctx->p.h_zero};
     const unsigned fb = blocks_for((int64_t)A.Nfaces * ngl, 256);
     {
         KernelScope ks(ctx, "fused_face");
         if (av)
-            k_fused_face<true><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, A.EdgeData, A.q, A.nx, A.ny, A.scal,
+            k_fused_face<true><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, A.q, A.nx, A.ny, A.scal,
                                                             A.bL, A.bR, fstar, fgrad, fgrad + (size_t)A.Nfaces * ngl * 2);
         else
-            k_fused_face<false><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, A.EdgeData, A.q, A.nx, A.ny,
+            k_fused_face<false><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, A.q, A.nx, A.ny,
                                                              A.scal, A.bL, A.bR, fstar, nullptr, nullptr);
         int rc = launch_check(ctx, "fused_face");
         if (rc) return rc;

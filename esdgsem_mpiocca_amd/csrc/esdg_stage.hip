@@ -1,0 +1,515 @@
+// esdg_stage.hip -- element-centric single-kernel RK stage over a batch of scenarios (see
+// esdg_stage.h).  Per element and stage it performs, in the reference's arithmetic and order:
+//   CollectEdgeData + calcNumFluxes for its 4 sides    (SolidWalls/InnerOuter/OceanBoundary.okl,
+//                                                       calcESFlux.okl / calcESFluxRotated.okl)
+//   VolumeKernel (flux differencing)                   (VolumeKernelFluxDiffV99.okl)
+//   SurfaceKernel                                      (SurfaceKernel.okl, DBSurf = 0)
+//   calcRK + UpdateKernel                              (rkLS.okl, UpdateKernelLS.okl)
+//   [PosPres]                                          (PosPres.okl)
+//   [FindLambdaMax of the new state, last stage]      (FindLambdaMax.okl)
+// and the dt of DGtimeloop (src/deviceclass.cpp:899-930) runs on the device.
+#include "esdg_device.h"
+#include "esdg_stage.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+namespace esdg {
+
+// Flux-differencing D of each degree (ngl 3..8) in constant memory: read through the scalar cache
+// into SGPRs, as a kernel argument would be, whatever scenario of the batch a block serves.  It
+// depends on the degree only; StageBatch::init fills it and refuses a batch whose D differs from
+// the one already loaded for that degree.
+__constant__ double c_stage_D[9][64];
+
+template <int NGL>
+struct StageCfg {
+    using V = VolCfg<NGL>;
+    static constexpr int EPW = V::EPW;
+    static constexpr int FSZ = 12 * NGL;              // face fluxes [side][v][k] of one element
+    static constexpr int ELDS = V::ESZ + FSZ;         // doubles of LDS per element
+    static constexpr int LDS = EPW * ELDS;            // doubles of LDS per block
+};
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+template <int LO, int HI>
+struct MaxLds {
+    static constexpr int value = cmax(StageCfg<LO>::LDS, MaxLds<LO + 1, HI>::value);
+};
+template <int HI>
+struct MaxLds<HI, HI> {
+    static constexpr int value = StageCfg<HI>::LDS;
+};
+
+// Volume term, surface lift, RK update, limiter and wave speed of one block of elements; the
+// face fluxes of the elements' sides are in LDS (FS).  Pointer parameters are __restrict__ so
+// that the geometry loads schedule as in a kernel taking them as arguments.
+template <int NGL>
+__device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, double* FS, double g, double tol,
+                                           double zero_tol, int positivity, double rkA, double rkC, double dt, bool last,
+                                           const double* __restrict__ qi, double* __restrict__ qo,
+                                           double* __restrict__ gRK, const double* __restrict__ Yeta,
+                                           const double* __restrict__ Xeta, const double* __restrict__ Yxi,
+                                           const double* __restrict__ Xxi, const double* __restrict__ Jac,
+                                           const double* __restrict__ Bx, const double* __restrict__ By,
+                                           const double* __restrict__ EleSizes, const double* __restrict__ GLw,
+                                           unsigned long long* __restrict__ lmax) {
+    using C = VolCfg<NGL>;
+    const int lane = threadIdx.x;
+    const double half_g = 0.5 * g, fourth_g = 0.25 * g;
+    double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], Y[NGL], X[NGL];
+    double a1[NGL], a2[NGL], a3[NGL];
+#pragma unroll
+    for (int c = 0; c < NGL; ++c) { a1[c] = 0.0; a2[c] = 0.0; a3[c] = 0.0; }
+
+    // ---- row phase: eta-row r ---------------------------------------------------------------
+    if (act) {
+        const int64_t qb = (int64_t)e * C::NN * 3 + r * NGL;
+        const int64_t xb = (int64_t)e * C::NN + r * NGL;
+#pragma unroll
+        for (int c = 0; c < NGL; ++c) {
+            h[c] = qi[qb + c];
+            hu[c] = qi[qb + C::NN + c];
+            hv[c] = qi[qb + 2 * C::NN + c];
+            Y[c] = Yeta[xb + c];
+            X[c] = Xeta[xb + c];
+        }
+#pragma unroll
+        for (int c = 0; c < NGL; ++c) {
+            const double inv = (h[c] > 0) ? 1.0 / h[c] : 0.0;
+            u[c] = hu[c] * inv;
+            v[c] = hv[c] * inv;
+            P[0 * C::PLANE + r * C::RS + c] = h[c];
+            P[1 * C::PLANE + r * C::RS + c] = hu[c];
+            P[2 * C::PLANE + r * C::RS + c] = hv[c];
+            P[3 * C::PLANE + r * C::RS + c] = inv;
+        }
+        line_sweep<NGL, false>(c_stage_D[NGL], fourth_g, half_g, h, hu, hv, u, v, Y, X, a1, a2, a3);
+    }
+    __syncthreads();
+
+    // ---- column phase: xi-column r ----------------------------------------------------------
+    const int64_t xb = (int64_t)e * C::NN + r;
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) {
+            h[k] = P[0 * C::PLANE + k * C::RS + r];
+            hu[k] = P[1 * C::PLANE + k * C::RS + r];
+            hv[k] = P[2 * C::PLANE + k * C::RS + r];
+            const double inv = P[3 * C::PLANE + k * C::RS + r];
+            u[k] = hu[k] * inv;
+            v[k] = hv[k] * inv;
+            Y[k] = Yxi[xb + k * NGL];
+            X[k] = Xxi[xb + k * NGL];
+        }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < NGL; ++c) {
+            P[0 * C::PLANE + r * C::RS + c] = a1[c];
+            P[1 * C::PLANE + r * C::RS + c] = a2[c];
+            P[2 * C::PLANE + r * C::RS + c] = a3[c];
+        }
+    }
+    __syncthreads();
+    double b1[NGL], b2[NGL], b3[NGL], rJ[NGL];
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) {
+            b1[k] = P[0 * C::PLANE + k * C::RS + r];
+            b2[k] = P[1 * C::PLANE + k * C::RS + r];
+            b3[k] = P[2 * C::PLANE + k * C::RS + r];
+        }
+        line_sweep<NGL, true>(c_stage_D[NGL], fourth_g, half_g, h, hu, hv, u, v, Y, X, b1, b2, b3);
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) {
+            rJ[k] = Jac[xb + k * NGL];
+            b1[k] = -rJ[k] * b1[k];
+            b2[k] = -rJ[k] * b2[k] - g * h[k] * Bx[xb + k * NGL];
+            b3[k] = -rJ[k] * b3[k] - g * h[k] * By[xb + k * NGL];
+        }
+        // surface lift: bottom/top sides first, then right/left (SurfaceKernel.okl).  FS holds
+        // -ms F* (ms = +-1, so the sign is exact and -ms F* rJ equals the okl's -ms rJ F* bitwise).
+        {
+            const double* F0 = FS;
+            const double* F2 = FS + 2 * 3 * NGL;
+            b1[0] += F0[r] * rJ[0];
+            b2[0] += F0[NGL + r] * rJ[0];
+            b3[0] += F0[2 * NGL + r] * rJ[0];
+            b1[NGL - 1] += F2[r] * rJ[NGL - 1];
+            b2[NGL - 1] += F2[NGL + r] * rJ[NGL - 1];
+            b3[NGL - 1] += F2[2 * NGL + r] * rJ[NGL - 1];
+        }
+        if (r == 0 || r == NGL - 1) {
+            const double* F = FS + ((r == 0) ? 3 : 1) * 3 * NGL;
+#pragma unroll
+            for (int k = 0; k < NGL; ++k) {
+                b1[k] += F[k] * rJ[k];
+                b2[k] += F[NGL + k] * rJ[k];
+                b3[k] += F[2 * NGL + k] * rJ[k];
+            }
+        }
+        // low-storage RK: gRK = A gRK + Qt; q_new = q + (C dt) gRK (q re-read: an L2 hit, and it
+        // keeps hu, hv out of the registers live across the column sweep)
+        const double cdt = rkC * dt;
+        const int64_t ob = (int64_t)e * C::NN * 3 + r;
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) {
+            const int64_t id = ob + k * NGL;
+            const double g1 = rkA * gRK[id] + b1[k];
+            const double g2 = rkA * gRK[id + C::NN] + b2[k];
+            const double g3 = rkA * gRK[id + 2 * C::NN] + b3[k];
+            gRK[id] = g1;
+            gRK[id + C::NN] = g2;
+            gRK[id + 2 * C::NN] = g3;
+            b1[k] = qi[id] + cdt * g1;
+            b2[k] = qi[id + C::NN] + cdt * g2;
+            b3[k] = qi[id + 2 * C::NN] + cdt * g3;
+        }
+    }
+    if (positivity) {
+        // PosPres.okl on the new state: element means in the okl's serial order (xi outer, eta inner)
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < NGL; ++k) {
+                P[0 * C::PLANE + k * C::RS + r] = b1[k];
+                P[1 * C::PLANE + k * C::RS + r] = b2[k];
+                P[2 * C::PLANE + k * C::RS + r] = b3[k];
+                P[3 * C::PLANE + k * C::RS + r] = Jac[xb + k * NGL];
+            }
+        }
+        __syncthreads();
+        if (act && r == 0) {
+            double hmin = 0.0, ha = 0.0, hua = 0.0, hva = 0.0;
+            for (int l = 0; l < NGL; ++l)
+                for (int k = 0; k < NGL; ++k) {
+                    const int o = k * C::RS + l;
+                    const double w = GLw[l] * GLw[k] / P[3 * C::PLANE + o];
+                    ha += P[o] * w;
+                    hua += P[C::PLANE + o] * w;
+                    hva += P[2 * C::PLANE + o] * w;
+                    hmin = fmin(hmin, P[o]);
+                }
+            const double size = EleSizes[e];
+            ha = ha / size;
+            if (fabs(ha) < zero_tol) ha = 0.0;
+            FS[0] = ha;
+            FS[1] = hua / size;
+            FS[2] = hva / size;
+            FS[3] = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
+            FS[4] = hmin;
+        }
+        __syncthreads();
+        if (act) {
+            const double ha = FS[0], hua = FS[1], hva = FS[2], th = FS[3], hmin = FS[4];
+#pragma unroll
+            for (int k = 0; k < NGL; ++k) {
+                if (hmin < 0.0) {
+                    double hn = 0.0, hun = 0.0, hvn = 0.0;
+                    if (ha > 0.0) {
+                        hn = fabs(th * (b1[k] - ha) + ha);
+                        if (hn > tol) {
+                            hun = th * (b2[k] - hua) + hua;
+                            hvn = th * (b3[k] - hva) + hva;
+                        }
+                    }
+                    if (ha < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
+                    b1[k] = hn; b2[k] = hun; b3[k] = hvn;
+                } else if (b1[k] < tol) {
+                    b2[k] = 0.0; b3[k] = 0.0;
+                }
+            }
+        }
+    }
+    double lam = 0.0;
+    if (act) {
+        const int64_t ob = (int64_t)e * C::NN * 3 + r;
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) {
+            const int64_t id = ob + k * NGL;
+            qo[id] = b1[k];
+            qo[id + C::NN] = b2[k];
+            qo[id + 2 * C::NN] = b3[k];
+        }
+        if (last) {
+            // FindLambdaMax.okl on the new state (the next step's dt)
+#pragma unroll
+            for (int k = 0; k < NGL; ++k) {
+                const double hh = b1[k];
+                const double Qinv = (hh > tol) ? 1.0 / hh : 0.0;
+                const double c = sqrt(g * hh);
+                lam = fmax(lam, c + fmax(fabs(b2[k] * Qinv), fabs(b3[k] * Qinv)));
+            }
+        }
+    }
+    if (last) {
+        for (int off = 32; off > 0; off >>= 1) lam = fmax(lam, __shfl_xor(lam, off));
+        if (lane == 0) atomicMax(lmax, (unsigned long long)__double_as_longlong(lam));
+    }
+}
+
+
+template <int NGL>
+__device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* lds, const GatherConsts& gc, int src,
+                                           double rkA, double rkC, bool last) {
+    using C = VolCfg<NGL>;
+    using SC = StageCfg<NGL>;
+    const int lane = threadIdx.x;
+    const int el = lane / NGL, r = lane % NGL;
+    const int e = e0 + el;
+    const bool act = (el < C::EPW) && (e < S.Nelem);
+    double* P = lds + (act ? el : 0) * SC::ELDS;     // planes: h, hu, hv, 1/h (later: the row sums)
+    double* FS = P + C::ESZ;                         // face fluxes of the element's sides
+    const double* qi = S.q[src];
+    double* qo = S.q[src ^ 1];
+    const double g = S.g;
+    const FluxConsts fc{g, S.geomFace, S.tol, S.ES};
+
+    // ---- fluxes of the element's four sides: lane r takes side node r of every side ----------
+    if (act) {
+#pragma unroll 1
+        for (int s = 0; s < 4; ++s) {
+            const int f = S.E2E[e * 4 + s] - 1;
+            const int rev = S.EOR[e * 4 + s];
+            const int i = rev * r + (1 - rev) * (NGL - 1 - r);      // face node of side node r
+            const int* E = S.EdgeData + 8 * (int64_t)f;
+            const int rv = E[4];
+            const int ir = (1 - rv) * (NGL - 1 - i) + rv * i;
+            const int64_t t = (int64_t)f * NGL + i, tR = (int64_t)f * NGL + ir;
+            const double nx = S.nx[t], ny = S.ny[t], sc = S.scal[t];
+            const FaceState L = gathered(qi, NGL, E, true, i, nx, ny, gc, S.bL[t], S.bR[t]);
+            const FaceState R = gathered(qi, NGL, E, false, ir, S.nx[tR], S.ny[tR], gc, S.bL[tR], S.bR[tR]);
+            double F0, F1, F2;
+            if (S.num_flux == ESDG_FLUX_ES_ROTATED)
+                flux_es_rotated(fc, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, S.bL[t], S.bR[tR], F0, F1, F2);
+            else
+                flux_es(fc, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, S.bL[t], S.bR[tR], F0, F1, F2);
+            const double nms = -(double)S.EMS[e * 4 + s];
+            FS[(s * 3 + 0) * NGL + r] = nms * (sc * S.geomFace * F0);
+            FS[(s * 3 + 1) * NGL + r] = nms * (sc * S.geomFace * F1);
+            FS[(s * 3 + 2) * NGL + r] = nms * (sc * S.geomFace * F2);
+        }
+    }
+
+    stage_core<NGL>(e, r, act, P, FS, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo, S.gRK, S.Yeta,
+                    S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax);
+}
+
+// Non-inlined entry for the mixed-degree kernel: every degree keeps its own register allocation
+// instead of one allocation for the union of all six bodies.
+template <int NGL>
+__device__ __attribute__((noinline)) void stage_body_call(const StageScn& S, int e0, double* lds, const GatherConsts& gc,
+                                                          int src, double rkA, double rkC, bool last) {
+    stage_body<NGL>(S, e0, lds, gc, src, rkA, rkC, last);
+}
+
+template <int LO, int HI>
+__global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ scns, const StageTask* __restrict__ tasks,
+                                                 GatherConsts gc, int src, double rkA, double rkC, int last) {
+    __shared__ double lds[MaxLds<LO, HI>::value];
+    const StageTask tk = tasks[blockIdx.x];
+    const StageScn& S = scns[tk.scn];
+    switch (S.ngl) {
+#define ESDG_CASE(n)                                                                                      \
+    case n:                                                                                               \
+        if constexpr (n >= LO && n <= HI) {                                                               \
+            if constexpr (LO == HI) stage_body<n>(S, tk.e0, lds, gc, src, rkA, rkC, last != 0);           \
+            else stage_body_call<n>(S, tk.e0, lds, gc, src, rkA, rkC, last != 0);                         \
+        }                                                                                                 \
+        break;
+        ESDG_CASE(3) ESDG_CASE(4) ESDG_CASE(5) ESDG_CASE(6) ESDG_CASE(7) ESDG_CASE(8)
+#undef ESDG_CASE
+        default: break;
+    }
+}
+
+// lambda_max of the state in buffer src, per scenario (FindLambdaMax.okl + the host max loop)
+__global__ void __launch_bounds__(256) k_stage_lambda(const StageScn* __restrict__ scns,
+                                                      const StageTask* __restrict__ tasks, int src) {
+    const StageTask tk = tasks[blockIdx.x];
+    const StageScn& S = scns[tk.scn];
+    const int ngl2 = S.ngl * S.ngl;
+    const int epw = 64 / S.ngl;
+    double lam = 0.0;
+    for (int t = threadIdx.x; t < epw * ngl2; t += blockDim.x) {
+        const int e = tk.e0 + t / ngl2;
+        if (e >= S.Nelem) continue;
+        const int64_t id = (int64_t)e * ngl2 * 3 + t % ngl2;
+        const double* q = S.q[src];
+        const double hh = q[id];
+        const double Qinv = (hh > S.tol) ? 1.0 / hh : 0.0;
+        const double c = sqrt(S.g * hh);
+        lam = fmax(lam, c + fmax(fabs(q[id + ngl2] * Qinv), fabs(q[id + 2 * ngl2] * Qinv)));
+    }
+    for (int off = 32; off > 0; off >>= 1) lam = fmax(lam, __shfl_xor(lam, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(S.lmax, (unsigned long long)__double_as_longlong(lam));
+}
+
+// DGtimeloop's dt (src/deviceclass.cpp:899-930) per scenario: while t < T, dt = min(T - t,
+// dti / lambda_max) and t += dt; once t >= T (or after a dt that is not positive and finite, which
+// is flagged) dt = 0 and the stages leave the state unchanged.  lambda_max is reset for the next
+// step's reduction.
+__global__ void k_stage_dt(const StageScn* __restrict__ scns, int nscn) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscn) return;
+    const StageScn& S = scns[s];
+    const double lm = __longlong_as_double((long long)*S.lmax);
+    *S.lmax = 0ull;
+    double* ck = S.clock;
+    const double t = ck[1];
+    double dt = 0.0;
+    if (t < S.T && ck[4] == 0.0) {
+        const double dt_i = S.dti / lm;
+        dt = fmin(S.T - t, dt_i);
+        if (!(dt > 0.0) || isinf(dt)) {
+            ck[4] = 1.0;
+            dt = 0.0;
+        } else {
+            ck[1] = t + dt;
+            ck[2] += 1.0;
+            ck[3] = dt;
+        }
+    }
+    ck[0] = dt;
+}
+
+// Loads each degree's D into c_stage_D once per device; a later batch must bring the same D.
+static int load_degree_operators(esdg_ctx* ctx, const std::vector<StageScn>& scns) {
+    static std::mutex mtx;
+    static std::map<int, std::map<int, std::vector<double>>> loaded;   // device -> ngl -> D
+    std::lock_guard<std::mutex> lock(mtx);
+    for (const StageScn& S : scns) {
+        const int n2 = S.ngl * S.ngl;
+        std::vector<double> D(n2);
+        int rc = esdg_memcpy(ctx, D.data(), S.D, n2 * sizeof(double), ESDG_D2H);
+        if (rc) return rc;
+        auto& slot = loaded[ctx->device][S.ngl];
+        if (slot.empty()) {
+            rc = hip_check(ctx, hipSetDevice(ctx->device), "set device");
+            if (!rc)
+                rc = hip_check(ctx, hipMemcpyToSymbol(HIP_SYMBOL(c_stage_D), D.data(), n2 * sizeof(double),
+                                                      (size_t)S.ngl * 64 * sizeof(double), hipMemcpyHostToDevice),
+                               "D to constant memory");
+            if (rc) return rc;
+            slot = D;
+        } else if (std::memcmp(slot.data(), D.data(), n2 * sizeof(double)) != 0) {
+            return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage: D differs from the one loaded for this degree");
+        }
+    }
+    return ESDG_OK;
+}
+
+int StageBatch::init(esdg_ctx* ctx, const std::vector<StageScn>& scns, int bc_kind, double hz) {
+    host = scns;
+    bc = bc_kind;
+    h_zero = hz;
+    std::vector<StageTask> tasks;
+    ngl_lo = 99;
+    ngl_hi = 0;
+    for (int s = 0; s < (int)scns.size(); ++s) {
+        const int ngl = scns[s].ngl;
+        if (ngl < 3 || ngl > 8) return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage supports 2 <= N <= 7");
+        ngl_lo = std::min(ngl_lo, ngl);
+        ngl_hi = std::max(ngl_hi, ngl);
+        const int epw = 64 / ngl;
+        for (int e0 = 0; e0 < scns[s].Nelem; e0 += epw) tasks.push_back(StageTask{s, e0});
+    }
+    if (tasks.empty()) return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage needs at least one element");
+    int rc = load_degree_operators(ctx, scns);
+    if (rc) return rc;
+    ntask = (int)tasks.size();
+    rc = esdg_malloc(ctx, (void**)&d_scn, scns.size() * sizeof(StageScn));
+    if (!rc) rc = esdg_malloc(ctx, (void**)&d_task, tasks.size() * sizeof(StageTask));
+    if (!rc) rc = esdg_memcpy(ctx, d_scn, scns.data(), scns.size() * sizeof(StageScn), ESDG_H2D);
+    if (!rc) rc = esdg_memcpy(ctx, d_task, tasks.data(), tasks.size() * sizeof(StageTask), ESDG_H2D);
+    return rc;
+}
+
+int StageBatch::lambda(esdg_ctx* ctx, int src) {
+    KernelScope ks(ctx, "stage_lambda");
+    k_stage_lambda<<<ntask, 64, 0, ctx->stream>>>(d_scn, d_task, src);
+    return launch_check(ctx, "stage_lambda");
+}
+
+template <int LO, int HI>
+static void launch_stage(esdg_ctx* ctx, const StageBatch& b, GatherConsts gc, int src, double rkA, double rkC, int last) {
+    k_stage<LO, HI><<<b.ntask, 64, 0, ctx->stream>>>(b.d_scn, b.d_task, gc, src, rkA, rkC, last);
+}
+
+int StageBatch::step(esdg_ctx* ctx, const double* A, const double* C, int stages, int& src) {
+    const int ns = (int)host.size();
+    {
+        KernelScope ks(ctx, "stage_dt");
+        k_stage_dt<<<(ns + 63) / 64, 64, 0, ctx->stream>>>(d_scn, ns);
+        int rc = launch_check(ctx, "stage_dt");
+        if (rc) return rc;
+    }
+    GatherConsts gc{bc, h_zero};
+    for (int s = 0; s < stages; ++s) {
+        KernelScope ks(ctx, "stage");
+        const int last = (s == stages - 1);
+        if (ngl_lo != ngl_hi) launch_stage<3, 8>(ctx, *this, gc, src, A[s], C[s], last);
+        else switch (ngl_lo) {
+            case 3: launch_stage<3, 3>(ctx, *this, gc, src, A[s], C[s], last); break;
+            case 4: launch_stage<4, 4>(ctx, *this, gc, src, A[s], C[s], last); break;
+            case 5: launch_stage<5, 5>(ctx, *this, gc, src, A[s], C[s], last); break;
+            case 6: launch_stage<6, 6>(ctx, *this, gc, src, A[s], C[s], last); break;
+            case 7: launch_stage<7, 7>(ctx, *this, gc, src, A[s], C[s], last); break;
+            default: launch_stage<8, 8>(ctx, *this, gc, src, A[s], C[s], last); break;
+        }
+        int rc = launch_check(ctx, "stage");
+        if (rc) return rc;
+        src ^= 1;
+    }
+    return ESDG_OK;
+}
+
+int StageBatch::run(esdg_ctx* ctx, const double* A, const double* C, int stages, int steps, bool graph, int& src) {
+    int n = 0;
+    if (graph && !ctx->profile && steps >= 2) {
+        // two steps flip the buffers an even number of times, so the graph that starts from buffer
+        // `src` ends there and can be replayed
+        const int start = src;
+        if (!exec[start]) {
+            hipGraph_t gph = nullptr;
+            int rc = hip_check(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+            if (rc) return rc;
+            int s2 = start;
+            for (int k = 0; k < 2 && rc == ESDG_OK; ++k) rc = step(ctx, A, C, stages, s2);
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &gph);
+            if (rc) {
+                if (gph) (void)hipGraphDestroy(gph);
+                return rc;
+            }
+            rc = hip_check(ctx, e, "graph capture end");
+            if (rc) return rc;
+            rc = hip_check(ctx, hipGraphInstantiate(&exec[start], gph, nullptr, nullptr, 0), "graph instantiate");
+            (void)hipGraphDestroy(gph);
+            if (rc) return rc;
+        }
+        for (; n + 2 <= steps; n += 2) {
+            int rc = hip_check(ctx, hipGraphLaunch(exec[start], ctx->stream), "graph launch");
+            if (rc) return rc;
+        }
+    }
+    for (; n < steps; ++n) {
+        int rc = step(ctx, A, C, stages, src);
+        if (rc) return rc;
+    }
+    return ESDG_OK;
+}
+
+void StageBatch::release(esdg_ctx* ctx) {
+    for (auto& x : exec)
+        if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
+    if (d_scn) esdg_free(ctx, d_scn);
+    if (d_task) esdg_free(ctx, d_task);
+    d_scn = nullptr;
+    d_task = nullptr;
+}
+
+}  // namespace esdg

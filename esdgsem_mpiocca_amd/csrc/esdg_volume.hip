@@ -35,68 +35,6 @@ struct SurfIn {
     double* gRK;
 };
 
-template <int NGL>
-struct VolCfg {
-    static constexpr int EPW = 64 / NGL;        // elements per wave
-    static constexpr int NN = NGL * NGL;
-    static constexpr int RS = NGL + 1;          // LDS row stride (padding against bank conflicts)
-    static constexpr int PLANE = NGL * RS;      // one field of one element in LDS
-    static constexpr int ESZ = 4 * PLANE;       // h, hu, hv, 1/h (the row sums reuse three planes)
-};
-
-// Symmetric two-point EC flux along one line (F# of the reference for metrics (Y, X) = (Yeta, Xeta);
-// G# = -F# evaluated with (Yxi, Xxi)).  Operands per node: h, hu, hv, u = hu/h, v = hv/h.
-struct PairOut {
-    double f1, f2, f3;
-};
-
-__device__ __forceinline__ PairOut ec_pair(double fourth_g, double half_g, double hL, double huL, double hvL,
-                                           double uL, double vL, double YL, double XL, double hR, double huR,
-                                           double hvR, double uR, double vR, double YR, double XR) {
-    const double s = hL + hR;
-    const double h_A = fourth_g * s * s;
-    const double h2_A = 0.5 * (hL * hL + hR * hR);
-    const double hu_A = 0.5 * (huL + huR);
-    const double hv_A = 0.5 * (hvL + hvR);
-    const double u_A = 0.5 * (uL + uR);
-    const double v_A = 0.5 * (vL + vR);
-    const double Y_A = 0.5 * (YL + YR);
-    const double X_A = 0.5 * (XL + XR);
-    const double pr = h_A - half_g * h2_A;
-    PairOut o;
-    o.f1 = Y_A * hu_A - X_A * hv_A;
-    o.f2 = Y_A * (hu_A * u_A + pr) - X_A * hv_A * u_A;
-    o.f3 = Y_A * (hu_A * v_A) - X_A * (hv_A * v_A + pr);
-    return o;
-}
-
-// All pairs of one line of NGL nodes; acc[k] += sgn * D(k,l) F(k,l), l ascending for every k.
-template <int NGL, bool NEG>
-__device__ __forceinline__ void line_sweep(const double* __restrict__ D, double fourth_g, double half_g,
-                                           const double (&h)[NGL], const double (&hu)[NGL], const double (&hv)[NGL],
-                                           const double (&u)[NGL], const double (&v)[NGL],
-                                           const double (&Y)[NGL], const double (&X)[NGL], double (&a1)[NGL],
-                                           double (&a2)[NGL], double (&a3)[NGL]) {
-#pragma unroll
-    for (int c = 0; c < NGL; ++c) {
-#pragma unroll
-        for (int l = c; l < NGL; ++l) {
-            const PairOut F = ec_pair(fourth_g, half_g, h[c], hu[c], hv[c], u[c], v[c], Y[c], X[c], h[l], hu[l],
-                                      hv[l], u[l], v[l], Y[l], X[l]);
-            const double dcl = NEG ? -D[c * NGL + l] : D[c * NGL + l];
-            a1[c] += dcl * F.f1;
-            a2[c] += dcl * F.f2;
-            a3[c] += dcl * F.f3;
-            if (l != c) {
-                const double dlc = NEG ? -D[l * NGL + c] : D[l * NGL + c];
-                a1[l] += dlc * F.f1;
-                a2[l] += dlc * F.f2;
-                a3[l] += dlc * F.f3;
-            }
-        }
-    }
-}
-
 template <int NGL, int MODE>
 __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, const double* __restrict__ Jac,
                                                         const double* __restrict__ Yxi, const double* __restrict__ Yeta,

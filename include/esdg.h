@@ -174,7 +174,8 @@ typedef struct esdg_run_params {
     int positivity;               /* PositivityPreserving */
     int rk_order, rk_ssp;         /* rkorder (3 or 4), rkSSP */
     int flux_differencing;        /* FluxDifferencing */
-    int fused;                    /* 0: the reference's kernel sequence; 1: fused stage kernels */
+    int fused;                    /* 0: the reference's kernel sequence; 1: fused stage kernels;
+                                     2: single-kernel stage (no artificial viscosity, 2 <= N <= 7) */
 } esdg_run_params;
 
 /* Host arrays handed over once, as deviceclass::copyDeviceVariables (deviceclass.cpp:577-626). */

@@ -19,7 +19,11 @@ def dev():
     return E.device()
 
 
-def run_gpu(dev, case, rp, steps, fused=False):
+SCHEDULES = [0, 1, 2]
+SCHEDULE_IDS = ["reference_sequence", "fused", "single_kernel"]
+
+
+def run_gpu(dev, case, rp, steps, fused=0):
     ctx = dev.Context(dev.make_params(case, rp))
     try:
         s = dev.Solver(ctx, case, rp, fused=fused)
@@ -38,7 +42,7 @@ def fields_rel_l2(case, a, b):
             for v in range(3)]
 
 
-@pytest.mark.parametrize("fused", [False, True], ids=["reference_sequence", "fused"])
+@pytest.mark.parametrize("fused", SCHEDULES, ids=SCHEDULE_IDS)
 def test_c0_bump_200_steps(dev, fused):
     """Periodic Cartesian 16x16, N=4, EC volume + ES surface, LSRK4, CFL 0.5, 200 steps."""
     case = Case.gaussian_bump(N=4, nel=16)
@@ -51,7 +55,7 @@ def test_c0_bump_200_steps(dev, fused):
     assert h <= 1e-12 and hu <= 1e-12 and hv <= 1e-12, (h, hu, hv)
 
 
-@pytest.mark.parametrize("fused", [False, True], ids=["reference_sequence", "fused"])
+@pytest.mark.parametrize("fused", SCHEDULES, ids=SCHEDULE_IDS)
 def test_c0_lake_at_rest(dev, fused):
     case = Case.gaussian_bump(N=4, nel=16, lake_at_rest=True)
     rp = RunParams(num_flux=0)
@@ -63,7 +67,7 @@ def test_c0_lake_at_rest(dev, fused):
     assert abs(total_mass(case, qg) - total_mass(case, case["q"])) <= 1e-13
 
 
-@pytest.mark.parametrize("fused", [False, True], ids=["reference_sequence", "fused"])
+@pytest.mark.parametrize("fused", [0, 1], ids=SCHEDULE_IDS[:2])
 def test_dam_break_curved_av(dev, fused):
     """Curved 12x12 N=7 dam break with modal sensor + artificial viscosity, rotated ES flux."""
     case = Case.dam_break(N=7, nel=12, noise=1e-3, seed=0)
@@ -77,7 +81,7 @@ def test_dam_break_curved_av(dev, fused):
     assert max(errs) <= 1e-12, errs
 
 
-@pytest.mark.parametrize("fused", [False, True], ids=["reference_sequence", "fused"])
+@pytest.mark.parametrize("fused", SCHEDULES, ids=SCHEDULE_IDS)
 def test_flood_positivity(dev, fused):
     """Wet/dry flood on a curved 10x10 N=4 mesh with the positivity limiter."""
     case = Case.flood(N=4, nel=10, seed=1)
@@ -110,7 +114,44 @@ def test_fused_equals_reference_sequence_bitwise(dev, av, pos):
     case = Case.dam_break(N=7, nel=8, noise=1e-3, seed=5) if not pos else Case.flood(N=5, nel=8, seed=2)
     rp = RunParams(num_flux=3, artificial_viscosity=av, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5,
                    positivity=pos, pos_pres_tol=1e-3, CFL=0.25 if pos else 0.5)
-    qa, na, ta = run_gpu(dev, case, rp, 10, fused=False)
-    qb, nb, tb = run_gpu(dev, case, rp, 10, fused=True)
+    qa, na, ta = run_gpu(dev, case, rp, 10, fused=0)
+    qb, nb, tb = run_gpu(dev, case, rp, 10, fused=1)
     assert na == nb and ta == tb
     assert np.array_equal(qa, qb) or np.abs(qa - qb).max() <= 1e-15 * np.abs(qa).max()
+
+
+@pytest.mark.parametrize("pos", [0, 1])
+def test_single_kernel_equals_fused_bitwise(dev, pos):
+    """The single-kernel stage (device dt, ping-pong state, graph replay) performs the fused
+    schedule's arithmetic: identical state and time after 11 steps (odd: graph pairs + one eager)."""
+    case = Case.dam_break(N=7, nel=8, noise=1e-3, seed=5) if not pos else Case.flood(N=5, nel=8, seed=2)
+    rp = RunParams(num_flux=3, positivity=pos, pos_pres_tol=1e-3, CFL=0.25 if pos else 0.5)
+    qa, na, ta = run_gpu(dev, case, rp, 11, fused=1)
+    qb, nb, tb = run_gpu(dev, case, rp, 11, fused=2)
+    assert na == nb == 11 and ta == tb
+    assert np.array_equal(qa, qb)
+
+
+def test_single_kernel_stops_at_T_and_resumes(dev):
+    """Device-side clock: steps issued after t reaches T change nothing and are not counted; a
+    second run() continues from the first one's state and time."""
+    case = Case.gaussian_bump(N=4, nel=8)
+    rp_free = RunParams(num_flux=0)
+    _, _, t9 = run_gpu(dev, case, rp_free, 9, fused=1)
+    _, _, t10 = run_gpu(dev, case, rp_free, 10, fused=1)
+    rp = RunParams(num_flux=0, T=0.5 * (t9 + t10))      # the 10th step is shortened to end at T
+    qo, no, to, _ = O.timeloop(case, rp, 100)
+    assert 10 <= no <= 11
+    ctx = dev.Context(dev.make_params(case, rp))
+    try:
+        s = dev.Solver(ctx, case, rp, fused=2)
+        n1, t1 = s.run(7)
+        n2, t2 = s.run(100)
+        q = s.get_q()
+        s.close()
+    finally:
+        ctx.close()
+    assert n1 == 7 and n1 + n2 == no
+    assert t1 < rp.T and t2 == to
+    h, hu, hv = fields_rel_l2(case, q, qo)
+    assert max(h, hu, hv) <= 1e-12, (h, hu, hv)
