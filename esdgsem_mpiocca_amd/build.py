@@ -38,6 +38,7 @@ HIP_SOURCES = [
     "esdg_volume.hip",
     "esdg_fused.hip",
     "esdg_stage.hip",
+    "esdg_ensemble.hip",
     "esdg_deviceclass.hip",
 ]
 

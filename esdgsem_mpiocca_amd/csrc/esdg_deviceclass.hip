@@ -251,7 +251,7 @@ class deviceclass {
         if (!stage_ready) {
             StageScn S{};
             S.ngl = ngl; S.Nelem = Nelem; S.Nfaces = Nfaces; S.ES = ctx_->p.ES; S.num_flux = ctx_->p.num_flux;
-            S.positivity = rp_.positivity;
+            S.positivity = rp_.positivity; S.bc = ctx_->p.bc_kind; S.h_zero = ctx_->p.h_zero;
             S.g = ctx_->p.g; S.geomFace = ctx_->p.geomFace; S.tol = ctx_->p.PosPresTOL; S.zero_tol = ctx_->p.ZeroTOL;
             S.dti = rp_.min_ele_size / (ngl) * rp_.CFL;
             S.T = rp_.T;
@@ -260,7 +260,7 @@ class deviceclass {
             S.EleSizes = o_EleSizes; S.GLw = o_GLw;
             S.EdgeData = o_EdgeData; S.E2E = o_ElemToEdge; S.EMS = o_ElemEdgeMasterSlave; S.EOR = o_ElemEdgeOrientation;
             S.q[0] = o_q; S.q[1] = o_q2; S.gRK = o_gRK; S.clock = o_clock; S.lmax = o_lmax;
-            rc = stage_.init(ctx_, std::vector<StageScn>{S}, ctx_->p.bc_kind, ctx_->p.h_zero);
+            rc = stage_.init(ctx_, std::vector<StageScn>{S});
             if (rc) return rc;
             stage_ready = true;
         }
@@ -273,7 +273,7 @@ class deviceclass {
         int issued = 0;
         while (!rc && issued < max_steps && ck[1] < rp_.T && ck[4] == 0.0) {
             const int chunk = std::min(max_steps - issued, 64);
-            rc = stage_.run(ctx_, A, Cc, stages, chunk, true, src);
+            rc = stage_.run(ctx_, A, Cc, stages, chunk, src);
             if (!rc) rc = esdg_memcpy(ctx_, ck, o_clock, sizeof ck, ESDG_D2H);
             issued += chunk;
         }

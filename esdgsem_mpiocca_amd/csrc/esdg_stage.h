@@ -18,8 +18,8 @@
 namespace esdg {
 
 struct StageScn {
-    int ngl, Nelem, Nfaces, ES, num_flux, positivity;
-    double g, geomFace, tol, zero_tol;
+    int ngl, Nelem, Nfaces, ES, num_flux, positivity, bc;
+    double g, geomFace, tol, zero_tol, h_zero;
     double dti;         // (min_ele_size / ngl) * CFL            (dt_i = dti / lambda_max)
     double T;
     const double *Jac, *Yxi, *Yeta, *Xxi, *Xeta, *Bx, *By, *D, *nx, *ny, *scal, *bL, *bR, *EleSizes, *GLw;
@@ -32,29 +32,46 @@ struct StageScn {
 };
 
 struct StageTask {
-    int scn, e0;
+    int scn, e0, ngl, pad;
+};
+
+// How the tasks of a stage are ordered and launched (the autotuner's search space).
+struct StageSchedule {
+    int order = 0;          // 0 scenario by scenario; 1 tiles of geometry-sharing scenarios
+                            // interleaved (tile-major); 2 degree descending, then scenario by scenario
+    int blocks_per_cu = 0;  // 0: one block per task; k > 0: persistent grid of k blocks per CU
+    int per_degree = 0;     // 1: one launch per degree group instead of one mixed-degree launch
+    int xcd = 0;            // 1: each XCD takes a contiguous range of the task list
+    int graph = 1;          // replay pairs of steps from a captured hipGraph
 };
 
 // Host side: device copies of the scenario table and the task list, plus the per-step sequence.
 class StageBatch {
   public:
-    int init(esdg_ctx* ctx, const std::vector<StageScn>& scns, int bc_kind, double h_zero);
+    int init(esdg_ctx* ctx, const std::vector<StageScn>& scns);
+    int set_schedule(esdg_ctx* ctx, const StageSchedule& sc);
     // initial lambda_max of every scenario from the state in buffer `src`
     int lambda(esdg_ctx* ctx, int src);
     // one time step: dt kernel, then the RK stages; src flips once per stage
     int step(esdg_ctx* ctx, const double* A, const double* C, int stages, int& src);
-    // `steps` steps: pairs of steps replayed from a captured graph (graph = true), the rest eager
-    int run(esdg_ctx* ctx, const double* A, const double* C, int stages, int steps, bool graph, int& src);
+    // `steps` steps: pairs of steps replayed from a captured graph (schedule.graph), the rest eager
+    int run(esdg_ctx* ctx, const double* A, const double* C, int stages, int steps, int& src);
     void release(esdg_ctx* ctx);
 
+    struct Range {
+        int beg, n, lo, hi, grid, persistent;
+    };
     std::vector<StageScn> host;
+    StageSchedule sched;
+    std::vector<Range> ranges;
     StageScn* d_scn = nullptr;
     StageTask* d_task = nullptr;
     int ntask = 0;
-    int ngl_lo = 0, ngl_hi = 0;   // degree range of the batch (one instantiation when equal)
-    int bc = 0;
-    double h_zero = 0.0;
+    int ncu = 0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};   // graph of two steps, one per starting parity
+
+  private:
+    void drop_graphs();
 };
 
 }  // namespace esdg

@@ -254,8 +254,8 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
 
 
 template <int NGL>
-__device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* lds, const GatherConsts& gc, int src,
-                                           double rkA, double rkC, bool last) {
+__device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* lds, int src, double rkA, double rkC,
+                                           bool last) {
     using C = VolCfg<NGL>;
     using SC = StageCfg<NGL>;
     const int lane = threadIdx.x;
@@ -268,6 +268,7 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
     double* qo = S.q[src ^ 1];
     const double g = S.g;
     const FluxConsts fc{g, S.geomFace, S.tol, S.ES};
+    const GatherConsts gc{S.bc, S.h_zero};
 
     // ---- fluxes of the element's four sides: lane r takes side node r of every side ----------
     if (act) {
@@ -302,28 +303,54 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
 // Non-inlined entry for the mixed-degree kernel: every degree keeps its own register allocation
 // instead of one allocation for the union of all six bodies.
 template <int NGL>
-__device__ __attribute__((noinline)) void stage_body_call(const StageScn& S, int e0, double* lds, const GatherConsts& gc,
-                                                          int src, double rkA, double rkC, bool last) {
-    stage_body<NGL>(S, e0, lds, gc, src, rkA, rkC, last);
+__device__ __attribute__((noinline)) void stage_body_call(const StageScn& S, int e0, double* lds, int src, double rkA,
+                                                          double rkC, bool last) {
+    stage_body<NGL>(S, e0, lds, src, rkA, rkC, last);
 }
 
 template <int LO, int HI>
-__global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ scns, const StageTask* __restrict__ tasks,
-                                                 GatherConsts gc, int src, double rkA, double rkC, int last) {
-    __shared__ double lds[MaxLds<LO, HI>::value];
-    const StageTask tk = tasks[blockIdx.x];
+__device__ __forceinline__ void stage_task(const StageScn* __restrict__ scns, const StageTask& tk, double* lds, int src,
+                                           double rkA, double rkC, bool last) {
     const StageScn& S = scns[tk.scn];
-    switch (S.ngl) {
-#define ESDG_CASE(n)                                                                                      \
-    case n:                                                                                               \
-        if constexpr (n >= LO && n <= HI) {                                                               \
-            if constexpr (LO == HI) stage_body<n>(S, tk.e0, lds, gc, src, rkA, rkC, last != 0);           \
-            else stage_body_call<n>(S, tk.e0, lds, gc, src, rkA, rkC, last != 0);                         \
-        }                                                                                                 \
+    switch (tk.ngl) {
+#define ESDG_CASE(n)                                                                \
+    case n:                                                                         \
+        if constexpr (n >= LO && n <= HI) {                                         \
+            if constexpr (LO == HI) stage_body<n>(S, tk.e0, lds, src, rkA, rkC, last);      \
+            else stage_body_call<n>(S, tk.e0, lds, src, rkA, rkC, last);                   \
+        }                                                                           \
         break;
         ESDG_CASE(3) ESDG_CASE(4) ESDG_CASE(5) ESDG_CASE(6) ESDG_CASE(7) ESDG_CASE(8)
 #undef ESDG_CASE
         default: break;
+    }
+}
+
+// One RK stage over tasks[0, n).  Non-persistent: block b takes one task.  Persistent: a grid of
+// gridDim.x blocks loops over the tasks.  With xcd = 1 the task list is cut into 8 contiguous
+// parts and the blocks that the dispatcher places on XCD x (b mod 8 = x) take part x, so tasks
+// that are adjacent in the list (one element tile of several scenarios that share geometry)
+// meet in the same L2.  Every block leaves after a bounded number of tasks.
+template <int LO, int HI, bool PERSIST>
+__global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ scns, const StageTask* __restrict__ tasks,
+                                                 int n, int xcd, int src, double rkA, double rkC, int last) {
+    __shared__ double lds[MaxLds<LO, HI>::value];
+    const int b = blockIdx.x;
+    int beg = 0, cnt = n, i0 = b, stride = gridDim.x;
+    if (xcd) {
+        const int q = n >> 3, rr = n & 7, x = b & 7;
+        cnt = q + (x < rr ? 1 : 0);
+        beg = x * q + (x < rr ? x : rr);
+        i0 = b >> 3;
+        stride = gridDim.x >> 3;
+    }
+    if constexpr (!PERSIST) {
+        if (i0 < cnt) stage_task<LO, HI>(scns, tasks[beg + i0], lds, src, rkA, rkC, last != 0);
+    } else {
+        for (int i = i0; i < cnt; i += stride) {
+            if (i != i0) __syncthreads();   // LDS of the previous task is free
+            stage_task<LO, HI>(scns, tasks[beg + i], lds, src, rkA, rkC, last != 0);
+        }
     }
 }
 
@@ -332,8 +359,8 @@ __global__ void __launch_bounds__(256) k_stage_lambda(const StageScn* __restrict
                                                       const StageTask* __restrict__ tasks, int src) {
     const StageTask tk = tasks[blockIdx.x];
     const StageScn& S = scns[tk.scn];
-    const int ngl2 = S.ngl * S.ngl;
-    const int epw = 64 / S.ngl;
+    const int ngl2 = tk.ngl * tk.ngl;
+    const int epw = 64 / tk.ngl;
     double lam = 0.0;
     for (int t = threadIdx.x; t < epw * ngl2; t += blockDim.x) {
         const int e = tk.e0 + t / ngl2;
@@ -403,30 +430,101 @@ static int load_degree_operators(esdg_ctx* ctx, const std::vector<StageScn>& scn
     return ESDG_OK;
 }
 
-int StageBatch::init(esdg_ctx* ctx, const std::vector<StageScn>& scns, int bc_kind, double hz) {
+int StageBatch::init(esdg_ctx* ctx, const std::vector<StageScn>& scns) {
     host = scns;
-    bc = bc_kind;
-    h_zero = hz;
-    std::vector<StageTask> tasks;
-    ngl_lo = 99;
-    ngl_hi = 0;
-    for (int s = 0; s < (int)scns.size(); ++s) {
-        const int ngl = scns[s].ngl;
-        if (ngl < 3 || ngl > 8) return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage supports 2 <= N <= 7");
-        ngl_lo = std::min(ngl_lo, ngl);
-        ngl_hi = std::max(ngl_hi, ngl);
-        const int epw = 64 / ngl;
-        for (int e0 = 0; e0 < scns[s].Nelem; e0 += epw) tasks.push_back(StageTask{s, e0});
+    if (scns.empty()) return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage needs at least one scenario");
+    for (const StageScn& S : scns) {
+        if (S.ngl < 3 || S.ngl > 8) return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage supports 2 <= N <= 7");
+        if (S.Nelem <= 0) return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage needs at least one element");
     }
-    if (tasks.empty()) return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage needs at least one element");
     int rc = load_degree_operators(ctx, scns);
     if (rc) return rc;
-    ntask = (int)tasks.size();
+    rc = hip_check(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
+    if (rc) return rc;
     rc = esdg_malloc(ctx, (void**)&d_scn, scns.size() * sizeof(StageScn));
-    if (!rc) rc = esdg_malloc(ctx, (void**)&d_task, tasks.size() * sizeof(StageTask));
     if (!rc) rc = esdg_memcpy(ctx, d_scn, scns.data(), scns.size() * sizeof(StageScn), ESDG_H2D);
-    if (!rc) rc = esdg_memcpy(ctx, d_task, tasks.data(), tasks.size() * sizeof(StageTask), ESDG_H2D);
+    if (!rc) rc = set_schedule(ctx, StageSchedule{});
     return rc;
+}
+
+void StageBatch::drop_graphs() {
+    for (auto& x : exec)
+        if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
+}
+
+int StageBatch::set_schedule(esdg_ctx* ctx, const StageSchedule& sc) {
+    if (sc.order < 0 || sc.order > 2 || sc.blocks_per_cu < 0 || sc.blocks_per_cu > 64)
+        return set_error(ctx, ESDG_ERR_ARG, "stage schedule: order must be 0..2 and blocks_per_cu 0..64");
+    const int ns = (int)host.size();
+    // scenario visiting order
+    std::vector<int> scn(ns);
+    for (int s = 0; s < ns; ++s) scn[s] = s;
+    if (sc.order == 2 || sc.per_degree)
+        std::stable_sort(scn.begin(), scn.end(), [&](int a, int b) { return host[a].ngl > host[b].ngl; });
+    std::vector<StageTask> tasks;
+    auto tiles = [&](int s) { const int epw = 64 / host[s].ngl; return (host[s].Nelem + epw - 1) / epw; };
+    if (sc.order == 1) {
+        // groups of scenarios on the same geometry, in order of first appearance; tile-major inside
+        std::vector<int> done(ns, 0);
+        for (int a = 0; a < ns; ++a) {
+            const int s0 = scn[a];
+            if (done[s0]) continue;
+            std::vector<int> grp;
+            for (int b = a; b < ns; ++b) {
+                const int s1 = scn[b];
+                if (!done[s1] && host[s1].Jac == host[s0].Jac && host[s1].ngl == host[s0].ngl &&
+                    host[s1].Nelem == host[s0].Nelem) {
+                    grp.push_back(s1);
+                    done[s1] = 1;
+                }
+            }
+            const int epw = 64 / host[s0].ngl;
+            for (int t = 0; t < tiles(s0); ++t)
+                for (int s1 : grp) tasks.push_back(StageTask{s1, t * epw, host[s1].ngl, 0});
+        }
+    } else {
+        for (int s : scn) {
+            const int epw = 64 / host[s].ngl;
+            for (int t = 0; t < tiles(s); ++t) tasks.push_back(StageTask{s, t * epw, host[s].ngl, 0});
+        }
+    }
+    // launch ranges
+    std::vector<Range> rg;
+    auto add_range = [&](int beg, int n, int lo, int hi) {
+        Range r{beg, n, lo, hi, n, 0};
+        const int pgrid = sc.blocks_per_cu * ncu;
+        if (sc.blocks_per_cu > 0 && pgrid < n && (!sc.xcd || pgrid >= 8)) {
+            r.grid = sc.xcd ? (pgrid / 8) * 8 : pgrid;
+            r.persistent = 1;
+        }
+        rg.push_back(r);
+    };
+    if (sc.per_degree) {
+        int beg = 0;
+        while (beg < (int)tasks.size()) {
+            int end = beg;
+            while (end < (int)tasks.size() && tasks[end].ngl == tasks[beg].ngl) ++end;
+            add_range(beg, end - beg, tasks[beg].ngl, tasks[beg].ngl);
+            beg = end;
+        }
+    } else {
+        int lo = 99, hi = 0;
+        for (const StageTask& t : tasks) { lo = std::min(lo, t.ngl); hi = std::max(hi, t.ngl); }
+        add_range(0, (int)tasks.size(), lo, hi);
+    }
+    if (d_task && (int)tasks.size() != ntask) {
+        esdg_free(ctx, d_task);
+        d_task = nullptr;
+    }
+    int rc = ESDG_OK;
+    if (!d_task) rc = esdg_malloc(ctx, (void**)&d_task, tasks.size() * sizeof(StageTask));
+    if (!rc) rc = esdg_memcpy(ctx, d_task, tasks.data(), tasks.size() * sizeof(StageTask), ESDG_H2D);
+    if (rc) return rc;
+    ntask = (int)tasks.size();
+    ranges = rg;
+    sched = sc;
+    drop_graphs();
+    return ESDG_OK;
 }
 
 int StageBatch::lambda(esdg_ctx* ctx, int src) {
@@ -436,8 +534,14 @@ int StageBatch::lambda(esdg_ctx* ctx, int src) {
 }
 
 template <int LO, int HI>
-static void launch_stage(esdg_ctx* ctx, const StageBatch& b, GatherConsts gc, int src, double rkA, double rkC, int last) {
-    k_stage<LO, HI><<<b.ntask, 64, 0, ctx->stream>>>(b.d_scn, b.d_task, gc, src, rkA, rkC, last);
+static void launch_stage(esdg_ctx* ctx, const StageBatch& b, const StageBatch::Range& r, int src, double rkA,
+                         double rkC, int last) {
+    if (r.persistent)
+        k_stage<LO, HI, true><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, src, rkA, rkC,
+                                                               last);
+    else
+        k_stage<LO, HI, false><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, src, rkA, rkC,
+                                                                last);
 }
 
 int StageBatch::step(esdg_ctx* ctx, const double* A, const double* C, int stages, int& src) {
@@ -448,29 +552,30 @@ int StageBatch::step(esdg_ctx* ctx, const double* A, const double* C, int stages
         int rc = launch_check(ctx, "stage_dt");
         if (rc) return rc;
     }
-    GatherConsts gc{bc, h_zero};
     for (int s = 0; s < stages; ++s) {
-        KernelScope ks(ctx, "stage");
         const int last = (s == stages - 1);
-        if (ngl_lo != ngl_hi) launch_stage<3, 8>(ctx, *this, gc, src, A[s], C[s], last);
-        else switch (ngl_lo) {
-            case 3: launch_stage<3, 3>(ctx, *this, gc, src, A[s], C[s], last); break;
-            case 4: launch_stage<4, 4>(ctx, *this, gc, src, A[s], C[s], last); break;
-            case 5: launch_stage<5, 5>(ctx, *this, gc, src, A[s], C[s], last); break;
-            case 6: launch_stage<6, 6>(ctx, *this, gc, src, A[s], C[s], last); break;
-            case 7: launch_stage<7, 7>(ctx, *this, gc, src, A[s], C[s], last); break;
-            default: launch_stage<8, 8>(ctx, *this, gc, src, A[s], C[s], last); break;
+        for (const Range& r : ranges) {
+            KernelScope ks(ctx, "stage");
+            if (r.lo != r.hi) launch_stage<3, 8>(ctx, *this, r, src, A[s], C[s], last);
+            else switch (r.lo) {
+                case 3: launch_stage<3, 3>(ctx, *this, r, src, A[s], C[s], last); break;
+                case 4: launch_stage<4, 4>(ctx, *this, r, src, A[s], C[s], last); break;
+                case 5: launch_stage<5, 5>(ctx, *this, r, src, A[s], C[s], last); break;
+                case 6: launch_stage<6, 6>(ctx, *this, r, src, A[s], C[s], last); break;
+                case 7: launch_stage<7, 7>(ctx, *this, r, src, A[s], C[s], last); break;
+                default: launch_stage<8, 8>(ctx, *this, r, src, A[s], C[s], last); break;
+            }
+            int rc = launch_check(ctx, "stage");
+            if (rc) return rc;
         }
-        int rc = launch_check(ctx, "stage");
-        if (rc) return rc;
         src ^= 1;
     }
     return ESDG_OK;
 }
 
-int StageBatch::run(esdg_ctx* ctx, const double* A, const double* C, int stages, int steps, bool graph, int& src) {
+int StageBatch::run(esdg_ctx* ctx, const double* A, const double* C, int stages, int steps, int& src) {
     int n = 0;
-    if (graph && !ctx->profile && steps >= 2) {
+    if (sched.graph && !ctx->profile && steps >= 2) {
         // two steps flip the buffers an even number of times, so the graph that starts from buffer
         // `src` ends there and can be replayed
         const int start = src;
@@ -504,8 +609,7 @@ int StageBatch::run(esdg_ctx* ctx, const double* A, const double* C, int stages,
 }
 
 void StageBatch::release(esdg_ctx* ctx) {
-    for (auto& x : exec)
-        if (x) { (void)hipGraphExecDestroy(x); x = nullptr; }
+    drop_graphs();
     if (d_scn) esdg_free(ctx, d_scn);
     if (d_task) esdg_free(ctx, d_task);
     d_scn = nullptr;

@@ -85,9 +85,9 @@ def lib():
     return _LIB
 
 
-def make_params(case: Case, rp: RunParams, bc_kind: int = 0, h_zero: float = 0.0) -> Params:
-    return Params(case.ngl, rp.ES, rp.num_flux, bc_kind, rp.g, case.geom_face, rp.pos_pres_tol,
-                  rp.zero_tol, rp.eps0, rp.sigma_min, rp.sigma_max, h_zero)
+def make_params(case: Case, rp: RunParams) -> Params:
+    return Params(case.ngl, rp.ES, rp.num_flux, rp.bc_kind, rp.g, case.geom_face, rp.pos_pres_tol,
+                  rp.zero_tol, rp.eps0, rp.sigma_min, rp.sigma_max, rp.h_zero)
 
 
 class DeviceArray:
@@ -192,35 +192,49 @@ class Context:
             self.handle = None
 
 
-class Solver:
-    """deviceclass mirror: buffers + DGtimeloop on the device for one Case."""
+def make_inputs(case: Case, rp: RunParams, q=None):
+    """esdg_inputs of a Case (host pointers into the Case's own arrays, so two calls on the same
+    Case hand over the same pointers) and the list of arrays that must outlive the call."""
+    a = case.arrays
+    src = {
+        "GLw": a["w_GL"], "nx": a["nx"], "ny": a["ny"], "scal": a["scal"], "Yxi": a["yXi"],
+        "Yeta": a["yEta"], "Xxi": a["xXi"], "Xeta": a["xEta"], "B": a["b"], "Bx": a["Bx"],
+        "By": a["By"], "D": a["D"], "Dstrong": a["Dstrong"], "Dhat": a["Dhat"],
+        "Jac": a["Jac"], "x": a["x"], "y": a["y"], "q": a["q"] if q is None else q,
+        "EleSizes": a["EleSizes"], "VdmInv": a["VdmInv"],
+        "ElemEdgeMasterSlave": a["ElemEdgeMasterSlave"],
+        "ElemEdgeOrientation": a["ElemEdgeOrientation"], "ElemToEdge": a["ElemToEdge"],
+        "EdgeData": a["EdgeData"], "EdgeReversed": a["EdgeReversed"]}
+    if not rp.flux_differencing:
+        # FluxDifferencing = 0 builds D without the factor 2 (src/basis.cpp:108-119)
+        src["D"] = a["Dstrong"]
+    keep = []
+    inp = Inputs()
+    inp.Nelem, inp.Nfaces = case.Nelem, case.Nfaces
+    for n in _INPUT_DOUBLES:
+        arr = np.ascontiguousarray(src[n], dtype=np.float64)
+        keep.append(arr)
+        setattr(inp, n, arr.ctypes.data_as(C.POINTER(C.c_double)))
+    for n in _INPUT_INTS:
+        arr = np.ascontiguousarray(src[n], dtype=np.int32)
+        keep.append(arr)
+        setattr(inp, n, arr.ctypes.data_as(C.POINTER(C.c_int)))
+    return inp, keep
 
-    def __init__(self, ctx: Context, case: Case, rp: RunParams, fused: bool = False):
+
+def make_run_params(case: Case, rp: RunParams, fused: int = 0) -> RunParamsC:
+    return RunParamsC(rp.CFL, rp.DFL, rp.T, case.min_ele_size, rp.artificial_viscosity,
+                      rp.positivity, rp.rk_order, rp.rk_ssp, rp.flux_differencing, int(fused))
+
+
+class Solver:
+    """deviceclass mirror: buffers + DGtimeloop on the device for one Case.
+    fused: 0 the reference's kernel sequence, 1 fused stage kernels, 2 single-kernel stage."""
+
+    def __init__(self, ctx: Context, case: Case, rp: RunParams, fused: int = 0):
         self.ctx, self.case, self.rp = ctx, case, rp
-        a = case.arrays
-        self._keep = {
-            "GLw": a["w_GL"], "nx": a["nx"], "ny": a["ny"], "scal": a["scal"], "Yxi": a["yXi"],
-            "Yeta": a["yEta"], "Xxi": a["xXi"], "Xeta": a["xEta"], "B": a["b"], "Bx": a["Bx"],
-            "By": a["By"], "D": a["D"], "Dstrong": a["Dstrong"], "Dhat": a["Dhat"],
-            "Jac": a["Jac"], "x": a["x"], "y": a["y"], "q": a["q"], "EleSizes": a["EleSizes"],
-            "VdmInv": a["VdmInv"], "ElemEdgeMasterSlave": a["ElemEdgeMasterSlave"],
-            "ElemEdgeOrientation": a["ElemEdgeOrientation"], "ElemToEdge": a["ElemToEdge"],
-            "EdgeData": a["EdgeData"], "EdgeReversed": a["EdgeReversed"]}
-        if not rp.flux_differencing:
-            # FluxDifferencing = 0 builds D without the factor 2 (src/basis.cpp:108-119)
-            self._keep["D"] = a["Dstrong"]
-        inp = Inputs()
-        inp.Nelem, inp.Nfaces = case.Nelem, case.Nfaces
-        for n in _INPUT_DOUBLES:
-            arr = np.ascontiguousarray(self._keep[n], dtype=np.float64)
-            self._keep[n] = arr
-            setattr(inp, n, arr.ctypes.data_as(C.POINTER(C.c_double)))
-        for n in _INPUT_INTS:
-            arr = np.ascontiguousarray(self._keep[n], dtype=np.int32)
-            self._keep[n] = arr
-            setattr(inp, n, arr.ctypes.data_as(C.POINTER(C.c_int)))
-        rpc = RunParamsC(rp.CFL, rp.DFL, rp.T, case.min_ele_size, rp.artificial_viscosity,
-                         rp.positivity, rp.rk_order, rp.rk_ssp, rp.flux_differencing, int(fused))
+        inp, self._keep = make_inputs(case, rp)
+        rpc = make_run_params(case, rp, fused)
         h = C.c_void_p()
         ctx.check(lib().esdg_solver_create(C.c_void_p(ctx.handle), C.byref(inp), C.byref(rpc),
                                            C.byref(h)))
@@ -256,4 +270,84 @@ class Solver:
     def close(self):
         if self.handle:
             lib().esdg_solver_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+
+class EnsembleSchedule(C.Structure):
+    _fields_ = [("order", C.c_int), ("blocks_per_cu", C.c_int), ("per_degree", C.c_int),
+                ("xcd", C.c_int), ("graph", C.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Ensemble:
+    """esdg_ensemble (include/esdg_ensemble.h): independent scenarios advanced together.
+
+    members: list of (case, rp, q) with q None for the case's own initial state.  Members that
+    share a Case share its mesh arrays on the device."""
+
+    def __init__(self, ctx: Context, members):
+        self.ctx = ctx
+        n = len(members)
+        self.members = members
+        P = (Params * n)()
+        I = (Inputs * n)()
+        R = (RunParamsC * n)()
+        self._keep = []
+        for i, (case, rp, q) in enumerate(members):
+            P[i] = make_params(case, rp)
+            inp, keep = make_inputs(case, rp, q)
+            I[i] = inp
+            self._keep.append(keep)
+            R[i] = make_run_params(case, rp)
+        h = C.c_void_p()
+        ctx.check(lib().esdg_ensemble_create(C.c_void_p(ctx.handle), n, P, I, R, C.byref(h)))
+        self.handle = h.value
+        self._keep = None      # the library holds device copies now
+
+    def set_schedule(self, **kw):
+        sc = self.schedule()
+        for k, v in kw.items():
+            setattr(sc, k, int(v))
+        self.ctx.check(lib().esdg_ensemble_set_schedule(C.c_void_p(self.handle), C.byref(sc)))
+
+    def schedule(self) -> EnsembleSchedule:
+        sc = EnsembleSchedule()
+        self.ctx.check(lib().esdg_ensemble_get_schedule(C.c_void_p(self.handle), C.byref(sc)))
+        return sc
+
+    def autotune(self, trial_steps=2):
+        sc, ms = EnsembleSchedule(), C.c_double()
+        self.ctx.check(lib().esdg_ensemble_autotune(C.c_void_p(self.handle), int(trial_steps),
+                                                    C.byref(sc), C.byref(ms)))
+        return sc.as_dict(), ms.value
+
+    def run(self, steps: int):
+        self.ctx.check(lib().esdg_ensemble_run(C.c_void_p(self.handle), int(steps)))
+
+    def run_async(self, steps: int):
+        self.ctx.check(lib().esdg_ensemble_run_async(C.c_void_p(self.handle), int(steps)))
+
+    def get_q(self, i: int) -> np.ndarray:
+        case = self.members[i][0]
+        out = np.empty(case.Nelem * case.ngl ** 2 * 3)
+        self.ctx.check(lib().esdg_ensemble_get_q(C.c_void_p(self.handle), int(i),
+                                                 out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def status(self, i: int):
+        t, n, dt = C.c_double(), C.c_int(), C.c_double()
+        self.ctx.check(lib().esdg_ensemble_status(C.c_void_p(self.handle), int(i), C.byref(t),
+                                                  C.byref(n), C.byref(dt)))
+        return t.value, n.value, dt.value
+
+    def nbytes(self) -> int:
+        b = C.c_size_t()
+        self.ctx.check(lib().esdg_ensemble_bytes(C.c_void_p(self.handle), C.byref(b)))
+        return b.value
+
+    def close(self):
+        if self.handle:
+            lib().esdg_ensemble_destroy(C.c_void_p(self.handle))
             self.handle = None

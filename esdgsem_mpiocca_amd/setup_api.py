@@ -88,6 +88,8 @@ class RunParams:
     rk_order: int = 4
     rk_ssp: int = 0
     zero_tol: float = 1.0e-5    # hard-coded in the reference (src/deviceclass.cpp:148)
+    bc_kind: int = 0            # 0 SolidWalls, 1 InnerOuter, 2 OceanBoundary (deviceclass.cpp:363-415)
+    h_zero: float = 0.0         # still-water level of the ocean boundary (define h_zero)
 
 
 class Case:
@@ -144,6 +146,24 @@ class Case:
         return cls(CASE_FLOOD, MESH_CURVED_SINE, N, nel, nel, 0.0, 10.0, 0.0, 10.0, 0, 0,
                    perturb_amp=perturb, perturb_len=5.0, p=(level_left, x_dam, level_right, 0),
                    seed=seed)
+
+    def with_state(self, q):
+        """A copy of this case with another initial state q (mesh arrays shared, not copied)."""
+        import copy
+        c = copy.copy(self)
+        c.arrays = dict(self.arrays)
+        c.arrays["q"] = np.ascontiguousarray(q, dtype=np.float64).reshape(self.arrays["q"].shape)
+        return c
+
+    @classmethod
+    def ocean_uplift(cls, N=4, nel=64, amp=1.0, axis_a=50e3, axis_c=30e3, angle=0.0, length=800e3):
+        """nel^2 Cartesian quads on [0, length]^2: 4000 m ocean with a linear shelf to 50 m over the
+        last 20% of x, still-water level 4000 (b = 4000 - depth), elliptic Gaussian sea-surface
+        uplift of amplitude amp, axes axis_a, axis_c (m) rotated by angle, centred at (40%, 50%).
+        Run it with RunParams(bc_kind=2, h_zero=4000): open boundary over the deep ocean (b <= 6.7),
+        walls over the shelf."""
+        return cls(CASE_OCEAN_UPLIFT, MESH_CARTESIAN, N, nel, nel, 0.0, length, 0.0, length, 0, 0,
+                   amp=amp, p=(axis_a, axis_c, angle, 0))
 
     @classmethod
     def constant(cls, N, nel, h=1.0, hu=0.0, hv=0.0, curved=True, perturb=0.05):

@@ -1,0 +1,67 @@
+/*
+ * esdg_ensemble.h -- batched ensembles of independent shallow-water scenarios on one GPU.
+ *
+ * No counterpart in the reference, which runs one scenario per process (main.cpp:925-1051 drives
+ * one deviceclass).  Each scenario is what one deviceclass::DGtimeloop would advance
+ * (src/deviceclass.cpp:868-1130), with its own mesh size, degree, constants, boundary variant,
+ * CFL and end time; every scenario's result equals that of a single esdg_solver with fused = 2
+ * (and hence of the reference kernel sequence) bit for bit.  All scenarios advance together: one
+ * launch per RK stage covers every element of every scenario, the dt of each scenario is computed
+ * on the device, and pairs of steps are replayed from a captured hipGraph.
+ *
+ * Host arrays are uploaded once per distinct (pointer, size): scenarios that pass the same mesh
+ * and bathymetry arrays share one device copy.
+ *
+ * Supported per scenario: 2 <= N <= 7 (3 <= ngl <= 8), flux differencing, ES or rotated ES flux,
+ * walls / inner-outer / ocean boundaries, positivity limiter on or off, no artificial viscosity,
+ * low-storage RK of order 3 or 4 (the same order for the whole ensemble).
+ */
+#ifndef ESDG_ENSEMBLE_H
+#define ESDG_ENSEMBLE_H
+
+#include "esdg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct esdg_ensemble esdg_ensemble;
+
+/* Task ordering and launch shape of the stage kernel (the autotuner's search space). */
+typedef struct esdg_ensemble_schedule {
+    int order;          /* 0: scenario by scenario; 1: element tiles of scenarios that share a
+                           mesh interleaved (tile-major); 2: highest degree first */
+    int blocks_per_cu;  /* 0: one block per element tile; k > 0: persistent grid of k blocks per CU */
+    int per_degree;     /* 1: one launch per degree group and stage instead of one mixed launch */
+    int xcd;            /* 1: each XCD takes a contiguous part of the task list */
+    int graph;          /* 1: replay pairs of steps from a captured hipGraph */
+} esdg_ensemble_schedule;
+
+/* params[i], inputs[i], rp[i] describe scenario i (rp[i].fused is ignored).  Only the inputs the
+ * stage reads are used: GLw, nx, ny, scal, Yxi, Yeta, Xxi, Xeta, B, Bx, By, D, Jac, q, EleSizes,
+ * ElemEdgeMasterSlave, ElemEdgeOrientation, ElemToEdge, EdgeData. */
+int esdg_ensemble_create(esdg_ctx* ctx, int nscn, const esdg_params* params, const esdg_inputs* inputs,
+                         const esdg_run_params* rp, esdg_ensemble** out);
+int esdg_ensemble_set_schedule(esdg_ensemble* ens, const esdg_ensemble_schedule* sc);
+int esdg_ensemble_get_schedule(const esdg_ensemble* ens, esdg_ensemble_schedule* sc);
+/* Times trial_steps steps (after one warm-up step) of every candidate schedule, keeps the fastest
+ * and restores the ensemble's state, time and step counts.  best / best_ms_per_step may be null. */
+int esdg_ensemble_autotune(esdg_ensemble* ens, int trial_steps, esdg_ensemble_schedule* best,
+                           double* best_ms_per_step);
+/* Advances every scenario by `steps` steps; a scenario whose time has reached its T stays put
+ * and does not count the steps.  Blocks until done. */
+int esdg_ensemble_run(esdg_ensemble* ens, int steps);
+/* Issues `steps` steps on the context's stream without waiting (for timing loops). */
+int esdg_ensemble_run_async(esdg_ensemble* ens, int steps);
+int esdg_ensemble_get_q(esdg_ensemble* ens, int scn, double* host_q);   /* blocking D2H copy */
+/* Time, steps taken and last dt of scenario scn.  Returns ESDG_ERR_STATE if its dt was ever not
+ * positive and finite (the scenario then stopped). */
+int esdg_ensemble_status(esdg_ensemble* ens, int scn, double* t, int* steps_taken, double* last_dt);
+/* Device bytes held by the ensemble (state, shared geometry, tables). */
+int esdg_ensemble_bytes(const esdg_ensemble* ens, size_t* bytes);
+int esdg_ensemble_destroy(esdg_ensemble* ens);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -13,7 +13,8 @@ _LIB = None
 class OracleParams(C.Structure):
     _fields_ = [("ngl", C.c_int), ("ES", C.c_int), ("g", C.c_double), ("geomFace", C.c_double),
                 ("PosPresTOL", C.c_double), ("ZeroTOL", C.c_double), ("eps0", C.c_double),
-                ("sigmaMin", C.c_double), ("sigmaMax", C.c_double)]
+                ("sigmaMin", C.c_double), ("sigmaMax", C.c_double), ("bc_kind", C.c_int),
+                ("h_zero", C.c_double)]
 
 
 _RUN_DOUBLES = ("Jac", "Yxi", "Yeta", "Xxi", "Xeta", "Bx", "By", "B", "D", "Dhat", "Dstrong",
@@ -42,7 +43,7 @@ def lib():
 
 def params(case, rp) -> OracleParams:
     return OracleParams(case.ngl, rp.ES, rp.g, case.geom_face, rp.pos_pres_tol, rp.zero_tol,
-                        rp.eps0, rp.sigma_min, rp.sigma_max)
+                        rp.eps0, rp.sigma_min, rp.sigma_max, rp.bc_kind, rp.h_zero)
 
 
 def _arg(a):

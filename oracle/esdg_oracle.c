@@ -96,6 +96,62 @@ void oracle_collect_edge_data_walls(const oracle_params* p, int Nfaces, const in
     }
 }
 
+/* CollectEdgeData for the three boundary variants the reference selects at build time
+ * (deviceclass.cpp:363-415):
+ *   0 SolidWalls.okl:133-175   exterior h = interior h, momentum mirrored about the face
+ *   1 InnerOuter.okl:132-152   exterior h = interior h, momentum zero
+ *   2 OceanBoundary.okl:128-190 where the interior bottom trace b > 6.7 as walls, elsewhere the
+ *                              still-water state h = h_zero - b with zero momentum
+ * The interior traces are those of SolidWalls.okl:74-132 in every variant. */
+void oracle_collect_edge_data(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
+                              const double* nx, const double* ny, const double* bL, const double* bR,
+                              double* qL, double* qR) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int* E = EdgeData + 8 * f;
+        const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+        for (int i = 0; i < ngl; ++i) {
+            const long eid = (long)f * ngl * NEQ + i;
+            const long xid = (long)f * ngl + i;
+            double L[3] = {0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0};
+            if (cpuL == 0) {
+                const long id = (long)iel * ngl2 * NEQ + side_node(ngl, isl, i);
+                L[0] = q[id]; L[1] = q[id + ngl2]; L[2] = q[id + 2 * ngl2];
+            }
+            if (cpuR == 0) {
+                const long id = (long)ier * ngl2 * NEQ + side_node(ngl, isr, i);
+                R[0] = q[id]; R[1] = q[id + ngl2]; R[2] = q[id + 2 * ngl2];
+            }
+            if (cpuL == -1) {
+                const int wall = (p->bc_kind == 0) || (p->bc_kind == 2 && bR[xid] > 6.7);
+                if (wall) {
+                    L[0] = R[0];
+                    wall_mirror(nx[xid], ny[xid], R[1], R[2], &L[1], &L[2]);
+                } else if (p->bc_kind == 1) {
+                    L[0] = R[0]; L[1] = 0.0; L[2] = 0.0;
+                } else {
+                    L[0] = p->h_zero - bR[xid]; L[1] = 0.0; L[2] = 0.0;
+                }
+            }
+            if (cpuR == -1) {
+                const int wall = (p->bc_kind == 0) || (p->bc_kind == 2 && bL[xid] > 6.7);
+                if (wall) {
+                    R[0] = L[0];
+                    wall_mirror(nx[xid], ny[xid], L[1], L[2], &R[1], &R[2]);
+                } else if (p->bc_kind == 1) {
+                    R[0] = L[0]; R[1] = 0.0; R[2] = 0.0;
+                } else {
+                    R[0] = p->h_zero - bL[xid]; R[1] = 0.0; R[2] = 0.0;
+                }
+            }
+            for (int v = 0; v < 3; ++v) {
+                qL[eid + v * ngl] = L[v];
+                qR[eid + v * ngl] = R[v];
+            }
+        }
+    }
+}
+
 /* two-point EC volume fluxes F# (xi direction, eta metrics) and G# (eta direction, xi metrics):
  * okl/DG/VolumeKernelFluxDiffV99.okl:17-57 */
 static void fvol(double half_g, double fourth_g, double q1L, double q2L, double q3L, double iL,
@@ -731,7 +787,7 @@ void oracle_surface_visc(const oracle_params* p, int Nelem, const double* Jac, c
         }
 }
 
-/* deviceclass::DGtimeloop, src/deviceclass.cpp:868-1130, single rank, walls, with a step cap. */
+/* deviceclass::DGtimeloop, src/deviceclass.cpp:868-1130, single rank, with a step cap. */
 int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, double* t_out,
                     double* dt_hist) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
@@ -809,7 +865,7 @@ int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, doub
         if (dt_hist) dt_hist[step] = dt;
         if (r->rk_ssp) memcpy(Qtmp, q, sizeof(double) * nq);
         for (int s = 0; s < stages; ++s) {
-            oracle_collect_edge_data_walls(p, Nf, r->EdgeData, q, r->nx, r->ny, qL, qR);
+            oracle_collect_edge_data(p, Nf, r->EdgeData, q, r->nx, r->ny, bL, bR, qL, qR);
             oracle_volume_fluxdiff(p, Ne, r->Jac, r->Yxi, r->Yeta, r->Xxi, r->Xeta, q, r->D, r->Bx, r->By, Qt);
             if (r->num_flux == 3)
                 oracle_num_flux_es_rotated(p, Nf, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);

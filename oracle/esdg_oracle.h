@@ -22,6 +22,8 @@ typedef struct oracle_params {
     double PosPresTOL;
     double ZeroTOL;     /* hard-coded 1e-5 in the reference (src/deviceclass.cpp:148) */
     double eps0, sigmaMin, sigmaMax;
+    int bc_kind;        /* exterior state of boundary faces: 0 walls, 1 inner/outer, 2 ocean */
+    double h_zero;      /* still-water level of the ocean boundary (define h_zero) */
 } oracle_params;
 
 /* kernels (one per okl kernel; argument order as the deviceclass call sites) */
@@ -30,6 +32,10 @@ void oracle_collect_edge_data_bottom(const oracle_params* p, int Nfaces, const i
 void oracle_collect_edge_data_walls(const oracle_params* p, int Nfaces, const int* EdgeData,
                                     const double* q, const double* nx, const double* ny,
                                     double* qL, double* qR);
+/* CollectEdgeData with the boundary variant p->bc_kind (deviceclass.cpp:363-415) */
+void oracle_collect_edge_data(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
+                              const double* nx, const double* ny, const double* bL, const double* bR,
+                              double* qL, double* qR);
 void oracle_volume_fluxdiff(const oracle_params* p, int Nelem, const double* Jac, const double* Yxi,
                             const double* Yeta, const double* Xxi, const double* Xeta, const double* q,
                             const double* D, const double* Bx, const double* By, double* Qt);

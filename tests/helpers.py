@@ -57,7 +57,7 @@ def oracle_residual(case, rp, q, av_visc=None):
     q = np.ascontiguousarray(q, dtype=np.float64)
     qL, qR, bL, bR = np.zeros(fn * 3), np.zeros(fn * 3), np.zeros(fn), np.zeros(fn)
     O.call("collect_edge_data_bottom", p, Nf, a["EdgeData"], a["b"], bL, bR)
-    O.call("collect_edge_data_walls", p, Nf, a["EdgeData"], q, a["nx"], a["ny"], qL, qR)
+    O.call("collect_edge_data", p, Nf, a["EdgeData"], q, a["nx"], a["ny"], bL, bR, qL, qR)
     Qt = np.zeros(nn * 3)
     O.call("volume_fluxdiff", p, Ne, a["Jac"], a["yXi"], a["yEta"], a["xXi"], a["xEta"], q, a["D"], a["Bx"],
            a["By"], Qt)

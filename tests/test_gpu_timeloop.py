@@ -95,6 +95,19 @@ def test_flood_positivity(dev, fused):
     assert np.isfinite(Q).all() and (Q[:, 0] >= 0).all()
 
 
+@pytest.mark.parametrize("fused", SCHEDULES, ids=SCHEDULE_IDS)
+def test_ocean_uplift_open_boundary(dev, fused):
+    """Tsunami-like uplift over a 4000 m ocean with a shelf: open (still-water) boundary over the
+    deep part, walls over the shelf, N=5 Cartesian 8x8, 30 steps."""
+    case = Case.ocean_uplift(N=5, nel=8, amp=3.0, axis_a=80e3, axis_c=40e3, angle=0.6)
+    rp = RunParams(num_flux=3, bc_kind=2, h_zero=4000.0)
+    qo, no, to, _ = O.timeloop(case, rp, 30)
+    qg, ng, tg = run_gpu(dev, case, rp, 30, fused=fused)
+    assert ng == no == 30 and abs(tg - to) <= 1e-14 * to
+    h, hu, hv = fields_rel_l2(case, qg, qo)
+    assert max(h, hu, hv) <= 1e-12, (h, hu, hv)
+
+
 def test_entropy_conservative_flux(dev):
     """ES=0 (EC surface flux): total entropy changes only by the time-integration error."""
     case = Case.gaussian_bump(N=4, nel=8)

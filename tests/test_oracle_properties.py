@@ -186,6 +186,58 @@ def test_wall_gather_mirror_is_involution():
             assert abs(y[eL, side[sL](k)] - y[eR, side[sR](k)]) < 1e-13
 
 
+def test_boundary_variants_exterior_states():
+    """CollectEdgeData variants (SolidWalls / InnerOuter / OceanBoundary): interior traces are the
+    same in all three; boundary exteriors follow each okl's rule, and the ocean rule takes both of
+    its branches on the shelf mesh."""
+    c = Case.ocean_uplift(N=3, nel=5, amp=2.0)
+    q = perturbed_state(c, seed=7)
+    fn = c.Nfaces * c.ngl
+    bL, bR = np.zeros(fn), np.zeros(fn)
+    p0 = O.params(c, RunParams())
+    O.call("collect_edge_data_bottom", p0, c.Nfaces, c["EdgeData"], c["b"], bL, bR)
+    out = {}
+    for bc in (0, 1, 2):
+        p = O.params(c, RunParams(bc_kind=bc, h_zero=4000.0))
+        qL, qR = np.zeros(fn * 3), np.zeros(fn * 3)
+        O.call("collect_edge_data", p, c.Nfaces, c["EdgeData"], q, c["nx"], c["ny"], bL, bR, qL, qR)
+        out[bc] = (qL.reshape(-1, 3, c.ngl), qR.reshape(-1, 3, c.ngl))
+    wL, wR = np.zeros(fn * 3), np.zeros(fn * 3)
+    O.call("collect_edge_data_walls", p0, c.Nfaces, c["EdgeData"], q, c["nx"], c["ny"], wL, wR)
+    assert np.array_equal(out[0][0].ravel(), wL) and np.array_equal(out[0][1].ravel(), wR)
+    ED = c["EdgeData"].reshape(-1, 8)
+    bnd = ED[:, 6] == -1
+    assert bnd.any() and not (ED[:, 5] == -1).any()
+    for bc in (1, 2):
+        assert np.array_equal(out[bc][0], out[0][0])              # left traces: always interior
+        assert np.array_equal(out[bc][1][~bnd], out[0][1][~bnd])  # interior faces unchanged
+    L, R1 = out[0][0][bnd], out[1][1][bnd]
+    assert np.array_equal(R1[:, 0], L[:, 0]) and not R1[:, 1:].any()
+    R2, Rw = out[2][1][bnd], out[0][1][bnd]
+    bi = bL.reshape(-1, c.ngl)[bnd]
+    wall = bi > 6.7
+    assert wall.any() and (~wall).any()
+    for v in range(3):
+        assert np.array_equal(R2[:, v][wall], Rw[:, v][wall])
+    assert np.array_equal(R2[:, 0][~wall], 4000.0 - bi[~wall])
+    assert not R2[:, 1][~wall].any() and not R2[:, 2][~wall].any()
+
+
+def test_open_boundaries_keep_still_water():
+    """Still water h + b = h_zero at rest: the ocean boundary's exterior state equals the interior
+    one on the open part and the residual vanishes to roundoff; a wrong still-water level does not.
+    The inner/outer variant keeps a lake at rest likewise."""
+    c = Case.ocean_uplift(N=4, nel=6, amp=0.0)
+    sc = term_scale(c, c["q"])
+    for nf in (0, 3):
+        Qt = oracle_residual(c, RunParams(num_flux=nf, bc_kind=2, h_zero=4000.0), c["q"])
+        assert np.abs(Qt).max() < 1e-14 * sc, np.abs(Qt).max() / sc
+        Qt = oracle_residual(c, RunParams(num_flux=nf, bc_kind=1), c["q"])
+        assert np.abs(Qt).max() < 1e-14 * sc
+    Qt = oracle_residual(c, RunParams(num_flux=0, bc_kind=2, h_zero=4001.0), c["q"])
+    assert np.abs(Qt).max() > 1e-6 * sc
+
+
 # --------------------------------------------------------------------------------------------
 # whole residual invariants
 # --------------------------------------------------------------------------------------------
