@@ -113,9 +113,10 @@ class Case:
                 ptr, cnt, isint = C.c_void_p(), C.c_int64(), C.c_int()
                 _check(_lib().esdg_case_array(h, name.encode(), C.byref(ptr), C.byref(cnt),
                                               C.byref(isint)))
-                ct = C.c_int32 if isint.value else C.c_double
-                buf = (ct * cnt.value).from_address(ptr.value) if cnt.value else (ct * 0)()
-                self.arrays[name] = np.array(np.ctypeslib.as_array(buf), copy=True)
+                out = np.empty(cnt.value, dtype=np.int32 if isint.value else np.float64)
+                if cnt.value:
+                    C.memmove(out.ctypes.data, ptr.value, out.nbytes)
+                self.arrays[name] = out
         finally:
             _lib().esdg_case_destroy(h)
 

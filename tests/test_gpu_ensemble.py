@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from esdgsem_mpiocca_amd import Case, RunParams
+from esdgsem_mpiocca_amd.scenarios import uplift_state
 from helpers import rel_l2
 from oracle import binding as O
 
@@ -19,20 +20,6 @@ pytestmark = pytest.mark.gpu
 def dev():
     import esdgsem_mpiocca_amd as E
     return E.device()
-
-
-def uplift_state(case, amp, a, c, angle, length=800e3):
-    """The ocean-uplift initial state of Case.ocean_uplift on an existing ocean mesh."""
-    x, y = case["x"], case["y"]
-    b = case["b"]
-    cx, cy = 0.4 * length, 0.5 * length
-    cs, sn = np.cos(angle), np.sin(angle)
-    xr = cs * (x - cx) + sn * (y - cy)
-    yr = -sn * (x - cx) + cs * (y - cy)
-    h = (4000.0 - b) + amp * np.exp(-(xr / a) ** 2 - (yr / c) ** 2)
-    q = np.zeros((case.Nelem, 3, case.ngl ** 2))
-    q[:, 0] = h.reshape(case.Nelem, -1)
-    return q.ravel()
 
 
 def members():
