@@ -39,6 +39,7 @@ HIP_SOURCES = [
     "esdg_fused.hip",
     "esdg_stage.hip",
     "esdg_ensemble.hip",
+    "esdg_mlp.hip",
     "esdg_deviceclass.hip",
 ]
 

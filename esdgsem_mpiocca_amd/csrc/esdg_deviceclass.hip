@@ -11,6 +11,7 @@
 
 #include "esdg_fused.h"
 #include "esdg_internal.h"
+#include "esdg_mlp_internal.h"
 #include "esdg_stage.h"
 
 namespace esdg {
@@ -144,6 +145,11 @@ class deviceclass {
         if (rc == ESDG_OK) rc = esdg_reduce_max(ctx_, Nelem, o_LambdaMax, o_red);
         if (rc == ESDG_OK && rp_.artificial_viscosity) {
             rc = esdg_shock_capturing(ctx_, Nelem, o_q, o_B, o_VdmInv, o_EleSizes, o_ViscPara, o_ViscForPlot);
+            if (rc == ESDG_OK && ind_) {
+                // learned indicator: classes of the step's first stage, viscosity kept where not smooth
+                rc = mlp_classify(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, nullptr, o_estat);
+                if (rc == ESDG_OK) rc = mlp_mask_viscosity(ctx_, Nelem, o_estat, o_ViscPara);
+            }
             if (rc == ESDG_OK) rc = esdg_reduce_max(ctx_, Nelem, o_ViscPara, o_red + 1);
         }
         double red[2] = {0.0, 0.0};
@@ -231,6 +237,11 @@ class deviceclass {
                 if (rc) return rc;
             }
             for (int s = 0; s < stages; ++s) {
+                if (ind_ && rp_.artificial_viscosity) {
+                    if (s > 0) rc = mlp_classify(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, nullptr, o_estat);
+                    if (rc) return rc;
+                    ind_element_stages += Nelem;
+                }
                 if (rp_.fused) rc = fused_.stage(ctx_, s, A[s], Cc[s] * dt, fused_args());
                 else rc = stage(s, dt);
                 if (rc) return rc;
@@ -300,10 +311,11 @@ class deviceclass {
             if (p) esdg_free(ctx_, p);
         fused_.release(ctx_);
         stage_.release(ctx_);
-        for (void* p : {(void*)o_q2, (void*)o_clock, (void*)o_lmax})
+        for (void* p : {(void*)o_q2, (void*)o_clock, (void*)o_lmax, (void*)o_estat, (void*)o_icount})
             if (p) esdg_free(ctx_, p);
         o_q2 = o_clock = nullptr;
-        o_lmax = nullptr;
+        o_lmax = o_icount = nullptr;
+        o_estat = nullptr;
         o_q = o_Qt = o_gRK = o_Qtmp = nullptr;
         return ESDG_OK;
     }
@@ -320,6 +332,11 @@ class deviceclass {
         a.EdgeData = o_EdgeData; a.EdgeReversed = o_EdgeReversed; a.ElemToEdge = o_ElemToEdge;
         a.EMS = o_ElemEdgeMasterSlave; a.EOR = o_ElemEdgeOrientation;
         a.ViscPara = o_ViscPara;
+        if (ind_) {
+            a.estat = o_estat;
+            a.audit = ind_audit;
+            a.counters = o_icount;
+        }
         return a;
     }
 
@@ -350,6 +367,13 @@ class deviceclass {
     FusedStage fused_;
     StageBatch stage_;
     bool stage_ready = false;
+    // learned troubled-cell indicator (esdg_mlp.h)
+    esdg_mlp* ind_ = nullptr;
+    float ind_lo = 0.0f, ind_hi = 1.0f;
+    int ind_audit = 0;
+    int64_t ind_element_stages = 0;
+    signed char* o_estat = nullptr;
+    unsigned long long* o_icount = nullptr;
     double *o_q2 = nullptr, *o_clock = nullptr;
     unsigned long long* o_lmax = nullptr;
     double *o_q = nullptr, *o_Qt = nullptr, *o_gRK = nullptr, *o_Qtmp = nullptr;
@@ -428,6 +452,52 @@ int esdg_solver_buffer(esdg_solver* s, const char* name, void** dptr) {
     if (!s || !name || !dptr) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_buffer: null argument");
     *dptr = s->dc.buffer(name);
     if (!*dptr) return set_error(s->dc.ctx_, ESDG_ERR_ARG, std::string("esdg_solver_buffer: no buffer ") + name);
+    return ESDG_OK;
+}
+
+int esdg_solver_set_indicator(esdg_solver* s, esdg_mlp* m, float lo, float hi) {
+    if (!s) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_set_indicator: null solver");
+    deviceclass& d = s->dc;
+    if (!m) {
+        d.ind_ = nullptr;
+        return ESDG_OK;
+    }
+    if (d.rp_.fused != 1 || !d.rp_.artificial_viscosity)
+        return set_error(d.ctx_, ESDG_ERR_ARG, "the learned indicator needs the fused schedule with artificial viscosity");
+    if (d.ngl != 8) return set_error(d.ctx_, ESDG_ERR_ARG, "the learned indicator is defined for N = 7");
+    if (!(lo <= hi)) return set_error(d.ctx_, ESDG_ERR_ARG, "esdg_solver_set_indicator: lo > hi");
+    int rc = ESDG_OK;
+    if (!d.o_estat) rc = esdg_malloc(d.ctx_, (void**)&d.o_estat, (size_t)d.Nelem);
+    if (!rc && !d.o_icount) rc = esdg_malloc(d.ctx_, (void**)&d.o_icount, 4 * sizeof(unsigned long long));
+    if (rc) return rc;
+    d.ind_ = m;
+    d.ind_lo = lo;
+    d.ind_hi = hi;
+    return ESDG_OK;
+}
+
+int esdg_solver_indicator_audit(esdg_solver* s, int audit) {
+    if (!s) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_indicator_audit: null solver");
+    s->dc.ind_audit = audit != 0;
+    return ESDG_OK;
+}
+
+int esdg_solver_indicator_counts(esdg_solver* s, int64_t* flagged, int64_t* false_negatives, int64_t* sensor_positives,
+                                 int64_t* element_stages) {
+    if (!s) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_indicator_counts: null solver");
+    deviceclass& d = s->dc;
+    unsigned long long c[4] = {0, 0, 0, 0};
+    if (d.o_icount) {
+        int rc = esdg_memcpy(d.ctx_, c, d.o_icount, sizeof c, ESDG_D2H);
+        if (!rc) rc = esdg_fill(d.ctx_, reinterpret_cast<double*>(d.o_icount), 0.0, 4);
+        if (!rc) rc = esdg_finish(d.ctx_);
+        if (rc) return rc;
+    }
+    if (flagged) *flagged = (int64_t)c[0];
+    if (false_negatives) *false_negatives = (int64_t)c[1];
+    if (sensor_positives) *sensor_positives = (int64_t)c[2];
+    if (element_stages) *element_stages = d.ind_element_stages;
+    d.ind_element_stages = 0;
     return ESDG_OK;
 }
 

@@ -15,6 +15,12 @@ struct FusedArgs {
     const double *nx, *ny, *scal, *bL, *bR;
     const int *EdgeData, *EdgeReversed, *ElemToEdge, *EMS, *EOR;
     double* ViscPara;   // element viscosity of the stage (written by the fused kernels)
+    // learned indicator (null: the modal sensor everywhere).  estat[e]: 0 smooth, 1 troubled,
+    // 2 unsure (the sensor decides); audit evaluates the sensor everywhere and counts into
+    // counters[0] elements with viscosity, [1] sensor positives declared smooth, [2] sensor positives.
+    const signed char* estat = nullptr;
+    int audit = 0;
+    unsigned long long* counters = nullptr;
 };
 
 class FusedStage {

@@ -74,8 +74,16 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
 // per node, all intermediate fields in LDS.  Writes the scaled gradients (read by the viscous face
 // kernel) as element-side traces, the element viscosity and Qt(hu, hv) += viscous volume term.
 // ---------------------------------------------------------------------------------------------
-template <int NGL>
+//
+// With the learned indicator (LEARN): estat[e] = 0 (smooth) forces zero viscosity, 1 and 2 keep the
+// sensor's value.  A block whose elements and their face neighbours are all smooth has nothing to
+// contribute (the viscous terms and face fluxes it feeds are multiplied by zero viscosities): it
+// writes zero viscosities and leaves, unless audit is set.
+template <int NGL, bool LEARN>
 __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorConsts sc,
+                                                  const signed char* __restrict__ estat, int audit,
+                                                  unsigned long long* __restrict__ counters,
+                                                  const int* __restrict__ ED,
                                                   const double* __restrict__ q, const double* __restrict__ b,
                                                   const double* __restrict__ VdmInv, const double* __restrict__ Dh,
                                                   const double* __restrict__ Ds, const double* __restrict__ Jac,
@@ -91,9 +99,27 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
     __shared__ double sA[EB::THREADS], sB[EB::THREADS], sC[EB::THREADS], sE[EB::THREADS];
     __shared__ double sEps[EB::EPB];
     __shared__ double sSum[EB::EPB][4];
+    __shared__ int sActive;
     const int el = threadIdx.x / NN, n = threadIdx.x % NN;
     const int e = blockIdx.x * EB::EPB + el;
     const bool act = e < Nelem;
+    if (LEARN) {
+        if (threadIdx.x == 0) sActive = audit;
+        __syncthreads();
+        if (act && n < 5) {
+            int nb = e;
+            if (n < 4) {
+                const int f = E2E[e * 4 + n] - 1;
+                nb = (ED[8 * f + 6] == -1) ? e : (ED[8 * f] == e ? ED[8 * f + 1] : ED[8 * f]);
+            }
+            if (estat[nb] != 0) sActive = 1;
+        }
+        __syncthreads();
+        if (!sActive) {
+            if (act && n == 0) visc[e] = 0.0;
+            return;
+        }
+    }
     if (threadIdx.x < NN) {
         sV[threadIdx.x] = VdmInv[threadIdx.x];
         sDh[threadIdx.x] = Dh[threadIdx.x];
@@ -147,6 +173,15 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
             if (ee < Nelem) {
                 const double* S4 = sSum[threadIdx.x];
                 v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
+                if (LEARN) {
+                    const double plain = v;
+                    if (estat[ee] == 0) v = 0.0;
+                    if (counters) {
+                        if (v > 0.0) atomicAdd(counters, 1ull);
+                        if (audit && plain > 0.0 && estat[ee] == 0) atomicAdd(counters + 1, 1ull);
+                        if (audit && plain > 0.0) atomicAdd(counters + 2, 1ull);
+                    }
+                }
                 visc[ee] = v;
             }
             sEps[threadIdx.x] = v;
@@ -257,18 +292,21 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl,
     double N0 = 0.0, N1 = 0.0;
     if (!(cpuL == -1 || cpuR == -1)) {
         // both sides present (procL = procR = 0): the boundary-face branches of the gather never apply;
-        // the right element's trace is read at the reversed index, as CollectEdgeDataGradient does
-        const int inc = (rv == 0) ? (ngl - 1) - i : i;
-        const double* TL = gtr + ((int64_t)iel * 4 + isl) * 4 * ngl + i;
-        const double* TR = gtr + ((int64_t)ier * 4 + isr) * 4 * ngl + inc;
-        const double nx = nxa[t], ny = nya[t];
-        const double f2L = nx * TL[0] + ny * TL[2 * ngl];
-        const double f2R = nx * TR[0] + ny * TR[2 * ngl];
-        const double f3L = nx * TL[ngl] + ny * TL[3 * ngl];
-        const double f3R = nx * TR[ngl] + ny * TR[3 * ngl];
+        // the right element's trace is read at the reversed index, as CollectEdgeDataGradient does.
+        // Zero viscosity on both sides makes the flux zero: the traces are then not read.
         const double m = fmax(visc[iel], visc[ier]);
-        N0 = 0.5 * m * (f2L + f2R);
-        N1 = 0.5 * m * (f3L + f3R);
+        if (m != 0.0) {
+            const int inc = (rv == 0) ? (ngl - 1) - i : i;
+            const double* TL = gtr + ((int64_t)iel * 4 + isl) * 4 * ngl + i;
+            const double* TR = gtr + ((int64_t)ier * 4 + isr) * 4 * ngl + inc;
+            const double nx = nxa[t], ny = nya[t];
+            const double f2L = nx * TL[0] + ny * TL[2 * ngl];
+            const double f2R = nx * TR[0] + ny * TR[2 * ngl];
+            const double f3L = nx * TL[ngl] + ny * TL[3 * ngl];
+            const double f3R = nx * TR[ngl] + ny * TR[3 * ngl];
+            N0 = 0.5 * m * (f2L + f2R);
+            N1 = 0.5 * m * (f3L + f3R);
+        }
     }
     const double sc = scal[t];
     SPV[eid] = -sc * geomFace * N0;
@@ -391,9 +429,15 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     SensorConsts sc{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
     const double* SGX = F.fgrad;
     const double* SGY = F.fgrad + (size_t)A.Nfaces * NGL * 2;
-    k_fused_av<NGL><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
-        A.Nelem, ctx->p.PosPresTOL, sc, A.q, A.B, A.VdmInv, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
-        A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, A.Qt);
+    if (A.estat)
+        k_fused_av<NGL, true><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
+            A.Nelem, ctx->p.PosPresTOL, sc, A.estat, A.audit, A.counters, A.EdgeData, A.q, A.B, A.VdmInv, A.Dhat,
+            A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta, A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara,
+            A.Qt);
+    else
+        k_fused_av<NGL, false><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
+            A.Nelem, ctx->p.PosPresTOL, sc, nullptr, 0, nullptr, A.EdgeData, A.q, A.B, A.VdmInv, A.Dhat, A.Dstrong,
+            A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta, A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, A.Qt);
 }
 
 template <int NGL>

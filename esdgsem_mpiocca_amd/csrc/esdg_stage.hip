@@ -272,7 +272,7 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
 
     // ---- fluxes of the element's four sides: lane r takes side node r of every side ----------
     if (act) {
-#pragma unroll 1
+#pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int f = S.E2E[e * 4 + s] - 1;
             const int rev = S.EOR[e * 4 + s];

@@ -442,7 +442,9 @@ namespace {
 
 // Bathymetry and initial state at every node (the benchmark configurations).
 //   GAUSSIAN_BUMP: b = 0.8 exp(-5((x-1)^2+(y-1)^2)); h = 1 - b + amp*exp(-20((x-0.7)^2+(y-1)^2)).
-//   DAM_BREAK:     b = 0.1 sin(x) cos(y); h = 2 for x < p0 (default 5) else 1, plus amp*(2U-1).
+//   DAM_BREAK:     b = 0.1 sin(x) cos(y); h = 2 for x < p0 (default 5) else 1, plus amp*(2U-1);
+//                  with p1 (height ratio) or p2 (angle) set: h = p1 on the side (x - p0, y - yc).n < 0
+//                  of the line through (p0, yc) with normal n = (cos p2, sin p2), 1 elsewhere.
 //   FLOOD:         b = (0.3/4) sum_{m<16} cos(2 pi (k_m x + l_m y)/Lx + phi_m), integers
 //                  k_m, l_m in [0,3] (not both 0), phi_m in [0, 2 pi), all seeded; water level
 //                  p0 for x < p1, level p2 elsewhere; h = max(0, level - b).  Wet/dry fronts.
@@ -486,7 +488,15 @@ void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<do
                 case ESDG_CASE_DAM_BREAK: {
                     const double xd = (s.p0 != 0.0) ? s.p0 : 5.0;
                     bb = 0.1 * std::sin(X) * std::cos(Y);
-                    h = (X < xd) ? 2.0 : 1.0;
+                    if (s.p1 == 0.0 && s.p2 == 0.0) {
+                        h = (X < xd) ? 2.0 : 1.0;
+                    } else {
+                        // step of height ratio p1 (h = p1 | 1) across the line through (xd, centre y)
+                        // with normal angle p2
+                        const double ratio = (s.p1 != 0.0) ? s.p1 : 2.0;
+                        const double side = (X - xd) * std::cos(s.p2) + (Y - 0.5 * (s.yL + s.yR)) * std::sin(s.p2);
+                        h = (side < 0.0) ? ratio : 1.0;
+                    }
                     h += s.amp * (2.0 * uniform01(s.seed, qid) - 1.0);
                     break;
                 }

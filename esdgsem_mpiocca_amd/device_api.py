@@ -262,6 +262,21 @@ class Solver:
         self.ctx.check(lib().esdg_solver_last_dt(C.c_void_p(self.handle), C.byref(d)))
         return d.value
 
+    def set_indicator(self, mlp, lo: float = 0.5, hi: float = 0.5):
+        """Learned troubled-cell indicator in the AV path (None: the modal sensor)."""
+        self.ctx.check(lib().esdg_solver_set_indicator(C.c_void_p(self.handle),
+                                                       C.c_void_p(mlp.handle if mlp else None),
+                                                       C.c_float(lo), C.c_float(hi)))
+
+    def indicator_audit(self, on: bool = True):
+        self.ctx.check(lib().esdg_solver_indicator_audit(C.c_void_p(self.handle), int(on)))
+
+    def indicator_counts(self):
+        v = [C.c_int64() for _ in range(4)]
+        self.ctx.check(lib().esdg_solver_indicator_counts(C.c_void_p(self.handle), *[C.byref(x) for x in v]))
+        return dict(zip(("flagged", "false_negatives", "sensor_positives", "element_stages"),
+                        [x.value for x in v]))
+
     def buffer_ptr(self, name: str) -> int:
         p = C.c_void_p()
         self.ctx.check(lib().esdg_solver_buffer(C.c_void_p(self.handle), name.encode(), C.byref(p)))
@@ -351,3 +366,64 @@ class Ensemble:
         if self.handle:
             lib().esdg_ensemble_destroy(C.c_void_p(self.handle))
             self.handle = None
+
+
+MLP_NIN, MLP_NHID = 72, 32
+MLP_NPARAM = 72 * 32 + 32 + 32 * 32 + 32 + 32 + 1
+
+
+class MLP:
+    """esdg_mlp (include/esdg_mlp.h): the learned troubled-cell indicator on the device."""
+
+    def __init__(self, ctx: Context, seed: int = 0, params=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().esdg_mlp_create(C.c_void_p(ctx.handle), C.c_uint64(seed), C.byref(h)))
+        self.handle = h.value
+        if params is not None:
+            self.set_params(params)
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        assert a.size == MLP_NPARAM
+        self.ctx.check(lib().esdg_mlp_set_params(C.c_void_p(self.handle), a.ctypes.data_as(C.c_void_p)))
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(MLP_NPARAM, dtype=np.float32)
+        self.ctx.check(lib().esdg_mlp_get_params(C.c_void_p(self.handle), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def infer(self, feat_dev: DeviceArray, n: int, out_dev: DeviceArray):
+        self.ctx.check(lib().esdg_mlp_infer(C.c_void_p(self.handle), int(n), C.c_void_p(feat_dev.ptr),
+                                            C.c_void_p(out_dev.ptr)))
+
+    def indicator(self, Nelem, q_dev, B_dev, V_dev, out_dev):
+        self.ctx.check(lib().esdg_mlp_indicator(C.c_void_p(self.handle), int(Nelem), C.c_void_p(q_dev.ptr),
+                                                C.c_void_p(B_dev.ptr), C.c_void_p(V_dev.ptr),
+                                                C.c_void_p(out_dev.ptr)))
+
+    def loss_grad(self, n, feat_dev, label_dev, idx_dev, pos_weight, grad_dev, loss_dev):
+        self.ctx.check(lib().esdg_mlp_loss_grad(C.c_void_p(self.handle), int(n), C.c_void_p(feat_dev.ptr),
+                                                C.c_void_p(label_dev.ptr),
+                                                C.c_void_p(idx_dev.ptr if idx_dev is not None else None),
+                                                C.c_float(pos_weight), C.c_void_p(grad_dev.ptr),
+                                                C.c_void_p(loss_dev.ptr)))
+
+    def train_step(self, n, feat_dev, label_dev, idx_dev, pos_weight, lr, want_loss=False) -> float:
+        loss = C.c_float(float("nan"))
+        self.ctx.check(lib().esdg_mlp_train_step(C.c_void_p(self.handle), int(n), C.c_void_p(feat_dev.ptr),
+                                                 C.c_void_p(label_dev.ptr),
+                                                 C.c_void_p(idx_dev.ptr if idx_dev is not None else None),
+                                                 C.c_float(pos_weight), C.c_float(lr),
+                                                 C.byref(loss) if want_loss else None))
+        return loss.value
+
+    def close(self):
+        if self.handle:
+            lib().esdg_mlp_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+
+def mlp_features(ctx: Context, Nelem, q_dev, B_dev, V_dev, feat_dev):
+    ctx.check(lib().esdg_mlp_features(C.c_void_p(ctx.handle), int(Nelem), C.c_void_p(q_dev.ptr),
+                                      C.c_void_p(B_dev.ptr), C.c_void_p(V_dev.ptr), C.c_void_p(feat_dev.ptr)))

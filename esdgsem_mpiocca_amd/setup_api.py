@@ -135,10 +135,13 @@ class Case:
                    amp=0.0 if lake_at_rest else 0.01)
 
     @classmethod
-    def dam_break(cls, N=7, nel=512, noise=1e-3, seed=0, perturb=0.05):
-        """nel^2 curved quads on [0,10]^2 (x,y += 0.05 sin(pi x/5) sin(pi y/5)), walls, dam break."""
+    def dam_break(cls, N=7, nel=512, noise=1e-3, seed=0, perturb=0.05, ratio=0.0, angle=0.0):
+        """nel^2 curved quads on [0,10]^2 (x,y += 0.05 sin(pi x/5) sin(pi y/5)), walls, dam break:
+        h = 2 | 1 across x = 5, or with ratio/angle set h = ratio | 1 across the line through (5, 5)
+        with normal angle `angle`."""
         return cls(CASE_DAM_BREAK, MESH_CURVED_SINE, N, nel, nel, 0.0, 10.0, 0.0, 10.0, 0, 0,
-                   perturb_amp=perturb, perturb_len=5.0, amp=noise, p=(5.0, 0, 0, 0), seed=seed)
+                   perturb_amp=perturb, perturb_len=5.0, amp=noise, p=(5.0, ratio, angle, 0),
+                   seed=seed)
 
     @classmethod
     def flood(cls, N=7, nel=2048, seed=0, level_left=0.8, x_dam=3.0, level_right=-0.2,

@@ -52,7 +52,8 @@ enum esdg_mesh_kind {
 
 enum esdg_case_kind {
     ESDG_CASE_GAUSSIAN_BUMP = 0,  /* b = 0.8 e^{-5 r1^2}, h = 1 - b + amp e^{-20 r2^2}, u = v = 0 */
-    ESDG_CASE_DAM_BREAK = 1,      /* b = 0.1 sin x cos y, h = 2 (x<5) else 1, + seeded U(-noise,noise) */
+    ESDG_CASE_DAM_BREAK = 1,      /* b = 0.1 sin x cos y, h = 2 (x<5) else 1, + seeded U(-noise,noise);
+                                     p1 = height ratio, p2 = step angle (radians) when set */
     ESDG_CASE_FLOOD = 2,          /* 16-mode random-phase Fourier bathymetry, reservoir over dry land */
     ESDG_CASE_OCEAN_UPLIFT = 3,   /* 4000 m ocean with linear shelf to 50 m, elliptic Gaussian uplift */
     ESDG_CASE_CONSTANT = 4        /* constant state (h0, hu0, hv0) over flat b=0: free-stream test */

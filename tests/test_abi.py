@@ -16,7 +16,7 @@ def declared(header):
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header,lib", [("esdg.h", "libesdg.so"), ("esdg_ensemble.h", "libesdg.so"),
+@pytest.mark.parametrize("header,lib", [("esdg.h", "libesdg.so"), ("esdg_ensemble.h", "libesdg.so"), ("esdg_mlp.h", "libesdg.so"),
                                         ("esdg_setup.h", "libesdg_setup.so")])
 def test_exports(header, lib):
     path = os.path.join(PKG, lib)
