@@ -165,6 +165,73 @@ def ensemble_bench(dev, args):
     }
 
 
+def c4_bench(dev, args):
+    """Wet/dry flood on the curved mesh family, N=7, positivity limiter, artificial viscosity: the
+    modal sensor everywhere against the learned indicator (modal-sensor fallback)."""
+    import numpy as np
+    from esdgsem_mpiocca_amd import indicator
+    params, meta = indicator.load_weights()
+    if params is None:
+        raise SystemExit("no trained indicator weights (esdgsem_mpiocca_amd/indicator_weights.json)")
+    t0 = time.perf_counter()
+    from esdgsem_mpiocca_amd import Case
+    case = Case.flood(N=7, nel=args.c4_nel, seed=0)
+    rp = indicator.av_run_params(positivity=1, pos_pres_tol=1e-3, CFL=0.4)
+    print(f"# c4: {case.Nelem} elements, case built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    out = {}
+    qs = {}
+    for mode in ("plain", "learned", "audit"):
+        ctx = dev.Context(dev.make_params(case, rp), device=0)
+        try:
+            solver = dev.Solver(ctx, case, rp, fused=1)
+            net = None
+            if mode != "plain":
+                net = dev.MLP(ctx, params=params)
+                solver.set_indicator(net, args.ind_lo, args.ind_hi)
+                solver.indicator_audit(mode == "audit")
+            steps = args.c4_steps if mode != "audit" else args.c4_audit_steps
+            done, wall = 0, 0.0
+            chunk = max(1, min(steps, 50))
+            while done < steps:
+                k = min(chunk, steps - done)
+                t1 = time.perf_counter()
+                n, t_sim = solver.run(k)
+                ctx.finish()
+                wall += time.perf_counter() - t1
+                done += n
+                print(f"# c4 {mode}: {done}/{steps} steps, {1e3 * wall / done:.1f} ms/step", file=sys.stderr,
+                      flush=True)
+                if n != k:
+                    raise SystemExit(f"c4 {mode}: solver stopped after {done} steps")
+            counts = solver.indicator_counts() if net else None
+            if mode != "audit":
+                qs[mode] = solver.get_q()
+                dofs = 3 * case.ngl ** 2 * case.Nelem
+                out[mode] = {"dof_stage_updates_per_s": dofs * 5 * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                             "steps": steps, "sim_time": t_sim}
+                if counts:
+                    out[mode]["viscous_element_stage_fraction"] = counts["flagged"] / max(counts["element_stages"], 1)
+            else:
+                out["audit"] = {"steps": steps, **counts,
+                                "false_negative_rate": counts["false_negatives"] / max(counts["sensor_positives"], 1)}
+            solver.close()
+            if net:
+                net.close()
+        finally:
+            ctx.close()
+    A = qs["learned"].reshape(case.Nelem, 3, -1)
+    B = qs["plain"].reshape(case.Nelem, 3, -1)
+    out["l2_change_vs_plain"] = [float(np.linalg.norm(A[:, v] - B[:, v]) / max(np.linalg.norm(B[:, v]), 1e-300))
+                                 for v in range(3)]
+    out["finite"] = bool(np.isfinite(A).all() and np.isfinite(B).all())
+    out["min_h"] = float(min(A[:, 0].min(), B[:, 0].min()))
+    out["speedup_learned_vs_plain"] = out["learned"]["dof_stage_updates_per_s"] / out["plain"]["dof_stage_updates_per_s"]
+    out["indicator"] = {"lo": args.ind_lo, "hi": args.ind_hi, "validation": meta.get("validation")}
+    out["config"] = {"workload": f"curved flood {args.c4_nel}x{args.c4_nel}, N=7, 16-mode Fourier bathymetry "
+                                 "(amplitude 0.3), reservoir over dry land, positivity limiter, AV, rotated ES, LSRK4(5)"}
+    return out
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
     if not os.path.exists(path):
@@ -186,7 +253,12 @@ def main():
     ap.add_argument("--cpu-sample-nel", type=int, default=128)
     ap.add_argument("--cpu-sample-steps", type=int, default=15)
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
-    ap.add_argument("--workload", choices=["c1", "ensemble"], default="c1",
+    ap.add_argument("--c4-nel", type=int, default=2048)
+    ap.add_argument("--c4-steps", type=int, default=1000)
+    ap.add_argument("--c4-audit-steps", type=int, default=20)
+    ap.add_argument("--ind-lo", type=float, default=0.05)
+    ap.add_argument("--ind-hi", type=float, default=0.5)
+    ap.add_argument("--workload", choices=["c1", "ensemble", "c4"], default="c1",
                     help="c1: the curved dam break (headline line, plus the ensemble unless --ensemble 0); "
                          "ensemble: the ensemble only")
     ap.add_argument("--ensemble", type=int, default=1, help="also run the ensemble benchmark")
@@ -206,8 +278,8 @@ def main():
     import esdgsem_mpiocca_amd as E
     dev = E.device()
 
-    if args.workload == "ensemble":
-        out = ensemble_bench(dev, args)
+    if args.workload in ("ensemble", "c4"):
+        out = ensemble_bench(dev, args) if args.workload == "ensemble" else c4_bench(dev, args)
         out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                     "dtype": "f64", "data": "synthetic"})
         print(json.dumps(out), flush=True)

@@ -29,6 +29,7 @@ class FusedStage {
     int setup(esdg_ctx* ctx, const int* EdgeData, const int* ElemToEdge, const int* EMS, const int* EOR,
               const double* Jac, const double* nx, const double* ny, const double* scal);
     // One RK stage: residual, low-storage accumulate and update (gRK = rkA gRK + Qt; q += cdt gRK).
+    // With AV: face -> AV element pass -> viscous face -> volume + surface + viscous + RK [-> limiter].
     int stage(esdg_ctx* ctx, int s, double rkA, double cdt, const FusedArgs& a);
     void release(esdg_ctx* ctx);
     void* buffer(const std::string& name);
@@ -41,6 +42,7 @@ class FusedStage {
     double* fgrad = nullptr;    // BR1 face fluxes [X|Y][f][2][ngl] (AV)
     double* grad = nullptr;     // side traces of the scaled velocity gradients [e][side][4][ngl] (AV)
     double* fvisc = nullptr;    // viscous face fluxes [f][2][ngl] (AV)
+    double* qvisc = nullptr;    // viscous volume term [e][2][ngl^2] (hu, hv) (AV)
 };
 
 }  // namespace esdg

@@ -4,21 +4,25 @@
 // does with it:
 //   CollectEdgeData + calcNumFluxes [+ calcNumFluxesGradient]      -> fused_face (one pass over faces,
 //                                                                     traces read straight from q)
-//   VolumeKernel + SurfaceKernel                                    -> fused_volume (esdg_volume.hip)
 //   [ShockCapturing + calcGradient + SurfaceKernelGradient + scaleGradient + VolumeKernelViscose]
-//                                                                   -> fused_av (one element pass)
+//                                                                   -> fused_av (one element pass; the
+//                                                                     viscous volume term goes to Qv)
 //   [CollectEdgeDataGradient + calcNumFluxesViscose]                -> fused_viscous_face
-//   [SurfaceKernelVisc] + calcRK + UpdateKernel [+ PosPres]         -> fused_update, or folded into
-//                                                                     fused_volume when AV is off
-// Every value is computed with the reference kernel's expressions in the reference's order, and Qt
-// receives its contributions in the reference's order (volume, surface, viscous volume, viscous
-// surface), so the fused stage reproduces the kernel-by-kernel results.
+//   VolumeKernel + SurfaceKernel [+ Qv + SurfaceKernelVisc] + calcRK + UpdateKernel
+//                                                                   -> fused_volume (esdg_volume.hip),
+//                                                                     RK in place
+//   [PosPres]                                                       -> pos_pres
+// The viscous terms depend only on the stage-start state and the face fluxes, so the AV pass runs
+// before the volume pass.  Every value is computed with the reference kernel's expressions in the
+// reference's order, and Qt receives its contributions in the reference's order (volume, surface,
+// viscous volume, viscous surface), so the fused stage reproduces the kernel-by-kernel results.
 #include "esdg_device.h"
 #include "esdg_fused.h"
 
 namespace esdg {
 
-int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt);
+int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
+                 const double* Qv, const double* SPV);
 
 // ---------------------------------------------------------------------------------------------
 // fused_face: CollectEdgeData (SolidWalls.okl:16-207) + calcNumFluxes (calcESFlux.okl /
@@ -72,7 +76,7 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
 // fused_av: ShockCapturing.okl:17-234 + calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148
 // + scaleGradient.okl:17-60 + VolumeKernelViscose.okl:17-150 for one block of elements, one thread
 // per node, all intermediate fields in LDS.  Writes the scaled gradients (read by the viscous face
-// kernel) as element-side traces, the element viscosity and Qt(hu, hv) += viscous volume term.
+// kernel) as element-side traces, the element viscosity and the viscous volume term Qv(hu, hv).
 // ---------------------------------------------------------------------------------------------
 //
 // With the learned indicator (LEARN): estat[e] = 0 (smooth) forces zero viscosity, 1 and 2 keep the
@@ -92,7 +96,7 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
                                                   const int* __restrict__ E2E, const int* __restrict__ EMS,
                                                   const int* __restrict__ EOR, const double* __restrict__ SGX,
                                                   const double* __restrict__ SGY, double* __restrict__ gtr,
-                                                  double* __restrict__ visc, double* __restrict__ Qt) {
+                                                  double* __restrict__ visc, double* __restrict__ Qv) {
     using EB = ElemBlock<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sV[NN], sDh[NN], sDs[NN];
@@ -267,9 +271,9 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
             d3 += sDs[a * NGL + l] * F3[c * NGL + l];
             d3 += sDs[c * NGL + l] * G3[l * NGL + a];
         }
-        double* o = Qt + (int64_t)e * NN * ESDG_NEQ + n;
-        o[NN] += rJ * d2;
-        o[2 * NN] += rJ * d3;
+        double* o = Qv + (int64_t)e * NN * 2 + n;
+        o[0] = rJ * d2;
+        o[NN] = rJ * d3;
     }
 }
 
@@ -313,116 +317,6 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl,
     SPV[eid + ngl] = -sc * geomFace * N1;
 }
 
-// ---------------------------------------------------------------------------------------------
-// fused_update: SurfaceKernelVisc.okl:17-144 + rkLS.okl + UpdateKernelLS.okl [+ PosPres.okl]
-// ---------------------------------------------------------------------------------------------
-template <int NGL, bool VISC, bool POS>
-__global__ void __launch_bounds__(256) k_fused_update(int Nelem, double rkA, double cdt, double tol, double zero_tol,
-                                                      const double* __restrict__ Jac, const int* __restrict__ E2E,
-                                                      const int* __restrict__ EMS, const int* __restrict__ EOR,
-                                                      const double* __restrict__ SPV, const double* __restrict__ Qt,
-                                                      double* __restrict__ gRK, double* __restrict__ q,
-                                                      const double* __restrict__ EleSizes,
-                                                      const double* __restrict__ GLw) {
-    using EB = ElemBlock<NGL>;
-    constexpr int NN = NGL * NGL;
-    __shared__ double sH[POS ? EB::THREADS : 1], sHU[POS ? EB::THREADS : 1], sHV[POS ? EB::THREADS : 1];
-    __shared__ double sJ[POS ? EB::THREADS : 1];
-    __shared__ double sAvg[EB::EPB][5];
-    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
-    const int e = blockIdx.x * EB::EPB + el;
-    const bool act = e < Nelem;
-    const int a = n % NGL, c = n / NGL;
-    const int64_t id = (int64_t)e * NN * ESDG_NEQ + n;
-    double q1 = 0.0, q2 = 0.0, q3 = 0.0;
-    if (act) {
-        double t1 = Qt[id], t2 = Qt[id + NN], t3 = Qt[id + 2 * NN];
-        if (VISC) {
-            const double rJ = Jac[(int64_t)e * NN + n];
-#pragma unroll
-            for (int ph = 0; ph < 2; ++ph) {
-                int side = -1, t = 0;
-                if (ph == 0) {
-                    if (c == 0) { side = 0; t = a; }
-                    else if (c == NGL - 1) { side = 2; t = a; }
-                } else {
-                    if (a == NGL - 1) { side = 1; t = c; }
-                    else if (a == 0) { side = 3; t = c; }
-                }
-                if (side < 0) continue;
-                const int ifa = E2E[e * 4 + side] - 1;
-                const int rev = EOR[e * 4 + side];
-                const int ms = EMS[e * 4 + side];
-                const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
-                const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
-                t2 += -ms * rJ * SPV[fl];
-                t3 += -ms * rJ * SPV[fl + NGL];
-            }
-        }
-        double g1 = gRK[id], g2 = gRK[id + NN], g3 = gRK[id + 2 * NN];
-        g1 = rkA * g1 + t1;
-        g2 = rkA * g2 + t2;
-        g3 = rkA * g3 + t3;
-        gRK[id] = g1; gRK[id + NN] = g2; gRK[id + 2 * NN] = g3;
-        q1 = q[id]; q2 = q[id + NN]; q3 = q[id + 2 * NN];
-        q1 += cdt * g1; q2 += cdt * g2; q3 += cdt * g3;
-        if (!POS) { q[id] = q1; q[id + NN] = q2; q[id + 2 * NN] = q3; }
-    }
-    if (!POS) return;
-    // positivity limiter on the updated state (PosPres.okl:17-223); node n is (xi = a, eta = c)
-    if (act) {
-        sH[threadIdx.x] = q1; sHU[threadIdx.x] = q2; sHV[threadIdx.x] = q3;
-        sJ[threadIdx.x] = Jac[(int64_t)e * NN + n];
-    }
-    __syncthreads();
-    if (threadIdx.x < EB::EPB) {
-        const int ee = blockIdx.x * EB::EPB + threadIdx.x;
-        if (ee < Nelem) {
-            const double* H = sH + threadIdx.x * NN;
-            const double* HU = sHU + threadIdx.x * NN;
-            const double* HV = sHV + threadIdx.x * NN;
-            const double* JJ = sJ + threadIdx.x * NN;
-            double hmin = 0.0, ha = 0.0, hua = 0.0, hva = 0.0;
-            for (int l = 0; l < NGL; ++l)
-                for (int k = 0; k < NGL; ++k) {
-                    const int nn = k * NGL + l;
-                    const double w = GLw[l] * GLw[k] / JJ[nn];
-                    ha += H[nn] * w;
-                    hua += HU[nn] * w;
-                    hva += HV[nn] * w;
-                    hmin = fmin(hmin, H[nn]);
-                }
-            const double size = EleSizes[ee];
-            ha = ha / size;
-            if (fabs(ha) < zero_tol) ha = 0.0;
-            sAvg[threadIdx.x][0] = ha;
-            sAvg[threadIdx.x][1] = hua / size;
-            sAvg[threadIdx.x][2] = hva / size;
-            sAvg[threadIdx.x][3] = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
-            sAvg[threadIdx.x][4] = hmin;
-        }
-    }
-    __syncthreads();
-    if (act) {
-        const double ha = sAvg[el][0], hua = sAvg[el][1], hva = sAvg[el][2], th = sAvg[el][3], hmin = sAvg[el][4];
-        if (hmin < 0.0) {
-            double hn = 0.0, hun = 0.0, hvn = 0.0;
-            if (ha > 0.0) {
-                hn = fabs(th * (q1 - ha) + ha);
-                if (hn > tol) {
-                    hun = th * (q2 - hua) + hua;
-                    hvn = th * (q3 - hva) + hva;
-                }
-            }
-            if (ha < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
-            q1 = hn; q2 = hun; q3 = hvn;
-        } else if (q1 < tol) {
-            q2 = 0.0; q3 = 0.0;
-        }
-        q[id] = q1; q[id + NN] = q2; q[id + 2 * NN] = q3;
-    }
-}
-
 template <int NGL>
 void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     using EB = ElemBlock<NGL>;
@@ -433,29 +327,13 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
         k_fused_av<NGL, true><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
             A.Nelem, ctx->p.PosPresTOL, sc, A.estat, A.audit, A.counters, A.EdgeData, A.q, A.B, A.VdmInv, A.Dhat,
             A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta, A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara,
-            A.Qt);
+            F.qvisc);
     else
         k_fused_av<NGL, false><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
             A.Nelem, ctx->p.PosPresTOL, sc, nullptr, 0, nullptr, A.EdgeData, A.q, A.B, A.VdmInv, A.Dhat, A.Dstrong,
-            A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta, A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, A.Qt);
+            A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta, A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, F.qvisc);
 }
 
-template <int NGL>
-void launch_fused_update(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F, double rkA, double cdt, bool visc,
-                         bool pos) {
-    using EB = ElemBlock<NGL>;
-    const unsigned nb = blocks_for(A.Nelem, EB::EPB);
-    const double tol = ctx->p.PosPresTOL, zt = ctx->p.ZeroTOL;
-#define ESDG_UPD(V, P)                                                                                             \
-    k_fused_update<NGL, V, P><<<nb, EB::THREADS, 0, ctx->stream>>>(A.Nelem, rkA, cdt, tol, zt, A.Jac, A.ElemToEdge,  \
-                                                                   A.EMS, A.EOR, F.fvisc, A.Qt, A.gRK, A.q,          \
-                                                                   A.EleSizes, A.GLw)
-    if (visc && pos) ESDG_UPD(true, true);
-    else if (visc) ESDG_UPD(true, false);
-    else if (pos) ESDG_UPD(false, true);
-    else ESDG_UPD(false, false);
-#undef ESDG_UPD
-}
 
 int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     Nelem = Ne; Nfaces = Nf; av = with_av;
@@ -465,6 +343,7 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fgrad, fn * 4 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * 16 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fvisc, fn * 2 * sizeof(double));
+    if (!rc && av) rc = esdg_malloc(ctx, (void**)&qvisc, (size_t)Ne * 2 * ngl * ngl * sizeof(double));
     ready = (rc == ESDG_OK);
     return rc;
 }
@@ -494,12 +373,11 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     }
     if (!av) {
         // volume + surface + RK in one element pass; the limiter (if on) follows as its own pass
-        int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr);
+        int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr);
         if (!rc && A.positivity) rc = esdg_pos_pres(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q);
         return rc;
     }
-    int rc = fused_volume(ctx, A, false, rkA, cdt, fstar, A.Qt);
-    if (rc) return rc;
+    int rc = ESDG_OK;
     {
         KernelScope ks(ctx, "fused_av");
         ESDG_NGL_DISPATCH(ngl, launch_fused_av, ctx, A, *this);
@@ -513,16 +391,14 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         rc = launch_check(ctx, "fused_viscous_face");
         if (rc) return rc;
     }
-    {
-        KernelScope ks(ctx, "fused_update");
-        ESDG_NGL_DISPATCH(ngl, launch_fused_update, ctx, A, *this, rkA, cdt, true, A.positivity != 0);
-        rc = launch_check(ctx, "fused_update");
-    }
+    // volume + surface + viscous terms + RK in one element pass, then the limiter
+    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc);
+    if (!rc && A.positivity) rc = esdg_pos_pres(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q);
     return rc;
 }
 
 void FusedStage::release(esdg_ctx* ctx) {
-    for (double** p : {&qn, &fstar, &fgrad, &grad, &fvisc})
+    for (double** p : {&qn, &fstar, &fgrad, &grad, &fvisc, &qvisc})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
     ready = false;
 }
@@ -532,6 +408,7 @@ void* FusedStage::buffer(const std::string& name) {
     if (name == "FGrad") return fgrad;
     if (name == "Grad") return grad;
     if (name == "FVisc") return fvisc;
+    if (name == "QVisc") return qvisc;
     return nullptr;
 }
 

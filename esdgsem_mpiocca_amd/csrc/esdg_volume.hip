@@ -26,13 +26,21 @@ namespace esdg {
 //   VOL_SURF      add the surface lift of the face fluxes (okl/DG/SurfaceKernel.okl, DBSurf = 0), store Qt
 //   VOL_SURF_RK   as VOL_SURF, then the low-storage RK step gRK = A gRK + Qt, q += cdt gRK in place
 //                 (okl/RungeKutta/rkLS.okl + UpdateKernelLS.okl); Qt never reaches HBM
-enum VolMode { VOL_ONLY = 0, VOL_SURF = 1, VOL_SURF_RK = 2 };
+//   VOL_SURF_VISC_RK  as VOL_SURF_RK with the viscous terms of the artificial viscosity added
+//                 after the surface lift, in the reference's order: the viscous volume term
+//                 (VolumeKernelViscose.okl, precomputed per node; skipped where the element's
+//                 viscosity is zero, where it is +-0) and the viscous surface lift
+//                 (SurfaceKernelVisc.okl)
+enum VolMode { VOL_ONLY = 0, VOL_SURF = 1, VOL_SURF_RK = 2, VOL_SURF_VISC_RK = 3 };
 
 struct SurfIn {
     const int *E2E, *EMS, *EOR;
     const double* FStar;
     double rkA, cdt;
     double* gRK;
+    const double* Qv;      // viscous volume term [e][2][ngl^2] (hu, hv)
+    const double* visc;    // element viscosity
+    const double* SPV;     // viscous face fluxes [f][2][ngl]
 };
 
 template <int NGL, int MODE>
@@ -154,7 +162,39 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 }
             }
         }
-        if (MODE == VOL_SURF_RK) {
+        if (MODE == VOL_SURF_VISC_RK) {
+            if (si.visc[e] != 0.0) {
+                const double* qv = si.Qv + (int64_t)e * C::NN * 2 + r;
+#pragma unroll
+                for (int k = 0; k < NGL; ++k) {
+                    b2[k] += qv[k * NGL];
+                    b3[k] += qv[C::NN + k * NGL];
+                }
+            }
+#pragma unroll
+            for (int ph = 0; ph < 2; ++ph) {
+#pragma unroll
+                for (int k = 0; k < NGL; ++k) {
+                    int side = -1, t = 0;
+                    if (ph == 0) {
+                        if (k == 0) { side = 0; t = r; }
+                        else if (k == NGL - 1) { side = 2; t = r; }
+                    } else {
+                        if (r == NGL - 1) { side = 1; t = k; }
+                        else if (r == 0) { side = 3; t = k; }
+                    }
+                    if (side < 0) continue;
+                    const int ifa = si.E2E[e * 4 + side] - 1;
+                    const int rev = si.EOR[e * 4 + side];
+                    const int ms = si.EMS[e * 4 + side];
+                    const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
+                    const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
+                    b2[k] += -ms * rJ[k] * si.SPV[fl];
+                    b3[k] += -ms * rJ[k] * si.SPV[fl + NGL];
+                }
+            }
+        }
+        if (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
 #pragma unroll
             for (int k = 0; k < NGL; ++k) {
                 const int64_t id = ob + k * NGL;
@@ -245,10 +285,13 @@ void launch_volume_fluxdiff(esdg_ctx* ctx, int Nelem, const double* Jac, const d
 
 template <int NGL>
 void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar,
-                         double* Qt) {
+                         double* Qt, const double* Qv, const double* SPV) {
     using C = VolCfg<NGL>;
-    SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK};
-    if (rk)
+    SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV};
+    if (rk && Qv)
+        k_volume_fluxdiff<NGL, VOL_SURF_VISC_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
+            a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
+    else if (rk)
         k_volume_fluxdiff<NGL, VOL_SURF_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
     else
@@ -256,10 +299,12 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, Qt, si);
 }
 
-// Fused volume + surface lift (+ RK update in place when rk): called by the fused stage.
-int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt) {
+// Fused volume + surface lift (+ viscous terms when Qv is given) (+ RK update in place when rk):
+// called by the fused stage.
+int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
+                 const double* Qv, const double* SPV) {
     KernelScope ks(ctx, "fused_volume");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV);
     return launch_check(ctx, "fused_volume");
 }
 

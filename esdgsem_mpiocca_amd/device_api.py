@@ -125,6 +125,22 @@ class DeviceArray:
             self.ptr = None
 
 
+class DeviceView:
+    """Non-owning view of a device buffer owned by the library (e.g. a solver's state)."""
+
+    def __init__(self, ctx: "Context", ptr: int, shape, dtype):
+        self.ctx, self.ptr = ctx, ptr
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.ctx.check(lib().esdg_memcpy(self.ctx.handle, out.ctypes.data_as(C.c_void_p),
+                                         C.c_void_p(self.ptr), self.nbytes, ESDG_D2H))
+        return out
+
+
 class Context:
     """One HIP device + stream (esdg_ctx).  Kernel entries are called as ``ctx.<name>(...)``."""
 
@@ -151,7 +167,7 @@ class Context:
         fn = getattr(lib(), "esdg_" + name)
         cargs = [C.c_void_p(self.handle)]
         for a in args:
-            if isinstance(a, DeviceArray):
+            if isinstance(a, (DeviceArray, DeviceView)):
                 cargs.append(C.c_void_p(a.ptr))
             elif a is None:
                 cargs.append(C.c_void_p(0))
@@ -276,6 +292,9 @@ class Solver:
         self.ctx.check(lib().esdg_solver_indicator_counts(C.c_void_p(self.handle), *[C.byref(x) for x in v]))
         return dict(zip(("flagged", "false_negatives", "sensor_positives", "element_stages"),
                         [x.value for x in v]))
+
+    def view(self, name: str, shape, dtype=np.float64) -> DeviceView:
+        return DeviceView(self.ctx, self.buffer_ptr(name), shape, dtype)
 
     def buffer_ptr(self, name: str) -> int:
         p = C.c_void_p()
