@@ -189,6 +189,71 @@ __device__ __forceinline__ void flux_lf(const FluxConsts& c, double nx, double n
     NF2 = (nx * FF2 + ny * GG2) - 0.5 * lam * (hvR - hvL);
 }
 
+// Roe flux: okl/RiemannSolvers/calcRoeFlux.okl:17-230 (and F_Roe of calcCombinedFluxOcean.okl).
+// Dry sides keep h with zero velocity and momentum; |.| of a double is fabs.
+__device__ __forceinline__ void flux_roe(const FluxConsts& c, double nx, double ny, double hL, double q2L, double q3L,
+                                         double hR, double q2R, double q3R, double& NF0, double& NF1, double& NF2) {
+    const double g = c.g, half = 0.5;
+    double uL = 0.0, vL = 0.0, huL = 0.0, hvL = 0.0, uR = 0.0, vR = 0.0, huR = 0.0, hvR = 0.0;
+    if (hL > c.tol) { uL = q2L / hL; vL = q3L / hL; huL = q2L; hvL = q3L; }
+    if (hR > c.tol) { uR = q2R / hR; vR = q3R / hR; huR = q2R; hvR = q3R; }
+    const double F1L = huL, F1R = huR, G1L = hvL, G1R = hvR;
+    const double F2L = huL * uL + half * g * hL * hL;
+    const double F2R = (huR * uR + half * g * hR * hR);
+    const double G2L = huL * vL, G2R = huR * vR;
+    const double F3L = huL * vL, F3R = huR * vR;
+    const double G3L = (hvL * vL + half * g * hL * hL);
+    const double G3R = (hvR * vR + half * g * hR * hR);
+    const double rh = half * (hL + hR);
+    const double ru = (uL * sqrt(hL) + uR * sqrt(hR)) / (sqrt(hL) + sqrt(hR));
+    const double rv = (vL * sqrt(hL) + vR * sqrt(hR)) / (sqrt(hL) + sqrt(hR));
+    const double rc = sqrt(g * rh);
+    const double rn = nx * ru + ny * rv;
+    const double l1 = fabs(rn - rc), l2 = fabs(rn), l3 = fabs(rn + rc);
+    const double R11 = 1.0, R12 = 0.0, R13 = 1.0;
+    const double R21 = ru - rc * nx, R22 = -ny, R23 = ru + rc * nx;
+    const double R31 = rv - rc * ny, R32 = nx, R33 = rv + rc * ny;
+    const double o2c = half / rc;
+    const double L11 = l1 * (rn + rc) * o2c, L12 = l1 * (-nx * o2c), L13 = l1 * (-ny * o2c);
+    const double L21 = l2 * (-ny * ru + nx * rv), L22 = l2 * (-ny), L23 = l2 * (nx);
+    const double L31 = l3 * (-(rn - rc) * o2c), L32 = l3 * (nx * o2c), L33 = l3 * (ny * o2c);
+    const double A11 = R11 * L11 + R12 * L21 + R13 * L31;
+    const double A12 = R11 * L12 + R12 * L22 + R13 * L32;
+    const double A13 = R11 * L13 + R12 * L23 + R13 * L33;
+    const double A21 = R21 * L11 + R22 * L21 + R23 * L31;
+    const double A22 = R21 * L12 + R22 * L22 + R23 * L32;
+    const double A23 = R21 * L13 + R22 * L23 + R23 * L33;
+    const double A31 = R31 * L11 + R32 * L21 + R33 * L31;
+    const double A32 = R31 * L12 + R32 * L22 + R33 * L32;
+    const double A33 = R31 * L13 + R32 * L23 + R33 * L33;
+    const double dH = hR - hL, dHu = huR - huL, dHv = hvR - hvL;
+    const double d1 = A11 * dH + A12 * dHu + A13 * dHv;
+    const double d2 = A21 * dH + A22 * dHu + A23 * dHv;
+    const double d3 = A31 * dH + A32 * dHu + A33 * dHv;
+    NF0 = (nx * (half * (F1L + F1R)) + ny * (half * (G1L + G1R))) - half * d1;
+    NF1 = (nx * (half * (F2L + F2R)) + ny * (half * (G2L + G2R))) - half * d2;
+    NF2 = (nx * (half * (F3L + F3R)) + ny * (half * (G3L + G3R))) - half * d3;
+}
+
+// The numerical flux of one face node for NumFlux = kind (src/deviceclass.cpp:442-477).  cpuR is
+// EdgeData's right rank (-1 on a boundary face), bL the face's left bottom trace, bR the right one
+// at the node read by the flux.  Kind 5 (calcCombinedFluxOcean.okl:330-420) takes the Roe flux on
+// boundary faces over the open ocean (bL < 6.7), the rotated ES flux elsewhere.
+__device__ __forceinline__ void numerical_flux(int kind, const FluxConsts& c, int cpuR, double nx, double ny,
+                                               double hL, double huL, double hvL, double hR, double huR, double hvR,
+                                               double bL, double bR, double& F0, double& F1, double& F2) {
+    switch (kind) {
+        case ESDG_FLUX_ES_ROTATED: flux_es_rotated(c, nx, ny, hL, huL, hvL, hR, huR, hvR, bL, bR, F0, F1, F2); break;
+        case ESDG_FLUX_LF: flux_lf(c, nx, ny, hL, huL, hvL, hR, huR, hvR, F0, F1, F2); break;
+        case ESDG_FLUX_ROE: flux_roe(c, nx, ny, hL, huL, hvL, hR, huR, hvR, F0, F1, F2); break;
+        case ESDG_FLUX_OCEAN:
+            if (cpuR == -1 && bL < 6.7) flux_roe(c, nx, ny, hL, huL, hvL, hR, huR, hvR, F0, F1, F2);
+            else flux_es_rotated(c, nx, ny, hL, huL, hvL, hR, huR, hvR, bL, bR, F0, F1, F2);
+            break;
+        default: flux_es(c, nx, ny, hL, huL, hvL, hR, huR, hvR, bL, bR, F0, F1, F2); break;
+    }
+}
+
 __device__ __forceinline__ int surf_node(int ngl, int side, int k) {
     switch (side) {
         case 0: return k;

@@ -104,9 +104,9 @@ class deviceclass {
         if (rp_.fused == 2 && (ngl < 3 || ngl > 8))
             return set_error(ctx_, ESDG_ERR_ARG, "the single-kernel stage supports 2 <= N <= 7");
         const int nf = ctx_->p.num_flux;
-        if (nf != ESDG_FLUX_ES && nf != ESDG_FLUX_ES_ROTATED && nf != ESDG_FLUX_LF)
-            return set_error(ctx_, ESDG_ERR_ARG, "num_flux must be 0, 1 or 3");
-        if (rp_.fused && nf == ESDG_FLUX_LF) return set_error(ctx_, ESDG_ERR_ARG, "the fused stage supports the ES fluxes");
+        if (nf != ESDG_FLUX_ES && nf != ESDG_FLUX_ES_ROTATED && nf != ESDG_FLUX_LF && nf != ESDG_FLUX_ROE &&
+            nf != ESDG_FLUX_OCEAN)
+            return set_error(ctx_, ESDG_ERR_ARG, "num_flux must be 0, 1, 3, 4 or 5");
         // RK coefficients (src/RungeKutta.cpp:18-90)
         if (rp_.rk_order == 4) {
             stages = 5;

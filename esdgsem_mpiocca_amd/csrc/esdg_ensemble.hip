@@ -71,8 +71,8 @@ int check_scenario(esdg_ctx* ctx, int i, const esdg_params& p, const esdg_inputs
     if (p.ngl < 3 || p.ngl > 8) return set_error(ctx, ESDG_ERR_ARG, at + "needs 2 <= N <= 7");
     if (in.Nelem <= 0 || in.Nfaces <= 0) return set_error(ctx, ESDG_ERR_ARG, at + "empty mesh");
     if ((int64_t)in.Nelem * p.ngl * p.ngl * 3 >= (int64_t)INT32_MAX * 8) return set_error(ctx, ESDG_ERR_ARG, at + "mesh too large");
-    if (p.num_flux != ESDG_FLUX_ES && p.num_flux != ESDG_FLUX_ES_ROTATED)
-        return set_error(ctx, ESDG_ERR_ARG, at + "num_flux must be 0 (ES) or 3 (rotated ES)");
+    if (p.num_flux < ESDG_FLUX_ES || p.num_flux > ESDG_FLUX_OCEAN || p.num_flux == 2)
+        return set_error(ctx, ESDG_ERR_ARG, at + "num_flux must be 0, 1, 3, 4 or 5");
     if (p.bc_kind < ESDG_BC_WALLS || p.bc_kind > ESDG_BC_OCEAN) return set_error(ctx, ESDG_ERR_ARG, at + "bad bc_kind");
     if (r.artificial_viscosity) return set_error(ctx, ESDG_ERR_ARG, at + "artificial viscosity is not supported");
     if (!r.flux_differencing) return set_error(ctx, ESDG_ERR_ARG, at + "needs flux differencing");

@@ -47,10 +47,7 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
     const FaceState L = gathered(q, ngl, E, true, i, nx, ny, gc, BL[t], BR[t]);
     const FaceState R = gathered(q, ngl, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
     double F0, F1, F2;
-    if (kind == ESDG_FLUX_ES_ROTATED)
-        flux_es_rotated(c, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
-    else
-        flux_es(c, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
+    numerical_flux(kind, c, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
     const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
     SP[eid] = sc * c.geomFace * F0;
     SP[eid + ngl] = sc * c.geomFace * F1;

@@ -92,6 +92,7 @@ __global__ void __launch_bounds__(256) k_collect(int Nfaces, int ngl, int bc, do
 // formed in the reference's order.
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_num_fluxes(int Nfaces, int ngl, int kind, FluxConsts c,
+                                                    const int* __restrict__ ED,
                                                     const int* __restrict__ ER, const double* __restrict__ nxa,
                                                     const double* __restrict__ nya, const double* __restrict__ scal,
                                                     const double* __restrict__ qL, const double* __restrict__ qR,
@@ -109,12 +110,8 @@ __global__ void __launch_bounds__(256) k_num_fluxes(int Nfaces, int ngl, int kin
     double F0, F1, F2;
     const double hL = qL[eid], huL = qL[eid + ngl], hvL = qL[eid + 2 * ngl];
     const double hR = qR[eidR], huR = qR[eidR + ngl], hvR = qR[eidR + 2 * ngl];
-    if (kind == ESDG_FLUX_ES_ROTATED)
-        flux_es_rotated(c, nx, ny, hL, huL, hvL, hR, huR, hvR, BL[t], BR[xidR], F0, F1, F2);
-    else if (kind == ESDG_FLUX_LF)
-        flux_lf(c, nx, ny, hL, huL, hvL, hR, huR, hvR, F0, F1, F2);
-    else
-        flux_es(c, nx, ny, hL, huL, hvL, hR, huR, hvR, BL[t], BR[xidR], F0, F1, F2);
+    const int cpuR = (kind == ESDG_FLUX_OCEAN) ? ED[8 * (int64_t)f + 6] : 0;
+    numerical_flux(kind, c, cpuR, nx, ny, hL, huL, hvL, hR, huR, hvR, BL[t], BR[xidR], F0, F1, F2);
     SP[eid] = sc * c.geomFace * F0;
     SP[eid + ngl] = sc * c.geomFace * F1;
     SP[eid + 2 * ngl] = sc * c.geomFace * F2;
@@ -628,19 +625,20 @@ int esdg_collect_edge_data(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const
 int esdg_calc_num_fluxes(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const int* EdgeReversed, const double* nx,
                          const double* ny, const double* scal, const double* qL, const double* qR, const double* bL,
                          const double* bR, double* SurfaceParts) {
-    (void)EdgeData;
     ESDG_REQUIRE(ctx, ctx, "null context");
     ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeReversed && nx && ny && scal && qL && qR && bL && bR && SurfaceParts,
                  "esdg_calc_num_fluxes: bad argument");
     const int kind = ctx->p.num_flux;
-    ESDG_REQUIRE(ctx, kind == ESDG_FLUX_ES || kind == ESDG_FLUX_ES_ROTATED || kind == ESDG_FLUX_LF,
-                 "esdg_calc_num_fluxes: num_flux must be 0 (ES), 1 (LF) or 3 (rotated ES)");
+    ESDG_REQUIRE(ctx, kind == ESDG_FLUX_ES || kind == ESDG_FLUX_ES_ROTATED || kind == ESDG_FLUX_LF ||
+                          kind == ESDG_FLUX_ROE || kind == ESDG_FLUX_OCEAN,
+                 "esdg_calc_num_fluxes: num_flux must be 0 (ES), 1 (LF), 3 (rotated ES), 4 (Roe) or 5 (ocean)");
+    ESDG_REQUIRE(ctx, kind != ESDG_FLUX_OCEAN || EdgeData, "esdg_calc_num_fluxes: EdgeData needed for num_flux 5");
     if (Nfaces == 0) return ESDG_OK;
     ESDG_ENTRY("calc_num_fluxes");
     const int ngl = ctx->p.ngl;
     FluxConsts c{ctx->p.g, ctx->p.geomFace, ctx->p.PosPresTOL, ctx->p.ES};
-    k_num_fluxes<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(Nfaces, ngl, kind, c, EdgeReversed, nx, ny,
-                                                                                   scal, qL, qR, bL, bR, SurfaceParts);
+    k_num_fluxes<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(
+        Nfaces, ngl, kind, c, EdgeData, EdgeReversed, nx, ny, scal, qL, qR, bL, bR, SurfaceParts);
     return launch_check(ctx, "calc_num_fluxes");
 }
 

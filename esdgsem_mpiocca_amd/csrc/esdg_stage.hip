@@ -285,10 +285,8 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
             const FaceState L = gathered(qi, NGL, E, true, i, nx, ny, gc, S.bL[t], S.bR[t]);
             const FaceState R = gathered(qi, NGL, E, false, ir, S.nx[tR], S.ny[tR], gc, S.bL[tR], S.bR[tR]);
             double F0, F1, F2;
-            if (S.num_flux == ESDG_FLUX_ES_ROTATED)
-                flux_es_rotated(fc, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, S.bL[t], S.bR[tR], F0, F1, F2);
-            else
-                flux_es(fc, nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, S.bL[t], S.bR[tR], F0, F1, F2);
+            numerical_flux(S.num_flux, fc, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, S.bL[t], S.bR[tR], F0, F1,
+                           F2);
             const double nms = -(double)S.EMS[e * 4 + s];
             FS[(s * 3 + 0) * NGL + r] = nms * (sc * S.geomFace * F0);
             FS[(s * 3 + 1) * NGL + r] = nms * (sc * S.geomFace * F1);

@@ -39,7 +39,9 @@ enum esdg_num_flux {
     ESDG_FLUX_ES = 0,          /* okl/RiemannSolvers/calcESFlux.okl */
     ESDG_FLUX_LF = 1,          /* okl/RiemannSolvers/calcLaxFriedrich.okl */
     ESDG_FLUX_ES_ROTATED = 3,  /* okl/RiemannSolvers/calcESFluxRotated.okl */
-    ESDG_FLUX_ROE = 4          /* okl/RiemannSolvers/calcRoeFlux.okl */
+    ESDG_FLUX_ROE = 4,         /* okl/RiemannSolvers/calcRoeFlux.okl */
+    ESDG_FLUX_OCEAN = 5        /* okl/RiemannSolvers/calcCombinedFluxOcean.okl (Roe on open-ocean
+                                  boundary faces, rotated ES elsewhere) */
 };
 
 /* Boundary condition of CollectEdgeData (src/deviceclass.cpp:363-415). */

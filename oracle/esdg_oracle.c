@@ -374,6 +374,96 @@ void oracle_num_flux_es_rotated(const oracle_params* p, int Nfaces, const int* E
     }
 }
 
+/* Roe flux at one face node (Cartesian momentum): okl/RiemannSolvers/calcRoeFlux.okl:17-230 and
+ * F_Roe of calcCombinedFluxOcean.okl:17-140.  Dry sides (h <= tol) keep h, zero velocity and
+ * momentum; |.| of a double is fabs. */
+static void roe_node(const oracle_params* p, double nx, double ny, double hL, double q2L, double q3L,
+                     double hR, double q2R, double q3R, double* F) {
+    const double g = p->g, tol = p->PosPresTOL, half = 0.5;
+    double uL = 0.0, vL = 0.0, huL = 0.0, hvL = 0.0, uR = 0.0, vR = 0.0, huR = 0.0, hvR = 0.0;
+    if (hL > tol) { uL = q2L / hL; vL = q3L / hL; huL = q2L; hvL = q3L; }
+    if (hR > tol) { uR = q2R / hR; vR = q3R / hR; huR = q2R; hvR = q3R; }
+    const double F1L = huL, F1R = huR, G1L = hvL, G1R = hvR;
+    const double F2L = huL * uL + half * g * hL * hL;
+    const double F2R = (huR * uR + half * g * hR * hR);
+    const double G2L = huL * vL, G2R = huR * vR;
+    const double F3L = huL * vL, F3R = huR * vR;
+    const double G3L = (hvL * vL + half * g * hL * hL);
+    const double G3R = (hvR * vR + half * g * hR * hR);
+    const double rh = half * (hL + hR);
+    const double ru = (uL * sqrt(hL) + uR * sqrt(hR)) / (sqrt(hL) + sqrt(hR));
+    const double rv = (vL * sqrt(hL) + vR * sqrt(hR)) / (sqrt(hL) + sqrt(hR));
+    const double rc = sqrt(g * rh);
+    const double rn = nx * ru + ny * rv;
+    const double l1 = fabs(rn - rc), l2 = fabs(rn), l3 = fabs(rn + rc);
+    const double R11 = 1.0, R12 = 0.0, R13 = 1.0;
+    const double R21 = ru - rc * nx, R22 = -ny, R23 = ru + rc * nx;
+    const double R31 = rv - rc * ny, R32 = nx, R33 = rv + rc * ny;
+    const double o2c = half / rc;
+    const double L11 = l1 * (rn + rc) * o2c, L12 = l1 * (-nx * o2c), L13 = l1 * (-ny * o2c);
+    const double L21 = l2 * (-ny * ru + nx * rv), L22 = l2 * (-ny), L23 = l2 * (nx);
+    const double L31 = l3 * (-(rn - rc) * o2c), L32 = l3 * (nx * o2c), L33 = l3 * (ny * o2c);
+    const double A11 = R11 * L11 + R12 * L21 + R13 * L31;
+    const double A12 = R11 * L12 + R12 * L22 + R13 * L32;
+    const double A13 = R11 * L13 + R12 * L23 + R13 * L33;
+    const double A21 = R21 * L11 + R22 * L21 + R23 * L31;
+    const double A22 = R21 * L12 + R22 * L22 + R23 * L32;
+    const double A23 = R21 * L13 + R22 * L23 + R23 * L33;
+    const double A31 = R31 * L11 + R32 * L21 + R33 * L31;
+    const double A32 = R31 * L12 + R32 * L22 + R33 * L32;
+    const double A33 = R31 * L13 + R32 * L23 + R33 * L33;
+    const double dH = hR - hL, dHu = huR - huL, dHv = hvR - hvL;
+    const double d1 = A11 * dH + A12 * dHu + A13 * dHv;
+    const double d2 = A21 * dH + A22 * dHu + A23 * dHv;
+    const double d3 = A31 * dH + A32 * dHu + A33 * dHv;
+    F[0] = (nx * (half * (F1L + F1R)) + ny * (half * (G1L + G1R))) - half * d1;
+    F[1] = (nx * (half * (F2L + F2R)) + ny * (half * (G2L + G2R))) - half * d2;
+    F[2] = (nx * (half * (F3L + F3R)) + ny * (half * (G3L + G3R))) - half * d3;
+}
+
+/* NumFlux 4: okl/RiemannSolvers/calcRoeFlux.okl */
+void oracle_num_flux_roe(const oracle_params* p, int Nfaces, const int* EdgeReversed,
+                         const double* nxa, const double* nya, const double* scal, const double* qL,
+                         const double* qR, const double* BL, const double* BR, double* SP) {
+    (void)BL; (void)BR;
+    const int ngl = p->ngl;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int rv = EdgeReversed[f];
+        for (int i = 0; i < ngl; ++i) {
+            const long xid = (long)f * ngl + i;
+            const long eid = (long)f * ngl * NEQ + i;
+            const long eidR = (long)f * ngl * NEQ + rev_idx(ngl, rv, i);
+            double F[3];
+            roe_node(p, nxa[xid], nya[xid], qL[eid], qL[eid + ngl], qL[eid + 2 * ngl], qR[eidR], qR[eidR + ngl],
+                     qR[eidR + 2 * ngl], F);
+            for (int v = 0; v < 3; ++v) SP[eid + v * ngl] = scal[xid] * p->geomFace * F[v];
+        }
+    }
+}
+
+/* NumFlux 5: okl/RiemannSolvers/calcCombinedFluxOcean.okl:330-420 -- the Roe flux on boundary
+ * faces whose left bottom trace is below 6.7 (the open ocean), the rotated ES flux elsewhere. */
+void oracle_num_flux_ocean(const oracle_params* p, int Nfaces, const int* EdgeData, const int* EdgeReversed,
+                           const double* nxa, const double* nya, const double* scal, const double* qL,
+                           const double* qR, const double* BL, const double* BR, double* SP) {
+    const int ngl = p->ngl;
+    oracle_num_flux_es_rotated(p, Nfaces, EdgeReversed, nxa, nya, scal, qL, qR, BL, BR, SP);
+    for (int f = 0; f < Nfaces; ++f) {
+        if (EdgeData[8 * f + 6] != -1) continue;
+        const int rv = EdgeReversed[f];
+        for (int i = 0; i < ngl; ++i) {
+            const long xid = (long)f * ngl + i;
+            if (!(BL[xid] < 6.7)) continue;
+            const long eid = (long)f * ngl * NEQ + i;
+            const long eidR = (long)f * ngl * NEQ + rev_idx(ngl, rv, i);
+            double F[3];
+            roe_node(p, nxa[xid], nya[xid], qL[eid], qL[eid + ngl], qL[eid + 2 * ngl], qR[eidR], qR[eidR + ngl],
+                     qR[eidR + 2 * ngl], F);
+            for (int v = 0; v < 3; ++v) SP[eid + v * ngl] = scal[xid] * p->geomFace * F[v];
+        }
+    }
+}
+
 /* node index and face-node index of trace point k of side s, for the surface kernels
  * (okl/DG/SurfaceKernel.okl:34-72 sides 0/2, :92-130 sides 1/3) */
 static int surf_node(int ngl, int side, int k) {
@@ -869,6 +959,12 @@ int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, doub
             oracle_volume_fluxdiff(p, Ne, r->Jac, r->Yxi, r->Yeta, r->Xxi, r->Xeta, q, r->D, r->Bx, r->By, Qt);
             if (r->num_flux == 3)
                 oracle_num_flux_es_rotated(p, Nf, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);
+            else if (r->num_flux == 1)
+                oracle_num_flux_lf(p, Nf, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);
+            else if (r->num_flux == 4)
+                oracle_num_flux_roe(p, Nf, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);
+            else if (r->num_flux == 5)
+                oracle_num_flux_ocean(p, Nf, r->EdgeData, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);
             else
                 oracle_num_flux_es(p, Nf, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);
             oracle_surface(p, Ne, r->Jac, r->EMS, r->EOR, r->ElemToEdge, SP, DB, DB, Qt);

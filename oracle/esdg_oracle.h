@@ -46,6 +46,12 @@ void oracle_num_flux_es_rotated(const oracle_params* p, int Nfaces, const int* E
                                 const double* nx, const double* ny, const double* scal,
                                 const double* qL, const double* qR, const double* bL,
                                 const double* bR, double* SurfaceParts);
+void oracle_num_flux_roe(const oracle_params* p, int Nfaces, const int* EdgeReversed,
+                         const double* nx, const double* ny, const double* scal, const double* qL,
+                         const double* qR, const double* bL, const double* bR, double* SP);
+void oracle_num_flux_ocean(const oracle_params* p, int Nfaces, const int* EdgeData, const int* EdgeReversed,
+                           const double* nx, const double* ny, const double* scal, const double* qL,
+                           const double* qR, const double* bL, const double* bR, double* SP);
 void oracle_num_flux_lf(const oracle_params* p, int Nfaces, const int* EdgeReversed,
                         const double* nx, const double* ny, const double* scal, const double* qL,
                         const double* qR, const double* bL, const double* bR, double* SurfaceParts);

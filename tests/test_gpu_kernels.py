@@ -122,7 +122,7 @@ def test_volume_kernel(S):
         assert rel_l2(a, b) <= 1e-13, (v, rel_l2(a, b))
 
 
-@pytest.mark.parametrize("flux", [0, 1, 3])
+@pytest.mark.parametrize("flux", [0, 1, 3, 4, 5])
 def test_calc_num_fluxes(dev, flux):
     _, case, rp = CASES[1]
     rp = RunParams(**{**rp.__dict__, "num_flux": flux})
@@ -134,8 +134,14 @@ def test_calc_num_fluxes(dev, flux):
         bL, bR = np.zeros(fn), np.zeros(fn)
         O.call("collect_edge_data_bottom", S.p, S.Nf, S.h["EdgeData"], S.h["B"], bL, bR)
         SP = np.zeros(fn * 3)
-        name = {0: "num_flux_es", 1: "num_flux_lf", 3: "num_flux_es_rotated"}[flux]
-        O.call(name, S.p, S.Nf, S.h["ER"], S.h["nx"], S.h["ny"], S.h["scal"], qL, qR, bL, bR, SP)
+        if flux == 5:
+            # boundary bottom traces on both sides of 6.7 so that both branches are taken
+            bL = np.where(np.arange(fn) % 2 == 0, bL + 5.0, bL + 8.0)
+            O.call("num_flux_ocean", S.p, S.Nf, S.h["EdgeData"], S.h["ER"], S.h["nx"], S.h["ny"], S.h["scal"],
+                   qL, qR, bL, bR, SP)
+        else:
+            name = {0: "num_flux_es", 1: "num_flux_lf", 3: "num_flux_es_rotated", 4: "num_flux_roe"}[flux]
+            O.call(name, S.p, S.Nf, S.h["ER"], S.h["nx"], S.h["ny"], S.h["scal"], qL, qR, bL, bR, SP)
         dqL, dqR, dbL, dbR = (S.ctx.array(a) for a in (qL, qR, bL, bR))
         dSP = S.ctx.empty(fn * 3)
         S.ctx.calc_num_fluxes(S.Nf, S.d["EdgeData"], S.d["ER"], S.d["nx"], S.d["ny"], S.d["scal"], dqL, dqR,

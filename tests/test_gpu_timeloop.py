@@ -108,6 +108,20 @@ def test_ocean_uplift_open_boundary(dev, fused):
     assert max(h, hu, hv) <= 1e-12, (h, hu, hv)
 
 
+@pytest.mark.parametrize("fused", SCHEDULES, ids=SCHEDULE_IDS)
+@pytest.mark.parametrize("num_flux", [4, 5], ids=["roe", "ocean_combined"])
+def test_roe_and_combined_ocean_flux(dev, fused, num_flux):
+    """NumFlux 4 (Roe) and 5 (Roe on the open-ocean boundary, rotated ES elsewhere) over the ocean
+    uplift with its open/wall boundary, 20 steps."""
+    case = Case.ocean_uplift(N=4, nel=8, amp=3.0, axis_a=90e3, axis_c=70e3, angle=0.4)
+    rp = RunParams(num_flux=num_flux, bc_kind=2, h_zero=4000.0)
+    qo, no, to, _ = O.timeloop(case, rp, 20)
+    qg, ng, tg = run_gpu(dev, case, rp, 20, fused=fused)
+    assert ng == no == 20 and abs(tg - to) <= 1e-14 * to
+    errs = fields_rel_l2(case, qg, qo)
+    assert max(errs) <= 1e-12, errs
+
+
 def test_entropy_conservative_flux(dev):
     """ES=0 (EC surface flux): total entropy changes only by the time-integration error."""
     case = Case.gaussian_bump(N=4, nel=8)

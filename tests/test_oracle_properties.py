@@ -129,6 +129,73 @@ def test_surface_flux_consistency_and_entropy(name):
     assert ((jv * (Fes - Fec)).sum(1) < -1e-6).mean() > 0.9
 
 
+def supersonic_faces(M, seed, sign, tangential=True):
+    """Faces with both states moving along sign*n faster than the largest wave speed."""
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0, 2 * np.pi, M)
+    n = np.stack([np.cos(th), np.sin(th)], 1)
+    t = np.stack([-n[:, 1], n[:, 0]], 1)
+    cmax = np.sqrt(G * 1.0)
+    out = []
+    for _ in range(2):
+        h = rng.uniform(0.5, 1.0, M)
+        un = sign * rng.uniform(2.0, 3.0, M) * cmax
+        ut = rng.uniform(-1, 1, M) if tangential else np.zeros(M)
+        vel = un[:, None] * n + ut[:, None] * t
+        out.append(np.stack([h, h * vel[:, 0], h * vel[:, 1]], 1))
+    return out[0], out[1], n
+
+
+def test_roe_flux_consistency_and_upwinding():
+    """calcRoeFlux.okl: F(u, u) = n.f(u); with every Roe eigenvalue positive (negative) and no
+    tangential velocity the flux is the left (right) physical flux -- Roe's property of the
+    sqrt(h)-weighted averages.  With tangential velocity it is not: the okl's second left
+    eigenvector carries +u_t where -u_t belongs (calcRoeFlux.okl:161, calcCombinedFluxOcean.okl:120),
+    and the oracle keeps that sign, so the defect shows exactly there."""
+    qL, qR, n = random_faces(400, 4)
+    scale = G * 4.0
+    assert np.abs(flux("num_flux_roe", qL, qL, n) - phys_flux(qL, n)).max() < 1e-14 * scale
+    for sign in (1.0, -1.0):
+        qL, qR, n = supersonic_faces(300, 5, sign, tangential=False)
+        F = flux("num_flux_roe", qL, qR, n)
+        ref = phys_flux(qL, n) if sign > 0 else phys_flux(qR, n)
+        assert np.abs(F - ref).max() < 1e-12 * scale, np.abs(F - ref).max()
+        qL, qR, n = supersonic_faces(300, 5, sign, tangential=True)
+        F = flux("num_flux_roe", qL, qR, n)
+        ref = phys_flux(qL, n) if sign > 0 else phys_flux(qR, n)
+        assert np.abs(F[:, 0] - ref[:, 0]).max() < 1e-12 * scale      # mass: the sign does not enter
+        assert np.abs(F[:, 1:] - ref[:, 1:]).max() > 1e-3
+    # subsonic: the Roe dissipation is not zero
+    qL, qR, n = random_faces(400, 6)
+    F = flux("num_flux_roe", qL, qR, n)
+    assert np.abs(F - 0.5 * (phys_flux(qL, n) + phys_flux(qR, n))).max() > 1e-3
+
+
+def test_ocean_flux_selects_roe_on_open_boundary_faces():
+    """calcCombinedFluxOcean.okl: Roe on boundary faces with bL < 6.7, rotated ES everywhere else."""
+    M = 500
+    qL, qR, n = random_faces(M, 7)
+    rng = np.random.default_rng(8)
+    bL = rng.uniform(0.0, 20.0, M)
+    bR = rng.uniform(0.0, 20.0, M)
+    ED = np.zeros((M, 8), np.int32)
+    bnd = rng.uniform(size=M) < 0.5
+    ED[bnd, 6] = -1
+    p = O.OracleParams(1, 1, G, 1.0, 1e-12, 1e-5, 0.0, 0.0, 0.0)
+    QL = np.ascontiguousarray(qL.reshape(-1))
+    QR = np.ascontiguousarray(qR.reshape(-1))
+    ones = np.ones(M, np.int32)
+    nx, ny = np.ascontiguousarray(n[:, 0]), np.ascontiguousarray(n[:, 1])
+    SP = np.zeros(3 * M)
+    O.call("num_flux_ocean", p, M, np.ascontiguousarray(ED.reshape(-1)), ones, nx, ny, np.ones(M), QL, QR, bL, bR, SP)
+    roe = flux("num_flux_roe", qL, qR, n, bL, bR)
+    rot = flux("num_flux_es_rotated", qL, qR, n, bL, bR)
+    sel = bnd & (bL < 6.7)
+    assert sel.any() and (~sel).any()
+    expect = np.where(sel[:, None], roe, rot)
+    assert np.array_equal(SP.reshape(M, 3), expect)
+
+
 def test_es_dissipation_matrix_independent():
     """F_ES - F_EC = -1/2 R |Lambda| R^T [v] with the eigen-structure of calcESFlux.okl."""
     qL, qR, n = random_faces(200, 5)
