@@ -97,6 +97,8 @@ class deviceclass {
         if (rp_.rk_order == 4 && rp_.rk_ssp) return set_error(ctx_, ESDG_ERR_ARG, "rk_order 4 has no SSP variant (rkSSP must be 0)");
         if (rp_.artificial_viscosity && ngl < 3) return set_error(ctx_, ESDG_ERR_ARG, "artificial viscosity needs N >= 2");
         if (rp_.fused < 0 || rp_.fused > 2) return set_error(ctx_, ESDG_ERR_ARG, "fused must be 0, 1 or 2");
+        if (rp_.fused && ctx_->p.bc_kind == ESDG_BC_CONVERGENCE)
+            return set_error(ctx_, ESDG_ERR_ARG, "the convergence-test boundary runs on the reference kernel sequence");
         if (rp_.fused && (rp_.rk_ssp || !rp_.flux_differencing))
             return set_error(ctx_, ESDG_ERR_ARG, "the fused stage supports flux differencing with low-storage RK only");
         if (rp_.fused == 2 && rp_.artificial_viscosity)
@@ -203,6 +205,11 @@ class deviceclass {
                                                        o_qGradientYR, o_SurfacePartsVisc);
             if (!rc) rc = esdg_surface_kernel_visc(ctx_, Nelem, o_Jac, o_ElemEdgeMasterSlave, o_ElemEdgeOrientation,
                                                    o_ElemToEdge, o_SurfacePartsVisc, o_Qt);
+            if (rc) return rc;
+        }
+        // manufactured source of the convergence test (src/deviceclass.cpp:1098-1102)
+        if (ctx_->p.bc_kind == ESDG_BC_CONVERGENCE) {
+            rc = esdg_add_source(ctx_, Nelem, o_Bx, o_By, o_B, o_x, o_y, t + (rp_.rk_ssp ? rkD[s] : Bc[s]) * dt, o_Qt);
             if (rc) return rc;
         }
         if (rp_.rk_ssp) {

@@ -73,7 +73,8 @@ int check_scenario(esdg_ctx* ctx, int i, const esdg_params& p, const esdg_inputs
     if ((int64_t)in.Nelem * p.ngl * p.ngl * 3 >= (int64_t)INT32_MAX * 8) return set_error(ctx, ESDG_ERR_ARG, at + "mesh too large");
     if (p.num_flux < ESDG_FLUX_ES || p.num_flux > ESDG_FLUX_OCEAN || p.num_flux == 2)
         return set_error(ctx, ESDG_ERR_ARG, at + "num_flux must be 0, 1, 3, 4 or 5");
-    if (p.bc_kind < ESDG_BC_WALLS || p.bc_kind > ESDG_BC_OCEAN) return set_error(ctx, ESDG_ERR_ARG, at + "bad bc_kind");
+    if (p.bc_kind < ESDG_BC_WALLS || p.bc_kind > ESDG_BC_OCEAN)
+        return set_error(ctx, ESDG_ERR_ARG, at + "bc_kind must be walls, inner/outer or ocean");
     if (r.artificial_viscosity) return set_error(ctx, ESDG_ERR_ARG, at + "artificial viscosity is not supported");
     if (!r.flux_differencing) return set_error(ctx, ESDG_ERR_ARG, at + "needs flux differencing");
     if (r.rk_ssp || (r.rk_order != 3 && r.rk_order != 4)) return set_error(ctx, ESDG_ERR_ARG, at + "needs low-storage RK 3 or 4");

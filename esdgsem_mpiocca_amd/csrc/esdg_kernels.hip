@@ -39,8 +39,18 @@ __global__ void __launch_bounds__(256) k_collect_bottom(int Nfaces, int ngl, con
 // CollectEdgeData: okl/GatherEdgeData/SolidWalls.okl:16-207 (walls), InnerOuter.okl,
 // OceanBoundary.okl (exterior state variants)
 // ---------------------------------------------------------------------------------------------
+// exterior state of the convergence test: Qnodal of okl/GatherEdgeData/Dirichlet_ConvTest.okl:17-31
+__device__ __forceinline__ void manufactured_state(double x, double y, double t, double b, double& q1, double& q2,
+                                                   double& q3) {
+    q1 = (8.0 + cos(x) * sin(y) * cos(t) - b);
+    q2 = 0.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
+    q3 = 1.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
+}
+
 __global__ void __launch_bounds__(256) k_collect(int Nfaces, int ngl, int bc, double h_zero, const int* __restrict__ ED,
-                                                 const double* __restrict__ q, const double* __restrict__ nxa,
+                                                 const double* __restrict__ q, const double* __restrict__ xa,
+                                                 const double* __restrict__ ya, double time,
+                                                 const double* __restrict__ nxa,
                                                  const double* __restrict__ nya, const double* __restrict__ bLa,
                                                  const double* __restrict__ bRa, double* __restrict__ qL,
                                                  double* __restrict__ qR) {
@@ -63,6 +73,16 @@ __global__ void __launch_bounds__(256) k_collect(int Nfaces, int ngl, int bc, do
     if (bc == ESDG_BC_WALLS) {
         if (cpuL == -1) { L0 = R0; wall_mirror(nx, ny, R1, R2, L1, L2); }
         if (cpuR == -1) { R0 = L0; wall_mirror(nx, ny, L1, L2, R1, R2); }
+    } else if (bc == ESDG_BC_CONVERGENCE) {
+        // Dirichlet_ConvTest.okl:150-165: the manufactured solution at the interior node's position
+        if (cpuL == -1) {
+            const int64_t xid = (int64_t)ier * ngl2 + side_node(ngl, isr, i);
+            manufactured_state(xa[xid], ya[xid], time, bRa[t], L0, L1, L2);
+        }
+        if (cpuR == -1) {
+            const int64_t xid = (int64_t)iel * ngl2 + side_node(ngl, isl, i);
+            manufactured_state(xa[xid], ya[xid], time, bLa[t], R0, R1, R2);
+        }
     } else if (bc == ESDG_BC_INNER_OUTER) {
         // okl/GatherEdgeData/InnerOuter.okl: exterior keeps h, momentum zero
         if (cpuL == -1) { L0 = R0; L1 = 0.0; L2 = 0.0; }
@@ -115,6 +135,29 @@ __global__ void __launch_bounds__(256) k_num_fluxes(int Nfaces, int ngl, int kin
     SP[eid] = sc * c.geomFace * F0;
     SP[eid + ngl] = sc * c.geomFace * F1;
     SP[eid + 2 * ngl] = sc * c.geomFace * F2;
+}
+
+// Manufactured source of the convergence test, okl/ManufacturedSolutions/S_ConvTest.okl:17-95
+// (h = 8 + cos x sin y cos t - b, u = 1/2, v = 3/2), added to Qt at node (e, n).
+__global__ void __launch_bounds__(256) k_add_source(int Nelem, int ngl, double g, const double* __restrict__ Bx,
+                                                    const double* __restrict__ By, const double* __restrict__ B,
+                                                    const double* __restrict__ xa, const double* __restrict__ ya,
+                                                    double t, double* __restrict__ Qt) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ngl2 = ngl * ngl;
+    if (k >= (int64_t)Nelem * ngl2) return;
+    const int64_t e = k / ngl2, n = k % ngl2;
+    const double sinY = sin(ya[k]), sinX = sin(xa[k]), cosY = cos(ya[k]), cosX = cos(xa[k]);
+    const double H = 8.0 + cosX * sinY * cos(t);
+    const double H_t = -cosX * sinY * sin(t);
+    const double H_x = -sinX * sinY * cos(t);
+    const double H_y = cosX * cosY * cos(t);
+    const double u = 0.5, v = 1.5;
+    const double bx = Bx[k], by = By[k], b = B[k];
+    double* o = Qt + e * ngl2 * ESDG_NEQ + n;
+    o[0] += H_t + u * (H_x - bx) + v * (H_y - by);
+    o[ngl2] += u * H_t + u * u * (H_x - bx) + u * v * (H_y - by) + g * H_x * (H - b);
+    o[2 * ngl2] += v * H_t + u * v * (H_x - bx) + v * v * (H_y - by) + g * H_y * (H - b);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -609,17 +652,31 @@ int esdg_collect_edge_data_bottom(esdg_ctx* ctx, int Nfaces, const int* EdgeData
 int esdg_collect_edge_data(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const double* q, const double* x,
                            const double* y, const double* nx, const double* ny, const double* bL, const double* bR,
                            double* qL, double* qR, double t) {
-    (void)x; (void)y; (void)t;
     ESDG_REQUIRE(ctx, ctx, "null context");
     ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeData && q && nx && ny && qL && qR, "esdg_collect_edge_data: bad argument");
-    ESDG_REQUIRE(ctx, ctx->p.bc_kind != ESDG_BC_OCEAN || (bL && bR), "esdg_collect_edge_data: ocean boundary needs bL, bR");
-    ESDG_REQUIRE(ctx, ctx->p.bc_kind >= 0 && ctx->p.bc_kind <= 2, "esdg_collect_edge_data: unknown boundary kind");
+    ESDG_REQUIRE(ctx, (ctx->p.bc_kind != ESDG_BC_OCEAN && ctx->p.bc_kind != ESDG_BC_CONVERGENCE) || (bL && bR),
+                 "esdg_collect_edge_data: this boundary needs bL, bR");
+    ESDG_REQUIRE(ctx, ctx->p.bc_kind != ESDG_BC_CONVERGENCE || (x && y),
+                 "esdg_collect_edge_data: the convergence-test boundary needs x, y");
+    ESDG_REQUIRE(ctx, ctx->p.bc_kind >= 0 && ctx->p.bc_kind <= 3, "esdg_collect_edge_data: unknown boundary kind");
     if (Nfaces == 0) return ESDG_OK;
     ESDG_ENTRY("collect_edge_data");
     const int ngl = ctx->p.ngl;
-    k_collect<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(Nfaces, ngl, ctx->p.bc_kind, ctx->p.h_zero,
-                                                                                EdgeData, q, nx, ny, bL, bR, qL, qR);
+    k_collect<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(
+        Nfaces, ngl, ctx->p.bc_kind, ctx->p.h_zero, EdgeData, q, x, y, t, nx, ny, bL, bR, qL, qR);
     return launch_check(ctx, "collect_edge_data");
+}
+
+int esdg_add_source(esdg_ctx* ctx, int Nelem, const double* Bx, const double* By, const double* B, const double* x,
+                    const double* y, double t, double* Qt) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, Nelem >= 0 && Bx && By && B && x && y && Qt, "esdg_add_source: bad argument");
+    if (Nelem == 0) return ESDG_OK;
+    ESDG_ENTRY("add_source");
+    const int ngl = ctx->p.ngl;
+    k_add_source<<<blocks_for((int64_t)Nelem * ngl * ngl, 256), 256, 0, ctx->stream>>>(Nelem, ngl, ctx->p.g, Bx, By, B, x,
+                                                                                      y, t, Qt);
+    return launch_check(ctx, "add_source");
 }
 
 int esdg_calc_num_fluxes(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const int* EdgeReversed, const double* nx,

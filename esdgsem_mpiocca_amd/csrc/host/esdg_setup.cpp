@@ -452,6 +452,7 @@ namespace {
 //                  b = 4000 - depth, h = depth + eta, eta = amp*exp(-(xr/a)^2-(yr/c)^2) with
 //                  rotation p2 (radians), axes a = p0, c = p1 (metres), centre at 40% / 50%.
 //   CONSTANT:      h = p0, hu = p1, hv = p2, b = 0.
+//   CONVERGENCE:   b = 2 + 0.5 sin(2 pi y) + cos(2 pi x); h = 8 + cos x sin y - b; hu = h/2, hv = 3h/2.
 void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<double>& y) {
     const esdg_case_spec& s = C.spec;
     const int ngl2 = C.B.ngl * C.B.ngl;
@@ -520,6 +521,14 @@ void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<do
                     const double xr = c * (X - cx) + sn * (Y - cy);
                     const double yr = -sn * (X - cx) + c * (Y - cy);
                     h = depth + s.amp * std::exp(-(xr / s.p0) * (xr / s.p0) - (yr / s.p1) * (yr / s.p1));
+                    break;
+                }
+                case ESDG_CASE_CONVERGENCE: {
+                    // SW2D::InitQNodal case 1 with SW2D bathymetry case 1, at t = 0
+                    bb = 2.0 + 0.5 * std::sin(2.0 * M_PI * Y) + std::cos(2.0 * M_PI * X);
+                    h = 8.0 + std::cos(X) * std::sin(Y) * std::cos(0.0) - bb;
+                    hu = h * 0.5;
+                    hv = h * 1.5;
                     break;
                 }
                 case ESDG_CASE_CONSTANT:

@@ -60,7 +60,7 @@ class CaseInfo(C.Structure):
                 ("min_ele_size", C.c_double)]
 
 
-CASE_GAUSSIAN_BUMP, CASE_DAM_BREAK, CASE_FLOOD, CASE_OCEAN_UPLIFT, CASE_CONSTANT = range(5)
+CASE_GAUSSIAN_BUMP, CASE_DAM_BREAK, CASE_FLOOD, CASE_OCEAN_UPLIFT, CASE_CONSTANT, CASE_CONVERGENCE = range(6)
 MESH_CARTESIAN, MESH_CURVED_SINE = 0, 1
 
 DOUBLE_ARRAYS = ("x", "y", "xXi", "xEta", "yXi", "yEta", "Jac", "b", "Bx", "By", "q", "nx", "ny",
@@ -168,6 +168,14 @@ class Case:
         walls over the shelf."""
         return cls(CASE_OCEAN_UPLIFT, MESH_CARTESIAN, N, nel, nel, 0.0, length, 0.0, length, 0, 0,
                    amp=amp, p=(axis_a, axis_c, angle, 0))
+
+    @classmethod
+    def convergence(cls, N=3, nel=8, curved=False, perturb=0.02):
+        """Manufactured solution of the convergence test on nel^2 quads of [0,1]^2 (optionally the
+        curved mesh family); run it with RunParams(bc_kind=3): Dirichlet exterior states and the
+        manufactured source."""
+        return cls(CASE_CONVERGENCE, MESH_CURVED_SINE if curved else MESH_CARTESIAN, N, nel, nel, 0.0, 1.0,
+                   0.0, 1.0, 0, 0, perturb_amp=perturb if curved else 0.0, perturb_len=0.5)
 
     @classmethod
     def constant(cls, N, nel, h=1.0, hu=0.0, hv=0.0, curved=True, perturb=0.05):

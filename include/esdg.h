@@ -48,7 +48,10 @@ enum esdg_num_flux {
 enum esdg_bc {
     ESDG_BC_WALLS = 0,         /* okl/GatherEdgeData/SolidWalls.okl */
     ESDG_BC_INNER_OUTER = 1,   /* okl/GatherEdgeData/InnerOuter.okl (zero momentum outside) */
-    ESDG_BC_OCEAN = 2          /* okl/GatherEdgeData/OceanBoundary.okl (open where b <= 6.7) */
+    ESDG_BC_OCEAN = 2,         /* okl/GatherEdgeData/OceanBoundary.okl (open where b <= 6.7) */
+    ESDG_BC_CONVERGENCE = 3    /* okl/GatherEdgeData/Dirichlet_ConvTest.okl: the exterior state is the
+                                  manufactured solution at the stage time; the time loop adds its
+                                  source term okl/ManufacturedSolutions/S_ConvTest.okl (Testcase 1) */
 };
 
 typedef struct esdg_params {
@@ -162,6 +165,11 @@ int esdg_calc_num_fluxes_viscose(esdg_ctx* ctx, int Nfaces, const int* EdgeData,
 int esdg_surface_kernel_visc(esdg_ctx* ctx, int Nelem, const double* Jac,
                              const int* ElemEdgeMasterSlave, const int* ElemEdgeOrientation,
                              const int* ElemToEdge, const double* SurfacePartsVisc, double* Qt);
+
+/* addS (src/deviceclass.cpp:1100-1102, okl/ManufacturedSolutions/S_ConvTest.okl): Qt += source of
+ * the manufactured solution h = 8 + cos x sin y cos t - b, u = 1/2, v = 3/2 at time t. */
+int esdg_add_source(esdg_ctx* ctx, int Nelem, const double* Bx, const double* By, const double* B,
+                    const double* x, const double* y, double t, double* Qt);
 
 /* Device-wide max of x[0..n) into *d_out (device pointer).  Replaces the host max loop of
  * GetGlobalLambdaMax / GetGlobalViscParaMax (src/MPI_Communication.cpp:490-541); max is exact. */

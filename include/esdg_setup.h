@@ -56,7 +56,10 @@ enum esdg_case_kind {
                                      p1 = height ratio, p2 = step angle (radians) when set */
     ESDG_CASE_FLOOD = 2,          /* 16-mode random-phase Fourier bathymetry, reservoir over dry land */
     ESDG_CASE_OCEAN_UPLIFT = 3,   /* 4000 m ocean with linear shelf to 50 m, elliptic Gaussian uplift */
-    ESDG_CASE_CONSTANT = 4        /* constant state (h0, hu0, hv0) over flat b=0: free-stream test */
+    ESDG_CASE_CONSTANT = 4,       /* constant state (h0, hu0, hv0) over flat b=0: free-stream test */
+    ESDG_CASE_CONVERGENCE = 5     /* manufactured solution of Testcase 1 (SW2D.cpp:312-318, 829-833):
+                                     b = 2 + sin(2 pi y)/2 + cos(2 pi x), h = 8 + cos x sin y - b,
+                                     hu = h/2, hv = 3h/2 at t = 0 */
 };
 
 typedef struct esdg_case_spec {

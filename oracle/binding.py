@@ -20,6 +20,7 @@ class OracleParams(C.Structure):
 _RUN_DOUBLES = ("Jac", "Yxi", "Yeta", "Xxi", "Xeta", "Bx", "By", "B", "D", "Dhat", "Dstrong",
                 "VdmInv", "nx", "ny", "scal", "EleSizes", "GLw")
 _RUN_INTS = ("EdgeData", "EdgeReversed", "ElemToEdge", "EMS", "EOR")
+_RUN_COORDS = ("x", "y")
 
 
 class OracleRun(C.Structure):
@@ -28,7 +29,8 @@ class OracleRun(C.Structure):
                  ("artificial_viscosity", C.c_int), ("positivity", C.c_int), ("rk_order", C.c_int),
                  ("rk_ssp", C.c_int), ("max_steps", C.c_int)]
                 + [(n, C.POINTER(C.c_double)) for n in _RUN_DOUBLES]
-                + [(n, C.POINTER(C.c_int)) for n in _RUN_INTS])
+                + [(n, C.POINTER(C.c_int)) for n in _RUN_INTS]
+                + [(n, C.POINTER(C.c_double)) for n in _RUN_COORDS])
 
 
 def lib():
@@ -94,6 +96,9 @@ def case_run_inputs(case, rp, steps, T=None):
     for n in _RUN_INTS:
         keep[n] = np.ascontiguousarray(m[n], dtype=np.int32)
         setattr(r, n, keep[n].ctypes.data_as(C.POINTER(C.c_int)))
+    for n in _RUN_COORDS:
+        keep[n] = np.ascontiguousarray(a[n], dtype=np.float64)
+        setattr(r, n, keep[n].ctypes.data_as(C.POINTER(C.c_double)))
     return r, keep
 
 

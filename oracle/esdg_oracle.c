@@ -152,6 +152,72 @@ void oracle_collect_edge_data(const oracle_params* p, int Nfaces, const int* Edg
     }
 }
 
+/* Qnodal of okl/GatherEdgeData/Dirichlet_ConvTest.okl:17-31 */
+static void manufactured_state(double x, double y, double t, double b, double* q) {
+    q[0] = (8.0 + cos(x) * sin(y) * cos(t) - b);
+    q[1] = 0.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
+    q[2] = 1.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
+}
+
+/* okl/GatherEdgeData/Dirichlet_ConvTest.okl:33-190: interior traces as SolidWalls.okl, the exterior of
+ * a boundary face is the manufactured solution at the interior node's position and time t, with
+ * the interior side's bottom trace. */
+void oracle_collect_edge_data_conv(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
+                                   const double* x, const double* y, const double* bL, const double* bR, double t,
+                                   double* qL, double* qR) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    for (int f = 0; f < Nfaces; ++f) {
+        const int* E = EdgeData + 8 * f;
+        const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+        for (int i = 0; i < ngl; ++i) {
+            const long eid = (long)f * ngl * NEQ + i;
+            const long xid = (long)f * ngl + i;
+            double L[3] = {0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0};
+            if (cpuL == 0) {
+                const long id = (long)iel * ngl2 * NEQ + side_node(ngl, isl, i);
+                L[0] = q[id]; L[1] = q[id + ngl2]; L[2] = q[id + 2 * ngl2];
+            }
+            if (cpuR == 0) {
+                const long id = (long)ier * ngl2 * NEQ + side_node(ngl, isr, i);
+                R[0] = q[id]; R[1] = q[id + ngl2]; R[2] = q[id + 2 * ngl2];
+            }
+            if (cpuL == -1) {
+                const long nd = (long)ier * ngl2 + side_node(ngl, isr, i);
+                manufactured_state(x[nd], y[nd], t, bR[xid], L);
+            }
+            if (cpuR == -1) {
+                const long nd = (long)iel * ngl2 + side_node(ngl, isl, i);
+                manufactured_state(x[nd], y[nd], t, bL[xid], R);
+            }
+            for (int v = 0; v < 3; ++v) {
+                qL[eid + v * ngl] = L[v];
+                qR[eid + v * ngl] = R[v];
+            }
+        }
+    }
+}
+
+/* okl/ManufacturedSolutions/S_ConvTest.okl:17-95 (h = 8 + cos x sin y cos t - b, u = 1/2, v = 3/2) */
+void oracle_add_source(const oracle_params* p, int Nelem, const double* Bx, const double* By, const double* B,
+                       const double* x, const double* y, double t, double* Qt) {
+    const int ngl2 = p->ngl * p->ngl;
+    const double g = p->g;
+    for (int e = 0; e < Nelem; ++e)
+        for (int n = 0; n < ngl2; ++n) {
+            const long k = (long)e * ngl2 + n;
+            const double sinY = sin(y[k]), sinX = sin(x[k]), cosY = cos(y[k]), cosX = cos(x[k]);
+            const double H = 8.0 + cosX * sinY * cos(t);
+            const double H_t = -cosX * sinY * sin(t);
+            const double H_x = -sinX * sinY * cos(t);
+            const double H_y = cosX * cosY * cos(t);
+            const double u = 0.5, v = 1.5;
+            double* o = Qt + (long)e * ngl2 * NEQ + n;
+            o[0] += H_t + u * (H_x - Bx[k]) + v * (H_y - By[k]);
+            o[ngl2] += u * H_t + u * u * (H_x - Bx[k]) + u * v * (H_y - By[k]) + g * H_x * (H - B[k]);
+            o[2 * ngl2] += v * H_t + u * v * (H_x - Bx[k]) + v * v * (H_y - By[k]) + g * H_y * (H - B[k]);
+        }
+}
+
 /* two-point EC volume fluxes F# (xi direction, eta metrics) and G# (eta direction, xi metrics):
  * okl/DG/VolumeKernelFluxDiffV99.okl:17-57 */
 static void fvol(double half_g, double fourth_g, double q1L, double q2L, double q3L, double iL,
@@ -955,7 +1021,12 @@ int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, doub
         if (dt_hist) dt_hist[step] = dt;
         if (r->rk_ssp) memcpy(Qtmp, q, sizeof(double) * nq);
         for (int s = 0; s < stages; ++s) {
-            oracle_collect_edge_data(p, Nf, r->EdgeData, q, r->nx, r->ny, bL, bR, qL, qR);
+            const double rkD[3] = {0.0, 1.0, 0.5};
+            const double t_stage = t + (r->rk_ssp ? rkD[s] : Bc[s]) * dt;
+            if (p->bc_kind == 3)
+                oracle_collect_edge_data_conv(p, Nf, r->EdgeData, q, r->x, r->y, bL, bR, t_stage, qL, qR);
+            else
+                oracle_collect_edge_data(p, Nf, r->EdgeData, q, r->nx, r->ny, bL, bR, qL, qR);
             oracle_volume_fluxdiff(p, Ne, r->Jac, r->Yxi, r->Yeta, r->Xxi, r->Xeta, q, r->D, r->Bx, r->By, Qt);
             if (r->num_flux == 3)
                 oracle_num_flux_es_rotated(p, Nf, r->EdgeReversed, r->nx, r->ny, r->scal, qL, qR, bL, bR, SP);
@@ -979,6 +1050,7 @@ int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, doub
                 oracle_num_fluxes_viscose(p, Nf, r->EdgeData, r->nx, r->ny, r->scal, VL, VR, gXL, gXR, gYL, gYR, SPV);
                 oracle_surface_visc(p, Ne, r->Jac, r->EMS, r->EOR, r->ElemToEdge, SPV, Qt);
             }
+            if (p->bc_kind == 3) oracle_add_source(p, Ne, r->Bx, r->By, r->B, r->x, r->y, t_stage, Qt);
             if (r->rk_ssp) {
                 oracle_rk_ssp_update(p, Ne, A[s], Bc[s], C[s], dt, Qt, Qtmp, q);
             } else {

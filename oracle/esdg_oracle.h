@@ -36,6 +36,13 @@ void oracle_collect_edge_data_walls(const oracle_params* p, int Nfaces, const in
 void oracle_collect_edge_data(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
                               const double* nx, const double* ny, const double* bL, const double* bR,
                               double* qL, double* qR);
+/* Dirichlet_ConvTest.okl: interior traces, exterior = manufactured solution at time t */
+void oracle_collect_edge_data_conv(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
+                                   const double* x, const double* y, const double* bL, const double* bR, double t,
+                                   double* qL, double* qR);
+/* S_ConvTest.okl addS: Qt += manufactured source at time t */
+void oracle_add_source(const oracle_params* p, int Nelem, const double* Bx, const double* By, const double* B,
+                       const double* x, const double* y, double t, double* Qt);
 void oracle_volume_fluxdiff(const oracle_params* p, int Nelem, const double* Jac, const double* Yxi,
                             const double* Yeta, const double* Xxi, const double* Xeta, const double* q,
                             const double* D, const double* Bx, const double* By, double* Qt);
@@ -109,6 +116,7 @@ typedef struct oracle_run {
     const double *Jac, *Yxi, *Yeta, *Xxi, *Xeta, *Bx, *By, *B, *D, *Dhat, *Dstrong, *VdmInv;
     const double *nx, *ny, *scal, *EleSizes, *GLw;
     const int *EdgeData, *EdgeReversed, *ElemToEdge, *EMS, *EOR;
+    const double *x, *y;   /* node coordinates (the convergence-test boundary and source) */
 } oracle_run;
 
 int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, double* t_out,

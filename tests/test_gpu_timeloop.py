@@ -122,6 +122,27 @@ def test_roe_and_combined_ocean_flux(dev, fused, num_flux):
     assert max(errs) <= 1e-12, errs
 
 
+def test_manufactured_solution_parity_and_convergence(dev):
+    """Convergence test (Dirichlet_ConvTest.okl + S_ConvTest.okl): the reference kernel sequence on
+    the GPU against the oracle, and its error against the exact solution at order N+1."""
+    from test_oracle_properties import manufactured
+    case = Case.convergence(N=3, nel=6, curved=True)
+    rp = RunParams(num_flux=3, bc_kind=3, CFL=0.2)
+    qo, no, to, _ = O.timeloop(case, rp, 20)
+    qg, ng, tg = run_gpu(dev, case, rp, 20, fused=0)
+    assert ng == no and abs(tg - to) <= 1e-14 * to
+    assert max(fields_rel_l2(case, qg, qo)) <= 1e-12
+    errs = []
+    for nel in (4, 8, 16):
+        c = Case.convergence(N=3, nel=nel)
+        rpT = RunParams(num_flux=0, bc_kind=3, CFL=0.2, T=0.2)
+        q, n, t = run_gpu(dev, c, rpT, 100000, fused=0)
+        assert t == 0.2
+        errs.append(np.abs(q - manufactured(c, t)).max())
+    eoc = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert (eoc >= 3.7).all(), (errs, eoc)
+
+
 def test_entropy_conservative_flux(dev):
     """ES=0 (EC surface flux): total entropy changes only by the time-integration error."""
     case = Case.gaussian_bump(N=4, nel=8)

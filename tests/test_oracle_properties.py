@@ -529,3 +529,30 @@ def test_timeloop_mass_conservation_bump():
     m0 = (wJ * c["q"].reshape(c.Nelem, 3, -1)[:, 0]).sum()
     m1 = (wJ * q.reshape(c.Nelem, 3, -1)[:, 0]).sum()
     assert abs(m1 - m0) < 1e-13 * m0
+
+
+def manufactured(case, t):
+    """The convergence test's exact solution (SW2D.cpp:312-318): h = 8 + cos x sin y cos t - b."""
+    x, y, b = case["x"], case["y"], case["b"]
+    h = 8.0 + np.cos(x) * np.sin(y) * np.cos(t) - b
+    q = np.zeros((case.Nelem, 3, case.ngl ** 2))
+    q[:, 0] = h.reshape(case.Nelem, -1)
+    q[:, 1] = 0.5 * q[:, 0]
+    q[:, 2] = 1.5 * q[:, 0]
+    return q.ravel()
+
+
+@pytest.mark.parametrize("N,eoc_min", [(2, 2.7), (3, 3.7)])
+def test_manufactured_solution_converges_at_order_N_plus_1(N, eoc_min):
+    """Whole stage + time loop (volume, ES flux, surface lift, Dirichlet exterior states, manufactured
+    source, LSRK4) against the exact solution of the convergence test: max-norm error falls as
+    h^(N+1).  A wrong sign or index anywhere in the residual stalls the convergence."""
+    errs = []
+    for nel in (4, 8):
+        c = Case.convergence(N=N, nel=nel)
+        rp = RunParams(num_flux=0, bc_kind=3, CFL=0.2, T=0.1)
+        q, n, t, _ = O.timeloop(c, rp, 100000)
+        assert t == 0.1
+        errs.append(np.abs(q - manufactured(c, t)).max())
+    eoc = np.log2(errs[0] / errs[1])
+    assert eoc >= eoc_min, (errs, eoc)
