@@ -97,8 +97,8 @@ class deviceclass {
         if (rp_.rk_order == 4 && rp_.rk_ssp) return set_error(ctx_, ESDG_ERR_ARG, "rk_order 4 has no SSP variant (rkSSP must be 0)");
         if (rp_.artificial_viscosity && ngl < 3) return set_error(ctx_, ESDG_ERR_ARG, "artificial viscosity needs N >= 2");
         if (rp_.fused < 0 || rp_.fused > 2) return set_error(ctx_, ESDG_ERR_ARG, "fused must be 0, 1 or 2");
-        if (rp_.fused && ctx_->p.bc_kind == ESDG_BC_CONVERGENCE)
-            return set_error(ctx_, ESDG_ERR_ARG, "the convergence-test boundary runs on the reference kernel sequence");
+        if (rp_.fused && (ctx_->p.bc_kind == ESDG_BC_CONVERGENCE || ctx_->p.bc_kind == ESDG_BC_THREE_MOUND))
+            return set_error(ctx_, ESDG_ERR_ARG, "time-dependent boundaries run on the reference kernel sequence");
         if (rp_.fused && (rp_.rk_ssp || !rp_.flux_differencing))
             return set_error(ctx_, ESDG_ERR_ARG, "the fused stage supports flux differencing with low-storage RK only");
         if (rp_.fused == 2 && rp_.artificial_viscosity)

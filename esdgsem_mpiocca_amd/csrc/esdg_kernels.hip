@@ -47,7 +47,33 @@ __device__ __forceinline__ void manufactured_state(double x, double y, double t,
     q3 = 1.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
 }
 
-__global__ void __launch_bounds__(256) k_collect(int Nfaces, int ngl, int bc, double h_zero, const int* __restrict__ ED,
+// exterior state of 3MoundInflow.okl:110-240 for the interior state (h, hu, hv) at node x
+__device__ __forceinline__ void three_mound_state(double nx, double ny, double x, double time, double tol,
+                                                  double zero_tol, double h, double hu, double hv, double& o0,
+                                                  double& o1, double& o2) {
+    if (fabs(nx) < zero_tol) {
+        const double tx = -ny, ty = nx;
+        double vx, vy;
+        if (h > tol) { vx = hu / h; vy = hv / h; }
+        else { vx = 0.0; vy = 0.0; }
+        const double v_n = nx * vx + ny * vy;
+        const double v_t = tx * vx + ty * vy;
+        const double wn = -v_n, wt = v_t;
+        const double wx = nx * wn + tx * wt;
+        const double wy = ny * wn + ty * wt;
+        o0 = h;
+        o1 = h * wx;
+        o2 = h * wy;
+    } else if (fabs(x) < zero_tol) {
+        if (time <= 300) { o0 = 0.5; o1 = 0.5; o2 = 0.0; }
+        else { o0 = 1.0; o1 = 1.0; o2 = 0.0; }
+    } else {
+        o0 = h; o1 = hu; o2 = hv;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_collect(int Nfaces, int ngl, int bc, double h_zero, double tol,
+                                                 double zero_tol, const int* __restrict__ ED,
                                                  const double* __restrict__ q, const double* __restrict__ xa,
                                                  const double* __restrict__ ya, double time,
                                                  const double* __restrict__ nxa,
@@ -82,6 +108,15 @@ __global__ void __launch_bounds__(256) k_collect(int Nfaces, int ngl, int bc, do
         if (cpuR == -1) {
             const int64_t xid = (int64_t)iel * ngl2 + side_node(ngl, isl, i);
             manufactured_state(xa[xid], ya[xid], time, bLa[t], R0, R1, R2);
+        }
+    } else if (bc == ESDG_BC_THREE_MOUND) {
+        if (cpuL == -1) {
+            const int64_t xid = (int64_t)ier * ngl2 + side_node(ngl, isr, i);
+            three_mound_state(nx, ny, xa[xid], time, tol, zero_tol, R0, R1, R2, L0, L1, L2);
+        }
+        if (cpuR == -1) {
+            const int64_t xid = (int64_t)iel * ngl2 + side_node(ngl, isl, i);
+            three_mound_state(nx, ny, xa[xid], time, tol, zero_tol, L0, L1, L2, R0, R1, R2);
         }
     } else if (bc == ESDG_BC_INNER_OUTER) {
         // okl/GatherEdgeData/InnerOuter.okl: exterior keeps h, momentum zero
@@ -656,14 +691,15 @@ int esdg_collect_edge_data(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const
     ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeData && q && nx && ny && qL && qR, "esdg_collect_edge_data: bad argument");
     ESDG_REQUIRE(ctx, (ctx->p.bc_kind != ESDG_BC_OCEAN && ctx->p.bc_kind != ESDG_BC_CONVERGENCE) || (bL && bR),
                  "esdg_collect_edge_data: this boundary needs bL, bR");
-    ESDG_REQUIRE(ctx, ctx->p.bc_kind != ESDG_BC_CONVERGENCE || (x && y),
-                 "esdg_collect_edge_data: the convergence-test boundary needs x, y");
-    ESDG_REQUIRE(ctx, ctx->p.bc_kind >= 0 && ctx->p.bc_kind <= 3, "esdg_collect_edge_data: unknown boundary kind");
+    ESDG_REQUIRE(ctx, (ctx->p.bc_kind != ESDG_BC_CONVERGENCE && ctx->p.bc_kind != ESDG_BC_THREE_MOUND) || (x && y),
+                 "esdg_collect_edge_data: this boundary needs x, y");
+    ESDG_REQUIRE(ctx, ctx->p.bc_kind >= 0 && ctx->p.bc_kind <= 4, "esdg_collect_edge_data: unknown boundary kind");
     if (Nfaces == 0) return ESDG_OK;
     ESDG_ENTRY("collect_edge_data");
     const int ngl = ctx->p.ngl;
     k_collect<<<blocks_for((int64_t)Nfaces * ngl, 256), 256, 0, ctx->stream>>>(
-        Nfaces, ngl, ctx->p.bc_kind, ctx->p.h_zero, EdgeData, q, x, y, t, nx, ny, bL, bR, qL, qR);
+        Nfaces, ngl, ctx->p.bc_kind, ctx->p.h_zero, ctx->p.PosPresTOL, ctx->p.ZeroTOL, EdgeData, q, x, y, t, nx, ny, bL,
+        bR, qL, qR);
     return launch_check(ctx, "collect_edge_data");
 }
 

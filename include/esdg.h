@@ -49,9 +49,12 @@ enum esdg_bc {
     ESDG_BC_WALLS = 0,         /* okl/GatherEdgeData/SolidWalls.okl */
     ESDG_BC_INNER_OUTER = 1,   /* okl/GatherEdgeData/InnerOuter.okl (zero momentum outside) */
     ESDG_BC_OCEAN = 2,         /* okl/GatherEdgeData/OceanBoundary.okl (open where b <= 6.7) */
-    ESDG_BC_CONVERGENCE = 3    /* okl/GatherEdgeData/Dirichlet_ConvTest.okl: the exterior state is the
+    ESDG_BC_CONVERGENCE = 3,   /* okl/GatherEdgeData/Dirichlet_ConvTest.okl: the exterior state is the
                                   manufactured solution at the stage time; the time loop adds its
                                   source term okl/ManufacturedSolutions/S_ConvTest.okl (Testcase 1) */
+    ESDG_BC_THREE_MOUND = 4    /* okl/GatherEdgeData/3MoundInflow.okl: walls on faces with n_x = 0,
+                                  inflow (1/2, 1/2, 0), from t > 300 (1, 1, 0), at x = 0, outflow
+                                  elsewhere (Testcase 32) */
 };
 
 typedef struct esdg_params {

@@ -159,12 +159,39 @@ static void manufactured_state(double x, double y, double t, double b, double* q
     q[2] = 1.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
 }
 
-/* okl/GatherEdgeData/Dirichlet_ConvTest.okl:33-190: interior traces as SolidWalls.okl, the exterior of
- * a boundary face is the manufactured solution at the interior node's position and time t, with
- * the interior side's bottom trace. */
-void oracle_collect_edge_data_conv(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
-                                   const double* x, const double* y, const double* bL, const double* bR, double t,
-                                   double* qL, double* qR) {
+/* exterior state of okl/GatherEdgeData/3MoundInflow.okl:110-240: velocity mirror on faces with
+ * |n_x| < ZeroTOL, the inflow state at x = 0 (switching at t = 300), a copy elsewhere (outflow) */
+static void three_mound_state(const oracle_params* p, double nx, double ny, double x, double t, const double* in,
+                              double* out) {
+    if (fabs(nx) < p->ZeroTOL) {
+        const double tx = -ny, ty = nx;
+        double vx, vy;
+        if (in[0] > p->PosPresTOL) { vx = in[1] / in[0]; vy = in[2] / in[0]; }
+        else { vx = 0.0; vy = 0.0; }
+        const double v_n = nx * vx + ny * vy;
+        const double v_t = tx * vx + ty * vy;
+        const double wn = -v_n, wt = v_t;
+        const double wx = nx * wn + tx * wt;
+        const double wy = ny * wn + ty * wt;
+        out[0] = in[0];
+        out[1] = in[0] * wx;
+        out[2] = in[0] * wy;
+    } else if (fabs(x) < p->ZeroTOL) {
+        if (t <= 300) { out[0] = 0.5; out[1] = 0.5; out[2] = 0.0; }
+        else { out[0] = 1.0; out[1] = 1.0; out[2] = 0.0; }
+    } else {
+        out[0] = in[0]; out[1] = in[1]; out[2] = in[2];
+    }
+}
+
+/* The time-dependent boundary variants (bc_kind 3, 4).  Interior traces as SolidWalls.okl; the
+ * exterior of a boundary face is
+ *   3  okl/GatherEdgeData/Dirichlet_ConvTest.okl:33-190: the manufactured solution at the
+ *      interior node's position and time t, with the interior side's bottom trace;
+ *   4  okl/GatherEdgeData/3MoundInflow.okl: three_mound_state of the interior trace. */
+void oracle_collect_edge_data_xt(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
+                                 const double* x, const double* y, const double* nx, const double* ny,
+                                 const double* bL, const double* bR, double t, double* qL, double* qR) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
     for (int f = 0; f < Nfaces; ++f) {
         const int* E = EdgeData + 8 * f;
@@ -183,11 +210,13 @@ void oracle_collect_edge_data_conv(const oracle_params* p, int Nfaces, const int
             }
             if (cpuL == -1) {
                 const long nd = (long)ier * ngl2 + side_node(ngl, isr, i);
-                manufactured_state(x[nd], y[nd], t, bR[xid], L);
+                if (p->bc_kind == 4) three_mound_state(p, nx[xid], ny[xid], x[nd], t, R, L);
+                else manufactured_state(x[nd], y[nd], t, bR[xid], L);
             }
             if (cpuR == -1) {
                 const long nd = (long)iel * ngl2 + side_node(ngl, isl, i);
-                manufactured_state(x[nd], y[nd], t, bL[xid], R);
+                if (p->bc_kind == 4) three_mound_state(p, nx[xid], ny[xid], x[nd], t, L, R);
+                else manufactured_state(x[nd], y[nd], t, bL[xid], R);
             }
             for (int v = 0; v < 3; ++v) {
                 qL[eid + v * ngl] = L[v];
@@ -1023,8 +1052,8 @@ int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, doub
         for (int s = 0; s < stages; ++s) {
             const double rkD[3] = {0.0, 1.0, 0.5};
             const double t_stage = t + (r->rk_ssp ? rkD[s] : Bc[s]) * dt;
-            if (p->bc_kind == 3)
-                oracle_collect_edge_data_conv(p, Nf, r->EdgeData, q, r->x, r->y, bL, bR, t_stage, qL, qR);
+            if (p->bc_kind == 3 || p->bc_kind == 4)
+                oracle_collect_edge_data_xt(p, Nf, r->EdgeData, q, r->x, r->y, r->nx, r->ny, bL, bR, t_stage, qL, qR);
             else
                 oracle_collect_edge_data(p, Nf, r->EdgeData, q, r->nx, r->ny, bL, bR, qL, qR);
             oracle_volume_fluxdiff(p, Ne, r->Jac, r->Yxi, r->Yeta, r->Xxi, r->Xeta, q, r->D, r->Bx, r->By, Qt);

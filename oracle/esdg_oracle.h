@@ -36,10 +36,11 @@ void oracle_collect_edge_data_walls(const oracle_params* p, int Nfaces, const in
 void oracle_collect_edge_data(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
                               const double* nx, const double* ny, const double* bL, const double* bR,
                               double* qL, double* qR);
-/* Dirichlet_ConvTest.okl: interior traces, exterior = manufactured solution at time t */
-void oracle_collect_edge_data_conv(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
-                                   const double* x, const double* y, const double* bL, const double* bR, double t,
-                                   double* qL, double* qR);
+/* time-dependent boundaries: 3 Dirichlet_ConvTest.okl (manufactured solution at time t),
+ * 4 3MoundInflow.okl (walls / inflow at x = 0 / outflow) */
+void oracle_collect_edge_data_xt(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
+                                 const double* x, const double* y, const double* nx, const double* ny,
+                                 const double* bL, const double* bR, double t, double* qL, double* qR);
 /* S_ConvTest.okl addS: Qt += manufactured source at time t */
 void oracle_add_source(const oracle_params* p, int Nelem, const double* Bx, const double* By, const double* B,
                        const double* x, const double* y, double t, double* Qt);

@@ -91,6 +91,39 @@ def test_collect_edge_data_boundary_variants(dev, bc):
         s.close()
 
 
+@pytest.mark.parametrize("bc,t", [(3, 0.37), (4, 123.0), (4, 400.0)],
+                         ids=["convergence", "three_mound_early", "three_mound_late"])
+def test_collect_edge_data_time_dependent(dev, bc, t):
+    """Dirichlet_ConvTest.okl (manufactured exterior state) and 3MoundInflow.okl (walls, inflow at
+    x = 0 switching at t = 300, outflow): the exterior states use the node coordinates and t."""
+    from esdgsem_mpiocca_amd import Case
+    from esdgsem_mpiocca_amd import setup_api as SA
+    if bc == 3:
+        case = Case.convergence(N=4, nel=5, curved=True)
+    else:
+        case = Case(SA.CASE_CONSTANT, SA.MESH_CARTESIAN, 4, 6, 3, 0.0, 4.0, 0.0, 2.0, p=(1.0, 0.2, 0.1, 0))
+    rp = RunParams(bc_kind=bc)
+    s = Setup(dev, case, rp)
+    try:
+        fn = s.Nf * s.ngl
+        bL, bR = np.zeros(fn), np.zeros(fn)
+        O.call("collect_edge_data_bottom", s.p, s.Nf, s.h["EdgeData"], s.h["B"], bL, bR)
+        qL, qR = np.zeros(fn * 3), np.zeros(fn * 3)
+        O.call("collect_edge_data_xt", s.p, s.Nf, s.h["EdgeData"], s.q, s.h["x"], s.h["y"], s.h["nx"], s.h["ny"],
+               bL, bR, t, qL, qR)
+        dq = s.ctx.array(s.q)
+        dL, dR = s.ctx.empty(fn * 3), s.ctx.empty(fn * 3)
+        dbL, dbR = s.ctx.array(bL), s.ctx.array(bR)
+        s.ctx.collect_edge_data(s.Nf, s.d["EdgeData"], dq, s.d["x"], s.d["y"], s.d["nx"], s.d["ny"], dbL, dbR,
+                                dL, dR, t)
+        assert np.array_equal(dL.download(), qL)
+        gR = dR.download()
+        # the manufactured state goes through cos/sin: device and host libm may differ in the last ulp
+        assert np.abs(gR - qR).max() <= 4e-15 * np.abs(qR).max()
+    finally:
+        s.close()
+
+
 def test_collect_edge_data_walls(S):
     fn = S.Nf * S.ngl
     qL, qR = np.zeros(fn * 3), np.zeros(fn * 3)

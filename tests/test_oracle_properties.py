@@ -290,6 +290,43 @@ def test_boundary_variants_exterior_states():
     assert not R2[:, 1][~wall].any() and not R2[:, 2][~wall].any()
 
 
+@pytest.mark.parametrize("t", [100.0, 400.0])
+def test_three_mound_boundary_states(t):
+    """3MoundInflow.okl exterior states on a channel: reflecting walls where n_x = 0, the inflow
+    state (1/2, 1/2, 0) -- (1, 1, 0) after t = 300 -- at x = 0, a copy of the interior elsewhere."""
+    c = Case(S.CASE_CONSTANT, S.MESH_CARTESIAN, 3, 6, 3, 0.0, 4.0, 0.0, 2.0, p=(1.0, 0.2, 0.1, 0))
+    q = perturbed_state(c, seed=11)
+    p = O.params(c, RunParams(bc_kind=4))
+    fn = c.Nfaces * c.ngl
+    bL, bR, qL, qR = np.zeros(fn), np.zeros(fn), np.zeros(fn * 3), np.zeros(fn * 3)
+    O.call("collect_edge_data_xt", p, c.Nfaces, c["EdgeData"], q, c["x"], c["y"], c["nx"], c["ny"], bL, bR, t,
+           qL, qR)
+    ED = c["EdgeData"].reshape(-1, 8)
+    L3, R3 = qL.reshape(-1, 3, c.ngl), qR.reshape(-1, 3, c.ngl)
+    nx, ny = c["nx"].reshape(-1, c.ngl), c["ny"].reshape(-1, c.ngl)
+    x = c["x"].reshape(c.Nelem, -1)
+    side = {0: lambda k: k, 2: lambda k: c.ngl ** 2 - c.ngl + k, 3: lambda k: k * c.ngl,
+            1: lambda k: (k + 1) * c.ngl - 1}
+    inflow = (0.5, 0.5, 0.0) if t <= 300 else (1.0, 1.0, 0.0)
+    seen = set()
+    for f in np.nonzero(ED[:, 6] == -1)[0]:
+        for k in range(c.ngl):
+            L, R = L3[f, :, k], R3[f, :, k]
+            if abs(nx[f, k]) < 1e-5:
+                seen.add("wall")
+                assert R[0] == L[0]
+                mn = lambda v: v[1] * nx[f, k] + v[2] * ny[f, k]
+                mt = lambda v: -v[1] * ny[f, k] + v[2] * nx[f, k]
+                assert abs(mn(R) + mn(L)) < 1e-14 and abs(mt(R) - mt(L)) < 1e-14
+            elif abs(x[ED[f, 0], side[ED[f, 2]](k)]) < 1e-5:
+                seen.add("inflow")
+                assert tuple(R) == inflow
+            else:
+                seen.add("outflow")
+                assert np.array_equal(R, L)
+    assert seen == {"wall", "inflow", "outflow"}
+
+
 def test_open_boundaries_keep_still_water():
     """Still water h + b = h_zero at rest: the ocean boundary's exterior state equals the interior
     one on the open part and the residual vanishes to roundoff; a wrong still-water level does not.
