@@ -12,6 +12,10 @@
  */
 #include "esdg_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -37,6 +41,7 @@ static int side_node(int ngl, int side, int k) {
 void oracle_collect_edge_data_bottom(const oracle_params* p, int Nfaces, const int* EdgeData,
                                      const double* B, double* bL, double* bR) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int* E = EdgeData + 8 * f;
         const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
@@ -65,6 +70,7 @@ void oracle_collect_edge_data_walls(const oracle_params* p, int Nfaces, const in
                                     const double* q, const double* nx, const double* ny,
                                     double* qL, double* qR) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int* E = EdgeData + 8 * f;
         const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
@@ -107,6 +113,7 @@ void oracle_collect_edge_data(const oracle_params* p, int Nfaces, const int* Edg
                               const double* nx, const double* ny, const double* bL, const double* bR,
                               double* qL, double* qR) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int* E = EdgeData + 8 * f;
         const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
@@ -193,6 +200,7 @@ void oracle_collect_edge_data_xt(const oracle_params* p, int Nfaces, const int* 
                                  const double* x, const double* y, const double* nx, const double* ny,
                                  const double* bL, const double* bR, double t, double* qL, double* qR) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int* E = EdgeData + 8 * f;
         const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
@@ -231,6 +239,7 @@ void oracle_add_source(const oracle_params* p, int Nelem, const double* Bx, cons
                        const double* x, const double* y, double t, double* Qt) {
     const int ngl2 = p->ngl * p->ngl;
     const double g = p->g;
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e)
         for (int n = 0; n < ngl2; ++n) {
             const long k = (long)e * ngl2 + n;
@@ -288,8 +297,9 @@ void oracle_volume_fluxdiff(const oracle_params* p, int Nelem, const double* Jac
                             const double* D, const double* Bx, const double* By, double* Qt) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
     const double g = p->g, half_g = 0.5 * g, fourth_g = 0.25 * g;
-    double* inv = (double*)malloc(sizeof(double) * ngl2);
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e) {
+        double inv[ngl2];
         const double* Q1 = q + (long)e * ngl2 * NEQ;
         const double* Q2 = Q1 + ngl2;
         const double* Q3 = Q2 + ngl2;
@@ -326,7 +336,6 @@ void oracle_volume_fluxdiff(const oracle_params* p, int Nelem, const double* Jac
                 out[2 * ngl2] = -rJ * FD3 - g * Q1[n] * By[x0 + n];
             }
     }
-    free(inv);
 }
 
 /* index of the right state for left-ordered face node i (calcESFlux.okl:24-26) */
@@ -340,6 +349,7 @@ void oracle_num_flux_es(const oracle_params* p, int Nfaces, const int* EdgeRever
                         const double* qR, const double* BL, const double* BR, double* SP) {
     const int ngl = p->ngl;
     const double g = p->g, tol = p->PosPresTOL;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int rv = EdgeReversed[f];
         for (int i = 0; i < ngl; ++i) {
@@ -406,6 +416,7 @@ void oracle_num_flux_es_rotated(const oracle_params* p, int Nfaces, const int* E
                                 const double* BR, double* SP) {
     const int ngl = p->ngl;
     const double g = p->g, tol = p->PosPresTOL;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int rv = EdgeReversed[f];
         for (int i = 0; i < ngl; ++i) {
@@ -522,6 +533,7 @@ void oracle_num_flux_roe(const oracle_params* p, int Nfaces, const int* EdgeReve
                          const double* qR, const double* BL, const double* BR, double* SP) {
     (void)BL; (void)BR;
     const int ngl = p->ngl;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int rv = EdgeReversed[f];
         for (int i = 0; i < ngl; ++i) {
@@ -543,6 +555,7 @@ void oracle_num_flux_ocean(const oracle_params* p, int Nfaces, const int* EdgeDa
                            const double* qR, const double* BL, const double* BR, double* SP) {
     const int ngl = p->ngl;
     oracle_num_flux_es_rotated(p, Nfaces, EdgeReversed, nxa, nya, scal, qL, qR, BL, BR, SP);
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         if (EdgeData[8 * f + 6] != -1) continue;
         const int rv = EdgeReversed[f];
@@ -576,6 +589,7 @@ void oracle_surface(const oracle_params* p, int Nelem, const double* Jac, const 
                     const double* DB2, double* Qt) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
     static const int order[4] = {0, 2, 1, 3};
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e)
         for (int o = 0; o < 4; ++o) {
             const int s = order[o];
@@ -599,6 +613,7 @@ void oracle_surface(const oracle_params* p, int Nelem, const double* Jac, const 
 /* okl/DG/FindLambdaMax.okl:17-100 */
 void oracle_find_lambda_max(const oracle_params* p, int Nelem, const double* q, double* LambdaMax) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e) {
         double m = 0.0;
         for (int n = 0; n < ngl2; ++n) {
@@ -617,6 +632,7 @@ void oracle_find_lambda_max(const oracle_params* p, int Nelem, const double* q, 
 /* okl/RungeKutta/rkLS.okl:17-89 */
 void oracle_rk_ls_calc(const oracle_params* p, int Nelem, const double* Qt, double rkA, double* gRK) {
     const long n = (long)Nelem * p->ngl * p->ngl * NEQ;
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) gRK[i] = rkA * gRK[i] + Qt[i];
 }
 
@@ -624,6 +640,7 @@ void oracle_rk_ls_calc(const oracle_params* p, int Nelem, const double* Qt, doub
 void oracle_rk_ls_update(const oracle_params* p, int Nelem, double rkC, double dt, const double* gRK,
                          double* q) {
     const long n = (long)Nelem * p->ngl * p->ngl * NEQ;
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) q[i] += rkC * dt * gRK[i];
 }
 
@@ -631,6 +648,7 @@ void oracle_rk_ls_update(const oracle_params* p, int Nelem, double rkC, double d
 void oracle_rk_ssp_update(const oracle_params* p, int Nelem, double a, double b, double c, double dt,
                           const double* Qt, const double* qTmp, double* q) {
     const long n = (long)Nelem * p->ngl * p->ngl * NEQ;
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) q[i] = a * qTmp[i] + b * q[i] + c * dt * Qt[i];
 }
 
@@ -638,10 +656,11 @@ void oracle_rk_ssp_update(const oracle_params* p, int Nelem, double a, double b,
 void oracle_shock_capturing(const oracle_params* p, int Nelem, const double* q, const double* b,
                             const double* VdmInv, double* ViscPara, double* ViscForPlot) {
     const int ngl = p->ngl, ngl2 = ngl * ngl, N = ngl - 1;
-    double* U = (double*)malloc(sizeof(double) * ngl2);
-    double* M1 = (double*)malloc(sizeof(double) * ngl2);
-    double* m = (double*)malloc(sizeof(double) * ngl2);
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e) {
+        double U[ngl2];
+        double M1[ngl2];
+        double m[ngl2];
         for (int a = 0; a < ngl; ++a)
             for (int c = 0; c < ngl; ++c)
                 U[a * ngl + c] = b[(long)e * ngl2 + c * ngl + a] + q[(long)e * ngl2 * NEQ + c * ngl + a];
@@ -681,7 +700,6 @@ void oracle_shock_capturing(const oracle_params* p, int Nelem, const double* q, 
         ViscPara[e] = visc;
         if (ViscForPlot) ViscForPlot[e] = visc;
     }
-    free(U); free(M1); free(m);
 }
 
 /* okl/Positivity/PosPres.okl:17-223 */
@@ -689,6 +707,7 @@ void oracle_pos_pres(const oracle_params* p, int Nelem, const double* EleSizes, 
                      const double* Jac, double* q) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
     const double tol = p->PosPresTOL;
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e) {
         double* Q1 = q + (long)e * ngl2 * NEQ;
         double* Q2 = Q1 + ngl2;
@@ -733,6 +752,7 @@ void oracle_num_fluxes_gradient(const oracle_params* p, int Nfaces, const int* E
                                 const double* qL, const double* qR, double* SGX, double* SGY) {
     const int ngl = p->ngl;
     const double tol = p->PosPresTOL;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int cpuL = EdgeData[8 * f + 5], cpuR = EdgeData[8 * f + 6];
         for (int i = 0; i < ngl; ++i) {
@@ -760,9 +780,10 @@ void oracle_calc_gradient(const oracle_params* p, int Nelem, const double* Jac, 
                           const double* Yeta, const double* Xxi, const double* Xeta, const double* q,
                           const double* Dh, double* gX, double* gY) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
-    double* u = (double*)malloc(sizeof(double) * ngl2);
-    double* v = (double*)malloc(sizeof(double) * ngl2);
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e) {
+        double u[ngl2];
+        double v[ngl2];
         const double* Q = q + (long)e * ngl2 * NEQ;
         for (int n = 0; n < ngl2; ++n) {
             const double h = Q[n];
@@ -793,7 +814,6 @@ void oracle_calc_gradient(const oracle_params* p, int Nelem, const double* Jac, 
                 gY[id + ngl2] = rJ * v_y;
             }
     }
-    free(u); free(v);
 }
 
 /* okl/BR1_Gradient/SurfaceKernelGradient.okl:16-148 */
@@ -802,6 +822,7 @@ void oracle_surface_gradient(const oracle_params* p, int Nelem, const double* Ja
                              double* gX, double* gY) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
     static const int order[4] = {0, 2, 1, 3};
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e)
         for (int o = 0; o < 4; ++o) {
             const int s = order[o];
@@ -826,6 +847,7 @@ void oracle_surface_gradient(const oracle_params* p, int Nelem, const double* Ja
 /* okl/BR1_Gradient/scaleGradient.okl:17-60 */
 void oracle_scale_gradient(const oracle_params* p, int Nelem, const double* q, double* gX, double* gY) {
     const int ngl2 = p->ngl * p->ngl;
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e)
         for (int n = 0; n < ngl2; ++n) {
             const double h = q[(long)e * ngl2 * NEQ + n];
@@ -845,6 +867,7 @@ void oracle_collect_edge_data_gradient(const oracle_params* p, int Nfaces, const
                                        double* VL, double* VR, double* gXL, double* gXR,
                                        double* gYL, double* gYR) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int* E = EdgeData + 8 * f;
         const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], rv = E[4], cpuL = E[5], cpuR = E[6];
@@ -885,11 +908,12 @@ void oracle_volume_viscose(const oracle_params* p, int Nelem, const double* Jac,
                            const double* gX, const double* gY, const double* Ds,
                            const double* ViscPara, double* Qt) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
-    double* F2 = (double*)malloc(sizeof(double) * ngl2);
-    double* F3 = (double*)malloc(sizeof(double) * ngl2);
-    double* G2 = (double*)malloc(sizeof(double) * ngl2);
-    double* G3 = (double*)malloc(sizeof(double) * ngl2);
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e) {
+        double F2[ngl2];
+        double F3[ngl2];
+        double G2[ngl2];
+        double G3[ngl2];
         const double eps = ViscPara[e];
         for (int n = 0; n < ngl2; ++n) {
             const long xid = (long)e * ngl2 + n;
@@ -917,7 +941,6 @@ void oracle_volume_viscose(const oracle_params* p, int Nelem, const double* Jac,
                 out[2 * ngl2] += rJ * d3;
             }
     }
-    free(F2); free(F3); free(G2); free(G3);
 }
 
 /* okl/ViscoseParts/calcNumFluxesViscose.okl:17-107 */
@@ -927,6 +950,7 @@ void oracle_num_fluxes_viscose(const oracle_params* p, int Nfaces, const int* Ed
                                const double* gXR, const double* gYL, const double* gYR,
                                double* SPV) {
     const int ngl = p->ngl;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int cpuL = EdgeData[8 * f + 5], cpuR = EdgeData[8 * f + 6];
         for (int i = 0; i < ngl; ++i) {
@@ -954,6 +978,7 @@ void oracle_surface_visc(const oracle_params* p, int Nelem, const double* Jac, c
                          const int* EOR, const int* E2E, const double* SPV, double* Qt) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
     static const int order[4] = {0, 2, 1, 3};
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e)
         for (int o = 0; o < 4; ++o) {
             const int s = order[o];
@@ -970,6 +995,15 @@ void oracle_surface_visc(const oracle_params* p, int Nelem, const double* Jac, c
                 out[2 * ngl2] += -ms * rJ * SPV[fl + ngl];
             }
         }
+}
+
+/* threads the element and face loops run on (OpenMP; 1 without it) */
+int oracle_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* deviceclass::DGtimeloop, src/deviceclass.cpp:868-1130, single rank, with a step cap. */
@@ -1106,6 +1140,7 @@ void oracle_num_flux_lf(const oracle_params* p, int Nfaces, const int* EdgeRever
     (void)BL; (void)BR;
     const int ngl = p->ngl;
     const double g = p->g, tol = p->PosPresTOL;
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < Nfaces; ++f) {
         const int rv = EdgeReversed[f];
         for (int i = 0; i < ngl; ++i) {
@@ -1150,9 +1185,10 @@ void oracle_volume_std(const oracle_params* p, int Nelem, const double* Jac, con
                        const double* D, const double* Bx, const double* By, double* Qt) {
     const int ngl = p->ngl, ngl2 = ngl * ngl;
     const double g = p->g, half_g = 0.5 * g;
-    double* F = (double*)malloc(sizeof(double) * ngl2 * 3);
-    double* Gf = (double*)malloc(sizeof(double) * ngl2 * 3);
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e) {
+        double F[ngl2*3];
+        double Gf[ngl2*3];
         const double* Q = q + (long)e * ngl2 * NEQ;
         for (int n = 0; n < ngl2; ++n) {
             const long xid = (long)e * ngl2 + n;
@@ -1184,5 +1220,4 @@ void oracle_volume_std(const oracle_params* p, int Nelem, const double* Jac, con
                 out[2 * ngl2] = -Jac[xid] * d[2] - g * Q[n] * By[xid];
             }
     }
-    free(F); free(Gf);
 }

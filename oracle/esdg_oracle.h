@@ -120,6 +120,8 @@ typedef struct oracle_run {
     const double *x, *y;   /* node coordinates (the convergence-test boundary and source) */
 } oracle_run;
 
+/* threads used by the element and face loops (OpenMP) */
+int oracle_threads(void);
 int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, double* t_out,
                     double* dt_hist);
 
