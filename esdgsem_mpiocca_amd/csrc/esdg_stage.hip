@@ -298,14 +298,6 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
                     S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax);
 }
 
-// Non-inlined entry for the mixed-degree kernel: every degree keeps its own register allocation
-// instead of one allocation for the union of all six bodies.
-template <int NGL>
-__device__ __attribute__((noinline)) void stage_body_call(const StageScn& S, int e0, double* lds, int src, double rkA,
-                                                          double rkC, bool last) {
-    stage_body<NGL>(S, e0, lds, src, rkA, rkC, last);
-}
-
 template <int LO, int HI>
 __device__ __forceinline__ void stage_task(const StageScn* __restrict__ scns, const StageTask& tk, double* lds, int src,
                                            double rkA, double rkC, bool last) {
@@ -313,10 +305,7 @@ __device__ __forceinline__ void stage_task(const StageScn* __restrict__ scns, co
     switch (tk.ngl) {
 #define ESDG_CASE(n)                                                                \
     case n:                                                                         \
-        if constexpr (n >= LO && n <= HI) {                                         \
-            if constexpr (LO == HI) stage_body<n>(S, tk.e0, lds, src, rkA, rkC, last);      \
-            else stage_body_call<n>(S, tk.e0, lds, src, rkA, rkC, last);                   \
-        }                                                                           \
+        if constexpr (n >= LO && n <= HI) stage_body<n>(S, tk.e0, lds, src, rkA, rkC, last);        \
         break;
         ESDG_CASE(3) ESDG_CASE(4) ESDG_CASE(5) ESDG_CASE(6) ESDG_CASE(7) ESDG_CASE(8)
 #undef ESDG_CASE

@@ -93,7 +93,11 @@ def generate(dev, nel=512, runs=16, snapshot_steps=(15, 15, 15, 15), smooth_per_
             ratio, angle = rng.uniform(1.2, 5.0), rng.uniform(0.0, np.pi)
             solver.set_q(dam_break_state(case, ratio, angle, 1e-3, rng))
             for k in snapshot_steps:
-                solver.run(k)
+                try:
+                    solver.run(k)
+                except dev.EsdgDeviceError as e:   # a run that breaks down contributes what it has
+                    log(f"# run {r}: ratio {ratio:.2f} angle {angle:.2f} stopped: {e}")
+                    break
                 F, visc = lab()
                 pos = np.nonzero(visc > 0)[0]
                 neg = np.nonzero(visc <= 0)[0]

@@ -1,0 +1,83 @@
+"""GPU tests of the trained troubled-cell indicator (esdgsem_mpiocca_amd/indicator_weights.json).
+
+On the curved dam break of the benchmark family (N = 7, a smaller mesh), against the modal sensor
+everywhere (the judge):
+- the false-negative rate (element-stages the sensor marks troubled but the network declares
+  smooth) stays small;
+- the solution with the learned indicator stays within 1e-3 (relative L2) of the modal-sensor one;
+- what the trained network gains over the method without it: an untrained network of the same
+  shape, and no viscosity at all, both move the solution much further from the modal-sensor one.
+"""
+import numpy as np
+import pytest
+
+from esdgsem_mpiocca_amd import Case, indicator
+
+pytestmark = pytest.mark.gpu
+
+PARAMS, META = indicator.load_weights()
+needs_weights = pytest.mark.skipif(PARAMS is None, reason="no trained weights committed")
+
+LO, HI = 0.05, 0.5
+STEPS = 40
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import esdgsem_mpiocca_amd as E
+    return E.device()
+
+
+@pytest.fixture(scope="module")
+def case():
+    return Case.dam_break(N=7, nel=48, noise=1e-3, seed=0)
+
+
+def run(dev, case, rp, params=None, seed=None, audit=False):
+    ctx = dev.Context(dev.make_params(case, rp))
+    try:
+        s = dev.Solver(ctx, case, rp, fused=1)
+        net = None
+        if params is not None or seed is not None:
+            net = dev.MLP(ctx, seed=seed or 0, params=params)
+            s.set_indicator(net, LO, HI)
+            s.indicator_audit(audit)
+        n, _ = s.run(STEPS)
+        q = s.get_q()
+        counts = s.indicator_counts() if net else None
+        s.close()
+        if net:
+            net.close()
+    finally:
+        ctx.close()
+    assert n == STEPS
+    return q, counts
+
+
+def rel(case, a, b):
+    A, B = a.reshape(case.Nelem, 3, -1), b.reshape(case.Nelem, 3, -1)
+    return max(np.linalg.norm(A[:, v] - B[:, v]) / np.linalg.norm(B[:, v]) for v in range(3))
+
+
+@needs_weights
+def test_trained_indicator_against_the_modal_sensor(dev, case):
+    rp = indicator.av_run_params()
+    plain, _ = run(dev, case, rp)
+    learned, _ = run(dev, case, rp, params=PARAMS)
+    _, audit = run(dev, case, rp, params=PARAMS, audit=True)
+    fn_rate = audit["false_negatives"] / max(audit["sensor_positives"], 1)
+    assert audit["sensor_positives"] > 0
+    assert fn_rate <= 0.05, audit
+    assert np.isfinite(learned).all()
+    assert rel(case, learned, plain) <= 1e-3
+    # without the trained weights
+    untrained, uaudit = run(dev, case, rp, seed=12345, audit=True)
+    no_av, _ = run(dev, case, indicator.av_run_params(artificial_viscosity=0))
+    d_learned = rel(case, learned, plain)
+    d_untrained = rel(case, untrained, plain)
+    d_none = rel(case, no_av, plain)
+    print(f"false negatives {fn_rate:.4f} (untrained "
+          f"{uaudit['false_negatives'] / max(uaudit['sensor_positives'], 1):.4f}); "
+          f"rel L2 to the sensor run: learned {d_learned:.2e}, untrained {d_untrained:.2e}, no AV {d_none:.2e}")
+    assert d_none > 10 * d_learned
+    assert d_untrained > d_learned or uaudit["false_negatives"] > 10 * audit["false_negatives"]
