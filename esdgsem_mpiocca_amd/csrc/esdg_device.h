@@ -79,6 +79,54 @@ __device__ __forceinline__ FaceState gathered(const double* __restrict__ q, int 
     return left ? L : R;
 }
 
+// Element side traces of the state, [e][side][v][k] with k in the side's node order (side_node):
+// 12 ngl doubles per element, contiguous per side and field.  tr_put stores node n = j*ngl + i of
+// element e into every side it lies on.
+__device__ __forceinline__ void tr_put(double* __restrict__ tr, int ngl, int64_t e, int i, int j, double h, double hu,
+                                       double hv) {
+    double* T = tr + e * 12 * ngl;
+    auto put = [&](int side, int k) {
+        double* t = T + side * 3 * ngl + k;
+        t[0] = h; t[ngl] = hu; t[2 * ngl] = hv;
+    };
+    if (j == 0) put(0, i);
+    if (j == ngl - 1) put(2, i);
+    if (i == ngl - 1) put(1, j);
+    if (i == 0) put(3, j);
+}
+
+// gathered() with the interior traces read from the side-trace buffer instead of the state
+__device__ __forceinline__ FaceState gathered_tr(const double* __restrict__ tr, int ngl, const int* E, bool left, int k,
+                                                 double nx, double ny, const GatherConsts& gc, double bl, double br) {
+    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+    FaceState L{0.0, 0.0, 0.0}, R{0.0, 0.0, 0.0};
+    if (cpuL == 0) {
+        const double* t = tr + ((int64_t)iel * 4 + isl) * 3 * ngl + k;
+        L = FaceState{t[0], t[ngl], t[2 * ngl]};
+    }
+    if (cpuR == 0) {
+        const double* t = tr + ((int64_t)ier * 4 + isr) * 3 * ngl + k;
+        R = FaceState{t[0], t[ngl], t[2 * ngl]};
+    }
+    if (gc.bc == ESDG_BC_INNER_OUTER) {
+        if (cpuL == -1) { L.h = R.h; L.hu = 0.0; L.hv = 0.0; }
+        if (cpuR == -1) { R.h = L.h; R.hu = 0.0; R.hv = 0.0; }
+    } else if (gc.bc == ESDG_BC_OCEAN) {
+        if (cpuL == -1) {
+            if (br > 6.7) { L.h = R.h; wall_mirror(nx, ny, R.hu, R.hv, L.hu, L.hv); }
+            else { L.h = gc.h_zero - br; L.hu = 0.0; L.hv = 0.0; }
+        }
+        if (cpuR == -1) {
+            if (bl > 6.7) { R.h = L.h; wall_mirror(nx, ny, L.hu, L.hv, R.hu, R.hv); }
+            else { R.h = gc.h_zero - bl; R.hu = 0.0; R.hv = 0.0; }
+        }
+    } else {
+        if (cpuL == -1) { L.h = R.h; wall_mirror(nx, ny, R.hu, R.hv, L.hu, L.hv); }
+        if (cpuR == -1) { R.h = L.h; wall_mirror(nx, ny, L.hu, L.hv, R.hu, R.hv); }
+    }
+    return left ? L : R;
+}
+
 struct FluxConsts {
     double g, geomFace, tol;
     int ES;

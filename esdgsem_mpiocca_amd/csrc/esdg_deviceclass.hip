@@ -141,17 +141,25 @@ class deviceclass {
         return ESDG_OK;
     }
 
+    // The element viscosity of the current state (ShockCapturing, src/deviceclass.cpp:918 and
+    // :1045): the modal sensor, or with the learned indicator the network's classes and the sensor
+    // where the network does not settle the element as smooth.
+    int viscosity() {
+        if (!ind_) return esdg_shock_capturing(ctx_, Nelem, o_q, o_B, o_VdmInv, o_EleSizes, o_ViscPara, o_ViscForPlot);
+        int rc = mlp_classify(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, nullptr, o_estat);
+        if (rc == ESDG_OK)
+            rc = shock_capturing_masked(ctx_, Nelem, o_q, o_B, o_VdmInv, o_estat, ind_audit, o_icount, o_ViscPara,
+                                        o_ViscForPlot);
+        ind_element_stages += Nelem;
+        return rc;
+    }
+
     // One time step's dt (src/deviceclass.cpp:899-928), reductions on the device.
     int timestep(double* dt_out) {
         int rc = esdg_find_lambda_max(ctx_, Nelem, o_q, o_LambdaMax);
         if (rc == ESDG_OK) rc = esdg_reduce_max(ctx_, Nelem, o_LambdaMax, o_red);
         if (rc == ESDG_OK && rp_.artificial_viscosity) {
-            rc = esdg_shock_capturing(ctx_, Nelem, o_q, o_B, o_VdmInv, o_EleSizes, o_ViscPara, o_ViscForPlot);
-            if (rc == ESDG_OK && ind_) {
-                // learned indicator: classes of the step's first stage, viscosity kept where not smooth
-                rc = mlp_classify(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, nullptr, o_estat);
-                if (rc == ESDG_OK) rc = mlp_mask_viscosity(ctx_, Nelem, o_estat, o_ViscPara);
-            }
+            rc = viscosity();
             if (rc == ESDG_OK) rc = esdg_reduce_max(ctx_, Nelem, o_ViscPara, o_red + 1);
         }
         double red[2] = {0.0, 0.0};
@@ -231,6 +239,10 @@ class deviceclass {
             bottom_done = true;
         }
         if (rp_.fused == 2) return stage_timeloop(max_steps, steps_taken);
+        if (rp_.fused == 1) {
+            rc = fused_.begin(ctx_, o_q);
+            if (rc) return rc;
+        }
         int n = 0;
         while (t < rp_.T && n < max_steps) {
             double dt = 0.0;
@@ -244,10 +256,10 @@ class deviceclass {
                 if (rc) return rc;
             }
             for (int s = 0; s < stages; ++s) {
-                if (ind_ && rp_.artificial_viscosity) {
-                    if (s > 0) rc = mlp_classify(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, nullptr, o_estat);
+                // the fused stage takes the viscosity of its state from here (stage 0: the time step's)
+                if (rp_.fused && rp_.artificial_viscosity && s > 0) {
+                    rc = viscosity();
                     if (rc) return rc;
-                    ind_element_stages += Nelem;
                 }
                 if (rp_.fused) rc = fused_.stage(ctx_, s, A[s], Cc[s] * dt, fused_args());
                 else rc = stage(s, dt);
@@ -339,11 +351,6 @@ class deviceclass {
         a.EdgeData = o_EdgeData; a.EdgeReversed = o_EdgeReversed; a.ElemToEdge = o_ElemToEdge;
         a.EMS = o_ElemEdgeMasterSlave; a.EOR = o_ElemEdgeOrientation;
         a.ViscPara = o_ViscPara;
-        if (ind_) {
-            a.estat = o_estat;
-            a.audit = ind_audit;
-            a.counters = o_icount;
-        }
         return a;
     }
 
@@ -489,8 +496,8 @@ int esdg_solver_indicator_audit(esdg_solver* s, int audit) {
     return ESDG_OK;
 }
 
-int esdg_solver_indicator_counts(esdg_solver* s, int64_t* flagged, int64_t* false_negatives, int64_t* sensor_positives,
-                                 int64_t* element_stages) {
+int esdg_solver_indicator_counts(esdg_solver* s, int64_t* viscous, int64_t* false_negatives, int64_t* sensor_positives,
+                                 int64_t* not_smooth, int64_t* element_stages) {
     if (!s) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_indicator_counts: null solver");
     deviceclass& d = s->dc;
     unsigned long long c[4] = {0, 0, 0, 0};
@@ -500,9 +507,10 @@ int esdg_solver_indicator_counts(esdg_solver* s, int64_t* flagged, int64_t* fals
         if (!rc) rc = esdg_finish(d.ctx_);
         if (rc) return rc;
     }
-    if (flagged) *flagged = (int64_t)c[0];
+    if (viscous) *viscous = (int64_t)c[0];
     if (false_negatives) *false_negatives = (int64_t)c[1];
     if (sensor_positives) *sensor_positives = (int64_t)c[2];
+    if (not_smooth) *not_smooth = (int64_t)c[3];
     if (element_stages) *element_stages = d.ind_element_stages;
     d.ind_element_stages = 0;
     return ESDG_OK;

@@ -14,13 +14,9 @@ struct FusedArgs {
     const double *Jac, *Yxi, *Yeta, *Xxi, *Xeta, *B, *Bx, *By, *D, *Dhat, *Dstrong, *VdmInv, *GLw, *EleSizes;
     const double *nx, *ny, *scal, *bL, *bR;
     const int *EdgeData, *EdgeReversed, *ElemToEdge, *EMS, *EOR;
-    double* ViscPara;   // element viscosity of the stage (written by the fused kernels)
-    // learned indicator (null: the modal sensor everywhere).  estat[e]: 0 smooth, 1 troubled,
-    // 2 unsure (the sensor decides); audit evaluates the sensor everywhere and counts into
-    // counters[0] elements with viscosity, [1] sensor positives declared smooth, [2] sensor positives.
-    const signed char* estat = nullptr;
-    int audit = 0;
-    unsigned long long* counters = nullptr;
+    // element viscosity of the stage: the caller evaluates it (the modal sensor, or the learned
+    // indicator with the sensor) on the stage's state before the stage runs
+    const double* ViscPara;
 };
 
 class FusedStage {
@@ -31,6 +27,9 @@ class FusedStage {
     // One RK stage: residual, low-storage accumulate and update (gRK = rkA gRK + Qt; q += cdt gRK).
     // With AV: face -> AV element pass -> viscous face -> volume + surface + viscous + RK [-> limiter].
     int stage(esdg_ctx* ctx, int s, double rkA, double cdt, const FusedArgs& a);
+    // side traces of q (the face pass reads them; each stage's update refreshes them): called
+    // whenever q was set from outside the stages
+    int begin(esdg_ctx* ctx, const double* q);
     void release(esdg_ctx* ctx);
     void* buffer(const std::string& name);
 
@@ -43,6 +42,7 @@ class FusedStage {
     double* grad = nullptr;     // side traces of the scaled velocity gradients [e][side][4][ngl] (AV)
     double* fvisc = nullptr;    // viscous face fluxes [f][2][ngl] (AV)
     double* qvisc = nullptr;    // viscous volume term [e][2][ngl^2] (hu, hv) (AV)
+    double* tr = nullptr;       // side traces of the state [e][side][3][ngl]
 };
 
 }  // namespace esdg

@@ -3,10 +3,16 @@
 // Reference sequence per RK stage (src/deviceclass.cpp:957-1122) and what the fused schedule
 // does with it:
 //   CollectEdgeData + calcNumFluxes [+ calcNumFluxesGradient]      -> fused_face (one pass over faces,
-//                                                                     traces read straight from q)
-//   [ShockCapturing + calcGradient + SurfaceKernelGradient + scaleGradient + VolumeKernelViscose]
+//                                                                     traces read from the side-trace
+//                                                                     buffer the previous update wrote)
+//   [ShockCapturing]                                                -> the sensor pass, run by the caller
+//                                                                     before the stage (the time step's
+//                                                                     sensor serves stage 0, as there)
+//   [calcGradient + SurfaceKernelGradient + scaleGradient + VolumeKernelViscose]
 //                                                                   -> fused_av (one element pass; the
-//                                                                     viscous volume term goes to Qv)
+//                                                                     viscous volume term goes to Qv;
+//                                                                     blocks with no viscosity in reach
+//                                                                     leave at once)
 //   [CollectEdgeDataGradient + calcNumFluxesViscose]                -> fused_viscous_face
 //   VolumeKernel + SurfaceKernel [+ Qv + SurfaceKernelVisc] + calcRK + UpdateKernel
 //                                                                   -> fused_volume (esdg_volume.hip),
@@ -22,7 +28,10 @@
 namespace esdg {
 
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV);
+                 const double* Qv, const double* SPV, double* tr);
+int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
+                    double* tr);
+int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr);
 
 // ---------------------------------------------------------------------------------------------
 // fused_face: CollectEdgeData (SolidWalls.okl:16-207) + calcNumFluxes (calcESFlux.okl /
@@ -30,7 +39,7 @@ int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double 
 // ---------------------------------------------------------------------------------------------
 template <bool AV>
 __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kind, FluxConsts c, GatherConsts gc,
-                                                    const int* __restrict__ ED, const double* __restrict__ q,
+                                                    const int* __restrict__ ED, const double* __restrict__ tr,
                                                     const double* __restrict__ nxa, const double* __restrict__ nya,
                                                     const double* __restrict__ scal, const double* __restrict__ BL,
                                                     const double* __restrict__ BR, double* __restrict__ SP,
@@ -43,9 +52,10 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
     const int ir = (1 - rv) * (ngl - 1 - i) + rv * i;
     const int64_t xidR = (int64_t)f * ngl + ir;
     const double nx = nxa[t], ny = nya[t], sc = scal[t];
-    // qL at node i and qR at node ir, exactly as the gather kernel stores them
-    const FaceState L = gathered(q, ngl, E, true, i, nx, ny, gc, BL[t], BR[t]);
-    const FaceState R = gathered(q, ngl, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
+    // qL at node i and qR at node ir, exactly as the gather kernel stores them (the element side
+    // traces come from the side-trace buffer: contiguous per side, unlike the columns of q)
+    const FaceState L = gathered_tr(tr, ngl, E, true, i, nx, ny, gc, BL[t], BR[t]);
+    const FaceState R = gathered_tr(tr, ngl, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
     double F0, F1, F2;
     numerical_flux(kind, c, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
     const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
@@ -55,7 +65,7 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
     if (AV) {
         // calcNumFluxesGradient reads qL and qR unreversed at node i
         const int cpuL = E[5], cpuR = E[6];
-        const FaceState Ri = (rv == 1) ? R : gathered(q, ngl, E, false, i, nx, ny, gc, BL[t], BR[t]);
+        const FaceState Ri = (rv == 1) ? R : gathered_tr(tr, ngl, E, false, i, nx, ny, gc, BL[t], BR[t]);
         const FaceState LL = (cpuL == -1) ? Ri : L;
         const FaceState RR = (cpuR == -1) ? L : Ri;
         const double iL = (LL.h > c.tol) ? 1.0 / LL.h : 0.0;
@@ -76,53 +86,44 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
 // kernel) as element-side traces, the element viscosity and the viscous volume term Qv(hu, hv).
 // ---------------------------------------------------------------------------------------------
 //
-// With the learned indicator (LEARN): estat[e] = 0 (smooth) forces zero viscosity, 1 and 2 keep the
-// sensor's value.  A block whose elements and their face neighbours are all smooth has nothing to
-// contribute (the viscous terms and face fluxes it feeds are multiplied by zero viscosities): it
-// writes zero viscosities and leaves, unless audit is set.
-template <int NGL, bool LEARN>
-__global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorConsts sc,
-                                                  const signed char* __restrict__ estat, int audit,
-                                                  unsigned long long* __restrict__ counters,
-                                                  const int* __restrict__ ED,
-                                                  const double* __restrict__ q, const double* __restrict__ b,
-                                                  const double* __restrict__ VdmInv, const double* __restrict__ Dh,
+// The element viscosity comes in (evaluated on the stage's state before the stage).  A block
+// whose elements and their face neighbours all have zero viscosity has nothing to contribute: its
+// gradient traces feed only faces of zero viscosity (whose fluxes are zero without reading them)
+// and its viscous volume term is skipped where the viscosity is zero.  Such a block leaves at once.
+template <int NGL>
+__global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const int* __restrict__ ED,
+                                                  const double* __restrict__ q, const double* __restrict__ Dh,
                                                   const double* __restrict__ Ds, const double* __restrict__ Jac,
                                                   const double* __restrict__ Yxi, const double* __restrict__ Yeta,
                                                   const double* __restrict__ Xxi, const double* __restrict__ Xeta,
                                                   const int* __restrict__ E2E, const int* __restrict__ EMS,
                                                   const int* __restrict__ EOR, const double* __restrict__ SGX,
                                                   const double* __restrict__ SGY, double* __restrict__ gtr,
-                                                  double* __restrict__ visc, double* __restrict__ Qv) {
+                                                  const double* __restrict__ visc, double* __restrict__ Qv) {
     using EB = ElemBlock<NGL>;
     constexpr int NN = NGL * NGL;
-    __shared__ double sV[NN], sDh[NN], sDs[NN];
+    __shared__ double sDh[NN], sDs[NN];
     __shared__ double sA[EB::THREADS], sB[EB::THREADS], sC[EB::THREADS], sE[EB::THREADS];
     __shared__ double sEps[EB::EPB];
-    __shared__ double sSum[EB::EPB][4];
     __shared__ int sActive;
     const int el = threadIdx.x / NN, n = threadIdx.x % NN;
     const int e = blockIdx.x * EB::EPB + el;
     const bool act = e < Nelem;
-    if (LEARN) {
-        if (threadIdx.x == 0) sActive = audit;
-        __syncthreads();
-        if (act && n < 5) {
-            int nb = e;
-            if (n < 4) {
-                const int f = E2E[e * 4 + n] - 1;
-                nb = (ED[8 * f + 6] == -1) ? e : (ED[8 * f] == e ? ED[8 * f + 1] : ED[8 * f]);
-            }
-            if (estat[nb] != 0) sActive = 1;
+    if (threadIdx.x == 0) sActive = 0;
+    __syncthreads();
+    if (act && n < 5) {
+        int nb = e;
+        if (n < 4) {
+            const int f = E2E[e * 4 + n] - 1;
+            nb = (ED[8 * f + 6] == -1) ? e : (ED[8 * f] == e ? ED[8 * f + 1] : ED[8 * f]);
         }
-        __syncthreads();
-        if (!sActive) {
-            if (act && n == 0) visc[e] = 0.0;
-            return;
-        }
+        const double vn = visc[nb];
+        if (vn != 0.0) sActive = 1;
+        if (n == 4) sEps[el] = vn;
     }
+    __syncthreads();
+    if (!sActive) return;
     if (threadIdx.x < NN) {
-        sV[threadIdx.x] = VdmInv[threadIdx.x];
         sDh[threadIdx.x] = Dh[threadIdx.x];
         sDs[threadIdx.x] = Ds[threadIdx.x];
     }
@@ -132,62 +133,11 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, SensorC
     if (act) {
         const int64_t id = (int64_t)e * NN * ESDG_NEQ + n;
         h = q[id]; hu = q[id + NN]; hv = q[id + 2 * NN];
-        // sensor input: slot (a', c') = n' holds h+b at node (xi = a', eta = c'), i.e. node c'*NGL + a'
-        const int aa = n / NGL, cc = n % NGL;
-        const int node = cc * NGL + aa;
-        sA[threadIdx.x] = b[(int64_t)e * NN + node] + q[(int64_t)e * NN * ESDG_NEQ + node];
         const double Qinv = (h > tol) ? 1.0 / h : 0.0;
         sB[threadIdx.x] = hu * Qinv;                // u at node n
         sC[threadIdx.x] = hv * Qinv;                // v at node n
     }
     __syncthreads();
-    // ---- sensor: modal transform of h + b ----
-    double* U = sA + el * NN;
-    {
-        const int aa = n / NGL, cc = n % NGL;
-        double s = 0.0;
-        if (act) {
-#pragma unroll
-            for (int l = 0; l < NGL; ++l) s += sV[aa * NGL + l] * U[l * NGL + cc];
-        }
-        __syncthreads();
-        if (act) sE[threadIdx.x] = s;
-        __syncthreads();
-        double* M = sE + el * NN;
-        s = 0.0;
-        if (act) {
-#pragma unroll
-            for (int l = 0; l < NGL; ++l) s += M[aa * NGL + l] * sV[cc * NGL + l];
-        }
-        __syncthreads();
-        if (act) U[n] = s;
-        __syncthreads();
-        // the four energy sums of an element run on four threads, each in the okl's serial order
-        if (threadIdx.x < 4 * EB::EPB) {
-            const int ee = threadIdx.x >> 2, which = threadIdx.x & 3;
-            if (blockIdx.x * EB::EPB + ee < Nelem) sSum[ee][which] = sensor_sum<NGL>(sA + ee * NN, which);
-        }
-        __syncthreads();
-        if (threadIdx.x < EB::EPB) {
-            const int ee = blockIdx.x * EB::EPB + threadIdx.x;
-            double v = 0.0;
-            if (ee < Nelem) {
-                const double* S4 = sSum[threadIdx.x];
-                v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
-                if (LEARN) {
-                    const double plain = v;
-                    if (estat[ee] == 0) v = 0.0;
-                    if (counters) {
-                        if (v > 0.0) atomicAdd(counters, 1ull);
-                        if (audit && plain > 0.0 && estat[ee] == 0) atomicAdd(counters + 1, 1ull);
-                        if (audit && plain > 0.0) atomicAdd(counters + 2, 1ull);
-                    }
-                }
-                visc[ee] = v;
-            }
-            sEps[threadIdx.x] = v;
-        }
-    }
     // ---- BR1 gradient (weak form volume part) ----
     const double* u = sB + el * NN;
     const double* v = sC + el * NN;
@@ -317,18 +267,11 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl,
 template <int NGL>
 void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     using EB = ElemBlock<NGL>;
-    SensorConsts sc{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
     const double* SGX = F.fgrad;
     const double* SGY = F.fgrad + (size_t)A.Nfaces * NGL * 2;
-    if (A.estat)
-        k_fused_av<NGL, true><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
-            A.Nelem, ctx->p.PosPresTOL, sc, A.estat, A.audit, A.counters, A.EdgeData, A.q, A.B, A.VdmInv, A.Dhat,
-            A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta, A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara,
-            F.qvisc);
-    else
-        k_fused_av<NGL, false><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
-            A.Nelem, ctx->p.PosPresTOL, sc, nullptr, 0, nullptr, A.EdgeData, A.q, A.B, A.VdmInv, A.Dhat, A.Dstrong,
-            A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta, A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, F.qvisc);
+    k_fused_av<NGL><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
+        A.Nelem, ctx->p.PosPresTOL, A.EdgeData, A.q, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
+        A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, F.qvisc);
 }
 
 
@@ -337,6 +280,7 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     const int ngl = ctx->p.ngl;
     const size_t fn = (size_t)Nf * ngl;
     int rc = esdg_malloc(ctx, (void**)&fstar, fn * 3 * sizeof(double));
+    if (!rc) rc = esdg_malloc(ctx, (void**)&tr, (size_t)Ne * 12 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fgrad, fn * 4 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * 16 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fvisc, fn * 2 * sizeof(double));
@@ -350,6 +294,11 @@ int FusedStage::setup(esdg_ctx*, const int*, const int*, const int*, const int*,
     return ESDG_OK;
 }
 
+int FusedStage::begin(esdg_ctx* ctx, const double* q) {
+    if (!ready) return set_error(ctx, ESDG_ERR_STATE, "fused stage used before init");
+    return write_traces(ctx, Nelem, q, tr);
+}
+
 int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArgs& A) {
     if (!ready) return set_error(ctx, ESDG_ERR_STATE, "fused stage used before init");
     const int ngl = ctx->p.ngl;
@@ -360,18 +309,20 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     {
         KernelScope ks(ctx, "fused_face");
         if (av)
-            k_fused_face<true><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, A.q, A.nx, A.ny, A.scal,
+            k_fused_face<true><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, tr, A.nx, A.ny, A.scal,
                                                             A.bL, A.bR, fstar, fgrad, fgrad + (size_t)A.Nfaces * ngl * 2);
         else
-            k_fused_face<false><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, A.q, A.nx, A.ny,
+            k_fused_face<false><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, tr, A.nx, A.ny,
                                                              A.scal, A.bL, A.bR, fstar, nullptr, nullptr);
         int rc = launch_check(ctx, "fused_face");
         if (rc) return rc;
     }
     if (!av) {
         // volume + surface + RK in one element pass; the limiter (if on) follows as its own pass
-        int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr);
-        if (!rc && A.positivity) rc = esdg_pos_pres(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q);
+        // with the limiter on, the limiter writes the traces of the limited state
+        double* vtr = A.positivity ? nullptr : tr;
+        int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr, vtr);
+        if (!rc && A.positivity) rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr);
         return rc;
     }
     int rc = ESDG_OK;
@@ -389,13 +340,13 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         if (rc) return rc;
     }
     // volume + surface + viscous terms + RK in one element pass, then the limiter
-    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc);
-    if (!rc && A.positivity) rc = esdg_pos_pres(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q);
+    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr);
+    if (!rc && A.positivity) rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr);
     return rc;
 }
 
 void FusedStage::release(esdg_ctx* ctx) {
-    for (double** p : {&qn, &fstar, &fgrad, &grad, &fvisc, &qvisc})
+    for (double** p : {&qn, &fstar, &fgrad, &grad, &fvisc, &qvisc, &tr})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
     ready = false;
 }
@@ -406,6 +357,7 @@ void* FusedStage::buffer(const std::string& name) {
     if (name == "Grad") return grad;
     if (name == "FVisc") return fvisc;
     if (name == "QVisc") return qvisc;
+    if (name == "Traces") return tr;
     return nullptr;
 }
 

@@ -309,18 +309,42 @@ __global__ void __launch_bounds__(256) k_update_ssp(int64_t n, double A, double 
 
 // ---------------------------------------------------------------------------------------------
 // ShockCapturing: okl/ShockCapturing/ShockCapturing.okl:17-234.  Node (a = xi, c = eta).
+// The four energy sums of an element run on four threads, each in the okl's serial order.
+//
+// MASK (learned indicator): estat[e] = 0 (the network settled the element as smooth) gives zero
+// viscosity; 1 and 2 keep the sensor's value.  Without audit the sensor is evaluated only in
+// blocks that hold a non-smooth element.  counters: [0] element-stages with viscosity,
+// [1] sensor positives declared smooth and [2] sensor positives (both audit only),
+// [3] element-stages the network did not settle as smooth.
 // ---------------------------------------------------------------------------------------------
-template <int NGL>
+template <int NGL, bool MASK>
 __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts sc, const double* __restrict__ q,
                                                          const double* __restrict__ b, const double* __restrict__ VdmInv,
+                                                         const signed char* __restrict__ estat, int audit,
+                                                         unsigned long long* __restrict__ counters,
                                                          double* __restrict__ visc, double* __restrict__ visc_plot) {
     using EB = ElemBlock<NGL>;
     __shared__ double sV[NGL * NGL];
     __shared__ double sU[EB::THREADS];
     __shared__ double sM[EB::THREADS];
+    __shared__ double sSum[EB::EPB][4];
+    __shared__ int sNeed;
     const int el = threadIdx.x / EB::NODES, n = threadIdx.x % EB::NODES;
     const int e = blockIdx.x * EB::EPB + el;
     const int a = n / NGL, cc = n % NGL;   // thread's (xi, eta) slot in the modal arrays
+    if (MASK) {
+        if (threadIdx.x == 0) sNeed = audit;
+        __syncthreads();
+        if (n == 0 && e < Nelem && estat[e] != 0) sNeed = 1;
+        __syncthreads();
+        if (!sNeed) {
+            if (n == 0 && e < Nelem) {
+                visc[e] = 0.0;
+                if (visc_plot) visc_plot[e] = 0.0;
+            }
+            return;
+        }
+    }
     if (threadIdx.x < NGL * NGL) sV[threadIdx.x] = VdmInv[threadIdx.x];
     if (e < Nelem) {
         // slot (a, cc) holds node (xi = a, eta = cc), i.e. node index cc*NGL + a
@@ -346,10 +370,27 @@ __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts
     __syncthreads();
     if (e < Nelem) U[n] = s;   // modal coefficients m[a][cc]
     __syncthreads();
+    if (threadIdx.x < 4 * EB::EPB) {
+        const int ee = threadIdx.x >> 2, which = threadIdx.x & 3;
+        if (blockIdx.x * EB::EPB + ee < Nelem) sSum[ee][which] = sensor_sum<NGL>(sU + ee * EB::NODES, which);
+    }
+    __syncthreads();
     if (threadIdx.x < EB::EPB) {
         const int ee = blockIdx.x * EB::EPB + threadIdx.x;
         if (ee < Nelem) {
-            const double v = sensor_from_modes<NGL>(sU + threadIdx.x * EB::NODES, sc);
+            const double* S4 = sSum[threadIdx.x];
+            double v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
+            if (MASK) {
+                const int st = estat[ee];
+                const double plain = v;
+                if (st == 0) v = 0.0;
+                if (counters) {
+                    if (v > 0.0) atomicAdd(counters, 1ull);
+                    if (audit && plain > 0.0 && st == 0) atomicAdd(counters + 1, 1ull);
+                    if (audit && plain > 0.0) atomicAdd(counters + 2, 1ull);
+                    if (st != 0) atomicAdd(counters + 3, 1ull);
+                }
+            }
             visc[ee] = v;
             if (visc_plot) visc_plot[ee] = v;
         }
@@ -359,10 +400,12 @@ __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts
 // ---------------------------------------------------------------------------------------------
 // PosPres: okl/Positivity/PosPres.okl:17-223
 // ---------------------------------------------------------------------------------------------
+// With tr (the fused schedule), the side traces of the limited state are written as well.
 template <int NGL>
 __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double zero_tol,
                                                   const double* __restrict__ EleSizes, const double* __restrict__ GLw,
-                                                  const double* __restrict__ Jac, double* __restrict__ q) {
+                                                  const double* __restrict__ Jac, double* __restrict__ q,
+                                                  double* __restrict__ tr) {
     using EB = ElemBlock<NGL>;
     __shared__ double sH[EB::THREADS], sHU[EB::THREADS], sHV[EB::THREADS], sJ[EB::THREADS];
     __shared__ double sAvg[EB::EPB][5];   // hAvg, huAvg, hvAvg, theta, hMin
@@ -420,11 +463,28 @@ __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double 
             q[id] = hn;
             q[id + EB::NODES] = hun;
             q[id + 2 * EB::NODES] = hvn;
+            if (tr) tr_put(tr, NGL, e, n % NGL, n / NGL, hn, hun, hvn);
         } else if (h < tol) {
             q[id + EB::NODES] = 0.0;
             q[id + 2 * EB::NODES] = 0.0;
+            if (tr) tr_put(tr, NGL, e, n % NGL, n / NGL, h, 0.0, 0.0);
+        } else if (tr) {
+            tr_put(tr, NGL, e, n % NGL, n / NGL, h, sHU[threadIdx.x], sHV[threadIdx.x]);
         }
     }
+}
+
+// side traces of a state (the fused schedule's face pass reads them)
+template <int NGL>
+__global__ void __launch_bounds__(256) k_write_traces(int Nelem, const double* __restrict__ q, double* __restrict__ tr) {
+    constexpr int NN = NGL * NGL;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nelem * NN) return;
+    const int64_t e = t / NN;
+    const int n = (int)(t % NN);
+    const int64_t id = e * NN * ESDG_NEQ + n;
+    const int i = n % NGL, j = n / NGL;
+    if (i == 0 || j == 0 || i == NGL - 1 || j == NGL - 1) tr_put(tr, NGL, e, i, j, q[id], q[id + NN], q[id + 2 * NN]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -639,16 +699,48 @@ void launch_lambda(esdg_ctx* ctx, int Nelem, const double* q, double* LM) {
     k_lambda_max<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.g, ctx->p.PosPresTOL, q, LM);
 }
 template <int NGL>
-void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* V, double* visc, double* vp) {
+void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* V,
+                   const signed char* estat, int audit, unsigned long long* counters, double* visc, double* vp) {
     using EB = ElemBlock<NGL>;
     SensorConsts sc{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
-    k_shock_capturing<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, sc, q, B, V, visc, vp);
+    if (estat)
+        k_shock_capturing<NGL, true><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
+            Nelem, sc, q, B, V, estat, audit, counters, visc, vp);
+    else
+        k_shock_capturing<NGL, false><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
+            Nelem, sc, q, B, V, nullptr, 0, nullptr, visc, vp);
 }
 template <int NGL>
-void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q) {
+void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q,
+                    double* tr) {
     using EB = ElemBlock<NGL>;
     k_pos_pres<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.PosPresTOL, ctx->p.ZeroTOL,
-                                                                                  sizes, GLw, Jac, q);
+                                                                                  sizes, GLw, Jac, q, tr);
+}
+template <int NGL>
+void launch_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr) {
+    k_write_traces<NGL><<<blocks_for((int64_t)Nelem * NGL * NGL, 256), 256, 0, ctx->stream>>>(Nelem, q, tr);
+}
+
+// limiter that also refreshes the side traces (fused schedule), and the trace writer
+int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
+                    double* tr) {
+    KernelScope ks(ctx, "pos_pres");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, tr);
+    return launch_check(ctx, "pos_pres");
+}
+// the modal sensor with the learned indicator's classes (zero viscosity where estat = 0)
+int shock_capturing_masked(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* VdmInv,
+                           const signed char* estat, int audit, unsigned long long* counters, double* visc,
+                           double* visc_plot) {
+    KernelScope ks(ctx, "shock_capturing");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_sensor, ctx, Nelem, q, B, VdmInv, estat, audit, counters, visc, visc_plot);
+    return launch_check(ctx, "shock_capturing");
+}
+int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr) {
+    KernelScope ks(ctx, "write_traces");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_traces, ctx, Nelem, q, tr);
+    return launch_check(ctx, "write_traces");
 }
 template <int NGL>
 void launch_gradient(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta, const double* Xxi,
@@ -812,7 +904,7 @@ int esdg_shock_capturing(esdg_ctx* ctx, int Nelem, const double* q, const double
     ESDG_REQUIRE(ctx, ctx->p.ngl >= 3, "esdg_shock_capturing: needs ngl >= 3");
     if (Nelem == 0) return ESDG_OK;
     ESDG_ENTRY("shock_capturing");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_sensor, ctx, Nelem, q, B, VdmInv, ViscPara, ViscForPlot);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_sensor, ctx, Nelem, q, B, VdmInv, nullptr, 0, nullptr, ViscPara, ViscForPlot);
     return launch_check(ctx, "shock_capturing");
 }
 
@@ -821,7 +913,7 @@ int esdg_pos_pres(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double
     ESDG_REQUIRE(ctx, Nelem >= 0 && EleSizes && GLw && Jac && q, "esdg_pos_pres: bad argument");
     if (Nelem == 0) return ESDG_OK;
     ESDG_ENTRY("pos_pres");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, nullptr);
     return launch_check(ctx, "pos_pres");
 }
 

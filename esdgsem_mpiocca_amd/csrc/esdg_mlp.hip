@@ -404,17 +404,6 @@ struct esdg_mlp {
 };
 
 namespace esdg {
-__global__ void k_mask_visc(int Nelem, const signed char* __restrict__ estat, double* __restrict__ visc) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < Nelem && estat[e] == 0) visc[e] = 0.0;
-}
-
-int mlp_mask_viscosity(esdg_ctx* ctx, int Nelem, const signed char* estat, double* visc) {
-    KernelScope ks(ctx, "mlp_mask");
-    k_mask_visc<<<blocks_for(Nelem, 256), 256, 0, ctx->stream>>>(Nelem, estat, visc);
-    return launch_check(ctx, "mlp_mask");
-}
-
 int mlp_classify(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B, const double* VdmInv,
                  float lo, float hi, float* p, signed char* estat) {
     KernelScope ks(ctx, "mlp_indicator");

@@ -41,6 +41,7 @@ struct SurfIn {
     const double* Qv;      // viscous volume term [e][2][ngl^2] (hu, hv)
     const double* visc;    // element viscosity
     const double* SPV;     // viscous face fluxes [f][2][ngl]
+    double* tr;            // side traces of the updated state (RK modes; may be null)
 };
 
 template <int NGL, int MODE>
@@ -205,9 +206,19 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 si.gRK[id] = g1;
                 si.gRK[id + C::NN] = g2;
                 si.gRK[id + 2 * C::NN] = g3;
-                q[id] = q[id] + si.cdt * g1;
-                q[id + C::NN] = q[id + C::NN] + si.cdt * g2;
-                q[id + 2 * C::NN] = q[id + 2 * C::NN] + si.cdt * g3;
+                const double n1 = q[id] + si.cdt * g1;
+                const double n2 = q[id + C::NN] + si.cdt * g2;
+                const double n3 = q[id + 2 * C::NN] + si.cdt * g3;
+                q[id] = n1;
+                q[id + C::NN] = n2;
+                q[id + 2 * C::NN] = n3;
+                // the new state also goes to this column of the element's LDS planes (each lane
+                // writes only the slots it read the row sums from), for the side traces below
+                if (si.tr) {
+                    S[0 * C::PLANE + k * C::RS + r] = n1;
+                    S[1 * C::PLANE + k * C::RS + r] = n2;
+                    S[2 * C::PLANE + k * C::RS + r] = n3;
+                }
             }
         } else {
 #pragma unroll
@@ -215,6 +226,24 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 Qt[ob + k * NGL] = b1[k];
                 Qt[ob + C::NN + k * NGL] = b2[k];
                 Qt[ob + 2 * C::NN + k * NGL] = b3[k];
+            }
+        }
+    }
+    if constexpr (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
+        // side traces of the new state, [e][side][v][k] (read by the next stage's face pass): the
+        // block's elements own one contiguous run of the trace buffer, written lane-consecutively
+        if (si.tr) {
+            __syncthreads();
+            constexpr int TPE = 12 * NGL;
+            const int nact = min(C::EPW, Nelem - (int)blockIdx.x * C::EPW);
+            double* T = si.tr + (int64_t)blockIdx.x * C::EPW * TPE;
+            for (int t = lane; t < nact * TPE; t += 64) {
+                const int el2 = t / TPE, rem = t - el2 * TPE;
+                const int m = rem / NGL, k = rem - m * NGL;
+                const int side = m / 3, v = m - 3 * side;
+                const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : k;
+                const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : k;
+                T[t] = s[el2 * C::ESZ + v * C::PLANE + row * C::RS + col];
             }
         }
     }
@@ -285,9 +314,9 @@ void launch_volume_fluxdiff(esdg_ctx* ctx, int Nelem, const double* Jac, const d
 
 template <int NGL>
 void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar,
-                         double* Qt, const double* Qv, const double* SPV) {
+                         double* Qt, const double* Qv, const double* SPV, double* tr) {
     using C = VolCfg<NGL>;
-    SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV};
+    SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV, rk ? tr : nullptr};
     if (rk && Qv)
         k_volume_fluxdiff<NGL, VOL_SURF_VISC_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
@@ -299,12 +328,12 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, Qt, si);
 }
 
-// Fused volume + surface lift (+ viscous terms when Qv is given) (+ RK update in place when rk):
-// called by the fused stage.
+// Fused volume + surface lift (+ viscous terms when Qv is given) (+ RK update in place when rk,
+// with the side traces of the new state written to tr when given): called by the fused stage.
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV) {
+                 const double* Qv, const double* SPV, double* tr) {
     KernelScope ks(ctx, "fused_volume");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr);
     return launch_check(ctx, "fused_volume");
 }
 

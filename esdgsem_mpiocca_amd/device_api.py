@@ -288,9 +288,9 @@ class Solver:
         self.ctx.check(lib().esdg_solver_indicator_audit(C.c_void_p(self.handle), int(on)))
 
     def indicator_counts(self):
-        v = [C.c_int64() for _ in range(4)]
+        v = [C.c_int64() for _ in range(5)]
         self.ctx.check(lib().esdg_solver_indicator_counts(C.c_void_p(self.handle), *[C.byref(x) for x in v]))
-        return dict(zip(("flagged", "false_negatives", "sensor_positives", "element_stages"),
+        return dict(zip(("viscous", "false_negatives", "sensor_positives", "not_smooth", "element_stages"),
                         [x.value for x in v]))
 
     def view(self, name: str, shape, dtype=np.float64) -> DeviceView:

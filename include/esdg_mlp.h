@@ -60,19 +60,21 @@ int esdg_mlp_loss_grad(esdg_mlp* m, int n, const float* feat, const float* label
 int esdg_mlp_train_step(esdg_mlp* m, int n, const float* feat, const float* label, const int* idx,
                         float pos_weight, float lr, float* loss);
 
-/* Learned indicator in a solver's artificial-viscosity path (fused schedule, N = 7): before each
- * stage the network flags elements; p >= hi: troubled, p <= lo: smooth, in between the modal
- * sensor decides.  Smooth elements get no viscosity; troubled ones the modal sensor's value.
- * Elements with no troubled element among themselves and their face neighbours skip the BR1 and
- * viscous work.  m = null restores the modal sensor everywhere. */
+/* Learned indicator in a solver's artificial-viscosity path (fused schedule, N = 7): at every
+ * stage the network classifies the elements; p >= hi: troubled, p <= lo: smooth, in between
+ * unsure.  Smooth elements get no viscosity; troubled and unsure ones the modal sensor's value,
+ * and the sensor is evaluated only in element blocks holding such an element.  Elements with no
+ * viscosity among themselves and their face neighbours skip the BR1 and viscous work (as they do
+ * without the indicator).  m = null restores the modal sensor everywhere. */
 int esdg_solver_set_indicator(esdg_solver* s, esdg_mlp* m, float lo, float hi);
-/* Counts since the last call: element-stages flagged troubled by the indicator, and the modal
- * sensor's positives among the element-stages the network declared smooth (false negatives);
- * also the modal sensor's positives overall (needs the sensor evaluated everywhere: audit = 1 in
- * esdg_solver_indicator_audit). */
+/* audit = 1 evaluates the modal sensor everywhere (the solution is unchanged) so that the counts
+ * below include the sensor's verdict on the elements the network declared smooth. */
 int esdg_solver_indicator_audit(esdg_solver* s, int audit);
-int esdg_solver_indicator_counts(esdg_solver* s, int64_t* flagged, int64_t* false_negatives,
-                                 int64_t* sensor_positives, int64_t* element_stages);
+/* Counts since the last call, in element-stages: with viscosity; the sensor's positives among
+ * those the network declared smooth (false negatives, audit only); the sensor's positives (audit
+ * only); not declared smooth by the network (troubled or unsure); all. */
+int esdg_solver_indicator_counts(esdg_solver* s, int64_t* viscous, int64_t* false_negatives,
+                                 int64_t* sensor_positives, int64_t* not_smooth, int64_t* element_stages);
 
 #ifdef __cplusplus
 }

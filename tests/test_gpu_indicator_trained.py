@@ -5,8 +5,10 @@ everywhere (the judge):
 - the false-negative rate (element-stages the sensor marks troubled but the network declares
   smooth) stays small;
 - the solution with the learned indicator stays within 1e-3 (relative L2) of the modal-sensor one;
-- what the trained network gains over the method without it: an untrained network of the same
-  shape, and no viscosity at all, both move the solution much further from the modal-sensor one.
+- what the trained network gains: no viscosity at all moves the solution much further from the
+  modal-sensor one, and an untrained network of the same shape either defers nearly every element
+  to the sensor (p between the thresholds, so nothing is skipped) or misses far more troubled
+  elements.
 """
 import numpy as np
 import pytest
@@ -80,4 +82,11 @@ def test_trained_indicator_against_the_modal_sensor(dev, case):
           f"{uaudit['false_negatives'] / max(uaudit['sensor_positives'], 1):.4f}); "
           f"rel L2 to the sensor run: learned {d_learned:.2e}, untrained {d_untrained:.2e}, no AV {d_none:.2e}")
     assert d_none > 10 * d_learned
-    assert d_untrained > d_learned or uaudit["false_negatives"] > 10 * audit["false_negatives"]
+    # the trained network settles most element-stages as smooth itself: what it leaves to the
+    # sensor (troubled or unsure) is a small fraction
+    frac = audit["not_smooth"] / audit["element_stages"]
+    ufrac = uaudit["not_smooth"] / uaudit["element_stages"]
+    print(f"element-stages left to the sensor: learned {frac:.4f}, untrained {ufrac:.4f}; sensor positives "
+          f"{audit['sensor_positives'] / audit['element_stages']:.4f}")
+    assert frac <= 0.25
+    assert ufrac > 2 * frac or uaudit["false_negatives"] > 10 * max(audit["false_negatives"], 1)

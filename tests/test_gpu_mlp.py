@@ -178,9 +178,11 @@ def test_indicator_bands_reduce_to_known_schedules(dev):
     troubled, c1 = run_av(dev, case, rp, 8, indicator=True, lo=-1.0, hi=-1.0)
     unsure, _ = run_av(dev, case, rp, 8, indicator=True, lo=-1.0, hi=2.0)
     assert np.array_equal(plain, troubled) and np.array_equal(plain, unsure)
-    assert c1["flagged"] > 0 and c1["element_stages"] == 8 * 5 * case.Nelem
+    assert c1["viscous"] > 0 and c1["element_stages"] == 8 * 5 * case.Nelem
+    assert c1["not_smooth"] == c1["element_stages"]
     no_av_rp = RunParams(**{**rp.__dict__, "artificial_viscosity": 0})
     no_av, _ = run_av(dev, case, no_av_rp, 8)
     smooth, c0 = run_av(dev, case, rp, 8, indicator=True, lo=2.0, hi=2.0, audit=True)
     assert np.array_equal(no_av, smooth)
-    assert c0["flagged"] == 0 and c0["false_negatives"] == c0["sensor_positives"] > 0
+    assert c0["viscous"] == 0 and c0["false_negatives"] == c0["sensor_positives"] > 0
+    assert c0["not_smooth"] == 0
