@@ -60,7 +60,7 @@ KERNELS = ["volume_kernel", "collect_edge_data", "calc_num_fluxes", "surface_ker
            "surface_kernel_gradient", "scale_gradient", "collect_edge_data_gradient",
            "volume_kernel_viscose", "calc_num_fluxes_viscose", "surface_kernel_visc",
            "find_lambda_max", "reduce_max", "pos_pres", "fused_face", "fused_volume", "fused_av",
-           "fused_viscous_face", "fused_update", "fused_timestep"]
+           "fused_viscous_face", "fused_update", "fused_timestep", "mlp_indicator", "write_traces"]
 
 
 def c1_params(RunParams):
@@ -186,7 +186,9 @@ def c4_bench(dev, args):
     t0 = time.perf_counter()
     from esdgsem_mpiocca_amd import Case
     case = Case.flood(N=7, nel=args.c4_nel, seed=0)
-    rp = indicator.av_run_params(positivity=1, pos_pres_tol=1e-2, CFL=0.3)
+    # CFL 0.1: the dt formula's CFL/ngl stays below the smallest GLL weight fraction 1/(N(N+1)),
+    # the step size under which the limiter keeps the element means positive
+    rp = indicator.av_run_params(positivity=1, pos_pres_tol=1e-2, CFL=0.1)
     print(f"# c4: {case.Nelem} elements, case built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     out = {}
     qs = {}
@@ -199,16 +201,28 @@ def c4_bench(dev, args):
                 net = dev.MLP(ctx, params=params)
                 solver.set_indicator(net, args.ind_lo, args.ind_hi)
                 solver.indicator_audit(mode == "audit")
-            steps = args.c4_steps if mode != "audit" else args.c4_audit_steps
+            steps = args.c4_steps if mode != "audit" else (args.c4_audit_steps or args.c4_steps)
             done, wall = 0, 0.0
             chunk = max(1, min(steps, 50))
+            per_kernel = {}
             while done < steps:
                 k = min(chunk, steps - done)
+                last = done + k >= steps and mode != "audit"
+                if last:   # per-kernel times from the last chunk
+                    ctx.profile(True)
+                    ctx.profile_reset()
                 t1 = time.perf_counter()
                 n, t_sim = solver.run(k)
                 ctx.finish()
                 wall += time.perf_counter() - t1
                 done += n
+                if last:
+                    for kn in KERNELS:
+                        launches, ms = ctx.profile_query(kn)
+                        if launches:
+                            per_kernel[kn] = {"avg_us": round(1e3 * ms / launches, 2),
+                                              "ms_per_step": round(ms / n, 4)}
+                    ctx.profile(False)
                 print(f"# c4 {mode}: {done}/{steps} steps, {1e3 * wall / done:.1f} ms/step", file=sys.stderr,
                       flush=True)
                 if n != k:
@@ -218,7 +232,7 @@ def c4_bench(dev, args):
                 qs[mode] = solver.get_q()
                 dofs = 3 * case.ngl ** 2 * case.Nelem
                 out[mode] = {"dof_stage_updates_per_s": dofs * 5 * steps / wall, "ms_per_step": 1e3 * wall / steps,
-                             "steps": steps, "sim_time": t_sim}
+                             "steps": steps, "sim_time": t_sim, "kernels_last_chunk": per_kernel}
                 if counts:
                     es = max(counts["element_stages"], 1)
                     out[mode]["viscous_element_stage_fraction"] = counts["viscous"] / es
@@ -267,7 +281,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
     ap.add_argument("--c4-nel", type=int, default=2048)
     ap.add_argument("--c4-steps", type=int, default=1000)
-    ap.add_argument("--c4-audit-steps", type=int, default=20)
+    ap.add_argument("--c4-audit-steps", type=int, default=0, help="0: as many as --c4-steps")
     ap.add_argument("--ind-lo", type=float, default=0.05)
     ap.add_argument("--ind-hi", type=float, default=0.5)
     ap.add_argument("--workload", choices=["c1", "ensemble", "c4"], default="c1",

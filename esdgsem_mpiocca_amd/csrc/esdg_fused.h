@@ -43,6 +43,8 @@ class FusedStage {
     double* fvisc = nullptr;    // viscous face fluxes [f][2][ngl] (AV)
     double* qvisc = nullptr;    // viscous volume term [e][2][ngl^2] (hu, hv) (AV)
     double* tr = nullptr;       // side traces of the state [e][side][3][ngl]
+    int* active = nullptr;      // == seq: the element has viscosity in reach this stage (AV)
+    int seq = 0;                // stage counter for active
 };
 
 }  // namespace esdg

@@ -22,6 +22,8 @@
 // before the volume pass.  Every value is computed with the reference kernel's expressions in the
 // reference's order, and Qt receives its contributions in the reference's order (volume, surface,
 // viscous volume, viscous surface), so the fused stage reproduces the kernel-by-kernel results.
+#include <climits>
+
 #include "esdg_device.h"
 #include "esdg_fused.h"
 
@@ -43,7 +45,9 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
                                                     const double* __restrict__ nxa, const double* __restrict__ nya,
                                                     const double* __restrict__ scal, const double* __restrict__ BL,
                                                     const double* __restrict__ BR, double* __restrict__ SP,
-                                                    double* __restrict__ SGX, double* __restrict__ SGY) {
+                                                    double* __restrict__ SGX, double* __restrict__ SGY,
+                                                    const double* __restrict__ visc, int* __restrict__ active,
+                                                    int seq) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)Nfaces * ngl) return;
     const int f = (int)(t / ngl), i = (int)(t % ngl);
@@ -76,6 +80,15 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
         SGX[gid + ngl] = nx * sc * c.geomFace * 0.5 * (vL + vR);
         SGY[gid] = ny * sc * c.geomFace * 0.5 * (uL + uR);
         SGY[gid + ngl] = ny * sc * c.geomFace * 0.5 * (vL + vR);
+        // elements with viscosity in reach (on themselves or a face neighbour) need the viscous
+        // work: a face with viscosity on either side marks both its elements for this stage
+        if (i == 0) {
+            const bool hasL = cpuL == 0, hasR = cpuR == 0;
+            if ((hasL && visc[E[0]] != 0.0) || (hasR && visc[E[1]] != 0.0)) {
+                if (hasL) active[E[0]] = seq;
+                if (hasR) active[E[1]] = seq;
+            }
+        }
     }
 }
 
@@ -89,7 +102,8 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
 // The element viscosity comes in (evaluated on the stage's state before the stage).  A block
 // whose elements and their face neighbours all have zero viscosity has nothing to contribute: its
 // gradient traces feed only faces of zero viscosity (whose fluxes are zero without reading them)
-// and its viscous volume term is skipped where the viscosity is zero.  Such a block leaves at once.
+// and its viscous volume term is skipped where the viscosity is zero.  Such a block leaves at once;
+// the face pass marks the others (active[e] == seq).
 template <int NGL>
 __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const int* __restrict__ ED,
                                                   const double* __restrict__ q, const double* __restrict__ Dh,
@@ -99,7 +113,8 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const i
                                                   const int* __restrict__ E2E, const int* __restrict__ EMS,
                                                   const int* __restrict__ EOR, const double* __restrict__ SGX,
                                                   const double* __restrict__ SGY, double* __restrict__ gtr,
-                                                  const double* __restrict__ visc, double* __restrict__ Qv) {
+                                                  const double* __restrict__ visc, const int* __restrict__ active,
+                                                  int seq, double* __restrict__ Qv) {
     using EB = ElemBlock<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sDh[NN], sDs[NN];
@@ -111,15 +126,9 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const i
     const bool act = e < Nelem;
     if (threadIdx.x == 0) sActive = 0;
     __syncthreads();
-    if (act && n < 5) {
-        int nb = e;
-        if (n < 4) {
-            const int f = E2E[e * 4 + n] - 1;
-            nb = (ED[8 * f + 6] == -1) ? e : (ED[8 * f] == e ? ED[8 * f + 1] : ED[8 * f]);
-        }
-        const double vn = visc[nb];
-        if (vn != 0.0) sActive = 1;
-        if (n == 4) sEps[el] = vn;
+    if (act && n == 0) {
+        sEps[el] = visc[e];
+        if (active[e] == seq) sActive = 1;
     }
     __syncthreads();
     if (!sActive) return;
@@ -271,7 +280,7 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     const double* SGY = F.fgrad + (size_t)A.Nfaces * NGL * 2;
     k_fused_av<NGL><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
         A.Nelem, ctx->p.PosPresTOL, A.EdgeData, A.q, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
-        A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, F.qvisc);
+        A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, F.active, F.seq, F.qvisc);
 }
 
 
@@ -281,6 +290,7 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     const size_t fn = (size_t)Nf * ngl;
     int rc = esdg_malloc(ctx, (void**)&fstar, fn * 3 * sizeof(double));
     if (!rc) rc = esdg_malloc(ctx, (void**)&tr, (size_t)Ne * 12 * ngl * sizeof(double));
+    if (!rc && av) rc = esdg_malloc(ctx, (void**)&active, ((size_t)Ne + 1) / 2 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fgrad, fn * 4 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * 16 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fvisc, fn * 2 * sizeof(double));
@@ -306,14 +316,25 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     FluxConsts fc{ctx->p.g, ctx->p.geomFace, ctx->p.PosPresTOL, ctx->p.ES};
     GatherConsts gc{ctx->p.bc_kind, ctx->p.h_zero};
     const unsigned fb = blocks_for((int64_t)A.Nfaces * ngl, 256);
+    if (av) {
+        // stage mark of the elements with viscosity in reach; the marks of earlier stages are
+        // other values, so nothing needs clearing (except when the counter wraps)
+        if (++seq == INT_MAX) {
+            int rc = esdg_fill(ctx, reinterpret_cast<double*>(active), 0.0, ((size_t)Nelem + 1) / 2);
+            if (rc) return rc;
+            seq = 1;
+        }
+    }
     {
         KernelScope ks(ctx, "fused_face");
         if (av)
             k_fused_face<true><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, tr, A.nx, A.ny, A.scal,
-                                                            A.bL, A.bR, fstar, fgrad, fgrad + (size_t)A.Nfaces * ngl * 2);
+                                                            A.bL, A.bR, fstar, fgrad, fgrad + (size_t)A.Nfaces * ngl * 2,
+                                                            A.ViscPara, active, seq);
         else
             k_fused_face<false><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, tr, A.nx, A.ny,
-                                                             A.scal, A.bL, A.bR, fstar, nullptr, nullptr);
+                                                             A.scal, A.bL, A.bR, fstar, nullptr, nullptr, nullptr,
+                                                             nullptr, 0);
         int rc = launch_check(ctx, "fused_face");
         if (rc) return rc;
     }
@@ -348,6 +369,8 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
 void FusedStage::release(esdg_ctx* ctx) {
     for (double** p : {&qn, &fstar, &fgrad, &grad, &fvisc, &qvisc, &tr})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
+    if (active) { esdg_free(ctx, active); active = nullptr; }
+    seq = 0;
     ready = false;
 }
 

@@ -308,7 +308,11 @@ __global__ void __launch_bounds__(256) k_update_ssp(int64_t n, double A, double 
 }
 
 // ---------------------------------------------------------------------------------------------
-// ShockCapturing: okl/ShockCapturing/ShockCapturing.okl:17-234.  Node (a = xi, c = eta).
+// ShockCapturing: okl/ShockCapturing/ShockCapturing.okl:17-234.
+// One thread per element row (eta = r), 256 / NGL elements per block, so that many elements are
+// in flight per CU.  The modal coefficients m[a][c] = sum_l T[a][l] V[c][l] with
+// T[a][c] = sum_l V[a][l] u(xi = l, eta = c), u = h + b, each sum over l in ascending order as in
+// the okl; thread r forms column r of T from its own row of u, then row r of m from row r of T.
 // The four energy sums of an element run on four threads, each in the okl's serial order.
 //
 // MASK (learned indicator): estat[e] = 0 (the network settled the element as smooth) gives zero
@@ -317,83 +321,95 @@ __global__ void __launch_bounds__(256) k_update_ssp(int64_t n, double A, double 
 // [1] sensor positives declared smooth and [2] sensor positives (both audit only),
 // [3] element-stages the network did not settle as smooth.
 // ---------------------------------------------------------------------------------------------
+template <int NGL>
+struct RowBlock {
+    static constexpr int EPB = 256 / NGL;            // elements per block
+    static constexpr int THREADS = EPB * NGL;
+    static constexpr int RS = NGL + 1;               // padded LDS row stride of an element's modes
+    static constexpr int ES = NGL * RS + 1;          // padded LDS stride between elements
+};
+
 template <int NGL, bool MASK>
 __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts sc, const double* __restrict__ q,
                                                          const double* __restrict__ b, const double* __restrict__ VdmInv,
                                                          const signed char* __restrict__ estat, int audit,
                                                          unsigned long long* __restrict__ counters,
                                                          double* __restrict__ visc, double* __restrict__ visc_plot) {
-    using EB = ElemBlock<NGL>;
-    __shared__ double sV[NGL * NGL];
-    __shared__ double sU[EB::THREADS];
-    __shared__ double sM[EB::THREADS];
-    __shared__ double sSum[EB::EPB][4];
+    using RB = RowBlock<NGL>;
+    constexpr int NN = NGL * NGL;
+    __shared__ double sV[NN];
+    __shared__ double sT[RB::EPB * RB::ES];
+    __shared__ double sSum[RB::EPB][4];
     __shared__ int sNeed;
-    const int el = threadIdx.x / EB::NODES, n = threadIdx.x % EB::NODES;
-    const int e = blockIdx.x * EB::EPB + el;
-    const int a = n / NGL, cc = n % NGL;   // thread's (xi, eta) slot in the modal arrays
+    const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
+    const int e = blockIdx.x * RB::EPB + el;
+    const bool act = (el < RB::EPB) && (e < Nelem);
     if (MASK) {
         if (threadIdx.x == 0) sNeed = audit;
         __syncthreads();
-        if (n == 0 && e < Nelem && estat[e] != 0) sNeed = 1;
+        if (act && r == 0 && estat[e] != 0) sNeed = 1;
         __syncthreads();
         if (!sNeed) {
-            if (n == 0 && e < Nelem) {
+            if (act && r == 0) {
                 visc[e] = 0.0;
                 if (visc_plot) visc_plot[e] = 0.0;
             }
             return;
         }
     }
-    if (threadIdx.x < NGL * NGL) sV[threadIdx.x] = VdmInv[threadIdx.x];
-    if (e < Nelem) {
-        // slot (a, cc) holds node (xi = a, eta = cc), i.e. node index cc*NGL + a
-        const int node = cc * NGL + a;
-        sU[threadIdx.x] = b[(int64_t)e * EB::NODES + node] + q[(int64_t)e * EB::NODES * ESDG_NEQ + node];
-    }
-    __syncthreads();
-    double* U = sU + el * EB::NODES;
-    double* M = sM + el * EB::NODES;
-    double s = 0.0;
-    if (e < Nelem) {
+    if (threadIdx.x < NN) sV[threadIdx.x] = VdmInv[threadIdx.x];
+    double u[NGL];
+    if (act) {
+        const int64_t xb = (int64_t)e * NN + r * NGL;
+        const int64_t qb = (int64_t)e * NN * ESDG_NEQ + r * NGL;
 #pragma unroll
-        for (int l = 0; l < NGL; ++l) s += sV[a * NGL + l] * U[l * NGL + cc];
+        for (int i = 0; i < NGL; ++i) u[i] = b[xb + i] + q[qb + i];
     }
     __syncthreads();
-    if (e < Nelem) M[n] = s;
-    __syncthreads();
-    s = 0.0;
-    if (e < Nelem) {
+    double* T = sT + (act ? el : 0) * RB::ES;
+    if (act) {
+        // column c = r of T
 #pragma unroll
-        for (int l = 0; l < NGL; ++l) s += M[a * NGL + l] * sV[cc * NGL + l];
-    }
-    __syncthreads();
-    if (e < Nelem) U[n] = s;   // modal coefficients m[a][cc]
-    __syncthreads();
-    if (threadIdx.x < 4 * EB::EPB) {
-        const int ee = threadIdx.x >> 2, which = threadIdx.x & 3;
-        if (blockIdx.x * EB::EPB + ee < Nelem) sSum[ee][which] = sensor_sum<NGL>(sU + ee * EB::NODES, which);
-    }
-    __syncthreads();
-    if (threadIdx.x < EB::EPB) {
-        const int ee = blockIdx.x * EB::EPB + threadIdx.x;
-        if (ee < Nelem) {
-            const double* S4 = sSum[threadIdx.x];
-            double v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
-            if (MASK) {
-                const int st = estat[ee];
-                const double plain = v;
-                if (st == 0) v = 0.0;
-                if (counters) {
-                    if (v > 0.0) atomicAdd(counters, 1ull);
-                    if (audit && plain > 0.0 && st == 0) atomicAdd(counters + 1, 1ull);
-                    if (audit && plain > 0.0) atomicAdd(counters + 2, 1ull);
-                    if (st != 0) atomicAdd(counters + 3, 1ull);
-                }
-            }
-            visc[ee] = v;
-            if (visc_plot) visc_plot[ee] = v;
+        for (int a = 0; a < NGL; ++a) {
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < NGL; ++l) s += sV[a * NGL + l] * u[l];
+            T[a * RB::RS + r] = s;
         }
+    }
+    __syncthreads();
+    if (act) {
+        // row a = r of m, in place of row r of T (only this thread reads and writes that row)
+        double t[NGL];
+#pragma unroll
+        for (int l = 0; l < NGL; ++l) t[l] = T[r * RB::RS + l];
+#pragma unroll
+        for (int c = 0; c < NGL; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < NGL; ++l) s += t[l] * sV[c * NGL + l];
+            T[r * RB::RS + c] = s;
+        }
+    }
+    __syncthreads();
+    if (act && r < 4) sSum[el][r] = sensor_sum<NGL, RB::RS>(T, r);
+    __syncthreads();
+    if (act && r == 0) {
+        const double* S4 = sSum[el];
+        double v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
+        if (MASK) {
+            const int st = estat[e];
+            const double plain = v;
+            if (st == 0) v = 0.0;
+            if (counters) {
+                if (v > 0.0) atomicAdd(counters, 1ull);
+                if (audit && plain > 0.0 && st == 0) atomicAdd(counters + 1, 1ull);
+                if (audit && plain > 0.0) atomicAdd(counters + 2, 1ull);
+                if (st != 0) atomicAdd(counters + 3, 1ull);
+            }
+        }
+        visc[e] = v;
+        if (visc_plot) visc_plot[e] = v;
     }
 }
 
@@ -701,13 +717,13 @@ void launch_lambda(esdg_ctx* ctx, int Nelem, const double* q, double* LM) {
 template <int NGL>
 void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* V,
                    const signed char* estat, int audit, unsigned long long* counters, double* visc, double* vp) {
-    using EB = ElemBlock<NGL>;
+    using RB = RowBlock<NGL>;
     SensorConsts sc{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
     if (estat)
-        k_shock_capturing<NGL, true><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
+        k_shock_capturing<NGL, true><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(
             Nelem, sc, q, B, V, estat, audit, counters, visc, vp);
     else
-        k_shock_capturing<NGL, false><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
+        k_shock_capturing<NGL, false><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(
             Nelem, sc, q, B, V, nullptr, 0, nullptr, visc, vp);
 }
 template <int NGL>

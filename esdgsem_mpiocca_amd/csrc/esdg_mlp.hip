@@ -30,83 +30,112 @@ __device__ inline uint16_t f2bf(float f) {
 __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 __device__ inline float rbf(float f) { return bf2f(f2bf(f)); }
 
-template <typename T>
-__device__ inline T wave_sum(T v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-template <typename T>
-__device__ inline T wave_max(T v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
-}
+// Element features with one thread per element row (eta = r) and FEPB = 32 elements per
+// 256-thread block, so that many elements are in flight per CU.  Thread (el, r) returns the
+// normalised depths x[8r + i] (i < 8) in xn[] and the modal-shell feature of shell r in *modal:
+//   x[n] = (h_n - mean h) / max |h - mean h| (0 for a constant element),
+//   modal[k] = max(log10(E_k / E), -16) / 8 (-2 when E = 0), E_k the energy of the modal
+//   coefficients m = V u V^T of u = h + b with max(index) = k, E their total.
+constexpr int FEPB = 32;
+constexpr int FRS = 9;            // padded row stride of an element's 8x8 LDS tile
+constexpr int FES = 8 * FRS + 1;  // padded stride between elements
+struct FeatLds {
+    double V[64];
+    double T[FEPB * FES];
+    double red[FEPB][8];
+};
 
-// Features of element e by one wave (lane = node n = j*8 + i); returns the lane's feature values:
-// lane n gets x[n]; lanes 0..7 also get x[64 + lane] in *modal.  sU, sT: 64 doubles of LDS each.
-__device__ void element_features(const double* __restrict__ q, const double* __restrict__ B, const double* sV, int e,
-                                 double* sU, double* sT, float& xn, float& modal) {
-    const int n = threadIdx.x & 63;
-    const int64_t id = (int64_t)e * 64 * 3 + n;
-    const double h = q[id];
-    const double u = B[(int64_t)e * 64 + n] + h;
-    const double mean = wave_sum(h) * (1.0 / 64.0);
-    const double dev = h - mean;
-    const double s = wave_max(fabs(dev));
-    xn = (s > 0.0) ? (float)(dev / s) : 0.0f;
-    // modal coefficients m(k1, k2) = sum_{i,j} V[k1][i] V[k2][j] u(i, j)
-    sU[n] = u;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    {
-        const int j = n >> 3, k1 = n & 7;
+__device__ void row_features(const double* __restrict__ q, const double* __restrict__ B, FeatLds& L, int e, bool act,
+                             float* xn, float& modal) {
+    const int el = threadIdx.x >> 3, r = threadIdx.x & 7;
+    double h[8], u[8];
+    double rs = 0.0;
+    if (act) {
+        const int64_t qb = (int64_t)e * 64 * 3 + r * 8;
+        const int64_t xb = (int64_t)e * 64 + r * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            h[i] = q[qb + i];
+            u[i] = B[xb + i] + h[i];
+            rs += h[i];
+        }
+    }
+    L.red[el][r] = rs;
+    __syncthreads();
+    double mean = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mean += L.red[el][k];
+    mean *= (1.0 / 64.0);
+    double dev[8], mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        dev[i] = act ? h[i] - mean : 0.0;
+        mx = fmax(mx, fabs(dev[i]));
+    }
+    __syncthreads();
+    L.red[el][r] = mx;
+    double* T = L.T + el * FES;
+    // column r of T = V u (sum over xi)
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
         double t = 0.0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) t += sV[k1 * 8 + i] * sU[j * 8 + i];
-        sT[n] = t;   // t(j, k1)
+        for (int l = 0; l < 8; ++l) t += L.V[a * 8 + l] * u[l];
+        T[a * FRS + r] = act ? t : 0.0;
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    double m = 0.0;
-    {
-        const int k2 = n >> 3, k1 = n & 7;
+    __syncthreads();
+    double sdev = 0.0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) m += sV[k2 * 8 + j] * sT[j * 8 + k1];
+    for (int k = 0; k < 8; ++k) sdev = fmax(sdev, L.red[el][k]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) xn[i] = (sdev > 0.0) ? (float)(dev[i] / sdev) : 0.0f;
+    // row r of m = T V^T (sum over eta), in place of row r of T (only this thread touches it)
+    {
+        double t[8];
+#pragma unroll
+        for (int l = 0; l < 8; ++l) t[l] = T[r * FRS + l];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            double m = 0.0;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) m += L.V[c * 8 + l] * t[l];
+            T[r * FRS + c] = m;
+        }
     }
-    const int shell = max(n >> 3, n & 7);
-    const double e2 = m * m;
-    double Ek[8];
+    __syncthreads();
+    // shell r: row r up to the diagonal and column r above it
+    double Ek = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (c <= r) Ek += T[r * FRS + c] * T[r * FRS + c];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+        if (a < r) Ek += T[a * FRS + r] * T[a * FRS + r];
+    L.red[el][r] = Ek;
+    __syncthreads();
     double Et = 0.0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        Ek[k] = wave_sum(shell == k ? e2 : 0.0);
-        Et += Ek[k];
-    }
-    modal = -2.0f;
-    if (n < 8 && Et > 0.0) {
-        double mine = Ek[0];
-#pragma unroll
-        for (int k = 1; k < 8; ++k)
-            if (n == k) mine = Ek[k];
-        modal = (float)(fmax(log10(mine / Et), -16.0) / 8.0);
-    }
-    __builtin_amdgcn_wave_barrier();
+    for (int k = 0; k < 8; ++k) Et += L.red[el][k];
+    modal = (Et > 0.0) ? (float)(fmax(log10(Ek / Et), -16.0) / 8.0) : -2.0f;
 }
 
-// features to global FP32 rows [e][72]; one wave per element
+// features to global FP32 rows [e][72]
 __global__ void __launch_bounds__(256) k_mlp_features(int Nelem, const double* __restrict__ q,
                                                       const double* __restrict__ B, const double* __restrict__ V,
                                                       float* __restrict__ feat) {
-    __shared__ double sV[64];
-    __shared__ double sU[4][64], sT[4][64];
-    const int w = threadIdx.x >> 6, n = threadIdx.x & 63;
-    if (threadIdx.x < 64) sV[threadIdx.x] = V[threadIdx.x];
+    __shared__ FeatLds L;
+    if (threadIdx.x < 64) L.V[threadIdx.x] = V[threadIdx.x];
     __syncthreads();
-    for (int e = blockIdx.x * 4 + w; e < Nelem; e += gridDim.x * 4) {
-        float xn, modal;
-        element_features(q, B, sV, e, sU[w], sT[w], xn, modal);
+    const int el = threadIdx.x >> 3, r = threadIdx.x & 7;
+    const int e = blockIdx.x * FEPB + el;
+    const bool act = e < Nelem;
+    float xn[8], modal;
+    row_features(q, B, L, e, act, xn, modal);
+    if (act) {
         float* row = feat + (int64_t)e * NIN;
-        row[n] = xn;
-        if (n < 8) row[64 + n] = modal;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) row[r * 8 + i] = xn[i];
+        row[64 + r] = modal;
     }
 }
 
@@ -198,45 +227,43 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, const float* __restric
 }
 
 // Features + inference fused over the elements of a state; p per element and, when estat is
-// given, the class: 0 smooth (p <= lo), 1 troubled (p >= hi), 2 unsure.
+// given, the class: 0 smooth (p <= lo), 1 troubled (p >= hi), 2 unsure.  A block takes 32 elements:
+// features by row threads into two 16-row BF16 tiles, then waves 0 and 1 run one tile each.
 __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* __restrict__ q,
                                                        const double* __restrict__ B, const double* __restrict__ V,
                                                        MlpDev W, float lo, float hi, float* __restrict__ p,
                                                        signed char* __restrict__ estat) {
     __shared__ __attribute__((aligned(16))) uint16_t sW1[NH * KP], sW2[NH * NH];
     __shared__ float sw3[NH], sb[2 * NH + 1];
-    __shared__ __attribute__((aligned(16))) uint16_t sX[4][16 * KP], sH[4][16 * NH];
-    __shared__ float sZ[4][16];
-    __shared__ double sV[64];
-    __shared__ double sU[4][64], sT[4][64];
+    __shared__ __attribute__((aligned(16))) uint16_t sX[2][16 * KP], sH[2][16 * NH];
+    __shared__ float sZ[2][16];
+    __shared__ FeatLds L;
     load_weights(W, sW1, sW2, sw3, sb);
-    if (threadIdx.x < 64) sV[threadIdx.x] = V[threadIdx.x];
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    for (int t = l; t < 16 * KP; t += 64) sX[w][t] = 0;
+    if (threadIdx.x < 64) L.V[threadIdx.x] = V[threadIdx.x];
+    for (int t = threadIdx.x; t < 2 * 16 * KP; t += 256) (&sX[0][0])[t] = 0;
     __syncthreads();
-    for (int e0 = (blockIdx.x * 4 + w) * 16; e0 < Nelem; e0 += gridDim.x * 64) {
-        for (int r = 0; r < 16; ++r) {
-            const int e = e0 + r;
-            if (e < Nelem) {
-                float xn, modal;
-                element_features(q, B, sV, e, sU[w], sT[w], xn, modal);
-                sX[w][r * KP + l] = f2bf(xn);
-                if (l < 8) sX[w][r * KP + 64 + l] = f2bf(modal);
-            } else {
-                sX[w][r * KP + l] = 0;
-                if (l < 8) sX[w][r * KP + 64 + l] = 0;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    const int el = threadIdx.x >> 3, r = threadIdx.x & 7;
+    const int e = blockIdx.x * FEPB + el;
+    const bool act = e < Nelem;
+    float xn[8], modal;
+    row_features(q, B, L, e, act, xn, modal);
+    if (act) {
+        uint16_t* row = sX[el >> 4] + (el & 15) * KP;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) row[r * 8 + i] = f2bf(xn[i]);
+        row[64 + r] = f2bf(modal);
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (w < 2) {
         mlp_tile(sX[w], sH[w], sW1, sW2, sw3, sb, sb + NH, sb[2 * NH], sZ[w]);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        if (l < 16 && e0 + l < Nelem) {
+        const int ee = blockIdx.x * FEPB + w * 16 + l;
+        if (l < 16 && ee < Nelem) {
             const float pv = sigmoidf_(sZ[w][l]);
-            if (p) p[e0 + l] = pv;
-            if (estat) estat[e0 + l] = (pv <= lo) ? 0 : (pv >= hi ? 1 : 2);
+            if (p) p[ee] = pv;
+            if (estat) estat[ee] = (pv <= lo) ? 0 : (pv >= hi ? 1 : 2);
         }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -407,8 +434,7 @@ namespace esdg {
 int mlp_classify(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B, const double* VdmInv,
                  float lo, float hi, float* p, signed char* estat) {
     KernelScope ks(ctx, "mlp_indicator");
-    const int blocks = (int)std::min<int64_t>((Nelem + 63) / 64, 4096);
-    k_mlp_indicator<<<blocks, 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, m->dev(), lo, hi, p, estat);
+    k_mlp_indicator<<<blocks_for(Nelem, FEPB), 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, m->dev(), lo, hi, p, estat);
     return launch_check(ctx, "mlp_indicator");
 }
 }  // namespace esdg
@@ -491,8 +517,7 @@ int esdg_mlp_features(esdg_ctx* ctx, int Nelem, const double* q, const double* B
     ESDG_REQUIRE(ctx, ctx->p.ngl == 8, "esdg_mlp_features: the indicator is defined for N = 7 (ngl = 8)");
     if (Nelem == 0) return ESDG_OK;
     KernelScope ks(ctx, "mlp_features");
-    const int blocks = (int)std::min<int64_t>((Nelem + 3) / 4, 65536);
-    k_mlp_features<<<blocks, 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, feat);
+    k_mlp_features<<<blocks_for(Nelem, FEPB), 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, feat);
     return launch_check(ctx, "mlp_features");
 }
 
