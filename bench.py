@@ -51,8 +51,11 @@ def volume_flops(nelem, ngl):
 # Yxi/Yeta/Xxi/Xeta 4, 1/J, Bx, By, gRK 3 read and gRK 3, q 3 written (19 doubles); per element
 # side node F* 3 [+ viscous face flux 2] read.  (The viscous volume term, read only where the
 # viscosity is non-zero, is not counted.)
-def fused_volume_bytes(nelem, ngl, av):
-    return 8 * nelem * (19 * ngl * ngl + (12 + (8 if av else 0)) * ngl)
+def fused_volume_bytes(nelem, ngl, av, traces=True):
+    """Per node: q 3, Yxi/Yeta/Xxi/Xeta 4, 1/J, Bx, By, gRK 3 read, gRK 3 and q 3 written; per
+    element side node: F* 3 (and the viscous face flux 2 with AV) read, and without the limiter the
+    side traces of the new state (3) written."""
+    return 8 * nelem * (19 * ngl * ngl + (12 + (8 if av else 0) + (12 if traces else 0)) * ngl)
 
 
 KERNELS = ["volume_kernel", "collect_edge_data", "calc_num_fluxes", "surface_kernel", "calc_rk",
@@ -187,28 +190,35 @@ def c4_bench(dev, args):
     from esdgsem_mpiocca_amd import Case
     case = Case.flood(N=7, nel=args.c4_nel, seed=0)
     # CFL 0.1: the dt formula's CFL/ngl stays below the smallest GLL weight fraction 1/(N(N+1)),
-    # the step size under which the limiter keeps the element means positive
-    rp = indicator.av_run_params(positivity=1, pos_pres_tol=1e-2, CFL=0.1)
+    # the step size under which the limiter keeps the element means positive.  Dry tolerance 0.1:
+    # without the partial-dry treatment, thinner layers carry spurious velocities at the fronts
+    # (tolerances 1e-2 and 5e-2 reach NaN or |u| of hundreds of m/s within 1000 steps at 2048^2).
+    rp = indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1)
     print(f"# c4: {case.Nelem} elements, case built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     out = {}
     qs = {}
+    dofs = 3 * case.ngl ** 2 * case.Nelem
+    t_end = None
+    # plain: --c4-steps steps; learned and audit: to the plain run's end time, so that the states
+    # are compared at the same simulated time (dt depends on the viscosity through the DFL limit)
     for mode in ("plain", "learned", "audit"):
-        ctx = dev.Context(dev.make_params(case, rp), device=0)
+        rpm = rp if mode == "plain" else indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1, T=t_end)
+        ctx = dev.Context(dev.make_params(case, rpm), device=0)
         try:
-            solver = dev.Solver(ctx, case, rp, fused=1)
+            solver = dev.Solver(ctx, case, rpm, fused=1)
             net = None
             if mode != "plain":
                 net = dev.MLP(ctx, params=params)
                 solver.set_indicator(net, args.ind_lo, args.ind_hi)
                 solver.indicator_audit(mode == "audit")
-            steps = args.c4_steps if mode != "audit" else (args.c4_audit_steps or args.c4_steps)
-            done, wall = 0, 0.0
-            chunk = max(1, min(steps, 50))
+            steps = args.c4_steps if mode == "plain" else 2 * args.c4_steps
+            done, wall, t_sim = 0, 0.0, 0.0
+            chunk = max(1, min(args.c4_steps, 50))
             per_kernel = {}
             while done < steps:
                 k = min(chunk, steps - done)
-                last = done + k >= steps and mode != "audit"
-                if last:   # per-kernel times from the last chunk
+                prof = mode != "audit" and not per_kernel and done + k >= min(args.c4_steps, steps) - chunk
+                if prof:   # per-kernel times from one chunk near the end
                     ctx.profile(True)
                     ctx.profile_reset()
                 t1 = time.perf_counter()
@@ -216,29 +226,32 @@ def c4_bench(dev, args):
                 ctx.finish()
                 wall += time.perf_counter() - t1
                 done += n
-                if last:
+                if prof:
                     for kn in KERNELS:
                         launches, ms = ctx.profile_query(kn)
                         if launches:
                             per_kernel[kn] = {"avg_us": round(1e3 * ms / launches, 2),
-                                              "ms_per_step": round(ms / n, 4)}
+                                              "ms_per_step": round(ms / max(n, 1), 4)}
                     ctx.profile(False)
-                print(f"# c4 {mode}: {done}/{steps} steps, {1e3 * wall / done:.1f} ms/step", file=sys.stderr,
-                      flush=True)
+                print(f"# c4 {mode}: {done} steps, t = {t_sim:.6g}, {1e3 * wall / max(done, 1):.1f} ms/step",
+                      file=sys.stderr, flush=True)
                 if n != k:
+                    if mode != "plain" and t_sim >= t_end:
+                        break
                     raise SystemExit(f"c4 {mode}: solver stopped after {done} steps")
+            if mode == "plain":
+                t_end = t_sim
             counts = solver.indicator_counts() if net else None
             if mode != "audit":
                 qs[mode] = solver.get_q()
-                dofs = 3 * case.ngl ** 2 * case.Nelem
-                out[mode] = {"dof_stage_updates_per_s": dofs * 5 * steps / wall, "ms_per_step": 1e3 * wall / steps,
-                             "steps": steps, "sim_time": t_sim, "kernels_last_chunk": per_kernel}
+                out[mode] = {"dof_stage_updates_per_s": dofs * 5 * done / wall, "ms_per_step": 1e3 * wall / done,
+                             "steps": done, "sim_time": t_sim, "kernels": per_kernel}
                 if counts:
                     es = max(counts["element_stages"], 1)
                     out[mode]["viscous_element_stage_fraction"] = counts["viscous"] / es
                     out[mode]["sensor_evaluated_fraction"] = counts["not_smooth"] / es
             else:
-                out["audit"] = {"steps": steps, **counts,
+                out["audit"] = {"steps": done, **counts,
                                 "false_negative_rate": counts["false_negatives"] / max(counts["sensor_positives"], 1)}
             solver.close()
             if net:
@@ -281,7 +294,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
     ap.add_argument("--c4-nel", type=int, default=2048)
     ap.add_argument("--c4-steps", type=int, default=1000)
-    ap.add_argument("--c4-audit-steps", type=int, default=0, help="0: as many as --c4-steps")
+
     ap.add_argument("--ind-lo", type=float, default=0.05)
     ap.add_argument("--ind-hi", type=float, default=0.5)
     ap.add_argument("--workload", choices=["c1", "ensemble", "c4"], default="c1",
@@ -354,7 +367,7 @@ def main():
     roofline = None
     if vol_name in per_kernel:
         avg_s = per_kernel[vol_name]["avg_us"] * 1e-6
-        alg = (fused_volume_bytes(case.Nelem, case.ngl, rp.artificial_viscosity) if args.fused
+        alg = (fused_volume_bytes(case.Nelem, case.ngl, rp.artificial_viscosity, not rp.positivity) if args.fused
                else volume_bytes(case.Nelem, case.ngl))
         gbs = alg / avg_s / 1e9
         tfl = volume_flops(case.Nelem, case.ngl) / avg_s / 1e12

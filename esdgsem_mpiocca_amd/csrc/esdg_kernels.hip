@@ -416,76 +416,108 @@ __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts
 // ---------------------------------------------------------------------------------------------
 // PosPres: okl/Positivity/PosPres.okl:17-223
 // ---------------------------------------------------------------------------------------------
-// With tr (the fused schedule), the side traces of the limited state are written as well.
+// One thread per element row (eta = r), 256 / NGL elements per block.  The element means run on
+// three threads (h, hu, hv), each over the node products in the okl's serial order (xi outer, eta
+// inner); the products h w, w = GLw[xi] GLw[eta] / Jinv, are formed by the row threads.  With tr
+// (the fused schedule), the side traces of the limited state are written as well, from LDS and
+// lane-consecutively.
 template <int NGL>
 __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double zero_tol,
                                                   const double* __restrict__ EleSizes, const double* __restrict__ GLw,
                                                   const double* __restrict__ Jac, double* __restrict__ q,
                                                   double* __restrict__ tr) {
-    using EB = ElemBlock<NGL>;
-    __shared__ double sH[EB::THREADS], sHU[EB::THREADS], sHV[EB::THREADS], sJ[EB::THREADS];
-    __shared__ double sAvg[EB::EPB][5];   // hAvg, huAvg, hvAvg, theta, hMin
-    const int el = threadIdx.x / EB::NODES, n = threadIdx.x % EB::NODES;
-    const int e = blockIdx.x * EB::EPB + el;
-    const int64_t id = (int64_t)e * EB::NODES * ESDG_NEQ + n;
-    if (e < Nelem) {
-        sH[threadIdx.x] = q[id];
-        sHU[threadIdx.x] = q[id + EB::NODES];
-        sHV[threadIdx.x] = q[id + 2 * EB::NODES];
-        sJ[threadIdx.x] = Jac[(int64_t)e * EB::NODES + n];
+    using RB = RowBlock<NGL>;
+    constexpr int NN = NGL * NGL;
+    __shared__ double sP[3][RB::EPB * NN];
+    __shared__ double sMin[RB::EPB][NGL];
+    __shared__ double sSum[RB::EPB][3];
+    const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
+    const int e = blockIdx.x * RB::EPB + el;
+    const bool act = e < Nelem;
+    const int64_t qb = (int64_t)e * NN * ESDG_NEQ + r * NGL;
+    double h[NGL], hu[NGL], hv[NGL];
+    double* P0 = sP[0] + el * NN;
+    double* P1 = sP[1] + el * NN;
+    double* P2 = sP[2] + el * NN;
+    if (act) {
+        const int64_t xb = (int64_t)e * NN + r * NGL;
+        const double gr = GLw[r];
+        double mn = 0.0;
+#pragma unroll
+        for (int i = 0; i < NGL; ++i) {
+            h[i] = q[qb + i];
+            hu[i] = q[qb + NN + i];
+            hv[i] = q[qb + 2 * NN + i];
+            const double w = GLw[i] * gr / Jac[xb + i];
+            P0[r * NGL + i] = h[i] * w;
+            P1[r * NGL + i] = hu[i] * w;
+            P2[r * NGL + i] = hv[i] * w;
+            mn = fmin(mn, h[i]);
+        }
+        sMin[el][r] = mn;
     }
     __syncthreads();
-    if (threadIdx.x < EB::EPB) {
-        const int ee = blockIdx.x * EB::EPB + threadIdx.x;
-        if (ee < Nelem) {
-            const double* H = sH + threadIdx.x * EB::NODES;
-            const double* HU = sHU + threadIdx.x * EB::NODES;
-            const double* HV = sHV + threadIdx.x * EB::NODES;
-            const double* JJ = sJ + threadIdx.x * EB::NODES;
-            double hmin = 0.0, ha = 0.0, hua = 0.0, hva = 0.0;
-            for (int l = 0; l < NGL; ++l)
-                for (int k = 0; k < NGL; ++k) {
-                    const int nn = k * NGL + l;
-                    const double w = GLw[l] * GLw[k] / JJ[nn];
-                    ha += H[nn] * w;
-                    hua += HU[nn] * w;
-                    hva += HV[nn] * w;
-                    hmin = fmin(hmin, H[nn]);
+    if (act && r < 3) {
+        const double* P = sP[r] + el * NN;
+        double a = 0.0;
+        for (int l = 0; l < NGL; ++l)
+            for (int k = 0; k < NGL; ++k) a += P[k * NGL + l];
+        sSum[el][r] = a;
+    }
+    __syncthreads();
+    if (act) {
+        double hmin = 0.0;
+#pragma unroll
+        for (int k = 0; k < NGL; ++k) hmin = fmin(hmin, sMin[el][k]);
+        const double size = EleSizes[e];
+        double ha = sSum[el][0] / size;
+        if (fabs(ha) < zero_tol) ha = 0.0;
+        const double hua = sSum[el][1] / size, hva = sSum[el][2] / size;
+        const double th = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
+#pragma unroll
+        for (int i = 0; i < NGL; ++i) {
+            if (hmin < 0.0) {
+                double hn = 0.0, hun = 0.0, hvn = 0.0;
+                if (ha > 0.0) {
+                    hn = fabs(th * (h[i] - ha) + ha);
+                    if (hn > tol) {
+                        hun = th * (hu[i] - hua) + hua;
+                        hvn = th * (hv[i] - hva) + hva;
+                    }
                 }
-            const double size = EleSizes[ee];
-            ha = ha / size;
-            if (fabs(ha) < zero_tol) ha = 0.0;
-            sAvg[threadIdx.x][0] = ha;
-            sAvg[threadIdx.x][1] = hua / size;
-            sAvg[threadIdx.x][2] = hva / size;
-            sAvg[threadIdx.x][3] = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
-            sAvg[threadIdx.x][4] = hmin;
+                if (ha < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
+                h[i] = hn; hu[i] = hun; hv[i] = hvn;
+                q[qb + i] = hn;
+                q[qb + NN + i] = hun;
+                q[qb + 2 * NN + i] = hvn;
+            } else if (h[i] < tol) {
+                hu[i] = 0.0; hv[i] = 0.0;
+                q[qb + NN + i] = 0.0;
+                q[qb + 2 * NN + i] = 0.0;
+            }
         }
     }
-    __syncthreads();
-    if (e < Nelem) {
-        const double ha = sAvg[el][0], hua = sAvg[el][1], hva = sAvg[el][2], th = sAvg[el][3], hmin = sAvg[el][4];
-        const double h = sH[threadIdx.x];
-        if (hmin < 0.0) {
-            double hn = 0.0, hun = 0.0, hvn = 0.0;
-            if (ha > 0.0) {
-                hn = fabs(th * (h - ha) + ha);
-                if (hn > tol) {
-                    hun = th * (sHU[threadIdx.x] - hua) + hua;
-                    hvn = th * (sHV[threadIdx.x] - hva) + hva;
-                }
+    if (tr) {
+        __syncthreads();   // the products are no longer read
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < NGL; ++i) {
+                P0[r * NGL + i] = h[i];
+                P1[r * NGL + i] = hu[i];
+                P2[r * NGL + i] = hv[i];
             }
-            if (ha < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
-            q[id] = hn;
-            q[id + EB::NODES] = hun;
-            q[id + 2 * EB::NODES] = hvn;
-            if (tr) tr_put(tr, NGL, e, n % NGL, n / NGL, hn, hun, hvn);
-        } else if (h < tol) {
-            q[id + EB::NODES] = 0.0;
-            q[id + 2 * EB::NODES] = 0.0;
-            if (tr) tr_put(tr, NGL, e, n % NGL, n / NGL, h, 0.0, 0.0);
-        } else if (tr) {
-            tr_put(tr, NGL, e, n % NGL, n / NGL, h, sHU[threadIdx.x], sHV[threadIdx.x]);
+        }
+        __syncthreads();
+        constexpr int TPE = 12 * NGL;
+        const int nact = min(RB::EPB, Nelem - (int)blockIdx.x * RB::EPB);
+        double* T = tr + (int64_t)blockIdx.x * RB::EPB * TPE;
+        for (int t = threadIdx.x; t < nact * TPE; t += RB::THREADS) {
+            const int el2 = t / TPE, rem = t - el2 * TPE;
+            const int m = rem / NGL, k = rem - m * NGL;
+            const int side = m / 3, v = m - 3 * side;
+            const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : k;
+            const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : k;
+            T[t] = sP[v][el2 * NN + row * NGL + col];
         }
     }
 }
@@ -729,8 +761,8 @@ void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, c
 template <int NGL>
 void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q,
                     double* tr) {
-    using EB = ElemBlock<NGL>;
-    k_pos_pres<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.PosPresTOL, ctx->p.ZeroTOL,
+    using RB = RowBlock<NGL>;
+    k_pos_pres<NGL><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.PosPresTOL, ctx->p.ZeroTOL,
                                                                                   sizes, GLw, Jac, q, tr);
 }
 template <int NGL>

@@ -6,7 +6,7 @@ the modal sensor and artificial viscosity (the sigma thresholds of the benchmark
 - dam breaks: step height ratio U(1.2, 5), step angle U(0, pi), a sharp step or one spread over
   0.5 .. 2 elements, noise U(0, 2e-3), b = 0.1 sin x cos y, eps0 grown with the step height;
 - floods: seeded 16-mode Fourier bathymetry, a reservoir level U(0.5, 1) behind x = U(2, 5) over
-  a dry or shallow plain (level U(-0.3, 0.1)), with the positivity limiter (tolerance 1e-2,
+  a dry or shallow plain (level U(-0.3, 0.1)), with the positivity limiter (dry tolerance 0.1,
   CFL 0.1, the flood benchmark's settings).
 Every state is labelled by the modal sensor (troubled = positive viscosity, i.e. sigma above
 sigma_min).  Per snapshot at most a fixed number of troubled elements is kept and smooth ones are
@@ -100,7 +100,7 @@ def _scenario(rng):
         left, x_dam, right = rng.uniform(0.5, 1.0), rng.uniform(2.0, 5.0), rng.uniform(-0.3, 0.1)
         case = Case.flood(N=7, nel=nel, seed=int(rng.integers(1 << 30)), level_left=left, x_dam=x_dam,
                           level_right=right)
-        rp = av_run_params(positivity=1, pos_pres_tol=1e-2, CFL=0.1)
+        rp = av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1)
         return case, rp, None, f"flood nel {nel} levels {left:.2f}|{right:.2f} at x {x_dam:.2f}"
     ratio, angle = rng.uniform(1.2, 5.0), rng.uniform(0.0, np.pi)
     width = 0.0 if rng.uniform() < 0.5 else rng.uniform(0.5, 2.0) * 10.0 / nel
