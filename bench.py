@@ -200,14 +200,18 @@ def c4_bench(dev, args):
     dofs = 3 * case.ngl ** 2 * case.Nelem
     t_end = None
     # plain: --c4-steps steps; learned and audit: to the plain run's end time, so that the states
-    # are compared at the same simulated time (dt depends on the viscosity through the DFL limit)
-    for mode in ("plain", "learned", "audit"):
-        rpm = rp if mode == "plain" else indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1, T=t_end)
+    # are compared at the same simulated time (dt depends on the viscosity through the DFL limit).
+    # control: the plain method with eps0 changed by 1e-12 (relative), to the same time: how far
+    # roundoff-sized differences travel in this flow, the scale to read the learned change against.
+    for mode in ("plain", "learned", "audit", "control"):
+        eps0 = 0.005 * (1.0 + 1e-12) if mode == "control" else 0.005
+        rpm = rp if mode == "plain" else indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1, T=t_end,
+                                                                 eps0=eps0)
         ctx = dev.Context(dev.make_params(case, rpm), device=0)
         try:
             solver = dev.Solver(ctx, case, rpm, fused=1)
             net = None
-            if mode != "plain":
+            if mode in ("learned", "audit"):
                 net = dev.MLP(ctx, params=params)
                 solver.set_indicator(net, args.ind_lo, args.ind_hi)
                 solver.indicator_audit(mode == "audit")
@@ -217,7 +221,7 @@ def c4_bench(dev, args):
             per_kernel = {}
             while done < steps:
                 k = min(chunk, steps - done)
-                prof = mode != "audit" and not per_kernel and done + k >= min(args.c4_steps, steps) - chunk
+                prof = mode in ("plain", "learned") and not per_kernel and done + k >= min(args.c4_steps, steps) - chunk
                 if prof:   # per-kernel times from one chunk near the end
                     ctx.profile(True)
                     ctx.profile_reset()
@@ -242,7 +246,9 @@ def c4_bench(dev, args):
             if mode == "plain":
                 t_end = t_sim
             counts = solver.indicator_counts() if net else None
-            if mode != "audit":
+            if mode == "control":
+                qs[mode] = solver.get_q()
+            elif mode != "audit":
                 qs[mode] = solver.get_q()
                 out[mode] = {"dof_stage_updates_per_s": dofs * 5 * done / wall, "ms_per_step": 1e3 * wall / done,
                              "steps": done, "sim_time": t_sim, "kernels": per_kernel}
@@ -260,8 +266,11 @@ def c4_bench(dev, args):
             ctx.close()
     A = qs["learned"].reshape(case.Nelem, 3, -1)
     B = qs["plain"].reshape(case.Nelem, 3, -1)
+    Cq = qs["control"].reshape(case.Nelem, 3, -1)
     out["l2_change_vs_plain"] = [float(np.linalg.norm(A[:, v] - B[:, v]) / max(np.linalg.norm(B[:, v]), 1e-300))
                                  for v in range(3)]
+    out["control_l2_change_eps0_1e-12"] = [float(np.linalg.norm(Cq[:, v] - B[:, v]) /
+                                                 max(np.linalg.norm(B[:, v]), 1e-300)) for v in range(3)]
     out["finite"] = bool(np.isfinite(A).all() and np.isfinite(B).all())
     out["min_h"] = float(min(A[:, 0].min(), B[:, 0].min()))
     out["speedup_learned_vs_plain"] = out["learned"]["dof_stage_updates_per_s"] / out["plain"]["dof_stage_updates_per_s"]
@@ -289,8 +298,8 @@ def main():
     ap.add_argument("--fused", type=int, default=1,
                     help="1: fused stage schedule (default); 0: the reference's kernel-by-kernel sequence")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-nel", type=int, default=256)
-    ap.add_argument("--cpu-sample-steps", type=int, default=20)
+    ap.add_argument("--cpu-sample-nel", type=int, default=512)
+    ap.add_argument("--cpu-sample-steps", type=int, default=25)
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
     ap.add_argument("--c4-nel", type=int, default=2048)
     ap.add_argument("--c4-steps", type=int, default=1000)
