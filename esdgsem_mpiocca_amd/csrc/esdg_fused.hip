@@ -92,144 +92,164 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
     }
 }
 
+// Element blocks of the AV pass per chunk: a workgroup reads the marks of a chunk at once and runs
+// only the marked element blocks, so that unmarked ones cost no launch.
+template <int NGL>
+struct AvChunk {
+    static constexpr int EPB = ElemBlock<NGL>::EPB;
+    static constexpr int CH = (256 / EPB) < 16 ? (256 / EPB) : 16;
+};
+
 // ---------------------------------------------------------------------------------------------
-// fused_av: ShockCapturing.okl:17-234 + calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148
-// + scaleGradient.okl:17-60 + VolumeKernelViscose.okl:17-150 for one block of elements, one thread
-// per node, all intermediate fields in LDS.  Writes the scaled gradients (read by the viscous face
-// kernel) as element-side traces, the element viscosity and the viscous volume term Qv(hu, hv).
+// fused_av: calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148 + scaleGradient.okl:17-60
+// + VolumeKernelViscose.okl:17-150 for one block of elements, one thread per node, all
+// intermediate fields in LDS.  Writes the scaled gradients (read by the viscous face kernel) as
+// element-side traces and the viscous volume term Qv(hu, hv).
 // ---------------------------------------------------------------------------------------------
 //
 // The element viscosity comes in (evaluated on the stage's state before the stage).  A block
 // whose elements and their face neighbours all have zero viscosity has nothing to contribute: its
 // gradient traces feed only faces of zero viscosity (whose fluxes are zero without reading them)
-// and its viscous volume term is skipped where the viscosity is zero.  Such a block leaves at once;
-// the face pass marks the others (active[e] == seq).
+// and its viscous volume term is skipped where the viscosity is zero.  The face pass marks the
+// others (active[e] == seq); a workgroup takes a chunk of element blocks and runs the marked ones.
 template <int NGL>
-__global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const int* __restrict__ ED,
-                                                  const double* __restrict__ q, const double* __restrict__ Dh,
-                                                  const double* __restrict__ Ds, const double* __restrict__ Jac,
-                                                  const double* __restrict__ Yxi, const double* __restrict__ Yeta,
-                                                  const double* __restrict__ Xxi, const double* __restrict__ Xeta,
-                                                  const int* __restrict__ E2E, const int* __restrict__ EMS,
-                                                  const int* __restrict__ EOR, const double* __restrict__ SGX,
-                                                  const double* __restrict__ SGY, double* __restrict__ gtr,
-                                                  const double* __restrict__ visc, const int* __restrict__ active,
-                                                  int seq, double* __restrict__ Qv) {
+__global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const double* __restrict__ q,
+                                                  const double* __restrict__ Dh, const double* __restrict__ Ds,
+                                                  const double* __restrict__ Jac, const double* __restrict__ Yxi,
+                                                  const double* __restrict__ Yeta, const double* __restrict__ Xxi,
+                                                  const double* __restrict__ Xeta, const int* __restrict__ E2E,
+                                                  const int* __restrict__ EMS, const int* __restrict__ EOR,
+                                                  const double* __restrict__ SGX, const double* __restrict__ SGY,
+                                                  double* __restrict__ gtr, const double* __restrict__ visc,
+                                                  const int* __restrict__ active, int seq, double* __restrict__ Qv) {
     using EB = ElemBlock<NGL>;
+    using AC = AvChunk<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sDh[NN], sDs[NN];
     __shared__ double sA[EB::THREADS], sB[EB::THREADS], sC[EB::THREADS], sE[EB::THREADS];
     __shared__ double sEps[EB::EPB];
-    __shared__ int sActive;
-    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
-    const int e = blockIdx.x * EB::EPB + el;
-    const bool act = e < Nelem;
-    if (threadIdx.x == 0) sActive = 0;
-    __syncthreads();
-    if (act && n == 0) {
-        sEps[el] = visc[e];
-        if (active[e] == seq) sActive = 1;
-    }
-    __syncthreads();
-    if (!sActive) return;
+    __shared__ int sFlag[AC::CH], sList[AC::CH], sCount;
     if (threadIdx.x < NN) {
         sDh[threadIdx.x] = Dh[threadIdx.x];
         sDs[threadIdx.x] = Ds[threadIdx.x];
     }
-    const int a = n % NGL, c = n / NGL;           // node n = (xi = a, eta = c)
-    const int64_t xid = (int64_t)e * NN + n;
-    double h = 0.0, hu = 0.0, hv = 0.0;
-    if (act) {
-        const int64_t id = (int64_t)e * NN * ESDG_NEQ + n;
-        h = q[id]; hu = q[id + NN]; hv = q[id + 2 * NN];
-        const double Qinv = (h > tol) ? 1.0 / h : 0.0;
-        sB[threadIdx.x] = hu * Qinv;                // u at node n
-        sC[threadIdx.x] = hv * Qinv;                // v at node n
-    }
-    __syncthreads();
-    // ---- BR1 gradient (weak form volume part) ----
-    const double* u = sB + el * NN;
-    const double* v = sC + el * NN;
-    double gx0 = 0.0, gx1 = 0.0, gy0 = 0.0, gy1 = 0.0;
-    double ye = 0.0, yx = 0.0, xe = 0.0, xx = 0.0, rJ = 0.0;
-    if (act) {
-        double u_xi = 0.0, u_eta = 0.0, v_xi = 0.0, v_eta = 0.0;
-#pragma unroll
-        for (int l = 0; l < NGL; ++l) {
-            u_xi += sDh[a * NGL + l] * u[c * NGL + l];
-            u_eta += sDh[c * NGL + l] * u[l * NGL + a];
-            v_xi += sDh[a * NGL + l] * v[c * NGL + l];
-            v_eta += sDh[c * NGL + l] * v[l * NGL + a];
+    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
+    const int nblk = (Nelem + EB::EPB - 1) / EB::EPB;
+    for (int chunk = blockIdx.x; chunk * AC::CH < nblk; chunk += gridDim.x) {
+        // the chunk's element blocks holding a marked element, in a list
+        if (threadIdx.x < AC::CH) sFlag[threadIdx.x] = 0;
+        if (threadIdx.x == 0) sCount = 0;
+        __syncthreads();
+        if (threadIdx.x < AC::CH * EB::EPB) {
+            const int ee = chunk * AC::CH * EB::EPB + threadIdx.x;
+            if (ee < Nelem && active[ee] == seq) sFlag[threadIdx.x / EB::EPB] = 1;
         }
-        ye = Yeta[xid]; yx = Yxi[xid]; xe = Xeta[xid]; xx = Xxi[xid]; rJ = Jac[xid];
-        const double u_x = ye * u_xi - yx * u_eta;
-        const double v_x = ye * v_xi - yx * v_eta;
-        const double u_y = -xe * u_xi + xx * u_eta;
-        const double v_y = -xe * v_xi + xx * v_eta;
-        gx0 = rJ * u_x; gx1 = rJ * v_x; gy0 = rJ * u_y; gy1 = rJ * v_y;
-        // surface lift of the BR1 face fluxes: sides 0/2 first, then 1/3 (SurfaceKernelGradient.okl)
-#pragma unroll
-        for (int ph = 0; ph < 2; ++ph) {
-            int side = -1, t = 0;
-            if (ph == 0) {
-                if (c == 0) { side = 0; t = a; }
-                else if (c == NGL - 1) { side = 2; t = a; }
-            } else {
-                if (a == NGL - 1) { side = 1; t = c; }
-                else if (a == 0) { side = 3; t = c; }
+        __syncthreads();
+        if (threadIdx.x < AC::CH && sFlag[threadIdx.x]) sList[atomicAdd(&sCount, 1)] = threadIdx.x;
+        __syncthreads();
+        const int cnt = sCount;
+        for (int li = 0; li < cnt; ++li) {
+            const int e = (chunk * AC::CH + sList[li]) * EB::EPB + el;
+            const bool act = e < Nelem;
+            if (act && n == 0) sEps[el] = visc[e];
+            const int a = n % NGL, c = n / NGL;           // node n = (xi = a, eta = c)
+            const int64_t xid = (int64_t)e * NN + n;
+            double h = 0.0, hu = 0.0, hv = 0.0;
+            if (act) {
+                const int64_t id = (int64_t)e * NN * ESDG_NEQ + n;
+                h = q[id]; hu = q[id + NN]; hv = q[id + 2 * NN];
+                const double Qinv = (h > tol) ? 1.0 / h : 0.0;
+                sB[threadIdx.x] = hu * Qinv;                // u at node n
+                sC[threadIdx.x] = hv * Qinv;                // v at node n
             }
-            if (side < 0) continue;
-            const int ifa = E2E[e * 4 + side] - 1;
-            const int rev = EOR[e * 4 + side];
-            const int ms = EMS[e * 4 + side];
-            const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
-            const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
-            if (ph == 0) gx0 += ms * SGX[fl] * rJ;
-            else gx0 += ms * rJ * SGX[fl];
-            gx1 += ms * rJ * SGX[fl + NGL];
-            gy0 += ms * rJ * SGY[fl];
-            gy1 += ms * rJ * SGY[fl + NGL];
-        }
-        // scaleGradient: multiply by h
-        gx0 = h * gx0; gx1 = h * gx1; gy0 = h * gy0; gy1 = h * gy1;
-        // side traces of the scaled gradients, [e][side][gX0, gX1, gY0, gY1][k] (k in the side's order)
-        double* T = gtr + (int64_t)e * 16 * NGL;
-        auto put = [&](int side, int k) {
-            double* t = T + side * 4 * NGL + k;
-            t[0] = gx0; t[NGL] = gx1; t[2 * NGL] = gy0; t[3 * NGL] = gy1;
-        };
-        if (c == 0) put(0, a);
-        if (c == NGL - 1) put(2, a);
-        if (a == NGL - 1) put(1, c);
-        if (a == 0) put(3, c);
-    }
-    __syncthreads();   // sEps ready; u, v no longer needed
-    // ---- viscous volume term: contravariant fluxes, then Dstrong ----
-    if (act) {
-        const double eps = sEps[el];
-        const double f2 = eps * gx0, f3 = eps * gx1, g2 = eps * gy0, g3 = eps * gy1;
-        sA[threadIdx.x] = ye * f2 - xe * g2;      // F2
-        sB[threadIdx.x] = -yx * f2 + xx * g2;     // G2
-        sC[threadIdx.x] = ye * f3 - xe * g3;      // F3
-        sE[threadIdx.x] = -yx * f3 + xx * g3;     // G3
-    }
-    __syncthreads();
-    if (act) {
-        const double* F2 = sA + el * NN;
-        const double* G2 = sB + el * NN;
-        const double* F3 = sC + el * NN;
-        const double* G3 = sE + el * NN;
-        double d2 = 0.0, d3 = 0.0;
+            __syncthreads();
+            // ---- BR1 gradient (weak form volume part) ----
+            const double* u = sB + el * NN;
+            const double* v = sC + el * NN;
+            double gx0 = 0.0, gx1 = 0.0, gy0 = 0.0, gy1 = 0.0;
+            double ye = 0.0, yx = 0.0, xe = 0.0, xx = 0.0, rJ = 0.0;
+            if (act) {
+                double u_xi = 0.0, u_eta = 0.0, v_xi = 0.0, v_eta = 0.0;
 #pragma unroll
-        for (int l = 0; l < NGL; ++l) {
-            d2 += sDs[a * NGL + l] * F2[c * NGL + l];
-            d2 += sDs[c * NGL + l] * G2[l * NGL + a];
-            d3 += sDs[a * NGL + l] * F3[c * NGL + l];
-            d3 += sDs[c * NGL + l] * G3[l * NGL + a];
+                for (int l = 0; l < NGL; ++l) {
+                    u_xi += sDh[a * NGL + l] * u[c * NGL + l];
+                    u_eta += sDh[c * NGL + l] * u[l * NGL + a];
+                    v_xi += sDh[a * NGL + l] * v[c * NGL + l];
+                    v_eta += sDh[c * NGL + l] * v[l * NGL + a];
+                }
+                ye = Yeta[xid]; yx = Yxi[xid]; xe = Xeta[xid]; xx = Xxi[xid]; rJ = Jac[xid];
+                const double u_x = ye * u_xi - yx * u_eta;
+                const double v_x = ye * v_xi - yx * v_eta;
+                const double u_y = -xe * u_xi + xx * u_eta;
+                const double v_y = -xe * v_xi + xx * v_eta;
+                gx0 = rJ * u_x; gx1 = rJ * v_x; gy0 = rJ * u_y; gy1 = rJ * v_y;
+                // surface lift of the BR1 face fluxes: sides 0/2 first, then 1/3 (SurfaceKernelGradient.okl)
+#pragma unroll
+                for (int ph = 0; ph < 2; ++ph) {
+                    int side = -1, t = 0;
+                    if (ph == 0) {
+                        if (c == 0) { side = 0; t = a; }
+                        else if (c == NGL - 1) { side = 2; t = a; }
+                    } else {
+                        if (a == NGL - 1) { side = 1; t = c; }
+                        else if (a == 0) { side = 3; t = c; }
+                    }
+                    if (side < 0) continue;
+                    const int ifa = E2E[e * 4 + side] - 1;
+                    const int rev = EOR[e * 4 + side];
+                    const int ms = EMS[e * 4 + side];
+                    const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
+                    const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
+                    if (ph == 0) gx0 += ms * SGX[fl] * rJ;
+                    else gx0 += ms * rJ * SGX[fl];
+                    gx1 += ms * rJ * SGX[fl + NGL];
+                    gy0 += ms * rJ * SGY[fl];
+                    gy1 += ms * rJ * SGY[fl + NGL];
+                }
+                // scaleGradient: multiply by h
+                gx0 = h * gx0; gx1 = h * gx1; gy0 = h * gy0; gy1 = h * gy1;
+                // side traces of the scaled gradients, [e][side][gX0, gX1, gY0, gY1][k] (k in the side's order)
+                double* T = gtr + (int64_t)e * 16 * NGL;
+                auto put = [&](int side, int k) {
+                    double* t = T + side * 4 * NGL + k;
+                    t[0] = gx0; t[NGL] = gx1; t[2 * NGL] = gy0; t[3 * NGL] = gy1;
+                };
+                if (c == 0) put(0, a);
+                if (c == NGL - 1) put(2, a);
+                if (a == NGL - 1) put(1, c);
+                if (a == 0) put(3, c);
+            }
+            __syncthreads();   // sEps ready; u, v no longer needed
+            // ---- viscous volume term: contravariant fluxes, then Dstrong ----
+            if (act) {
+                const double eps = sEps[el];
+                const double f2 = eps * gx0, f3 = eps * gx1, g2 = eps * gy0, g3 = eps * gy1;
+                sA[threadIdx.x] = ye * f2 - xe * g2;      // F2
+                sB[threadIdx.x] = -yx * f2 + xx * g2;     // G2
+                sC[threadIdx.x] = ye * f3 - xe * g3;      // F3
+                sE[threadIdx.x] = -yx * f3 + xx * g3;     // G3
+            }
+            __syncthreads();
+            if (act) {
+                const double* F2 = sA + el * NN;
+                const double* G2 = sB + el * NN;
+                const double* F3 = sC + el * NN;
+                const double* G3 = sE + el * NN;
+                double d2 = 0.0, d3 = 0.0;
+#pragma unroll
+                for (int l = 0; l < NGL; ++l) {
+                    d2 += sDs[a * NGL + l] * F2[c * NGL + l];
+                    d2 += sDs[c * NGL + l] * G2[l * NGL + a];
+                    d3 += sDs[a * NGL + l] * F3[c * NGL + l];
+                    d3 += sDs[c * NGL + l] * G3[l * NGL + a];
+                }
+                double* o = Qv + (int64_t)e * NN * 2 + n;
+                o[0] = rJ * d2;
+                o[NN] = rJ * d3;
+            }
+
+        __syncthreads();   // LDS free for the next element block
         }
-        double* o = Qv + (int64_t)e * NN * 2 + n;
-        o[0] = rJ * d2;
-        o[NN] = rJ * d3;
     }
 }
 
@@ -278,8 +298,11 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     using EB = ElemBlock<NGL>;
     const double* SGX = F.fgrad;
     const double* SGY = F.fgrad + (size_t)A.Nfaces * NGL * 2;
-    k_fused_av<NGL><<<blocks_for(A.Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(
-        A.Nelem, ctx->p.PosPresTOL, A.EdgeData, A.q, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
+    using AC = AvChunk<NGL>;
+    const int64_t nblk = (A.Nelem + EB::EPB - 1) / EB::EPB;
+    const unsigned grid = (unsigned)std::min<int64_t>((nblk + AC::CH - 1) / AC::CH, 8192);
+    k_fused_av<NGL><<<grid, EB::THREADS, 0, ctx->stream>>>(
+        A.Nelem, ctx->p.PosPresTOL, A.q, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
         A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, F.active, F.seq, F.qvisc);
 }
 
