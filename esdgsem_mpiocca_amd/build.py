@@ -35,6 +35,7 @@ HIP_FLAGS = [
 HIP_SOURCES = [
     "esdg_runtime.hip",
     "esdg_kernels.hip",
+    "esdg_analysis.hip",
     "esdg_volume.hip",
     "esdg_fused.hip",
     "esdg_stage.hip",

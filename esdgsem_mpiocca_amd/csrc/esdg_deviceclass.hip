@@ -53,6 +53,13 @@ class deviceclass {
             A(&o_SurfacePartsVisc, fn * 2);
             A(&o_ViscParaL, Nfaces); A(&o_ViscParaR, Nfaces);
         }
+        // analysis buffers (src/deviceclass.cpp:324-342): arrival times start at 1000.0, maximum
+        // elevations at 0
+        if (rp.arrival_times) {
+            A(&o_ArrivalTimings, nodes);
+            if (rc == ESDG_OK) rc = esdg_fill(ctx_, o_ArrivalTimings, 1000.0, nodes);
+        }
+        if (rp.maximum_elevation) A(&o_MaximumElevation, nodes);
         if (rc == ESDG_OK && rp.fused == 1) rc = fused_.init(ctx_, Nelem, Nfaces, rp.artificial_viscosity != 0);
         if (rp.fused == 2) {
             // single-kernel stage: second state buffer (ping-pong), device clock, lambda_max cell
@@ -103,6 +110,8 @@ class deviceclass {
             return set_error(ctx_, ESDG_ERR_ARG, "the fused stage supports flux differencing with low-storage RK only");
         if (rp_.fused == 2 && rp_.artificial_viscosity)
             return set_error(ctx_, ESDG_ERR_ARG, "the single-kernel stage runs without artificial viscosity");
+        if (rp_.fused == 2 && (rp_.arrival_times || rp_.maximum_elevation))
+            return set_error(ctx_, ESDG_ERR_ARG, "the analysis buffers are updated by the fused 0 and 1 schedules");
         if (rp_.fused == 2 && (ngl < 3 || ngl > 8))
             return set_error(ctx_, ESDG_ERR_ARG, "the single-kernel stage supports 2 <= N <= 7");
         const int nf = ctx_->p.num_flux;
@@ -268,6 +277,10 @@ class deviceclass {
             t += dt;
             last_dt = dt;
             ++n;
+            // src/deviceclass.cpp:1132-1141
+            if (rp_.arrival_times) rc = esdg_arrival_times(ctx_, Nelem, t, o_q, o_B, o_ArrivalTimings);
+            if (!rc && rp_.maximum_elevation) rc = esdg_maximum_elevation(ctx_, Nelem, t, o_q, o_B, o_MaximumElevation);
+            if (rc) return rc;
         }
         if (steps_taken) *steps_taken = n;
         return esdg_finish(ctx_);
@@ -325,7 +338,7 @@ class deviceclass {
                         o_DBSurf2, o_EleSizes, o_LambdaMax, o_ViscPara, o_ViscForPlot, o_red, o_EdgeData, o_EdgeReversed,
                         o_ElemToEdge, o_ElemEdgeMasterSlave, o_ElemEdgeOrientation, o_qGradientX, o_qGradientY,
                         o_SurfGradientX, o_SurfGradientY, o_qGradientXL, o_qGradientXR, o_qGradientYL, o_qGradientYR,
-                        o_SurfacePartsVisc, o_ViscParaL, o_ViscParaR};
+                        o_SurfacePartsVisc, o_ViscParaL, o_ViscParaR, o_ArrivalTimings, o_MaximumElevation};
         for (void* p : ptrs)
             if (p) esdg_free(ctx_, p);
         fused_.release(ctx_);
@@ -365,6 +378,8 @@ class deviceclass {
         if (n == "SurfaceParts") return o_SurfaceParts;
         if (n == "LambdaMax") return o_LambdaMax;
         if (n == "ViscPara") return o_ViscPara;
+        if (n == "ArrivalTimings") return o_ArrivalTimings;
+        if (n == "MaximumElevation") return o_MaximumElevation;
         if (n == "qGradientX") return o_qGradientX;
         if (n == "qGradientY") return o_qGradientY;
         if (n == "bL") return o_bL;
@@ -398,6 +413,7 @@ class deviceclass {
     double *o_qL = nullptr, *o_qR = nullptr, *o_bL = nullptr, *o_bR = nullptr, *o_SurfaceParts = nullptr;
     double *o_DBSurf1 = nullptr, *o_DBSurf2 = nullptr;
     double *o_EleSizes = nullptr, *o_LambdaMax = nullptr, *o_ViscPara = nullptr, *o_ViscForPlot = nullptr, *o_red = nullptr;
+    double *o_ArrivalTimings = nullptr, *o_MaximumElevation = nullptr;
     int *o_EdgeData = nullptr, *o_EdgeReversed = nullptr, *o_ElemToEdge = nullptr, *o_ElemEdgeMasterSlave = nullptr,
         *o_ElemEdgeOrientation = nullptr;
     double *o_qGradientX = nullptr, *o_qGradientY = nullptr, *o_SurfGradientX = nullptr, *o_SurfGradientY = nullptr;

@@ -76,6 +76,8 @@ int check_scenario(esdg_ctx* ctx, int i, const esdg_params& p, const esdg_inputs
     if (p.bc_kind < ESDG_BC_WALLS || p.bc_kind > ESDG_BC_OCEAN)
         return set_error(ctx, ESDG_ERR_ARG, at + "bc_kind must be walls, inner/outer or ocean");
     if (r.artificial_viscosity) return set_error(ctx, ESDG_ERR_ARG, at + "artificial viscosity is not supported");
+    if (r.arrival_times || r.maximum_elevation)
+        return set_error(ctx, ESDG_ERR_ARG, at + "the analysis buffers are not kept for ensembles");
     if (!r.flux_differencing) return set_error(ctx, ESDG_ERR_ARG, at + "needs flux differencing");
     if (r.rk_ssp || (r.rk_order != 3 && r.rk_order != 4)) return set_error(ctx, ESDG_ERR_ARG, at + "needs low-storage RK 3 or 4");
     if (!(r.CFL > 0.0) || !(r.min_ele_size > 0.0)) return set_error(ctx, ESDG_ERR_ARG, at + "CFL and min_ele_size must be > 0");

@@ -32,7 +32,8 @@ class RunParamsC(C.Structure):
     _fields_ = [("CFL", C.c_double), ("DFL", C.c_double), ("T", C.c_double),
                 ("min_ele_size", C.c_double), ("artificial_viscosity", C.c_int),
                 ("positivity", C.c_int), ("rk_order", C.c_int), ("rk_ssp", C.c_int),
-                ("flux_differencing", C.c_int), ("fused", C.c_int)]
+                ("flux_differencing", C.c_int), ("fused", C.c_int), ("arrival_times", C.c_int),
+                ("maximum_elevation", C.c_int)]
 
 
 _INPUT_DOUBLES = ("GLw", "nx", "ny", "scal", "Yxi", "Yeta", "Xxi", "Xeta", "B", "Bx", "By", "D",
@@ -175,6 +176,10 @@ class Context:
                 cargs.append(C.c_int(int(a)))
             elif isinstance(a, (float, np.floating)):
                 cargs.append(C.c_double(float(a)))
+            elif isinstance(a, np.ndarray):   # host output (e.g. an integral), C-contiguous
+                if not a.flags.c_contiguous:
+                    raise TypeError("host arrays must be C-contiguous")
+                cargs.append(C.c_void_p(a.ctypes.data))
             else:
                 raise TypeError(f"unsupported argument {a!r}")
         self.check(fn(*cargs))
@@ -240,7 +245,8 @@ def make_inputs(case: Case, rp: RunParams, q=None):
 
 def make_run_params(case: Case, rp: RunParams, fused: int = 0) -> RunParamsC:
     return RunParamsC(rp.CFL, rp.DFL, rp.T, case.min_ele_size, rp.artificial_viscosity,
-                      rp.positivity, rp.rk_order, rp.rk_ssp, rp.flux_differencing, int(fused))
+                      rp.positivity, rp.rk_order, rp.rk_ssp, rp.flux_differencing, int(fused),
+                      int(rp.arrival_times), int(rp.maximum_elevation))
 
 
 class Solver:
@@ -277,6 +283,11 @@ class Solver:
         d = C.c_double()
         self.ctx.check(lib().esdg_solver_last_dt(C.c_void_p(self.handle), C.byref(d)))
         return d.value
+
+    def time(self) -> float:
+        t = C.c_double()
+        self.ctx.check(lib().esdg_solver_time(C.c_void_p(self.handle), C.byref(t)))
+        return t.value
 
     def set_indicator(self, mlp, lo: float = 0.5, hi: float = 0.5):
         """Learned troubled-cell indicator in the AV path (None: the modal sensor)."""

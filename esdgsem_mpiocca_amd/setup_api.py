@@ -90,6 +90,8 @@ class RunParams:
     zero_tol: float = 1.0e-5    # hard-coded in the reference (src/deviceclass.cpp:148)
     bc_kind: int = 0            # 0 SolidWalls, 1 InnerOuter, 2 OceanBoundary (deviceclass.cpp:363-415)
     h_zero: float = 0.0         # still-water level of the ocean boundary (define h_zero)
+    arrival_times: int = 0      # calcArrivalTimes (deviceclass.cpp:1132-1135)
+    maximum_elevation: int = 0  # calcMaximumElevation (deviceclass.cpp:1138-1141)
 
 
 class Case:
@@ -205,17 +207,23 @@ def rk_coefficients(order: int, ssp: int = 0):
     return A[:n].copy(), B[:n].copy(), Cc[:n].copy()
 
 
+def _dptr(keep: list, a):
+    """Pointer to a contiguous float64 view of a; the array is kept alive in `keep`."""
+    arr = np.ascontiguousarray(a, dtype=np.float64)
+    keep.append(arr)
+    return arr.ctypes.data_as(C.POINTER(C.c_double))
+
+
 def total_entropy(case: Case, q: np.ndarray, g: float) -> float:
-    out = C.c_double()
-    ptr = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
-    _check(_lib().esdg_total_entropy(case.ngl, case.Nelem, C.c_double(g), ptr(case["w_GL"]),
-                                     ptr(q), ptr(case["b"]), ptr(case["Jac"]), C.byref(out)))
+    out, keep = C.c_double(), []
+    _check(_lib().esdg_total_entropy(case.ngl, case.Nelem, C.c_double(g), _dptr(keep, case["w_GL"]),
+                                     _dptr(keep, q), _dptr(keep, case["b"]), _dptr(keep, case["Jac"]),
+                                     C.byref(out)))
     return out.value
 
 
 def total_mass(case: Case, q: np.ndarray) -> float:
-    out = C.c_double()
-    ptr = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
-    _check(_lib().esdg_total_mass(case.ngl, case.Nelem, ptr(case["w_GL"]), ptr(q),
-                                  ptr(case["Jac"]), C.byref(out)))
+    out, keep = C.c_double(), []
+    _check(_lib().esdg_total_mass(case.ngl, case.Nelem, _dptr(keep, case["w_GL"]), _dptr(keep, q),
+                                  _dptr(keep, case["Jac"]), C.byref(out)))
     return out.value

@@ -174,6 +174,25 @@ int esdg_surface_kernel_visc(esdg_ctx* ctx, int Nelem, const double* Jac,
 int esdg_add_source(esdg_ctx* ctx, int Nelem, const double* Bx, const double* By, const double* B,
                     const double* x, const double* y, double t, double* Qt);
 
+/* ---- analysis and solution integrals --------------------------------------------------- */
+/* ArrivalTimes (okl/Analysis/ArrivalTimes.okl:16-62; called after every step when
+ * calcArrivalTimes, src/deviceclass.cpp:1132-1135): where ArrivalTimings == 1000.0 (its initial
+ * value, :324-333) and |h + b - h_zero| > 5e-5, set it to t.  Layout [Nelem][ngl][ngl]. */
+int esdg_arrival_times(esdg_ctx* ctx, int Nelem, double t, const double* q, const double* B,
+                       double* ArrivalTimings);
+/* MaximumElevation (okl/Analysis/MaximumElevation.okl:16-55; :1138-1141): MaximumElevation =
+ * max(MaximumElevation, h + b - h_zero) (fmax; initial value 0, :337-342).  t is unused. */
+int esdg_maximum_elevation(esdg_ctx* ctx, int Nelem, double t, const double* q, const double* B,
+                           double* MaximumElevation);
+/* basis::calcTotalMass (src/basis.cpp:1088-1111) on device arrays: sum of h J w_i w_j (Jac = 1/J);
+ * basis::calcTotalEntropy (:1004-1055): sum of (h|u|^2/2 + g h^2/2 + g h b) J w_i w_j, u = v = 0
+ * where h <= 1e-12.  Device reductions in a fixed order; the result (host) differs from the
+ * reference's serial sum by rounding only.  Blocking. */
+int esdg_total_mass_dev(esdg_ctx* ctx, int Nelem, const double* q, const double* Jac, const double* GLw,
+                        double* host_mass);
+int esdg_total_entropy_dev(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* Jac,
+                           const double* GLw, double* host_entropy);
+
 /* Device-wide max of x[0..n) into *d_out (device pointer).  Replaces the host max loop of
  * GetGlobalLambdaMax / GetGlobalViscParaMax (src/MPI_Communication.cpp:490-541); max is exact. */
 int esdg_reduce_max(esdg_ctx* ctx, int n, const double* x, double* d_out);
@@ -189,6 +208,8 @@ typedef struct esdg_run_params {
     int flux_differencing;        /* FluxDifferencing */
     int fused;                    /* 0: the reference's kernel sequence; 1: fused stage kernels;
                                      2: single-kernel stage (no artificial viscosity, 2 <= N <= 7) */
+    int arrival_times;            /* calcArrivalTimes: buffer "ArrivalTimings" (fused 0 or 1) */
+    int maximum_elevation;        /* calcMaximumElevation: buffer "MaximumElevation" (fused 0 or 1) */
 } esdg_run_params;
 
 /* Host arrays handed over once, as deviceclass::copyDeviceVariables (deviceclass.cpp:577-626). */

@@ -1221,3 +1221,61 @@ void oracle_volume_std(const oracle_params* p, int Nelem, const double* Jac, con
             }
     }
 }
+
+/* okl/Analysis/ArrivalTimes.okl:40-55: first time |h + b - h_zero| exceeds 5e-5 (sentinel 1000) */
+void oracle_arrival_times(const oracle_params* p, int Nelem, double t, const double* q, const double* b,
+                          double* arrival) {
+    const int ngl2 = p->ngl * p->ngl;
+    for (int e = 0; e < Nelem; ++e)
+        for (int k = 0; k < ngl2; ++k) {
+            const long x = (long)e * ngl2 + k;
+            if (arrival[x] == 1000.0 && fabs((q[(long)e * ngl2 * NEQ + k] + b[x]) - p->h_zero) > 0.00005)
+                arrival[x] = t;
+        }
+}
+
+/* okl/Analysis/MaximumElevation.okl:44-48 */
+void oracle_maximum_elevation(const oracle_params* p, int Nelem, const double* q, const double* b,
+                              double* max_elevation) {
+    const int ngl2 = p->ngl * p->ngl;
+    for (int e = 0; e < Nelem; ++e)
+        for (int k = 0; k < ngl2; ++k) {
+            const long x = (long)e * ngl2 + k;
+            const double fs = q[(long)e * ngl2 * NEQ + k] + b[x] - p->h_zero;
+            max_elevation[x] = fmax(max_elevation[x], fs);
+        }
+}
+
+/* src/basis.cpp:1088-1111 (J there is the inverse Jacobian, as Jac here) */
+void oracle_total_mass(const oracle_params* p, int Nelem, const double* q, const double* Jac,
+                       const double* GLw, double* out) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    double m = 0.0;
+    for (int e = 0; e < Nelem; ++e)
+        for (int j = 0; j < ngl; ++j)
+            for (int i = 0; i < ngl; ++i) {
+                const long x = (long)e * ngl2 + j * ngl + i;
+                m += q[(long)e * ngl2 * NEQ + j * ngl + i] / Jac[x] * GLw[i] * GLw[j];
+            }
+    out[0] = m;
+}
+
+/* src/basis.cpp:1004-1055 */
+void oracle_total_entropy(const oracle_params* p, int Nelem, const double* q, const double* b,
+                          const double* Jac, const double* GLw, double* out) {
+    const int ngl = p->ngl, ngl2 = ngl * ngl;
+    const double g = p->g;
+    double s = 0.0;
+    for (int e = 0; e < Nelem; ++e)
+        for (int j = 0; j < ngl; ++j)
+            for (int i = 0; i < ngl; ++i) {
+                const long x = (long)e * ngl2 + j * ngl + i;
+                const long id = (long)e * ngl2 * NEQ + j * ngl + i;
+                const double h = q[id], hu = q[id + ngl2], hv = q[id + 2 * ngl2];
+                double u = 0.0, v = 0.0;
+                if (h > 1e-12) { u = hu / h; v = hv / h; }
+                const double E = 0.5 * h * (u * u + v * v) + 0.5 * g * h * h + g * h * b[x];
+                s += E / Jac[x] * GLw[i] * GLw[j];
+            }
+    out[0] = s;
+}

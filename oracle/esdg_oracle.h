@@ -108,6 +108,18 @@ void oracle_surface_visc(const oracle_params* p, int Nelem, const double* Jac, c
 /* DGtimeloop (src/deviceclass.cpp:868-1130) with a step cap: advances q in place for at most
  * max_steps steps or until t >= T.  num_flux: 0 ES, 3 rotated ES.  Returns the number of steps
  * taken; *t_out gets the final time, dt_hist (nullable, max_steps entries) the step sizes. */
+/* okl/Analysis/ArrivalTimes.okl:16-62 and MaximumElevation.okl:16-55 (element-wise) */
+void oracle_arrival_times(const oracle_params* p, int Nelem, double t, const double* q, const double* b,
+                          double* arrival);
+void oracle_maximum_elevation(const oracle_params* p, int Nelem, const double* q, const double* b,
+                              double* max_elevation);
+/* basis::calcTotalMass (src/basis.cpp:1088-1111) and basis::calcTotalEntropy with
+ * calcEntropyPointwise (:1004-1055): one serial sum in the reference's order; out[0] */
+void oracle_total_mass(const oracle_params* p, int Nelem, const double* q, const double* Jac,
+                       const double* GLw, double* out);
+void oracle_total_entropy(const oracle_params* p, int Nelem, const double* q, const double* b,
+                          const double* Jac, const double* GLw, double* out);
+
 typedef struct oracle_run {
     int Nelem, Nfaces;
     double CFL, DFL, T, min_ele_size;

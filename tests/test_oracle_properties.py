@@ -593,3 +593,65 @@ def test_manufactured_solution_converges_at_order_N_plus_1(N, eoc_min):
         errs.append(np.abs(q - manufactured(c, t)).max())
     eoc = np.log2(errs[0] / errs[1])
     assert eoc >= eoc_min, (errs, eoc)
+
+
+# --------------------------------------------------------------------------------------------
+# analysis kernels and solution integrals (okl/Analysis, src/basis.cpp:1004-1111)
+# --------------------------------------------------------------------------------------------
+def test_integrals_closed_forms():
+    """Mass of h = 1 is the domain area; the lake-at-rest entropy integral matches an independent
+    tensor-product quadrature; the setup library's host diagnostics equal the oracle bit for bit."""
+    case = Case.dam_break(N=4, nel=6, noise=0.0)
+    p = O.params(case, RunParams())
+    ngl, E = case.ngl, case.Nelem
+    q = np.zeros((E, 3, ngl * ngl))
+    q[:, 0] = 1.0
+    out = np.zeros(1)
+    O.call("total_mass", p, E, q.ravel(), case["Jac"], case["w_GL"], out)
+    assert abs(out[0] - 100.0) <= 1e-12 * 100.0
+    b = case["b"].reshape(E, ngl * ngl)
+    q[:, 0] = 2.0 - b
+    O.call("total_entropy", p, E, q.ravel(), case["b"], case["Jac"], case["w_GL"], out)
+    w = case["w_GL"]
+    W = np.outer(w, w).ravel()                      # node j*ngl + i -> w_j w_i
+    J = 1.0 / case["Jac"].reshape(E, ngl * ngl)
+    h = q[:, 0]
+    ref = np.sum((0.5 * G * h * h + G * h * b) * J * W)
+    assert abs(out[0] - ref) <= 1e-12 * abs(ref)
+    assert out[0] == S.total_entropy(case, q.ravel(), G)
+    O.call("total_mass", p, E, q.ravel(), case["Jac"], case["w_GL"], out)
+    assert out[0] == S.total_mass(case, q.ravel())
+
+
+def test_arrival_times_and_maximum_elevation():
+    """Arrival time: set once, at the first call where |h + b - h_zero| > 5e-5; maximum
+    elevation: the running max of h + b - h_zero, starting from 0."""
+    case = Case.dam_break(N=3, nel=4, noise=0.0)
+    rp = RunParams(h_zero=1.5)
+    p = O.params(case, rp)
+    E, n2 = case.Nelem, case.ngl ** 2
+    b = case["b"].reshape(E, n2)
+    rng = np.random.default_rng(3)
+    arr = np.full(E * n2, 1000.0)
+    mx = np.zeros(E * n2)
+    q = np.zeros((E, 3, n2))
+    q[:, 0] = 1.5 - b                                 # at rest: nothing arrives
+    O.call("arrival_times", p, E, 0.25, q.ravel(), case["b"], arr)
+    assert (arr == 1000.0).all()
+    wet = rng.uniform(size=(E, n2)) < 0.3
+    q[:, 0] = 1.5 - b + np.where(wet, 1e-3, 2e-5)     # 2e-5 stays below the 5e-5 threshold
+    O.call("arrival_times", p, E, 0.5, q.ravel(), case["b"], arr)
+    assert np.array_equal(arr.reshape(E, n2) == 0.5, wet) and (arr[~wet.ravel()] == 1000.0).all()
+    q[:, 0] = 1.5 - b + 1e-2
+    O.call("arrival_times", p, E, 0.75, q.ravel(), case["b"], arr)
+    assert np.array_equal(arr.reshape(E, n2), np.where(wet, 0.5, 0.75))
+    for level in (-0.2, 0.3, 0.1):
+        q[:, 0] = 1.5 - b + level * rng.uniform(size=(E, n2))
+        O.call("maximum_elevation", p, E, q.ravel(), case["b"], mx)
+    # the running max from 0: recompute independently
+    rng = np.random.default_rng(3)
+    rng.uniform(size=(E, n2))
+    ref = np.zeros((E, n2))
+    for level in (-0.2, 0.3, 0.1):
+        ref = np.maximum(ref, (1.5 - b + level * rng.uniform(size=(E, n2))) + b - 1.5)
+    assert np.allclose(mx.reshape(E, n2), ref, rtol=0, atol=1e-15)
