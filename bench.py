@@ -280,12 +280,16 @@ def c4_bench(dev, args):
     return out
 
 
-def load_traffic(name):
-    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+def load_profile_json(fname):
+    path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f)
+
+
+def load_traffic(name):
+    return load_profile_json(f"{name}_traffic.json")
 
 
 def main():
@@ -388,6 +392,13 @@ def main():
                     "avg_launch_us": round(per_kernel[vol_name]["avg_us"], 2),
                     "fp64_tflops_algorithmic": round(tfl, 2),
                     "fp64_frac_of_peak": round(tfl / FP64_PEAK_TFLOPS, 4)}
+        # executed FP64 operations (PMC counts of the same kernel on the same workload), when
+        # measured for this configuration
+        fl = load_profile_json(f"{vol_name}_flops.json")
+        if fl and args.fused and case.Nelem == 262144 and case.ngl == 8 and rp.artificial_viscosity:
+            ex = fl["executed_fp64_flops_per_launch"] / avg_s / 1e12
+            roofline["fp64_tflops_executed"] = round(ex, 2)
+            roofline["fp64_executed_frac_of_peak"] = round(ex / FP64_PEAK_TFLOPS, 4)
 
     cpu = None
     if not args.no_cpu_baseline:
