@@ -21,7 +21,9 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <fstream>
 #include <map>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -231,13 +233,239 @@ struct Geometry {
     std::vector<double> enx, eny, escal;                     // per element side node [e][side][k]
     std::vector<int> edge_info;                              // reference Mesh::EdgeInfo [f][7]
     std::vector<int> elem_to_edge;                           // [e][4], 1-based
+    std::vector<double> b;                                   // bathymetry from the mesh file (read_bottom)
 };
+
+// Metric terms by the nodal derivative matrix and J (Mesh.cpp:1722-1742) of one element's nodes.
+void element_metrics(int ngl, const double* D, const double* xe, const double* ye, double* xxi, double* xeta,
+                     double* yxi, double* yeta, double* Je) {
+    for (int j = 0; j < ngl; j++)
+        for (int i = 0; i < ngl; i++) {
+            const int ij = j * ngl + i;
+            double ax = 0.0, bx = 0.0, ay = 0.0, by = 0.0;
+            for (int k = 0; k < ngl; k++) {
+                ax += D[i * ngl + k] * xe[j * ngl + k];
+                bx += D[j * ngl + k] * xe[k * ngl + i];
+                ay += D[i * ngl + k] * ye[j * ngl + k];
+                by += D[j * ngl + k] * ye[k * ngl + i];
+            }
+            xxi[ij] = ax; xeta[ij] = bx; yxi[ij] = ay; yeta[ij] = by;
+            Je[ij] = xxi[ij] * yeta[ij] - xeta[ij] * yxi[ij];
+        }
+}
+
+// Outward unit normals and surface Jacobians of the four sides (Mesh.cpp:1744-1806).
+void side_normals(int ngl, const double* xxi, const double* xeta, const double* yxi, const double* yeta,
+                  const double* Je, double* nxs, double* nys, double* scs) {
+    for (int j = 0; j < ngl; j++) {
+        const int s1 = ngl + j, s3 = 3 * ngl + j;
+        const int Nj = j * ngl + ngl - 1, Zj = j * ngl;
+        double ye_ = yeta[Nj], xe_ = xeta[Nj], Jt = Je[Nj];
+        scs[s1] = std::sqrt(ye_ * ye_ + xe_ * xe_);
+        nxs[s1] = std::copysign(1.0, Jt) * (ye_ / scs[s1]);
+        nys[s1] = std::copysign(1.0, Jt) * (-xe_ / scs[s1]);
+        ye_ = yeta[Zj]; xe_ = xeta[Zj]; Jt = Je[Zj];
+        scs[s3] = std::sqrt(ye_ * ye_ + xe_ * xe_);
+        nxs[s3] = -std::copysign(1.0, Jt) * (ye_ / scs[s3]);
+        nys[s3] = -std::copysign(1.0, Jt) * (-xe_ / scs[s3]);
+    }
+    for (int i = 0; i < ngl; i++) {
+        const int s0 = i, s2 = 2 * ngl + i;
+        const int iZ = i, iN = (ngl - 1) * ngl + i;
+        double yx = yxi[iZ], xx = xxi[iZ], Jt = Je[iZ];
+        scs[s0] = std::sqrt(yx * yx + xx * xx);
+        nxs[s0] = -std::copysign(1.0, Jt) * (-yx / scs[s0]);
+        nys[s0] = -std::copysign(1.0, Jt) * (xx / scs[s0]);
+        yx = yxi[iN]; xx = xxi[iN]; Jt = Je[iN];
+        scs[s2] = std::sqrt(yx * yx + xx * xx);
+        nxs[s2] = std::copysign(1.0, Jt) * (-yx / scs[s2]);
+        nys[s2] = std::copysign(1.0, Jt) * (xx / scs[s2]);
+    }
+}
+
+// ---- ISM-V2 meshes (Mesh::ReadMesh, src/Mesh.cpp:698-1503) --------------------------------
+// Boundary curves are given, or built from their end corners, at the Chebyshev-Lobatto points
+// -cos(k pi / M) of the file's curve order M and evaluated by barycentric Lagrange interpolation
+// (Mesh.cpp:1832-1858, 1946-1977); an element with a given curve is mapped by the transfinite
+// (Coons) map of its four curves, a straight-sided one bilinearly (Mesh.cpp:1506-1665).
+struct CurveBasis {
+    int M = 0;
+    std::vector<double> xc, wb;
+    explicit CurveBasis(int order) : M(order), xc(order + 1), wb(order + 1, 1.0) {
+        for (int k = 0; k <= M; ++k) xc[k] = -std::cos(k * M_PI / M);
+        for (int j = 1; j <= M; j++)
+            for (int k = 0; k < j; k++) {
+                wb[k] = wb[k] * (xc[k] - xc[j]);
+                wb[j] = wb[j] * (xc[j] - xc[k]);
+            }
+        for (int j = 0; j <= M; j++) wb[j] = 1.0 / wb[j];
+    }
+    // barycentric form 2; a point within 1e-12 of a node takes that node's value
+    double at(const double* f, double xpt) const {
+        bool hit = false;
+        double out = 0.0;
+        for (int j = 0; j <= M; j++)
+            if (std::fabs(xpt - xc[j]) < std::pow(10.0, -12)) { hit = true; out = f[j]; }
+        if (hit) return out;
+        double num = 0.0, den = 0.0;
+        for (int j = 0; j <= M; j++) {
+            const double t = wb[j] / (xpt - xc[j]);
+            num += t * f[j];
+            den += t;
+        }
+        return num / den;
+    }
+    // Coons map of the four curves (sides 1, 2: xi = -1..1 along eta = -1 / xi = 1; 3, 4 opposite)
+    double transfinite(const double* g1, const double* g2, const double* g3, const double* g4, double psi,
+                       double eta) const {
+        const double x1_ref = at(g1, -1.0), x2_ref = at(g1, 1.0), x3_ref = at(g3, 1.0), x4_ref = at(g3, -1.0);
+        const double x1_cp = at(g1, psi), x2_cp = at(g2, eta), x3_cp = at(g3, psi), x4_cp = at(g4, eta);
+        return 0.5 * ((1.0 - psi) * x4_cp + (1.0 + psi) * x2_cp + (1.0 - eta) * x1_cp + (1.0 + eta) * x3_cp) -
+               0.25 * ((1.0 - psi) * ((1.0 - eta) * x1_ref + (1.0 + eta) * x4_ref) +
+                       (1.0 + psi) * ((1.0 - eta) * x2_ref + (1.0 + eta) * x3_ref));
+    }
+};
+
+inline double bilinear(const double* c, double psi, double eta) {
+    return 0.25 * (c[0] * (1.0 - psi) * (1.0 - eta) + c[1] * (1.0 + psi) * (1.0 - eta) +
+                   c[2] * (1.0 + psi) * (1.0 + eta) + c[3] * (1.0 - psi) * (1.0 + eta));
+}
+
+Geometry read_ism(const esdg_case_spec& s, const Basis& B) {
+    if (!s.mesh_file) throw std::invalid_argument("ISM mesh kind without a mesh file");
+    std::ifstream in(s.mesh_file);
+    if (!in) throw std::invalid_argument(std::string("mesh file not found: ") + s.mesh_file);
+    const bool rb = s.read_bottom != 0;
+    std::string line;
+    auto next = [&](const char* what) -> std::istringstream {
+        if (!std::getline(in, line)) throw std::invalid_argument(std::string("ISM mesh: unexpected end of file in ") + what);
+        return std::istringstream(line);
+    };
+    next("format");
+    int nn = 0, ne = 0, nel = 0, order = 0;
+    {
+        auto ls = next("header");
+        if (!(ls >> nn >> ne >> nel >> order) || nn < 1 || ne < 1 || nel < 1 || order < 1)
+            throw std::invalid_argument("ISM mesh: bad header");
+    }
+    const int ngl = B.ngl, ngl2 = ngl * ngl;
+    if ((int64_t)nel * ngl2 * 3 > (int64_t)INT32_MAX) throw std::invalid_argument("mesh too large for 32-bit node indices");
+    const CurveBasis CB(order);
+    std::vector<double> xn(nn), yn(nn), bn(nn, 0.0);
+    for (int i = 0; i < nn; ++i) {
+        auto ls = next("nodes");
+        if (!(ls >> xn[i] >> yn[i]) || (rb && !(ls >> bn[i]))) throw std::invalid_argument("ISM mesh: bad node line");
+    }
+    Geometry G;
+    G.ngl = ngl; G.Nelem = nel; G.Nfaces = ne;
+    G.edge_info.assign((size_t)ne * 7, 0);
+    G.elem_to_edge.assign((size_t)nel * 4, 0);
+    for (int is = 0; is < ne; ++is) {
+        auto ls = next("edges");
+        int n0, n1, eL, eR, sL, sR;
+        if (!(ls >> n0 >> n1 >> eL >> eR >> sL >> sR)) throw std::invalid_argument("ISM mesh: bad edge line");
+        if (eL < 1 || eL > nel || eR < 0 || eR > nel || sL < 1 || sL > 4 || std::abs(sR) > 4)
+            throw std::invalid_argument("ISM mesh: edge references a missing element or side");
+        int* E = &G.edge_info[(size_t)is * 7];
+        E[0] = n0 - 1; E[1] = n1 - 1; E[2] = eL - 1; E[3] = eR - 1; E[4] = sL - 1;
+        G.elem_to_edge[(size_t)(eL - 1) * 4 + sL - 1] = is + 1;
+        // a negative right side marks a reversed node order on the right element
+        E[5] = (sR < 0) ? -sR - 1 : sR - 1;
+        E[6] = (sR < 0) ? 0 : 1;
+        if (E[3] != -1) G.elem_to_edge[(size_t)(eR - 1) * 4 + E[5]] = is + 1;
+    }
+    const size_t nnodes = (size_t)nel * ngl2;
+    G.x.assign(nnodes, 0.0); G.y.assign(nnodes, 0.0);
+    G.xXi.assign(nnodes, 0.0); G.xEta.assign(nnodes, 0.0); G.yXi.assign(nnodes, 0.0); G.yEta.assign(nnodes, 0.0);
+    G.J.assign(nnodes, 0.0);
+    if (rb) G.b.assign(nnodes, 0.0);
+    G.enx.assign((size_t)nel * 4 * ngl, 0.0);
+    G.eny.assign((size_t)nel * 4 * ngl, 0.0);
+    G.escal.assign((size_t)nel * 4 * ngl, 0.0);
+    const int M = order;
+    std::vector<double> gx(4 * (M + 1)), gy(4 * (M + 1)), gb(4 * (M + 1));
+    std::vector<double> xe(ngl2), ye(ngl2), be(ngl2), xxi(ngl2), xeta(ngl2), yxi(ngl2), yeta(ngl2), Je(ngl2);
+    const double* D = B.D0.data();
+    // side j runs from corner first[j] to corner last[j] (Mesh.cpp:1014-1031)
+    const int first[4] = {0, 1, 3, 0}, last[4] = {1, 2, 2, 3};
+    for (int ie = 0; ie < nel; ++ie) {
+        int c[4], given[4];
+        {
+            auto ls = next("element corners");
+            if (!(ls >> c[0] >> c[1] >> c[2] >> c[3])) throw std::invalid_argument("ISM mesh: bad element corners");
+            for (int j = 0; j < 4; ++j) {
+                c[j] -= 1;
+                if (c[j] < 0 || c[j] >= nn) throw std::invalid_argument("ISM mesh: corner references a missing node");
+            }
+        }
+        {
+            auto ls = next("curve flags");
+            if (!(ls >> given[0] >> given[1] >> given[2] >> given[3])) throw std::invalid_argument("ISM mesh: bad curve flags");
+        }
+        const bool curved = given[0] || given[1] || given[2] || given[3];
+        double cx[4], cy[4], cb[4];
+        for (int j = 0; j < 4; ++j) {
+            cx[j] = xn[c[j]]; cy[j] = yn[c[j]]; cb[j] = bn[c[j]];
+            double* Gx = &gx[j * (M + 1)];
+            double* Gy = &gy[j * (M + 1)];
+            double* Gb = &gb[j * (M + 1)];
+            if (!given[j]) {
+                const int a = c[first[j]], z = c[last[j]];
+                for (int k = 0; k <= M; ++k) {
+                    Gx[k] = ((xn[z] - xn[a]) * CB.xc[k] + xn[a] + xn[z]) / 2.0;
+                    Gy[k] = ((yn[z] - yn[a]) * CB.xc[k] + yn[a] + yn[z]) / 2.0;
+                    if (rb) Gb[k] = ((bn[z] - bn[a]) * CB.xc[k] + bn[a] + bn[z]) / 2.0;
+                }
+            } else {
+                for (int k = 0; k <= M; ++k) {
+                    auto ls = next("boundary curve");
+                    if (!(ls >> Gx[k] >> Gy[k]) || (rb && !(ls >> Gb[k])))
+                        throw std::invalid_argument("ISM mesh: bad boundary curve point");
+                }
+            }
+        }
+        next("boundary names");
+        const double *g1x = &gx[0], *g2x = &gx[M + 1], *g3x = &gx[2 * (M + 1)], *g4x = &gx[3 * (M + 1)];
+        const double *g1y = &gy[0], *g2y = &gy[M + 1], *g3y = &gy[2 * (M + 1)], *g4y = &gy[3 * (M + 1)];
+        const double *g1b = &gb[0], *g2b = &gb[M + 1], *g3b = &gb[2 * (M + 1)], *g4b = &gb[3 * (M + 1)];
+        for (int j = 0; j < ngl; j++)
+            for (int i = 0; i < ngl; i++) {
+                const int ij = j * ngl + i;
+                const double psi = B.xGL[i], eta = B.xGL[j];
+                if (curved) {
+                    xe[ij] = CB.transfinite(g1x, g2x, g3x, g4x, psi, eta);
+                    ye[ij] = CB.transfinite(g1y, g2y, g3y, g4y, psi, eta);
+                    if (rb) be[ij] = CB.transfinite(g1b, g2b, g3b, g4b, psi, eta);
+                } else {
+                    xe[ij] = bilinear(cx, psi, eta);
+                    ye[ij] = bilinear(cy, psi, eta);
+                    if (rb) be[ij] = bilinear(cb, psi, eta);
+                }
+            }
+        element_metrics(ngl, D, xe.data(), ye.data(), xxi.data(), xeta.data(), yxi.data(), yeta.data(), Je.data());
+        const size_t base = (size_t)ie * ngl2;
+        for (int k = 0; k < ngl2; ++k) {
+            G.x[base + k] = xe[k]; G.y[base + k] = ye[k];
+            G.xXi[base + k] = xxi[k]; G.xEta[base + k] = xeta[k];
+            G.yXi[base + k] = yxi[k]; G.yEta[base + k] = yeta[k];
+            G.J[base + k] = Je[k];
+            if (rb) G.b[base + k] = be[k];
+        }
+        side_normals(ngl, xxi.data(), xeta.data(), yxi.data(), yeta.data(), Je.data(), &G.enx[(size_t)ie * 4 * ngl],
+                     &G.eny[(size_t)ie * 4 * ngl], &G.escal[(size_t)ie * 4 * ngl]);
+    }
+    for (int ie = 0; ie < nel; ++ie)
+        for (int j = 0; j < 4; ++j)
+            if (G.elem_to_edge[(size_t)ie * 4 + j] < 1) throw std::invalid_argument("ISM mesh: element side without an edge");
+    return G;
+}
 
 // Structured quad topology and node placement of Mesh::GenerateMesh (Mesh.cpp:316-696).  For
 // the Cartesian kind the metrics, J and side normals are the generator's constants; for the
 // curved kind the nodes are displaced and the metrics/normals are recomputed the way
 // Mesh::ConstructMappedGeometry does.
 Geometry make_mesh(const esdg_case_spec& s, const Basis& B) {
+    if (s.mesh_kind == ESDG_MESH_ISM) return read_ism(s, B);
     Geometry G;
     const int ngl = B.ngl, ngl2 = ngl * ngl;
     const int NelemX = s.nelx, NelemY = s.nely;
@@ -356,54 +584,15 @@ Geometry make_mesh(const esdg_case_spec& s, const Basis& B) {
                 xe[k] = xc + d;
                 ye[k] = yc + d;
             }
-            // metric terms by the nodal derivative matrix (Mesh.cpp:1722-1742)
-            for (int j = 0; j < ngl; j++)
-                for (int i = 0; i < ngl; i++) {
-                    const int ij = j * ngl + i;
-                    double ax = 0.0, bx = 0.0, ay = 0.0, by = 0.0;
-                    for (int k = 0; k < ngl; k++) {
-                        ax += D[i * ngl + k] * xe[j * ngl + k];
-                        bx += D[j * ngl + k] * xe[k * ngl + i];
-                        ay += D[i * ngl + k] * ye[j * ngl + k];
-                        by += D[j * ngl + k] * ye[k * ngl + i];
-                    }
-                    xxi[ij] = ax; xeta[ij] = bx; yxi[ij] = ay; yeta[ij] = by;
-                    Je[ij] = xxi[ij] * yeta[ij] - xeta[ij] * yxi[ij];
-                }
+            element_metrics(ngl, D, xe.data(), ye.data(), xxi.data(), xeta.data(), yxi.data(), yeta.data(), Je.data());
             for (int k = 0; k < ngl2; ++k) {
                 G.x[base + k] = xe[k]; G.y[base + k] = ye[k];
                 G.xXi[base + k] = xxi[k]; G.xEta[base + k] = xeta[k];
                 G.yXi[base + k] = yxi[k]; G.yEta[base + k] = yeta[k];
                 G.J[base + k] = Je[k];
             }
-            // outward normals and surface Jacobians per side (Mesh.cpp:1744-1806)
-            double* nxs = &G.enx[(size_t)e * 4 * ngl];
-            double* nys = &G.eny[(size_t)e * 4 * ngl];
-            double* scs = &G.escal[(size_t)e * 4 * ngl];
-            for (int j = 0; j < ngl; j++) {
-                const int s1 = ngl + j, s3 = 3 * ngl + j;
-                const int Nj = j * ngl + ngl - 1, Zj = j * ngl;
-                double ye_ = yeta[Nj], xe_ = xeta[Nj], Jt = Je[Nj];
-                scs[s1] = std::sqrt(ye_ * ye_ + xe_ * xe_);
-                nxs[s1] = std::copysign(1.0, Jt) * (ye_ / scs[s1]);
-                nys[s1] = std::copysign(1.0, Jt) * (-xe_ / scs[s1]);
-                ye_ = yeta[Zj]; xe_ = xeta[Zj]; Jt = Je[Zj];
-                scs[s3] = std::sqrt(ye_ * ye_ + xe_ * xe_);
-                nxs[s3] = -std::copysign(1.0, Jt) * (ye_ / scs[s3]);
-                nys[s3] = -std::copysign(1.0, Jt) * (-xe_ / scs[s3]);
-            }
-            for (int i = 0; i < ngl; i++) {
-                const int s0 = i, s2 = 2 * ngl + i;
-                const int iZ = i, iN = (ngl - 1) * ngl + i;
-                double yx = yxi[iZ], xx = xxi[iZ], Jt = Je[iZ];
-                scs[s0] = std::sqrt(yx * yx + xx * xx);
-                nxs[s0] = -std::copysign(1.0, Jt) * (-yx / scs[s0]);
-                nys[s0] = -std::copysign(1.0, Jt) * (xx / scs[s0]);
-                yx = yxi[iN]; xx = xxi[iN]; Jt = Je[iN];
-                scs[s2] = std::sqrt(yx * yx + xx * xx);
-                nxs[s2] = std::copysign(1.0, Jt) * (-yx / scs[s2]);
-                nys[s2] = std::copysign(1.0, Jt) * (xx / scs[s2]);
-            }
+            side_normals(ngl, xxi.data(), xeta.data(), yxi.data(), yeta.data(), Je.data(), &G.enx[(size_t)e * 4 * ngl],
+                         &G.eny[(size_t)e * 4 * ngl], &G.escal[(size_t)e * 4 * ngl]);
         }
     } else if (s.mesh_kind != ESDG_MESH_CARTESIAN) {
         throw std::invalid_argument("unknown mesh kind");
@@ -453,7 +642,8 @@ namespace {
 //                  rotation p2 (radians), axes a = p0, c = p1 (metres), centre at 40% / 50%.
 //   CONSTANT:      h = p0, hu = p1, hv = p2, b = 0.
 //   CONVERGENCE:   b = 2 + 0.5 sin(2 pi y) + cos(2 pi x); h = 8 + cos x sin y - b; hu = h/2, hv = 3h/2.
-void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<double>& y) {
+void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<double>& y,
+                const std::vector<double>* bmesh) {
     const esdg_case_spec& s = C.spec;
     const int ngl2 = C.B.ngl * C.B.ngl;
     const size_t nn = (size_t)C.Nelem * ngl2;
@@ -481,14 +671,16 @@ void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<do
             const size_t qid = (size_t)e * ngl2 * 3 + k;
             const double X = x[id], Y = y[id];
             double bb = 0.0, h = 0.0, hu = 0.0, hv = 0.0;
+            // the case's bathymetry, or the mesh file's where it carries one
+            auto pick = [&](double formula) { return bmesh ? (*bmesh)[id] : formula; };
             switch (s.case_kind) {
                 case ESDG_CASE_GAUSSIAN_BUMP:
-                    bb = 0.8 * std::exp(-5.0 * ((X - 1.0) * (X - 1.0) + (Y - 1.0) * (Y - 1.0)));
+                    bb = pick(0.8 * std::exp(-5.0 * ((X - 1.0) * (X - 1.0) + (Y - 1.0) * (Y - 1.0))));
                     h = 1.0 - bb + s.amp * std::exp(-20.0 * ((X - 0.7) * (X - 0.7) + (Y - 1.0) * (Y - 1.0)));
                     break;
                 case ESDG_CASE_DAM_BREAK: {
                     const double xd = (s.p0 != 0.0) ? s.p0 : 5.0;
-                    bb = 0.1 * std::sin(X) * std::cos(Y);
+                    bb = pick(0.1 * std::sin(X) * std::cos(Y));
                     if (s.p1 == 0.0 && s.p2 == 0.0) {
                         h = (X < xd) ? 2.0 : 1.0;
                     } else {
@@ -505,7 +697,7 @@ void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<do
                     double acc = 0.0;
                     for (int m = 0; m < 16; ++m)
                         acc += std::cos(2.0 * M_PI * (km[m] * (X - s.xL) + lm[m] * (Y - s.yL)) / Lx + ph[m]);
-                    bb = 0.3 / 4.0 * acc;
+                    bb = pick(0.3 / 4.0 * acc);
                     const double level = (X < s.p1) ? s.p0 : s.p2;
                     h = level - bb;
                     if (h < 0.0) h = 0.0;
@@ -515,7 +707,7 @@ void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<do
                     const double xs = s.xL + 0.8 * Lx;
                     double depth = 4000.0;
                     if (X > xs) depth = 4000.0 + (50.0 - 4000.0) * (X - xs) / (0.2 * Lx);
-                    bb = 4000.0 - depth;
+                    bb = pick(4000.0 - depth);
                     const double cx = s.xL + 0.4 * Lx, cy = s.yL + 0.5 * Ly;
                     const double c = std::cos(s.p2), sn = std::sin(s.p2);
                     const double xr = c * (X - cx) + sn * (Y - cy);
@@ -525,14 +717,14 @@ void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<do
                 }
                 case ESDG_CASE_CONVERGENCE: {
                     // SW2D::InitQNodal case 1 with SW2D bathymetry case 1, at t = 0
-                    bb = 2.0 + 0.5 * std::sin(2.0 * M_PI * Y) + std::cos(2.0 * M_PI * X);
+                    bb = pick(2.0 + 0.5 * std::sin(2.0 * M_PI * Y) + std::cos(2.0 * M_PI * X));
                     h = 8.0 + std::cos(X) * std::sin(Y) * std::cos(0.0) - bb;
                     hu = h * 0.5;
                     hv = h * 1.5;
                     break;
                 }
                 case ESDG_CASE_CONSTANT:
-                    bb = 0.0; h = s.p0; hu = s.p1; hv = s.p2;
+                    bb = pick(0.0); h = s.p0; hu = s.p1; hv = s.p2;
                     break;
                 default:
                     throw std::invalid_argument("unknown case kind");
@@ -600,7 +792,7 @@ void build_case(esdg_case& C) {
     }
     C.min_ele_size = std::sqrt(tmin);
 
-    make_state(C, G.x, G.y);
+    make_state(C, G.x, G.y, G.b.empty() ? nullptr : &G.b);
 
     // split-form bathymetry gradient (SW2D.cpp:1048-1105)
     const std::vector<double>& b = C.d["b"];

@@ -51,7 +51,7 @@ class CaseSpec(C.Structure):
         ("periodic_x", C.c_int), ("periodic_y", C.c_int),
         ("perturb_amp", C.c_double), ("perturb_len", C.c_double), ("g", C.c_double),
         ("amp", C.c_double), ("p0", C.c_double), ("p1", C.c_double), ("p2", C.c_double),
-        ("p3", C.c_double), ("seed", C.c_uint64),
+        ("p3", C.c_double), ("seed", C.c_uint64), ("mesh_file", C.c_char_p), ("read_bottom", C.c_int),
     ]
 
 
@@ -61,7 +61,7 @@ class CaseInfo(C.Structure):
 
 
 CASE_GAUSSIAN_BUMP, CASE_DAM_BREAK, CASE_FLOOD, CASE_OCEAN_UPLIFT, CASE_CONSTANT, CASE_CONVERGENCE = range(6)
-MESH_CARTESIAN, MESH_CURVED_SINE = 0, 1
+MESH_CARTESIAN, MESH_CURVED_SINE, MESH_ISM = 0, 1, 2
 
 DOUBLE_ARRAYS = ("x", "y", "xXi", "xEta", "yXi", "yEta", "Jac", "b", "Bx", "By", "q", "nx", "ny",
                  "scal", "EleSizes", "x_GL", "w_GL", "D", "D0", "Dstrong", "Dhat", "VdmInv")
@@ -99,9 +99,11 @@ class Case:
 
     def __init__(self, case_kind, mesh_kind, N, nelx, nely, xL, xR, yL, yR, periodic_x=0,
                  periodic_y=0, perturb_amp=0.0, perturb_len=1.0, g=9.81, amp=0.0, p=(0, 0, 0, 0),
-                 seed=0):
+                 seed=0, mesh_file=None, read_bottom=0):
+        self._mesh_file = None if mesh_file is None else os.fsencode(mesh_file)
         spec = CaseSpec(case_kind, mesh_kind, N, nelx, nely, xL, xR, yL, yR, periodic_x,
-                        periodic_y, perturb_amp, perturb_len, g, amp, p[0], p[1], p[2], p[3], seed)
+                        periodic_y, perturb_amp, perturb_len, g, amp, p[0], p[1], p[2], p[3], seed,
+                        self._mesh_file, int(read_bottom))
         self.spec = spec
         h = C.c_void_p()
         _check(_lib().esdg_case_create(C.byref(spec), C.byref(h)))
@@ -128,6 +130,15 @@ class Case:
     @property
     def geom_face(self) -> float:
         return 1.0 / float(self.arrays["w_GL"][0])
+
+    @classmethod
+    def from_ism(cls, path, N, case_kind=CASE_CONSTANT, p=(1.0, 0.0, 0.0, 0.0), amp=0.0, seed=0,
+                 read_bottom=0, bounds=(0.0, 1.0, 0.0, 1.0)):
+        """An ISM-V2 mesh file (Mesh::ReadMesh) with a case's state; `bounds` only feeds the case
+        formulas that use the domain extent."""
+        xL, xR, yL, yR = bounds
+        return cls(case_kind, MESH_ISM, N, 0, 0, xL, xR, yL, yR, amp=amp, p=p, seed=seed,
+                   mesh_file=path, read_bottom=read_bottom)
 
     # --- the benchmark configurations -------------------------------------------------
     @classmethod

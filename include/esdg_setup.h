@@ -45,9 +45,12 @@ int esdg_rk_coefficients(int order, int ssp, int* stages, double* A, double* B, 
 
 enum esdg_mesh_kind {
     ESDG_MESH_CARTESIAN = 0,      /* Mesh::GenerateMesh, src/Mesh.cpp:316-696 */
-    ESDG_MESH_CURVED_SINE = 1     /* same topology, nodes x += a*sin(pi*x/w)*sin(pi*y/w), same in y,
+    ESDG_MESH_CURVED_SINE = 1,    /* same topology, nodes x += a*sin(pi*x/w)*sin(pi*y/w), same in y,
                                      metrics and normals as Mesh::ConstructMappedGeometry
                                      (src/Mesh.cpp:1667-1808) */
+    ESDG_MESH_ISM = 2             /* ISM-V2 mesh file (Mesh::ReadMesh, src/Mesh.cpp:698-1503): corner
+                                     nodes, edges, Chebyshev-Lobatto boundary curves, transfinite map
+                                     (:1506-1808); spec.mesh_file, spec.read_bottom */
 };
 
 enum esdg_case_kind {
@@ -76,6 +79,9 @@ typedef struct esdg_case_spec {
     double amp;           /* case amplitude (bump height, noise amplitude, uplift amplitude ...) */
     double p0, p1, p2, p3;/* case parameters, meaning per case kind (see esdg_setup.cpp) */
     uint64_t seed;        /* seeds the counter-based generator (SplitMix64) */
+    const char* mesh_file;/* ESDG_MESH_ISM: path of the ISM-V2 file (nelx, nely, x/y bounds unused) */
+    int read_bottom;      /* ESDG_MESH_ISM: b from the file's third node/curve column (ReadBottom)
+                             instead of the case's bathymetry */
 } esdg_case_spec;
 
 typedef struct esdg_case esdg_case;

@@ -80,6 +80,38 @@ int ref_cartesian_mesh(const char* workdir, int N, int nelx, int nely, double xL
     return 0;
 }
 
+// ISM-V2 mesh through Mesh::InitMesh (file branch): Mesh.cpp:62-307, 698-1808.  Sizes first
+// (arrays null), then the arrays.
+int ref_ism_mesh(const char* path, int N, int read_bottom, int* Nelem_out, int* Nfaces_out, double* x,
+                 double* y, double* xXi, double* xEta, double* yXi, double* yEta, double* J, double* b,
+                 double* nx, double* ny, double* scal, int* EdgeInfo, int* ElemToEdge, int* MasterSlave,
+                 int* Orientation) {
+    basis B(N, 1);
+    const int ngl = N + 1, ngl2 = ngl * ngl;
+    Mesh M(B.x_GL, ngl, 0, read_bottom, 0);
+    M.InitMesh(std::string(path), false, 0);
+    const int Ne = M.m_num_elements, Nf = M.m_num_edges;
+    *Nelem_out = Ne;
+    *Nfaces_out = Nf;
+    if (!x) return 0;
+    std::memcpy(x, M.x_global, sizeof(double) * Ne * ngl2);
+    std::memcpy(y, M.y_global, sizeof(double) * Ne * ngl2);
+    std::memcpy(xXi, M.xXi_global, sizeof(double) * Ne * ngl2);
+    std::memcpy(xEta, M.xEta_global, sizeof(double) * Ne * ngl2);
+    std::memcpy(yXi, M.yXi_global, sizeof(double) * Ne * ngl2);
+    std::memcpy(yEta, M.yEta_global, sizeof(double) * Ne * ngl2);
+    std::memcpy(J, M.J_global, sizeof(double) * Ne * ngl2);
+    if (read_bottom) std::memcpy(b, M.b_global, sizeof(double) * Ne * ngl2);
+    std::memcpy(nx, M.NormalsX, sizeof(double) * Nf * ngl);
+    std::memcpy(ny, M.NormalsY, sizeof(double) * Nf * ngl);
+    std::memcpy(scal, M.Scal, sizeof(double) * Nf * ngl);
+    std::memcpy(EdgeInfo, M.EdgeInfo, sizeof(int) * Nf * 7);
+    std::memcpy(ElemToEdge, M.ElementToEdge, sizeof(int) * Ne * 4);
+    std::memcpy(MasterSlave, M.ElemEdgeMasterSlave, sizeof(int) * Ne * 4);
+    std::memcpy(Orientation, M.ElemEdgeOrientation, sizeof(int) * Ne * 4);
+    return 0;
+}
+
 int ref_bathymetry_gradient(int Nelem, int ngl, double g, const double* x, const double* y,
                             const double* b, const double* D0, const double* yEta, const double* yXi,
                             const double* xEta, const double* xXi, double* Bx, double* By,

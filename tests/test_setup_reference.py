@@ -161,3 +161,107 @@ def test_golden_setup_fixture():
         assert same_bits(g[f"bump_{k}"], c[k]), k
     for k in ("EdgeData", "ElemToEdge", "ElemEdgeMasterSlave", "ElemEdgeOrientation"):
         assert np.array_equal(g[f"bump_{k}"], c[k]), k
+
+
+# --------------------------------------------------------------------------------------------
+# ISM-V2 meshes (Mesh::ReadMesh, src/Mesh.cpp:698-1808) against the reference's own reader
+# --------------------------------------------------------------------------------------------
+MESH_DIR = "/root/reference/meshFiles"          # the reference's shipped mesh files (read here only)
+FIXTURE_MESHES = os.path.join(ROOT, "tests", "golden", "meshes")
+
+
+def ref_ism(ref, path, N, read_bottom=0):
+    ne, nf = C.c_int(), C.c_int()
+    null = [None] * 15
+    assert ref.ref_ism_mesh(path.encode(), N, read_bottom, C.byref(ne), C.byref(nf), *null) == 0
+    Ne, Nf, ngl = ne.value, nf.value, N + 1
+    nn = Ne * ngl * ngl
+    arr = {k: np.zeros(nn) for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "J", "b")}
+    fa = {k: np.zeros(Nf * ngl) for k in ("nx", "ny", "scal")}
+    EI = np.zeros(Nf * 7, np.int32)
+    E2E, MS, OR = (np.zeros(Ne * 4, np.int32) for _ in range(3))
+    assert ref.ref_ism_mesh(path.encode(), N, read_bottom, C.byref(ne), C.byref(nf),
+                            *[P(arr[k]) for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "J", "b")],
+                            *[P(fa[k]) for k in ("nx", "ny", "scal")], P(EI), P(E2E), P(MS), P(OR)) == 0
+    return arr, fa, EI.reshape(-1, 7), E2E, MS, OR
+
+
+def check_ism(ref, path, N, read_bottom=0):
+    arr, fa, EI, E2E, MS, OR = ref_ism(ref, path, N, read_bottom)
+    c = Case.from_ism(path, N, read_bottom=read_bottom)
+    assert c.Nelem * c.ngl ** 2 == len(arr["x"]) and c.Nfaces == len(EI)
+    for k in ("x", "y", "xXi", "xEta", "yXi", "yEta"):
+        assert same_bits(arr[k], c[k]), k
+    assert same_bits(1.0 / arr["J"], c["Jac"])
+    for k in ("nx", "ny", "scal"):
+        assert same_bits(fa[k], c[k]), k
+    if read_bottom:
+        assert same_bits(arr["b"], c["b"])
+    ED = c["EdgeData"].reshape(-1, 8)
+    assert np.array_equal(ED[:, :5], EI[:, 2:7])
+    assert np.array_equal(ED[:, 6], np.where(EI[:, 3] == -1, -1, 0))
+    assert np.array_equal(c["ElemToEdge"], E2E)
+    assert np.array_equal(c["ElemEdgeMasterSlave"], MS)
+    assert np.array_equal(c["ElemEdgeOrientation"], OR)
+    return c
+
+
+def shipped_meshes():
+    return sorted(os.listdir(MESH_DIR)) if os.path.isdir(MESH_DIR) else []
+
+
+@pytest.mark.parametrize("N", [3, 7])
+@pytest.mark.parametrize("name", shipped_meshes())
+def test_ism_mesh_bit_exact(ref, name, N):
+    """Every mesh file the reference ships, read by the product and by the reference."""
+    c = check_ism(ref, os.path.join(MESH_DIR, name), N)
+    # the geometry is sound: positive Jacobians, unit normals, sum of scal w = side lengths > 0
+    assert (c["Jac"] > 0).all()
+    assert np.allclose(np.hypot(c["nx"], c["ny"]), 1.0, rtol=0, atol=1e-14)
+
+
+def with_bottom_column(src, dst):
+    """An ISM-V2 file with a third (bathymetry) column on node and curve lines:
+    b = 0.1 x - 0.05 y + 0.02 x y, evaluated from the file's own coordinates."""
+    lines = open(src).read().splitlines()
+    nn, ne, nel, order = (int(v) for v in lines[1].split())
+    bfun = lambda x, y: 0.1 * x - 0.05 * y + 0.02 * x * y      # noqa: E731
+    out = lines[:2]
+    i = 2
+    for _ in range(nn):
+        x, y = (float(v) for v in lines[i].split()[:2])
+        out.append(f"{lines[i]} {bfun(x, y)!r}")
+        i += 1
+    out += lines[i:i + ne]
+    i += ne
+    for _ in range(nel):
+        out.append(lines[i]); out.append(lines[i + 1])
+        flags = [int(v) for v in lines[i + 1].split()]
+        i += 2
+        for f in flags:
+            if f:
+                for _ in range(order + 1):
+                    x, y = (float(v) for v in lines[i].split()[:2])
+                    out.append(f"{lines[i]} {bfun(x, y)!r}")
+                    i += 1
+        out.append(lines[i])
+        i += 1
+    open(dst, "w").write("\n".join(out) + "\n")
+
+
+@pytest.mark.parametrize("name", ["PeriodicXandY3.mesh", "CircleInCircle.mesh"])
+def test_ism_mesh_bottom_column_bit_exact(ref, name, tmp_path):
+    """ReadBottom: b from the file's third column, mapped like the coordinates."""
+    dst = str(tmp_path / name)
+    with_bottom_column(os.path.join(FIXTURE_MESHES, name), dst)
+    c = check_ism(ref, dst, 4, read_bottom=1)
+    assert np.ptp(c["b"]) > 0.1
+
+
+def test_ism_fixture_meshes_match_the_shipped_files():
+    """The fixture copies under tests/golden/meshes are the reference's files, unchanged."""
+    if not shipped_meshes():
+        pytest.skip("reference mesh files absent")
+    for name in os.listdir(FIXTURE_MESHES):
+        assert open(os.path.join(FIXTURE_MESHES, name), "rb").read() == \
+            open(os.path.join(MESH_DIR, name), "rb").read()
