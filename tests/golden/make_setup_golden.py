@@ -2,8 +2,9 @@
 
 Run in a checkout where oracle/_ref/libesdg_ref_setup.so is built (`make -C oracle ref`, which
 needs the reference sources).  The fixture holds only data: the LGL basis operators for N = 1..10,
-the RK tables, and the 16x16 periodic Cartesian N = 4 mesh arrays with the Gaussian-bump
-bathymetry gradient -- every array as the reference computes it.
+the RK tables, the 16x16 periodic Cartesian N = 4 mesh arrays with the Gaussian-bump
+bathymetry gradient, and the N = 4 geometry of the two ISM-V2 fixture meshes (tests/golden/meshes)
+-- every array as the reference computes it.
 """
 import ctypes as C
 import os
@@ -80,6 +81,28 @@ def main():
     m = C.c_double()
     ref.ref_element_sizes(N, nel * nel, P(out["bump_Jac"]), P(sizes), C.byref(m))
     out["bump_EleSizes"] = sizes
+    # the reference's reader on the two fixture meshes (tests/golden/meshes), N = 4
+    for name in ("PeriodicXandY3", "CircleInCircle"):
+        mpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "meshes", name + ".mesh")
+        ne, nf = C.c_int(), C.c_int()
+        ref.ref_ism_mesh(mpath.encode(), 4, 0, C.byref(ne), C.byref(nf), *([None] * 15))
+        mn = ne.value * 25
+        a = {k: np.zeros(mn) for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "J", "b")}
+        f = {k: np.zeros(nf.value * 5) for k in ("nx", "ny", "scal")}
+        ei = np.zeros(nf.value * 7, np.int32)
+        e2e, ms, orr = (np.zeros(ne.value * 4, np.int32) for _ in range(3))
+        ref.ref_ism_mesh(mpath.encode(), 4, 0, C.byref(ne), C.byref(nf),
+                         *[P(a[k]) for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "J", "b")],
+                         *[P(f[k]) for k in ("nx", "ny", "scal")], P(ei), P(e2e), P(ms), P(orr))
+        for k in ("x", "y", "xXi", "xEta", "yXi", "yEta"):
+            out[f"ism_{name}_{k}"] = a[k]
+        out[f"ism_{name}_Jac"] = 1.0 / a["J"]
+        for k in ("nx", "ny", "scal"):
+            out[f"ism_{name}_{k}"] = f[k]
+        out[f"ism_{name}_EdgeInfo"] = ei
+        out[f"ism_{name}_ElemToEdge"] = e2e
+        out[f"ism_{name}_ElemEdgeMasterSlave"] = ms
+        out[f"ism_{name}_ElemEdgeOrientation"] = orr
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "setup_golden.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -161,6 +161,14 @@ def test_golden_setup_fixture():
         assert same_bits(g[f"bump_{k}"], c[k]), k
     for k in ("EdgeData", "ElemToEdge", "ElemEdgeMasterSlave", "ElemEdgeOrientation"):
         assert np.array_equal(g[f"bump_{k}"], c[k]), k
+    for name in ("PeriodicXandY3", "CircleInCircle"):
+        c = Case.from_ism(os.path.join(ROOT, "tests", "golden", "meshes", name + ".mesh"), 4)
+        for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "Jac", "nx", "ny", "scal"):
+            assert same_bits(g[f"ism_{name}_{k}"], c[k]), (name, k)
+        EI = g[f"ism_{name}_EdgeInfo"].reshape(-1, 7)
+        assert np.array_equal(c["EdgeData"].reshape(-1, 8)[:, :5], EI[:, 2:7]), name
+        for k in ("ElemToEdge", "ElemEdgeMasterSlave", "ElemEdgeOrientation"):
+            assert np.array_equal(g[f"ism_{name}_{k}"], c[k]), (name, k)
 
 
 # --------------------------------------------------------------------------------------------
