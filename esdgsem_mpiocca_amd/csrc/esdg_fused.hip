@@ -23,6 +23,7 @@
 // reference's order, and Qt receives its contributions in the reference's order (volume, surface,
 // viscous volume, viscous surface), so the fused stage reproduces the kernel-by-kernel results.
 #include <climits>
+#include <type_traits>
 
 #include "esdg_device.h"
 #include "esdg_fused.h"
@@ -39,8 +40,10 @@ int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr);
 // fused_face: CollectEdgeData (SolidWalls.okl:16-207) + calcNumFluxes (calcESFlux.okl /
 // calcESFluxRotated.okl) [+ calcNumFluxesGradient.okl:16-135], one thread per face node.
 // ---------------------------------------------------------------------------------------------
-template <bool AV>
-__global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kind, FluxConsts c, GatherConsts gc,
+// KIND: the numerical flux, a template argument so that each instance holds only its own flux
+// (fewer registers, more waves in flight).
+template <bool AV, int KIND>
+__global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxConsts c, GatherConsts gc,
                                                     const int* __restrict__ ED, const double* __restrict__ tr,
                                                     const double* __restrict__ nxa, const double* __restrict__ nya,
                                                     const double* __restrict__ scal, const double* __restrict__ BL,
@@ -61,7 +64,7 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, int kin
     const FaceState L = gathered_tr(tr, ngl, E, true, i, nx, ny, gc, BL[t], BR[t]);
     const FaceState R = gathered_tr(tr, ngl, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
     double F0, F1, F2;
-    numerical_flux(kind, c, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
+    numerical_flux(KIND, c, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
     const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
     SP[eid] = sc * c.geomFace * F0;
     SP[eid + ngl] = sc * c.geomFace * F1;
@@ -350,14 +353,26 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     }
     {
         KernelScope ks(ctx, "fused_face");
-        if (av)
-            k_fused_face<true><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, tr, A.nx, A.ny, A.scal,
-                                                            A.bL, A.bR, fstar, fgrad, fgrad + (size_t)A.Nfaces * ngl * 2,
-                                                            A.ViscPara, active, seq);
-        else
-            k_fused_face<false><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, kind, fc, gc, A.EdgeData, tr, A.nx, A.ny,
-                                                             A.scal, A.bL, A.bR, fstar, nullptr, nullptr, nullptr,
-                                                             nullptr, 0);
+        double* fSGY = av ? fgrad + (size_t)A.Nfaces * ngl * 2 : nullptr;
+        auto launch = [&](auto av_tag, auto kind_tag) {
+            constexpr bool AVK = decltype(av_tag)::value;
+            constexpr int KK = decltype(kind_tag)::value;
+            k_fused_face<AVK, KK><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, fc, gc, A.EdgeData, tr, A.nx, A.ny,
+                                                             A.scal, A.bL, A.bR, fstar, AVK ? fgrad : nullptr,
+                                                             fSGY, AVK ? A.ViscPara : nullptr,
+                                                             AVK ? active : nullptr, seq);
+        };
+        auto by_kind = [&](auto av_tag) {
+            switch (kind) {
+                case ESDG_FLUX_ES: launch(av_tag, std::integral_constant<int, ESDG_FLUX_ES>{}); break;
+                case ESDG_FLUX_LF: launch(av_tag, std::integral_constant<int, ESDG_FLUX_LF>{}); break;
+                case ESDG_FLUX_ES_ROTATED: launch(av_tag, std::integral_constant<int, ESDG_FLUX_ES_ROTATED>{}); break;
+                case ESDG_FLUX_ROE: launch(av_tag, std::integral_constant<int, ESDG_FLUX_ROE>{}); break;
+                default: launch(av_tag, std::integral_constant<int, ESDG_FLUX_OCEAN>{}); break;
+            }
+        };
+        if (av) by_kind(std::true_type{});
+        else by_kind(std::false_type{});
         int rc = launch_check(ctx, "fused_face");
         if (rc) return rc;
     }
