@@ -336,6 +336,13 @@ Geometry read_ism(const esdg_case_spec& s, const Basis& B) {
     std::ifstream in(s.mesh_file);
     if (!in) throw std::invalid_argument(std::string("mesh file not found: ") + s.mesh_file);
     const bool rb = s.read_bottom != 0;
+    // ConvertToKM (Mesh.cpp:770-776): longitude scaled, latitude by the Mercator projection
+    const double earth_radius = 6.378e3;
+    auto to_km = [&](double& x, double& y) {
+        if (!s.convert_to_km) return;
+        x = x * earth_radius * M_PI / 180.0;
+        y = earth_radius * std::asinh(std::tan(y / 180.0 * M_PI));
+    };
     std::string line;
     auto next = [&](const char* what) -> std::istringstream {
         if (!std::getline(in, line)) throw std::invalid_argument(std::string("ISM mesh: unexpected end of file in ") + what);
@@ -355,6 +362,7 @@ Geometry read_ism(const esdg_case_spec& s, const Basis& B) {
     for (int i = 0; i < nn; ++i) {
         auto ls = next("nodes");
         if (!(ls >> xn[i] >> yn[i]) || (rb && !(ls >> bn[i]))) throw std::invalid_argument("ISM mesh: bad node line");
+        to_km(xn[i], yn[i]);
     }
     Geometry G;
     G.ngl = ngl; G.Nelem = nel; G.Nfaces = ne;
@@ -421,6 +429,7 @@ Geometry read_ism(const esdg_case_spec& s, const Basis& B) {
                     auto ls = next("boundary curve");
                     if (!(ls >> Gx[k] >> Gy[k]) || (rb && !(ls >> Gb[k])))
                         throw std::invalid_argument("ISM mesh: bad boundary curve point");
+                    to_km(Gx[k], Gy[k]);
                 }
             }
         }

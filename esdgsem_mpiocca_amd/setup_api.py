@@ -52,6 +52,7 @@ class CaseSpec(C.Structure):
         ("perturb_amp", C.c_double), ("perturb_len", C.c_double), ("g", C.c_double),
         ("amp", C.c_double), ("p0", C.c_double), ("p1", C.c_double), ("p2", C.c_double),
         ("p3", C.c_double), ("seed", C.c_uint64), ("mesh_file", C.c_char_p), ("read_bottom", C.c_int),
+        ("convert_to_km", C.c_int),
     ]
 
 
@@ -99,11 +100,11 @@ class Case:
 
     def __init__(self, case_kind, mesh_kind, N, nelx, nely, xL, xR, yL, yR, periodic_x=0,
                  periodic_y=0, perturb_amp=0.0, perturb_len=1.0, g=9.81, amp=0.0, p=(0, 0, 0, 0),
-                 seed=0, mesh_file=None, read_bottom=0):
+                 seed=0, mesh_file=None, read_bottom=0, convert_to_km=0):
         self._mesh_file = None if mesh_file is None else os.fsencode(mesh_file)
         spec = CaseSpec(case_kind, mesh_kind, N, nelx, nely, xL, xR, yL, yR, periodic_x,
                         periodic_y, perturb_amp, perturb_len, g, amp, p[0], p[1], p[2], p[3], seed,
-                        self._mesh_file, int(read_bottom))
+                        self._mesh_file, int(read_bottom), int(convert_to_km))
         self.spec = spec
         h = C.c_void_p()
         _check(_lib().esdg_case_create(C.byref(spec), C.byref(h)))
@@ -133,12 +134,12 @@ class Case:
 
     @classmethod
     def from_ism(cls, path, N, case_kind=CASE_CONSTANT, p=(1.0, 0.0, 0.0, 0.0), amp=0.0, seed=0,
-                 read_bottom=0, bounds=(0.0, 1.0, 0.0, 1.0)):
+                 read_bottom=0, convert_to_km=0, bounds=(0.0, 1.0, 0.0, 1.0)):
         """An ISM-V2 mesh file (Mesh::ReadMesh) with a case's state; `bounds` only feeds the case
         formulas that use the domain extent."""
         xL, xR, yL, yR = bounds
         return cls(case_kind, MESH_ISM, N, 0, 0, xL, xR, yL, yR, amp=amp, p=p, seed=seed,
-                   mesh_file=path, read_bottom=read_bottom)
+                   mesh_file=path, read_bottom=read_bottom, convert_to_km=convert_to_km)
 
     # --- the benchmark configurations -------------------------------------------------
     @classmethod

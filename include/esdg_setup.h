@@ -82,6 +82,8 @@ typedef struct esdg_case_spec {
     const char* mesh_file;/* ESDG_MESH_ISM: path of the ISM-V2 file (nelx, nely, x/y bounds unused) */
     int read_bottom;      /* ESDG_MESH_ISM: b from the file's third node/curve column (ReadBottom)
                              instead of the case's bathymetry */
+    int convert_to_km;    /* ESDG_MESH_ISM: coordinates in degrees, mapped to km by the Mercator
+                             projection of ReadMesh (ConvertToKM, src/Mesh.cpp:770-776) */
 } esdg_case_spec;
 
 typedef struct esdg_case esdg_case;

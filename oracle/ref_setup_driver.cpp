@@ -82,13 +82,13 @@ int ref_cartesian_mesh(const char* workdir, int N, int nelx, int nely, double xL
 
 // ISM-V2 mesh through Mesh::InitMesh (file branch): Mesh.cpp:62-307, 698-1808.  Sizes first
 // (arrays null), then the arrays.
-int ref_ism_mesh(const char* path, int N, int read_bottom, int* Nelem_out, int* Nfaces_out, double* x,
+int ref_ism_mesh(const char* path, int N, int read_bottom, int to_km, int* Nelem_out, int* Nfaces_out, double* x,
                  double* y, double* xXi, double* xEta, double* yXi, double* yEta, double* J, double* b,
                  double* nx, double* ny, double* scal, int* EdgeInfo, int* ElemToEdge, int* MasterSlave,
                  int* Orientation) {
     basis B(N, 1);
     const int ngl = N + 1, ngl2 = ngl * ngl;
-    Mesh M(B.x_GL, ngl, 0, read_bottom, 0);
+    Mesh M(B.x_GL, ngl, 0, read_bottom, to_km);
     M.InitMesh(std::string(path), false, 0);
     const int Ne = M.m_num_elements, Nf = M.m_num_edges;
     *Nelem_out = Ne;

@@ -85,13 +85,13 @@ def main():
     for name in ("PeriodicXandY3", "CircleInCircle"):
         mpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "meshes", name + ".mesh")
         ne, nf = C.c_int(), C.c_int()
-        ref.ref_ism_mesh(mpath.encode(), 4, 0, C.byref(ne), C.byref(nf), *([None] * 15))
+        ref.ref_ism_mesh(mpath.encode(), 4, 0, 0, C.byref(ne), C.byref(nf), *([None] * 15))
         mn = ne.value * 25
         a = {k: np.zeros(mn) for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "J", "b")}
         f = {k: np.zeros(nf.value * 5) for k in ("nx", "ny", "scal")}
         ei = np.zeros(nf.value * 7, np.int32)
         e2e, ms, orr = (np.zeros(ne.value * 4, np.int32) for _ in range(3))
-        ref.ref_ism_mesh(mpath.encode(), 4, 0, C.byref(ne), C.byref(nf),
+        ref.ref_ism_mesh(mpath.encode(), 4, 0, 0, C.byref(ne), C.byref(nf),
                          *[P(a[k]) for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "J", "b")],
                          *[P(f[k]) for k in ("nx", "ny", "scal")], P(ei), P(e2e), P(ms), P(orr))
         for k in ("x", "y", "xXi", "xEta", "yXi", "yEta"):

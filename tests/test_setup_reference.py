@@ -178,25 +178,25 @@ MESH_DIR = "/root/reference/meshFiles"          # the reference's shipped mesh f
 FIXTURE_MESHES = os.path.join(ROOT, "tests", "golden", "meshes")
 
 
-def ref_ism(ref, path, N, read_bottom=0):
+def ref_ism(ref, path, N, read_bottom=0, to_km=0):
     ne, nf = C.c_int(), C.c_int()
     null = [None] * 15
-    assert ref.ref_ism_mesh(path.encode(), N, read_bottom, C.byref(ne), C.byref(nf), *null) == 0
+    assert ref.ref_ism_mesh(path.encode(), N, read_bottom, to_km, C.byref(ne), C.byref(nf), *null) == 0
     Ne, Nf, ngl = ne.value, nf.value, N + 1
     nn = Ne * ngl * ngl
     arr = {k: np.zeros(nn) for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "J", "b")}
     fa = {k: np.zeros(Nf * ngl) for k in ("nx", "ny", "scal")}
     EI = np.zeros(Nf * 7, np.int32)
     E2E, MS, OR = (np.zeros(Ne * 4, np.int32) for _ in range(3))
-    assert ref.ref_ism_mesh(path.encode(), N, read_bottom, C.byref(ne), C.byref(nf),
+    assert ref.ref_ism_mesh(path.encode(), N, read_bottom, to_km, C.byref(ne), C.byref(nf),
                             *[P(arr[k]) for k in ("x", "y", "xXi", "xEta", "yXi", "yEta", "J", "b")],
                             *[P(fa[k]) for k in ("nx", "ny", "scal")], P(EI), P(E2E), P(MS), P(OR)) == 0
     return arr, fa, EI.reshape(-1, 7), E2E, MS, OR
 
 
-def check_ism(ref, path, N, read_bottom=0):
-    arr, fa, EI, E2E, MS, OR = ref_ism(ref, path, N, read_bottom)
-    c = Case.from_ism(path, N, read_bottom=read_bottom)
+def check_ism(ref, path, N, read_bottom=0, to_km=0):
+    arr, fa, EI, E2E, MS, OR = ref_ism(ref, path, N, read_bottom, to_km)
+    c = Case.from_ism(path, N, read_bottom=read_bottom, convert_to_km=to_km)
     assert c.Nelem * c.ngl ** 2 == len(arr["x"]) and c.Nfaces == len(EI)
     for k in ("x", "y", "xXi", "xEta", "yXi", "yEta"):
         assert same_bits(arr[k], c[k]), k
@@ -273,3 +273,9 @@ def test_ism_fixture_meshes_match_the_shipped_files():
     for name in os.listdir(FIXTURE_MESHES):
         assert open(os.path.join(FIXTURE_MESHES, name), "rb").read() == \
             open(os.path.join(MESH_DIR, name), "rb").read()
+
+
+def test_ism_mesh_degrees_to_km_bit_exact(ref):
+    """ConvertToKM: the fixture mesh read as degrees (it spans +-1 degree) and projected to km."""
+    c = check_ism(ref, os.path.join(FIXTURE_MESHES, "PeriodicXandY3.mesh"), 4, to_km=1)
+    assert 200.0 < np.ptp(c["x"]) < 250.0        # 2 degrees of longitude at the equator: 222.6 km
