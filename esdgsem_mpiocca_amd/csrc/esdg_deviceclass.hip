@@ -104,7 +104,7 @@ class deviceclass {
         if (rp_.rk_order == 4 && rp_.rk_ssp) return set_error(ctx_, ESDG_ERR_ARG, "rk_order 4 has no SSP variant (rkSSP must be 0)");
         if (rp_.artificial_viscosity && ngl < 3) return set_error(ctx_, ESDG_ERR_ARG, "artificial viscosity needs N >= 2");
         if (rp_.fused < 0 || rp_.fused > 2) return set_error(ctx_, ESDG_ERR_ARG, "fused must be 0, 1 or 2");
-        if (rp_.fused && (ctx_->p.bc_kind == ESDG_BC_CONVERGENCE || ctx_->p.bc_kind == ESDG_BC_THREE_MOUND))
+        if (rp_.fused && (is_manufactured(ctx_->p.bc_kind) || ctx_->p.bc_kind == ESDG_BC_THREE_MOUND))
             return set_error(ctx_, ESDG_ERR_ARG, "time-dependent boundaries run on the reference kernel sequence");
         if (rp_.fused && (rp_.rk_ssp || !rp_.flux_differencing))
             return set_error(ctx_, ESDG_ERR_ARG, "the fused stage supports flux differencing with low-storage RK only");
@@ -224,8 +224,8 @@ class deviceclass {
                                                    o_ElemToEdge, o_SurfacePartsVisc, o_Qt);
             if (rc) return rc;
         }
-        // manufactured source of the convergence test (src/deviceclass.cpp:1098-1102)
-        if (ctx_->p.bc_kind == ESDG_BC_CONVERGENCE) {
+        // manufactured source of the convergence tests (src/deviceclass.cpp:1098-1102)
+        if (is_manufactured(ctx_->p.bc_kind)) {
             rc = esdg_add_source(ctx_, Nelem, o_Bx, o_By, o_B, o_x, o_y, t + (rp_.rk_ssp ? rkD[s] : Bc[s]) * dt, o_Qt);
             if (rc) return rc;
         }

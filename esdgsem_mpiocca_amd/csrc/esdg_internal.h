@@ -53,6 +53,22 @@ int launch_check(esdg_ctx* ctx, const char* name);
 // Element / face counts are int32 (indices fit); byte offsets are 64-bit.
 inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
+// The manufactured solutions h = H0 + cos(kx) sin(ky) cos t - b, u = 1/2, v = 3/2 selected by the
+// boundary kinds of the convergence tests (src/deviceclass.cpp:363-387): Dirichlet_ConvTest.okl and
+// S_ConvTest.okl (H0 = 8, k = 1), ...ConvTest3 (20, 1), ...ConvTest4 (20, 1e-3), ...ConvTest5 (20000, 1e-6).
+__host__ __device__ inline bool is_manufactured(int bc) {
+    return bc == ESDG_BC_CONVERGENCE || (bc >= ESDG_BC_CONV_TEST3 && bc <= ESDG_BC_CONV_TEST5);
+}
+struct Manufactured { double H0, k; };
+__host__ __device__ inline Manufactured manufactured_of(int bc) {
+    switch (bc) {
+        case ESDG_BC_CONV_TEST3: return {20.0, 1.0};
+        case ESDG_BC_CONV_TEST4: return {20.0, 0.001};
+        case ESDG_BC_CONV_TEST5: return {20000.0, 0.000001};
+        default: return {8.0, 1.0};
+    }
+}
+
 }  // namespace esdg
 
 #define ESDG_REQUIRE(ctx, cond, msg)                                        \

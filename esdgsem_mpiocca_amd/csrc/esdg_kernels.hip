@@ -39,12 +39,13 @@ __global__ void __launch_bounds__(256) k_collect_bottom(int Nfaces, int ngl, con
 // CollectEdgeData: okl/GatherEdgeData/SolidWalls.okl:16-207 (walls), InnerOuter.okl,
 // OceanBoundary.okl (exterior state variants)
 // ---------------------------------------------------------------------------------------------
-// exterior state of the convergence test: Qnodal of okl/GatherEdgeData/Dirichlet_ConvTest.okl:17-31
-__device__ __forceinline__ void manufactured_state(double x, double y, double t, double b, double& q1, double& q2,
-                                                   double& q3) {
-    q1 = (8.0 + cos(x) * sin(y) * cos(t) - b);
-    q2 = 0.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
-    q3 = 1.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
+// exterior state of the convergence tests: Qnodal of okl/GatherEdgeData/Dirichlet_ConvTest.okl:17-31
+// (H0 = 8, k = 1; Dirichlet_ConvTest3/4/5.okl:17-30 with their H0, k).  k = 1 multiplies exactly.
+__device__ __forceinline__ void manufactured_state(Manufactured m, double x, double y, double t, double b, double& q1,
+                                                   double& q2, double& q3) {
+    q1 = (m.H0 + cos(m.k * x) * sin(m.k * y) * cos(t) - b);
+    q2 = 0.5 * (m.H0 + cos(m.k * x) * sin(m.k * y) * cos(t) - b);
+    q3 = 1.5 * (m.H0 + cos(m.k * x) * sin(m.k * y) * cos(t) - b);
 }
 
 // exterior state of 3MoundInflow.okl:110-240 for the interior state (h, hu, hv) at node x
@@ -99,15 +100,16 @@ __global__ void __launch_bounds__(256) k_collect(int Nfaces, int ngl, int bc, do
     if (bc == ESDG_BC_WALLS) {
         if (cpuL == -1) { L0 = R0; wall_mirror(nx, ny, R1, R2, L1, L2); }
         if (cpuR == -1) { R0 = L0; wall_mirror(nx, ny, L1, L2, R1, R2); }
-    } else if (bc == ESDG_BC_CONVERGENCE) {
+    } else if (is_manufactured(bc)) {
         // Dirichlet_ConvTest.okl:150-165: the manufactured solution at the interior node's position
+        const Manufactured m = manufactured_of(bc);
         if (cpuL == -1) {
             const int64_t xid = (int64_t)ier * ngl2 + side_node(ngl, isr, i);
-            manufactured_state(xa[xid], ya[xid], time, bRa[t], L0, L1, L2);
+            manufactured_state(m, xa[xid], ya[xid], time, bRa[t], L0, L1, L2);
         }
         if (cpuR == -1) {
             const int64_t xid = (int64_t)iel * ngl2 + side_node(ngl, isl, i);
-            manufactured_state(xa[xid], ya[xid], time, bLa[t], R0, R1, R2);
+            manufactured_state(m, xa[xid], ya[xid], time, bLa[t], R0, R1, R2);
         }
     } else if (bc == ESDG_BC_THREE_MOUND) {
         if (cpuL == -1) {
@@ -172,21 +174,22 @@ __global__ void __launch_bounds__(256) k_num_fluxes(int Nfaces, int ngl, int kin
     SP[eid + 2 * ngl] = sc * c.geomFace * F2;
 }
 
-// Manufactured source of the convergence test, okl/ManufacturedSolutions/S_ConvTest.okl:17-95
-// (h = 8 + cos x sin y cos t - b, u = 1/2, v = 3/2), added to Qt at node (e, n).
-__global__ void __launch_bounds__(256) k_add_source(int Nelem, int ngl, double g, const double* __restrict__ Bx,
-                                                    const double* __restrict__ By, const double* __restrict__ B,
-                                                    const double* __restrict__ xa, const double* __restrict__ ya,
-                                                    double t, double* __restrict__ Qt) {
+// Manufactured source of the convergence tests, okl/ManufacturedSolutions/S_ConvTest.okl:17-95
+// (h = H0 + cos(kx) sin(ky) cos t - b, u = 1/2, v = 3/2; S_ConvTest3/4/5.okl:60-95 differ only in
+// H0 and k), added to Qt at node (e, n).
+__global__ void __launch_bounds__(256) k_add_source(int Nelem, int ngl, double g, Manufactured m,
+                                                    const double* __restrict__ Bx, const double* __restrict__ By,
+                                                    const double* __restrict__ B, const double* __restrict__ xa,
+                                                    const double* __restrict__ ya, double t, double* __restrict__ Qt) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int ngl2 = ngl * ngl;
     if (k >= (int64_t)Nelem * ngl2) return;
     const int64_t e = k / ngl2, n = k % ngl2;
-    const double sinY = sin(ya[k]), sinX = sin(xa[k]), cosY = cos(ya[k]), cosX = cos(xa[k]);
-    const double H = 8.0 + cosX * sinY * cos(t);
+    const double sinY = sin(m.k * ya[k]), sinX = sin(m.k * xa[k]), cosY = cos(m.k * ya[k]), cosX = cos(m.k * xa[k]);
+    const double H = m.H0 + cosX * sinY * cos(t);
     const double H_t = -cosX * sinY * sin(t);
-    const double H_x = -sinX * sinY * cos(t);
-    const double H_y = cosX * cosY * cos(t);
+    const double H_x = -m.k * sinX * sinY * cos(t);
+    const double H_y = m.k * cosX * cosY * cos(t);
     const double u = 0.5, v = 1.5;
     const double bx = Bx[k], by = By[k], b = B[k];
     double* o = Qt + e * ngl2 * ESDG_NEQ + n;
@@ -829,11 +832,12 @@ int esdg_collect_edge_data(esdg_ctx* ctx, int Nfaces, const int* EdgeData, const
                            double* qL, double* qR, double t) {
     ESDG_REQUIRE(ctx, ctx, "null context");
     ESDG_REQUIRE(ctx, Nfaces >= 0 && EdgeData && q && nx && ny && qL && qR, "esdg_collect_edge_data: bad argument");
-    ESDG_REQUIRE(ctx, (ctx->p.bc_kind != ESDG_BC_OCEAN && ctx->p.bc_kind != ESDG_BC_CONVERGENCE) || (bL && bR),
+    ESDG_REQUIRE(ctx, (ctx->p.bc_kind != ESDG_BC_OCEAN && !is_manufactured(ctx->p.bc_kind)) || (bL && bR),
                  "esdg_collect_edge_data: this boundary needs bL, bR");
-    ESDG_REQUIRE(ctx, (ctx->p.bc_kind != ESDG_BC_CONVERGENCE && ctx->p.bc_kind != ESDG_BC_THREE_MOUND) || (x && y),
+    ESDG_REQUIRE(ctx, (!is_manufactured(ctx->p.bc_kind) && ctx->p.bc_kind != ESDG_BC_THREE_MOUND) || (x && y),
                  "esdg_collect_edge_data: this boundary needs x, y");
-    ESDG_REQUIRE(ctx, ctx->p.bc_kind >= 0 && ctx->p.bc_kind <= 4, "esdg_collect_edge_data: unknown boundary kind");
+    ESDG_REQUIRE(ctx, ctx->p.bc_kind >= ESDG_BC_WALLS && ctx->p.bc_kind <= ESDG_BC_CONV_TEST5,
+                 "esdg_collect_edge_data: unknown boundary kind");
     if (Nfaces == 0) return ESDG_OK;
     ESDG_ENTRY("collect_edge_data");
     const int ngl = ctx->p.ngl;
@@ -850,8 +854,8 @@ int esdg_add_source(esdg_ctx* ctx, int Nelem, const double* Bx, const double* By
     if (Nelem == 0) return ESDG_OK;
     ESDG_ENTRY("add_source");
     const int ngl = ctx->p.ngl;
-    k_add_source<<<blocks_for((int64_t)Nelem * ngl * ngl, 256), 256, 0, ctx->stream>>>(Nelem, ngl, ctx->p.g, Bx, By, B, x,
-                                                                                      y, t, Qt);
+    k_add_source<<<blocks_for((int64_t)Nelem * ngl * ngl, 256), 256, 0, ctx->stream>>>(
+        Nelem, ngl, ctx->p.g, manufactured_of(ctx->p.bc_kind), Bx, By, B, x, y, t, Qt);
     return launch_check(ctx, "add_source");
 }
 

@@ -650,7 +650,10 @@ namespace {
 //                  b = 4000 - depth, h = depth + eta, eta = amp*exp(-(xr/a)^2-(yr/c)^2) with
 //                  rotation p2 (radians), axes a = p0, c = p1 (metres), centre at 40% / 50%.
 //   CONSTANT:      h = p0, hu = p1, hv = p2, b = 0.
-//   CONVERGENCE:   b = 2 + 0.5 sin(2 pi y) + cos(2 pi x); h = 8 + cos x sin y - b; hu = h/2, hv = 3h/2.
+//   CONVERGENCE:   p0 = the reference's Testcase number.  1 (or 0): b = 2 + 0.5 sin(2 pi y) + cos(2 pi x),
+//                  h = 8 + cos x sin y - b; 8: b = 2 + sin y + cos x, h = 20 + cos x sin y - b;
+//                  88: b = 0, h = 20 + cos(x/1000) sin(y/1000) - b; 89: b = 0,
+//                  h = 20000 + cos(1e-6 x) sin(1e-6 y) - b.  hu = h/2, hv = 3h/2 (t = 0).
 void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<double>& y,
                 const std::vector<double>* bmesh) {
     const esdg_case_spec& s = C.spec;
@@ -725,9 +728,24 @@ void make_state(esdg_case& C, const std::vector<double>& x, const std::vector<do
                     break;
                 }
                 case ESDG_CASE_CONVERGENCE: {
-                    // SW2D::InitQNodal case 1 with SW2D bathymetry case 1, at t = 0
-                    bb = pick(2.0 + 0.5 * std::sin(2.0 * M_PI * Y) + std::cos(2.0 * M_PI * X));
-                    h = 8.0 + std::cos(X) * std::sin(Y) * std::cos(0.0) - bb;
+                    // SW2D::InitQNodal cases 1, 8, 88, 89 (src/SW2D.cpp:312-318, 388-393, 760-775) with
+                    // the bathymetry of the same Testcase (:829-833, 858-862, default b = 0), at t = 0
+                    const int tc = (int)s.p0;
+                    if (tc == 8) {
+                        bb = pick(2.0 + std::sin(Y) + std::cos(X));
+                        h = 20.0 + std::cos(X) * std::sin(Y) * std::cos(0.0) - bb;
+                    } else if (tc == 88) {
+                        bb = pick(0.0);
+                        h = 20.0 + std::cos(0.001 * X) * std::sin(0.001 * Y) * std::cos(0.0) - bb;
+                    } else if (tc == 89) {
+                        bb = pick(0.0);
+                        h = 20000.0 + std::cos(0.000001 * X) * std::sin(0.000001 * Y) * std::cos(0.0) - bb;
+                    } else if (tc == 0 || tc == 1) {
+                        bb = pick(2.0 + 0.5 * std::sin(2.0 * M_PI * Y) + std::cos(2.0 * M_PI * X));
+                        h = 8.0 + std::cos(X) * std::sin(Y) * std::cos(0.0) - bb;
+                    } else {
+                        throw std::invalid_argument("convergence case: p0 must be Testcase 1, 8, 88 or 89");
+                    }
                     hu = h * 0.5;
                     hv = h * 1.5;
                     break;

@@ -64,6 +64,10 @@ class CaseInfo(C.Structure):
 CASE_GAUSSIAN_BUMP, CASE_DAM_BREAK, CASE_FLOOD, CASE_OCEAN_UPLIFT, CASE_CONSTANT, CASE_CONVERGENCE = range(6)
 MESH_CARTESIAN, MESH_CURVED_SINE, MESH_ISM = 0, 1, 2
 
+# boundary kind (Dirichlet exterior state + manufactured source) of each convergence Testcase
+# (src/deviceclass.cpp:363-387): Dirichlet_ConvTest.okl, ...ConvTest3, ...ConvTest4, ...ConvTest5
+CONVERGENCE_BC = {1: 3, 8: 5, 88: 6, 89: 7}
+
 DOUBLE_ARRAYS = ("x", "y", "xXi", "xEta", "yXi", "yEta", "Jac", "b", "Bx", "By", "q", "nx", "ny",
                  "scal", "EleSizes", "x_GL", "w_GL", "D", "D0", "Dstrong", "Dhat", "VdmInv")
 INT_ARRAYS = ("EdgeData", "EdgeReversed", "ElemToEdge", "ElemEdgeMasterSlave",
@@ -89,7 +93,8 @@ class RunParams:
     rk_order: int = 4
     rk_ssp: int = 0
     zero_tol: float = 1.0e-5    # hard-coded in the reference (src/deviceclass.cpp:148)
-    bc_kind: int = 0            # 0 SolidWalls, 1 InnerOuter, 2 OceanBoundary (deviceclass.cpp:363-415)
+    bc_kind: int = 0            # 0 SolidWalls, 1 InnerOuter, 2 OceanBoundary, 3 and 5-7 manufactured
+                                # solutions, 4 3MoundInflow (deviceclass.cpp:363-415)
     h_zero: float = 0.0         # still-water level of the ocean boundary (define h_zero)
     arrival_times: int = 0      # calcArrivalTimes (deviceclass.cpp:1132-1135)
     maximum_elevation: int = 0  # calcMaximumElevation (deviceclass.cpp:1138-1141)
@@ -184,12 +189,19 @@ class Case:
                    amp=amp, p=(axis_a, axis_c, angle, 0))
 
     @classmethod
-    def convergence(cls, N=3, nel=8, curved=False, perturb=0.02):
-        """Manufactured solution of the convergence test on nel^2 quads of [0,1]^2 (optionally the
-        curved mesh family); run it with RunParams(bc_kind=3): Dirichlet exterior states and the
-        manufactured source."""
-        return cls(CASE_CONVERGENCE, MESH_CURVED_SINE if curved else MESH_CARTESIAN, N, nel, nel, 0.0, 1.0,
-                   0.0, 1.0, 0, 0, perturb_amp=perturb if curved else 0.0, perturb_len=0.5)
+    def convergence(cls, N=3, nel=8, curved=False, perturb=0.02, testcase=1, length=None):
+        """Manufactured solution of convergence Testcase 1, 8, 88 or 89 on nel^2 quads of
+        [0, length]^2 (optionally the curved mesh family).  The default length puts the solution's
+        wave number k times length at 1 (1, 1, 1e3, 1e6).  Run it with
+        RunParams(bc_kind=CONVERGENCE_BC[testcase]): Dirichlet exterior states and the manufactured
+        source."""
+        if testcase not in CONVERGENCE_BC:
+            raise ValueError("convergence Testcase must be 1, 8, 88 or 89")
+        if length is None:
+            length = {1: 1.0, 8: 1.0, 88: 1.0e3, 89: 1.0e6}[testcase]
+        return cls(CASE_CONVERGENCE, MESH_CURVED_SINE if curved else MESH_CARTESIAN, N, nel, nel, 0.0, length,
+                   0.0, length, 0, 0, perturb_amp=perturb * length if curved else 0.0, perturb_len=0.5 * length,
+                   p=(testcase, 0, 0, 0))
 
     @classmethod
     def constant(cls, N, nel, h=1.0, hu=0.0, hv=0.0, curved=True, perturb=0.05):

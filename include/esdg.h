@@ -52,9 +52,16 @@ enum esdg_bc {
     ESDG_BC_CONVERGENCE = 3,   /* okl/GatherEdgeData/Dirichlet_ConvTest.okl: the exterior state is the
                                   manufactured solution at the stage time; the time loop adds its
                                   source term okl/ManufacturedSolutions/S_ConvTest.okl (Testcase 1) */
-    ESDG_BC_THREE_MOUND = 4    /* okl/GatherEdgeData/3MoundInflow.okl: walls on faces with n_x = 0,
+    ESDG_BC_THREE_MOUND = 4,   /* okl/GatherEdgeData/3MoundInflow.okl: walls on faces with n_x = 0,
                                   inflow (1/2, 1/2, 0), from t > 300 (1, 1, 0), at x = 0, outflow
                                   elsewhere (Testcase 32) */
+    /* The other manufactured solutions h = H0 + cos(kx) sin(ky) cos t - b, hu = h/2, hv = 3h/2
+     * (src/deviceclass.cpp:363-387): Dirichlet exterior state and source term as kind 3. */
+    ESDG_BC_CONV_TEST3 = 5,    /* Dirichlet_ConvTest3.okl + S_ConvTest3.okl: H0 = 20, k = 1 (Testcase 8) */
+    ESDG_BC_CONV_TEST4 = 6,    /* Dirichlet_ConvTest4.okl + S_ConvTest4.okl: H0 = 20, k = 1e-3 (Testcase 88,
+                                  coordinates in km) */
+    ESDG_BC_CONV_TEST5 = 7     /* Dirichlet_ConvTest5.okl + S_ConvTest5.okl: H0 = 20000, k = 1e-6
+                                  (Testcase 89, coordinates in m) */
 };
 
 typedef struct esdg_params {
@@ -169,8 +176,9 @@ int esdg_surface_kernel_visc(esdg_ctx* ctx, int Nelem, const double* Jac,
                              const int* ElemEdgeMasterSlave, const int* ElemEdgeOrientation,
                              const int* ElemToEdge, const double* SurfacePartsVisc, double* Qt);
 
-/* addS (src/deviceclass.cpp:1100-1102, okl/ManufacturedSolutions/S_ConvTest.okl): Qt += source of
- * the manufactured solution h = 8 + cos x sin y cos t - b, u = 1/2, v = 3/2 at time t. */
+/* addS (src/deviceclass.cpp:1100-1102, okl/ManufacturedSolutions/S_ConvTest{,3,4,5}.okl): Qt += source
+ * of the manufactured solution h = H0 + cos(kx) sin(ky) cos t - b, u = 1/2, v = 3/2 at time t, with
+ * H0, k of the context's bc_kind (ESDG_BC_CONV_TEST3..5; H0 = 8, k = 1 for every other kind). */
 int esdg_add_source(esdg_ctx* ctx, int Nelem, const double* Bx, const double* By, const double* B,
                     const double* x, const double* y, double t, double* Qt);
 

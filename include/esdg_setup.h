@@ -60,9 +60,11 @@ enum esdg_case_kind {
     ESDG_CASE_FLOOD = 2,          /* 16-mode random-phase Fourier bathymetry, reservoir over dry land */
     ESDG_CASE_OCEAN_UPLIFT = 3,   /* 4000 m ocean with linear shelf to 50 m, elliptic Gaussian uplift */
     ESDG_CASE_CONSTANT = 4,       /* constant state (h0, hu0, hv0) over flat b=0: free-stream test */
-    ESDG_CASE_CONVERGENCE = 5     /* manufactured solution of Testcase 1 (SW2D.cpp:312-318, 829-833):
-                                     b = 2 + sin(2 pi y)/2 + cos(2 pi x), h = 8 + cos x sin y - b,
-                                     hu = h/2, hv = 3h/2 at t = 0 */
+    ESDG_CASE_CONVERGENCE = 5     /* manufactured solution of Testcase p0 (SW2D.cpp:312-318, 829-833 for
+                                     p0 = 1 or 0): b = 2 + sin(2 pi y)/2 + cos(2 pi x), h = 8 + cos x sin y - b;
+                                     p0 = 8: b = 2 + sin y + cos x, h = 20 + cos x sin y - b;
+                                     p0 = 88 / 89: b = 0, h = 20 + cos(x/1e3) sin(y/1e3) - b /
+                                     20000 + cos(x/1e6) sin(y/1e6) - b; hu = h/2, hv = 3h/2 at t = 0 */
 };
 
 typedef struct esdg_case_spec {

@@ -159,11 +159,26 @@ void oracle_collect_edge_data(const oracle_params* p, int Nfaces, const int* Edg
     }
 }
 
-/* Qnodal of okl/GatherEdgeData/Dirichlet_ConvTest.okl:17-31 */
-static void manufactured_state(double x, double y, double t, double b, double* q) {
-    q[0] = (8.0 + cos(x) * sin(y) * cos(t) - b);
-    q[1] = 0.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
-    q[2] = 1.5 * (8.0 + cos(x) * sin(y) * cos(t) - b);
+/* Mean depth H0 and wave number k of the manufactured solution h = H0 + cos(kx) sin(ky) cos t - b
+ * chosen by the boundary kind (src/deviceclass.cpp:363-387): 3 Dirichlet_ConvTest.okl /
+ * S_ConvTest.okl (Testcase 1), 5 ...ConvTest3 (Testcase 8), 6 ...ConvTest4 (88), 7 ...ConvTest5 (89). */
+static int is_manufactured(int bc) { return bc == 3 || (bc >= 5 && bc <= 7); }
+static void manufactured_constants(int bc, double* H0, double* k) {
+    switch (bc) {
+        case 5: *H0 = 20.0; *k = 1.0; break;        /* Dirichlet_ConvTest3.okl:25-27 */
+        case 6: *H0 = 20.0; *k = 0.001; break;      /* Dirichlet_ConvTest4.okl:26-28 */
+        case 7: *H0 = 20000.0; *k = 0.000001; break;/* Dirichlet_ConvTest5.okl:26-28 */
+        default: *H0 = 8.0; *k = 1.0; break;        /* Dirichlet_ConvTest.okl:26-28 */
+    }
+}
+
+/* Qnodal of okl/GatherEdgeData/Dirichlet_ConvTest.okl:17-31 (and its variants) */
+static void manufactured_state(int bc, double x, double y, double t, double b, double* q) {
+    double H0, k;
+    manufactured_constants(bc, &H0, &k);
+    q[0] = (H0 + cos(k * x) * sin(k * y) * cos(t) - b);
+    q[1] = 0.5 * (H0 + cos(k * x) * sin(k * y) * cos(t) - b);
+    q[2] = 1.5 * (H0 + cos(k * x) * sin(k * y) * cos(t) - b);
 }
 
 /* exterior state of okl/GatherEdgeData/3MoundInflow.okl:110-240: velocity mirror on faces with
@@ -191,10 +206,10 @@ static void three_mound_state(const oracle_params* p, double nx, double ny, doub
     }
 }
 
-/* The time-dependent boundary variants (bc_kind 3, 4).  Interior traces as SolidWalls.okl; the
+/* The time-dependent boundary variants (bc_kind 3-7).  Interior traces as SolidWalls.okl; the
  * exterior of a boundary face is
- *   3  okl/GatherEdgeData/Dirichlet_ConvTest.okl:33-190: the manufactured solution at the
- *      interior node's position and time t, with the interior side's bottom trace;
+ *   3, 5-7  okl/GatherEdgeData/Dirichlet_ConvTest{,3,4,5}.okl:33-190: the manufactured solution at
+ *      the interior node's position and time t, with the interior side's bottom trace;
  *   4  okl/GatherEdgeData/3MoundInflow.okl: three_mound_state of the interior trace. */
 void oracle_collect_edge_data_xt(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
                                  const double* x, const double* y, const double* nx, const double* ny,
@@ -219,12 +234,12 @@ void oracle_collect_edge_data_xt(const oracle_params* p, int Nfaces, const int* 
             if (cpuL == -1) {
                 const long nd = (long)ier * ngl2 + side_node(ngl, isr, i);
                 if (p->bc_kind == 4) three_mound_state(p, nx[xid], ny[xid], x[nd], t, R, L);
-                else manufactured_state(x[nd], y[nd], t, bR[xid], L);
+                else manufactured_state(p->bc_kind, x[nd], y[nd], t, bR[xid], L);
             }
             if (cpuR == -1) {
                 const long nd = (long)iel * ngl2 + side_node(ngl, isl, i);
                 if (p->bc_kind == 4) three_mound_state(p, nx[xid], ny[xid], x[nd], t, L, R);
-                else manufactured_state(x[nd], y[nd], t, bL[xid], R);
+                else manufactured_state(p->bc_kind, x[nd], y[nd], t, bL[xid], R);
             }
             for (int v = 0; v < 3; ++v) {
                 qL[eid + v * ngl] = L[v];
@@ -234,20 +249,23 @@ void oracle_collect_edge_data_xt(const oracle_params* p, int Nfaces, const int* 
     }
 }
 
-/* okl/ManufacturedSolutions/S_ConvTest.okl:17-95 (h = 8 + cos x sin y cos t - b, u = 1/2, v = 3/2) */
+/* okl/ManufacturedSolutions/S_ConvTest.okl:17-95 (h = 8 + cos x sin y cos t - b, u = 1/2, v = 3/2);
+ * S_ConvTest3/4/5.okl:60-95 with their H0 and k: H_x and H_y carry the chain-rule factor k */
 void oracle_add_source(const oracle_params* p, int Nelem, const double* Bx, const double* By, const double* B,
                        const double* x, const double* y, double t, double* Qt) {
     const int ngl2 = p->ngl * p->ngl;
     const double g = p->g;
+    double H0, kw;
+    manufactured_constants(p->bc_kind, &H0, &kw);
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < Nelem; ++e)
         for (int n = 0; n < ngl2; ++n) {
             const long k = (long)e * ngl2 + n;
-            const double sinY = sin(y[k]), sinX = sin(x[k]), cosY = cos(y[k]), cosX = cos(x[k]);
-            const double H = 8.0 + cosX * sinY * cos(t);
+            const double sinY = sin(kw * y[k]), sinX = sin(kw * x[k]), cosY = cos(kw * y[k]), cosX = cos(kw * x[k]);
+            const double H = H0 + cosX * sinY * cos(t);
             const double H_t = -cosX * sinY * sin(t);
-            const double H_x = -sinX * sinY * cos(t);
-            const double H_y = cosX * cosY * cos(t);
+            const double H_x = -kw * sinX * sinY * cos(t);
+            const double H_y = kw * cosX * cosY * cos(t);
             const double u = 0.5, v = 1.5;
             double* o = Qt + (long)e * ngl2 * NEQ + n;
             o[0] += H_t + u * (H_x - Bx[k]) + v * (H_y - By[k]);
@@ -1086,7 +1104,7 @@ int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, doub
         for (int s = 0; s < stages; ++s) {
             const double rkD[3] = {0.0, 1.0, 0.5};
             const double t_stage = t + (r->rk_ssp ? rkD[s] : Bc[s]) * dt;
-            if (p->bc_kind == 3 || p->bc_kind == 4)
+            if (is_manufactured(p->bc_kind) || p->bc_kind == 4)
                 oracle_collect_edge_data_xt(p, Nf, r->EdgeData, q, r->x, r->y, r->nx, r->ny, bL, bR, t_stage, qL, qR);
             else
                 oracle_collect_edge_data(p, Nf, r->EdgeData, q, r->nx, r->ny, bL, bR, qL, qR);
@@ -1113,7 +1131,7 @@ int oracle_timeloop(const oracle_params* p, const oracle_run* r, double* q, doub
                 oracle_num_fluxes_viscose(p, Nf, r->EdgeData, r->nx, r->ny, r->scal, VL, VR, gXL, gXR, gYL, gYR, SPV);
                 oracle_surface_visc(p, Ne, r->Jac, r->EMS, r->EOR, r->ElemToEdge, SPV, Qt);
             }
-            if (p->bc_kind == 3) oracle_add_source(p, Ne, r->Bx, r->By, r->B, r->x, r->y, t_stage, Qt);
+            if (is_manufactured(p->bc_kind)) oracle_add_source(p, Ne, r->Bx, r->By, r->B, r->x, r->y, t_stage, Qt);
             if (r->rk_ssp) {
                 oracle_rk_ssp_update(p, Ne, A[s], Bc[s], C[s], dt, Qt, Qtmp, q);
             } else {

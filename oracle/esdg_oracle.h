@@ -22,7 +22,8 @@ typedef struct oracle_params {
     double PosPresTOL;
     double ZeroTOL;     /* hard-coded 1e-5 in the reference (src/deviceclass.cpp:148) */
     double eps0, sigmaMin, sigmaMax;
-    int bc_kind;        /* exterior state of boundary faces: 0 walls, 1 inner/outer, 2 ocean */
+    int bc_kind;        /* exterior state of boundary faces: 0 walls, 1 inner/outer, 2 ocean,
+                           3 manufactured (ConvTest), 4 three-mound inflow, 5-7 ConvTest3/4/5 */
     double h_zero;      /* still-water level of the ocean boundary (define h_zero) */
 } oracle_params;
 
@@ -36,12 +37,12 @@ void oracle_collect_edge_data_walls(const oracle_params* p, int Nfaces, const in
 void oracle_collect_edge_data(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
                               const double* nx, const double* ny, const double* bL, const double* bR,
                               double* qL, double* qR);
-/* time-dependent boundaries: 3 Dirichlet_ConvTest.okl (manufactured solution at time t),
+/* time-dependent boundaries: 3, 5-7 Dirichlet_ConvTest{,3,4,5}.okl (manufactured solution at time t),
  * 4 3MoundInflow.okl (walls / inflow at x = 0 / outflow) */
 void oracle_collect_edge_data_xt(const oracle_params* p, int Nfaces, const int* EdgeData, const double* q,
                                  const double* x, const double* y, const double* nx, const double* ny,
                                  const double* bL, const double* bR, double t, double* qL, double* qR);
-/* S_ConvTest.okl addS: Qt += manufactured source at time t */
+/* S_ConvTest{,3,4,5}.okl addS: Qt += manufactured source at time t (variant by p->bc_kind) */
 void oracle_add_source(const oracle_params* p, int Nelem, const double* Bx, const double* By, const double* B,
                        const double* x, const double* y, double t, double* Qt);
 void oracle_volume_fluxdiff(const oracle_params* p, int Nelem, const double* Jac, const double* Yxi,

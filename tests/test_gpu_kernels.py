@@ -91,15 +91,17 @@ def test_collect_edge_data_boundary_variants(dev, bc):
         s.close()
 
 
-@pytest.mark.parametrize("bc,t", [(3, 0.37), (4, 123.0), (4, 400.0)],
-                         ids=["convergence", "three_mound_early", "three_mound_late"])
+@pytest.mark.parametrize("bc,t", [(3, 0.37), (4, 123.0), (4, 400.0), (5, 0.37), (6, 0.37), (7, 0.37)],
+                         ids=["convergence", "three_mound_early", "three_mound_late", "conv_test3",
+                              "conv_test4", "conv_test5"])
 def test_collect_edge_data_time_dependent(dev, bc, t):
-    """Dirichlet_ConvTest.okl (manufactured exterior state) and 3MoundInflow.okl (walls, inflow at
-    x = 0 switching at t = 300, outflow): the exterior states use the node coordinates and t."""
+    """Dirichlet_ConvTest{,3,4,5}.okl (manufactured exterior state) and 3MoundInflow.okl (walls,
+    inflow at x = 0 switching at t = 300, outflow): the exterior states use the node coordinates and t."""
     from esdgsem_mpiocca_amd import Case
     from esdgsem_mpiocca_amd import setup_api as SA
-    if bc == 3:
-        case = Case.convergence(N=4, nel=5, curved=True)
+    if bc in (3, 5, 6, 7):
+        testcase = {3: 1, 5: 8, 6: 88, 7: 89}[bc]
+        case = Case.convergence(N=4, nel=5, curved=True, testcase=testcase)
     else:
         case = Case(SA.CASE_CONSTANT, SA.MESH_CARTESIAN, 4, 6, 3, 0.0, 4.0, 0.0, 2.0, p=(1.0, 0.2, 0.1, 0))
     rp = RunParams(bc_kind=bc)

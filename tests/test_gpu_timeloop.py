@@ -146,6 +146,22 @@ def test_manufactured_solution_parity_and_convergence(dev):
     assert (eoc >= 3.7).all(), (errs, eoc)
 
 
+@pytest.mark.parametrize("testcase,CFL", [(8, 0.2), (88, 0.2), (89, 0.01)])
+def test_manufactured_variants_match_oracle(dev, testcase, CFL):
+    """Testcases 8, 88, 89 (Dirichlet_ConvTest3/4/5.okl + S_ConvTest3/4/5.okl) on the curved mesh:
+    the reference kernel sequence on the GPU against the oracle over 20 steps; the state stays
+    within the discretisation error of the exact solution, whose time-dependent part has amplitude 1
+    in h (the oracle's errors: 4e-3, 7e-4, 5e-3)."""
+    from test_oracle_properties import manufactured
+    case = Case.convergence(N=3, nel=6, curved=True, testcase=testcase)
+    rp = RunParams(num_flux=0, bc_kind=S.CONVERGENCE_BC[testcase], CFL=CFL)
+    qo, no, to, _ = O.timeloop(case, rp, 20)
+    qg, ng, tg = run_gpu(dev, case, rp, 20, fused=0)
+    assert ng == no == 20 and abs(tg - to) <= 1e-14 * to
+    assert max(fields_rel_l2(case, qg, qo)) <= 1e-12
+    assert np.abs(qg - manufactured(case, tg, testcase)).max() <= 0.02
+
+
 def test_entropy_conservative_flux(dev):
     """ES=0 (EC surface flux): total entropy changes only by the time-integration error."""
     case = Case.gaussian_bump(N=4, nel=8)

@@ -568,15 +568,70 @@ def test_timeloop_mass_conservation_bump():
     assert abs(m1 - m0) < 1e-13 * m0
 
 
-def manufactured(case, t):
-    """The convergence test's exact solution (SW2D.cpp:312-318): h = 8 + cos x sin y cos t - b."""
+# mean depth H0 and wave number k of each convergence Testcase (SW2D.cpp:312-318, 388-393, 760-775)
+MANUFACTURED = {1: (8.0, 1.0), 8: (20.0, 1.0), 88: (20.0, 1e-3), 89: (20000.0, 1e-6)}
+
+
+def manufactured(case, t, testcase=1, dt=False):
+    """A convergence test's exact solution: h = H0 + cos(kx) sin(ky) cos t - b, hu = h/2, hv = 3h/2
+    (with dt=True its time derivative)."""
+    H0, k = MANUFACTURED[testcase]
     x, y, b = case["x"], case["y"], case["b"]
-    h = 8.0 + np.cos(x) * np.sin(y) * np.cos(t) - b
+    if dt:
+        h = -np.cos(k * x) * np.sin(k * y) * np.sin(t)
+    else:
+        h = H0 + np.cos(k * x) * np.sin(k * y) * np.cos(t) - b
     q = np.zeros((case.Nelem, 3, case.ngl ** 2))
     q[:, 0] = h.reshape(case.Nelem, -1)
     q[:, 1] = 0.5 * q[:, 0]
     q[:, 2] = 1.5 * q[:, 0]
     return q.ravel()
+
+
+def manufactured_residual(case, rp, q, t):
+    """Qt of one stage of a convergence test at time t: the Dirichlet gather, volume, ES flux,
+    surface lift and the manufactured source (the reference's kernel order, deviceclass.cpp:957-1102)."""
+    p = O.params(case, rp)
+    a = case.arrays
+    Ne, Nf, ngl = case.Nelem, case.Nfaces, case.ngl
+    fn = Nf * ngl
+    qL, qR, bL, bR = np.zeros(fn * 3), np.zeros(fn * 3), np.zeros(fn), np.zeros(fn)
+    O.call("collect_edge_data_bottom", p, Nf, a["EdgeData"], a["b"], bL, bR)
+    O.call("collect_edge_data_xt", p, Nf, a["EdgeData"], q, a["x"], a["y"], a["nx"], a["ny"], bL, bR, t, qL, qR)
+    Qt = np.zeros(Ne * ngl * ngl * 3)
+    O.call("volume_fluxdiff", p, Ne, a["Jac"], a["yXi"], a["yEta"], a["xXi"], a["xEta"], q, a["D"], a["Bx"],
+           a["By"], Qt)
+    SP, DB = np.zeros(fn * 3), np.zeros(fn)
+    O.call("num_flux_es", p, Nf, a["EdgeReversed"], a["nx"], a["ny"], a["scal"], qL, qR, bL, bR, SP)
+    O.call("surface", p, Ne, a["Jac"], a["ElemEdgeMasterSlave"], a["ElemEdgeOrientation"], a["ElemToEdge"], SP,
+           DB, DB, Qt)
+    O.call("add_source", p, Ne, a["Bx"], a["By"], a["b"], a["x"], a["y"], t, Qt)
+    return Qt
+
+
+@pytest.mark.parametrize("testcase", [1, 8, 88, 89])
+def test_manufactured_source_balances_the_residual(testcase):
+    """Dirichlet_ConvTest{,3,4,5}.okl + S_ConvTest{,3,4,5}.okl: on the exact solution the stage
+    residual plus the manufactured source is the exact time derivative up to the spatial truncation
+    error, which falls at order about N under refinement.  A wrong mean depth H0, a wrong or missing
+    wave-number factor k in the source's derivatives, or a sign error leaves an O(1) remainder."""
+    N, t = 3, 0.7
+    rp = RunParams(num_flux=0, bc_kind=S.CONVERGENCE_BC[testcase])
+    errs = []
+    for nel in (4, 8):
+        c = Case.convergence(N=N, nel=nel, testcase=testcase)
+        q_t = manufactured(c, t, testcase, dt=True)
+        Qt = manufactured_residual(c, rp, manufactured(c, t, testcase), t)
+        errs.append(np.abs(Qt - q_t).max() / np.abs(q_t).max())
+    eoc = np.log2(errs[0] / errs[1])
+    assert errs[1] < 5e-3 and eoc >= N - 0.6, (errs, eoc)
+    # the pin is sharp: the source of another Testcase leaves an O(1) remainder
+    other = {1: 8, 8: 1, 88: 89, 89: 88}[testcase]
+    c = Case.convergence(N=N, nel=8, testcase=testcase)
+    Qt = manufactured_residual(c, RunParams(num_flux=0, bc_kind=S.CONVERGENCE_BC[other]),
+                               manufactured(c, t, testcase), t)
+    q_t = manufactured(c, t, testcase, dt=True)
+    assert np.abs(Qt - q_t).max() / np.abs(q_t).max() > 0.1
 
 
 @pytest.mark.parametrize("N,eoc_min", [(2, 2.7), (3, 3.7)])
@@ -593,6 +648,19 @@ def test_manufactured_solution_converges_at_order_N_plus_1(N, eoc_min):
         errs.append(np.abs(q - manufactured(c, t)).max())
     eoc = np.log2(errs[0] / errs[1])
     assert eoc >= eoc_min, (errs, eoc)
+
+
+def test_testcase_8_converges_at_order_N_plus_1():
+    """Testcase 8 (Dirichlet_ConvTest3.okl + S_ConvTest3.okl, b = 2 + sin y + cos x, H0 = 20) through
+    the oracle's time loop: the error against the exact solution falls as h^(N+1)."""
+    errs = []
+    for nel in (4, 8):
+        c = Case.convergence(N=3, nel=nel, testcase=8)
+        q, n, t, _ = O.timeloop(c, RunParams(num_flux=0, bc_kind=5, CFL=0.2, T=0.1), 100000)
+        assert t == 0.1
+        errs.append(np.abs(q - manufactured(c, t, 8)).max())
+    eoc = np.log2(errs[0] / errs[1])
+    assert eoc >= 3.7, (errs, eoc)
 
 
 # --------------------------------------------------------------------------------------------
