@@ -51,11 +51,19 @@ def volume_flops(nelem, ngl):
 # Yxi/Yeta/Xxi/Xeta 4, 1/J, Bx, By, gRK 3 read and gRK 3, q 3 written (19 doubles); per element
 # side node F* 3 [+ viscous face flux 2] read.  (The viscous volume term, read only where the
 # viscosity is non-zero, is not counted.)
-def fused_volume_bytes(nelem, ngl, av, traces=True):
-    """Per node: q 3, Yxi/Yeta/Xxi/Xeta 4, 1/J, Bx, By, gRK 3 read, gRK 3 and q 3 written; per
-    element side node: F* 3 (and the viscous face flux 2 with AV) read, and without the limiter the
-    side traces of the new state (3) written."""
-    return 8 * nelem * (19 * ngl * ngl + (12 + (8 if av else 0) + (12 if traces else 0)) * ngl)
+def fused_volume_bytes(nelem, ngl, av, traces=True, sensor=False, stages=5):
+    """Average bytes per launch.  Per node: q 3, Yxi/Yeta/Xxi/Xeta 4, 1/J, Bx, By, gRK 3 read, gRK 3
+    and q 3 written; per element side node: F* 3 (and the viscous face flux 2 with AV) read, and
+    without the limiter the side traces of the new state (3) written.  Without the limiter the pass
+    also evaluates, on the new state, the sensor when `sensor` (b read per node; the viscosity and
+    its plot copy written per element) and, in the last of the step's `stages`, the element wave
+    speed (one double per element)."""
+    per_elem = 19 * ngl * ngl + (12 + (8 if av else 0) + (12 if traces else 0)) * ngl
+    if traces and sensor:
+        per_elem += ngl * ngl + 2
+    if traces:
+        per_elem += 1.0 / stages
+    return int(round(8 * nelem * per_elem))
 
 
 KERNELS = ["volume_kernel", "collect_edge_data", "calc_num_fluxes", "surface_kernel", "calc_rk",
@@ -380,7 +388,10 @@ def main():
     roofline = None
     if vol_name in per_kernel:
         avg_s = per_kernel[vol_name]["avg_us"] * 1e-6
-        alg = (fused_volume_bytes(case.Nelem, case.ngl, rp.artificial_viscosity, not rp.positivity) if args.fused
+        # this workload's viscosity is the modal sensor alone, which the fused schedule evaluates
+        # inside the state update
+        alg = (fused_volume_bytes(case.Nelem, case.ngl, rp.artificial_viscosity, not rp.positivity,
+                                  sensor=bool(rp.artificial_viscosity), stages=5) if args.fused
                else volume_bytes(case.Nelem, case.ngl))
         gbs = alg / avg_s / 1e9
         tfl = volume_flops(case.Nelem, case.ngl) / avg_s / 1e12

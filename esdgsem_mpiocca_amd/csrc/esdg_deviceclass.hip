@@ -164,11 +164,13 @@ class deviceclass {
     }
 
     // One time step's dt (src/deviceclass.cpp:899-928), reductions on the device.
+    // The fused stage may have evaluated FindLambdaMax and ShockCapturing of the current state
+    // already (lambda_fresh, visc_fresh): then the time step reduces those values.
     int timestep(double* dt_out) {
-        int rc = esdg_find_lambda_max(ctx_, Nelem, o_q, o_LambdaMax);
+        int rc = lambda_fresh ? ESDG_OK : esdg_find_lambda_max(ctx_, Nelem, o_q, o_LambdaMax);
         if (rc == ESDG_OK) rc = esdg_reduce_max(ctx_, Nelem, o_LambdaMax, o_red);
         if (rc == ESDG_OK && rp_.artificial_viscosity) {
-            rc = viscosity();
+            if (!visc_fresh) rc = viscosity();
             if (rc == ESDG_OK) rc = esdg_reduce_max(ctx_, Nelem, o_ViscPara, o_red + 1);
         }
         double red[2] = {0.0, 0.0};
@@ -252,6 +254,12 @@ class deviceclass {
             rc = fused_.begin(ctx_, o_q);
             if (rc) return rc;
         }
+        // q may have been set since the last call: its wave speeds and viscosity are evaluated anew
+        lambda_fresh = visc_fresh = false;
+        // Without the limiter the fused state update evaluates the sensor (every stage) and the
+        // wave speeds (last stage) of the state it writes, in place of the separate passes.
+        const bool fuse_next = rp_.fused == 1 && !rp_.positivity;
+        const bool fuse_sensor = fuse_next && rp_.artificial_viscosity && !ind_;
         int n = 0;
         while (t < rp_.T && n < max_steps) {
             double dt = 0.0;
@@ -266,12 +274,20 @@ class deviceclass {
             }
             for (int s = 0; s < stages; ++s) {
                 // the fused stage takes the viscosity of its state from here (stage 0: the time step's)
-                if (rp_.fused && rp_.artificial_viscosity && s > 0) {
+                if (rp_.fused && rp_.artificial_viscosity && s > 0 && !visc_fresh) {
                     rc = viscosity();
                     if (rc) return rc;
                 }
-                if (rp_.fused) rc = fused_.stage(ctx_, s, A[s], Cc[s] * dt, fused_args());
-                else rc = stage(s, dt);
+                if (rp_.fused) {
+                    FusedArgs fa = fused_args();
+                    if (fuse_sensor) { fa.next_visc = o_ViscPara; fa.next_visc_plot = o_ViscForPlot; }
+                    if (fuse_next && s == stages - 1) fa.next_lambda = o_LambdaMax;
+                    rc = fused_.stage(ctx_, s, A[s], Cc[s] * dt, fa);
+                    visc_fresh = fuse_sensor;
+                    lambda_fresh = fuse_next && s == stages - 1;
+                } else {
+                    rc = stage(s, dt);
+                }
                 if (rc) return rc;
             }
             t += dt;
@@ -378,11 +394,11 @@ class deviceclass {
         if (n == "SurfaceParts") return o_SurfaceParts;
         if (n == "LambdaMax") return o_LambdaMax;
         if (n == "ViscPara") return o_ViscPara;
+        if (n == "ViscForPlot") return o_ViscForPlot;
         if (n == "ArrivalTimings") return o_ArrivalTimings;
         if (n == "MaximumElevation") return o_MaximumElevation;
         if (n == "qGradientX") return o_qGradientX;
         if (n == "qGradientY") return o_qGradientY;
-        if (n == "bL") return o_bL;
         return fused_.buffer(n);
     }
 
@@ -393,6 +409,7 @@ class deviceclass {
     const double rkD[3] = {0.0, 1.0, 0.5};
     double t = 0.0, last_dt = 0.0;
     bool bottom_done = false;
+    bool lambda_fresh = false, visc_fresh = false;   // o_LambdaMax / o_ViscPara hold those of o_q
     FusedStage fused_;
     StageBatch stage_;
     bool stage_ready = false;

@@ -42,6 +42,13 @@ struct SurfIn {
     const double* visc;    // element viscosity
     const double* SPV;     // viscous face fluxes [f][2][ngl]
     double* tr;            // side traces of the updated state (RK modes; may be null)
+    // Of the updated state, when tr is given (RK modes; each may be null):
+    double* visc_next;     // ShockCapturing's element viscosity (and visc_plot, its copy)
+    double* visc_plot;
+    const double *B, *VdmInv;
+    SensorConsts sc;
+    double* lam;           // FindLambdaMax's element maximum wave speed
+    double tol;            // PosPresTOL (velocities are 0 where h <= tol)
 };
 
 template <int NGL, int MODE>
@@ -59,6 +66,8 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
     const bool act = (el < C::EPW) && (e < Nelem);
     double* S = s + (act ? el : 0) * C::ESZ;
     const double half_g = 0.5 * g, fourth_g = 0.25 * g;
+    double lam = 0.0;   // this column's maximum wave speed of the updated state (si.lam)
+    double bsen[NGL];   // b of row r (si.visc_next)
 
     double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], Y[NGL], X[NGL];
     double a1[NGL], a2[NGL], a3[NGL];
@@ -196,6 +205,12 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             }
         }
         if (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
+            // b of row r for the sensor below, loaded here so that its latency overlaps the update
+            if (si.visc_next) {
+                const int64_t bb = (int64_t)e * C::NN + r * NGL;
+#pragma unroll
+                for (int i = 0; i < NGL; ++i) bsen[i] = si.B[bb + i];
+            }
 #pragma unroll
             for (int k = 0; k < NGL; ++k) {
                 const int64_t id = ob + k * NGL;
@@ -219,6 +234,12 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                     S[1 * C::PLANE + k * C::RS + r] = n2;
                     S[2 * C::PLANE + k * C::RS + r] = n3;
                 }
+                // FindLambdaMax.okl:41-60 at this node (the element maximum is order-free)
+                if (si.lam) {
+                    const double Qinv = (n1 > si.tol) ? 1.0 / n1 : 0.0;
+                    const double c = sqrt(g * n1);
+                    lam = fmax(lam, c + fmax(fabs(n2 * Qinv), fabs(n3 * Qinv)));
+                }
             }
         } else {
 #pragma unroll
@@ -230,6 +251,9 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         }
     }
     if constexpr (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
+        __shared__ double sV[C::NN];
+        if (si.visc_next)
+            for (int t = lane; t < C::NN; t += 64) sV[t] = si.VdmInv[t];
         // side traces of the new state, [e][side][v][k] (read by the next stage's face pass): the
         // block's elements own one contiguous run of the trace buffer, written lane-consecutively
         if (si.tr) {
@@ -244,6 +268,59 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : k;
                 const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : k;
                 T[t] = s[el2 * C::ESZ + v * C::PLANE + row * C::RS + col];
+            }
+            // ShockCapturing.okl:63-217 on the updated state, the next stage's viscosity, with the
+            // arithmetic of k_shock_capturing: lane r forms column r of T = V (h + b) from row r of
+            // h + b, then row r of the modes m = T V^T in place; four lanes form the energy sums.
+            // Plane 3 (1/h) is free after the column phase and holds T and then m.
+            if (si.visc_next) {
+                __shared__ double sSum[C::EPW][4];
+                double* Tm = S + 3 * C::PLANE;
+                const double* V = sV;
+                if (act) {
+                    double w[NGL];
+#pragma unroll
+                    for (int i = 0; i < NGL; ++i) w[i] = bsen[i] + S[0 * C::PLANE + r * C::RS + i];
+#pragma unroll
+                    for (int a = 0; a < NGL; ++a) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int l = 0; l < NGL; ++l) acc += V[a * NGL + l] * w[l];
+                        Tm[a * C::RS + r] = acc;
+                    }
+                }
+                __syncthreads();
+                if (act) {
+                    double t[NGL];
+#pragma unroll
+                    for (int l = 0; l < NGL; ++l) t[l] = Tm[r * C::RS + l];
+#pragma unroll
+                    for (int c = 0; c < NGL; ++c) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int l = 0; l < NGL; ++l) acc += t[l] * V[c * NGL + l];
+                        Tm[r * C::RS + c] = acc;
+                    }
+                }
+                __syncthreads();
+                if (act && r < 4) sSum[el][r] = sensor_sum<NGL, C::RS>(Tm, r);
+                __syncthreads();
+                if (act && r == 0) {
+                    const double v = sensor_ramp(sSum[el][0], sSum[el][1], sSum[el][2], sSum[el][3], si.sc);
+                    si.visc_next[e] = v;
+                    if (si.visc_plot) si.visc_plot[e] = v;
+                }
+            }
+            if (si.lam) {
+                __shared__ double sLam[64];
+                sLam[lane] = lam;
+                __syncthreads();
+                if (act && r == 0) {
+                    double m = 0.0;
+#pragma unroll
+                    for (int k = 0; k < NGL; ++k) m = fmax(m, sLam[el * NGL + k]);
+                    si.lam[e] = m;
+                }
             }
         }
     }
@@ -317,6 +394,16 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
                          double* Qt, const double* Qv, const double* SPV, double* tr) {
     using C = VolCfg<NGL>;
     SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV, rk ? tr : nullptr};
+    if (rk && tr) {
+        // the updated state's viscosity and wave speed, for the next stage or time step
+        si.visc_next = a.next_visc;
+        si.visc_plot = a.next_visc ? a.next_visc_plot : nullptr;
+        si.B = a.B;
+        si.VdmInv = a.VdmInv;
+        si.sc = SensorConsts{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
+        si.lam = a.next_lambda;
+        si.tol = ctx->p.PosPresTOL;
+    }
     if (rk && Qv)
         k_volume_fluxdiff<NGL, VOL_SURF_VISC_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);

@@ -187,6 +187,37 @@ def test_fused_equals_reference_sequence_bitwise(dev, av, pos):
     assert np.array_equal(qa, qb) or np.abs(qa - qb).max() <= 1e-15 * np.abs(qa).max()
 
 
+@pytest.mark.parametrize("av", [0, 1])
+def test_fused_update_evaluates_sensor_and_wave_speed(dev, av):
+    """Without the limiter the fused state update also evaluates ShockCapturing (every stage) and
+    FindLambdaMax (last stage) of the state it writes: after a run, ViscPara, ViscForPlot and
+    LambdaMax equal those kernels applied to the final state, bit for bit."""
+    case = Case.dam_break(N=7, nel=8, noise=1e-3, seed=5)
+    rp = RunParams(num_flux=3, artificial_viscosity=av, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5)
+    E = case.Nelem
+    ctx = dev.Context(dev.make_params(case, rp))
+    try:
+        s = dev.Solver(ctx, case, rp, fused=1)
+        n, _ = s.run(6)
+        assert n == 6
+        lam = s.view("LambdaMax", E).download()
+        visc = s.view("ViscPara", E).download()
+        plot = s.view("ViscForPlot", E).download()
+        q = s.get_q()
+        s.close()
+        dq, dB = ctx.array(q), ctx.array(case["b"])
+        dV, dS = ctx.array(case["VdmInv"]), ctx.array(case["EleSizes"])
+        dvisc, dplot, dlam = ctx.empty(E), ctx.empty(E), ctx.empty(E)
+        ctx.find_lambda_max(E, dq, dlam)
+        ctx.shock_capturing(E, dq, dB, dV, dS, dvisc, dplot)
+        assert np.array_equal(lam, dlam.download()) and (lam > 0).all()
+        if av:
+            assert np.array_equal(visc, dvisc.download()) and np.array_equal(plot, dplot.download())
+            assert (visc > 0).any() and (visc == 0).any()
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("pos", [0, 1])
 def test_single_kernel_equals_fused_bitwise(dev, pos):
     """The single-kernel stage (device dt, ping-pong state, graph replay) performs the fused
