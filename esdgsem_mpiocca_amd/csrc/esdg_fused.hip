@@ -31,7 +31,7 @@
 namespace esdg {
 
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV, double* tr);
+                 const double* Qv, const double* SPV, double* tr, const int* active = nullptr, int seq = 0);
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
                     double* tr);
 int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr);
@@ -259,18 +259,24 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const d
 // ---------------------------------------------------------------------------------------------
 // fused_viscous_face: CollectEdgeDataGradient.okl:17-194 + calcNumFluxesViscose.okl:17-107
 // ---------------------------------------------------------------------------------------------
+// Only elements marked this stage (active[e] == seq: viscosity on themselves or a face neighbour)
+// read these fluxes; every face of an unmarked element has zero viscosity on both sides, so its
+// flux is +-0 and the volume pass skips it.  A face none of whose elements is marked is not written.
 __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl, double geomFace,
                                                             const int* __restrict__ ED,
                                                             const double* __restrict__ gtr,
                                                             const double* __restrict__ visc,
                                                             const double* __restrict__ nxa,
                                                             const double* __restrict__ nya,
-                                                            const double* __restrict__ scal, double* __restrict__ SPV) {
+                                                            const double* __restrict__ scal,
+                                                            const int* __restrict__ active, int seq,
+                                                            double* __restrict__ SPV) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)Nfaces * ngl) return;
     const int f = (int)(t / ngl), i = (int)(t % ngl);
     const int* E = ED + 8 * (int64_t)f;
     const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], rv = E[4], cpuL = E[5], cpuR = E[6];
+    if (!((cpuL == 0 && active[iel] == seq) || (cpuR == 0 && active[ier] == seq))) return;
     const int64_t eid = (int64_t)f * ngl * 2 + i;
     double N0 = 0.0, N1 = 0.0;
     if (!(cpuL == -1 || cpuR == -1)) {
@@ -394,12 +400,12 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     {
         KernelScope ks(ctx, "fused_viscous_face");
         k_fused_viscous_face<<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, ctx->p.geomFace, A.EdgeData, grad,
-                                                          A.ViscPara, A.nx, A.ny, A.scal, fvisc);
+                                                          A.ViscPara, A.nx, A.ny, A.scal, active, seq, fvisc);
         rc = launch_check(ctx, "fused_viscous_face");
         if (rc) return rc;
     }
     // volume + surface + viscous terms + RK in one element pass, then the limiter
-    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr);
+    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, active, seq);
     if (!rc && A.positivity) rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr);
     return rc;
 }

@@ -49,6 +49,10 @@ struct SurfIn {
     SensorConsts sc;
     double* lam;           // FindLambdaMax's element maximum wave speed
     double tol;            // PosPresTOL (velocities are 0 where h <= tol)
+    // VOL_SURF_VISC_RK: elements with active[e] != seq have zero viscosity on themselves and on
+    // every face neighbour; their viscous terms are +-0 and are not read
+    const int* active;
+    int seq;
 };
 
 template <int NGL, int MODE>
@@ -172,7 +176,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 }
             }
         }
-        if (MODE == VOL_SURF_VISC_RK) {
+        if (MODE == VOL_SURF_VISC_RK && si.active[e] == si.seq) {
             if (si.visc[e] != 0.0) {
                 const double* qv = si.Qv + (int64_t)e * C::NN * 2 + r;
 #pragma unroll
@@ -391,9 +395,11 @@ void launch_volume_fluxdiff(esdg_ctx* ctx, int Nelem, const double* Jac, const d
 
 template <int NGL>
 void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar,
-                         double* Qt, const double* Qv, const double* SPV, double* tr) {
+                         double* Qt, const double* Qv, const double* SPV, double* tr, const int* active, int seq) {
     using C = VolCfg<NGL>;
     SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV, rk ? tr : nullptr};
+    si.active = active;
+    si.seq = seq;
     if (rk && tr) {
         // the updated state's viscosity and wave speed, for the next stage or time step
         si.visc_next = a.next_visc;
@@ -418,9 +424,10 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
 // Fused volume + surface lift (+ viscous terms when Qv is given) (+ RK update in place when rk,
 // with the side traces of the new state written to tr when given): called by the fused stage.
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV, double* tr) {
+                 const double* Qv, const double* SPV, double* tr, const int* active, int seq) {
+    if (Qv && !active) return set_error(ctx, ESDG_ERR_STATE, "fused_volume: viscous terms need the stage marks");
     KernelScope ks(ctx, "fused_volume");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr, active, seq);
     return launch_check(ctx, "fused_volume");
 }
 
