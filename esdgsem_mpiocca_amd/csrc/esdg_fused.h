@@ -42,7 +42,6 @@ class FusedStage {
     bool av = false;
     double* qn = nullptr;       // reserved
     double* fstar = nullptr;    // face fluxes F* [f][3][ngl]
-    double* fgrad = nullptr;    // BR1 face fluxes [X|Y][f][2][ngl] (AV)
     double* grad = nullptr;     // side traces of the scaled velocity gradients [e][side][4][ngl] (AV)
     double* fvisc = nullptr;    // viscous face fluxes [f][2][ngl] (AV)
     double* qvisc = nullptr;    // viscous volume term [e][2][ngl^2] (hu, hv) (AV)

@@ -36,9 +36,31 @@ int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const doub
                     double* tr);
 int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr);
 
+// calcNumFluxesGradient.okl:16-135 at face node i: the BR1 face fluxes of u and v (x and y parts)
+// from qL and qR as the gather stores them, both read unreversed at node i, the exterior state
+// taking the place of a missing side.
+struct Br1Flux {
+    double x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ Br1Flux br1_face_flux(const double* __restrict__ tr, int ngl, const int* E, int i, double nx,
+                                                 double ny, double sc, double bl, double br, const GatherConsts& gc,
+                                                 double tol, double geomFace) {
+    const FaceState L = gathered_tr(tr, ngl, E, true, i, nx, ny, gc, bl, br);
+    const FaceState Ri = gathered_tr(tr, ngl, E, false, i, nx, ny, gc, bl, br);
+    const FaceState LL = (E[5] == -1) ? Ri : L;
+    const FaceState RR = (E[6] == -1) ? L : Ri;
+    const double iL = (LL.h > tol) ? 1.0 / LL.h : 0.0;
+    const double iR = (RR.h > tol) ? 1.0 / RR.h : 0.0;
+    const double uL = LL.hu * iL, uR = RR.hu * iR, vL = LL.hv * iL, vR = RR.hv * iR;
+    return Br1Flux{nx * sc * geomFace * 0.5 * (uL + uR), nx * sc * geomFace * 0.5 * (vL + vR),
+                   ny * sc * geomFace * 0.5 * (uL + uR), ny * sc * geomFace * 0.5 * (vL + vR)};
+}
+
 // ---------------------------------------------------------------------------------------------
 // fused_face: CollectEdgeData (SolidWalls.okl:16-207) + calcNumFluxes (calcESFlux.okl /
-// calcESFluxRotated.okl) [+ calcNumFluxesGradient.okl:16-135], one thread per face node.
+// calcESFluxRotated.okl), one thread per face node; with AV it also marks the elements with
+// viscosity in reach (the AV pass evaluates the BR1 face fluxes of those elements' sides itself).
 // ---------------------------------------------------------------------------------------------
 // KIND: the numerical flux, a template argument so that each instance holds only its own flux
 // (fewer registers, more waves in flight).
@@ -48,7 +70,6 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
                                                     const double* __restrict__ nxa, const double* __restrict__ nya,
                                                     const double* __restrict__ scal, const double* __restrict__ BL,
                                                     const double* __restrict__ BR, double* __restrict__ SP,
-                                                    double* __restrict__ SGX, double* __restrict__ SGY,
                                                     const double* __restrict__ visc, int* __restrict__ active,
                                                     int seq) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -70,21 +91,9 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
     SP[eid + ngl] = sc * c.geomFace * F1;
     SP[eid + 2 * ngl] = sc * c.geomFace * F2;
     if (AV) {
-        // calcNumFluxesGradient reads qL and qR unreversed at node i
-        const int cpuL = E[5], cpuR = E[6];
-        const FaceState Ri = (rv == 1) ? R : gathered_tr(tr, ngl, E, false, i, nx, ny, gc, BL[t], BR[t]);
-        const FaceState LL = (cpuL == -1) ? Ri : L;
-        const FaceState RR = (cpuR == -1) ? L : Ri;
-        const double iL = (LL.h > c.tol) ? 1.0 / LL.h : 0.0;
-        const double iR = (RR.h > c.tol) ? 1.0 / RR.h : 0.0;
-        const double uL = LL.hu * iL, uR = RR.hu * iR, vL = LL.hv * iL, vR = RR.hv * iR;
-        const int64_t gid = (int64_t)f * ngl * 2 + i;
-        SGX[gid] = nx * sc * c.geomFace * 0.5 * (uL + uR);
-        SGX[gid + ngl] = nx * sc * c.geomFace * 0.5 * (vL + vR);
-        SGY[gid] = ny * sc * c.geomFace * 0.5 * (uL + uR);
-        SGY[gid + ngl] = ny * sc * c.geomFace * 0.5 * (vL + vR);
         // elements with viscosity in reach (on themselves or a face neighbour) need the viscous
         // work: a face with viscosity on either side marks both its elements for this stage
+        const int cpuL = E[5], cpuR = E[6];
         if (i == 0) {
             const bool hasL = cpuL == 0, hasR = cpuR == 0;
             if ((hasL && visc[E[0]] != 0.0) || (hasR && visc[E[1]] != 0.0)) {
@@ -104,17 +113,19 @@ struct AvChunk {
 };
 
 // ---------------------------------------------------------------------------------------------
-// fused_av: calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148 + scaleGradient.okl:17-60
-// + VolumeKernelViscose.okl:17-150 for one block of elements, one thread per node, all
-// intermediate fields in LDS.  Writes the scaled gradients (read by the viscous face kernel) as
-// element-side traces and the viscous volume term Qv(hu, hv).
+// fused_av: calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148 (with the BR1 face fluxes
+// of calcNumFluxesGradient.okl evaluated per element side from the state's side traces) +
+// scaleGradient.okl:17-60 + VolumeKernelViscose.okl:17-150 for one block of elements, one thread
+// per node, all intermediate fields in LDS.  Writes the scaled gradients (read by the viscous face
+// kernel) as element-side traces and the viscous volume term Qv(hu, hv).
 // ---------------------------------------------------------------------------------------------
 //
 // The element viscosity comes in (evaluated on the stage's state before the stage).  A block
 // whose elements and their face neighbours all have zero viscosity has nothing to contribute: its
 // gradient traces feed only faces of zero viscosity (whose fluxes are zero without reading them)
 // and its viscous volume term is skipped where the viscosity is zero.  The face pass marks the
-// others (active[e] == seq); a workgroup takes a chunk of element blocks and runs the marked ones.
+// others (active[e] == seq); a workgroup takes a chunk of element blocks and runs the marked
+// elements of the blocks that hold one.
 template <int NGL>
 __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const double* __restrict__ q,
                                                   const double* __restrict__ Dh, const double* __restrict__ Ds,
@@ -122,7 +133,10 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const d
                                                   const double* __restrict__ Yeta, const double* __restrict__ Xxi,
                                                   const double* __restrict__ Xeta, const int* __restrict__ E2E,
                                                   const int* __restrict__ EMS, const int* __restrict__ EOR,
-                                                  const double* __restrict__ SGX, const double* __restrict__ SGY,
+                                                  const int* __restrict__ ED, const double* __restrict__ tr,
+                                                  const double* __restrict__ nxa, const double* __restrict__ nya,
+                                                  const double* __restrict__ scal, const double* __restrict__ BL,
+                                                  const double* __restrict__ BR, GatherConsts gc, double geomFace,
                                                   double* __restrict__ gtr, const double* __restrict__ visc,
                                                   const int* __restrict__ active, int seq, double* __restrict__ Qv) {
     using EB = ElemBlock<NGL>;
@@ -152,8 +166,10 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const d
         __syncthreads();
         const int cnt = sCount;
         for (int li = 0; li < cnt; ++li) {
+            // an unmarked element of a marked block has no reader: its gradient traces feed only
+            // faces of zero viscosity and its viscous volume term is skipped
             const int e = (chunk * AC::CH + sList[li]) * EB::EPB + el;
-            const bool act = e < Nelem;
+            const bool act = e < Nelem && active[e] == seq;
             if (act && n == 0) sEps[el] = visc[e];
             const int a = n % NGL, c = n / NGL;           // node n = (xi = a, eta = c)
             const int64_t xid = (int64_t)e * NN + n;
@@ -202,12 +218,14 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const d
                     const int rev = EOR[e * 4 + side];
                     const int ms = EMS[e * 4 + side];
                     const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
-                    const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
-                    if (ph == 0) gx0 += ms * SGX[fl] * rJ;
-                    else gx0 += ms * rJ * SGX[fl];
-                    gx1 += ms * rJ * SGX[fl + NGL];
-                    gy0 += ms * rJ * SGY[fl];
-                    gy1 += ms * rJ * SGY[fl + NGL];
+                    const int64_t fn = (int64_t)ifa * NGL + inc;
+                    const Br1Flux F = br1_face_flux(tr, NGL, ED + 8 * (int64_t)ifa, inc, nxa[fn], nya[fn], scal[fn],
+                                                    BL[fn], BR[fn], gc, tol, geomFace);
+                    if (ph == 0) gx0 += ms * F.x0 * rJ;
+                    else gx0 += ms * rJ * F.x0;
+                    gx1 += ms * rJ * F.x1;
+                    gy0 += ms * rJ * F.y0;
+                    gy1 += ms * rJ * F.y1;
                 }
                 // scaleGradient: multiply by h
                 gx0 = h * gx0; gx1 = h * gx1; gy0 = h * gy0; gy1 = h * gy1;
@@ -305,14 +323,13 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl,
 template <int NGL>
 void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     using EB = ElemBlock<NGL>;
-    const double* SGX = F.fgrad;
-    const double* SGY = F.fgrad + (size_t)A.Nfaces * NGL * 2;
     using AC = AvChunk<NGL>;
     const int64_t nblk = (A.Nelem + EB::EPB - 1) / EB::EPB;
     const unsigned grid = (unsigned)std::min<int64_t>((nblk + AC::CH - 1) / AC::CH, 8192);
     k_fused_av<NGL><<<grid, EB::THREADS, 0, ctx->stream>>>(
         A.Nelem, ctx->p.PosPresTOL, A.q, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
-        A.ElemToEdge, A.EMS, A.EOR, SGX, SGY, F.grad, A.ViscPara, F.active, F.seq, F.qvisc);
+        A.ElemToEdge, A.EMS, A.EOR, A.EdgeData, F.tr, A.nx, A.ny, A.scal, A.bL, A.bR,
+        GatherConsts{ctx->p.bc_kind, ctx->p.h_zero}, ctx->p.geomFace, F.grad, A.ViscPara, F.active, F.seq, F.qvisc);
 }
 
 
@@ -323,7 +340,6 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     int rc = esdg_malloc(ctx, (void**)&fstar, fn * 3 * sizeof(double));
     if (!rc) rc = esdg_malloc(ctx, (void**)&tr, (size_t)Ne * 12 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&active, ((size_t)Ne + 1) / 2 * sizeof(double));
-    if (!rc && av) rc = esdg_malloc(ctx, (void**)&fgrad, fn * 4 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * 16 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fvisc, fn * 2 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&qvisc, (size_t)Ne * 2 * ngl * ngl * sizeof(double));
@@ -359,13 +375,11 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     }
     {
         KernelScope ks(ctx, "fused_face");
-        double* fSGY = av ? fgrad + (size_t)A.Nfaces * ngl * 2 : nullptr;
         auto launch = [&](auto av_tag, auto kind_tag) {
             constexpr bool AVK = decltype(av_tag)::value;
             constexpr int KK = decltype(kind_tag)::value;
             k_fused_face<AVK, KK><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, fc, gc, A.EdgeData, tr, A.nx, A.ny,
-                                                             A.scal, A.bL, A.bR, fstar, AVK ? fgrad : nullptr,
-                                                             fSGY, AVK ? A.ViscPara : nullptr,
+                                                             A.scal, A.bL, A.bR, fstar, AVK ? A.ViscPara : nullptr,
                                                              AVK ? active : nullptr, seq);
         };
         auto by_kind = [&](auto av_tag) {
@@ -411,7 +425,7 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
 }
 
 void FusedStage::release(esdg_ctx* ctx) {
-    for (double** p : {&qn, &fstar, &fgrad, &grad, &fvisc, &qvisc, &tr})
+    for (double** p : {&qn, &fstar, &grad, &fvisc, &qvisc, &tr})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
     if (active) { esdg_free(ctx, active); active = nullptr; }
     seq = 0;
@@ -420,7 +434,6 @@ void FusedStage::release(esdg_ctx* ctx) {
 
 void* FusedStage::buffer(const std::string& name) {
     if (name == "FStar") return fstar;
-    if (name == "FGrad") return fgrad;
     if (name == "Grad") return grad;
     if (name == "FVisc") return fvisc;
     if (name == "QVisc") return qvisc;
