@@ -317,24 +317,25 @@ struct SensorConsts {
 
 // One of the four modal energy sums of ShockCapturing.okl:145-196, in the okl's serial order:
 // which = 0: zaehler1 (top shell), 1: zaehler2 (second shell), 2: nenner1 (all), 3: nenner2 (below N).
-// RS: row stride of the mode array (m[i][j] at m[i * RS + j]).
-template <int NGL, int RS = NGL>
+// RS: row stride of the mode array (m[i][j] at m[i * RS + j]).  SQUARED: m holds the squares of
+// the modes already (each square is the same rounded product, so the sums are bit-identical).
+template <int NGL, int RS = NGL, bool SQUARED = false>
 __device__ __forceinline__ double sensor_sum(const double* m, int which) {
     constexpr int N = NGL - 1;
+    auto sq = [&](int k) { return SQUARED ? m[k] : m[k] * m[k]; };
     double s = 0.0;
     if (which == 0) {
-        for (int i = 0; i < N; ++i) s += m[i * RS + N] * m[i * RS + N] + m[N * RS + i] * m[N * RS + i];
-        s += m[N * RS + N] * m[N * RS + N];
+        for (int i = 0; i < N; ++i) s += sq(i * RS + N) + sq(N * RS + i);
+        s += sq(N * RS + N);
     } else if (which == 1) {
-        for (int i = 0; i < N - 1; ++i)
-            s += m[i * RS + N - 1] * m[i * RS + N - 1] + m[(N - 1) * RS + i] * m[(N - 1) * RS + i];
-        s += m[(N - 1) * RS + N - 1] * m[(N - 1) * RS + N - 1];
+        for (int i = 0; i < N - 1; ++i) s += sq(i * RS + N - 1) + sq((N - 1) * RS + i);
+        s += sq((N - 1) * RS + N - 1);
     } else if (which == 2) {
         for (int i = 0; i < NGL; ++i)
-            for (int j = 0; j < NGL; ++j) s += m[i * RS + j] * m[i * RS + j];
+            for (int j = 0; j < NGL; ++j) s += sq(i * RS + j);
     } else {
         for (int i = 0; i < N; ++i)
-            for (int j = 0; j < N; ++j) s += m[i * RS + j] * m[i * RS + j];
+            for (int j = 0; j < N; ++j) s += sq(i * RS + j);
     }
     return s;
 }
