@@ -256,9 +256,9 @@ class deviceclass {
         }
         // q may have been set since the last call: its wave speeds and viscosity are evaluated anew
         lambda_fresh = visc_fresh = false;
-        // Without the limiter the fused state update evaluates the sensor (every stage) and the
-        // wave speeds (last stage) of the state it writes, in place of the separate passes.
-        const bool fuse_next = rp_.fused == 1 && !rp_.positivity;
+        // The fused state update (or the limiter after it) evaluates the sensor (every stage) and
+        // the wave speeds (last stage) of the state it writes, in place of the separate passes.
+        const bool fuse_next = rp_.fused == 1;
         const bool fuse_sensor = fuse_next && rp_.artificial_viscosity && !ind_;
         int n = 0;
         while (t < rp_.T && n < max_steps) {

@@ -33,7 +33,8 @@ namespace esdg {
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
                  const double* Qv, const double* SPV, double* tr, const int* active = nullptr, int seq = 0);
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
-                    double* tr);
+                    double* tr, double* next_visc, double* next_visc_plot, const double* B, const double* VdmInv,
+                    double* next_lambda);
 int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr);
 
 // calcNumFluxesGradient.okl:16-135 at face node i: the BR1 face fluxes of u and v (x and y parts)
@@ -401,7 +402,9 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         // with the limiter on, the limiter writes the traces of the limited state
         double* vtr = A.positivity ? nullptr : tr;
         int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr, vtr);
-        if (!rc && A.positivity) rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr);
+        if (!rc && A.positivity)
+            rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, A.next_visc, A.next_visc_plot, A.B,
+                                 A.VdmInv, A.next_lambda);
         return rc;
     }
     int rc = ESDG_OK;
@@ -420,7 +423,9 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     }
     // volume + surface + viscous terms + RK in one element pass, then the limiter
     rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, active, seq);
-    if (!rc && A.positivity) rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr);
+    if (!rc && A.positivity)
+        rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, A.next_visc, A.next_visc_plot, A.B,
+                             A.VdmInv, A.next_lambda);
     return rc;
 }
 

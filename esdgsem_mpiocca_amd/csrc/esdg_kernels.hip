@@ -423,17 +423,19 @@ __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts
 // three threads (h, hu, hv), each over the node products in the okl's serial order (xi outer, eta
 // inner); the products h w, w = GLw[xi] GLw[eta] / Jinv, are formed by the row threads.  With tr
 // (the fused schedule), the side traces of the limited state are written as well, from LDS and
-// lane-consecutively.
+// lane-consecutively.  With nx.visc / nx.lam (the fused schedule), ShockCapturing and
+// FindLambdaMax of the limited state follow, with the arithmetic of k_shock_capturing and
+// k_lambda_max, so that the next stage and time step need not run them.
 template <int NGL>
 __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double zero_tol,
                                                   const double* __restrict__ EleSizes, const double* __restrict__ GLw,
                                                   const double* __restrict__ Jac, double* __restrict__ q,
-                                                  double* __restrict__ tr) {
+                                                  double* __restrict__ tr, NextEval nx) {
     using RB = RowBlock<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sP[3][RB::EPB * NN];
     __shared__ double sMin[RB::EPB][NGL];
-    __shared__ double sSum[RB::EPB][3];
+    __shared__ double sSum[RB::EPB][4];
     const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
     const int e = blockIdx.x * RB::EPB + el;
     const bool act = e < Nelem;
@@ -521,6 +523,72 @@ __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double 
             const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : k;
             const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : k;
             T[t] = sP[v][el2 * NN + row * NGL + col];
+        }
+    }
+    if (nx.visc) {
+        // modal sensor of the limited row values: T = V (h + b) column r, then row r of m = T V^T,
+        // in plane 1 (unpadded rows), as k_shock_capturing does
+        __shared__ double sV[NN];
+        __syncthreads();   // the planes are no longer read
+        if (threadIdx.x < NN) sV[threadIdx.x] = nx.V[threadIdx.x];
+        double* Tm = sP[1] + el * NN;
+        double u[NGL];
+        if (act) {
+            const int64_t xb = (int64_t)e * NN + r * NGL;
+#pragma unroll
+            for (int i = 0; i < NGL; ++i) u[i] = nx.B[xb + i] + h[i];
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int a = 0; a < NGL; ++a) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < NGL; ++l) acc += sV[a * NGL + l] * u[l];
+                Tm[a * NGL + r] = acc;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            double t[NGL];
+#pragma unroll
+            for (int l = 0; l < NGL; ++l) t[l] = Tm[r * NGL + l];
+#pragma unroll
+            for (int c = 0; c < NGL; ++c) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < NGL; ++l) acc += t[l] * sV[c * NGL + l];
+                Tm[r * NGL + c] = acc;
+            }
+        }
+        __syncthreads();
+        if (act && r < 4) sSum[el][r] = sensor_sum<NGL, NGL>(Tm, r);
+        __syncthreads();
+        if (act && r == 0) {
+            const double v = sensor_ramp(sSum[el][0], sSum[el][1], sSum[el][2], sSum[el][3], nx.sc);
+            nx.visc[e] = v;
+            if (nx.visc_plot) nx.visc_plot[e] = v;
+        }
+    }
+    if (nx.lam) {
+        // FindLambdaMax.okl:41-60 over the limited row, then the element maximum (order-free)
+        double lam = 0.0;
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < NGL; ++i) {
+                const double Qinv = (h[i] > tol) ? 1.0 / h[i] : 0.0;
+                const double c = sqrt(nx.g * h[i]);
+                lam = fmax(lam, c + fmax(fabs(hu[i] * Qinv), fabs(hv[i] * Qinv)));
+            }
+        }
+        __syncthreads();   // sMin is no longer read
+        if (act) sMin[el][r] = lam;
+        __syncthreads();
+        if (act && r == 0) {
+            double m = 0.0;
+#pragma unroll
+            for (int k = 0; k < NGL; ++k) m = fmax(m, sMin[el][k]);
+            nx.lam[e] = m;
         }
     }
 }
@@ -763,21 +831,33 @@ void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, c
 }
 template <int NGL>
 void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q,
-                    double* tr) {
+                    double* tr, NextEval nx) {
     using RB = RowBlock<NGL>;
     k_pos_pres<NGL><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.PosPresTOL, ctx->p.ZeroTOL,
-                                                                                  sizes, GLw, Jac, q, tr);
+                                                                                  sizes, GLw, Jac, q, tr, nx);
 }
 template <int NGL>
 void launch_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr) {
     k_write_traces<NGL><<<blocks_for((int64_t)Nelem * NGL * NGL, 256), 256, 0, ctx->stream>>>(Nelem, q, tr);
 }
 
-// limiter that also refreshes the side traces (fused schedule), and the trace writer
+// limiter that also refreshes the side traces (fused schedule) and, when asked, evaluates the
+// sensor and the wave speeds of the limited state
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
-                    double* tr) {
+                    double* tr, double* next_visc, double* next_visc_plot, const double* B, const double* VdmInv,
+                    double* next_lambda) {
+    NextEval nx{};
+    if (next_visc) {
+        nx.visc = next_visc;
+        nx.visc_plot = next_visc_plot;
+        nx.B = B;
+        nx.V = VdmInv;
+        nx.sc = SensorConsts{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
+    }
+    nx.lam = next_lambda;
+    nx.g = ctx->p.g;
     KernelScope ks(ctx, "pos_pres");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, tr);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, tr, nx);
     return launch_check(ctx, "pos_pres");
 }
 // the modal sensor with the learned indicator's classes (zero viscosity where estat = 0)
@@ -965,7 +1045,7 @@ int esdg_pos_pres(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double
     ESDG_REQUIRE(ctx, Nelem >= 0 && EleSizes && GLw && Jac && q, "esdg_pos_pres: bad argument");
     if (Nelem == 0) return ESDG_OK;
     ESDG_ENTRY("pos_pres");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, nullptr);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, nullptr, NextEval{});
     return launch_check(ctx, "pos_pres");
 }
 

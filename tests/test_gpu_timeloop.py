@@ -187,13 +187,14 @@ def test_fused_equals_reference_sequence_bitwise(dev, av, pos):
     assert np.array_equal(qa, qb) or np.abs(qa - qb).max() <= 1e-15 * np.abs(qa).max()
 
 
-@pytest.mark.parametrize("av", [0, 1])
-def test_fused_update_evaluates_sensor_and_wave_speed(dev, av):
-    """Without the limiter the fused state update also evaluates ShockCapturing (every stage) and
-    FindLambdaMax (last stage) of the state it writes: after a run, ViscPara, ViscForPlot and
-    LambdaMax equal those kernels applied to the final state, bit for bit."""
-    case = Case.dam_break(N=7, nel=8, noise=1e-3, seed=5)
-    rp = RunParams(num_flux=3, artificial_viscosity=av, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5)
+@pytest.mark.parametrize("av,pos", [(0, 0), (1, 0), (1, 1)])
+def test_fused_update_evaluates_sensor_and_wave_speed(dev, av, pos):
+    """The fused state update (with the limiter when it is on) also evaluates ShockCapturing (every
+    stage) and FindLambdaMax (last stage) of the state it writes: after a run, ViscPara,
+    ViscForPlot and LambdaMax equal those kernels applied to the final state, bit for bit."""
+    case = Case.dam_break(N=7, nel=8, noise=1e-3, seed=5) if not pos else Case.flood(N=5, nel=8, seed=2)
+    rp = RunParams(num_flux=3, artificial_viscosity=av, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5,
+                   positivity=pos, pos_pres_tol=1e-3, CFL=0.25 if pos else 0.5)
     E = case.Nelem
     ctx = dev.Context(dev.make_params(case, rp))
     try:
