@@ -53,12 +53,13 @@ def volume_flops(nelem, ngl):
 # viscosity is non-zero, is not counted.)
 def fused_volume_bytes(nelem, ngl, av, traces=True, sensor=False, stages=5):
     """Average bytes per launch.  Per node: q 3, Yxi/Yeta/Xxi/Xeta 4, 1/J, Bx, By, gRK 3 read, gRK 3
-    and q 3 written; per element side node: F* 3 (and the viscous face flux 2 with AV) read, and
-    without the limiter the side traces of the new state (3) written.  Without the limiter the pass
-    also evaluates, on the new state, the sensor when `sensor` (b read per node; the viscosity and
-    its plot copy written per element) and, in the last of the step's `stages`, the element wave
-    speed (one double per element)."""
-    per_elem = 19 * ngl * ngl + (12 + (8 if av else 0) + (12 if traces else 0)) * ngl
+    and q 3 written; per element side node: F* 3 read, and without the limiter the side traces of
+    the new state (3) written.  Without the limiter the pass also evaluates, on the new state, the
+    sensor when `sensor` (b read per node; the viscosity and its plot copy written per element) and,
+    in the last of the step's `stages`, the element wave speed (one double per element).  The
+    viscous terms (with `av`) are read only for the elements with viscosity in reach, a few percent
+    of them, and are not counted."""
+    per_elem = 19 * ngl * ngl + (12 + (12 if traces else 0)) * ngl
     if traces and sensor:
         per_elem += ngl * ngl + 2
     if traces:
