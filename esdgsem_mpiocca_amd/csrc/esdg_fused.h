@@ -48,6 +48,8 @@ class FusedStage {
     double* qvisc = nullptr;    // viscous volume term [e][2][ngl^2] (hu, hv) (AV)
     double* tr = nullptr;       // side traces of the state [e][side][3][ngl]
     int* active = nullptr;      // == seq: the element has viscosity in reach this stage (AV)
+    int* alist = nullptr;       // the elements marked this stage, each once, in marking order (AV)
+    int* acount = nullptr;      // length of alist (the viscous face pass resets it) (AV)
     int seq = 0;                // stage counter for active
 };
 

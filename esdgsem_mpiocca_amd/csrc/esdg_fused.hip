@@ -72,7 +72,8 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
                                                     const double* __restrict__ scal, const double* __restrict__ BL,
                                                     const double* __restrict__ BR, double* __restrict__ SP,
                                                     const double* __restrict__ visc, int* __restrict__ active,
-                                                    int seq) {
+                                                    int seq, int Nelem, int* __restrict__ alist,
+                                                    int* __restrict__ acount) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)Nfaces * ngl) return;
     const int f = (int)(t / ngl), i = (int)(t % ngl);
@@ -98,20 +99,19 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
         if (i == 0) {
             const bool hasL = cpuL == 0, hasR = cpuR == 0;
             if ((hasL && visc[E[0]] != 0.0) || (hasR && visc[E[1]] != 0.0)) {
-                if (hasL) active[E[0]] = seq;
-                if (hasR) active[E[1]] = seq;
+                // the first face to mark an element this stage also appends it to the stage's list
+                auto mark = [&](int e) {
+                    if (atomicExch(&active[e], seq) != seq) {
+                        const int k = atomicAdd(acount, 1);
+                        if (k < Nelem) alist[k] = e;
+                    }
+                };
+                if (hasL) mark(E[0]);
+                if (hasR) mark(E[1]);
             }
         }
     }
 }
-
-// Element blocks of the AV pass per chunk: a workgroup reads the marks of a chunk at once and runs
-// only the marked element blocks, so that unmarked ones cost no launch.
-template <int NGL>
-struct AvChunk {
-    static constexpr int EPB = ElemBlock<NGL>::EPB;
-    static constexpr int CH = (256 / EPB) < 16 ? (256 / EPB) : 16;
-};
 
 // ---------------------------------------------------------------------------------------------
 // fused_av: calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148 (with the BR1 face fluxes
@@ -125,8 +125,9 @@ struct AvChunk {
 // whose elements and their face neighbours all have zero viscosity has nothing to contribute: its
 // gradient traces feed only faces of zero viscosity (whose fluxes are zero without reading them)
 // and its viscous volume term is skipped where the viscosity is zero.  The face pass marks the
-// others (active[e] == seq); a workgroup takes a chunk of element blocks and runs the marked
-// elements of the blocks that hold one.
+// others (active[e] == seq) and lists each once (alist[0 .. *acount)); the pass runs the listed
+// elements, EPB per workgroup iteration, in list order (each element's results depend on nothing
+// but its own inputs, so the order does not matter).
 template <int NGL>
 __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const double* __restrict__ q,
                                                   const double* __restrict__ Dh, const double* __restrict__ Ds,
@@ -139,38 +140,25 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const d
                                                   const double* __restrict__ scal, const double* __restrict__ BL,
                                                   const double* __restrict__ BR, GatherConsts gc, double geomFace,
                                                   double* __restrict__ gtr, const double* __restrict__ visc,
-                                                  const int* __restrict__ active, int seq, double* __restrict__ Qv) {
+                                                  const int* __restrict__ alist, const int* __restrict__ acount,
+                                                  double* __restrict__ Qv) {
     using EB = ElemBlock<NGL>;
-    using AC = AvChunk<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sDh[NN], sDs[NN];
     __shared__ double sA[EB::THREADS], sB[EB::THREADS], sC[EB::THREADS], sE[EB::THREADS];
     __shared__ double sEps[EB::EPB];
-    __shared__ int sFlag[AC::CH], sList[AC::CH], sCount;
     if (threadIdx.x < NN) {
         sDh[threadIdx.x] = Dh[threadIdx.x];
         sDs[threadIdx.x] = Ds[threadIdx.x];
     }
     const int el = threadIdx.x / NN, n = threadIdx.x % NN;
-    const int nblk = (Nelem + EB::EPB - 1) / EB::EPB;
-    for (int chunk = blockIdx.x; chunk * AC::CH < nblk; chunk += gridDim.x) {
-        // the chunk's element blocks holding a marked element, in a list
-        if (threadIdx.x < AC::CH) sFlag[threadIdx.x] = 0;
-        if (threadIdx.x == 0) sCount = 0;
-        __syncthreads();
-        if (threadIdx.x < AC::CH * EB::EPB) {
-            const int ee = chunk * AC::CH * EB::EPB + threadIdx.x;
-            if (ee < Nelem && active[ee] == seq) sFlag[threadIdx.x / EB::EPB] = 1;
-        }
-        __syncthreads();
-        if (threadIdx.x < AC::CH && sFlag[threadIdx.x]) sList[atomicAdd(&sCount, 1)] = threadIdx.x;
-        __syncthreads();
-        const int cnt = sCount;
-        for (int li = 0; li < cnt; ++li) {
-            // an unmarked element of a marked block has no reader: its gradient traces feed only
-            // faces of zero viscosity and its viscous volume term is skipped
-            const int e = (chunk * AC::CH + sList[li]) * EB::EPB + el;
-            const bool act = e < Nelem && active[e] == seq;
+    const int cnt = min(*acount, Nelem);
+    __syncthreads();
+    {
+        for (int grp = blockIdx.x; grp * EB::EPB < cnt; grp += gridDim.x) {
+            const int idx = grp * EB::EPB + el;
+            const bool act = idx < cnt;
+            const int e = act ? alist[idx] : 0;
             if (act && n == 0) sEps[el] = visc[e];
             const int a = n % NGL, c = n / NGL;           // node n = (xi = a, eta = c)
             const int64_t xid = (int64_t)e * NN + n;
@@ -289,9 +277,10 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl,
                                                             const double* __restrict__ nya,
                                                             const double* __restrict__ scal,
                                                             const int* __restrict__ active, int seq,
-                                                            double* __restrict__ SPV) {
+                                                            int* __restrict__ acount, double* __restrict__ SPV) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)Nfaces * ngl) return;
+    if (t == 0) *acount = 0;   // the AV pass has read the stage's list; the next face pass refills it
     const int f = (int)(t / ngl), i = (int)(t % ngl);
     const int* E = ED + 8 * (int64_t)f;
     const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], rv = E[4], cpuL = E[5], cpuR = E[6];
@@ -324,13 +313,15 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl,
 template <int NGL>
 void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     using EB = ElemBlock<NGL>;
-    using AC = AvChunk<NGL>;
+    // the list length is known on the device only: enough workgroups for a few percent of the
+    // elements in one pass, the rest loop
     const int64_t nblk = (A.Nelem + EB::EPB - 1) / EB::EPB;
-    const unsigned grid = (unsigned)std::min<int64_t>((nblk + AC::CH - 1) / AC::CH, 8192);
+    const unsigned grid = (unsigned)std::max<int64_t>(std::min<int64_t>(nblk, 4096), 1);
     k_fused_av<NGL><<<grid, EB::THREADS, 0, ctx->stream>>>(
         A.Nelem, ctx->p.PosPresTOL, A.q, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
         A.ElemToEdge, A.EMS, A.EOR, A.EdgeData, F.tr, A.nx, A.ny, A.scal, A.bL, A.bR,
-        GatherConsts{ctx->p.bc_kind, ctx->p.h_zero}, ctx->p.geomFace, F.grad, A.ViscPara, F.active, F.seq, F.qvisc);
+        GatherConsts{ctx->p.bc_kind, ctx->p.h_zero}, ctx->p.geomFace, F.grad, A.ViscPara, F.alist, F.acount,
+        F.qvisc);
 }
 
 
@@ -341,6 +332,8 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     int rc = esdg_malloc(ctx, (void**)&fstar, fn * 3 * sizeof(double));
     if (!rc) rc = esdg_malloc(ctx, (void**)&tr, (size_t)Ne * 12 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&active, ((size_t)Ne + 1) / 2 * sizeof(double));
+    if (!rc && av) rc = esdg_malloc(ctx, (void**)&alist, ((size_t)Ne + 1) / 2 * sizeof(double));
+    if (!rc && av) rc = esdg_malloc(ctx, (void**)&acount, 256);
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * 16 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fvisc, fn * 2 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&qvisc, (size_t)Ne * 2 * ngl * ngl * sizeof(double));
@@ -355,6 +348,11 @@ int FusedStage::setup(esdg_ctx*, const int*, const int*, const int*, const int*,
 
 int FusedStage::begin(esdg_ctx* ctx, const double* q) {
     if (!ready) return set_error(ctx, ESDG_ERR_STATE, "fused stage used before init");
+    // an interrupted stage may have left a list behind
+    if (acount) {
+        int rc = esdg_fill(ctx, reinterpret_cast<double*>(acount), 0.0, 1);
+        if (rc) return rc;
+    }
     return write_traces(ctx, Nelem, q, tr);
 }
 
@@ -381,7 +379,8 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
             constexpr int KK = decltype(kind_tag)::value;
             k_fused_face<AVK, KK><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, fc, gc, A.EdgeData, tr, A.nx, A.ny,
                                                              A.scal, A.bL, A.bR, fstar, AVK ? A.ViscPara : nullptr,
-                                                             AVK ? active : nullptr, seq);
+                                                             AVK ? active : nullptr, seq, A.Nelem,
+                                                             AVK ? alist : nullptr, AVK ? acount : nullptr);
         };
         auto by_kind = [&](auto av_tag) {
             switch (kind) {
@@ -417,7 +416,8 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     {
         KernelScope ks(ctx, "fused_viscous_face");
         k_fused_viscous_face<<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, ctx->p.geomFace, A.EdgeData, grad,
-                                                          A.ViscPara, A.nx, A.ny, A.scal, active, seq, fvisc);
+                                                          A.ViscPara, A.nx, A.ny, A.scal, active, seq, acount,
+                                                          fvisc);
         rc = launch_check(ctx, "fused_viscous_face");
         if (rc) return rc;
     }
@@ -432,7 +432,8 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
 void FusedStage::release(esdg_ctx* ctx) {
     for (double** p : {&qn, &fstar, &grad, &fvisc, &qvisc, &tr})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
-    if (active) { esdg_free(ctx, active); active = nullptr; }
+    for (int** p : {&active, &alist, &acount})
+        if (*p) { esdg_free(ctx, *p); *p = nullptr; }
     seq = 0;
     ready = false;
 }
