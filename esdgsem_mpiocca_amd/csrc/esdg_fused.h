@@ -49,7 +49,7 @@ class FusedStage {
     double* tr = nullptr;       // side traces of the state [e][side][3][ngl]
     int* active = nullptr;      // == seq: the element has viscosity in reach this stage (AV)
     int* alist = nullptr;       // the elements marked this stage, each once, in marking order (AV)
-    int* acount = nullptr;      // length of alist (the viscous face pass resets it) (AV)
+    int* acount = nullptr;      // length of alist for stage seq: acount[seq & 1] (AV)
     int seq = 0;                // stage counter for active
 };
 

@@ -93,16 +93,20 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
     SP[eid + ngl] = sc * c.geomFace * F1;
     SP[eid + 2 * ngl] = sc * c.geomFace * F2;
     if (AV) {
+        // the next stage's list length starts at zero (the previous stage, which used that slot,
+        // has finished)
+        if (t == 0) acount[(seq + 1) & 1] = 0;
         // elements with viscosity in reach (on themselves or a face neighbour) need the viscous
         // work: a face with viscosity on either side marks both its elements for this stage
         const int cpuL = E[5], cpuR = E[6];
         if (i == 0) {
             const bool hasL = cpuL == 0, hasR = cpuR == 0;
             if ((hasL && visc[E[0]] != 0.0) || (hasR && visc[E[1]] != 0.0)) {
-                // the first face to mark an element this stage also appends it to the stage's list
+                // the first face to mark an element this stage also appends it to the stage's list;
+                // the list length of stage seq is acount[seq & 1]
                 auto mark = [&](int e) {
                     if (atomicExch(&active[e], seq) != seq) {
-                        const int k = atomicAdd(acount, 1);
+                        const int k = atomicAdd(acount + (seq & 1), 1);
                         if (k < Nelem) alist[k] = e;
                     }
                 };
@@ -266,25 +270,13 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const d
 // ---------------------------------------------------------------------------------------------
 // fused_viscous_face: CollectEdgeDataGradient.okl:17-194 + calcNumFluxesViscose.okl:17-107
 // ---------------------------------------------------------------------------------------------
-// Only elements marked this stage (active[e] == seq: viscosity on themselves or a face neighbour)
-// read these fluxes; every face of an unmarked element has zero viscosity on both sides, so its
-// flux is +-0 and the volume pass skips it.  A face none of whose elements is marked is not written.
-__global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl, double geomFace,
-                                                            const int* __restrict__ ED,
-                                                            const double* __restrict__ gtr,
-                                                            const double* __restrict__ visc,
-                                                            const double* __restrict__ nxa,
-                                                            const double* __restrict__ nya,
-                                                            const double* __restrict__ scal,
-                                                            const int* __restrict__ active, int seq,
-                                                            int* __restrict__ acount, double* __restrict__ SPV) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)Nfaces * ngl) return;
-    if (t == 0) *acount = 0;   // the AV pass has read the stage's list; the next face pass refills it
-    const int f = (int)(t / ngl), i = (int)(t % ngl);
-    const int* E = ED + 8 * (int64_t)f;
-    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], rv = E[4], cpuL = E[5], cpuR = E[6];
-    if (!((cpuL == 0 && active[iel] == seq) || (cpuR == 0 && active[ier] == seq))) return;
+// calcNumFluxesViscose.okl at face node i of face f from the side traces of the scaled gradients
+__device__ __forceinline__ void viscous_face_node(int f, int i, int ngl, double geomFace, int iel, int ier, int isl,
+                                                  int isr, int rv, int cpuL, int cpuR, const double* __restrict__ gtr,
+                                                  const double* __restrict__ visc, const double* __restrict__ nxa,
+                                                  const double* __restrict__ nya, const double* __restrict__ scal,
+                                                  double* __restrict__ SPV) {
+    const int64_t t = (int64_t)f * ngl + i;
     const int64_t eid = (int64_t)f * ngl * 2 + i;
     double N0 = 0.0, N1 = 0.0;
     if (!(cpuL == -1 || cpuR == -1)) {
@@ -310,6 +302,38 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nfaces, int ngl,
     SPV[eid + ngl] = -sc * geomFace * N1;
 }
 
+// Only elements marked this stage (active[e] == seq: viscosity on themselves or a face neighbour)
+// read these fluxes; every face of an unmarked element has zero viscosity on both sides, so its
+// flux is +-0 and the volume pass skips it.  The pass runs over the side nodes of the listed
+// (marked) elements; each face node is evaluated once, by the face's left element if that is
+// marked, else by its right element.  Faces of no marked element are not written.
+__global__ void __launch_bounds__(256) k_fused_viscous_face(int Nelem, int ngl, double geomFace,
+                                                            const int* __restrict__ ED,
+                                                            const int* __restrict__ E2E,
+                                                            const double* __restrict__ gtr,
+                                                            const double* __restrict__ visc,
+                                                            const double* __restrict__ nxa,
+                                                            const double* __restrict__ nya,
+                                                            const double* __restrict__ scal,
+                                                            const int* __restrict__ active, int seq,
+                                                            const int* __restrict__ alist,
+                                                            const int* __restrict__ acount, double* __restrict__ SPV) {
+    const int64_t items = (int64_t)min(*acount, Nelem) * 4 * ngl;
+    for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < items;
+         it += (int64_t)gridDim.x * blockDim.x) {
+        const int li = (int)(it / (4 * ngl)), rem = (int)(it % (4 * ngl));
+        const int side = rem / ngl, i = rem % ngl;
+        const int e = alist[li];
+        const int f = E2E[e * 4 + side] - 1;
+        const int* E = ED + 8 * (int64_t)f;
+        const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], rv = E[4], cpuL = E[5], cpuR = E[6];
+        const int owner = (cpuL == 0 && active[iel] == seq) ? iel : ier;
+        if (e != owner) continue;
+        viscous_face_node(f, i, ngl, geomFace, iel, ier, isl, isr, rv, cpuL, cpuR, gtr, visc, nxa, nya, scal, SPV);
+    }
+}
+
+
 template <int NGL>
 void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     using EB = ElemBlock<NGL>;
@@ -320,8 +344,8 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     k_fused_av<NGL><<<grid, EB::THREADS, 0, ctx->stream>>>(
         A.Nelem, ctx->p.PosPresTOL, A.q, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
         A.ElemToEdge, A.EMS, A.EOR, A.EdgeData, F.tr, A.nx, A.ny, A.scal, A.bL, A.bR,
-        GatherConsts{ctx->p.bc_kind, ctx->p.h_zero}, ctx->p.geomFace, F.grad, A.ViscPara, F.alist, F.acount,
-        F.qvisc);
+        GatherConsts{ctx->p.bc_kind, ctx->p.h_zero}, ctx->p.geomFace, F.grad, A.ViscPara, F.alist,
+        F.acount + (F.seq & 1), F.qvisc);
 }
 
 
@@ -348,7 +372,7 @@ int FusedStage::setup(esdg_ctx*, const int*, const int*, const int*, const int*,
 
 int FusedStage::begin(esdg_ctx* ctx, const double* q) {
     if (!ready) return set_error(ctx, ESDG_ERR_STATE, "fused stage used before init");
-    // an interrupted stage may have left a list behind
+    // an interrupted stage may have left a list behind (both list lengths: 2 ints)
     if (acount) {
         int rc = esdg_fill(ctx, reinterpret_cast<double*>(acount), 0.0, 1);
         if (rc) return rc;
@@ -415,9 +439,10 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     }
     {
         KernelScope ks(ctx, "fused_viscous_face");
-        k_fused_viscous_face<<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, ctx->p.geomFace, A.EdgeData, grad,
-                                                          A.ViscPara, A.nx, A.ny, A.scal, active, seq, acount,
-                                                          fvisc);
+        const unsigned vb = (unsigned)std::max<int64_t>(std::min<int64_t>(((int64_t)A.Nelem * 4 * ngl + 255) / 256, 2048), 1);
+        k_fused_viscous_face<<<vb, 256, 0, ctx->stream>>>(A.Nelem, ngl, ctx->p.geomFace, A.EdgeData, A.ElemToEdge,
+                                                          grad, A.ViscPara, A.nx, A.ny, A.scal, active, seq, alist,
+                                                          acount + (seq & 1), fvisc);
         rc = launch_check(ctx, "fused_viscous_face");
         if (rc) return rc;
     }
