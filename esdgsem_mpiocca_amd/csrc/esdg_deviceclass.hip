@@ -155,10 +155,8 @@ class deviceclass {
     // where the network does not settle the element as smooth.
     int viscosity() {
         if (!ind_) return esdg_shock_capturing(ctx_, Nelem, o_q, o_B, o_VdmInv, o_EleSizes, o_ViscPara, o_ViscForPlot);
-        int rc = mlp_classify(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, nullptr, o_estat);
-        if (rc == ESDG_OK)
-            rc = shock_capturing_masked(ctx_, Nelem, o_q, o_B, o_VdmInv, o_estat, ind_audit, o_icount, o_ViscPara,
-                                        o_ViscForPlot);
+        const int rc = mlp_classify_sensor(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, o_estat, ind_audit,
+                                           o_icount, o_ViscPara, o_ViscForPlot);
         ind_element_stages += Nelem;
         return rc;
     }

@@ -317,12 +317,7 @@ __global__ void __launch_bounds__(256) k_update_ssp(int64_t n, double A, double 
 // T[a][c] = sum_l V[a][l] u(xi = l, eta = c), u = h + b, each sum over l in ascending order as in
 // the okl; thread r forms column r of T from its own row of u, then row r of m from row r of T.
 // The four energy sums of an element run on four threads, each in the okl's serial order.
-//
-// MASK (learned indicator): estat[e] = 0 (the network settled the element as smooth) gives zero
-// viscosity; 1 and 2 keep the sensor's value.  Without audit the sensor is evaluated only in
-// blocks that hold a non-smooth element.  counters: [0] element-stages with viscosity,
-// [1] sensor positives declared smooth and [2] sensor positives (both audit only),
-// [3] element-stages the network did not settle as smooth.
+// (With the learned indicator the indicator kernel evaluates the sensor itself, esdg_mlp.hip.)
 // ---------------------------------------------------------------------------------------------
 template <int NGL>
 struct RowBlock {
@@ -332,34 +327,18 @@ struct RowBlock {
     static constexpr int ES = NGL * RS + 1;          // padded LDS stride between elements
 };
 
-template <int NGL, bool MASK>
+template <int NGL>
 __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts sc, const double* __restrict__ q,
                                                          const double* __restrict__ b, const double* __restrict__ VdmInv,
-                                                         const signed char* __restrict__ estat, int audit,
-                                                         unsigned long long* __restrict__ counters,
                                                          double* __restrict__ visc, double* __restrict__ visc_plot) {
     using RB = RowBlock<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sV[NN];
     __shared__ double sT[RB::EPB * RB::ES];
     __shared__ double sSum[RB::EPB][4];
-    __shared__ int sNeed;
     const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
     const int e = blockIdx.x * RB::EPB + el;
     const bool act = (el < RB::EPB) && (e < Nelem);
-    if (MASK) {
-        if (threadIdx.x == 0) sNeed = audit;
-        __syncthreads();
-        if (act && r == 0 && estat[e] != 0) sNeed = 1;
-        __syncthreads();
-        if (!sNeed) {
-            if (act && r == 0) {
-                visc[e] = 0.0;
-                if (visc_plot) visc_plot[e] = 0.0;
-            }
-            return;
-        }
-    }
     if (threadIdx.x < NN) sV[threadIdx.x] = VdmInv[threadIdx.x];
     double u[NGL];
     if (act) {
@@ -399,18 +378,7 @@ __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts
     __syncthreads();
     if (act && r == 0) {
         const double* S4 = sSum[el];
-        double v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
-        if (MASK) {
-            const int st = estat[e];
-            const double plain = v;
-            if (st == 0) v = 0.0;
-            if (counters) {
-                if (v > 0.0) atomicAdd(counters, 1ull);
-                if (audit && plain > 0.0 && st == 0) atomicAdd(counters + 1, 1ull);
-                if (audit && plain > 0.0) atomicAdd(counters + 2, 1ull);
-                if (st != 0) atomicAdd(counters + 3, 1ull);
-            }
-        }
+        const double v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
         visc[e] = v;
         if (visc_plot) visc_plot[e] = v;
     }
@@ -818,16 +786,11 @@ void launch_lambda(esdg_ctx* ctx, int Nelem, const double* q, double* LM) {
     k_lambda_max<NGL><<<blocks_for(Nelem, EB::EPB), EB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.g, ctx->p.PosPresTOL, q, LM);
 }
 template <int NGL>
-void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* V,
-                   const signed char* estat, int audit, unsigned long long* counters, double* visc, double* vp) {
+void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* V, double* visc,
+                   double* vp) {
     using RB = RowBlock<NGL>;
     SensorConsts sc{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
-    if (estat)
-        k_shock_capturing<NGL, true><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(
-            Nelem, sc, q, B, V, estat, audit, counters, visc, vp);
-    else
-        k_shock_capturing<NGL, false><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(
-            Nelem, sc, q, B, V, nullptr, 0, nullptr, visc, vp);
+    k_shock_capturing<NGL><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(Nelem, sc, q, B, V, visc, vp);
 }
 template <int NGL>
 void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q,
@@ -859,14 +822,6 @@ int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const doub
     KernelScope ks(ctx, "pos_pres");
     ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, tr, nx);
     return launch_check(ctx, "pos_pres");
-}
-// the modal sensor with the learned indicator's classes (zero viscosity where estat = 0)
-int shock_capturing_masked(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* VdmInv,
-                           const signed char* estat, int audit, unsigned long long* counters, double* visc,
-                           double* visc_plot) {
-    KernelScope ks(ctx, "shock_capturing");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_sensor, ctx, Nelem, q, B, VdmInv, estat, audit, counters, visc, visc_plot);
-    return launch_check(ctx, "shock_capturing");
 }
 int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr) {
     KernelScope ks(ctx, "write_traces");
@@ -1036,7 +991,7 @@ int esdg_shock_capturing(esdg_ctx* ctx, int Nelem, const double* q, const double
     ESDG_REQUIRE(ctx, ctx->p.ngl >= 3, "esdg_shock_capturing: needs ngl >= 3");
     if (Nelem == 0) return ESDG_OK;
     ESDG_ENTRY("shock_capturing");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_sensor, ctx, Nelem, q, B, VdmInv, nullptr, 0, nullptr, ViscPara, ViscForPlot);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_sensor, ctx, Nelem, q, B, VdmInv, ViscPara, ViscForPlot);
     return launch_check(ctx, "shock_capturing");
 }
 

@@ -5,6 +5,7 @@
 #include <new>
 #include <vector>
 
+#include "esdg_device.h"
 #include "esdg_internal.h"
 #include "esdg_mlp.h"
 #include "esdg_mlp_internal.h"
@@ -229,10 +230,24 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, const float* __restric
 // Features + inference fused over the elements of a state; p per element and, when estat is
 // given, the class: 0 smooth (p <= lo), 1 troubled (p >= hi), 2 unsure.  A block takes 32 elements:
 // features by row threads into two 16-row BF16 tiles, then waves 0 and 1 run one tile each.
+// With MS.visc (and estat), the block then finishes ShockCapturing masked by the classes: the
+// feature pass left the modal coefficients m = V (h + b) V^T in LDS, formed with the sensor's
+// arithmetic (ShockCapturing.okl:63-129), so four threads per element form the energy sums from
+// them in the okl's order; the viscosity is the sensor's value, or 0 where the class is smooth.
+// counters (may be null): [0] element-stages with viscosity, [1] sensor positives declared smooth
+// and [2] sensor positives (both with audit only), [3] element-stages not declared smooth.
+struct MaskedSensor {
+    double* visc = nullptr;
+    double* visc_plot = nullptr;
+    SensorConsts sc{};
+    int audit = 0;
+    unsigned long long* counters = nullptr;
+};
+
 __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* __restrict__ q,
                                                        const double* __restrict__ B, const double* __restrict__ V,
                                                        MlpDev W, float lo, float hi, float* __restrict__ p,
-                                                       signed char* __restrict__ estat) {
+                                                       signed char* __restrict__ estat, MaskedSensor MS) {
     __shared__ __attribute__((aligned(16))) uint16_t sW1[NH * KP], sW2[NH * NH];
     __shared__ float sw3[NH], sb[2 * NH + 1];
     __shared__ __attribute__((aligned(16))) uint16_t sX[2][16 * KP], sH[2][16 * NH];
@@ -255,14 +270,37 @@ __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* 
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __shared__ signed char sSt[FEPB];
     if (w < 2) {
         mlp_tile(sX[w], sH[w], sW1, sW2, sw3, sb, sb + NH, sb[2 * NH], sZ[w]);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         const int ee = blockIdx.x * FEPB + w * 16 + l;
         if (l < 16 && ee < Nelem) {
             const float pv = sigmoidf_(sZ[w][l]);
+            const signed char st = (pv <= lo) ? 0 : (pv >= hi ? 1 : 2);
             if (p) p[ee] = pv;
-            if (estat) estat[ee] = (pv <= lo) ? 0 : (pv >= hi ? 1 : 2);
+            if (estat) estat[ee] = st;
+            sSt[w * 16 + l] = st;
+        }
+    }
+    if (MS.visc) {
+        __shared__ double sSum[FEPB][4];
+        __syncthreads();   // the classes are in sSt
+        if (act && r < 4) sSum[el][r] = sensor_sum<8, FRS>(L.T + el * FES, r);
+        __syncthreads();
+        if (act && r == 0) {
+            double v = sensor_ramp(sSum[el][0], sSum[el][1], sSum[el][2], sSum[el][3], MS.sc);
+            const int st = sSt[el];
+            const double plain = v;
+            if (st == 0) v = 0.0;
+            if (MS.counters) {
+                if (v > 0.0) atomicAdd(MS.counters, 1ull);
+                if (MS.audit && plain > 0.0 && st == 0) atomicAdd(MS.counters + 1, 1ull);
+                if (MS.audit && plain > 0.0) atomicAdd(MS.counters + 2, 1ull);
+                if (st != 0) atomicAdd(MS.counters + 3, 1ull);
+            }
+            MS.visc[e] = v;
+            if (MS.visc_plot) MS.visc_plot[e] = v;
         }
     }
 }
@@ -434,7 +472,24 @@ namespace esdg {
 int mlp_classify(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B, const double* VdmInv,
                  float lo, float hi, float* p, signed char* estat) {
     KernelScope ks(ctx, "mlp_indicator");
-    k_mlp_indicator<<<blocks_for(Nelem, FEPB), 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, m->dev(), lo, hi, p, estat);
+    k_mlp_indicator<<<blocks_for(Nelem, FEPB), 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, m->dev(), lo, hi, p, estat,
+                                                                       MaskedSensor{});
+    return launch_check(ctx, "mlp_indicator");
+}
+
+int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B,
+                        const double* VdmInv, float lo, float hi, signed char* estat, int audit,
+                        unsigned long long* counters, double* visc, double* visc_plot) {
+    if (ctx->p.ngl != 8) return set_error(ctx, ESDG_ERR_ARG, "the learned indicator needs N = 7");
+    MaskedSensor MS;
+    MS.visc = visc;
+    MS.visc_plot = visc_plot;
+    MS.sc = SensorConsts{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
+    MS.audit = audit;
+    MS.counters = counters;
+    KernelScope ks(ctx, "mlp_indicator");
+    k_mlp_indicator<<<blocks_for(Nelem, FEPB), 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, m->dev(), lo, hi, nullptr,
+                                                                       estat, MS);
     return launch_check(ctx, "mlp_indicator");
 }
 }  // namespace esdg

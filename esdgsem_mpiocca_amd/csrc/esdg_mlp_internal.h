@@ -10,12 +10,14 @@ namespace esdg {
 // null): 0 smooth (p <= lo), 1 troubled (p >= hi), 2 unsure.
 int mlp_classify(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B, const double* VdmInv,
                  float lo, float hi, float* p, signed char* estat);
-// The modal sensor (esdg_shock_capturing) with zero viscosity where estat[e] == 0; the sensor is
-// evaluated only in element blocks holding a non-smooth element unless audit is set.  counters
+// mlp_classify, and in the same pass the modal sensor with zero viscosity where the class is smooth
+// (from the modal coefficients the features already formed).  estat as mlp_classify; counters
 // (may be null, 4 entries, accumulated): [0] element-stages with viscosity, [1] sensor positives
 // declared smooth (audit), [2] sensor positives (audit), [3] element-stages not declared smooth.
-int shock_capturing_masked(esdg_ctx* ctx, int Nelem, const double* q, const double* B, const double* VdmInv,
-                           const signed char* estat, int audit, unsigned long long* counters, double* visc,
-                           double* visc_plot);
+// N = 7 only.
+int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B,
+                        const double* VdmInv, float lo, float hi, signed char* estat, int audit,
+                        unsigned long long* counters, double* visc, double* visc_plot);
+
 
 }  // namespace esdg
