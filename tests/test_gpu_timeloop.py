@@ -219,6 +219,23 @@ def test_fused_update_evaluates_sensor_and_wave_speed(dev, av, pos):
         ctx.close()
 
 
+@pytest.mark.parametrize("nel", [2, 8])
+def test_fused_every_element_viscous(dev, nel):
+    """sigma_max below every element's sensor value: every element has viscosity, so the face pass
+    lists all of them (the list is full) and the AV and viscous face passes run everywhere; the
+    fused schedule still equals the reference sequence.  nel = 2: each element is its own
+    neighbour's neighbour on several sides."""
+    case = Case.dam_break(N=7, nel=nel, noise=1e-3, seed=5)
+    rp = RunParams(num_flux=3, artificial_viscosity=1, eps0=0.002, sigma_min=-40.0, sigma_max=-39.0, CFL=0.5)
+    visc = np.zeros(case.Nelem)
+    O.call("shock_capturing", O.params(case, rp), case.Nelem, case["q"], case["b"], case["VdmInv"], visc, None)
+    assert (visc == rp.eps0).all()
+    qa, na, ta = run_gpu(dev, case, rp, 6, fused=0)
+    qb, nb, tb = run_gpu(dev, case, rp, 6, fused=1)
+    assert na == nb == 6 and ta == tb
+    assert np.array_equal(qa, qb) or np.abs(qa - qb).max() <= 1e-15 * np.abs(qa).max()
+
+
 @pytest.mark.parametrize("pos", [0, 1])
 def test_single_kernel_equals_fused_bitwise(dev, pos):
     """The single-kernel stage (device dt, ping-pong state, graph replay) performs the fused
