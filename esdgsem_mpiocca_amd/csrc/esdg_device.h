@@ -325,6 +325,7 @@ struct NextEval {
     SensorConsts sc{};
     double* lam = nullptr;
     double g = 0.0;
+    const double* dtp = nullptr;   // the device clock's dt: a kernel given dtp does nothing when it is 0
 };
 
 // One of the four modal energy sums of ShockCapturing.okl:145-196, in the okl's serial order:

@@ -65,6 +65,7 @@ class deviceclass {
             // single-kernel stage: second state buffer (ping-pong), device clock, lambda_max cell
             A(&o_q2, nodes * 3); A(&o_clock, 8); A(reinterpret_cast<double**>(&o_lmax), 1);
         }
+        if (rp.fused == 1) A(&o_clock, 8);   // the fused schedule's device clock
         return rc;
     }
 
@@ -251,6 +252,9 @@ class deviceclass {
         if (rp_.fused == 1) {
             rc = fused_.begin(ctx_, o_q);
             if (rc) return rc;
+            // without per-step host work (analysis buffers, the learned indicator's counts) dt and
+            // t stay on the device and the host runs ahead
+            if (!rp_.arrival_times && !rp_.maximum_elevation && !ind_) return clock_timeloop(max_steps, steps_taken);
         }
         // q may have been set since the last call: its wave speeds and viscosity are evaluated anew
         lambda_fresh = visc_fresh = false;
@@ -297,6 +301,50 @@ class deviceclass {
             if (rc) return rc;
         }
         if (steps_taken) *steps_taken = n;
+        return esdg_finish(ctx_);
+    }
+
+    // The fused schedule with dt and t on the device (fused_clock): steps are issued in chunks of
+    // up to 64 without a host synchronisation and the clock is read back after each chunk.  Steps
+    // issued after t reaches T change nothing and are not counted.  Same arithmetic as DGtimeloop.
+    int clock_timeloop(int max_steps, int* steps_taken) {
+        lambda_fresh = visc_fresh = false;
+        const bool fuse_sensor = rp_.artificial_viscosity != 0;
+        const double dti = rp_.min_ele_size / (ngl) * rp_.CFL;
+        const double dvc = rp_.DFL / (std::pow(ngl, 2)) * std::pow(rp_.min_ele_size, 2);
+        double ck[8] = {0.0, t, 0.0, last_dt, 0.0, 0.0, 0.0, 0.0};
+        int rc = esdg_memcpy(ctx_, o_clock, ck, sizeof ck, ESDG_H2D);
+        int issued = 0;
+        while (!rc && issued < max_steps && ck[1] < rp_.T) {
+            const int chunk = std::min(max_steps - issued, 64);
+            for (int k = 0; k < chunk && !rc; ++k) {
+                rc = lambda_fresh ? ESDG_OK : esdg_find_lambda_max(ctx_, Nelem, o_q, o_LambdaMax);
+                if (!rc) rc = esdg_reduce_max(ctx_, Nelem, o_LambdaMax, o_red);
+                if (!rc && rp_.artificial_viscosity) {
+                    if (!visc_fresh) rc = viscosity();
+                    if (!rc) rc = esdg_reduce_max(ctx_, Nelem, o_ViscPara, o_red + 1);
+                }
+                if (!rc) rc = fused_clock(ctx_, o_red, o_clock, dti, dvc, rp_.artificial_viscosity, rp_.T);
+                for (int s = 0; s < stages && !rc; ++s) {
+                    FusedArgs fa = fused_args();
+                    if (fuse_sensor) { fa.next_visc = o_ViscPara; fa.next_visc_plot = o_ViscForPlot; }
+                    if (s == stages - 1) fa.next_lambda = o_LambdaMax;
+                    fa.dt_dev = o_clock;
+                    fa.C_stage = Cc[s];
+                    rc = fused_.stage(ctx_, s, A[s], 0.0, fa);
+                    visc_fresh = fuse_sensor;
+                    lambda_fresh = s == stages - 1;
+                }
+            }
+            issued += chunk;
+            if (!rc) rc = esdg_memcpy(ctx_, ck, o_clock, sizeof ck, ESDG_D2H);
+            if (!rc && ck[4] != 0.0)
+                rc = set_error(ctx_, ESDG_ERR_STATE, "time step is not positive and finite (dt = " + std::to_string(ck[5]) + ")");
+        }
+        if (rc) return rc;
+        t = ck[1];
+        last_dt = ck[3];
+        if (steps_taken) *steps_taken = (int)ck[2];
         return esdg_finish(ctx_);
     }
 

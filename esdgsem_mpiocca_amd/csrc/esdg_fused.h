@@ -22,7 +22,17 @@ struct FusedArgs {
     // each element's value is read before it is replaced) and the element wave speeds into
     // next_lambda.
     double *next_visc = nullptr, *next_visc_plot = nullptr, *next_lambda = nullptr;
+    // Optional: dt from device memory (the device clock: clock[0] is the step's dt, 0 for a step
+    // issued after T), the update using C_stage * dt; a step with dt = 0 changes nothing.
+    const double* dt_dev = nullptr;
+    double C_stage = 0.0;
 };
+
+// The device clock of the fused schedule: clock = {dt, t, steps, last dt, error, bad dt, -, -}.
+// One thread: dt = min(T - t, dti / lambda_max [, dvc / eps_max]) from red = {lambda_max, eps_max}
+// (src/deviceclass.cpp:899-930, with the host's operation order), t += dt; once t >= T, or after
+// a dt that is not positive and finite (flagged), dt = 0.
+int fused_clock(esdg_ctx* ctx, const double* red, double* clock, double dti, double dvc, int av, double T);
 
 class FusedStage {
   public:

@@ -34,7 +34,7 @@ int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double 
                  const double* Qv, const double* SPV, double* tr, const int* active = nullptr, int seq = 0);
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
                     double* tr, double* next_visc, double* next_visc_plot, const double* B, const double* VdmInv,
-                    double* next_lambda);
+                    double* next_lambda, const double* dt_dev);
 int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr);
 
 // calcNumFluxesGradient.okl:16-135 at face node i: the BR1 face fluxes of u and v (x and y parts)
@@ -349,6 +349,32 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
 }
 
 
+__global__ void k_fused_clock(const double* __restrict__ red, double* __restrict__ ck, double dti, double dvc, int av,
+                              double T) {
+    const double t = ck[1];
+    double dt = 0.0;
+    if (t < T && ck[4] == 0.0) {
+        const double dt_i = dti / red[0];
+        dt = av ? fmin(T - t, fmin(dt_i, dvc / red[1])) : fmin(T - t, dt_i);
+        if (!(dt > 0.0) || isinf(dt)) {
+            ck[4] = 1.0;
+            ck[5] = dt;
+            dt = 0.0;
+        } else {
+            ck[1] = t + dt;
+            ck[2] += 1.0;
+            ck[3] = dt;
+        }
+    }
+    ck[0] = dt;
+}
+
+int fused_clock(esdg_ctx* ctx, const double* red, double* clock, double dti, double dvc, int av, double T) {
+    KernelScope ks(ctx, "fused_clock");
+    k_fused_clock<<<1, 1, 0, ctx->stream>>>(red, clock, dti, dvc, av, T);
+    return launch_check(ctx, "fused_clock");
+}
+
 int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     Nelem = Ne; Nfaces = Nf; av = with_av;
     const int ngl = ctx->p.ngl;
@@ -427,7 +453,7 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr, vtr);
         if (!rc && A.positivity)
             rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, A.next_visc, A.next_visc_plot, A.B,
-                                 A.VdmInv, A.next_lambda);
+                                 A.VdmInv, A.next_lambda, A.dt_dev);
         return rc;
     }
     int rc = ESDG_OK;
@@ -450,7 +476,7 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, active, seq);
     if (!rc && A.positivity)
         rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, A.next_visc, A.next_visc_plot, A.B,
-                             A.VdmInv, A.next_lambda);
+                             A.VdmInv, A.next_lambda, A.dt_dev);
     return rc;
 }
 

@@ -399,6 +399,7 @@ __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double 
                                                   const double* __restrict__ EleSizes, const double* __restrict__ GLw,
                                                   const double* __restrict__ Jac, double* __restrict__ q,
                                                   double* __restrict__ tr, NextEval nx) {
+    if (nx.dtp && *nx.dtp == 0.0) return;   // a step issued after T (device clock): nothing changes
     using RB = RowBlock<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sP[3][RB::EPB * NN];
@@ -808,8 +809,9 @@ void launch_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr) {
 // sensor and the wave speeds of the limited state
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
                     double* tr, double* next_visc, double* next_visc_plot, const double* B, const double* VdmInv,
-                    double* next_lambda) {
+                    double* next_lambda, const double* dt_dev) {
     NextEval nx{};
+    nx.dtp = dt_dev;
     if (next_visc) {
         nx.visc = next_visc;
         nx.visc_plot = next_visc_plot;

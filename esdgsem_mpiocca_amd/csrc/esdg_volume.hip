@@ -53,6 +53,10 @@ struct SurfIn {
     // every face neighbour; their viscous terms are +-0 and are not read
     const int* active;
     int seq;
+    // the device clock's dt (may be null): the update uses Cs * dt, and a block with dt = 0 (a step
+    // issued after T) does nothing
+    const double* dtp;
+    double Cs;
 };
 
 template <int NGL, int MODE>
@@ -63,6 +67,14 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                                                         const double* __restrict__ Bx, const double* __restrict__ By,
                                                         double* __restrict__ Qt, SurfIn si) {
     using C = VolCfg<NGL>;
+    double cdt = si.cdt;
+    if constexpr (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
+        if (si.dtp) {
+            const double d = *si.dtp;
+            if (d == 0.0) return;
+            cdt = si.Cs * d;
+        }
+    }
     __shared__ double s[C::EPW * C::ESZ];
     const int lane = threadIdx.x;
     const int el = lane / NGL, r = lane % NGL;
@@ -225,9 +237,9 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 si.gRK[id] = g1;
                 si.gRK[id + C::NN] = g2;
                 si.gRK[id + 2 * C::NN] = g3;
-                const double n1 = q[id] + si.cdt * g1;
-                const double n2 = q[id + C::NN] + si.cdt * g2;
-                const double n3 = q[id + 2 * C::NN] + si.cdt * g3;
+                const double n1 = q[id] + cdt * g1;
+                const double n2 = q[id + C::NN] + cdt * g2;
+                const double n3 = q[id + 2 * C::NN] + cdt * g3;
                 q[id] = n1;
                 q[id + C::NN] = n2;
                 q[id + 2 * C::NN] = n3;
@@ -409,6 +421,8 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
     SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV, rk ? tr : nullptr};
     si.active = active;
     si.seq = seq;
+    si.dtp = a.dt_dev;
+    si.Cs = a.C_stage;
     if (rk && tr) {
         // the updated state's viscosity and wave speed, for the next stage or time step
         si.visc_next = a.next_visc;

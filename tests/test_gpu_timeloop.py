@@ -248,19 +248,23 @@ def test_single_kernel_equals_fused_bitwise(dev, pos):
     assert np.array_equal(qa, qb)
 
 
-def test_single_kernel_stops_at_T_and_resumes(dev):
-    """Device-side clock: steps issued after t reaches T change nothing and are not counted; a
-    second run() continues from the first one's state and time."""
-    case = Case.gaussian_bump(N=4, nel=8)
-    rp_free = RunParams(num_flux=0)
-    _, _, t9 = run_gpu(dev, case, rp_free, 9, fused=1)
-    _, _, t10 = run_gpu(dev, case, rp_free, 10, fused=1)
-    rp = RunParams(num_flux=0, T=0.5 * (t9 + t10))      # the 10th step is shortened to end at T
+@pytest.mark.parametrize("fused,av", [(2, 0), (1, 0), (1, 1)], ids=["single_kernel", "fused", "fused_av"])
+def test_device_clock_stops_at_T_and_resumes(dev, fused, av):
+    """Device-side clock (the single-kernel stage, and the fused schedule without analysis buffers):
+    steps issued after t reaches T change nothing and are not counted; a second run() continues
+    from the first one's state and time."""
+    case = Case.gaussian_bump(N=4, nel=8) if not av else Case.dam_break(N=7, nel=8, noise=1e-3, seed=5)
+    extra = dict(artificial_viscosity=1, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5, num_flux=3) if av else \
+        dict(num_flux=0)
+    rp_free = RunParams(**extra)
+    _, _, t9 = run_gpu(dev, case, rp_free, 9, fused=0)
+    _, _, t10 = run_gpu(dev, case, rp_free, 10, fused=0)
+    rp = RunParams(**extra, T=0.5 * (t9 + t10))      # the 10th step is shortened to end at T
     qo, no, to, _ = O.timeloop(case, rp, 100)
     assert 10 <= no <= 11
     ctx = dev.Context(dev.make_params(case, rp))
     try:
-        s = dev.Solver(ctx, case, rp, fused=2)
+        s = dev.Solver(ctx, case, rp, fused=fused)
         n1, t1 = s.run(7)
         n2, t2 = s.run(100)
         q = s.get_q()
