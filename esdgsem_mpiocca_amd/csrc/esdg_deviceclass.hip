@@ -92,9 +92,6 @@ class deviceclass {
         C(o_ElemToEdge, in.ElemToEdge, Nelem * 4 * sizeof(int));
         C(o_EdgeData, in.EdgeData, (size_t)Nfaces * 8 * sizeof(int));
         C(o_EdgeReversed, in.EdgeReversed, Nfaces * sizeof(int));
-        if (rc == ESDG_OK && rp_.fused == 1)
-            rc = fused_.setup(ctx_, o_EdgeData, o_ElemToEdge, o_ElemEdgeMasterSlave, o_ElemEdgeOrientation, o_Jac, o_nx, o_ny,
-                              o_scal);
         return rc;
     }
 

@@ -37,8 +37,6 @@ int fused_clock(esdg_ctx* ctx, const double* red, double* clock, double dti, dou
 class FusedStage {
   public:
     int init(esdg_ctx* ctx, int Nelem, int Nfaces, bool av);
-    int setup(esdg_ctx* ctx, const int* EdgeData, const int* ElemToEdge, const int* EMS, const int* EOR,
-              const double* Jac, const double* nx, const double* ny, const double* scal);
     // One RK stage: residual, low-storage accumulate and update (gRK = rkA gRK + Qt; q += cdt gRK).
     // With AV: face -> AV element pass -> viscous face -> volume + surface + viscous + RK [-> limiter].
     int stage(esdg_ctx* ctx, int s, double rkA, double cdt, const FusedArgs& a);
@@ -51,7 +49,6 @@ class FusedStage {
     bool ready = false;
     int Nelem = 0, Nfaces = 0;
     bool av = false;
-    double* qn = nullptr;       // reserved
     double* fstar = nullptr;    // face fluxes F* [f][3][ngl]
     double* grad = nullptr;     // side traces of the scaled velocity gradients [e][side][4][ngl] (AV)
     double* fvisc = nullptr;    // viscous face fluxes [f][2][ngl] (AV)

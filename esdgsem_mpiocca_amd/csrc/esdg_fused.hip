@@ -391,11 +391,6 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     return rc;
 }
 
-int FusedStage::setup(esdg_ctx*, const int*, const int*, const int*, const int*, const double*, const double*,
-                      const double*, const double*) {
-    return ESDG_OK;
-}
-
 int FusedStage::begin(esdg_ctx* ctx, const double* q) {
     if (!ready) return set_error(ctx, ESDG_ERR_STATE, "fused stage used before init");
     // an interrupted stage may have left a list behind (both list lengths: 2 ints)
@@ -481,7 +476,7 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
 }
 
 void FusedStage::release(esdg_ctx* ctx) {
-    for (double** p : {&qn, &fstar, &grad, &fvisc, &qvisc, &tr})
+    for (double** p : {&fstar, &grad, &fvisc, &qvisc, &tr})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
     for (int** p : {&active, &alist, &acount})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
