@@ -72,7 +72,7 @@ KERNELS = ["volume_kernel", "collect_edge_data", "calc_num_fluxes", "surface_ker
            "surface_kernel_gradient", "scale_gradient", "collect_edge_data_gradient",
            "volume_kernel_viscose", "calc_num_fluxes_viscose", "surface_kernel_visc",
            "find_lambda_max", "reduce_max", "pos_pres", "fused_face", "fused_volume", "fused_av",
-           "fused_viscous_face", "fused_update", "fused_timestep", "mlp_indicator", "write_traces"]
+           "fused_viscous_face", "fused_clock", "mlp_indicator", "write_traces"]
 
 
 def c1_params(RunParams):
