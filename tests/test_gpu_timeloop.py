@@ -248,6 +248,23 @@ def test_single_kernel_equals_fused_bitwise(dev, pos):
     assert np.array_equal(qa, qb)
 
 
+@pytest.mark.parametrize("fused", [0, 1], ids=["host_clock", "device_clock"])
+def test_invalid_time_step_is_reported(dev, fused):
+    """CFL = 0 gives dt = 0, which DGtimeloop's check rejects: both time loops (dt read back every
+    step, and dt on the device) report the error and leave the state as it was."""
+    case = Case.gaussian_bump(N=4, nel=6)
+    rp = RunParams(num_flux=0, CFL=0.0)
+    ctx = dev.Context(dev.make_params(case, rp))
+    try:
+        s = dev.Solver(ctx, case, rp, fused=fused)
+        with pytest.raises(dev.EsdgDeviceError, match="not positive and finite"):
+            s.run(3)
+        assert np.array_equal(s.get_q(), case["q"])
+        s.close()
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("fused,av", [(2, 0), (1, 0), (1, 1)], ids=["single_kernel", "fused", "fused_av"])
 def test_device_clock_stops_at_T_and_resumes(dev, fused, av):
     """Device-side clock (the single-kernel stage, and the fused schedule without analysis buffers):
