@@ -1,6 +1,6 @@
 """GPU parity of the whole time loop (deviceclass::DGtimeloop) against the oracle.
 
-Acceptance: relative L2 <= 1e-12 after a fixed number of steps; lake at rest
+Acceptance bar: relative L2 <= 1e-12 after a fixed number of steps; lake at rest
 keeps |u|, |v| <= 1e-13.
 """
 import os
