@@ -483,15 +483,16 @@ __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double 
         }
         __syncthreads();
         constexpr int TPE = 12 * NGL;
-        const int nact = min(RB::EPB, Nelem - (int)blockIdx.x * RB::EPB);
-        double* T = tr + (int64_t)blockIdx.x * RB::EPB * TPE;
-        for (int t = threadIdx.x; t < nact * TPE; t += RB::THREADS) {
-            const int el2 = t / TPE, rem = t - el2 * TPE;
-            const int m = rem / NGL, k = rem - m * NGL;
-            const int side = m / 3, v = m - 3 * side;
-            const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : k;
-            const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : k;
-            T[t] = sP[v][el2 * NN + row * NGL + col];
+        // thread (el, r) writes side node r of every (side, v) plane of its element
+        if (act) {
+            double* T = tr + (int64_t)e * TPE + r;
+#pragma unroll
+            for (int m = 0; m < 12; ++m) {
+                const int side = m / 3, v = m % 3;
+                const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : r;
+                const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : r;
+                T[m * NGL] = sP[v][el * NN + row * NGL + col];
+            }
         }
     }
     if (nx.visc) {

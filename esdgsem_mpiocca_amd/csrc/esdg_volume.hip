@@ -275,15 +275,17 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         if (si.tr) {
             __syncthreads();
             constexpr int TPE = 12 * NGL;
-            const int nact = min(C::EPW, Nelem - (int)blockIdx.x * C::EPW);
-            double* T = si.tr + (int64_t)blockIdx.x * C::EPW * TPE;
-            for (int t = lane; t < nact * TPE; t += 64) {
-                const int el2 = t / TPE, rem = t - el2 * TPE;
-                const int m = rem / NGL, k = rem - m * NGL;
-                const int side = m / 3, v = m - 3 * side;
-                const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : k;
-                const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : k;
-                T[t] = s[el2 * C::ESZ + v * C::PLANE + row * C::RS + col];
+            // lane (el, r) writes side node r of every (side, v) plane of its element: one unrolled
+            // iteration per plane, so the LDS source of each store is fixed at compile time
+            double* T = si.tr + (int64_t)e * TPE + r;
+            if (act) {
+#pragma unroll
+                for (int m = 0; m < 12; ++m) {
+                    const int side = m / 3, v = m % 3;
+                    const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : r;
+                    const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : r;
+                    T[m * NGL] = S[v * C::PLANE + row * C::RS + col];
+                }
             }
             // ShockCapturing.okl:63-217 on the updated state, the next stage's viscosity, with the
             // arithmetic of k_shock_capturing: lane r forms column r of T = V (h + b) from row r of
