@@ -63,12 +63,12 @@ int esdg_mlp_train_step(esdg_mlp* m, int n, const float* feat, const float* labe
 /* Learned indicator in a solver's artificial-viscosity path (fused schedule, N = 7): at every
  * stage the network classifies the elements; p >= hi: troubled, p <= lo: smooth, in between
  * unsure.  Smooth elements get no viscosity; troubled and unsure ones the modal sensor's value,
- * and the sensor is evaluated only in element blocks holding such an element.  Elements with no
- * viscosity among themselves and their face neighbours skip the BR1 and viscous work (as they do
- * without the indicator).  m = null restores the modal sensor everywhere. */
+ * which the indicator kernel evaluates from the modal coefficients its features already formed.
+ * Elements with no viscosity among themselves and their face neighbours skip the BR1 and viscous
+ * work (as they do without the indicator).  m = null restores the modal sensor everywhere. */
 int esdg_solver_set_indicator(esdg_solver* s, esdg_mlp* m, float lo, float hi);
-/* audit = 1 evaluates the modal sensor everywhere (the solution is unchanged) so that the counts
- * below include the sensor's verdict on the elements the network declared smooth. */
+/* audit = 1 counts the sensor's verdict on the elements the network declared smooth too (the
+ * solution is unchanged), for the false-negative counts below. */
 int esdg_solver_indicator_audit(esdg_solver* s, int audit);
 /* Counts since the last call, in element-stages: with viscosity; the sensor's positives among
  * those the network declared smooth (false negatives, audit only); the sensor's positives (audit
