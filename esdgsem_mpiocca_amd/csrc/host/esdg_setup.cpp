@@ -16,6 +16,7 @@
 // The initial states are new (they define the benchmark configurations) and are documented at
 // make_state() below.
 #include "esdg_setup.h"
+#include "esdg_output.h"
 
 #include <cmath>
 #include <cstdint>
@@ -994,6 +995,83 @@ int esdg_total_mass(int ngl, int Nelem, const double* w_GL, const double* q, con
             }
     *out = acc;
     return 0;
+}
+
+// ---- Tecplot ASCII output (the reference's plot files, src/Plots.cpp) ------------------------
+// One POINT zone of ngl x ngl nodes per element, nodes eta outer / xi inner, numbers in the
+// stream's default format (6 significant digits), as the reference writes them.
+
+}  // extern "C"
+
+namespace {
+template <class Row>
+int write_zones(const char* path, const char* title, const char* vars, int Nelem, int ngl, Row row,
+                const char* trailer = "") {
+    if (!path || Nelem < 0 || ngl < 1) return fail("esdg_plot: bad argument");
+    std::ofstream f(path);
+    if (!f) return fail(std::string("esdg_plot: cannot open ") + path);
+    f << "TITLE = " << title << "\n";
+    f << "VARIABLES = " << vars << "\n";
+    const int ngl2 = ngl * ngl;
+    for (int ie = 0; ie < Nelem; ie++) {
+        f << "ZONE I =" << ngl << ",J=" << ngl << ",F=POINT\n";
+        for (int j = 0; j < ngl; ++j)
+            for (int i = 0; i < ngl; ++i) row(f, ie, (size_t)ie * ngl2 + j * ngl + i, (size_t)ie * ngl2 * 3 + j * ngl + i);
+    }
+    f << trailer;
+    f.close();
+    return f ? 0 : fail(std::string("esdg_plot: write failed: ") + path);
+}
+}  // namespace
+
+extern "C" {
+
+int esdg_plot_solution(const char* path, int Nelem, int ngl, int plot_var, const double* x, const double* y,
+                       const double* q, const double* b, double h_0) {
+    if (!x || !y || !q || !b) return fail("esdg_plot_solution: null argument");
+    const int ngl2 = ngl * ngl;
+    if (plot_var == 1)
+        return write_zones(path, "H_solution.tec", "\"x\",\"y\",\"H\",\"hu\",\"hv\",\"bottom\"", Nelem, ngl,
+                           [&](std::ofstream& f, int, size_t xid, size_t id) {
+                               const double H = q[id] + b[xid];
+                               f << x[xid] << " " << y[xid] << " " << H << " " << q[id + ngl2] << " "
+                                 << q[id + 2 * ngl2] << " " << b[xid] << " \n";
+                           });
+    if (plot_var == 2)
+        return write_zones(path, "H_solution.tec", "\"x\",\"y\",\"FreeSurface\",\"hu\",\"hv\",\"bottom\"", Nelem, ngl,
+                           [&](std::ofstream& f, int, size_t xid, size_t id) {
+                               const double H = q[id] + b[xid] - h_0;
+                               const double bottom = b[xid] - h_0;
+                               f << x[xid] << " " << y[xid] << " " << H << " " << q[id + ngl2] << " "
+                                 << q[id + 2 * ngl2] << " " << bottom << " \n";
+                           });
+    return fail("esdg_plot_solution: plot_var must be 1 or 2");
+}
+
+int esdg_plot_arrival_times(const char* path, int Nelem, int ngl, const double* x, const double* y, const double* arr) {
+    if (!x || !y || !arr) return fail("esdg_plot_arrival_times: null argument");
+    return write_zones(path, "Arrivals.tec", "\"x\",\"y\",\"Arrivaltime\",", Nelem, ngl,
+                       [&](std::ofstream& f, int, size_t xid, size_t) {
+                           f << x[xid] << " " << y[xid] << " " << arr[xid] << " \n";
+                       });
+}
+
+int esdg_plot_maximum_elevation(const char* path, int Nelem, int ngl, const double* x, const double* y,
+                                const double* mx) {
+    if (!x || !y || !mx) return fail("esdg_plot_maximum_elevation: null argument");
+    return write_zones(path, "MaximumElevations.tec", "\"x\",\"y\",\"MaximumElevation\",", Nelem, ngl,
+                       [&](std::ofstream& f, int, size_t xid, size_t) {
+                           f << x[xid] << " " << y[xid] << " " << mx[xid] << " \n";
+                       });
+}
+
+int esdg_plot_viscosity(const char* path, int Nelem, int ngl, const double* x, const double* y, const double* eps) {
+    if (!x || !y || !eps) return fail("esdg_plot_viscosity: null argument");
+    return write_zones(path, "ArtVisc.tec", "\"x\",\"y\",\"Eps\"", Nelem, ngl,
+                       [&](std::ofstream& f, int ie, size_t xid, size_t) {
+                           f << x[xid] << " " << y[xid] << " " << eps[ie] << " \n";
+                       },
+                       "Test");
 }
 
 }  // extern "C"

@@ -251,3 +251,27 @@ def total_mass(case: Case, q: np.ndarray) -> float:
     _check(_lib().esdg_total_mass(case.ngl, case.Nelem, _dptr(keep, case["w_GL"]), _dptr(keep, q),
                                   _dptr(keep, case["Jac"]), C.byref(out)))
     return out.value
+
+
+# ---- Tecplot output (include/esdg_output.h; the reference's src/Plots.cpp files) ----------------
+def plot_solution(path, case: Case, q: np.ndarray, plot_var: int = 1, h_0: float = 0.0):
+    """PlotSolution: plot_var 1 writes H = h + b, hu, hv, b; 2 the free surface h + b - h_0."""
+    keep = []
+    _check(_lib().esdg_plot_solution(os.fsencode(path), case.Nelem, case.ngl, int(plot_var), _dptr(keep, case["x"]),
+                                     _dptr(keep, case["y"]), _dptr(keep, q), _dptr(keep, case["b"]),
+                                     C.c_double(h_0)))
+
+
+def plot_nodal(path, case: Case, values: np.ndarray, kind: str):
+    """PlotArrivalTimings (kind "arrival") or PlotMaximumElevation (kind "maximum") of a per-node buffer."""
+    fn = {"arrival": _lib().esdg_plot_arrival_times, "maximum": _lib().esdg_plot_maximum_elevation}[kind]
+    keep = []
+    _check(fn(os.fsencode(path), case.Nelem, case.ngl, _dptr(keep, case["x"]), _dptr(keep, case["y"]),
+              _dptr(keep, values)))
+
+
+def plot_viscosity(path, case: Case, eps: np.ndarray):
+    """PlotViscoseParameter: the element viscosity at every node of its element."""
+    keep = []
+    _check(_lib().esdg_plot_viscosity(os.fsencode(path), case.Nelem, case.ngl, _dptr(keep, case["x"]),
+                                      _dptr(keep, case["y"]), _dptr(keep, eps)))

@@ -2,7 +2,7 @@
 //
 // This file is part of the test oracle, not of the product.  It is compiled together with the
 // reference's src/basis.cpp, src/RungeKutta.cpp, src/Mesh.cpp, src/SW2D.cpp and
-// src/ReadInput.cpp (compiled where they lie, never copied) into oracle/_ref/, and exposes their
+// src/ReadInput.cpp and src/Plots.cpp (compiled where they lie, never copied) into oracle/_ref/, and exposes their
 // results through a small C interface so the tests can pin the product's setup arrays
 // (esdg_setup.cpp) bit for bit against the reference's own code.
 #include <unistd.h>
@@ -11,10 +11,13 @@
 #include <cstring>
 #include <string>
 
+#include <sys/stat.h>
+
 #include "Mesh.h"
 #include "RungeKutta.h"
 #include "SW2D.h"
 #include "basis.h"
+#include "plots.h"
 
 extern "C" {
 
@@ -136,6 +139,25 @@ int ref_total_entropy_mass(int N, int Nelem, double g, const double* q, const do
     B.calcTotalEntropy(g, q, b, Jinv, entropy);
     B.calcTotalMass(q, Jinv, mass);
     return 0;
+}
+
+// The reference's plot writers (src/Plots.cpp), run in workdir: they write under movie/.
+// which: 0 PlotSolution, 1 PlotArrivalTimings, 2 PlotMaximumElevation, 3 PlotViscoseParameter
+// (values: q for 0, the per-node buffer for 1 and 2, the element viscosity for 3).
+int ref_plot(const char* workdir, int which, int Nelem, int ngl, int plot_var, const double* x, const double* y,
+             const double* values, const double* b, int count, double h0) {
+    char cwd[4096];
+    if (!getcwd(cwd, sizeof cwd)) return -1;
+    if (chdir(workdir) != 0) return -2;
+    mkdir("movie", 0755);
+    switch (which) {
+        case 0: PlotSolution(Nelem, ngl, plot_var, x, y, values, b, count, h0); break;
+        case 1: PlotArrivalTimings(Nelem, ngl, x, y, values); break;
+        case 2: PlotMaximumElevation(Nelem, ngl, x, y, values); break;
+        case 3: PlotViscoseParameter(Nelem, ngl, x, y, values, count); break;
+        default: break;
+    }
+    return chdir(cwd) == 0 ? 0 : -3;
 }
 
 }  // extern "C"
