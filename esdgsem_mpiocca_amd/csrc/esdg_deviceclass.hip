@@ -7,7 +7,9 @@
 // only two doubles come back per step; DBSurf1/2, never written when DiscBottom = 0 yet read by
 // SurfaceKernel (okl/DG/SurfaceKernel.okl:71-72), are zero-filled.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "esdg_fused.h"
 #include "esdg_internal.h"
@@ -524,6 +526,66 @@ int esdg_solver_set_q(esdg_solver* s, const double* host_q) {
     if (!s || !host_q) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_set_q: null argument");
     s->dc.t = 0.0;
     return esdg_memcpy(s->dc.ctx_, s->dc.o_q, host_q, (size_t)s->dc.Nelem * s->dc.ngl2 * 3 * sizeof(double), ESDG_H2D);
+}
+
+namespace {
+struct CheckpointHeader {
+    char magic[8];
+    int32_t version, ngl, Nelem, arrival, maximum, pad[3];
+    double t, last_dt;
+};
+}  // namespace
+
+int esdg_solver_checkpoint(esdg_solver* s, const char* path) {
+    if (!s || !path) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_checkpoint: null argument");
+    auto& d = s->dc;
+    const size_t nq = (size_t)d.Nelem * d.ngl2 * 3, nn = (size_t)d.Nelem * d.ngl2;
+    std::vector<double> q(nq), g(nq), a(d.o_ArrivalTimings ? nn : 0), m(d.o_MaximumElevation ? nn : 0);
+    int rc = esdg_memcpy(d.ctx_, q.data(), d.o_q, nq * sizeof(double), ESDG_D2H);
+    if (!rc) rc = esdg_memcpy(d.ctx_, g.data(), d.o_gRK, nq * sizeof(double), ESDG_D2H);
+    if (!rc && !a.empty()) rc = esdg_memcpy(d.ctx_, a.data(), d.o_ArrivalTimings, nn * sizeof(double), ESDG_D2H);
+    if (!rc && !m.empty()) rc = esdg_memcpy(d.ctx_, m.data(), d.o_MaximumElevation, nn * sizeof(double), ESDG_D2H);
+    if (rc) return rc;
+    CheckpointHeader h{};
+    std::memcpy(h.magic, "ESDGCKP1", 8);
+    h.version = 1; h.ngl = d.ngl; h.Nelem = d.Nelem; h.arrival = !a.empty(); h.maximum = !m.empty();
+    h.t = d.t; h.last_dt = d.last_dt;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_error(d.ctx_, ESDG_ERR_ARG, std::string("esdg_solver_checkpoint: cannot open ") + path);
+    bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+    for (const std::vector<double>* v : {&q, &g, &a, &m})
+        if (ok && !v->empty()) ok = std::fwrite(v->data(), sizeof(double), v->size(), f) == v->size();
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? ESDG_OK : set_error(d.ctx_, ESDG_ERR_ARG, std::string("esdg_solver_checkpoint: write failed: ") + path);
+}
+
+int esdg_solver_restart(esdg_solver* s, const char* path) {
+    if (!s || !path) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_restart: null argument");
+    auto& d = s->dc;
+    const size_t nq = (size_t)d.Nelem * d.ngl2 * 3, nn = (size_t)d.Nelem * d.ngl2;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_error(d.ctx_, ESDG_ERR_ARG, std::string("esdg_solver_restart: cannot open ") + path);
+    CheckpointHeader h{};
+    bool ok = std::fread(&h, sizeof h, 1, f) == 1;
+    std::string why;
+    if (!ok || std::memcmp(h.magic, "ESDGCKP1", 8) != 0 || h.version != 1) why = "not a checkpoint of this format";
+    else if (h.ngl != d.ngl || h.Nelem != d.Nelem) why = "degree or element count differ from the solver's";
+    else if ((h.arrival != 0) != (d.o_ArrivalTimings != nullptr) || (h.maximum != 0) != (d.o_MaximumElevation != nullptr))
+        why = "analysis buffers differ from the solver's";
+    std::vector<double> q(nq), g(nq), a(h.arrival ? nn : 0), m(h.maximum ? nn : 0);
+    if (why.empty())
+        for (std::vector<double>* v : {&q, &g, &a, &m})
+            if (!v->empty() && std::fread(v->data(), sizeof(double), v->size(), f) != v->size()) why = "file too short";
+    std::fclose(f);
+    if (!why.empty()) return set_error(d.ctx_, ESDG_ERR_ARG, "esdg_solver_restart: " + why + ": " + path);
+    int rc = esdg_memcpy(d.ctx_, d.o_q, q.data(), nq * sizeof(double), ESDG_H2D);
+    if (!rc) rc = esdg_memcpy(d.ctx_, d.o_gRK, g.data(), nq * sizeof(double), ESDG_H2D);
+    if (!rc && !a.empty()) rc = esdg_memcpy(d.ctx_, d.o_ArrivalTimings, a.data(), nn * sizeof(double), ESDG_H2D);
+    if (!rc && !m.empty()) rc = esdg_memcpy(d.ctx_, d.o_MaximumElevation, m.data(), nn * sizeof(double), ESDG_H2D);
+    if (rc) return rc;
+    d.t = h.t;
+    d.last_dt = h.last_dt;
+    return ESDG_OK;
 }
 
 int esdg_solver_time(const esdg_solver* s, double* t) {

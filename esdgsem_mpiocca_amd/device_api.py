@@ -289,6 +289,14 @@ class Solver:
         self.ctx.check(lib().esdg_solver_time(C.c_void_p(self.handle), C.byref(t)))
         return t.value
 
+    def checkpoint(self, path):
+        """Binary checkpoint between steps (t, last dt, q, the RK register, analysis buffers)."""
+        self.ctx.check(lib().esdg_solver_checkpoint(C.c_void_p(self.handle), os.fsencode(path)))
+
+    def restart(self, path):
+        """Restart from a checkpoint of a solver with the same degree, mesh size and buffers."""
+        self.ctx.check(lib().esdg_solver_restart(C.c_void_p(self.handle), os.fsencode(path)))
+
     def set_indicator(self, mlp, lo: float = 0.5, hi: float = 0.5):
         """Learned troubled-cell indicator in the AV path (None: the modal sensor)."""
         self.ctx.check(lib().esdg_solver_set_indicator(C.c_void_p(self.handle),

@@ -234,11 +234,21 @@ typedef struct esdg_solver esdg_solver;
 int esdg_solver_create(esdg_ctx* ctx, const esdg_inputs* in, const esdg_run_params* rp,
                        esdg_solver** out);
 /* DGtimeloop with a step cap: at most max_steps steps or until t >= T.  Returns steps taken
- * and the final time.  dt per step is computed on the device and read back as 8 bytes. */
+ * and the final time.  dt is computed on the device; the fused and single-kernel schedules keep
+ * the clock there and read it back every 64 steps (the fused schedule reads dt back every step
+ * when analysis buffers or the learned indicator are on). */
 int esdg_solver_run(esdg_solver* s, int max_steps, int* steps_taken, double* t_out);
 int esdg_solver_get_q(esdg_solver* s, double* host_q);        /* blocking D2H copy of q */
 int esdg_solver_set_q(esdg_solver* s, const double* host_q);  /* blocking H2D copy, t = 0 */
 int esdg_solver_time(const esdg_solver* s, double* t);
+/* Binary checkpoint of a solver between steps, and restart from one (no counterpart in the
+ * reference, which only writes plot files).  The file holds t, the last dt, q, the low-storage
+ * RK register and, when on, the ArrivalTimings / MaximumElevation buffers, so that a restarted
+ * run continues bit for bit.  Restart checks that the file matches the solver (ngl, Nelem,
+ * analysis buffers).  Layout: "ESDGCKP1", int32 {1, ngl, Nelem, arrival, maximum, 0, 0, 0},
+ * double {t, last dt}, then the arrays as doubles in the state layouts, host byte order. */
+int esdg_solver_checkpoint(esdg_solver* s, const char* path);
+int esdg_solver_restart(esdg_solver* s, const char* path);
 int esdg_solver_last_dt(const esdg_solver* s, double* dt);
 /* Device pointer of a named solver buffer ("q","Qt","gRK","ViscPara","LambdaMax", ...). */
 int esdg_solver_buffer(esdg_solver* s, const char* name, void** dptr);

@@ -248,6 +248,47 @@ def test_single_kernel_equals_fused_bitwise(dev, pos):
     assert np.array_equal(qa, qb)
 
 
+@pytest.mark.parametrize("fused,analysis", [(0, 1), (1, 0), (1, 1)], ids=["sequence", "fused", "fused_analysis"])
+def test_checkpoint_restart_continues_bitwise(dev, tmp_path, fused, analysis):
+    """10 steps in one run equal 5 steps, a checkpoint, a restart in a new solver and 5 more steps:
+    state, time and the analysis buffers bit for bit.  A solver of another size refuses the file."""
+    case = Case.dam_break(N=7, nel=8, noise=1e-3, seed=5)
+    rp = RunParams(num_flux=3, artificial_viscosity=1, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5,
+                   h_zero=1.5, maximum_elevation=analysis, arrival_times=analysis)
+    E, n2 = case.Nelem, case.ngl ** 2
+    ctx = dev.Context(dev.make_params(case, rp))
+    try:
+        a = dev.Solver(ctx, case, rp, fused=fused)
+        a.run(10)
+        qa, ta = a.get_q(), a.time()
+        ma = a.view("MaximumElevation", E * n2).download() if analysis else None
+        a.close()
+        b = dev.Solver(ctx, case, rp, fused=fused)
+        b.run(5)
+        ck = tmp_path / "state.ckp"
+        b.checkpoint(ck)
+        b.close()
+        c = dev.Solver(ctx, case, rp, fused=fused)
+        c.restart(ck)
+        n, _ = c.run(5)
+        assert n == 5
+        assert np.array_equal(c.get_q(), qa) and c.time() == ta
+        if analysis:
+            assert np.array_equal(c.view("MaximumElevation", E * n2).download(), ma)
+        c.close()
+    finally:
+        ctx.close()
+    other = Case.dam_break(N=7, nel=6, noise=1e-3, seed=5)
+    ctx = dev.Context(dev.make_params(other, rp))
+    try:
+        d = dev.Solver(ctx, other, rp, fused=fused)
+        with pytest.raises(dev.EsdgDeviceError, match="differ"):
+            d.restart(ck)
+        d.close()
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("fused", [0, 1], ids=["host_clock", "device_clock"])
 def test_invalid_time_step_is_reported(dev, fused):
     """CFL = 0 gives dt = 0, which DGtimeloop's check rejects: both time loops (dt read back every
