@@ -1065,6 +1065,30 @@ int esdg_plot_maximum_elevation(const char* path, int Nelem, int ngl, const doub
                        });
 }
 
+// time series of the solution integrals (one line per recorded time)
+static int write_series(const char* path, const char* title, const char* vars, int n, const double* t, const double* v,
+                        bool relative) {
+    if (!path || n < 0 || (n > 0 && (!t || !v))) return fail("esdg_plot: bad argument");
+    std::ofstream f(path);
+    if (!f) return fail(std::string("esdg_plot: cannot open ") + path);
+    f << "TITLE = " << title << "\n";
+    f << "VARIABLES = " << vars << "\n";
+    for (int i = 0; i < n; ++i) {
+        if (relative) f << t[i] << " " << (v[i] - v[0]) / v[0] << " \n";
+        else f << t[i] << " " << v[i] << " \n";
+    }
+    f.close();
+    return f ? 0 : fail(std::string("esdg_plot: write failed: ") + path);
+}
+
+int esdg_plot_entropy(const char* path, int n, const double* times, const double* total_entropy) {
+    return write_series(path, "Entropyplot.tec", "\"t\",\"Entropy\"", n, times, total_entropy, false);
+}
+
+int esdg_plot_mass(const char* path, int n, const double* times, const double* total_mass) {
+    return write_series(path, "Massplot.tec", "\"t\",\"Massdifference\"", n, times, total_mass, true);
+}
+
 int esdg_plot_viscosity(const char* path, int Nelem, int ngl, const double* x, const double* y, const double* eps) {
     if (!x || !y || !eps) return fail("esdg_plot_viscosity: null argument");
     return write_zones(path, "ArtVisc.tec", "\"x\",\"y\",\"Eps\"", Nelem, ngl,

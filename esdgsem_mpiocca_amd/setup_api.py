@@ -275,3 +275,10 @@ def plot_viscosity(path, case: Case, eps: np.ndarray):
     keep = []
     _check(_lib().esdg_plot_viscosity(os.fsencode(path), case.Nelem, case.ngl, _dptr(keep, case["x"]),
                                       _dptr(keep, case["y"]), _dptr(keep, eps)))
+
+
+def plot_series(path, times: np.ndarray, totals: np.ndarray, kind: str):
+    """PlotEntropy (kind "entropy") or PlotMass (kind "mass", written as (m_i - m_0) / m_0)."""
+    fn = {"entropy": _lib().esdg_plot_entropy, "mass": _lib().esdg_plot_mass}[kind]
+    keep = []
+    _check(fn(os.fsencode(path), len(times), _dptr(keep, times), _dptr(keep, totals)))

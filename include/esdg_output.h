@@ -30,6 +30,12 @@ int esdg_plot_maximum_elevation(const char* path, int Nelem, int ngl, const doub
 /* PlotViscoseParameter (src/Plots.cpp:349-399): the element viscosity at every node of the
  * element (the file ends with the reference's trailing "Test"). */
 int esdg_plot_viscosity(const char* path, int Nelem, int ngl, const double* x, const double* y, const double* eps);
+/* PlotEntropy (src/Plots.cpp:401-420): total entropy against time, n records. */
+int esdg_plot_entropy(const char* path, int n, const double* times, const double* total_entropy);
+/* PlotMass (src/Plots.cpp:438-457): relative mass change (m_i - m_0) / m_0 against time. */
+int esdg_plot_mass(const char* path, int n, const double* times, const double* total_mass);
+/* Not provided: PlotTimeSeries (the gauge records of one tsunami scenario's seven ports) and
+ * PlotFriction / PlotViscosity / PlotSolutionWithExact, which write fields outside this path. */
 
 #ifdef __cplusplus
 }

@@ -83,3 +83,22 @@ def test_plot_file_structure(tmp_path, data):
     assert visc.endswith(" \nTest")
     v = np.array([float(ln.split()[2]) for ln in visc.split("\n")[2:] if ln and not ln.startswith(("ZONE", "Test"))])
     assert np.allclose(v, np.repeat(eps, ngl * ngl), rtol=5e-6, atol=0)
+
+
+def test_series_files_equal_the_reference_writers(tmp_path):
+    if not os.path.exists(REF):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(4)
+    t = np.cumsum(rng.uniform(1e-4, 1e-2, size=17))
+    ent = 123.456 + np.cumsum(rng.normal(size=17)) * 1e-7
+    mass = 40.0 + rng.normal(size=17) * 1e-12
+    S.plot_series(tmp_path / "ours_ent.tec", t, ent, "entropy")
+    S.plot_series(tmp_path / "ours_mass.tec", t, mass, "mass")
+    ref = C.CDLL(REF)
+    ref.ref_plot.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4 + [C.c_int, C.c_double]
+    P = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.c_void_p)
+    wd = os.fsencode(str(tmp_path))
+    assert ref.ref_plot(wd, 4, 0, 0, 0, P(t), None, P(ent), None, len(t), 0.0) == 0
+    assert ref.ref_plot(wd, 5, 0, 0, 0, P(t), None, P(mass), None, len(t), 0.0) == 0
+    assert (tmp_path / "movie" / "Entropyplot.tec").read_bytes() == (tmp_path / "ours_ent.tec").read_bytes()
+    assert (tmp_path / "movie" / "Massplot.tec").read_bytes() == (tmp_path / "ours_mass.tec").read_bytes()
