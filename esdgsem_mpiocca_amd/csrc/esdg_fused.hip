@@ -31,7 +31,8 @@
 namespace esdg {
 
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV, double* tr, const int* active = nullptr, int seq = 0);
+                 const double* Qv, const double* SPV, double* tr, const int* active = nullptr, int seq = 0,
+                 const int* list = nullptr, const int* list_count = nullptr);
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
                     double* tr, double* next_visc, double* next_visc_plot, const double* B, const double* VdmInv,
                     double* next_lambda, const double* dt_dev);
@@ -468,7 +469,8 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         if (rc) return rc;
     }
     // volume + surface + viscous terms + RK in one element pass, then the limiter
-    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, active, seq);
+    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, active, seq,
+                      alist, acount + (seq & 1));
     if (!rc && A.positivity)
         rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, A.next_visc, A.next_visc_plot, A.B,
                              A.VdmInv, A.next_lambda, A.dt_dev);

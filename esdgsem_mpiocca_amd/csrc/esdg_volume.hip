@@ -53,6 +53,11 @@ struct SurfIn {
     // every face neighbour; their viscous terms are +-0 and are not read
     const int* active;
     int seq;
+    // the marked elements (list[0 .. *list_count)): with them given, VOL_SURF_RK leaves the marked
+    // elements out and VOL_SURF_VISC_RK runs over the list only, so that the viscous terms run in
+    // wavefronts of marked elements alone
+    const int* list;
+    const int* list_count;
     // the device clock's dt (may be null): the update uses Cs * dt, and a block with dt = 0 (a step
     // issued after T) does nothing
     const double* dtp;
@@ -78,8 +83,19 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
     __shared__ double s[C::EPW * C::ESZ];
     const int lane = threadIdx.x;
     const int el = lane / NGL, r = lane % NGL;
-    const int e = blockIdx.x * C::EPW + el;
-    const bool act = (el < C::EPW) && (e < Nelem);
+    int e = blockIdx.x * C::EPW + el;
+    bool act = (el < C::EPW) && (e < Nelem);
+    if constexpr (MODE == VOL_SURF_VISC_RK) {
+        if (si.list) {
+            // blocks past the end of the list do nothing (the count is the same for every lane)
+            const int cnt = min(*si.list_count, Nelem);
+            if ((int)blockIdx.x * C::EPW >= cnt) return;
+            act = (el < C::EPW) && (e < cnt);
+            e = act ? si.list[e] : 0;
+        }
+    } else if constexpr (MODE == VOL_SURF_RK) {
+        if (si.list && act && si.active[e] == si.seq) act = false;
+    }
     double* S = s + (act ? el : 0) * C::ESZ;
     const double half_g = 0.5 * g, fourth_g = 0.25 * g;
     double lam = 0.0;   // this column's maximum wave speed of the updated state (si.lam)
@@ -418,11 +434,14 @@ void launch_volume_fluxdiff(esdg_ctx* ctx, int Nelem, const double* Jac, const d
 
 template <int NGL>
 void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar,
-                         double* Qt, const double* Qv, const double* SPV, double* tr, const int* active, int seq) {
+                         double* Qt, const double* Qv, const double* SPV, double* tr, const int* active, int seq,
+                         const int* list, const int* list_count) {
     using C = VolCfg<NGL>;
     SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV, rk ? tr : nullptr};
     si.active = active;
     si.seq = seq;
+    si.list = list;
+    si.list_count = list_count;
     si.dtp = a.dt_dev;
     si.Cs = a.C_stage;
     if (rk && tr) {
@@ -435,10 +454,13 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
         si.lam = a.next_lambda;
         si.tol = ctx->p.PosPresTOL;
     }
-    if (rk && Qv)
+    if (rk && Qv) {
+        // the unmarked elements, then the marked ones from the list (disjoint sets, each updated once)
+        k_volume_fluxdiff<NGL, VOL_SURF_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
+            a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
         k_volume_fluxdiff<NGL, VOL_SURF_VISC_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
-    else if (rk)
+    } else if (rk)
         k_volume_fluxdiff<NGL, VOL_SURF_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
     else
@@ -449,10 +471,13 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
 // Fused volume + surface lift (+ viscous terms when Qv is given) (+ RK update in place when rk,
 // with the side traces of the new state written to tr when given): called by the fused stage.
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV, double* tr, const int* active, int seq) {
-    if (Qv && !active) return set_error(ctx, ESDG_ERR_STATE, "fused_volume: viscous terms need the stage marks");
+                 const double* Qv, const double* SPV, double* tr, const int* active, int seq, const int* list,
+                 const int* list_count) {
+    if (Qv && !(active && list && list_count))
+        return set_error(ctx, ESDG_ERR_STATE, "fused_volume: viscous terms need the stage marks and element list");
     KernelScope ks(ctx, "fused_volume");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr, active, seq);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr, active, seq, list,
+                      list_count);
     return launch_check(ctx, "fused_volume");
 }
 
