@@ -18,6 +18,10 @@
 struct esdg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for work that runs beside the main stream's, forked from
+    // and joined back into it by the two events; created on first use
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     esdg_params p{};
     std::string err;
     size_t bytes = 0;

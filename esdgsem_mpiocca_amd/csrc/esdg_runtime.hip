@@ -115,6 +115,10 @@ int esdg_destroy(esdg_ctx* ctx) {
     for (auto& kv : ctx->prof)
         for (auto& pe : kv.second.pending) { (void)hipEventDestroy(pe.first); (void)hipEventDestroy(pe.second); }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    for (hipEvent_t e : {ctx->fork, ctx->join})
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return ESDG_OK;

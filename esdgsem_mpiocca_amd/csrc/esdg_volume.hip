@@ -455,11 +455,32 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
         si.tol = ctx->p.PosPresTOL;
     }
     if (rk && Qv) {
-        // the unmarked elements, then the marked ones from the list (disjoint sets, each updated once)
+        // the listed marked elements on the side stream, beside the unmarked ones on the main stream
+        // (disjoint sets, each updated once; nothing else is read or written by both), joined back
+        // before anything that follows on the main stream
+        if (!ctx->side) {
+            if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) ctx->side = nullptr;
+            if (ctx->side && (hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
+                              hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess)) {
+                // without the events both launches stay on the main stream, in order
+                for (hipEvent_t* e : {&ctx->fork, &ctx->join})
+                    if (*e) { (void)hipEventDestroy(*e); *e = nullptr; }
+                (void)hipStreamDestroy(ctx->side);
+                ctx->side = nullptr;
+                (void)hipGetLastError();
+            }
+        }
+        hipStream_t ls = ctx->side ? ctx->side : ctx->stream;
+        if (ctx->side) {
+            (void)hipEventRecord(ctx->fork, ctx->stream);
+            (void)hipStreamWaitEvent(ctx->side, ctx->fork, 0);
+        }
+        k_volume_fluxdiff<NGL, VOL_SURF_VISC_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ls>>>(
+            a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
+        if (ctx->side) (void)hipEventRecord(ctx->join, ctx->side);
         k_volume_fluxdiff<NGL, VOL_SURF_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
-        k_volume_fluxdiff<NGL, VOL_SURF_VISC_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
-            a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
+        if (ctx->side) (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
     } else if (rk)
         k_volume_fluxdiff<NGL, VOL_SURF_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
