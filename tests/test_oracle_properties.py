@@ -723,3 +723,207 @@ def test_arrival_times_and_maximum_elevation():
     for level in (-0.2, 0.3, 0.1):
         ref = np.maximum(ref, (1.5 - b + level * rng.uniform(size=(E, n2))) + b - 1.5)
     assert np.allclose(mx.reshape(E, n2), ref, rtol=0, atol=1e-15)
+
+
+# --------------------------------------------------------------------------------------------
+# wave speed, Lax-Friedrichs flux, standard volume term, SSP-RK3 update
+# --------------------------------------------------------------------------------------------
+def spectral_radii(q):
+    """Spectral radii of the x- and y-flux Jacobians at states q[M, 3], from complex-step
+    Jacobians of the physical flux and their numerical eigenvalues (no wave-speed formula used)."""
+    def fx(s):
+        h, hu, hv = s
+        return np.array([hu, hu * hu / h + 0.5 * G * h * h, hu * hv / h])
+
+    def fy(s):
+        h, hu, hv = s
+        return np.array([hv, hu * hv / h, hv * hv / h + 0.5 * G * h * h])
+
+    out = np.zeros((len(q), 2))
+    for m, s in enumerate(q):
+        for k, f in enumerate((fx, fy)):
+            Jm = np.zeros((3, 3))
+            for c in range(3):
+                e = np.zeros(3, complex)
+                e[c] = 1e-30j
+                Jm[:, c] = f(s + e).imag / 1e-30
+            out[m, k] = np.abs(np.linalg.eigvals(Jm)).max()
+    return out
+
+
+def test_find_lambda_max_is_the_largest_flux_jacobian_eigenvalue():
+    """FindLambdaMax.okl:33-50: per element, the largest eigenvalue magnitude of the x- and
+    y-flux Jacobians over its nodes; a node with h <= PosPresTOL counts with zero velocity.
+    |u|+|v| or sqrt(u^2+v^2) in place of max(|u|,|v|), or a missing c, fails this."""
+    ngl, E, tol = 3, 40, 1e-3
+    n2 = ngl * ngl
+    rng = np.random.default_rng(11)
+    h = rng.uniform(0.3, 2.0, (E, n2))
+    u = rng.uniform(-2.0, 2.0, (E, n2)) * rng.choice([0.2, 1.0, 3.0], (E, 1))
+    v = rng.uniform(-2.0, 2.0, (E, n2)) * rng.choice([0.2, 1.0, 3.0], (E, 1))
+    q = np.stack([h, h * u, h * v], 1)                          # [e][var][node]
+    lam = np.zeros(E)
+    p = O.OracleParams(ngl, 1, G, 1.0, tol, 1e-5, 0.0, 0.0, 0.0)
+    O.call("find_lambda_max", p, E, np.ascontiguousarray(q.ravel()), lam)
+    states = np.moveaxis(q, 1, 2).reshape(-1, 3)
+    ref = spectral_radii(states).max(1).reshape(E, n2).max(1)
+    assert np.abs(lam - ref).max() <= 1e-13 * ref.max()
+    # dry nodes: velocity ignored at h <= tol (h == tol included), counted above it
+    qd = q.copy()
+    qd[0, 0, :] = 0.5 * tol
+    qd[0, 1:, :] = 50.0                                         # dry element, large momentum
+    qd[1, 0, :] = tol
+    qd[1, 1:, :] = 50.0
+    qd[2, 0, :] = 2.0 * tol
+    qd[2, 1, :] = 3.0 * tol
+    qd[2, 2, :] = -1.0 * tol                                    # u = 1.5, v = -0.5: wet, counted
+    O.call("find_lambda_max", p, E, np.ascontiguousarray(qd.ravel()), lam)
+    assert lam[0] == np.sqrt(G * 0.5 * tol)
+    assert lam[1] == np.sqrt(G * tol)
+    ref2 = spectral_radii(np.moveaxis(qd[2:3], 1, 2).reshape(-1, 3)).max()
+    assert abs(lam[2] - ref2) <= 1e-13 * ref2 and lam[2] > 1.5
+
+
+def lf_call(qL, qR, n, scal=1.0, geom=1.0, tol=1e-12):
+    M = len(qL)
+    p = O.OracleParams(1, 1, G, geom, tol, 1e-5, 0.0, 0.0, 0.0)
+    SP = np.zeros(3 * M)
+    O.call("num_flux_lf", p, M, np.ones(M, np.int32), np.ascontiguousarray(n[:, 0]),
+           np.ascontiguousarray(n[:, 1]), np.full(M, scal), np.ascontiguousarray(qL.reshape(-1)),
+           np.ascontiguousarray(qR.reshape(-1)), np.zeros(M), np.zeros(M), SP)
+    return SP.reshape(M, 3)
+
+
+def test_lax_friedrichs_flux_consistency_conservation_and_dissipation():
+    """calcLaxFriedrich.okl:17-170: F = n.(f(qL)+f(qR))/2 - lam/2 (qR - qL), with lam the largest
+    flux-Jacobian eigenvalue magnitude of both states in both coordinate directions; F(q, q) =
+    n.f(q); F(qL, qR, n) = -F(qR, qL, -n); the output is scaled by scal * geomFace."""
+    qL, qR, n = random_faces(300, 21)
+    F = lf_call(qL, qL, n)
+    assert np.abs(F - phys_flux(qL, n)).max() < 1e-13
+    F = lf_call(qL, qR, n)
+    Fm = lf_call(qR, qL, -n)
+    assert np.abs(F + Fm).max() < 1e-13
+    lam = np.maximum(spectral_radii(qL).max(1), spectral_radii(qR).max(1))
+    central = 0.5 * (phys_flux(qL, n) + phys_flux(qR, n))
+    ref = central - 0.5 * lam[:, None] * (qR - qL)
+    assert np.abs(F - ref).max() <= 1e-13 * np.abs(ref).max()
+    # the dissipation removes energy-norm of the jump: (qR - qL) . (F - central) < 0
+    assert (((qR - qL) * (F - central)).sum(1) < 0).all()
+    Fs = lf_call(qL, qR, n, scal=2.5, geom=3.0)
+    assert np.abs(Fs - 7.5 * F).max() <= 1e-15 * np.abs(Fs).max()
+    # a side with h <= PosPresTOL enters with zero momentum and zero wave speed
+    tol = 1e-3
+    qLd = qL.copy()
+    qLd[:, 0] = tol
+    qLd[:, 1:] = 7.0
+    Fd = lf_call(qLd, qR, n, tol=tol)
+    q0 = qLd.copy()
+    q0[:, 1:] = 0.0
+    lam_d = spectral_radii(qR).max(1)
+    fL = np.stack([np.zeros(len(qL)), 0.5 * G * tol * tol * n[:, 0], 0.5 * G * tol * tol * n[:, 1]], 1)
+    ref_d = 0.5 * (fL + phys_flux(qR, n)) - 0.5 * lam_d[:, None] * (qR - q0)
+    assert np.abs(Fd - ref_d).max() <= 1e-13 * np.abs(ref_d).max()
+
+
+def lgl_nodes(ngl):
+    """LGL nodes: -1, 1 and the roots of P_N' (numpy Legendre)."""
+    N = ngl - 1
+    inner = np.sort(L.legroots(L.legder([0] * N + [1]))) if N > 1 else np.array([])
+    return np.concatenate([[-1.0], inner, [1.0]])
+
+
+def d0_matrix(case):
+    """The plain nodal derivative matrix: Dstrong without its two corner terms."""
+    ngl = case.ngl
+    D0 = case["Dstrong"].reshape(ngl, ngl).copy()
+    w = case["w_GL"]
+    D0[0, 0] -= 1.0 / w[0]
+    D0[-1, -1] += 1.0 / w[-1]
+    return D0
+
+
+def test_standard_volume_term_is_the_exact_divergence_on_affine_elements():
+    """VolumeKernelV99.okl:17-167 with the plain derivative matrix returns -(df/dx + dg/dy) -
+    g h (0, Bx, By) exactly for fluxes of degree <= N on skewed affine elements (the divergence is
+    formed here by complex-step differentiation of the physical flux), and zero for a constant
+    state on the curved mesh.  A transposed D, a swapped metric or a dropped term fails this."""
+    case = Case.gaussian_bump(N=4, nel=2)
+    ngl, n2 = case.ngl, case.ngl ** 2
+    D0 = d0_matrix(case)
+    assert np.abs(D0 @ lgl_nodes(ngl) - 1.0).max() < 1e-13           # D0 differentiates x exactly
+    xi = lgl_nodes(ngl)
+    XI, ETA = np.meshgrid(xi, xi)                                      # [eta row][xi col]
+    rng = np.random.default_rng(5)
+    maps = [(0.3, 0.7, 0.9, 0.2, -0.25, 1.1), (-0.2, 0.4, 1.3, -0.5, 0.6, 0.8), (1.0, -1.0, 0.5, 0.0, 0.0, 0.5)]
+    E = len(maps)
+
+    def state(x, y):
+        h = 1.5 + 0.3 * x - 0.2 * y + 0.1 * x * y - 0.05 * y * y
+        u = 0.4 - 0.3 * x + 0.2 * y
+        v = -0.1 + 0.25 * x + 0.15 * y
+        return h, h * u, h * v
+
+    def fluxes(x, y):
+        h, hu, hv = state(x, y)
+        f = [hu, hu * hu / h + 0.5 * G * h * h, hu * hv / h]
+        g = [hv, hu * hv / h, hv * hv / h + 0.5 * G * h * h]
+        return f, g
+
+    arr = {k: np.zeros((E, n2)) for k in ("xXi", "xEta", "yXi", "yEta", "Jac", "Bx", "By")}
+    q = np.zeros((E, 3, n2))
+    ref = np.zeros((E, 3, n2))
+    for e, (x0, y0, a, b, c, d) in enumerate(maps):
+        x = (x0 + a * XI + b * ETA).ravel()
+        y = (y0 + c * XI + d * ETA).ravel()
+        arr["xXi"][e], arr["xEta"][e], arr["yXi"][e], arr["yEta"][e] = a, b, c, d
+        arr["Jac"][e] = 1.0 / (a * d - b * c)
+        arr["Bx"][e] = rng.uniform(-1, 1, n2)
+        arr["By"][e] = rng.uniform(-1, 1, n2)
+        q[e] = np.array(state(x, y))
+        hstep = 1e-30
+        fx = np.array(fluxes(x + 1j * hstep, y)[0]).imag / hstep
+        gy = np.array(fluxes(x, y + 1j * hstep)[1]).imag / hstep
+        ref[e] = -(fx + gy)
+        ref[e, 1] -= G * q[e, 0] * arr["Bx"][e]
+        ref[e, 2] -= G * q[e, 0] * arr["By"][e]
+    Qt = np.zeros(E * 3 * n2)
+    p = O.OracleParams(ngl, 1, G, 1.0, 1e-12, 1e-5, 0.0, 0.0, 0.0)
+    O.call("volume_std", p, E, *(np.ascontiguousarray(arr[k].ravel()) for k in ("Jac", "yXi", "yEta", "xXi", "xEta")),
+           np.ascontiguousarray(q.ravel()), np.ascontiguousarray(D0.ravel()),
+           np.ascontiguousarray(arr["Bx"].ravel()), np.ascontiguousarray(arr["By"].ravel()), Qt)
+    assert np.abs(Qt - ref.ravel()).max() <= 1e-12 * np.abs(ref).max()
+    # free stream on the curved mesh (flat bottom): the discrete metric identities cancel every term
+    cv = Case.dam_break(N=5, nel=3, seed=2)
+    qc = np.zeros((cv.Nelem, 3, cv.ngl ** 2))
+    qc[:, 0], qc[:, 1], qc[:, 2] = 1.3, 0.4, -0.2
+    Qc = np.zeros(qc.size)
+    z = np.zeros(cv.Nelem * cv.ngl ** 2)
+    a_ = cv.arrays
+    O.call("volume_std", O.params(cv, RunParams()), cv.Nelem, a_["Jac"], a_["yXi"], a_["yEta"], a_["xXi"],
+           a_["xEta"], np.ascontiguousarray(qc.ravel()), np.ascontiguousarray(d0_matrix(cv).ravel()), z, z, Qc)
+    assert np.abs(Qc).max() < 1e-14 * term_scale(cv, qc.ravel())
+
+
+def test_ssp_rk3_update_is_third_order():
+    """UpdateKernelSSP_V1.okl:17-98 (q = a qTmp + b q + c dt Qt, qTmp the step's start state)
+    driven with the reference's SSP table (RungeKutta.cpp:25-38) integrates y' = lam y at third
+    order; swapping a and b, or dropping dt, breaks the order."""
+    from esdgsem_mpiocca_amd import rk_coefficients
+    A, B, Cc = rk_coefficients(3, 1)
+    assert len(A) == 3
+    lam = -1.1 + 0.9j
+    p = O.OracleParams(1, 1, G, 1.0, 0.0, 1e-5, 0.0, 0.0, 0.0)
+    errs = []
+    for nsteps in (20, 40, 80):
+        dt = 1.0 / nsteps
+        y = np.array([1.0, 0.0, 0.0])
+        for _ in range(nsteps):
+            y0 = y.copy()
+            for s in range(3):
+                f = lam * (y[0] + 1j * y[1])
+                Qt = np.array([f.real, f.imag, 0.0])
+                O.call("rk_ssp_update", p, 1, float(A[s]), float(B[s]), float(Cc[s]), dt, Qt, y0, y)
+        errs.append(abs(y[0] + 1j * y[1] - np.exp(lam)))
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert (orders > 2.85).all() and (orders < 3.3).all(), orders
