@@ -19,7 +19,8 @@ struct esdg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     // a second stream (default priority) for work that runs beside the main stream's, forked from
-    // and joined back into it by the two events; created on first use
+    // and joined back into it by the two events inside one call; created by esdg_create (null if
+    // that failed: the work then runs on the main stream)
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     esdg_params p{};

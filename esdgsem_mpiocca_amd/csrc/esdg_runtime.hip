@@ -102,6 +102,18 @@ int esdg_create(esdg_ctx** out, int hip_device, const esdg_params* params) {
         delete ctx;
         return set_error(nullptr, ESDG_ERR_NOMEM, "esdg_create: scratch allocation failed");
     }
+    // The second stream and its fork/join events, created once here.  If any of them cannot be
+    // created, work meant for the second stream runs on the main stream in order (same results);
+    // the failure's error state is cleared so that no later launch check reports it.
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess) {
+        for (hipEvent_t* ev : {&ctx->fork, &ctx->join})
+            if (*ev) { (void)hipEventDestroy(*ev); *ev = nullptr; }
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
+        ctx->side = nullptr;
+        (void)hipGetLastError();
+    }
     *out = ctx;
     return ESDG_OK;
 }

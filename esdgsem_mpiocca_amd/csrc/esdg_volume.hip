@@ -457,19 +457,8 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
     if (rk && Qv) {
         // the listed marked elements on the side stream, beside the unmarked ones on the main stream
         // (disjoint sets, each updated once; nothing else is read or written by both), joined back
-        // before anything that follows on the main stream
-        if (!ctx->side) {
-            if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) ctx->side = nullptr;
-            if (ctx->side && (hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
-                              hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess)) {
-                // without the events both launches stay on the main stream, in order
-                for (hipEvent_t* e : {&ctx->fork, &ctx->join})
-                    if (*e) { (void)hipEventDestroy(*e); *e = nullptr; }
-                (void)hipStreamDestroy(ctx->side);
-                ctx->side = nullptr;
-                (void)hipGetLastError();
-            }
-        }
+        // before anything that follows on the main stream (without a second stream, created with
+        // the context, both launches stay on the main stream, in order)
         hipStream_t ls = ctx->side ? ctx->side : ctx->stream;
         if (ctx->side) {
             (void)hipEventRecord(ctx->fork, ctx->stream);

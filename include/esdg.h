@@ -13,8 +13,11 @@
  * are the fields of esdg_params and are fixed at esdg_create.  ngl = N+1 is a runtime value that
  * selects an ahead-of-time compiled kernel instance (2 <= ngl <= 10).
  *
- * Thread-safety: one context is one device stream, used from one host thread at a time
- * (the reference's one rank per device).
+ * Thread-safety: one context is used from one host thread at a time (the reference's one rank
+ * per device).  All of a context's work is ordered on its stream, esdg_stream(ctx).  With
+ * artificial viscosity the fused state update also runs one launch on a second stream that the
+ * context owns; that launch is forked from and joined back into the main stream inside the same
+ * call, so synchronising esdg_stream(ctx) (or esdg_finish) covers it.
  */
 #ifndef ESDG_H
 #define ESDG_H
