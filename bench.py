@@ -187,95 +187,115 @@ def ensemble_bench(dev, args):
     }
 
 
-def c4_bench(dev, args):
+def pos_pres_bytes(nelem, ngl, sensor=True, stages=5):
+    """Average algorithmic bytes per launch of the fused schedule's limiter pass (PosPres.okl plus
+    the side traces, sensor and wave speed of the limited state): per node q 3, 1/J and (for the
+    sensor) b read; per element the side traces (12 ngl) written, EleSizes read, the viscosity and
+    its plot copy written and, in the last of the step's stages, the wave speed.  The state is
+    rewritten only where the limiter changes it (data dependent, not counted)."""
+    per_elem = (4 + (1 if sensor else 0)) * ngl * ngl + 12 * ngl + 1 + (2 if sensor else 0) + 1.0 / stages
+    return int(round(8 * nelem * per_elem))
+
+
+def c4_bench(dev, args, steps):
     """Wet/dry flood on the curved mesh family, N=7, positivity limiter, artificial viscosity: the
-    modal sensor everywhere against the learned indicator (modal-sensor fallback)."""
+    modal sensor everywhere against the learned indicator (modal-sensor fallback).  The case is
+    built on the device (esdg_device_case_create) and handed to each solver device to device."""
     import numpy as np
     from esdgsem_mpiocca_amd import indicator
     params, meta = indicator.load_weights()
     if params is None:
         raise SystemExit("no trained indicator weights (esdgsem_mpiocca_amd/indicator_weights.json)")
-    t0 = time.perf_counter()
     from esdgsem_mpiocca_amd import Case
-    case = Case.flood(N=7, nel=args.c4_nel, seed=0)
+    spec = Case.flood(N=7, nel=args.c4_nel, seed=0, build=False)
     # CFL 0.1: the dt formula's CFL/ngl stays below the smallest GLL weight fraction 1/(N(N+1)),
     # the step size under which the limiter keeps the element means positive.  Dry tolerance 0.1:
     # without the partial-dry treatment, thinner layers carry spurious velocities at the fronts
     # (tolerances 1e-2 and 5e-2 reach NaN or |u| of hundreds of m/s within 1000 steps at 2048^2).
     rp = indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1)
-    print(f"# c4: {case.Nelem} elements, case built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    out = {}
+    t0 = time.perf_counter()
+    cctx = dev.Context(dev.make_params(spec, rp), device=0)
+    dcase = dev.DeviceCase(cctx, spec)
+    cctx.finish()
+    t_case = time.perf_counter() - t0
+    print(f"# c4: {dcase.Nelem} elements, case built on the device in {t_case:.2f} s", file=sys.stderr, flush=True)
+    out = {"case_build_s": round(t_case, 3)}
     qs = {}
-    dofs = 3 * case.ngl ** 2 * case.Nelem
+    dofs = 3 * dcase.ngl ** 2 * dcase.Nelem
     t_end = None
-    # plain: --c4-steps steps; learned and audit: to the plain run's end time, so that the states
-    # are compared at the same simulated time (dt depends on the viscosity through the DFL limit).
-    # control: the plain method with eps0 changed by 1e-12 (relative), to the same time: how far
-    # roundoff-sized differences travel in this flow, the scale to read the learned change against.
-    for mode in ("plain", "learned", "audit", "control"):
-        eps0 = 0.005 * (1.0 + 1e-12) if mode == "control" else 0.005
-        rpm = rp if mode == "plain" else indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1, T=t_end,
-                                                                 eps0=eps0)
-        ctx = dev.Context(dev.make_params(case, rpm), device=0)
-        try:
-            solver = dev.Solver(ctx, case, rpm, fused=1)
-            net = None
-            if mode in ("learned", "audit"):
-                net = dev.MLP(ctx, params=params)
-                solver.set_indicator(net, args.ind_lo, args.ind_hi)
-                solver.indicator_audit(mode == "audit")
-            steps = args.c4_steps if mode == "plain" else 2 * args.c4_steps
-            done, wall, t_sim = 0, 0.0, 0.0
-            chunk = max(1, min(args.c4_steps, 50))
-            per_kernel = {}
-            while done < steps:
-                k = min(chunk, steps - done)
-                prof = mode in ("plain", "learned") and not per_kernel and done + k >= min(args.c4_steps, steps) - chunk
-                if prof:   # per-kernel times from one chunk near the end
-                    ctx.profile(True)
-                    ctx.profile_reset()
-                t1 = time.perf_counter()
-                n, t_sim = solver.run(k)
-                ctx.finish()
-                wall += time.perf_counter() - t1
-                done += n
-                if prof:
-                    for kn in KERNELS:
-                        launches, ms = ctx.profile_query(kn)
-                        if launches:
-                            per_kernel[kn] = {"avg_us": round(1e3 * ms / launches, 2),
-                                              "ms_per_step": round(ms / max(n, 1), 4)}
-                    ctx.profile(False)
-                print(f"# c4 {mode}: {done} steps, t = {t_sim:.6g}, {1e3 * wall / max(done, 1):.1f} ms/step",
-                      file=sys.stderr, flush=True)
-                if n != k:
-                    if mode != "plain" and t_sim >= t_end:
-                        break
-                    raise SystemExit(f"c4 {mode}: solver stopped after {done} steps")
-            if mode == "plain":
-                t_end = t_sim
-            counts = solver.indicator_counts() if net else None
-            if mode == "control":
-                qs[mode] = solver.get_q()
-            elif mode != "audit":
-                qs[mode] = solver.get_q()
-                out[mode] = {"dof_stage_updates_per_s": dofs * 5 * done / wall, "ms_per_step": 1e3 * wall / done,
-                             "steps": done, "sim_time": t_sim, "kernels": per_kernel}
-                if counts:
-                    es = max(counts["element_stages"], 1)
-                    out[mode]["viscous_element_stage_fraction"] = counts["viscous"] / es
-                    out[mode]["sensor_evaluated_fraction"] = counts["not_smooth"] / es
-            else:
-                out["audit"] = {"steps": done, **counts,
-                                "false_negative_rate": counts["false_negatives"] / max(counts["sensor_positives"], 1)}
-            solver.close()
-            if net:
-                net.close()
-        finally:
-            ctx.close()
-    A = qs["learned"].reshape(case.Nelem, 3, -1)
-    B = qs["plain"].reshape(case.Nelem, 3, -1)
-    Cq = qs["control"].reshape(case.Nelem, 3, -1)
+    try:
+        # plain: `steps` steps; learned and audit: to the plain run's end time, so that the states
+        # are compared at the same simulated time (dt depends on the viscosity through the DFL
+        # limit).  control: the plain method with eps0 changed by 1e-12 (relative), to the same
+        # time: how far roundoff-sized differences travel in this flow, the scale to read the
+        # learned change against.
+        for mode in ("plain", "learned", "audit", "control"):
+            eps0 = 0.005 * (1.0 + 1e-12) if mode == "control" else 0.005
+            rpm = rp if mode == "plain" else indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1,
+                                                                     T=t_end, eps0=eps0)
+            ctx = dev.Context(dev.make_params(spec, rpm), device=0)
+            try:
+                solver = dev.Solver(ctx, dcase, rpm, fused=1)
+                net = None
+                if mode in ("learned", "audit"):
+                    net = dev.MLP(ctx, params=params)
+                    solver.set_indicator(net, args.ind_lo, args.ind_hi)
+                    solver.indicator_audit(mode == "audit")
+                cap = steps if mode == "plain" else 2 * steps
+                done, wall, t_sim = 0, 0.0, 0.0
+                chunk = max(1, min(steps, 50))
+                per_kernel = {}
+                while done < cap:
+                    k = min(chunk, cap - done)
+                    # per-kernel times from the second chunk (the first holds the run's first-step passes)
+                    prof = mode in ("plain", "learned") and not per_kernel and (done > 0 or cap <= chunk)
+                    if prof:
+                        ctx.profile(True)
+                        ctx.profile_reset()
+                    t1 = time.perf_counter()
+                    n, t_sim = solver.run(k)
+                    ctx.finish()
+                    wall += time.perf_counter() - t1
+                    done += n
+                    if prof:
+                        for kn in KERNELS:
+                            launches, ms = ctx.profile_query(kn)
+                            if launches:
+                                per_kernel[kn] = {"avg_us": round(1e3 * ms / launches, 2),
+                                                  "ms_per_step": round(ms / max(n, 1), 4)}
+                        ctx.profile(False)
+                    print(f"# c4 {mode}: {done} steps, t = {t_sim:.6g}, {1e3 * wall / max(done, 1):.1f} ms/step",
+                          file=sys.stderr, flush=True)
+                    if n != k:
+                        if mode != "plain" and t_sim >= t_end:
+                            break
+                        raise SystemExit(f"c4 {mode}: solver stopped after {done} steps")
+                if mode == "plain":
+                    t_end = t_sim
+                counts = solver.indicator_counts() if net else None
+                if mode != "audit":
+                    qs[mode] = solver.get_q()
+                if mode in ("plain", "learned"):
+                    out[mode] = {"dof_stage_updates_per_s": dofs * 5 * done / wall, "ms_per_step": 1e3 * wall / done,
+                                 "steps": done, "sim_time": t_sim, "kernels": per_kernel}
+                    if counts:
+                        es = max(counts["element_stages"], 1)
+                        out[mode]["viscous_element_stage_fraction"] = counts["viscous"] / es
+                        out[mode]["sensor_evaluated_fraction"] = counts["not_smooth"] / es
+                elif mode == "audit":
+                    out["audit"] = {"steps": done, **counts,
+                                    "false_negative_rate": counts["false_negatives"] / max(counts["sensor_positives"], 1)}
+                solver.close()
+                if net:
+                    net.close()
+            finally:
+                ctx.close()
+    finally:
+        dcase.close()
+        cctx.close()
+    A = qs["learned"].reshape(dcase.Nelem, 3, -1)
+    B = qs["plain"].reshape(dcase.Nelem, 3, -1)
+    Cq = qs["control"].reshape(dcase.Nelem, 3, -1)
     out["l2_change_vs_plain"] = [float(np.linalg.norm(A[:, v] - B[:, v]) / max(np.linalg.norm(B[:, v]), 1e-300))
                                  for v in range(3)]
     out["control_l2_change_eps0_1e-12"] = [float(np.linalg.norm(Cq[:, v] - B[:, v]) /
@@ -284,8 +304,30 @@ def c4_bench(dev, args):
     out["min_h"] = float(min(A[:, 0].min(), B[:, 0].min()))
     out["speedup_learned_vs_plain"] = out["learned"]["dof_stage_updates_per_s"] / out["plain"]["dof_stage_updates_per_s"]
     out["indicator"] = {"lo": args.ind_lo, "hi": args.ind_hi, "validation": meta.get("validation")}
+    # rooflines of the plain run's two dominant kernels (HIP events around every launch of the
+    # profiled chunk): the fused volume pass (no side traces or sensor: the limiter pass writes
+    # them) and the limiter pass
+    pk = out["plain"]["kernels"]
+    roof = {}
+    for name, alg in (("fused_volume", fused_volume_bytes(dcase.Nelem, dcase.ngl, 1, traces=False)),
+                      ("pos_pres", pos_pres_bytes(dcase.Nelem, dcase.ngl))):
+        if name in pk:
+            gbs = alg / (pk[name]["avg_us"] * 1e-6) / 1e9
+            roof[name] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": alg,
+                          "avg_launch_us": pk[name]["avg_us"]}
+    if "fused_volume" in roof:
+        tfl = volume_flops(dcase.Nelem, dcase.ngl) / (pk["fused_volume"]["avg_us"] * 1e-6) / 1e12
+        roof["fused_volume"]["fp64_tflops_algorithmic"] = round(tfl, 2)
+        roof["fused_volume"]["fp64_frac_of_peak"] = round(tfl / FP64_PEAK_TFLOPS, 4)
+    out["roofline"] = roof
+    out["metric"] = "FP64 DOF-stage updates/s at N=7 (curved wet/dry flood, limiter, AV)"
+    out["value"] = out["plain"]["dof_stage_updates_per_s"]
+    out["unit"] = "DOF-stage updates/s"
     out["config"] = {"workload": f"curved flood {args.c4_nel}x{args.c4_nel}, N=7, 16-mode Fourier bathymetry "
-                                 "(amplitude 0.3), reservoir over dry land, positivity limiter, AV, rotated ES, LSRK4(5)"}
+                                 "(amplitude 0.3), reservoir over dry land, positivity limiter, AV, rotated ES, "
+                                 "LSRK4(5); learned indicator with the modal sensor as fallback beside it",
+                     "Nelem": dcase.Nelem, "N": dcase.N, "dofs": dofs}
     return out
 
 
@@ -315,7 +357,10 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=25)
     ap.add_argument("--no-profile", action="store_true", help="time without per-launch events")
     ap.add_argument("--c4-nel", type=int, default=2048)
-    ap.add_argument("--c4-steps", type=int, default=1000)
+    ap.add_argument("--c4-steps", type=int, default=None,
+                    help="steps of the 2048^2 flood's plain run (default: 1000 with --workload c4, 100 beside the "
+                         "headline line)")
+    ap.add_argument("--c4", type=int, default=1, help="also run the 2048^2 flood beside the headline line")
 
     ap.add_argument("--ind-lo", type=float, default=0.05)
     ap.add_argument("--ind-hi", type=float, default=0.5)
@@ -340,7 +385,7 @@ def main():
     dev = E.device()
 
     if args.workload in ("ensemble", "c4"):
-        out = ensemble_bench(dev, args) if args.workload == "ensemble" else c4_bench(dev, args)
+        out = ensemble_bench(dev, args) if args.workload == "ensemble" else c4_bench(dev, args, args.c4_steps or 1000)
         out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                     "dtype": "f64", "data": "synthetic"})
         print(json.dumps(out), flush=True)
@@ -451,6 +496,11 @@ def main():
             out["ensemble"] = ensemble_bench(dev, args)
         except Exception as e:  # reported beside the headline figure; must not hide it
             out["ensemble"] = {"error": str(e)}
+    if args.c4:
+        try:
+            out["c4"] = c4_bench(dev, args, args.c4_steps or 100)
+        except (Exception, SystemExit) as e:  # reported beside the headline figure; must not hide it
+            out["c4"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
 
 

@@ -41,6 +41,7 @@ HIP_SOURCES = [
     "esdg_stage.hip",
     "esdg_ensemble.hip",
     "esdg_mlp.hip",
+    "esdg_geometry.hip",
     "esdg_deviceclass.hip",
 ]
 
@@ -60,13 +61,23 @@ def build_setup() -> str:
 
 
 def build_hip() -> str:
+    """Each source is compiled to an object of its own, in parallel (no device symbol crosses
+    translation units), then the objects are linked into libesdg.so."""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(PKG, "libesdg.so")
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     if not srcs:
         raise RuntimeError("no HIP sources found")
-    cmd = [HIPCC] + HIP_FLAGS + [f"-I{INCLUDE}", f"-I{CSRC}"] + srcs + [
-        "-o", out + ".tmp", f"-L{PKG}", "-lesdg_setup", "-Wl,-rpath,$ORIGIN"]
-    _run(cmd)
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, [HIPCC] + compile_flags + [f"-I{INCLUDE}", f"-I{CSRC}", "-c", s, "-o", o])
+                  for s, o in zip(srcs, objs)]:
+            f.result()
+    _run([HIPCC] + HIP_FLAGS + objs + ["-o", out + ".tmp", f"-L{PKG}", "-lesdg_setup", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <vector>
 
+#include "esdg_device_setup.h"
 #include "esdg_fused.h"
 #include "esdg_internal.h"
 #include "esdg_mlp_internal.h"
@@ -71,14 +72,15 @@ class deviceclass {
         return rc;
     }
 
-    // copyDeviceVariables (src/deviceclass.cpp:577-626)
-    int copyDeviceVariables(const esdg_inputs& in) {
+    // copyDeviceVariables (src/deviceclass.cpp:577-626); kind ESDG_H2D for host inputs, ESDG_D2D
+    // for inputs already on the device
+    int copyDeviceVariables(const esdg_inputs& in, int kind = ESDG_H2D) {
         const size_t nodes = (size_t)Nelem * ngl2, fn = (size_t)Nfaces * ngl;
         int rc = ESDG_OK;
         auto C = [&](void* d, const void* h, size_t bytes) {
             if (rc == ESDG_OK) {
                 if (!h) rc = set_error(ctx_, ESDG_ERR_ARG, "esdg_solver_create: missing input array");
-                else rc = esdg_memcpy(ctx_, d, h, bytes, ESDG_H2D);
+                else rc = esdg_memcpy(ctx_, d, h, bytes, kind);
             }
         };
         const size_t dd = sizeof(double);
@@ -494,7 +496,8 @@ using namespace esdg;
 
 extern "C" {
 
-int esdg_solver_create(esdg_ctx* ctx, const esdg_inputs* in, const esdg_run_params* rp, esdg_solver** out) {
+static int solver_create(esdg_ctx* ctx, const esdg_inputs* in, const esdg_run_params* rp, esdg_solver** out,
+                         int kind) {
     ESDG_REQUIRE(ctx, ctx && in && rp && out, "esdg_solver_create: null argument");
     *out = nullptr;
     ESDG_REQUIRE(ctx, in->Nelem > 0 && in->Nfaces > 0, "esdg_solver_create: empty mesh");
@@ -504,10 +507,18 @@ int esdg_solver_create(esdg_ctx* ctx, const esdg_inputs* in, const esdg_run_para
     if (!s) return set_error(ctx, ESDG_ERR_NOMEM, "esdg_solver_create: out of host memory");
     int rc = s->dc.initDeviceVariables(*in, *rp);
     if (!rc) rc = s->dc.buildDeviceKernels();
-    if (!rc) rc = s->dc.copyDeviceVariables(*in);
+    if (!rc) rc = s->dc.copyDeviceVariables(*in, kind);
     if (rc) { delete s; return rc; }
     *out = s;
     return ESDG_OK;
+}
+
+int esdg_solver_create(esdg_ctx* ctx, const esdg_inputs* in, const esdg_run_params* rp, esdg_solver** out) {
+    return solver_create(ctx, in, rp, out, ESDG_H2D);
+}
+
+int esdg_solver_create_dev(esdg_ctx* ctx, const esdg_inputs* in, const esdg_run_params* rp, esdg_solver** out) {
+    return solver_create(ctx, in, rp, out, ESDG_D2D);
 }
 
 int esdg_solver_run(esdg_solver* s, int max_steps, int* steps_taken, double* t_out) {

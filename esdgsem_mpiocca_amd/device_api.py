@@ -82,6 +82,13 @@ def lib():
         L.esdg_solver_last_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.esdg_solver_buffer.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]
         L.esdg_solver_destroy.argtypes = [C.c_void_p]
+        L.esdg_solver_create_dev.argtypes = [C.c_void_p, C.POINTER(Inputs), C.POINTER(RunParamsC),
+                                             C.POINTER(C.c_void_p)]
+        L.esdg_device_case_inputs.argtypes = [C.c_void_p, C.POINTER(Inputs)]
+        L.esdg_device_case_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p),
+                                             C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        L.esdg_device_case_get_info.argtypes = [C.c_void_p, C.c_void_p]
+        L.esdg_device_case_destroy.argtypes = [C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -249,17 +256,81 @@ def make_run_params(case: Case, rp: RunParams, fused: int = 0) -> RunParamsC:
                       int(rp.arrival_times), int(rp.maximum_elevation))
 
 
+class DeviceCase:
+    """esdg_device_case (include/esdg_device_setup.h): a structured-mesh case (Case.<config>(...,
+    build=False) gives its spec) built on the device of ctx.  Has the Case attributes the solver
+    uses (N, ngl, Nelem, Nfaces, min_ele_size); arrays stay on the device."""
+
+    def __init__(self, ctx: Context, case: Case):
+        from .setup_api import CaseInfo
+        self.ctx = ctx
+        h = C.c_void_p()
+        L = lib()
+        L.esdg_device_case_create.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
+        ctx.check(L.esdg_device_case_create(C.c_void_p(ctx.handle), C.byref(case.spec), C.byref(h)))
+        self.handle = h.value
+        info = CaseInfo()
+        ctx.check(L.esdg_device_case_get_info(C.c_void_p(self.handle), C.byref(info)))
+        self.N, self.ngl, self.Nelem, self.Nfaces = info.N, info.ngl, info.Nelem, info.Nfaces
+        self.min_ele_size = info.min_ele_size
+        self.spec = case.spec
+
+    @property
+    def geom_face(self) -> float:
+        from .setup_api import basis
+        return 1.0 / float(basis(self.N)["w_GL"][0])
+
+    def array_ptr(self, name: str):
+        p, n, isint = C.c_void_p(), C.c_int64(), C.c_int()
+        self.ctx.check(lib().esdg_device_case_array(C.c_void_p(self.handle), name.encode(), C.byref(p),
+                                                    C.byref(n), C.byref(isint)))
+        return p.value, n.value, bool(isint.value)
+
+    def view(self, name: str) -> DeviceView:
+        p, n, isint = self.array_ptr(name)
+        return DeviceView(self.ctx, p, n, np.int32 if isint else np.float64)
+
+    def download(self, name: str) -> np.ndarray:
+        return self.view(name).download()
+
+    def to_host_case(self) -> Case:
+        """A host Case holding copies of every device array (for the oracle and host checks)."""
+        from .setup_api import DOUBLE_ARRAYS, INT_ARRAYS
+        c = Case.__new__(Case)
+        c.spec = self.spec
+        c._mesh_file = None
+        c.N, c.ngl, c.Nelem, c.Nfaces = self.N, self.ngl, self.Nelem, self.Nfaces
+        c.min_ele_size = self.min_ele_size
+        c.arrays = {n: self.download(n) for n in DOUBLE_ARRAYS + INT_ARRAYS}
+        return c
+
+    def close(self):
+        if self.handle:
+            lib().esdg_device_case_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+
 class Solver:
-    """deviceclass mirror: buffers + DGtimeloop on the device for one Case.
+    """deviceclass mirror: buffers + DGtimeloop on the device for one Case (host arrays) or
+    DeviceCase (arrays already on the device, copied device to device).
     fused: 0 the reference's kernel sequence, 1 fused stage kernels, 2 single-kernel stage."""
 
-    def __init__(self, ctx: Context, case: Case, rp: RunParams, fused: int = 0):
+    def __init__(self, ctx: Context, case, rp: RunParams, fused: int = 0):
         self.ctx, self.case, self.rp = ctx, case, rp
-        inp, self._keep = make_inputs(case, rp)
         rpc = make_run_params(case, rp, fused)
         h = C.c_void_p()
-        ctx.check(lib().esdg_solver_create(C.c_void_p(ctx.handle), C.byref(inp), C.byref(rpc),
-                                           C.byref(h)))
+        if isinstance(case, DeviceCase):
+            if not rp.flux_differencing:
+                raise ValueError("a DeviceCase carries the flux-differencing D only")
+            inp = Inputs()
+            ctx.check(lib().esdg_device_case_inputs(C.c_void_p(case.handle), C.byref(inp)))
+            self._keep = [case]
+            ctx.check(lib().esdg_solver_create_dev(C.c_void_p(ctx.handle), C.byref(inp), C.byref(rpc),
+                                                   C.byref(h)))
+        else:
+            inp, self._keep = make_inputs(case, rp)
+            ctx.check(lib().esdg_solver_create(C.c_void_p(ctx.handle), C.byref(inp), C.byref(rpc),
+                                               C.byref(h)))
         self.handle = h.value
 
     def run(self, max_steps: int):

@@ -105,12 +105,23 @@ class Case:
 
     def __init__(self, case_kind, mesh_kind, N, nelx, nely, xL, xR, yL, yR, periodic_x=0,
                  periodic_y=0, perturb_amp=0.0, perturb_len=1.0, g=9.81, amp=0.0, p=(0, 0, 0, 0),
-                 seed=0, mesh_file=None, read_bottom=0, convert_to_km=0):
+                 seed=0, mesh_file=None, read_bottom=0, convert_to_km=0, build=True):
+        """build=False keeps only the spec and the sizes (for esdg_device_case_create, which
+        builds the arrays on the device); `arrays` is then None."""
         self._mesh_file = None if mesh_file is None else os.fsencode(mesh_file)
         spec = CaseSpec(case_kind, mesh_kind, N, nelx, nely, xL, xR, yL, yR, periodic_x,
                         periodic_y, perturb_amp, perturb_len, g, amp, p[0], p[1], p[2], p[3], seed,
                         self._mesh_file, int(read_bottom), int(convert_to_km))
         self.spec = spec
+        if not build:
+            if mesh_kind == MESH_ISM:
+                raise ValueError("an ISM mesh is read on the host: build=True")
+            self.N, self.ngl, self.Nelem = N, N + 1, nelx * nely
+            self.Nfaces = ((nelx + 1) * nely + (nely + 1) * nelx - (nelx if periodic_x else 0)
+                           - (nely if periodic_y else 0))
+            self.min_ele_size = None
+            self.arrays = None
+            return
         h = C.c_void_p()
         _check(_lib().esdg_case_create(C.byref(spec), C.byref(h)))
         try:
@@ -135,7 +146,8 @@ class Case:
 
     @property
     def geom_face(self) -> float:
-        return 1.0 / float(self.arrays["w_GL"][0])
+        w = self.arrays["w_GL"] if self.arrays is not None else basis(self.N)["w_GL"]
+        return 1.0 / float(w[0])
 
     @classmethod
     def from_ism(cls, path, N, case_kind=CASE_CONSTANT, p=(1.0, 0.0, 0.0, 0.0), amp=0.0, seed=0,
@@ -148,27 +160,27 @@ class Case:
 
     # --- the benchmark configurations -------------------------------------------------
     @classmethod
-    def gaussian_bump(cls, N=4, nel=16, lake_at_rest=False):
+    def gaussian_bump(cls, N=4, nel=16, lake_at_rest=False, build=True):
         """16x16 periodic Cartesian mesh on [0,2]^2, b = 0.8 exp(-5 r^2), h = 1-b (+0.01 bump)."""
         return cls(CASE_GAUSSIAN_BUMP, MESH_CARTESIAN, N, nel, nel, 0.0, 2.0, 0.0, 2.0, 1, 1,
-                   amp=0.0 if lake_at_rest else 0.01)
+                   amp=0.0 if lake_at_rest else 0.01, build=build)
 
     @classmethod
-    def dam_break(cls, N=7, nel=512, noise=1e-3, seed=0, perturb=0.05, ratio=0.0, angle=0.0):
+    def dam_break(cls, N=7, nel=512, noise=1e-3, seed=0, perturb=0.05, ratio=0.0, angle=0.0, build=True):
         """nel^2 curved quads on [0,10]^2 (x,y += 0.05 sin(pi x/5) sin(pi y/5)), walls, dam break:
         h = 2 | 1 across x = 5, or with ratio/angle set h = ratio | 1 across the line through (5, 5)
         with normal angle `angle`."""
         return cls(CASE_DAM_BREAK, MESH_CURVED_SINE, N, nel, nel, 0.0, 10.0, 0.0, 10.0, 0, 0,
                    perturb_amp=perturb, perturb_len=5.0, amp=noise, p=(5.0, ratio, angle, 0),
-                   seed=seed)
+                   seed=seed, build=build)
 
     @classmethod
     def flood(cls, N=7, nel=2048, seed=0, level_left=0.8, x_dam=3.0, level_right=-0.2,
-              perturb=0.05):
+              perturb=0.05, build=True):
         """nel^2 curved quads on [0,10]^2, random Fourier bathymetry, reservoir over dry land."""
         return cls(CASE_FLOOD, MESH_CURVED_SINE, N, nel, nel, 0.0, 10.0, 0.0, 10.0, 0, 0,
                    perturb_amp=perturb, perturb_len=5.0, p=(level_left, x_dam, level_right, 0),
-                   seed=seed)
+                   seed=seed, build=build)
 
     def with_state(self, q):
         """A copy of this case with another initial state q (mesh arrays shared, not copied)."""
@@ -179,14 +191,14 @@ class Case:
         return c
 
     @classmethod
-    def ocean_uplift(cls, N=4, nel=64, amp=1.0, axis_a=50e3, axis_c=30e3, angle=0.0, length=800e3):
+    def ocean_uplift(cls, N=4, nel=64, amp=1.0, axis_a=50e3, axis_c=30e3, angle=0.0, length=800e3, build=True):
         """nel^2 Cartesian quads on [0, length]^2: 4000 m ocean with a linear shelf to 50 m over the
         last 20% of x, still-water level 4000 (b = 4000 - depth), elliptic Gaussian sea-surface
         uplift of amplitude amp, axes axis_a, axis_c (m) rotated by angle, centred at (40%, 50%).
         Run it with RunParams(bc_kind=2, h_zero=4000): open boundary over the deep ocean (b <= 6.7),
         walls over the shelf."""
         return cls(CASE_OCEAN_UPLIFT, MESH_CARTESIAN, N, nel, nel, 0.0, length, 0.0, length, 0, 0,
-                   amp=amp, p=(axis_a, axis_c, angle, 0))
+                   amp=amp, p=(axis_a, axis_c, angle, 0), build=build)
 
     @classmethod
     def convergence(cls, N=3, nel=8, curved=False, perturb=0.02, testcase=1, length=None):

@@ -17,6 +17,7 @@ def declared(header):
 
 
 HEADERS = [("esdg.h", "libesdg.so"), ("esdg_ensemble.h", "libesdg.so"), ("esdg_mlp.h", "libesdg.so"),
+           ("esdg_device_setup.h", "libesdg.so"),
            ("esdg_setup.h", "libesdg_setup.so"), ("esdg_output.h", "libesdg_setup.so")]
 
 
