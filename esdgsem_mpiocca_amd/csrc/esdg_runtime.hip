@@ -2,6 +2,7 @@
 // reduction behind include/esdg.h.  Replaces the OCCA device/memory layer the reference uses
 // (src/deviceclass.cpp:36-57, 227-342) with one HIP stream per context.
 #include <cstring>
+#include <mutex>
 
 #include "esdg_internal.h"
 
@@ -72,6 +73,41 @@ __global__ void __launch_bounds__(256) max_partial_kernel(const double* __restri
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// The allocation-free variant of the max reduction (esdg_reduce_max_noalloc) keeps its block
+// partials in this file-scope array instead of the context's scratch: one partial per 256 values,
+// so it serves n <= ESDG_NOALLOC_MAX_N.  Each device holds its own copy until the process ends.
+// In host code the array's name is not a device address: kernels get the address that
+// hipGetSymbolAddress returned for the device (noalloc_scratch below), never the name.
+constexpr int NOALLOC_PARTIALS = ESDG_NOALLOC_MAX_N / 256;
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static double* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per device (by esdg_create
+// for its device, else by the first esdg_reduce_max_noalloc call there).  Null on failure, with
+// the message in *err.
+static double* noalloc_scratch(std::string* err) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        *err = std::string("no current HIP device: ") + hipGetErrorString(e);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_noalloc_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_noalloc_ptr[dev];
+}
+
 }  // namespace esdg
 
 using namespace esdg;
@@ -113,6 +149,18 @@ int esdg_create(esdg_ctx** out, int hip_device, const esdg_params* params) {
         if (ctx->side) (void)hipStreamDestroy(ctx->side);
         ctx->side = nullptr;
         (void)hipGetLastError();
+    }
+    // the allocation-free entry's scratch address on this device, looked up at set-up
+    std::string serr;
+    if (!noalloc_scratch(&serr)) {
+        (void)hipGetLastError();
+        for (hipEvent_t ev : {ctx->fork, ctx->join})
+            if (ev) (void)hipEventDestroy(ev);
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
+        (void)hipFree(ctx->d_scratch);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return set_error(nullptr, ESDG_ERR_HIP, "esdg_create: " + serr);
     }
     *out = ctx;
     return ESDG_OK;
@@ -233,6 +281,27 @@ int esdg_reduce_max(esdg_ctx* ctx, int n, const double* x, double* d_out) {
     max_partial_kernel<<<nb, 256, 0, ctx->stream>>>(x, n, ctx->d_scratch);
     max_partial_kernel<<<1, 256, 0, ctx->stream>>>(ctx->d_scratch, nb, d_out);
     return launch_check(ctx, "reduce_max");
+}
+
+int esdg_reduce_max_noalloc(void* stream, int n, const double* x, double* d_out) {
+    if (!x || !d_out || n < 0) return set_error(nullptr, ESDG_ERR_ARG, "esdg_reduce_max_noalloc: bad argument");
+    if (n > ESDG_NOALLOC_MAX_N)
+        return set_error(nullptr, ESDG_ERR_ARG,
+                         "esdg_reduce_max_noalloc: n = " + std::to_string(n) + " exceeds the maximum " +
+                             std::to_string(ESDG_NOALLOC_MAX_N) + " of its static scratch");
+    std::string err;
+    double* part = noalloc_scratch(&err);
+    if (!part) return set_error(nullptr, ESDG_ERR_HIP, "esdg_reduce_max_noalloc: " + err);
+    // every partial the second pass reads is written by the first (nb blocks, one partial each),
+    // so nothing left by an earlier call is read and nothing needs clearing
+    const int nb = std::max(1, (n + 255) / 256);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    max_partial_kernel<<<nb, 256, 0, s>>>(x, n, part);
+    max_partial_kernel<<<1, 256, 0, s>>>(part, nb, d_out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(nullptr, ESDG_ERR_HIP, std::string("esdg_reduce_max_noalloc: launch: ") + hipGetErrorString(e));
+    return ESDG_OK;
 }
 
 }  // extern "C"

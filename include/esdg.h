@@ -208,6 +208,18 @@ int esdg_total_entropy_dev(esdg_ctx* ctx, int Nelem, const double* q, const doub
  * GetGlobalLambdaMax / GetGlobalViscParaMax (src/MPI_Communication.cpp:490-541); max is exact. */
 int esdg_reduce_max(esdg_ctx* ctx, int n, const double* x, double* d_out);
 
+/* The max of esdg_reduce_max without a context and without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Launches on `stream` (a hipStream_t; null is
+ * the default stream) of the current device; the block partials live in a static device array of
+ * ESDG_NOALLOC_MAX_N / 256 doubles (128 KiB) that each device holds until the process ends, so n
+ * must be <= ESDG_NOALLOC_MAX_N (larger n is rejected with ESDG_ERR_ARG).  The array's address is
+ * looked up once per device (by esdg_create for its device, else by the first call there).  Every
+ * call on a device shares that array: calls that overlap on one device (two streams, two host
+ * threads) race.  Errors are reported through esdg_last_error(NULL). */
+#define ESDG_NOALLOC_MAX_N 4194304
+int esdg_reduce_max_noalloc(void* stream, int n, const double* x, double* d_out);
+
 /* ---- the time loop (deviceclass::DGtimeloop, src/deviceclass.cpp:630-1130) ------------- */
 
 typedef struct esdg_run_params {
