@@ -155,12 +155,12 @@ class deviceclass {
     // The element viscosity of the current state (ShockCapturing, src/deviceclass.cpp:918 and
     // :1045): the modal sensor, or with the learned indicator the network's classes and the sensor
     // where the network does not settle the element as smooth.
-    int viscosity() {
+    // clock_mode (the device clock of the fused schedule; 0 without it): the learned indicator skips
+    // a step issued after T (1: the step's first evaluation, 2: before a later stage)
+    int viscosity(int clock_mode = 0) {
         if (!ind_) return esdg_shock_capturing(ctx_, Nelem, o_q, o_B, o_VdmInv, o_EleSizes, o_ViscPara, o_ViscForPlot);
-        const int rc = mlp_classify_sensor(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, o_estat, ind_audit,
-                                           o_icount, o_ViscPara, o_ViscForPlot);
-        ind_element_stages += Nelem;
-        return rc;
+        return mlp_classify_sensor(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, o_estat, ind_audit, o_icount,
+                                   o_ViscPara, o_ViscForPlot, clock_mode ? o_clock : nullptr, rp_.T, clock_mode);
     }
 
     // One time step's dt (src/deviceclass.cpp:899-928), reductions on the device.
@@ -253,9 +253,9 @@ class deviceclass {
         if (rp_.fused == 1) {
             rc = fused_.begin(ctx_, o_q);
             if (rc) return rc;
-            // without per-step host work (analysis buffers, the learned indicator's counts) dt and
-            // t stay on the device and the host runs ahead
-            if (!rp_.arrival_times && !rp_.maximum_elevation && !ind_) return clock_timeloop(max_steps, steps_taken);
+            // without per-step host work (the analysis buffers take t from the host) dt and t stay
+            // on the device and the host runs ahead
+            if (!rp_.arrival_times && !rp_.maximum_elevation) return clock_timeloop(max_steps, steps_taken);
         }
         // q may have been set since the last call: its wave speeds and viscosity are evaluated anew
         lambda_fresh = visc_fresh = false;
@@ -310,7 +310,9 @@ class deviceclass {
     // issued after t reaches T change nothing and are not counted.  Same arithmetic as DGtimeloop.
     int clock_timeloop(int max_steps, int* steps_taken) {
         lambda_fresh = visc_fresh = false;
-        const bool fuse_sensor = rp_.artificial_viscosity != 0;
+        // with the learned indicator the network's classes come before the sensor, so the
+        // indicator pass runs before every stage instead of the sensor fused into the update
+        const bool fuse_sensor = rp_.artificial_viscosity && !ind_;
         const double dti = rp_.min_ele_size / (ngl) * rp_.CFL;
         const double dvc = rp_.DFL / (std::pow(ngl, 2)) * std::pow(rp_.min_ele_size, 2);
         double ck[8] = {0.0, t, 0.0, last_dt, 0.0, 0.0, 0.0, 0.0};
@@ -322,11 +324,13 @@ class deviceclass {
                 rc = lambda_fresh ? ESDG_OK : esdg_find_lambda_max(ctx_, Nelem, o_q, o_LambdaMax);
                 if (!rc) rc = esdg_reduce_max(ctx_, Nelem, o_LambdaMax, o_red);
                 if (!rc && rp_.artificial_viscosity) {
-                    if (!visc_fresh) rc = viscosity();
+                    if (!visc_fresh) rc = viscosity(1);
                     if (!rc) rc = esdg_reduce_max(ctx_, Nelem, o_ViscPara, o_red + 1);
                 }
                 if (!rc) rc = fused_clock(ctx_, o_red, o_clock, dti, dvc, rp_.artificial_viscosity, rp_.T);
                 for (int s = 0; s < stages && !rc; ++s) {
+                    if (rp_.artificial_viscosity && s > 0 && !visc_fresh) rc = viscosity(2);
+                    if (rc) break;
                     FusedArgs fa = fused_args();
                     if (fuse_sensor) { fa.next_visc = o_ViscPara; fa.next_visc_plot = o_ViscForPlot; }
                     if (s == stages - 1) fa.next_lambda = o_LambdaMax;
@@ -464,7 +468,6 @@ class deviceclass {
     esdg_mlp* ind_ = nullptr;
     float ind_lo = 0.0f, ind_hi = 1.0f;
     int ind_audit = 0;
-    int64_t ind_element_stages = 0;
     signed char* o_estat = nullptr;
     unsigned long long* o_icount = nullptr;
     double *o_q2 = nullptr, *o_clock = nullptr;
@@ -631,7 +634,7 @@ int esdg_solver_set_indicator(esdg_solver* s, esdg_mlp* m, float lo, float hi) {
     if (!(lo <= hi)) return set_error(d.ctx_, ESDG_ERR_ARG, "esdg_solver_set_indicator: lo > hi");
     int rc = ESDG_OK;
     if (!d.o_estat) rc = esdg_malloc(d.ctx_, (void**)&d.o_estat, (size_t)d.Nelem);
-    if (!rc && !d.o_icount) rc = esdg_malloc(d.ctx_, (void**)&d.o_icount, 4 * sizeof(unsigned long long));
+    if (!rc && !d.o_icount) rc = esdg_malloc(d.ctx_, (void**)&d.o_icount, 5 * sizeof(unsigned long long));
     if (rc) return rc;
     d.ind_ = m;
     d.ind_lo = lo;
@@ -649,10 +652,10 @@ int esdg_solver_indicator_counts(esdg_solver* s, int64_t* viscous, int64_t* fals
                                  int64_t* not_smooth, int64_t* element_stages) {
     if (!s) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_indicator_counts: null solver");
     deviceclass& d = s->dc;
-    unsigned long long c[4] = {0, 0, 0, 0};
+    unsigned long long c[5] = {0, 0, 0, 0, 0};
     if (d.o_icount) {
         int rc = esdg_memcpy(d.ctx_, c, d.o_icount, sizeof c, ESDG_D2H);
-        if (!rc) rc = esdg_fill(d.ctx_, reinterpret_cast<double*>(d.o_icount), 0.0, 4);
+        if (!rc) rc = esdg_fill(d.ctx_, reinterpret_cast<double*>(d.o_icount), 0.0, 5);
         if (!rc) rc = esdg_finish(d.ctx_);
         if (rc) return rc;
     }
@@ -660,8 +663,7 @@ int esdg_solver_indicator_counts(esdg_solver* s, int64_t* viscous, int64_t* fals
     if (false_negatives) *false_negatives = (int64_t)c[1];
     if (sensor_positives) *sensor_positives = (int64_t)c[2];
     if (not_smooth) *not_smooth = (int64_t)c[3];
-    if (element_stages) *element_stages = d.ind_element_stages;
-    d.ind_element_stages = 0;
+    if (element_stages) *element_stages = (int64_t)c[4];
     return ESDG_OK;
 }
 

@@ -242,12 +242,22 @@ struct MaskedSensor {
     SensorConsts sc{};
     int audit = 0;
     unsigned long long* counters = nullptr;
+    // the device clock (see mlp_classify_sensor): skip a step issued after T
+    const double* clock = nullptr;
+    double T = 0.0;
+    int clock_mode = 0;
 };
 
 __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* __restrict__ q,
                                                        const double* __restrict__ B, const double* __restrict__ V,
                                                        MlpDev W, float lo, float hi, float* __restrict__ p,
                                                        signed char* __restrict__ estat, MaskedSensor MS) {
+    if (MS.clock) {
+        // a step issued after T changes nothing: no evaluation, no counts (uniform per block)
+        if ((MS.clock_mode == 1 && MS.clock[1] >= MS.T) || (MS.clock_mode == 2 && MS.clock[0] == 0.0)) return;
+    }
+    if (MS.counters && threadIdx.x == 0)
+        atomicAdd(MS.counters + 4, (unsigned long long)min(FEPB, Nelem - (int)blockIdx.x * FEPB));
     __shared__ __attribute__((aligned(16))) uint16_t sW1[NH * KP], sW2[NH * NH];
     __shared__ float sw3[NH], sb[2 * NH + 1];
     __shared__ __attribute__((aligned(16))) uint16_t sX[2][16 * KP], sH[2][16 * NH];
@@ -479,9 +489,13 @@ int mlp_classify(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, c
 
 int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B,
                         const double* VdmInv, float lo, float hi, signed char* estat, int audit,
-                        unsigned long long* counters, double* visc, double* visc_plot) {
+                        unsigned long long* counters, double* visc, double* visc_plot, const double* clock,
+                        double T, int clock_mode) {
     if (ctx->p.ngl != 8) return set_error(ctx, ESDG_ERR_ARG, "the learned indicator needs N = 7");
     MaskedSensor MS;
+    MS.clock = clock;
+    MS.T = T;
+    MS.clock_mode = clock_mode;
     MS.visc = visc;
     MS.visc_plot = visc_plot;
     MS.sc = SensorConsts{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};

@@ -12,12 +12,17 @@ int mlp_classify(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, c
                  float lo, float hi, float* p, signed char* estat);
 // mlp_classify, and in the same pass the modal sensor with zero viscosity where the class is smooth
 // (from the modal coefficients the features already formed).  estat as mlp_classify; counters
-// (may be null, 4 entries, accumulated): [0] element-stages with viscosity, [1] sensor positives
-// declared smooth (audit), [2] sensor positives (audit), [3] element-stages not declared smooth.
-// N = 7 only.
+// (may be null, 5 entries, accumulated): [0] element-stages with viscosity, [1] sensor positives
+// declared smooth (audit), [2] sensor positives (audit), [3] element-stages not declared smooth,
+// [4] element-stages classified.
+// clock (may be null): the fused schedule's device clock {dt, t, ...}; the pass then does nothing
+// (and counts nothing) for a step issued after the end time T: with clock_mode 1 (the step's first
+// evaluation, before its dt is known) when t >= T, with clock_mode 2 (before a later stage) when
+// the step's dt is 0.  N = 7 only.
 int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B,
                         const double* VdmInv, float lo, float hi, signed char* estat, int audit,
-                        unsigned long long* counters, double* visc, double* visc_plot);
+                        unsigned long long* counters, double* visc, double* visc_plot,
+                        const double* clock = nullptr, double T = 0.0, int clock_mode = 0);
 
 
 }  // namespace esdg

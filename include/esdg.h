@@ -251,7 +251,7 @@ int esdg_solver_create(esdg_ctx* ctx, const esdg_inputs* in, const esdg_run_para
 /* DGtimeloop with a step cap: at most max_steps steps or until t >= T.  Returns steps taken
  * and the final time.  dt is computed on the device; the fused and single-kernel schedules keep
  * the clock there and read it back every 64 steps (the fused schedule reads dt back every step
- * when analysis buffers or the learned indicator are on). */
+ * when analysis buffers are on). */
 int esdg_solver_run(esdg_solver* s, int max_steps, int* steps_taken, double* t_out);
 int esdg_solver_get_q(esdg_solver* s, double* host_q);        /* blocking D2H copy of q */
 int esdg_solver_set_q(esdg_solver* s, const double* host_q);  /* blocking H2D copy, t = 0 */
