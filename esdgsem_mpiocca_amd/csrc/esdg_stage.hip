@@ -29,7 +29,7 @@ struct StageCfg {
     using V = VolCfg<NGL>;
     static constexpr int EPW = V::EPW;
     static constexpr int FSZ = 12 * NGL;              // face fluxes [side][v][k] of one element
-    static constexpr int ELDS = V::ESZ + FSZ;         // doubles of LDS per element
+    static constexpr int ELDS = V::ESZ + FSZ + ((8 - (V::ESZ + FSZ) % 16) + 16) % 16;   // per element (bank spread as ESZ)
     static constexpr int LDS = EPW * ELDS;            // doubles of LDS per block
 };
 
@@ -58,7 +58,6 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
                                            unsigned long long* __restrict__ lmax) {
     using C = VolCfg<NGL>;
     const int lane = threadIdx.x;
-    const double half_g = 0.5 * g, fourth_g = 0.25 * g;
     double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], Y[NGL], X[NGL];
     double a1[NGL], a2[NGL], a3[NGL];
 #pragma unroll
@@ -86,7 +85,7 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
             P[2 * C::PLANE + r * C::RS + c] = hv[c];
             P[3 * C::PLANE + r * C::RS + c] = inv;
         }
-        line_sweep<NGL, false>(c_stage_D[NGL], fourth_g, half_g, h, hu, hv, u, v, Y, X, a1, a2, a3);
+        line_sweep<NGL, false>(c_stage_D[NGL], g, h, hu, hv, u, v, Y, X, a1, a2, a3);
     }
     __syncthreads();
 
@@ -123,9 +122,10 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
             b2[k] = P[1 * C::PLANE + k * C::RS + r];
             b3[k] = P[2 * C::PLANE + k * C::RS + r];
         }
-        line_sweep<NGL, true>(c_stage_D[NGL], fourth_g, half_g, h, hu, hv, u, v, Y, X, b1, b2, b3);
+        line_sweep<NGL, true>(c_stage_D[NGL], g, h, hu, hv, u, v, Y, X, b1, b2, b3);
 #pragma unroll
         for (int k = 0; k < NGL; ++k) {
+            unscale_sums(b1[k], b2[k], b3[k]);
             rJ[k] = Jac[xb + k * NGL];
             b1[k] = -rJ[k] * b1[k];
             b2[k] = -rJ[k] * b2[k] - g * h[k] * Bx[xb + k * NGL];

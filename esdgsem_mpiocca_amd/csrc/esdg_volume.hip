@@ -97,7 +97,6 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         if (si.list && act && si.active[e] == si.seq) act = false;
     }
     double* S = s + (act ? el : 0) * C::ESZ;
-    const double half_g = 0.5 * g, fourth_g = 0.25 * g;
     double lam = 0.0;   // this column's maximum wave speed of the updated state (si.lam)
     double bsen[NGL];   // b of row r (si.visc_next)
 
@@ -128,7 +127,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             S[2 * C::PLANE + r * C::RS + c] = hv[c];
             S[3 * C::PLANE + r * C::RS + c] = inv;
         }
-        line_sweep<NGL, false>(D, fourth_g, half_g, h, hu, hv, u, v, Y, X, a1, a2, a3);
+        line_sweep<NGL, false>(D, g, h, hu, hv, u, v, Y, X, a1, a2, a3);
     }
     __syncthreads();
 
@@ -166,12 +165,13 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             b2[k] = S[1 * C::PLANE + k * C::RS + r];
             b3[k] = S[2 * C::PLANE + k * C::RS + r];
         }
-        line_sweep<NGL, true>(D, fourth_g, half_g, h, hu, hv, u, v, Y, X, b1, b2, b3);
+        line_sweep<NGL, true>(D, g, h, hu, hv, u, v, Y, X, b1, b2, b3);
         const int64_t xb = (int64_t)e * C::NN + r;
         const int64_t ob = (int64_t)e * C::NN * ESDG_NEQ + r;
         double rJ[NGL];
 #pragma unroll
         for (int k = 0; k < NGL; ++k) {
+            unscale_sums(b1[k], b2[k], b3[k]);
             rJ[k] = Jac[xb + k * NGL];
             b1[k] = -rJ[k] * b1[k];
             b2[k] = -rJ[k] * b2[k] - g * h[k] * Bx[xb + k * NGL];
