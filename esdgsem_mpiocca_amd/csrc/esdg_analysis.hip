@@ -83,6 +83,18 @@ __global__ void __launch_bounds__(256) k_integral_partial(int Nelem, int ngl, do
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// Surface elevation h + b - h_zero at the gauge nodes (deviceclass.cpp:850: the state index of
+// global node n is (n / ngl^2) (Neq - 1) ngl^2 + n).
+__global__ void __launch_bounds__(64) k_gauge_sample(int ngauges, int ngl2, double h_zero, const int* __restrict__ ids,
+                                                     const double* __restrict__ q, const double* __restrict__ b,
+                                                     double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ngauges) return;
+    const int64_t n = ids[k];
+    const int64_t id = (n / ngl2) * (ESDG_NEQ - 1) * ngl2 + n;
+    out[k] = q[id] + b[n] - h_zero;
+}
+
 // Pass 2: one block, a fixed tree over the RED_BLOCKS partial sums; the result to out[0].
 __global__ void __launch_bounds__(256) k_integral_final(const double* __restrict__ part, double* __restrict__ out) {
     __shared__ double s[256];
@@ -140,6 +152,16 @@ int esdg_maximum_elevation(esdg_ctx* ctx, int Nelem, double t, const double* q, 
     const int64_t n = (int64_t)Nelem * ngl2;
     k_maximum_elevation<<<grid_for(n), 256, 0, ctx->stream>>>(n, ngl2, ctx->p.h_zero, q, B, MaximumElevation);
     return launch_check(ctx, "maximum_elevation");
+}
+
+int esdg_gauge_sample(esdg_ctx* ctx, int ngauges, const int* node_ids, const double* q, const double* B, double* out) {
+    ESDG_REQUIRE(ctx, ctx, "null context");
+    ESDG_REQUIRE(ctx, ngauges >= 0 && (ngauges == 0 || (node_ids && q && B && out)), "esdg_gauge_sample: bad argument");
+    if (ngauges == 0) return ESDG_OK;
+    KernelScope ks(ctx, "gauge_sample");
+    k_gauge_sample<<<(ngauges + 63) / 64, 64, 0, ctx->stream>>>(ngauges, ctx->p.ngl * ctx->p.ngl, ctx->p.h_zero,
+                                                                node_ids, q, B, out);
+    return launch_check(ctx, "gauge_sample");
 }
 
 int esdg_total_mass_dev(esdg_ctx* ctx, int Nelem, const double* q, const double* Jac, const double* GLw, double* host_mass) {

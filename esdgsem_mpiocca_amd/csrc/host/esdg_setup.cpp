@@ -978,6 +978,27 @@ int esdg_plot_mass(const char* path, int n, const double* times, const double* t
     return write_series(path, "Massplot.tec", "\"t\",\"Massdifference\"", n, times, total_mass, true);
 }
 
+int esdg_plot_time_series(const char* path, int n, const double* times, int ngauges, const char* const* names,
+                          const double* series) {
+    if (!path || n < 0 || ngauges < 0 || (ngauges > 0 && !names) || (n > 0 && (!times || (ngauges > 0 && !series))))
+        return fail("esdg_plot_time_series: bad argument");
+    for (int k = 0; k < ngauges; ++k)
+        if (!names[k]) return fail("esdg_plot_time_series: null gauge name");
+    std::ofstream f(path);
+    if (!f) return fail(std::string("esdg_plot_time_series: cannot open ") + path);
+    f << "TITLE = Timeseries.tec\n";
+    f << "VARIABLES = \"t\"";
+    for (int k = 0; k < ngauges; ++k) f << ",\"" << names[k] << "\"";
+    f << "\n";
+    for (int i = 0; i < n; ++i) {
+        f << times[i];
+        for (int k = 0; k < ngauges; ++k) f << " " << series[(size_t)k * n + i];
+        f << " \n";
+    }
+    f.close();
+    return f ? 0 : fail(std::string("esdg_plot_time_series: write failed: ") + path);
+}
+
 int esdg_plot_viscosity(const char* path, int Nelem, int ngl, const double* x, const double* y, const double* eps) {
     if (!x || !y || !eps) return fail("esdg_plot_viscosity: null argument");
     return write_zones(path, "ArtVisc.tec", "\"x\",\"y\",\"Eps\"", Nelem, ngl,

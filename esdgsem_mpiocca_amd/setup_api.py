@@ -294,3 +294,15 @@ def plot_series(path, times: np.ndarray, totals: np.ndarray, kind: str):
     fn = {"entropy": _lib().esdg_plot_entropy, "mass": _lib().esdg_plot_mass}[kind]
     keep = []
     _check(fn(os.fsencode(path), len(times), _dptr(keep, times), _dptr(keep, totals)))
+
+
+REFERENCE_GAUGES = ("Chennai", "Tuticorin", "Visakhapatnam", "Paradip", "Kochi", "Mormugao", "Okha")
+
+
+def plot_time_series(path, times: np.ndarray, series: np.ndarray, names=REFERENCE_GAUGES):
+    """PlotTimeSeries: gauge records; series is [ngauges][len(times)], one column per name."""
+    series = np.ascontiguousarray(series, dtype=np.float64).reshape(len(names), len(times))
+    keep = []
+    cnames = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    _check(_lib().esdg_plot_time_series(os.fsencode(path), len(times), _dptr(keep, times), len(names), cnames,
+                                        _dptr(keep, series)))

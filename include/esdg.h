@@ -195,6 +195,12 @@ int esdg_arrival_times(esdg_ctx* ctx, int Nelem, double t, const double* q, cons
  * max(MaximumElevation, h + b - h_zero) (fmax; initial value 0, :337-342).  t is unused. */
 int esdg_maximum_elevation(esdg_ctx* ctx, int Nelem, double t, const double* q, const double* B,
                            double* MaximumElevation);
+/* The gauge samples of the time series DGtimeloop records at its plot times (createTimeSeries,
+ * src/deviceclass.cpp:841-858, 1196-1222): node_ids[k] is the global node index of gauge k
+ * (element node_ids[k] / ngl^2, local node node_ids[k] mod ngl^2); out[k] = h + b - h_zero there.
+ * node_ids and out are device arrays; esdg_plot_time_series (esdg_output.h) writes the records. */
+int esdg_gauge_sample(esdg_ctx* ctx, int ngauges, const int* node_ids, const double* q, const double* B,
+                      double* out);
 /* basis::calcTotalMass (src/basis.cpp:1088-1111) on device arrays: sum of h J w_i w_j (Jac = 1/J);
  * basis::calcTotalEntropy (:1004-1055): sum of (h|u|^2/2 + g h^2/2 + g h b) J w_i w_j, u = v = 0
  * where h <= 1e-12.  Device reductions in a fixed order; the result (host) differs from the

@@ -34,8 +34,14 @@ int esdg_plot_viscosity(const char* path, int Nelem, int ngl, const double* x, c
 int esdg_plot_entropy(const char* path, int n, const double* times, const double* total_entropy);
 /* PlotMass (src/Plots.cpp:438-457): relative mass change (m_i - m_0) / m_0 against time. */
 int esdg_plot_mass(const char* path, int n, const double* times, const double* total_mass);
-/* Not provided: PlotTimeSeries (the gauge records of one tsunami scenario's seven ports) and
- * PlotFriction / PlotViscosity / PlotSolutionWithExact, which write fields outside this path. */
+/* PlotTimeSeries (src/Plots.cpp:474-517): gauge records, one line per time with the surface
+ * elevation at each gauge.  series is [ngauges][n]; names label the columns.  With the reference's
+ * seven ports ("Chennai", "Tuticorin", "Visakhapatnam", "Paradip", "Kochi", "Mormugao", "Okha") the
+ * file is the reference's byte for byte; the samples come from esdg_gauge_sample. */
+int esdg_plot_time_series(const char* path, int n, const double* times, int ngauges, const char* const* names,
+                          const double* series);
+/* Not provided: PlotFriction / PlotViscosity / PlotSolutionWithExact, which write fields outside
+ * this path. */
 
 #ifdef __cplusplus
 }

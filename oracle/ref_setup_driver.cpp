@@ -143,7 +143,7 @@ int ref_total_entropy_mass(int N, int Nelem, double g, const double* q, const do
 
 // The reference's plot writers (src/Plots.cpp), run in workdir: they write under movie/.
 // which: 0 PlotSolution, 1 PlotArrivalTimings, 2 PlotMaximumElevation, 3 PlotViscoseParameter,
-// 4 PlotEntropy, 5 PlotMass (count records: times in x, totals in values)
+// 4 PlotEntropy, 5 PlotMass (count records: times in x, totals in values), 6 PlotTimeSeries
 // (values: q for 0, the per-node buffer for 1 and 2, the element viscosity for 3).
 int ref_plot(const char* workdir, int which, int Nelem, int ngl, int plot_var, const double* x, const double* y,
              const double* values, const double* b, int count, double h0) {
@@ -158,6 +158,10 @@ int ref_plot(const char* workdir, int which, int Nelem, int ngl, int plot_var, c
         case 3: PlotViscoseParameter(Nelem, ngl, x, y, values, count); break;
         case 4: PlotEntropy(count, x, values); break;     // times in x, totals in values
         case 5: PlotMass(count, x, values); break;
+        case 6:   // times in x, the seven gauge series [7][count] in values
+            PlotTimeSeries(count, x, values, values + count, values + 2 * count, values + 3 * count,
+                           values + 4 * count, values + 5 * count, values + 6 * count);
+            break;
         default: break;
     }
     return chdir(cwd) == 0 ? 0 : -3;

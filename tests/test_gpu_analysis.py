@@ -109,3 +109,23 @@ def test_single_kernel_stage_rejects_analysis(dev):
             dev.Solver(ctx, case, rp, fused=2)
     finally:
         ctx.close()
+
+
+def test_gauge_samples(dev):
+    """esdg_gauge_sample: h + b - h_zero at global node ids (the index rule of
+    src/deviceclass.cpp:850), exact; the last node of the mesh included."""
+    case = Case.ocean_uplift(N=5, nel=8, amp=3.0)
+    rp = RunParams(h_zero=4000.0)
+    E, n2 = case.Nelem, case.ngl ** 2
+    rng = np.random.default_rng(2)
+    ids = np.concatenate([rng.integers(0, E * n2, size=40), [0, E * n2 - 1]]).astype(np.int32)
+    Q = case["q"].reshape(E, 3, n2)
+    expect = Q[ids // n2, 0, ids % n2] + case["b"][ids] - 4000.0
+    ctx = dev.Context(dev.make_params(case, rp))
+    try:
+        out = ctx.empty(len(ids))
+        ctx.gauge_sample(len(ids), ctx.array(ids), ctx.array(case["q"]), ctx.array(case["b"]), out)
+        assert np.array_equal(out.download(), expect)
+        ctx.gauge_sample(0, None, None, None, None)
+    finally:
+        ctx.close()

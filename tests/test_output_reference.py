@@ -102,3 +102,23 @@ def test_series_files_equal_the_reference_writers(tmp_path):
     assert ref.ref_plot(wd, 5, 0, 0, 0, P(t), None, P(mass), None, len(t), 0.0) == 0
     assert (tmp_path / "movie" / "Entropyplot.tec").read_bytes() == (tmp_path / "ours_ent.tec").read_bytes()
     assert (tmp_path / "movie" / "Massplot.tec").read_bytes() == (tmp_path / "ours_mass.tec").read_bytes()
+
+
+def test_time_series_file_equals_the_reference_writer(tmp_path):
+    """PlotTimeSeries (src/Plots.cpp:474-517) with the reference's seven gauges, byte for byte."""
+    if not os.path.exists(REF):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(8)
+    t = np.cumsum(rng.uniform(1.0, 30.0, size=23))
+    series = rng.normal(size=(7, 23)) * np.array([1e-3, 0.5, 2.0, 1e-9, 3.0, 0.25, 7.0])[:, None]
+    series[2, 5] = 0.0
+    S.plot_time_series(tmp_path / "ours_ts.tec", t, series)
+    ref = C.CDLL(REF)
+    ref.ref_plot.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4 + [C.c_int, C.c_double]
+    P = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.c_void_p)
+    assert ref.ref_plot(os.fsencode(str(tmp_path)), 6, 0, 0, 0, P(t), None, P(series), None, len(t), 0.0) == 0
+    assert (tmp_path / "movie" / "Timeseries.tec").read_bytes() == (tmp_path / "ours_ts.tec").read_bytes()
+    # other gauge sets: one column per name
+    S.plot_time_series(tmp_path / "two.tec", t[:3], series[:2, :3], names=("A", "B"))
+    lines = (tmp_path / "two.tec").read_text().splitlines()
+    assert lines[1] == 'VARIABLES = "t","A","B"' and len(lines) == 5 and len(lines[2].split()) == 3
