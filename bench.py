@@ -76,7 +76,9 @@ KERNELS = ["volume_kernel", "collect_edge_data", "calc_num_fluxes", "surface_ker
 
 
 def c1_params(RunParams):
-    return RunParams(g=9.81, CFL=0.5, DFL=0.1, num_flux=3, ES=1, flux_differencing=1,
+    # DFL 0.05: with 0.1 the viscous step limit, which governs dt at 512^2 wherever the sensor fires,
+    # lets the run diverge within 100 steps (the oracle too); 0.05 keeps it bounded over 500 steps
+    return RunParams(g=9.81, CFL=0.5, DFL=0.05, num_flux=3, ES=1, flux_differencing=1,
                      artificial_viscosity=1, positivity=0, pos_pres_tol=1e-8, eps0=0.005,
                      sigma_min=-4.5, sigma_max=-2.5, rk_order=4, rk_ssp=0)
 
@@ -197,42 +199,33 @@ def pos_pres_bytes(nelem, ngl, sensor=True, stages=5):
     return int(round(8 * nelem * per_elem))
 
 
-def c4_bench(dev, args, steps):
-    """Wet/dry flood on the curved mesh family, N=7, positivity limiter, artificial viscosity: the
-    modal sensor everywhere against the learned indicator (modal-sensor fallback).  The case is
-    built on the device (esdg_device_case_create) and handed to each solver device to device."""
+def indicator_comparison(dev, args, spec, rp_kw, steps, modes):
+    """Plain modal sensor against the learned indicator (modal-sensor fallback) on one case built on
+    the device (esdg_device_case_create, handed to each solver device to device).  modes: "plain"
+    (steps steps, timed), "learned" (timed) and "audit" (false negatives counted) to the plain
+    run's end time, and optionally "control": the plain method with eps0 changed by 1e-12
+    (relative), to the same time -- how far roundoff-sized differences travel in the flow, the scale
+    to read the learned change against."""
     import numpy as np
     from esdgsem_mpiocca_amd import indicator
     params, meta = indicator.load_weights()
     if params is None:
         raise SystemExit("no trained indicator weights (esdgsem_mpiocca_amd/indicator_weights.json)")
-    from esdgsem_mpiocca_amd import Case
-    spec = Case.flood(N=7, nel=args.c4_nel, seed=0, build=False)
-    # CFL 0.1: the dt formula's CFL/ngl stays below the smallest GLL weight fraction 1/(N(N+1)),
-    # the step size under which the limiter keeps the element means positive.  Dry tolerance 0.1:
-    # without the partial-dry treatment, thinner layers carry spurious velocities at the fronts
-    # (tolerances 1e-2 and 5e-2 reach NaN or |u| of hundreds of m/s within 1000 steps at 2048^2).
-    rp = indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1)
+    rp = indicator.av_run_params(**rp_kw)
     t0 = time.perf_counter()
     cctx = dev.Context(dev.make_params(spec, rp), device=0)
     dcase = dev.DeviceCase(cctx, spec)
     cctx.finish()
     t_case = time.perf_counter() - t0
-    print(f"# c4: {dcase.Nelem} elements, case built on the device in {t_case:.2f} s", file=sys.stderr, flush=True)
+    print(f"# {dcase.Nelem} elements, case built on the device in {t_case:.2f} s", file=sys.stderr, flush=True)
     out = {"case_build_s": round(t_case, 3)}
     qs = {}
     dofs = 3 * dcase.ngl ** 2 * dcase.Nelem
     t_end = None
     try:
-        # plain: `steps` steps; learned and audit: to the plain run's end time, so that the states
-        # are compared at the same simulated time (dt depends on the viscosity through the DFL
-        # limit).  control: the plain method with eps0 changed by 1e-12 (relative), to the same
-        # time: how far roundoff-sized differences travel in this flow, the scale to read the
-        # learned change against.
-        for mode in ("plain", "learned", "audit", "control"):
-            eps0 = 0.005 * (1.0 + 1e-12) if mode == "control" else 0.005
-            rpm = rp if mode == "plain" else indicator.av_run_params(positivity=1, pos_pres_tol=0.1, CFL=0.1,
-                                                                     T=t_end, eps0=eps0)
+        for mode in modes:
+            eps0 = rp.eps0 * (1.0 + 1e-12) if mode == "control" else rp.eps0
+            rpm = rp if mode == "plain" else indicator.av_run_params(**{**rp_kw, "T": t_end, "eps0": eps0})
             ctx = dev.Context(dev.make_params(spec, rpm), device=0)
             try:
                 solver = dev.Solver(ctx, dcase, rpm, fused=1)
@@ -264,12 +257,12 @@ def c4_bench(dev, args, steps):
                                 per_kernel[kn] = {"avg_us": round(1e3 * ms / launches, 2),
                                                   "ms_per_step": round(ms / max(n, 1), 4)}
                         ctx.profile(False)
-                    print(f"# c4 {mode}: {done} steps, t = {t_sim:.6g}, {1e3 * wall / max(done, 1):.1f} ms/step",
+                    print(f"# {mode}: {done} steps, t = {t_sim:.6g}, {1e3 * wall / max(done, 1):.1f} ms/step",
                           file=sys.stderr, flush=True)
                     if n != k:
                         if mode != "plain" and t_sim >= t_end:
                             break
-                        raise SystemExit(f"c4 {mode}: solver stopped after {done} steps")
+                        raise SystemExit(f"{mode}: solver stopped after {done} steps")
                 if mode == "plain":
                     t_end = t_sim
                 counts = solver.indicator_counts() if net else None
@@ -293,31 +286,51 @@ def c4_bench(dev, args, steps):
     finally:
         dcase.close()
         cctx.close()
+
+    def change(a, b):
+        A, B = a.reshape(dcase.Nelem, 3, -1), b.reshape(dcase.Nelem, 3, -1)
+        return [float(np.linalg.norm(A[:, v] - B[:, v]) / max(np.linalg.norm(B[:, v]), 1e-300)) for v in range(3)]
+
+    out["l2_change_vs_plain"] = change(qs["learned"], qs["plain"])
+    if "control" in qs:
+        out["control_l2_change_eps0_1e-12"] = change(qs["control"], qs["plain"])
     A = qs["learned"].reshape(dcase.Nelem, 3, -1)
     B = qs["plain"].reshape(dcase.Nelem, 3, -1)
-    Cq = qs["control"].reshape(dcase.Nelem, 3, -1)
-    out["l2_change_vs_plain"] = [float(np.linalg.norm(A[:, v] - B[:, v]) / max(np.linalg.norm(B[:, v]), 1e-300))
-                                 for v in range(3)]
-    out["control_l2_change_eps0_1e-12"] = [float(np.linalg.norm(Cq[:, v] - B[:, v]) /
-                                                 max(np.linalg.norm(B[:, v]), 1e-300)) for v in range(3)]
     out["finite"] = bool(np.isfinite(A).all() and np.isfinite(B).all())
     out["min_h"] = float(min(A[:, 0].min(), B[:, 0].min()))
     out["speedup_learned_vs_plain"] = out["learned"]["dof_stage_updates_per_s"] / out["plain"]["dof_stage_updates_per_s"]
     out["indicator"] = {"lo": args.ind_lo, "hi": args.ind_hi, "validation": meta.get("validation")}
+    out["_dcase"] = (dcase.Nelem, dcase.ngl, dcase.N, dofs)
+    return out
+
+
+def c4_bench(dev, args, steps):
+    """Wet/dry flood on the curved mesh family (2048^2, N=7), positivity limiter, artificial
+    viscosity: the modal sensor everywhere against the learned indicator, with the rooflines of the
+    plain run's two dominant kernels."""
+    from esdgsem_mpiocca_amd import Case
+    spec = Case.flood(N=7, nel=args.c4_nel, seed=0, build=False)
+    # CFL 0.1: the dt formula's CFL/ngl stays below the smallest GLL weight fraction 1/(N(N+1)),
+    # the step size under which the limiter keeps the element means positive.  Dry tolerance 0.1:
+    # without the partial-dry treatment, thinner layers carry spurious velocities at the fronts
+    # (tolerances 1e-2 and 5e-2 reach NaN or |u| of hundreds of m/s within 1000 steps at 2048^2).
+    out = indicator_comparison(dev, args, spec, {"positivity": 1, "pos_pres_tol": 0.1, "CFL": 0.1}, steps,
+                               ("plain", "learned", "audit", "control"))
+    nelem, ngl, N, dofs = out.pop("_dcase")
     # rooflines of the plain run's two dominant kernels (HIP events around every launch of the
     # profiled chunk): the fused volume pass (no side traces or sensor: the limiter pass writes
     # them) and the limiter pass
     pk = out["plain"]["kernels"]
     roof = {}
-    for name, alg in (("fused_volume", fused_volume_bytes(dcase.Nelem, dcase.ngl, 1, traces=False)),
-                      ("pos_pres", pos_pres_bytes(dcase.Nelem, dcase.ngl))):
+    for name, alg in (("fused_volume", fused_volume_bytes(nelem, ngl, 1, traces=False)),
+                      ("pos_pres", pos_pres_bytes(nelem, ngl))):
         if name in pk:
             gbs = alg / (pk[name]["avg_us"] * 1e-6) / 1e9
             roof[name] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": alg,
                           "avg_launch_us": pk[name]["avg_us"]}
     if "fused_volume" in roof:
-        tfl = volume_flops(dcase.Nelem, dcase.ngl) / (pk["fused_volume"]["avg_us"] * 1e-6) / 1e12
+        tfl = volume_flops(nelem, ngl) / (pk["fused_volume"]["avg_us"] * 1e-6) / 1e12
         roof["fused_volume"]["fp64_tflops_algorithmic"] = round(tfl, 2)
         roof["fused_volume"]["fp64_frac_of_peak"] = round(tfl / FP64_PEAK_TFLOPS, 4)
     out["roofline"] = roof
@@ -327,7 +340,21 @@ def c4_bench(dev, args, steps):
     out["config"] = {"workload": f"curved flood {args.c4_nel}x{args.c4_nel}, N=7, 16-mode Fourier bathymetry "
                                  "(amplitude 0.3), reservoir over dry land, positivity limiter, AV, rotated ES, "
                                  "LSRK4(5); learned indicator with the modal sensor as fallback beside it",
-                     "Nelem": dcase.Nelem, "N": dcase.N, "dofs": dofs}
+                     "Nelem": nelem, "N": N, "dofs": dofs}
+    return out
+
+
+def c2_bench(dev, args, steps):
+    """The learned indicator evaluated on the 512^2 curved dam break at N=7 (the headline mesh)
+    against the plain-sensor FP64 run: false-negative rate over the run and the L2 change of the
+    state at equal simulated time (a flow without wet/dry fronts, where differences do not blow
+    up), with its throughput beside the plain method's."""
+    from esdgsem_mpiocca_amd import Case
+    spec = Case.dam_break(N=7, nel=args.nel, noise=1e-3, seed=0, build=False)
+    out = indicator_comparison(dev, args, spec, {}, steps, ("plain", "learned", "audit"))
+    nelem, ngl, N, dofs = out.pop("_dcase")
+    out["config"] = {"workload": f"curved dam break {args.nel}x{args.nel}, N=7, modal sensor + AV against the "
+                                 "learned indicator (modal sensor as fallback)", "Nelem": nelem, "N": N, "steps": steps}
     return out
 
 
@@ -361,6 +388,9 @@ def main():
                     help="steps of the 2048^2 flood's plain run (default: 1000 with --workload c4, 100 beside the "
                          "headline line)")
     ap.add_argument("--c4", type=int, default=1, help="also run the 2048^2 flood beside the headline line")
+    ap.add_argument("--c2", type=int, default=1,
+                    help="also evaluate the learned indicator on the headline dam break (plain vs learned)")
+    ap.add_argument("--c2-steps", type=int, default=200)
 
     ap.add_argument("--ind-lo", type=float, default=0.05)
     ap.add_argument("--ind-hi", type=float, default=0.5)
@@ -496,6 +526,11 @@ def main():
             out["ensemble"] = ensemble_bench(dev, args)
         except Exception as e:  # reported beside the headline figure; must not hide it
             out["ensemble"] = {"error": str(e)}
+    if args.c2:
+        try:
+            out["c2"] = c2_bench(dev, args, args.c2_steps)
+        except (Exception, SystemExit) as e:  # reported beside the headline figure; must not hide it
+            out["c2"] = {"error": str(e)}
     if args.c4:
         try:
             out["c4"] = c4_bench(dev, args, args.c4_steps or 100)
