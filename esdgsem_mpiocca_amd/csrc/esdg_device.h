@@ -26,6 +26,15 @@ struct ElemBlock {
     static constexpr int THREADS = EPB * NODES;
 };
 
+// One thread per element row (the sensor and limiter passes): 256 / NGL elements per block.
+template <int NGL>
+struct RowBlock {
+    static constexpr int EPB = 256 / NGL;            // elements per block
+    static constexpr int THREADS = EPB * NGL;
+    static constexpr int RS = NGL + 1;               // padded LDS row stride of an element's modes
+    static constexpr int ES = NGL * RS + 1;          // padded LDS stride between elements
+};
+
 __device__ __forceinline__ void wall_mirror(double nx, double ny, double vx, double vy, double& ox, double& oy) {
     const double v_n = nx * vx + ny * vy;
     const double v_t = -ny * vx + nx * vy;
@@ -370,6 +379,66 @@ __device__ __forceinline__ double sensor_ramp(double z1, double z2, double n1, d
 template <int NGL>
 __device__ __forceinline__ double sensor_from_modes(const double* m, const SensorConsts& c) {
     return sensor_ramp(sensor_sum<NGL>(m, 0), sensor_sum<NGL>(m, 1), sensor_sum<NGL>(m, 2), sensor_sum<NGL>(m, 3), c);
+}
+
+// ShockCapturing.okl:63-217 over the RowBlock<NGL> elements e0 .. e0 + EPB - 1 of a state, one
+// thread per element row (eta = r), called by every thread of the block.  The modal coefficients
+// m[a][c] = sum_l T[a][l] V[c][l] with T[a][c] = sum_l V[a][l] u(xi = l, eta = c), u = h + b,
+// each sum over l in ascending order as in the okl: thread r forms column r of T from its own row
+// of u, then row r of m from row r of T; four threads form the energy sums in the okl's order.
+// sV holds V (written by the caller before the call; the call's first barrier publishes it), sT
+// EPB * ES doubles, sSum EPB x 4.  Returns the element's viscosity on its row-0 thread.
+template <int NGL>
+__device__ __forceinline__ double sensor_rows(int Nelem, int e0, const SensorConsts& sc, const double* __restrict__ q,
+                                              const double* __restrict__ b, const double* sV, double* sT,
+                                              double (*sSum)[4]) {
+    using RB = RowBlock<NGL>;
+    constexpr int NN = NGL * NGL;
+    const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
+    const int e = e0 + el;
+    const bool act = (el < RB::EPB) && (e < Nelem);
+    double u[NGL];
+    if (act) {
+        const int64_t xb = (int64_t)e * NN + r * NGL;
+        const int64_t qb = (int64_t)e * NN * 3 + r * NGL;
+#pragma unroll
+        for (int i = 0; i < NGL; ++i) u[i] = b[xb + i] + q[qb + i];
+    }
+    __syncthreads();
+    double* T = sT + (act ? el : 0) * RB::ES;
+    if (act) {
+        // column c = r of T
+#pragma unroll
+        for (int a = 0; a < NGL; ++a) {
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < NGL; ++l) s += sV[a * NGL + l] * u[l];
+            T[a * RB::RS + r] = s;
+        }
+    }
+    __syncthreads();
+    if (act) {
+        // row a = r of m, in place of row r of T (only this thread reads and writes that row)
+        double t[NGL];
+#pragma unroll
+        for (int l = 0; l < NGL; ++l) t[l] = T[r * RB::RS + l];
+#pragma unroll
+        for (int c = 0; c < NGL; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < NGL; ++l) s += t[l] * sV[c * NGL + l];
+            T[r * RB::RS + c] = s;
+        }
+    }
+    __syncthreads();
+    if (act && r < 4) sSum[el][r] = sensor_sum<NGL, RB::RS>(T, r);
+    __syncthreads();
+    double v = 0.0;
+    if (act && r == 0) {
+        const double* S4 = sSum[el];
+        v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
+    }
+    return v;
 }
 
 // ---------------------------------------------------------------------------------------------

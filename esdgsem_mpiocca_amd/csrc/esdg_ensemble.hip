@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "esdg_ensemble.h"
+#include "esdg_mlp.h"
 #include "esdg_internal.h"
 #include "esdg_setup.h"
 #include "esdg_stage.h"
@@ -23,6 +24,7 @@ struct esdg_ensemble {
     std::vector<void*> owned;
     double* clocks = nullptr;             // 8 doubles per scenario
     unsigned long long* lmax = nullptr;   // one per scenario
+    unsigned long long* emax = nullptr;   // one per scenario (artificial viscosity)
     StageBatch batch;
     int src = 0;
     bool need_lambda = true;
@@ -75,7 +77,7 @@ int check_scenario(esdg_ctx* ctx, int i, const esdg_params& p, const esdg_inputs
         return set_error(ctx, ESDG_ERR_ARG, at + "num_flux must be 0, 1, 3, 4 or 5");
     if (p.bc_kind < ESDG_BC_WALLS || p.bc_kind > ESDG_BC_OCEAN)
         return set_error(ctx, ESDG_ERR_ARG, at + "bc_kind must be walls, inner/outer or ocean");
-    if (r.artificial_viscosity) return set_error(ctx, ESDG_ERR_ARG, at + "artificial viscosity is not supported");
+    if (r.artificial_viscosity && !(r.DFL > 0.0)) return set_error(ctx, ESDG_ERR_ARG, at + "DFL must be > 0");
     if (r.arrival_times || r.maximum_elevation)
         return set_error(ctx, ESDG_ERR_ARG, at + "the analysis buffers are not kept for ensembles");
     if (!r.flux_differencing) return set_error(ctx, ESDG_ERR_ARG, at + "needs flux differencing");
@@ -117,6 +119,8 @@ int esdg_ensemble_create(esdg_ctx* ctx, int nscn, const esdg_params* params, con
         rc = set_error(ctx, ESDG_ERR_ARG, "esdg_ensemble_create: RK coefficients");
     if (!rc) E->clocks = (double*)E->alloc((size_t)nscn * 8 * sizeof(double), &rc);
     if (!rc) E->lmax = (unsigned long long*)E->alloc((size_t)nscn * sizeof(unsigned long long), &rc);
+    if (!rc) E->emax = (unsigned long long*)E->alloc((size_t)nscn * sizeof(unsigned long long), &rc);
+    if (!rc) rc = esdg_fill(ctx, reinterpret_cast<double*>(E->emax), 0.0, nscn);
     const size_t dd = sizeof(double), di = sizeof(int);
     for (int i = 0; i < nscn && !rc; ++i) {
         const esdg_params& p = params[i];
@@ -172,6 +176,26 @@ int esdg_ensemble_create(esdg_ctx* ctx, int nscn, const esdg_params* params, con
         }
         S.clock = E->clocks + 8 * (size_t)i;
         S.lmax = E->lmax + i;
+        S.emax = E->emax + i;
+        if (r.artificial_viscosity) {
+            // the element viscosity and the BR1 / viscous work buffers, private to the scenario;
+            // the dt limit of src/deviceclass.cpp:922 with the solver's constant
+            S.av = 1;
+            S.dvc = r.DFL / (std::pow(ngl, 2)) * std::pow(r.min_ele_size, 2);
+            S.sc = SensorConsts{p.ZeroTOL, p.eps0, p.sigmaMin, p.sigmaMax};
+            S.B = dB;
+            S.VdmInv = (const double*)E->upload(in.VdmInv, ngl2 * dd, &rc);
+            if (!rc) S.Dhat = (const double*)E->upload(in.Dhat, ngl2 * dd, &rc);
+            if (!rc) S.Dstrong = (const double*)E->upload(in.Dstrong, ngl2 * dd, &rc);
+            if (!rc) S.visc = (double*)E->alloc(in.Nelem * dd, &rc);
+            if (!rc) S.gtr = (double*)E->alloc((size_t)in.Nelem * 16 * ngl * dd, &rc);
+            if (!rc) S.Qv = (double*)E->alloc(nodes * 2 * dd, &rc);
+            if (!rc) S.SPV = (double*)E->alloc(fn * 2 * dd, &rc);
+            if (!rc) S.active = (int*)E->alloc(((size_t)in.Nelem + 1) / 2 * dd, &rc);
+            if (!rc) rc = esdg_fill(ctx, S.visc, 0.0, in.Nelem);
+            if (!rc) rc = esdg_fill(ctx, reinterpret_cast<double*>(S.active), 0.0, ((size_t)in.Nelem + 1) / 2);
+            if (rc) break;
+        }
         E->scn.push_back(S);
         E->qcount.push_back(nodes * 3);
     }
@@ -280,6 +304,9 @@ int esdg_ensemble_autotune(esdg_ensemble* E, int trial_steps, esdg_ensemble_sche
         return r;
     };
     rc = copy_all(true);
+    // the learned indicator's counts are restored after the trials as well
+    unsigned long long icount[5] = {0, 0, 0, 0, 0};
+    if (!rc && E->batch.d_icount) rc = esdg_memcpy(ctx, icount, E->batch.d_icount, sizeof icount, ESDG_D2H);
     const int src0 = E->src;
     const StageSchedule keep = E->batch.sched;
     StageSchedule best_s = keep;
@@ -313,10 +340,41 @@ int esdg_ensemble_autotune(esdg_ensemble* E, int trial_steps, esdg_ensemble_sche
         if (!rc) rc = copy_all(false);
     }
     esdg_free(ctx, snap);
+    if (!rc && E->batch.d_icount) rc = esdg_memcpy(ctx, E->batch.d_icount, icount, sizeof icount, ESDG_H2D);
     if (!rc) rc = E->batch.set_schedule(ctx, best_s);
     if (rc) return rc;
     if (best) esdg_ensemble_get_schedule(E, best);
     if (best_ms) *best_ms = best_t;
+    return ESDG_OK;
+}
+
+int esdg_ensemble_set_indicator(esdg_ensemble* E, esdg_mlp* m, float lo, float hi) {
+    if (!E) return set_error(nullptr, ESDG_ERR_ARG, "esdg_ensemble_set_indicator: null ensemble");
+    return E->batch.set_indicator(E->ctx, m, lo, hi);
+}
+
+int esdg_ensemble_indicator_audit(esdg_ensemble* E, int audit) {
+    if (!E) return set_error(nullptr, ESDG_ERR_ARG, "esdg_ensemble_indicator_audit: null ensemble");
+    E->batch.ind_audit = audit != 0;
+    E->batch.drop_graphs();   // the audit flag is a kernel argument of the captured steps
+    return ESDG_OK;
+}
+
+int esdg_ensemble_indicator_counts(esdg_ensemble* E, int64_t* viscous, int64_t* false_negatives,
+                                   int64_t* sensor_positives, int64_t* not_smooth, int64_t* element_stages) {
+    if (!E) return set_error(nullptr, ESDG_ERR_ARG, "esdg_ensemble_indicator_counts: null ensemble");
+    unsigned long long c[5] = {0, 0, 0, 0, 0};
+    if (E->batch.d_icount) {
+        int rc = esdg_memcpy(E->ctx, c, E->batch.d_icount, sizeof c, ESDG_D2H);
+        if (!rc) rc = esdg_fill(E->ctx, reinterpret_cast<double*>(E->batch.d_icount), 0.0, 5);
+        if (!rc) rc = esdg_finish(E->ctx);
+        if (rc) return rc;
+    }
+    if (viscous) *viscous = (int64_t)c[0];
+    if (false_negatives) *false_negatives = (int64_t)c[1];
+    if (sensor_positives) *sensor_positives = (int64_t)c[2];
+    if (not_smooth) *not_smooth = (int64_t)c[3];
+    if (element_stages) *element_stages = (int64_t)c[4];
     return ESDG_OK;
 }
 

@@ -25,6 +25,7 @@
 #include <climits>
 #include <type_traits>
 
+#include "esdg_av_device.h"
 #include "esdg_device.h"
 #include "esdg_fused.h"
 
@@ -37,27 +38,6 @@ int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const doub
                     double* tr, double* next_visc, double* next_visc_plot, const double* B, const double* VdmInv,
                     double* next_lambda, const double* dt_dev);
 int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr);
-
-// calcNumFluxesGradient.okl:16-135 at face node i: the BR1 face fluxes of u and v (x and y parts)
-// from qL and qR as the gather stores them, both read unreversed at node i, the exterior state
-// taking the place of a missing side.
-struct Br1Flux {
-    double x0, x1, y0, y1;
-};
-
-__device__ __forceinline__ Br1Flux br1_face_flux(const double* __restrict__ tr, int ngl, const int* E, int i, double nx,
-                                                 double ny, double sc, double bl, double br, const GatherConsts& gc,
-                                                 double tol, double geomFace) {
-    const FaceState L = gathered_tr(tr, ngl, E, true, i, nx, ny, gc, bl, br);
-    const FaceState Ri = gathered_tr(tr, ngl, E, false, i, nx, ny, gc, bl, br);
-    const FaceState LL = (E[5] == -1) ? Ri : L;
-    const FaceState RR = (E[6] == -1) ? L : Ri;
-    const double iL = (LL.h > tol) ? 1.0 / LL.h : 0.0;
-    const double iR = (RR.h > tol) ? 1.0 / RR.h : 0.0;
-    const double uL = LL.hu * iL, uR = RR.hu * iR, vL = LL.hv * iL, vR = RR.hv * iR;
-    return Br1Flux{nx * sc * geomFace * 0.5 * (uL + uR), nx * sc * geomFace * 0.5 * (vL + vR),
-                   ny * sc * geomFace * 0.5 * (uL + uR), ny * sc * geomFace * 0.5 * (vL + vR)};
-}
 
 // ---------------------------------------------------------------------------------------------
 // fused_face: CollectEdgeData (SolidWalls.okl:16-207) + calcNumFluxes (calcESFlux.okl /
@@ -134,19 +114,10 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
 // elements, EPB per workgroup iteration, in list order (each element's results depend on nothing
 // but its own inputs, so the order does not matter).
 template <int NGL>
-__global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const double* __restrict__ q,
+__global__ void __launch_bounds__(256) k_fused_av(int Nelem, AvMesh M, const double* __restrict__ tr,
                                                   const double* __restrict__ Dh, const double* __restrict__ Ds,
-                                                  const double* __restrict__ Jac, const double* __restrict__ Yxi,
-                                                  const double* __restrict__ Yeta, const double* __restrict__ Xxi,
-                                                  const double* __restrict__ Xeta, const int* __restrict__ E2E,
-                                                  const int* __restrict__ EMS, const int* __restrict__ EOR,
-                                                  const int* __restrict__ ED, const double* __restrict__ tr,
-                                                  const double* __restrict__ nxa, const double* __restrict__ nya,
-                                                  const double* __restrict__ scal, const double* __restrict__ BL,
-                                                  const double* __restrict__ BR, GatherConsts gc, double geomFace,
-                                                  double* __restrict__ gtr, const double* __restrict__ visc,
-                                                  const int* __restrict__ alist, const int* __restrict__ acount,
-                                                  double* __restrict__ Qv) {
+                                                  const double* __restrict__ visc, const int* __restrict__ alist,
+                                                  const int* __restrict__ acount) {
     using EB = ElemBlock<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sDh[NN], sDs[NN];
@@ -156,153 +127,20 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, double tol, const d
         sDh[threadIdx.x] = Dh[threadIdx.x];
         sDs[threadIdx.x] = Ds[threadIdx.x];
     }
-    const int el = threadIdx.x / NN, n = threadIdx.x % NN;
+    const int el = threadIdx.x / NN;
     const int cnt = min(*acount, Nelem);
     __syncthreads();
-    {
-        for (int grp = blockIdx.x; grp * EB::EPB < cnt; grp += gridDim.x) {
-            const int idx = grp * EB::EPB + el;
-            const bool act = idx < cnt;
-            const int e = act ? alist[idx] : 0;
-            if (act && n == 0) sEps[el] = visc[e];
-            const int a = n % NGL, c = n / NGL;           // node n = (xi = a, eta = c)
-            const int64_t xid = (int64_t)e * NN + n;
-            double h = 0.0, hu = 0.0, hv = 0.0;
-            if (act) {
-                const int64_t id = (int64_t)e * NN * ESDG_NEQ + n;
-                h = q[id]; hu = q[id + NN]; hv = q[id + 2 * NN];
-                const double Qinv = (h > tol) ? 1.0 / h : 0.0;
-                sB[threadIdx.x] = hu * Qinv;                // u at node n
-                sC[threadIdx.x] = hv * Qinv;                // v at node n
-            }
-            __syncthreads();
-            // ---- BR1 gradient (weak form volume part) ----
-            const double* u = sB + el * NN;
-            const double* v = sC + el * NN;
-            double gx0 = 0.0, gx1 = 0.0, gy0 = 0.0, gy1 = 0.0;
-            double ye = 0.0, yx = 0.0, xe = 0.0, xx = 0.0, rJ = 0.0;
-            if (act) {
-                double u_xi = 0.0, u_eta = 0.0, v_xi = 0.0, v_eta = 0.0;
-#pragma unroll
-                for (int l = 0; l < NGL; ++l) {
-                    u_xi += sDh[a * NGL + l] * u[c * NGL + l];
-                    u_eta += sDh[c * NGL + l] * u[l * NGL + a];
-                    v_xi += sDh[a * NGL + l] * v[c * NGL + l];
-                    v_eta += sDh[c * NGL + l] * v[l * NGL + a];
-                }
-                ye = Yeta[xid]; yx = Yxi[xid]; xe = Xeta[xid]; xx = Xxi[xid]; rJ = Jac[xid];
-                const double u_x = ye * u_xi - yx * u_eta;
-                const double v_x = ye * v_xi - yx * v_eta;
-                const double u_y = -xe * u_xi + xx * u_eta;
-                const double v_y = -xe * v_xi + xx * v_eta;
-                gx0 = rJ * u_x; gx1 = rJ * v_x; gy0 = rJ * u_y; gy1 = rJ * v_y;
-                // surface lift of the BR1 face fluxes: sides 0/2 first, then 1/3 (SurfaceKernelGradient.okl)
-#pragma unroll
-                for (int ph = 0; ph < 2; ++ph) {
-                    int side = -1, t = 0;
-                    if (ph == 0) {
-                        if (c == 0) { side = 0; t = a; }
-                        else if (c == NGL - 1) { side = 2; t = a; }
-                    } else {
-                        if (a == NGL - 1) { side = 1; t = c; }
-                        else if (a == 0) { side = 3; t = c; }
-                    }
-                    if (side < 0) continue;
-                    const int ifa = E2E[e * 4 + side] - 1;
-                    const int rev = EOR[e * 4 + side];
-                    const int ms = EMS[e * 4 + side];
-                    const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
-                    const int64_t fn = (int64_t)ifa * NGL + inc;
-                    const Br1Flux F = br1_face_flux(tr, NGL, ED + 8 * (int64_t)ifa, inc, nxa[fn], nya[fn], scal[fn],
-                                                    BL[fn], BR[fn], gc, tol, geomFace);
-                    if (ph == 0) gx0 += ms * F.x0 * rJ;
-                    else gx0 += ms * rJ * F.x0;
-                    gx1 += ms * rJ * F.x1;
-                    gy0 += ms * rJ * F.y0;
-                    gy1 += ms * rJ * F.y1;
-                }
-                // scaleGradient: multiply by h
-                gx0 = h * gx0; gx1 = h * gx1; gy0 = h * gy0; gy1 = h * gy1;
-                // side traces of the scaled gradients, [e][side][gX0, gX1, gY0, gY1][k] (k in the side's order)
-                double* T = gtr + (int64_t)e * 16 * NGL;
-                auto put = [&](int side, int k) {
-                    double* t = T + side * 4 * NGL + k;
-                    t[0] = gx0; t[NGL] = gx1; t[2 * NGL] = gy0; t[3 * NGL] = gy1;
-                };
-                if (c == 0) put(0, a);
-                if (c == NGL - 1) put(2, a);
-                if (a == NGL - 1) put(1, c);
-                if (a == 0) put(3, c);
-            }
-            __syncthreads();   // sEps ready; u, v no longer needed
-            // ---- viscous volume term: contravariant fluxes, then Dstrong ----
-            if (act) {
-                const double eps = sEps[el];
-                const double f2 = eps * gx0, f3 = eps * gx1, g2 = eps * gy0, g3 = eps * gy1;
-                sA[threadIdx.x] = ye * f2 - xe * g2;      // F2
-                sB[threadIdx.x] = -yx * f2 + xx * g2;     // G2
-                sC[threadIdx.x] = ye * f3 - xe * g3;      // F3
-                sE[threadIdx.x] = -yx * f3 + xx * g3;     // G3
-            }
-            __syncthreads();
-            if (act) {
-                const double* F2 = sA + el * NN;
-                const double* G2 = sB + el * NN;
-                const double* F3 = sC + el * NN;
-                const double* G3 = sE + el * NN;
-                double d2 = 0.0, d3 = 0.0;
-#pragma unroll
-                for (int l = 0; l < NGL; ++l) {
-                    d2 += sDs[a * NGL + l] * F2[c * NGL + l];
-                    d2 += sDs[c * NGL + l] * G2[l * NGL + a];
-                    d3 += sDs[a * NGL + l] * F3[c * NGL + l];
-                    d3 += sDs[c * NGL + l] * G3[l * NGL + a];
-                }
-                double* o = Qv + (int64_t)e * NN * 2 + n;
-                o[0] = rJ * d2;
-                o[NN] = rJ * d3;
-            }
-
-        __syncthreads();   // LDS free for the next element block
-        }
+    for (int grp = blockIdx.x; grp * EB::EPB < cnt; grp += gridDim.x) {
+        const int idx = grp * EB::EPB + el;
+        const bool act = idx < cnt;
+        const int e = act ? alist[idx] : 0;
+        av_element_block<NGL>(M, TraceFromSides{tr}, e, act, act ? visc[e] : 0.0, sDh, sDs, sA, sB, sC, sE, sEps);
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // fused_viscous_face: CollectEdgeDataGradient.okl:17-194 + calcNumFluxesViscose.okl:17-107
 // ---------------------------------------------------------------------------------------------
-// calcNumFluxesViscose.okl at face node i of face f from the side traces of the scaled gradients
-__device__ __forceinline__ void viscous_face_node(int f, int i, int ngl, double geomFace, int iel, int ier, int isl,
-                                                  int isr, int rv, int cpuL, int cpuR, const double* __restrict__ gtr,
-                                                  const double* __restrict__ visc, const double* __restrict__ nxa,
-                                                  const double* __restrict__ nya, const double* __restrict__ scal,
-                                                  double* __restrict__ SPV) {
-    const int64_t t = (int64_t)f * ngl + i;
-    const int64_t eid = (int64_t)f * ngl * 2 + i;
-    double N0 = 0.0, N1 = 0.0;
-    if (!(cpuL == -1 || cpuR == -1)) {
-        // both sides present (procL = procR = 0): the boundary-face branches of the gather never apply;
-        // the right element's trace is read at the reversed index, as CollectEdgeDataGradient does.
-        // Zero viscosity on both sides makes the flux zero: the traces are then not read.
-        const double m = fmax(visc[iel], visc[ier]);
-        if (m != 0.0) {
-            const int inc = (rv == 0) ? (ngl - 1) - i : i;
-            const double* TL = gtr + ((int64_t)iel * 4 + isl) * 4 * ngl + i;
-            const double* TR = gtr + ((int64_t)ier * 4 + isr) * 4 * ngl + inc;
-            const double nx = nxa[t], ny = nya[t];
-            const double f2L = nx * TL[0] + ny * TL[2 * ngl];
-            const double f2R = nx * TR[0] + ny * TR[2 * ngl];
-            const double f3L = nx * TL[ngl] + ny * TL[3 * ngl];
-            const double f3R = nx * TR[ngl] + ny * TR[3 * ngl];
-            N0 = 0.5 * m * (f2L + f2R);
-            N1 = 0.5 * m * (f3L + f3R);
-        }
-    }
-    const double sc = scal[t];
-    SPV[eid] = -sc * geomFace * N0;
-    SPV[eid + ngl] = -sc * geomFace * N1;
-}
-
 // Only elements marked this stage (active[e] == seq: viscosity on themselves or a face neighbour)
 // read these fluxes; every face of an unmarked element has zero viscosity on both sides, so its
 // flux is +-0 and the volume pass skips it.  The pass runs over the side nodes of the listed
@@ -342,11 +180,11 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
     // elements in one pass, the rest loop
     const int64_t nblk = (A.Nelem + EB::EPB - 1) / EB::EPB;
     const unsigned grid = (unsigned)std::max<int64_t>(std::min<int64_t>(nblk, 4096), 1);
-    k_fused_av<NGL><<<grid, EB::THREADS, 0, ctx->stream>>>(
-        A.Nelem, ctx->p.PosPresTOL, A.q, A.Dhat, A.Dstrong, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta,
-        A.ElemToEdge, A.EMS, A.EOR, A.EdgeData, F.tr, A.nx, A.ny, A.scal, A.bL, A.bR,
-        GatherConsts{ctx->p.bc_kind, ctx->p.h_zero}, ctx->p.geomFace, F.grad, A.ViscPara, F.alist,
-        F.acount + (F.seq & 1), F.qvisc);
+    const AvMesh M{A.q, A.Jac, A.Yxi, A.Yeta, A.Xxi, A.Xeta, A.ElemToEdge, A.EMS, A.EOR, A.EdgeData,
+                   A.nx, A.ny, A.scal, A.bL, A.bR, GatherConsts{ctx->p.bc_kind, ctx->p.h_zero},
+                   ctx->p.geomFace, ctx->p.PosPresTOL, F.grad, F.qvisc};
+    k_fused_av<NGL><<<grid, EB::THREADS, 0, ctx->stream>>>(A.Nelem, M, F.tr, A.Dhat, A.Dstrong, A.ViscPara, F.alist,
+                                                          F.acount + (F.seq & 1));
 }
 
 

@@ -319,13 +319,6 @@ __global__ void __launch_bounds__(256) k_update_ssp(int64_t n, double A, double 
 // The four energy sums of an element run on four threads, each in the okl's serial order.
 // (With the learned indicator the indicator kernel evaluates the sensor itself, esdg_mlp.hip.)
 // ---------------------------------------------------------------------------------------------
-template <int NGL>
-struct RowBlock {
-    static constexpr int EPB = 256 / NGL;            // elements per block
-    static constexpr int THREADS = EPB * NGL;
-    static constexpr int RS = NGL + 1;               // padded LDS row stride of an element's modes
-    static constexpr int ES = NGL * RS + 1;          // padded LDS stride between elements
-};
 
 template <int NGL>
 __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts sc, const double* __restrict__ q,
@@ -336,51 +329,13 @@ __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts
     __shared__ double sV[NN];
     __shared__ double sT[RB::EPB * RB::ES];
     __shared__ double sSum[RB::EPB][4];
-    const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
-    const int e = blockIdx.x * RB::EPB + el;
-    const bool act = (el < RB::EPB) && (e < Nelem);
     if (threadIdx.x < NN) sV[threadIdx.x] = VdmInv[threadIdx.x];
-    double u[NGL];
-    if (act) {
-        const int64_t xb = (int64_t)e * NN + r * NGL;
-        const int64_t qb = (int64_t)e * NN * ESDG_NEQ + r * NGL;
-#pragma unroll
-        for (int i = 0; i < NGL; ++i) u[i] = b[xb + i] + q[qb + i];
-    }
-    __syncthreads();
-    double* T = sT + (act ? el : 0) * RB::ES;
-    if (act) {
-        // column c = r of T
-#pragma unroll
-        for (int a = 0; a < NGL; ++a) {
-            double s = 0.0;
-#pragma unroll
-            for (int l = 0; l < NGL; ++l) s += sV[a * NGL + l] * u[l];
-            T[a * RB::RS + r] = s;
-        }
-    }
-    __syncthreads();
-    if (act) {
-        // row a = r of m, in place of row r of T (only this thread reads and writes that row)
-        double t[NGL];
-#pragma unroll
-        for (int l = 0; l < NGL; ++l) t[l] = T[r * RB::RS + l];
-#pragma unroll
-        for (int c = 0; c < NGL; ++c) {
-            double s = 0.0;
-#pragma unroll
-            for (int l = 0; l < NGL; ++l) s += t[l] * sV[c * NGL + l];
-            T[r * RB::RS + c] = s;
-        }
-    }
-    __syncthreads();
-    if (act && r < 4) sSum[el][r] = sensor_sum<NGL, RB::RS>(T, r);
-    __syncthreads();
-    if (act && r == 0) {
-        const double* S4 = sSum[el];
-        const double v = sensor_ramp(S4[0], S4[1], S4[2], S4[3], sc);
-        visc[e] = v;
-        if (visc_plot) visc_plot[e] = v;
+    const int e0 = blockIdx.x * RB::EPB;
+    const double v = sensor_rows<NGL>(Nelem, e0, sc, q, b, sV, sT, sSum);
+    const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
+    if (el < RB::EPB && e0 + el < Nelem && r == 0) {
+        visc[e0 + el] = v;
+        if (visc_plot) visc_plot[e0 + el] = v;
     }
 }
 

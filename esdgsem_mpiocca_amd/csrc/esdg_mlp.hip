@@ -9,6 +9,7 @@
 #include "esdg_internal.h"
 #include "esdg_mlp.h"
 #include "esdg_mlp_internal.h"
+#include "esdg_stage.h"
 
 namespace esdg {
 
@@ -246,18 +247,19 @@ struct MaskedSensor {
     const double* clock = nullptr;
     double T = 0.0;
     int clock_mode = 0;
+    // a scenario of a batched ensemble: an element given viscosity marks itself and its face
+    // neighbours with `mark`; with stage0 its viscosity also enters the scenario's maximum
+    const StageScn* scn = nullptr;
+    int mark = 0;
+    int stage0 = 0;
 };
 
-__global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* __restrict__ q,
-                                                       const double* __restrict__ B, const double* __restrict__ V,
-                                                       MlpDev W, float lo, float hi, float* __restrict__ p,
-                                                       signed char* __restrict__ estat, MaskedSensor MS) {
-    if (MS.clock) {
-        // a step issued after T changes nothing: no evaluation, no counts (uniform per block)
-        if ((MS.clock_mode == 1 && MS.clock[1] >= MS.T) || (MS.clock_mode == 2 && MS.clock[0] == 0.0)) return;
-    }
+// The indicator over elements e0 .. e0 + FEPB - 1 of one state (all 256 threads of the block).
+__device__ void mlp_indicator_block(int Nelem, int e0, const double* __restrict__ q, const double* __restrict__ B,
+                                    const double* __restrict__ V, const MlpDev& W, float lo, float hi,
+                                    float* __restrict__ p, signed char* __restrict__ estat, const MaskedSensor& MS) {
     if (MS.counters && threadIdx.x == 0)
-        atomicAdd(MS.counters + 4, (unsigned long long)min(FEPB, Nelem - (int)blockIdx.x * FEPB));
+        atomicAdd(MS.counters + 4, (unsigned long long)min(FEPB, Nelem - e0));
     __shared__ __attribute__((aligned(16))) uint16_t sW1[NH * KP], sW2[NH * NH];
     __shared__ float sw3[NH], sb[2 * NH + 1];
     __shared__ __attribute__((aligned(16))) uint16_t sX[2][16 * KP], sH[2][16 * NH];
@@ -268,7 +270,7 @@ __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* 
     for (int t = threadIdx.x; t < 2 * 16 * KP; t += 256) (&sX[0][0])[t] = 0;
     __syncthreads();
     const int el = threadIdx.x >> 3, r = threadIdx.x & 7;
-    const int e = blockIdx.x * FEPB + el;
+    const int e = e0 + el;
     const bool act = e < Nelem;
     float xn[8], modal;
     row_features(q, B, L, e, act, xn, modal);
@@ -284,7 +286,7 @@ __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* 
     if (w < 2) {
         mlp_tile(sX[w], sH[w], sW1, sW2, sw3, sb, sb + NH, sb[2 * NH], sZ[w]);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        const int ee = blockIdx.x * FEPB + w * 16 + l;
+        const int ee = e0 + w * 16 + l;
         if (l < 16 && ee < Nelem) {
             const float pv = sigmoidf_(sZ[w][l]);
             const signed char st = (pv <= lo) ? 0 : (pv >= hi ? 1 : 2);
@@ -295,6 +297,10 @@ __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* 
     }
     if (MS.visc) {
         __shared__ double sSum[FEPB][4];
+        // the block's counts and largest viscosity gather in LDS; one global atomic each at the end
+        __shared__ unsigned long long sCnt[4], sMax;
+        if (threadIdx.x < 4) sCnt[threadIdx.x] = 0ull;
+        if (threadIdx.x == 4) sMax = 0ull;
         __syncthreads();   // the classes are in sSt
         if (act && r < 4) sSum[el][r] = sensor_sum<8, FRS>(L.T + el * FES, r);
         __syncthreads();
@@ -304,15 +310,63 @@ __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* 
             const double plain = v;
             if (st == 0) v = 0.0;
             if (MS.counters) {
-                if (v > 0.0) atomicAdd(MS.counters, 1ull);
-                if (MS.audit && plain > 0.0 && st == 0) atomicAdd(MS.counters + 1, 1ull);
-                if (MS.audit && plain > 0.0) atomicAdd(MS.counters + 2, 1ull);
-                if (st != 0) atomicAdd(MS.counters + 3, 1ull);
+                if (v > 0.0) atomicAdd(&sCnt[0], 1ull);
+                if (MS.audit && plain > 0.0 && st == 0) atomicAdd(&sCnt[1], 1ull);
+                if (MS.audit && plain > 0.0) atomicAdd(&sCnt[2], 1ull);
+                if (st != 0) atomicAdd(&sCnt[3], 1ull);
             }
             MS.visc[e] = v;
             if (MS.visc_plot) MS.visc_plot[e] = v;
+            if (MS.scn) {
+                if (v != 0.0) mark_reach(*MS.scn, e, MS.mark);
+                // non-negative doubles order as their bit patterns; a NaN stays out, as fmax leaves it
+                if (MS.stage0 && v > 0.0) atomicMax(&sMax, (unsigned long long)__double_as_longlong(v));
+            }
+        }
+        const bool to_max = MS.scn && MS.stage0;
+        if (MS.counters || to_max) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (MS.counters)
+                    for (int k = 0; k < 4; ++k)
+                        if (sCnt[k]) atomicAdd(MS.counters + k, sCnt[k]);
+                if (to_max && sMax) atomicMax(MS.scn->emax, sMax);
+            }
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* __restrict__ q,
+                                                       const double* __restrict__ B, const double* __restrict__ V,
+                                                       MlpDev W, float lo, float hi, float* __restrict__ p,
+                                                       signed char* __restrict__ estat, MaskedSensor MS) {
+    if (MS.clock) {
+        // a step issued after T changes nothing: no evaluation, no counts (uniform per block)
+        if ((MS.clock_mode == 1 && MS.clock[1] >= MS.T) || (MS.clock_mode == 2 && MS.clock[0] == 0.0)) return;
+    }
+    mlp_indicator_block(Nelem, (int)blockIdx.x * FEPB, q, B, V, W, lo, hi, p, estat, MS);
+}
+
+// The indicator over the learned scenarios of a batched ensemble: block b takes tile tasks[b] (32
+// elements of one N = 7 scenario) of the stage state in buffer src.  A scenario whose time has
+// reached its end (dt = 0) is skipped.
+__global__ void __launch_bounds__(256) k_ens_indicator(const StageScn* __restrict__ scns,
+                                                       const StageTask* __restrict__ tasks, int src,
+                                                       const int* __restrict__ mark, int stage, MlpDev W, float lo,
+                                                       float hi, int audit, unsigned long long* counters) {
+    const StageTask tk = tasks[blockIdx.x];
+    const StageScn& S = scns[tk.scn];
+    if (stage > 0 && S.clock[0] == 0.0) return;
+    if (stage == 0 && S.clock[1] >= S.T) return;
+    MaskedSensor MS;
+    MS.visc = S.visc;
+    MS.sc = S.sc;
+    MS.audit = audit;
+    MS.counters = counters;
+    MS.scn = &S;
+    MS.mark = *mark + stage;
+    MS.stage0 = stage == 0;
+    mlp_indicator_block(S.Nelem, tk.e0, S.q[src], S.B, S.VdmInv, W, lo, hi, nullptr, nullptr, MS);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -485,6 +539,15 @@ int mlp_classify(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, c
     k_mlp_indicator<<<blocks_for(Nelem, FEPB), 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, m->dev(), lo, hi, p, estat,
                                                                        MaskedSensor{});
     return launch_check(ctx, "mlp_indicator");
+}
+
+int mlp_ensemble_indicator(esdg_ctx* ctx, const esdg_mlp* m, const StageScn* scns, const StageTask* tasks, int ntask,
+                           int src, const int* mark, int stage, float lo, float hi, int audit,
+                           unsigned long long* counters) {
+    if (ntask == 0) return ESDG_OK;
+    KernelScope ks(ctx, "ens_indicator");
+    k_ens_indicator<<<ntask, 256, 0, ctx->stream>>>(scns, tasks, src, mark, stage, m->dev(), lo, hi, audit, counters);
+    return launch_check(ctx, "ens_indicator");
 }
 
 int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B,

@@ -3,6 +3,7 @@
 
 #include "esdg_internal.h"
 #include "esdg_mlp.h"
+#include "esdg_stage.h"
 
 namespace esdg {
 
@@ -23,6 +24,12 @@ int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const doubl
                         const double* VdmInv, float lo, float hi, signed char* estat, int audit,
                         unsigned long long* counters, double* visc, double* visc_plot,
                         const double* clock = nullptr, double T = 0.0, int clock_mode = 0);
+// The same pass over the learned scenarios of a batched ensemble (tasks: 32-element tiles of N = 7
+// scenarios, in buffer src of their state): each viscous element marks itself and its face
+// neighbours with *mark + stage; in stage 0 the viscosities enter each scenario's maximum.
+int mlp_ensemble_indicator(esdg_ctx* ctx, const esdg_mlp* m, const StageScn* scns, const StageTask* tasks, int ntask,
+                           int src, const int* mark, int stage, float lo, float hi, int audit,
+                           unsigned long long* counters);
 
 
 }  // namespace esdg

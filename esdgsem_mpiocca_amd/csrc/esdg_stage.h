@@ -13,7 +13,10 @@
 
 #include <vector>
 
+#include "esdg_device.h"
 #include "esdg_internal.h"
+
+struct esdg_mlp;
 
 namespace esdg {
 
@@ -29,11 +32,37 @@ struct StageScn {
     double* clock;      // [0] dt of the current step (0 once t >= T), [1] t, [2] steps taken,
                         // [3] dt of the last step taken, [4] 1 if a dt was not positive and finite
     unsigned long long* lmax;   // bit pattern of max lambda (non-negative doubles order as integers)
+    // artificial viscosity (av = 1): the modal sensor, or with learned = 1 (N = 7) the learned
+    // indicator with the sensor as fallback, evaluated on each stage's state; BR1 gradients and the
+    // viscous terms only where viscosity is in reach (active[e] == the stage's mark)
+    int av, learned;
+    double dvc;         // DFL / ngl^2 * min_ele_size^2            (dt_v = dvc / eps_max)
+    SensorConsts sc;
+    const double *B, *VdmInv, *Dhat, *Dstrong;
+    double* visc;       // element viscosity of the stage's state [Nelem]
+    double* gtr;        // side traces of the scaled velocity gradients [Nelem][4][4][ngl]
+    double* Qv;         // viscous volume term [Nelem][2][ngl^2] (hu, hv)
+    double* SPV;        // viscous face fluxes [Nfaces][2][ngl]
+    int* active;        // == mark: viscosity on the element or a face neighbour this stage
+    unsigned long long* emax;   // bit pattern of the step's max viscosity
 };
 
 struct StageTask {
     int scn, e0, ngl, pad;
 };
+
+// An element with viscosity marks itself and its face neighbours for this stage (the elements whose
+// viscous terms can be non-zero), as the fused face pass does for both elements of every face with
+// viscosity on either side.  Plain stores of one value: concurrent marks agree.
+__device__ __forceinline__ void mark_reach(const StageScn& S, int e, int m) {
+    S.active[e] = m;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int* E = S.EdgeData + 8 * (int64_t)(S.E2E[e * 4 + s] - 1);
+        if (E[5] == 0) S.active[E[0]] = m;
+        if (E[6] == 0) S.active[E[1]] = m;
+    }
+}
 
 // How the tasks of a stage are ordered and launched (the autotuner's search space).
 struct StageSchedule {
@@ -52,7 +81,8 @@ class StageBatch {
     int set_schedule(esdg_ctx* ctx, const StageSchedule& sc);
     // initial lambda_max of every scenario from the state in buffer `src`
     int lambda(esdg_ctx* ctx, int src);
-    // one time step: dt kernel, then the RK stages; src flips once per stage
+    // one time step: [marks, viscosity,] dt kernel, then the RK stages [each after its viscosity
+    // and viscous terms]; src flips once per stage
     int step(esdg_ctx* ctx, const double* A, const double* C, int stages, int& src);
     // `steps` steps: pairs of steps replayed from a captured graph (schedule.graph), the rest eager
     int run(esdg_ctx* ctx, const double* A, const double* C, int stages, int steps, int& src);
@@ -69,9 +99,28 @@ class StageBatch {
     int ntask = 0;
     int ncu = 0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};   // graph of two steps, one per starting parity
+    // artificial viscosity: element tiles of the AV scenarios for the sensor (row tiles), the
+    // learned indicator (32-element tiles of learned scenarios), the gradient pass (node tiles) and
+    // the viscous face pass; the step's mark counter lives on the device (graph replays advance it)
+    bool has_av = false;
+    StageTask *d_sens = nullptr, *d_ind = nullptr, *d_grad = nullptr, *d_vface = nullptr;
+    int nsens = 0, nind = 0, ngrad = 0, nvface = 0;
+    int* d_mark = nullptr;
+    unsigned long long* d_icount = nullptr;        // learned indicator counts (5, as the solver's)
+    const esdg_mlp* mlp = nullptr;
+    float ind_lo = 0.5f, ind_hi = 0.5f;
+    int ind_audit = 0;
+    int64_t marks_used = 0;
+    // the learned indicator for the AV scenarios with N = 7 (m = null: the modal sensor everywhere)
+    int set_indicator(esdg_ctx* ctx, const esdg_mlp* m, float lo, float hi);
+
+    void drop_graphs();
 
   private:
-    void drop_graphs();
+    int build_av_tasks(esdg_ctx* ctx);
+    int reset_marks(esdg_ctx* ctx);
+    int av_pass(esdg_ctx* ctx, int src, int stage);     // viscosity of the stage's state + marks
+    int av_terms(esdg_ctx* ctx, int src, int stage);    // gradients, viscous volume and face terms
 };
 
 }  // namespace esdg

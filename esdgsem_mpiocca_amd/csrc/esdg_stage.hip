@@ -8,7 +8,9 @@
 //   [PosPres]                                          (PosPres.okl)
 //   [FindLambdaMax of the new state, last stage]      (FindLambdaMax.okl)
 // and the dt of DGtimeloop (src/deviceclass.cpp:899-930) runs on the device.
+#include "esdg_av_device.h"
 #include "esdg_device.h"
+#include "esdg_mlp_internal.h"
 #include "esdg_stage.h"
 
 #include <algorithm>
@@ -46,7 +48,7 @@ struct MaxLds<HI, HI> {
 // Volume term, surface lift, RK update, limiter and wave speed of one block of elements; the
 // face fluxes of the elements' sides are in LDS (FS).  Pointer parameters are __restrict__ so
 // that the geometry loads schedule as in a kernel taking them as arguments.
-template <int NGL>
+template <int NGL, bool AV>
 __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, double* FS, double g, double tol,
                                            double zero_tol, int positivity, double rkA, double rkC, double dt, bool last,
                                            const double* __restrict__ qi, double* __restrict__ qo,
@@ -55,7 +57,7 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
                                            const double* __restrict__ Xxi, const double* __restrict__ Jac,
                                            const double* __restrict__ Bx, const double* __restrict__ By,
                                            const double* __restrict__ EleSizes, const double* __restrict__ GLw,
-                                           unsigned long long* __restrict__ lmax) {
+                                           unsigned long long* __restrict__ lmax, const StageScn* SV, int mark) {
     using C = VolCfg<NGL>;
     const int lane = threadIdx.x;
     double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], Y[NGL], X[NGL];
@@ -150,6 +152,44 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
                 b1[k] += F[k] * rJ[k];
                 b2[k] += F[NGL + k] * rJ[k];
                 b3[k] += F[2 * NGL + k] * rJ[k];
+            }
+        }
+        if constexpr (AV) {
+            // the viscous terms of an element with viscosity in reach, in the reference's order
+            // after the surface lift: the viscous volume term (where the element's viscosity is not
+            // zero), then the lift of the viscous face fluxes (VolumeKernelViscose.okl,
+            // SurfaceKernelVisc.okl), as the fused volume pass adds them
+            if (SV->av && SV->active[e] == mark) {
+                if (SV->visc[e] != 0.0) {
+                    const double* qv = SV->Qv + (int64_t)e * C::NN * 2 + r;
+#pragma unroll
+                    for (int k = 0; k < NGL; ++k) {
+                        b2[k] += qv[k * NGL];
+                        b3[k] += qv[C::NN + k * NGL];
+                    }
+                }
+#pragma unroll
+                for (int ph = 0; ph < 2; ++ph) {
+#pragma unroll
+                    for (int k = 0; k < NGL; ++k) {
+                        int side = -1, t = 0;
+                        if (ph == 0) {
+                            if (k == 0) { side = 0; t = r; }
+                            else if (k == NGL - 1) { side = 2; t = r; }
+                        } else {
+                            if (r == NGL - 1) { side = 1; t = k; }
+                            else if (r == 0) { side = 3; t = k; }
+                        }
+                        if (side < 0) continue;
+                        const int ifa = SV->E2E[e * 4 + side] - 1;
+                        const int rev = SV->EOR[e * 4 + side];
+                        const int ms = SV->EMS[e * 4 + side];
+                        const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
+                        const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
+                        b2[k] += -ms * rJ[k] * SV->SPV[fl];
+                        b3[k] += -ms * rJ[k] * SV->SPV[fl + NGL];
+                    }
+                }
             }
         }
         // low-storage RK: gRK = A gRK + Qt; q_new = q + (C dt) gRK (q re-read: an L2 hit, and it
@@ -253,9 +293,9 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
 }
 
 
-template <int NGL>
+template <int NGL, bool AV>
 __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* lds, int src, double rkA, double rkC,
-                                           bool last) {
+                                           bool last, int mark) {
     using C = VolCfg<NGL>;
     using SC = StageCfg<NGL>;
     const int lane = threadIdx.x;
@@ -294,18 +334,18 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
         }
     }
 
-    stage_core<NGL>(e, r, act, P, FS, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo, S.gRK, S.Yeta,
-                    S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax);
+    stage_core<NGL, AV>(e, r, act, P, FS, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo, S.gRK,
+                        S.Yeta, S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax, &S, mark);
 }
 
-template <int LO, int HI>
+template <int LO, int HI, bool AV>
 __device__ __forceinline__ void stage_task(const StageScn* __restrict__ scns, const StageTask& tk, double* lds, int src,
-                                           double rkA, double rkC, bool last) {
+                                           double rkA, double rkC, bool last, int mark) {
     const StageScn& S = scns[tk.scn];
     switch (tk.ngl) {
 #define ESDG_CASE(n)                                                                \
     case n:                                                                         \
-        if constexpr (n >= LO && n <= HI) stage_body<n>(S, tk.e0, lds, src, rkA, rkC, last);        \
+        if constexpr (n >= LO && n <= HI) stage_body<n, AV>(S, tk.e0, lds, src, rkA, rkC, last, mark);        \
         break;
         ESDG_CASE(3) ESDG_CASE(4) ESDG_CASE(5) ESDG_CASE(6) ESDG_CASE(7) ESDG_CASE(8)
 #undef ESDG_CASE
@@ -318,10 +358,12 @@ __device__ __forceinline__ void stage_task(const StageScn* __restrict__ scns, co
 // parts and the blocks that the dispatcher places on XCD x (b mod 8 = x) take part x, so tasks
 // that are adjacent in the list (one element tile of several scenarios that share geometry)
 // meet in the same L2.  Every block leaves after a bounded number of tasks.
-template <int LO, int HI, bool PERSIST>
+template <int LO, int HI, bool PERSIST, bool AV>
 __global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ scns, const StageTask* __restrict__ tasks,
-                                                 int n, int xcd, int src, double rkA, double rkC, int last) {
+                                                 int n, int xcd, int src, double rkA, double rkC, int last,
+                                                 const int* __restrict__ markp, int stage) {
     __shared__ double lds[MaxLds<LO, HI>::value];
+    const int mark = AV ? *markp + stage : 0;
     const int b = blockIdx.x;
     int beg = 0, cnt = n, i0 = b, stride = gridDim.x;
     if (xcd) {
@@ -332,11 +374,11 @@ __global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ sc
         stride = gridDim.x >> 3;
     }
     if constexpr (!PERSIST) {
-        if (i0 < cnt) stage_task<LO, HI>(scns, tasks[beg + i0], lds, src, rkA, rkC, last != 0);
+        if (i0 < cnt) stage_task<LO, HI, AV>(scns, tasks[beg + i0], lds, src, rkA, rkC, last != 0, mark);
     } else {
         for (int i = i0; i < cnt; i += stride) {
             if (i != i0) __syncthreads();   // LDS of the previous task is free
-            stage_task<LO, HI>(scns, tasks[beg + i], lds, src, rkA, rkC, last != 0);
+            stage_task<LO, HI, AV>(scns, tasks[beg + i], lds, src, rkA, rkC, last != 0, mark);
         }
     }
 }
@@ -364,21 +406,28 @@ __global__ void __launch_bounds__(256) k_stage_lambda(const StageScn* __restrict
 }
 
 // DGtimeloop's dt (src/deviceclass.cpp:899-930) per scenario: while t < T, dt = min(T - t,
-// dti / lambda_max) and t += dt; once t >= T (or after a dt that is not positive and finite, which
-// is flagged) dt = 0 and the stages leave the state unchanged.  lambda_max is reset for the next
-// step's reduction.
+// dti / lambda_max[, dvc / eps_max]) and t += dt; once t >= T (or after a dt that is not positive
+// and finite, which is flagged) dt = 0 and the stages leave the state unchanged.  lambda_max and
+// eps_max are reset for the next step's reductions.
 __global__ void k_stage_dt(const StageScn* __restrict__ scns, int nscn) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nscn) return;
     const StageScn& S = scns[s];
     const double lm = __longlong_as_double((long long)*S.lmax);
     *S.lmax = 0ull;
+    double em = 0.0;
+    if (S.av) {
+        em = __longlong_as_double((long long)*S.emax);
+        *S.emax = 0ull;
+    }
     double* ck = S.clock;
     const double t = ck[1];
     double dt = 0.0;
     if (t < S.T && ck[4] == 0.0) {
         const double dt_i = S.dti / lm;
-        dt = fmin(S.T - t, dt_i);
+        // with artificial viscosity also the viscous limit dvc / max eps (infinite where no element
+        // has viscosity), in the order of src/deviceclass.cpp:920-926
+        dt = S.av ? fmin(S.T - t, fmin(dt_i, S.dvc / em)) : fmin(S.T - t, dt_i);
         if (!(dt > 0.0) || isinf(dt)) {
             ck[4] = 1.0;
             dt = 0.0;
@@ -389,6 +438,139 @@ __global__ void k_stage_dt(const StageScn* __restrict__ scns, int nscn) {
         }
     }
     ck[0] = dt;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Artificial viscosity of the scenarios with av = 1.  Before each stage: the element viscosity of
+// the stage's state (ShockCapturing.okl, or the learned indicator's pass in esdg_mlp.hip), each
+// viscous element marking itself and its face neighbours with the stage's mark; the gradient pass
+// (calcGradient + SurfaceKernelGradient + scaleGradient + VolumeKernelViscose) over the marked
+// elements; the viscous face fluxes (CollectEdgeDataGradient + calcNumFluxesViscose) of the faces
+// of marked elements.  The stage kernel then adds the viscous terms of the marked elements.  The
+// same work as the fused schedule's AV passes (esdg_fused.hip), element by element, with the face
+// traces read from the state instead of a side-trace buffer: the same bits.
+// ---------------------------------------------------------------------------------------------
+
+// The marks of a step: stage s of the step marks with *mark + s.  Advanced by 8 at each step on
+// the device, so that replays of a captured step take fresh marks.
+__global__ void k_stage_mark(int* mark) { *mark += 8; }
+
+template <int LO, int HI>
+struct MaxRowLds {
+    static constexpr int value = cmax(RowBlock<LO>::EPB * RowBlock<LO>::ES, MaxRowLds<LO + 1, HI>::value);
+};
+template <int HI>
+struct MaxRowLds<HI, HI> {
+    static constexpr int value = RowBlock<HI>::EPB * RowBlock<HI>::ES;
+};
+
+template <int NGL>
+__device__ __forceinline__ void ens_sensor_body(const StageScn& S, int e0, int src, int m, bool stage0, double* sV,
+                                                double* sT, double (*sSum)[4], unsigned long long* sMax) {
+    if (threadIdx.x < NGL * NGL) sV[threadIdx.x] = S.VdmInv[threadIdx.x];
+    const double v = sensor_rows<NGL>(S.Nelem, e0, S.sc, S.q[src], S.B, sV, sT, sSum);
+    const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
+    const int e = e0 + el;
+    if (el < RowBlock<NGL>::EPB && e < S.Nelem && r == 0) {
+        S.visc[e] = v;
+        if (v != 0.0) mark_reach(S, e, m);
+        // non-negative doubles order as their bit patterns; a NaN stays out, as fmax leaves it
+        if (stage0 && v > 0.0) atomicMax(sMax, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+// ShockCapturing of the stage's state over RowBlock tiles (tasks: {scenario, first element, ngl}).
+// A scenario whose time has reached T (or whose step has dt = 0) is skipped; in stage 0 the
+// block's largest viscosity enters the scenario's eps_max.
+template <int LO, int HI>
+__global__ void __launch_bounds__(256) k_ens_sensor(const StageScn* __restrict__ scns,
+                                                    const StageTask* __restrict__ tasks, int src,
+                                                    const int* __restrict__ markp, int stage) {
+    __shared__ double sV[HI * HI];
+    __shared__ double sT[MaxRowLds<LO, HI>::value];
+    __shared__ double sSum[RowBlock<LO>::EPB][4];
+    __shared__ unsigned long long sMax;
+    const StageTask tk = tasks[blockIdx.x];
+    const StageScn& S = scns[tk.scn];
+    if (stage > 0 && S.clock[0] == 0.0) return;
+    if (stage == 0 && S.clock[1] >= S.T) return;
+    const int m = *markp + stage;
+    if (threadIdx.x == 0) sMax = 0ull;   // published by the sensor's first barrier
+    switch (tk.ngl) {
+#define ESDG_CASE(n)                                                                          \
+    case n:                                                                                   \
+        if constexpr (n >= LO && n <= HI) ens_sensor_body<n>(S, tk.e0, src, m, stage == 0, sV, sT, sSum, &sMax); \
+        break;
+        ESDG_CASE(3) ESDG_CASE(4) ESDG_CASE(5) ESDG_CASE(6) ESDG_CASE(7) ESDG_CASE(8)
+#undef ESDG_CASE
+        default: break;
+    }
+    if (stage == 0) {
+        __syncthreads();
+        if (threadIdx.x == 0 && sMax) atomicMax(S.emax, sMax);
+    }
+}
+
+template <int NGL>
+__device__ __forceinline__ void ens_grad_body(const StageScn& S, int e0, int src, int m, double* sDh, double* sDs,
+                                              double* sA, double* sB, double* sC, double* sE, double* sEps) {
+    using EB = ElemBlock<NGL>;
+    constexpr int NN = NGL * NGL;
+    const int el = threadIdx.x / NN;
+    const int e = e0 + el;
+    const bool act = el < EB::EPB && e < S.Nelem && S.active[e] == m;
+    if (!__syncthreads_or(act)) return;   // no marked element in the tile (uniform per block)
+    if (threadIdx.x < NN) {
+        sDh[threadIdx.x] = S.Dhat[threadIdx.x];
+        sDs[threadIdx.x] = S.Dstrong[threadIdx.x];
+    }
+    __syncthreads();
+    const AvMesh M{S.q[src], S.Jac, S.Yxi, S.Yeta, S.Xxi, S.Xeta, S.E2E, S.EMS, S.EOR, S.EdgeData,
+                   S.nx, S.ny, S.scal, S.bL, S.bR, GatherConsts{S.bc, S.h_zero}, S.geomFace, S.tol, S.gtr, S.Qv};
+    av_element_block<NGL>(M, TraceFromState{S.q[src]}, e, act, act ? S.visc[e] : 0.0, sDh, sDs, sA, sB, sC, sE, sEps);
+}
+
+// The gradient pass over ElemBlock tiles (tasks: {scenario, first element, ngl}): the marked
+// elements of each tile, one thread per node.
+template <int LO, int HI>
+__global__ void __launch_bounds__(256) k_ens_grad(const StageScn* __restrict__ scns,
+                                                  const StageTask* __restrict__ tasks, int src,
+                                                  const int* __restrict__ markp, int stage) {
+    __shared__ double sDh[HI * HI], sDs[HI * HI];
+    __shared__ double sA[256], sB[256], sC[256], sE[256];
+    __shared__ double sEps[ElemBlock<LO>::EPB];
+    const StageTask tk = tasks[blockIdx.x];
+    const StageScn& S = scns[tk.scn];
+    const int m = *markp + stage;
+    switch (tk.ngl) {
+#define ESDG_CASE(n)                                                                          \
+    case n:                                                                                   \
+        if constexpr (n >= LO && n <= HI) ens_grad_body<n>(S, tk.e0, src, m, sDh, sDs, sA, sB, sC, sE, sEps); \
+        break;
+        ESDG_CASE(3) ESDG_CASE(4) ESDG_CASE(5) ESDG_CASE(6) ESDG_CASE(7) ESDG_CASE(8)
+#undef ESDG_CASE
+        default: break;
+    }
+}
+
+// The viscous face fluxes over tiles of 256 face nodes (tasks: {scenario, first face node, ngl}):
+// a face node is evaluated when either element of its face is marked; the faces a marked element
+// reads are exactly those.
+__global__ void __launch_bounds__(256) k_ens_vface(const StageScn* __restrict__ scns,
+                                                   const StageTask* __restrict__ tasks,
+                                                   const int* __restrict__ markp, int stage) {
+    const StageTask tk = tasks[blockIdx.x];
+    const StageScn& S = scns[tk.scn];
+    const int m = *markp + stage;
+    const int ngl = tk.ngl;
+    const int64_t it = (int64_t)tk.e0 + threadIdx.x;
+    if (it >= (int64_t)S.Nfaces * ngl) return;
+    const int f = (int)(it / ngl), i = (int)(it % ngl);
+    const int* E = S.EdgeData + 8 * (int64_t)f;
+    const int iel = E[0], ier = E[1], cpuL = E[5], cpuR = E[6];
+    if (!((cpuL == 0 && S.active[iel] == m) || (cpuR == 0 && S.active[ier] == m))) return;
+    viscous_face_node(f, i, ngl, S.geomFace, iel, ier, E[2], E[3], E[4], cpuL, cpuR, S.gtr, S.visc, S.nx, S.ny,
+                      S.scal, S.SPV);
 }
 
 // Loads each degree's D into c_stage_D once per device; a later batch must bring the same D.
@@ -428,10 +610,107 @@ int StageBatch::init(esdg_ctx* ctx, const std::vector<StageScn>& scns) {
     if (rc) return rc;
     rc = hip_check(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
     if (rc) return rc;
+    has_av = false;
+    for (const StageScn& S : scns) has_av = has_av || S.av;
     rc = esdg_malloc(ctx, (void**)&d_scn, scns.size() * sizeof(StageScn));
     if (!rc) rc = esdg_memcpy(ctx, d_scn, scns.data(), scns.size() * sizeof(StageScn), ESDG_H2D);
+    if (!rc && has_av) {
+        rc = esdg_malloc(ctx, (void**)&d_mark, 64);
+        if (!rc) rc = esdg_malloc(ctx, (void**)&d_icount, 5 * sizeof(unsigned long long));
+        if (!rc) rc = esdg_fill(ctx, reinterpret_cast<double*>(d_mark), 0.0, 8);
+        if (!rc) rc = esdg_fill(ctx, reinterpret_cast<double*>(d_icount), 0.0, 5);
+        if (!rc) rc = build_av_tasks(ctx);
+    }
     if (!rc) rc = set_schedule(ctx, StageSchedule{});
     return rc;
+}
+
+// Element and face tiles of the AV scenarios for the sensor (RowBlock tiles), the learned
+// indicator (32-element tiles), the gradient pass (ElemBlock tiles) and the viscous face pass
+// (256 face nodes).
+int StageBatch::build_av_tasks(esdg_ctx* ctx) {
+    std::vector<StageTask> sens, ind, grad, vface;
+    for (int s = 0; s < (int)host.size(); ++s) {
+        const StageScn& S = host[s];
+        if (!S.av) continue;
+        const int ngl = S.ngl;
+        if (S.learned)
+            for (int e0 = 0; e0 < S.Nelem; e0 += 32) ind.push_back(StageTask{s, e0, ngl, 0});
+        else
+            for (int e0 = 0; e0 < S.Nelem; e0 += 256 / ngl) sens.push_back(StageTask{s, e0, ngl, 0});
+        const int epb = std::max(256 / (ngl * ngl), 1);
+        for (int e0 = 0; e0 < S.Nelem; e0 += epb) grad.push_back(StageTask{s, e0, ngl, 0});
+        for (int64_t k = 0; k < (int64_t)S.Nfaces * ngl; k += 256) vface.push_back(StageTask{s, (int)k, ngl, 0});
+    }
+    auto put = [&](const std::vector<StageTask>& v, StageTask*& d, int& n) -> int {
+        if (d) esdg_free(ctx, d);
+        d = nullptr;
+        n = (int)v.size();
+        if (v.empty()) return ESDG_OK;
+        int rc = esdg_malloc(ctx, (void**)&d, v.size() * sizeof(StageTask));
+        if (!rc) rc = esdg_memcpy(ctx, d, v.data(), v.size() * sizeof(StageTask), ESDG_H2D);
+        return rc;
+    };
+    int rc = put(sens, d_sens, nsens);
+    if (!rc) rc = put(ind, d_ind, nind);
+    if (!rc) rc = put(grad, d_grad, ngrad);
+    if (!rc) rc = put(vface, d_vface, nvface);
+    return rc;
+}
+
+int StageBatch::set_indicator(esdg_ctx* ctx, const esdg_mlp* m, float lo, float hi) {
+    if (m && !(lo <= hi)) return set_error(ctx, ESDG_ERR_ARG, "set_indicator: lo > hi");
+    int learned = 0;
+    for (StageScn& S : host) {
+        S.learned = (m && S.av && S.ngl == 8) ? 1 : 0;
+        learned += S.learned;
+    }
+    if (m && !learned)
+        return set_error(ctx, ESDG_ERR_ARG, "the learned indicator needs a scenario with N = 7 and artificial viscosity");
+    mlp = m;
+    ind_lo = lo;
+    ind_hi = hi;
+    drop_graphs();
+    int rc = esdg_memcpy(ctx, d_scn, host.data(), host.size() * sizeof(StageScn), ESDG_H2D);
+    if (!rc && has_av) rc = build_av_tasks(ctx);
+    return rc;
+}
+
+// A fresh start of the marks (the step counter would wrap): every mark array back to 0.
+int StageBatch::reset_marks(esdg_ctx* ctx) {
+    int rc = esdg_fill(ctx, reinterpret_cast<double*>(d_mark), 0.0, 8);
+    for (const StageScn& S : host)
+        if (!rc && S.av) rc = esdg_fill(ctx, reinterpret_cast<double*>(S.active), 0.0, ((size_t)S.Nelem + 1) / 2);
+    marks_used = 0;
+    return rc;
+}
+
+int StageBatch::av_pass(esdg_ctx* ctx, int src, int stage) {
+    if (nsens) {
+        KernelScope ks(ctx, "ens_sensor");
+        k_ens_sensor<3, 8><<<nsens, 256, 0, ctx->stream>>>(d_scn, d_sens, src, d_mark, stage);
+        int rc = launch_check(ctx, "ens_sensor");
+        if (rc) return rc;
+    }
+    if (nind) {
+        if (!mlp) return set_error(ctx, ESDG_ERR_STATE, "learned scenarios without a network");
+        int rc = mlp_ensemble_indicator(ctx, mlp, d_scn, d_ind, nind, src, d_mark, stage, ind_lo, ind_hi, ind_audit,
+                                        d_icount);
+        if (rc) return rc;
+    }
+    return ESDG_OK;
+}
+
+int StageBatch::av_terms(esdg_ctx* ctx, int src, int stage) {
+    {
+        KernelScope ks(ctx, "ens_grad");
+        k_ens_grad<3, 8><<<ngrad, 256, 0, ctx->stream>>>(d_scn, d_grad, src, d_mark, stage);
+        int rc = launch_check(ctx, "ens_grad");
+        if (rc) return rc;
+    }
+    KernelScope ks(ctx, "ens_viscous_face");
+    k_ens_vface<<<nvface, 256, 0, ctx->stream>>>(d_scn, d_vface, d_mark, stage);
+    return launch_check(ctx, "ens_viscous_face");
 }
 
 void StageBatch::drop_graphs() {
@@ -520,19 +799,41 @@ int StageBatch::lambda(esdg_ctx* ctx, int src) {
     return launch_check(ctx, "stage_lambda");
 }
 
-template <int LO, int HI>
+template <int LO, int HI, bool AV>
 static void launch_stage(esdg_ctx* ctx, const StageBatch& b, const StageBatch::Range& r, int src, double rkA,
-                         double rkC, int last) {
+                         double rkC, int last, int stage) {
     if (r.persistent)
-        k_stage<LO, HI, true><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, src, rkA, rkC,
-                                                               last);
+        k_stage<LO, HI, true, AV><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, src, rkA,
+                                                                   rkC, last, b.d_mark, stage);
     else
-        k_stage<LO, HI, false><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, src, rkA, rkC,
-                                                                last);
+        k_stage<LO, HI, false, AV><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, src,
+                                                                    rkA, rkC, last, b.d_mark, stage);
+}
+
+template <bool AV>
+static void launch_ranges(esdg_ctx* ctx, const StageBatch& b, const StageBatch::Range& r, int src, double rkA,
+                          double rkC, int last, int stage) {
+    if (r.lo != r.hi) launch_stage<3, 8, AV>(ctx, b, r, src, rkA, rkC, last, stage);
+    else switch (r.lo) {
+        case 3: launch_stage<3, 3, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
+        case 4: launch_stage<4, 4, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
+        case 5: launch_stage<5, 5, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
+        case 6: launch_stage<6, 6, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
+        case 7: launch_stage<7, 7, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
+        default: launch_stage<8, 8, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
+    }
 }
 
 int StageBatch::step(esdg_ctx* ctx, const double* A, const double* C, int stages, int& src) {
     const int ns = (int)host.size();
+    if (has_av) {
+        // the step's marks, then the viscosity of the step's state (its maximum enters the dt)
+        KernelScope ks(ctx, "stage_mark");
+        k_stage_mark<<<1, 1, 0, ctx->stream>>>(d_mark);
+        int rc = launch_check(ctx, "stage_mark");
+        if (!rc) rc = av_pass(ctx, src, 0);
+        if (rc) return rc;
+    }
     {
         KernelScope ks(ctx, "stage_dt");
         k_stage_dt<<<(ns + 63) / 64, 64, 0, ctx->stream>>>(d_scn, ns);
@@ -541,17 +842,15 @@ int StageBatch::step(esdg_ctx* ctx, const double* A, const double* C, int stages
     }
     for (int s = 0; s < stages; ++s) {
         const int last = (s == stages - 1);
+        if (has_av) {
+            int rc = (s > 0) ? av_pass(ctx, src, s) : ESDG_OK;
+            if (!rc) rc = av_terms(ctx, src, s);
+            if (rc) return rc;
+        }
         for (const Range& r : ranges) {
             KernelScope ks(ctx, "stage");
-            if (r.lo != r.hi) launch_stage<3, 8>(ctx, *this, r, src, A[s], C[s], last);
-            else switch (r.lo) {
-                case 3: launch_stage<3, 3>(ctx, *this, r, src, A[s], C[s], last); break;
-                case 4: launch_stage<4, 4>(ctx, *this, r, src, A[s], C[s], last); break;
-                case 5: launch_stage<5, 5>(ctx, *this, r, src, A[s], C[s], last); break;
-                case 6: launch_stage<6, 6>(ctx, *this, r, src, A[s], C[s], last); break;
-                case 7: launch_stage<7, 7>(ctx, *this, r, src, A[s], C[s], last); break;
-                default: launch_stage<8, 8>(ctx, *this, r, src, A[s], C[s], last); break;
-            }
+            if (has_av) launch_ranges<true>(ctx, *this, r, src, A[s], C[s], last, s);
+            else launch_ranges<false>(ctx, *this, r, src, A[s], C[s], last, s);
             int rc = launch_check(ctx, "stage");
             if (rc) return rc;
         }
@@ -562,6 +861,14 @@ int StageBatch::step(esdg_ctx* ctx, const double* A, const double* C, int stages
 
 int StageBatch::run(esdg_ctx* ctx, const double* A, const double* C, int stages, int steps, int& src) {
     int n = 0;
+    if (has_av) {
+        // marks advance by 8 per step: start them afresh long before the int would wrap
+        if (marks_used + (int64_t)steps + 2 > (int64_t)(1 << 27)) {
+            int rc = reset_marks(ctx);
+            if (rc) return rc;
+        }
+        marks_used += steps;
+    }
     if (sched.graph && !ctx->profile && steps >= 2) {
         // two steps flip the buffers an even number of times, so the graph that starts from buffer
         // `src` ends there and can be replayed
@@ -597,10 +904,14 @@ int StageBatch::run(esdg_ctx* ctx, const double* A, const double* C, int stages,
 
 void StageBatch::release(esdg_ctx* ctx) {
     drop_graphs();
+    for (StageTask** p : {&d_task, &d_sens, &d_ind, &d_grad, &d_vface})
+        if (*p) { esdg_free(ctx, *p); *p = nullptr; }
     if (d_scn) esdg_free(ctx, d_scn);
-    if (d_task) esdg_free(ctx, d_task);
+    if (d_mark) esdg_free(ctx, d_mark);
+    if (d_icount) esdg_free(ctx, d_icount);
     d_scn = nullptr;
-    d_task = nullptr;
+    d_mark = nullptr;
+    d_icount = nullptr;
 }
 
 }  // namespace esdg

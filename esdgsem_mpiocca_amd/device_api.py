@@ -471,6 +471,23 @@ class Ensemble:
         self.ctx.check(lib().esdg_ensemble_bytes(C.c_void_p(self.handle), C.byref(b)))
         return b.value
 
+    def set_indicator(self, mlp, lo: float = 0.5, hi: float = 0.5):
+        """Learned indicator for the AV members with N = 7 (None: the modal sensor everywhere).
+        The network object must stay open while the ensemble uses it."""
+        self._mlp = mlp
+        self.ctx.check(lib().esdg_ensemble_set_indicator(C.c_void_p(self.handle),
+                                                         C.c_void_p(mlp.handle if mlp else None),
+                                                         C.c_float(lo), C.c_float(hi)))
+
+    def indicator_audit(self, on: bool = True):
+        self.ctx.check(lib().esdg_ensemble_indicator_audit(C.c_void_p(self.handle), int(on)))
+
+    def indicator_counts(self):
+        v = [C.c_int64() for _ in range(5)]
+        self.ctx.check(lib().esdg_ensemble_indicator_counts(C.c_void_p(self.handle), *[C.byref(x) for x in v]))
+        return dict(zip(("viscous", "false_negatives", "sensor_positives", "not_smooth", "element_stages"),
+                        [x.value for x in v]))
+
     def close(self):
         if self.handle:
             lib().esdg_ensemble_destroy(C.c_void_p(self.handle))

@@ -13,13 +13,17 @@
  * and bathymetry arrays share one device copy.
  *
  * Supported per scenario: 2 <= N <= 7 (3 <= ngl <= 8), flux differencing, ES or rotated ES flux,
- * walls / inner-outer / ocean boundaries, positivity limiter on or off, no artificial viscosity,
- * low-storage RK of order 3 or 4 (the same order for the whole ensemble).
+ * walls / inner-outer / ocean boundaries, positivity limiter on or off, artificial viscosity on or
+ * off (the modal sensor, or for N = 7 the learned indicator, esdg_ensemble_set_indicator),
+ * low-storage RK of order 3 or 4 (the same order for the whole ensemble).  A scenario with
+ * artificial viscosity equals a single esdg_solver with fused = 1 bit for bit (the fused schedule
+ * is the one that runs artificial viscosity; its results equal the reference kernel sequence's).
  */
 #ifndef ESDG_ENSEMBLE_H
 #define ESDG_ENSEMBLE_H
 
 #include "esdg.h"
+#include "esdg_mlp.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -59,6 +63,14 @@ int esdg_ensemble_get_q(esdg_ensemble* ens, int scn, double* host_q);   /* block
 int esdg_ensemble_status(esdg_ensemble* ens, int scn, double* t, int* steps_taken, double* last_dt);
 /* Device bytes held by the ensemble (state, shared geometry, tables). */
 int esdg_ensemble_bytes(const esdg_ensemble* ens, size_t* bytes);
+/* The learned indicator (esdg_solver_set_indicator, esdg_mlp.h) for every scenario with artificial
+ * viscosity and N = 7; the others keep the modal sensor.  m = null restores the sensor everywhere.
+ * m must outlive its use by the ensemble. */
+int esdg_ensemble_set_indicator(esdg_ensemble* ens, esdg_mlp* m, float lo, float hi);
+int esdg_ensemble_indicator_audit(esdg_ensemble* ens, int audit);
+/* Counts since the last call, summed over the learned scenarios, as esdg_solver_indicator_counts. */
+int esdg_ensemble_indicator_counts(esdg_ensemble* ens, int64_t* viscous, int64_t* false_negatives,
+                                   int64_t* sensor_positives, int64_t* not_smooth, int64_t* element_stages);
 int esdg_ensemble_destroy(esdg_ensemble* ens);
 
 #ifdef __cplusplus
