@@ -189,6 +189,134 @@ def ensemble_bench(dev, args):
     }
 
 
+def ensemble_av_bench(dev, args):
+    """The seeded ensemble with artificial viscosity: batched with the learned indicator on the N=7
+    members (the modal sensor on N=4, 5, where the network is not defined), against per-scenario
+    loops of the plain method (modal sensor everywhere) on the fused (1), single-kernel (2) and
+    reference (0) schedules, and against the batched plain method.  Every member runs W + K steps
+    from its initial state (only the K steps timed); the L2 change of each member's final state
+    against the plain loop's is reported (max over members)."""
+    import numpy as np
+    from esdgsem_mpiocca_amd import indicator, scenarios
+    wparams, meta = indicator.load_weights()
+    if wparams is None:
+        raise SystemExit("no trained indicator weights (esdgsem_mpiocca_amd/indicator_weights.json)")
+    # eps0 in m^2/s: the viscous step limit (DFL 0.05) stays above the CFL step on every mesh
+    rp = scenarios.ocean_run_params(artificial_viscosity=1, eps0=1000.0, sigma_min=-4.5, sigma_max=-2.5, DFL=0.05)
+    members = scenarios.build(scenarios.draw(args.ens_n, seed=0), rp)
+    n = len(members)
+    K, W = args.ens_av_steps, args.ens_warmup
+    case0, rp0, _ = members[0]
+    out = {}
+    finals = {}
+
+    def batched(learned):
+        ctx = dev.Context(dev.make_params(case0, rp0), device=0)
+        try:
+            ens = dev.Ensemble(ctx, members)
+            net = None
+            if learned:
+                net = dev.MLP(ctx, params=wparams)
+                ens.set_indicator(net, args.ind_lo, args.ind_hi)
+            tuned = ens.autotune(trial_steps=2)[0] if args.ens_autotune else None
+            ens.run(W)
+            ctx.finish()
+            t0 = time.perf_counter()
+            ens.run_async(K)
+            ctx.finish()
+            wall = time.perf_counter() - t0
+            if any(ens.status(i)[1] != W + K for i in range(n)):
+                raise SystemExit("ensemble (AV): a scenario stopped early")
+            counts = ens.indicator_counts() if learned else None
+            finals["learned" if learned else "plain_batched"] = [ens.get_q(i) for i in range(n)]
+            # per-kernel launch times of one more step, eager
+            ctx.profile(True)
+            ctx.profile_reset()
+            ens.run(1)
+            kern = {}
+            for kn in ("stage", "ens_sensor", "ens_indicator", "ens_grad", "ens_viscous_face", "stage_dt", "stage_mark"):
+                launches, ms = ctx.profile_query(kn)
+                if launches:
+                    kern[kn] = {"launches": launches, "ms_per_step": round(ms, 3)}
+            ctx.profile(False)
+            ens.close()
+            if net:
+                net.close()
+        finally:
+            ctx.close()
+        r = {"scenario_steps_per_s": n * K / wall, "ms_per_step": 1e3 * wall / K, "schedule": tuned,
+             "kernels_one_step": kern}
+        if counts:
+            es = max(counts["element_stages"], 1)
+            r["indicator_counts"] = counts
+            r["viscous_element_stage_fraction"] = counts["viscous"] / es
+            r["sensor_evaluated_fraction"] = counts["not_smooth"] / es
+        return r
+
+    out["batched_learned"] = batched(True)
+    print(f"# ensemble AV, learned: {out['batched_learned']['scenario_steps_per_s']:.1f} scenario-steps/s",
+          file=sys.stderr, flush=True)
+    out["batched_plain"] = batched(False)
+    print(f"# ensemble AV, plain: {out['batched_plain']['scenario_steps_per_s']:.1f} scenario-steps/s",
+          file=sys.stderr, flush=True)
+    loops = {}
+    for fused, name in ((1, "fused"), (2, "single_kernel"), (0, "reference_sequence")):
+        solvers, tsum, qs = {}, 0.0, []
+        try:
+            for case, rpm, q in members:
+                key = id(case)
+                if key not in solvers:
+                    c = dev.Context(dev.make_params(case, rpm), device=0)
+                    solvers[key] = (c, dev.Solver(c, case, rpm, fused=fused))
+                c, s = solvers[key]
+                s.set_q(q)
+                s.run(W)
+                c.finish()
+                t1 = time.perf_counter()
+                got, _ = s.run(K)
+                tsum += time.perf_counter() - t1
+                if got != K:
+                    raise SystemExit("ensemble AV loop: a scenario stopped early")
+                if fused == 1:
+                    qs.append(s.get_q())
+        finally:
+            for c, s in solvers.values():
+                s.close()
+                c.close()
+        loops[name] = n * K / tsum
+        if fused == 1:
+            finals["plain_loop"] = qs
+        print(f"# ensemble AV, per-scenario loop ({name}): {loops[name]:.1f} scenario-steps/s",
+              file=sys.stderr, flush=True)
+
+    def worst_change(a_list, b_list):
+        worst = [0.0, 0.0, 0.0]
+        for (case, _, _), a, b in zip(members, a_list, b_list):
+            A, B = a.reshape(case.Nelem, 3, -1), b.reshape(case.Nelem, 3, -1)
+            for v in range(3):
+                nb = np.linalg.norm(B[:, v])
+                d = np.linalg.norm(A[:, v] - B[:, v])
+                worst[v] = max(worst[v], float(d / nb) if nb > 0 else float(d))
+        return worst
+
+    out["per_scenario_loop_plain"] = loops
+    lv = out["batched_learned"]["scenario_steps_per_s"]
+    out["speedup_learned_batched_vs_plain_loop"] = {k: lv / v for k, v in loops.items()}
+    out["l2_change_learned_vs_plain_loop_max"] = worst_change(finals["learned"], finals["plain_loop"])
+    out["plain_batched_equals_plain_loop"] = all(np.array_equal(a, b) for a, b in
+                                                 zip(finals["plain_batched"], finals["plain_loop"]))
+    out["finite"] = bool(all(np.isfinite(a).all() for a in finals["learned"]))
+    out["metric"] = "ensemble scenario-steps/s (AV; learned indicator on N=7 members)"
+    out["value"] = lv
+    out["unit"] = "scenario-steps/s"
+    out["steps"], out["warmup"] = K, W
+    out["indicator"] = {"lo": args.ind_lo, "hi": args.ind_hi}
+    out["config"] = {"workload": "the seeded ocean-uplift ensemble (256 scenarios, sizes {64..256}^2, N in {4,5,7}) "
+                                 "with artificial viscosity (eps0 1000 m^2/s, sigma -4.5..-2.5, DFL 0.05): learned "
+                                 "indicator on the N=7 members, modal sensor on the rest", "seed": 0}
+    return out
+
+
 def pos_pres_bytes(nelem, ngl, sensor=True, stages=5):
     """Average algorithmic bytes per launch of the fused schedule's limiter pass (PosPres.okl plus
     the side traces, sensor and wave speed of the limited state): per node q 3, 1/J and (for the
@@ -394,7 +522,7 @@ def main():
 
     ap.add_argument("--ind-lo", type=float, default=0.05)
     ap.add_argument("--ind-hi", type=float, default=0.5)
-    ap.add_argument("--workload", choices=["c1", "ensemble", "c4"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "ensemble", "ensemble_av", "c4"], default="c1",
                     help="c1: the curved dam break (headline line, plus the ensemble unless --ensemble 0); "
                          "ensemble: the ensemble only")
     ap.add_argument("--ensemble", type=int, default=1, help="also run the ensemble benchmark")
@@ -402,6 +530,9 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--ens-warmup", type=int, default=2)
     ap.add_argument("--ens-autotune", type=int, default=1)
+    ap.add_argument("--ens-av", type=int, default=1,
+                    help="also run the ensemble with AV: learned indicator batched vs the plain per-scenario loops")
+    ap.add_argument("--ens-av-steps", type=int, default=10)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -414,8 +545,13 @@ def main():
     import esdgsem_mpiocca_amd as E
     dev = E.device()
 
-    if args.workload in ("ensemble", "c4"):
-        out = ensemble_bench(dev, args) if args.workload == "ensemble" else c4_bench(dev, args, args.c4_steps or 1000)
+    if args.workload in ("ensemble", "ensemble_av", "c4"):
+        if args.workload == "ensemble":
+            out = ensemble_bench(dev, args)
+        elif args.workload == "ensemble_av":
+            out = ensemble_av_bench(dev, args)
+        else:
+            out = c4_bench(dev, args, args.c4_steps or 1000)
         out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                     "dtype": "f64", "data": "synthetic"})
         print(json.dumps(out), flush=True)
@@ -526,6 +662,11 @@ def main():
             out["ensemble"] = ensemble_bench(dev, args)
         except Exception as e:  # reported beside the headline figure; must not hide it
             out["ensemble"] = {"error": str(e)}
+    if args.ens_av:
+        try:
+            out["ensemble_av"] = ensemble_av_bench(dev, args)
+        except (Exception, SystemExit) as e:  # reported beside the headline figure; must not hide it
+            out["ensemble_av"] = {"error": str(e)}
     if args.c2:
         try:
             out["c2"] = c2_bench(dev, args, args.c2_steps)

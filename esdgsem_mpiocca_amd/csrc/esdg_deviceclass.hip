@@ -67,6 +67,16 @@ class deviceclass {
         if (rp.fused == 2) {
             // single-kernel stage: second state buffer (ping-pong), device clock, lambda_max cell
             A(&o_q2, nodes * 3); A(&o_clock, 8); A(reinterpret_cast<double**>(&o_lmax), 1);
+            if (rp.artificial_viscosity) {
+                // the stage's gradient side traces, marks and viscosity maximum (the viscous volume
+                // term goes to qGradientX, the viscous face fluxes to SurfacePartsVisc)
+                A(&o_gtr, (size_t)Nelem * 16 * ngl);
+                A(reinterpret_cast<double**>(&o_active), ((size_t)Nelem + 1) / 2);
+                A(reinterpret_cast<double**>(&o_emax), 1);
+                if (rc == ESDG_OK) rc = esdg_fill(ctx_, reinterpret_cast<double*>(o_active), 0.0, ((size_t)Nelem + 1) / 2);
+                if (rc == ESDG_OK) rc = esdg_fill(ctx_, reinterpret_cast<double*>(o_emax), 0.0, 1);
+                if (rc == ESDG_OK) rc = esdg_fill(ctx_, o_ViscPara, 0.0, Nelem);
+            }
         }
         if (rp.fused == 1) A(&o_clock, 8);   // the fused schedule's device clock
         return rc;
@@ -110,8 +120,6 @@ class deviceclass {
             return set_error(ctx_, ESDG_ERR_ARG, "time-dependent boundaries run on the reference kernel sequence");
         if (rp_.fused && (rp_.rk_ssp || !rp_.flux_differencing))
             return set_error(ctx_, ESDG_ERR_ARG, "the fused stage supports flux differencing with low-storage RK only");
-        if (rp_.fused == 2 && rp_.artificial_viscosity)
-            return set_error(ctx_, ESDG_ERR_ARG, "the single-kernel stage runs without artificial viscosity");
         if (rp_.fused == 2 && (rp_.arrival_times || rp_.maximum_elevation))
             return set_error(ctx_, ESDG_ERR_ARG, "the analysis buffers are updated by the fused 0 and 1 schedules");
         if (rp_.fused == 2 && (ngl < 3 || ngl > 8))
@@ -370,9 +378,29 @@ class deviceclass {
             S.EleSizes = o_EleSizes; S.GLw = o_GLw;
             S.EdgeData = o_EdgeData; S.E2E = o_ElemToEdge; S.EMS = o_ElemEdgeMasterSlave; S.EOR = o_ElemEdgeOrientation;
             S.q[0] = o_q; S.q[1] = o_q2; S.gRK = o_gRK; S.clock = o_clock; S.lmax = o_lmax;
+            S.emax = o_emax;
+            if (rp_.artificial_viscosity) {
+                S.av = 1;
+                S.dvc = rp_.DFL / (std::pow(ngl, 2)) * std::pow(rp_.min_ele_size, 2);
+                S.sc = SensorConsts{ctx_->p.ZeroTOL, ctx_->p.eps0, ctx_->p.sigmaMin, ctx_->p.sigmaMax};
+                S.B = o_B; S.VdmInv = o_VdmInv; S.Dhat = o_Dhat; S.Dstrong = o_Dstrong;
+                S.visc = o_ViscPara; S.gtr = o_gtr; S.Qv = o_qGradientX; S.SPV = o_SurfacePartsVisc;
+                S.active = o_active;
+            }
             rc = stage_.init(ctx_, std::vector<StageScn>{S});
             if (rc) return rc;
             stage_ready = true;
+            stage_ind = nullptr;
+        }
+        // the learned indicator as last set on the solver
+        if (rp_.artificial_viscosity && (stage_ind != ind_ || stage_lo != ind_lo || stage_hi != ind_hi)) {
+            rc = stage_.set_indicator(ctx_, ind_, ind_lo, ind_hi);
+            if (rc) return rc;
+            stage_ind = ind_; stage_lo = ind_lo; stage_hi = ind_hi;
+        }
+        if (stage_.ind_audit != ind_audit) {
+            stage_.ind_audit = ind_audit;
+            stage_.drop_graphs();
         }
         // the state is in o_q at every entry; the clock restarts from the host's t
         double ck[8] = {0.0, t, 0.0, last_dt, 0.0, 0.0, 0.0, 0.0};
@@ -390,6 +418,9 @@ class deviceclass {
         if (!rc && src == 1)
             rc = hip_check(ctx_, hipMemcpyAsync(o_q, o_q2, (size_t)Nelem * ngl2 * 3 * sizeof(double), hipMemcpyDeviceToDevice,
                                                 ctx_->stream), "state copy");
+        // ViscPara / ViscForPlot of the final state, as the fused schedule leaves them
+        if (!rc && rp_.artificial_viscosity && !ind_)
+            rc = esdg_shock_capturing(ctx_, Nelem, o_q, o_B, o_VdmInv, o_EleSizes, o_ViscPara, o_ViscForPlot);
         if (!rc) rc = esdg_finish(ctx_);
         if (rc) return rc;
         t = ck[1];
@@ -410,10 +441,12 @@ class deviceclass {
             if (p) esdg_free(ctx_, p);
         fused_.release(ctx_);
         stage_.release(ctx_);
-        for (void* p : {(void*)o_q2, (void*)o_clock, (void*)o_lmax, (void*)o_estat, (void*)o_icount})
+        for (void* p : {(void*)o_q2, (void*)o_clock, (void*)o_lmax, (void*)o_estat, (void*)o_icount, (void*)o_gtr,
+                        (void*)o_active, (void*)o_emax})
             if (p) esdg_free(ctx_, p);
-        o_q2 = o_clock = nullptr;
-        o_lmax = o_icount = nullptr;
+        o_q2 = o_clock = o_gtr = nullptr;
+        o_lmax = o_icount = o_emax = nullptr;
+        o_active = nullptr;
         o_estat = nullptr;
         o_q = o_Qt = o_gRK = o_Qtmp = nullptr;
         return ESDG_OK;
@@ -472,6 +505,12 @@ class deviceclass {
     unsigned long long* o_icount = nullptr;
     double *o_q2 = nullptr, *o_clock = nullptr;
     unsigned long long* o_lmax = nullptr;
+    // single-kernel stage with artificial viscosity
+    double* o_gtr = nullptr;
+    int* o_active = nullptr;
+    unsigned long long* o_emax = nullptr;
+    const esdg_mlp* stage_ind = nullptr;     // the indicator the stage batch was last given
+    float stage_lo = 0.0f, stage_hi = 0.0f;
     double *o_q = nullptr, *o_Qt = nullptr, *o_gRK = nullptr, *o_Qtmp = nullptr;
     double *o_D = nullptr, *o_Dstrong = nullptr, *o_Dhat = nullptr, *o_VdmInv = nullptr, *o_GLw = nullptr;
     double *o_Jac = nullptr, *o_Yxi = nullptr, *o_Yeta = nullptr, *o_Xxi = nullptr, *o_Xeta = nullptr;
@@ -628,8 +667,8 @@ int esdg_solver_set_indicator(esdg_solver* s, esdg_mlp* m, float lo, float hi) {
         d.ind_ = nullptr;
         return ESDG_OK;
     }
-    if (d.rp_.fused != 1 || !d.rp_.artificial_viscosity)
-        return set_error(d.ctx_, ESDG_ERR_ARG, "the learned indicator needs the fused schedule with artificial viscosity");
+    if (d.rp_.fused == 0 || !d.rp_.artificial_viscosity)
+        return set_error(d.ctx_, ESDG_ERR_ARG, "the learned indicator needs a fused schedule (1 or 2) with artificial viscosity");
     if (d.ngl != 8) return set_error(d.ctx_, ESDG_ERR_ARG, "the learned indicator is defined for N = 7");
     if (!(lo <= hi)) return set_error(d.ctx_, ESDG_ERR_ARG, "esdg_solver_set_indicator: lo > hi");
     int rc = ESDG_OK;
@@ -653,9 +692,10 @@ int esdg_solver_indicator_counts(esdg_solver* s, int64_t* viscous, int64_t* fals
     if (!s) return set_error(nullptr, ESDG_ERR_ARG, "esdg_solver_indicator_counts: null solver");
     deviceclass& d = s->dc;
     unsigned long long c[5] = {0, 0, 0, 0, 0};
-    if (d.o_icount) {
-        int rc = esdg_memcpy(d.ctx_, c, d.o_icount, sizeof c, ESDG_D2H);
-        if (!rc) rc = esdg_fill(d.ctx_, reinterpret_cast<double*>(d.o_icount), 0.0, 5);
+    unsigned long long* ic = d.rp_.fused == 2 ? d.stage_.d_icount : d.o_icount;
+    if (ic) {
+        int rc = esdg_memcpy(d.ctx_, c, ic, sizeof c, ESDG_D2H);
+        if (!rc) rc = esdg_fill(d.ctx_, reinterpret_cast<double*>(ic), 0.0, 5);
         if (!rc) rc = esdg_finish(d.ctx_);
         if (rc) return rc;
     }

@@ -100,7 +100,7 @@ class StageBatch {
     int ncu = 0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};   // graph of two steps, one per starting parity
     // artificial viscosity: element tiles of the AV scenarios for the sensor (row tiles), the
-    // learned indicator (32-element tiles of learned scenarios), the gradient pass (node tiles) and
+    // learned indicator (32-element tiles of learned scenarios), the gradient pass (256-element tiles) and
     // the viscous face pass; the step's mark counter lives on the device (graph replays advance it)
     bool has_av = false;
     StageTask *d_sens = nullptr, *d_ind = nullptr, *d_grad = nullptr, *d_vface = nullptr;

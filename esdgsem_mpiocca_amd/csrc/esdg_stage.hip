@@ -451,6 +451,9 @@ __global__ void k_stage_dt(const StageScn* __restrict__ scns, int nscn) {
 // traces read from the state instead of a side-trace buffer: the same bits.
 // ---------------------------------------------------------------------------------------------
 
+// Elements per task of the gradient pass (one flag per thread decides whether a tile has work)
+constexpr int GRAD_TILE = 256;
+
 // The marks of a step: stage s of the step marks with *mark + s.  Advanced by 8 at each step on
 // the device, so that replays of a captured step take fresh marks.
 __global__ void k_stage_mark(int* mark) { *mark += 8; }
@@ -516,22 +519,29 @@ __device__ __forceinline__ void ens_grad_body(const StageScn& S, int e0, int src
                                               double* sA, double* sB, double* sC, double* sE, double* sEps) {
     using EB = ElemBlock<NGL>;
     constexpr int NN = NGL * NGL;
-    const int el = threadIdx.x / NN;
-    const int e = e0 + el;
-    const bool act = el < EB::EPB && e < S.Nelem && S.active[e] == m;
-    if (!__syncthreads_or(act)) return;   // no marked element in the tile (uniform per block)
+    const int e1 = min(e0 + GRAD_TILE, S.Nelem);
+    // most tiles hold no marked element: one flag per thread, one barrier, and the block is done
+    const int et = e0 + (int)threadIdx.x;
+    if (!__syncthreads_or(et < e1 && S.active[et] == m)) return;
     if (threadIdx.x < NN) {
         sDh[threadIdx.x] = S.Dhat[threadIdx.x];
         sDs[threadIdx.x] = S.Dstrong[threadIdx.x];
     }
-    __syncthreads();
     const AvMesh M{S.q[src], S.Jac, S.Yxi, S.Yeta, S.Xxi, S.Xeta, S.E2E, S.EMS, S.EOR, S.EdgeData,
                    S.nx, S.ny, S.scal, S.bL, S.bR, GatherConsts{S.bc, S.h_zero}, S.geomFace, S.tol, S.gtr, S.Qv};
-    av_element_block<NGL>(M, TraceFromState{S.q[src]}, e, act, act ? S.visc[e] : 0.0, sDh, sDs, sA, sB, sC, sE, sEps);
+    const int el = threadIdx.x / NN;
+    for (int g = e0; g < e1; g += EB::EPB) {
+        const int e = g + el;
+        const bool act = el < EB::EPB && e < e1 && S.active[e] == m;
+        // the first barrier also publishes sDh, sDs; a group without a marked element is skipped
+        if (!__syncthreads_or(act)) continue;
+        av_element_block<NGL>(M, TraceFromState{S.q[src]}, e, act, act ? S.visc[e] : 0.0, sDh, sDs, sA, sB, sC, sE,
+                              sEps);
+    }
 }
 
-// The gradient pass over ElemBlock tiles (tasks: {scenario, first element, ngl}): the marked
-// elements of each tile, one thread per node.
+// The gradient pass over tiles of GRAD_TILE elements (tasks: {scenario, first element, ngl}): the
+// marked elements of each tile, in groups of ElemBlock<ngl>::EPB elements, one thread per node.
 template <int LO, int HI>
 __global__ void __launch_bounds__(256) k_ens_grad(const StageScn* __restrict__ scns,
                                                   const StageTask* __restrict__ tasks, int src,
@@ -553,9 +563,9 @@ __global__ void __launch_bounds__(256) k_ens_grad(const StageScn* __restrict__ s
     }
 }
 
-// The viscous face fluxes over tiles of 256 face nodes (tasks: {scenario, first face node, ngl}):
-// a face node is evaluated when either element of its face is marked; the faces a marked element
-// reads are exactly those.
+// The viscous face fluxes over tiles of 256 faces (tasks: {scenario, first face, ngl}), one thread
+// per face: a face is evaluated (all its nodes) when either of its elements is marked; the faces a
+// marked element reads are exactly those.
 __global__ void __launch_bounds__(256) k_ens_vface(const StageScn* __restrict__ scns,
                                                    const StageTask* __restrict__ tasks,
                                                    const int* __restrict__ markp, int stage) {
@@ -563,14 +573,15 @@ __global__ void __launch_bounds__(256) k_ens_vface(const StageScn* __restrict__ 
     const StageScn& S = scns[tk.scn];
     const int m = *markp + stage;
     const int ngl = tk.ngl;
-    const int64_t it = (int64_t)tk.e0 + threadIdx.x;
-    if (it >= (int64_t)S.Nfaces * ngl) return;
-    const int f = (int)(it / ngl), i = (int)(it % ngl);
+    const int f = tk.e0 + (int)threadIdx.x;
+    if (f >= S.Nfaces) return;
     const int* E = S.EdgeData + 8 * (int64_t)f;
     const int iel = E[0], ier = E[1], cpuL = E[5], cpuR = E[6];
     if (!((cpuL == 0 && S.active[iel] == m) || (cpuR == 0 && S.active[ier] == m))) return;
-    viscous_face_node(f, i, ngl, S.geomFace, iel, ier, E[2], E[3], E[4], cpuL, cpuR, S.gtr, S.visc, S.nx, S.ny,
-                      S.scal, S.SPV);
+    const int isl = E[2], isr = E[3], rv = E[4];
+    for (int i = 0; i < ngl; ++i)
+        viscous_face_node(f, i, ngl, S.geomFace, iel, ier, isl, isr, rv, cpuL, cpuR, S.gtr, S.visc, S.nx, S.ny, S.scal,
+                          S.SPV);
 }
 
 // Loads each degree's D into c_stage_D once per device; a later batch must bring the same D.
@@ -626,8 +637,8 @@ int StageBatch::init(esdg_ctx* ctx, const std::vector<StageScn>& scns) {
 }
 
 // Element and face tiles of the AV scenarios for the sensor (RowBlock tiles), the learned
-// indicator (32-element tiles), the gradient pass (ElemBlock tiles) and the viscous face pass
-// (256 face nodes).
+// indicator (32-element tiles), the gradient pass (GRAD_TILE elements) and the viscous face pass
+// (256 faces).
 int StageBatch::build_av_tasks(esdg_ctx* ctx) {
     std::vector<StageTask> sens, ind, grad, vface;
     for (int s = 0; s < (int)host.size(); ++s) {
@@ -638,9 +649,8 @@ int StageBatch::build_av_tasks(esdg_ctx* ctx) {
             for (int e0 = 0; e0 < S.Nelem; e0 += 32) ind.push_back(StageTask{s, e0, ngl, 0});
         else
             for (int e0 = 0; e0 < S.Nelem; e0 += 256 / ngl) sens.push_back(StageTask{s, e0, ngl, 0});
-        const int epb = std::max(256 / (ngl * ngl), 1);
-        for (int e0 = 0; e0 < S.Nelem; e0 += epb) grad.push_back(StageTask{s, e0, ngl, 0});
-        for (int64_t k = 0; k < (int64_t)S.Nfaces * ngl; k += 256) vface.push_back(StageTask{s, (int)k, ngl, 0});
+        for (int e0 = 0; e0 < S.Nelem; e0 += GRAD_TILE) grad.push_back(StageTask{s, e0, ngl, 0});
+        for (int f0 = 0; f0 < S.Nfaces; f0 += 256) vface.push_back(StageTask{s, f0, ngl, 0});
     }
     auto put = [&](const std::vector<StageTask>& v, StageTask*& d, int& n) -> int {
         if (d) esdg_free(ctx, d);

@@ -236,7 +236,8 @@ typedef struct esdg_run_params {
     int rk_order, rk_ssp;         /* rkorder (3 or 4), rkSSP */
     int flux_differencing;        /* FluxDifferencing */
     int fused;                    /* 0: the reference's kernel sequence; 1: fused stage kernels;
-                                     2: single-kernel stage (no artificial viscosity, 2 <= N <= 7) */
+                                     2: single-kernel stage (2 <= N <= 7; with artificial viscosity
+                                     the sensor, gradient and viscous face passes run before it) */
     int arrival_times;            /* calcArrivalTimes: buffer "ArrivalTimings" (fused 0 or 1) */
     int maximum_elevation;        /* calcMaximumElevation: buffer "MaximumElevation" (fused 0 or 1) */
 } esdg_run_params;

@@ -60,7 +60,7 @@ int esdg_mlp_loss_grad(esdg_mlp* m, int n, const float* feat, const float* label
 int esdg_mlp_train_step(esdg_mlp* m, int n, const float* feat, const float* label, const int* idx,
                         float pos_weight, float lr, float* loss);
 
-/* Learned indicator in a solver's artificial-viscosity path (fused schedule, N = 7): at every
+/* Learned indicator in a solver's artificial-viscosity path (fused schedules 1 and 2, N = 7): at every
  * stage the network classifies the elements; p >= hi: troubled, p <= lo: smooth, in between
  * unsure.  Smooth elements get no viscosity; troubled and unsure ones the modal sensor's value,
  * which the indicator kernel evaluates from the modal coefficients its features already formed.

@@ -29,10 +29,12 @@ def av(**kw):
     return RunParams(**{**base, **kw})
 
 
-def alone(dev, case, rp, q, steps, net_seed=None, lo=0.5, hi=0.5):
+def alone(dev, case, rp, q, steps, net_seed=None, lo=0.5, hi=0.5, fused=None, plot=False):
     ctx = dev.Context(dev.make_params(case, rp))
+    if fused is None:
+        fused = 1 if rp.artificial_viscosity else 2
     try:
-        s = dev.Solver(ctx, case, rp, fused=1 if rp.artificial_viscosity else 2)
+        s = dev.Solver(ctx, case, rp, fused=fused)
         net = None
         if net_seed is not None:
             net = dev.MLP(ctx, seed=net_seed)
@@ -43,6 +45,8 @@ def alone(dev, case, rp, q, steps, net_seed=None, lo=0.5, hi=0.5):
         n, t = s.run(steps)
         out = s.get_q()
         counts = s.indicator_counts() if net else None
+        if plot:
+            counts = s.view("ViscForPlot", case.Nelem).download()
         s.close()
         if net:
             net.close()
@@ -176,3 +180,25 @@ def test_learned_autotune_keeps_counts(dev, mem):
     for i in range(len(mem)):
         assert res[i][1:] == ref[i][1:], i
         assert np.array_equal(res[i][0], ref[i][0]), i
+
+
+@pytest.mark.parametrize("learned", [False, True], ids=["sensor", "learned"])
+def test_single_kernel_schedule_with_av_equals_fused(dev, mem, learned):
+    """The solver's single-kernel schedule (fused = 2, a batch of one) with artificial viscosity
+    equals the fused schedule bit for bit, with the learned indicator and its counts too; without
+    the indicator it leaves the final state's viscosity in ViscForPlot, as the fused schedule does."""
+    for i in (0, 2, 4, 5):
+        case, rp, q = mem[i]
+        seed = 3 if learned and case.ngl == 8 else None
+        if learned and seed is None:
+            continue
+        qa, na, ta, ca = alone(dev, case, rp, q, STEPS, net_seed=seed, lo=0.3, hi=0.7, fused=1)
+        qb, nb, tb, cb = alone(dev, case, rp, q, STEPS, net_seed=seed, lo=0.3, hi=0.7, fused=2)
+        assert (na, ta) == (nb, tb), i
+        assert np.array_equal(qa, qb), i
+        if learned:
+            assert ca == cb, i
+        else:
+            pa = alone(dev, case, rp, q, STEPS, fused=1, plot=True)[3]
+            pb = alone(dev, case, rp, q, STEPS, fused=2, plot=True)[3]
+            assert np.array_equal(pa, pb), i
