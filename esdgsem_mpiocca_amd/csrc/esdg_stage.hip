@@ -57,7 +57,8 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
                                            const double* __restrict__ Xxi, const double* __restrict__ Jac,
                                            const double* __restrict__ Bx, const double* __restrict__ By,
                                            const double* __restrict__ EleSizes, const double* __restrict__ GLw,
-                                           unsigned long long* __restrict__ lmax, const StageScn* SV, int mark) {
+                                           unsigned long long* __restrict__ lmax, const StageScn* SV, int mark,
+                                           const double* D) {
     using C = VolCfg<NGL>;
     const int lane = threadIdx.x;
     double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], Y[NGL], X[NGL];
@@ -87,7 +88,7 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
             P[2 * C::PLANE + r * C::RS + c] = hv[c];
             P[3 * C::PLANE + r * C::RS + c] = inv;
         }
-        line_sweep<NGL, false>(c_stage_D[NGL], g, h, hu, hv, u, v, Y, X, a1, a2, a3);
+        line_sweep<NGL, false>(D, g, h, hu, hv, u, v, Y, X, a1, a2, a3);
     }
     __syncthreads();
 
@@ -124,7 +125,7 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
             b2[k] = P[1 * C::PLANE + k * C::RS + r];
             b3[k] = P[2 * C::PLANE + k * C::RS + r];
         }
-        line_sweep<NGL, true>(c_stage_D[NGL], g, h, hu, hv, u, v, Y, X, b1, b2, b3);
+        line_sweep<NGL, true>(D, g, h, hu, hv, u, v, Y, X, b1, b2, b3);
 #pragma unroll
         for (int k = 0; k < NGL; ++k) {
             unscale_sums(b1[k], b2[k], b3[k]);
@@ -192,8 +193,8 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
                 }
             }
         }
-        // low-storage RK: gRK = A gRK + Qt; q_new = q + (C dt) gRK (q re-read: an L2 hit, and it
-        // keeps hu, hv out of the registers live across the column sweep)
+        // low-storage RK: gRK = A gRK + Qt; q_new = q + (C dt) gRK, q at node (k, r) being this
+        // lane's column input (h, hu, hv): the same bits, without a second read of q
         const double cdt = rkC * dt;
         const int64_t ob = (int64_t)e * C::NN * 3 + r;
 #pragma unroll
@@ -205,9 +206,9 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
             gRK[id] = g1;
             gRK[id + C::NN] = g2;
             gRK[id + 2 * C::NN] = g3;
-            b1[k] = qi[id] + cdt * g1;
-            b2[k] = qi[id + C::NN] + cdt * g2;
-            b3[k] = qi[id + 2 * C::NN] + cdt * g3;
+            b1[k] = h[k] + cdt * g1;
+            b2[k] = hu[k] + cdt * g2;
+            b3[k] = hv[k] + cdt * g3;
         }
     }
     if (positivity) {
@@ -294,11 +295,13 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
 
 
 template <int NGL, bool AV>
-__device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* lds, int src, double rkA, double rkC,
-                                           bool last, int mark) {
+__device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* lds, double* sD, int src, double rkA,
+                                           double rkC, bool last, int mark) {
     using C = VolCfg<NGL>;
     using SC = StageCfg<NGL>;
     const int lane = threadIdx.x;
+    // D of this degree in LDS (wave-uniform reads, broadcast): keeps it out of the scalar registers
+    for (int t = lane; t < NGL * NGL; t += 64) sD[t] = c_stage_D[NGL][t];
     const int el = lane / NGL, r = lane % NGL;
     const int e = e0 + el;
     const bool act = (el < C::EPW) && (e < S.Nelem);
@@ -334,18 +337,19 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
         }
     }
 
+    __syncthreads();   // sD written
     stage_core<NGL, AV>(e, r, act, P, FS, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo, S.gRK,
-                        S.Yeta, S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax, &S, mark);
+                        S.Yeta, S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax, &S, mark, sD);
 }
 
 template <int LO, int HI, bool AV>
-__device__ __forceinline__ void stage_task(const StageScn* __restrict__ scns, const StageTask& tk, double* lds, int src,
-                                           double rkA, double rkC, bool last, int mark) {
+__device__ __forceinline__ void stage_task(const StageScn* __restrict__ scns, const StageTask& tk, double* lds,
+                                           double* sD, int src, double rkA, double rkC, bool last, int mark) {
     const StageScn& S = scns[tk.scn];
     switch (tk.ngl) {
 #define ESDG_CASE(n)                                                                \
     case n:                                                                         \
-        if constexpr (n >= LO && n <= HI) stage_body<n, AV>(S, tk.e0, lds, src, rkA, rkC, last, mark);        \
+        if constexpr (n >= LO && n <= HI) stage_body<n, AV>(S, tk.e0, lds, sD, src, rkA, rkC, last, mark);    \
         break;
         ESDG_CASE(3) ESDG_CASE(4) ESDG_CASE(5) ESDG_CASE(6) ESDG_CASE(7) ESDG_CASE(8)
 #undef ESDG_CASE
@@ -363,6 +367,7 @@ __global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ sc
                                                  int n, int xcd, int src, double rkA, double rkC, int last,
                                                  const int* __restrict__ markp, int stage) {
     __shared__ double lds[MaxLds<LO, HI>::value];
+    __shared__ double sD[HI * HI];
     const int mark = AV ? *markp + stage : 0;
     const int b = blockIdx.x;
     int beg = 0, cnt = n, i0 = b, stride = gridDim.x;
@@ -374,11 +379,11 @@ __global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ sc
         stride = gridDim.x >> 3;
     }
     if constexpr (!PERSIST) {
-        if (i0 < cnt) stage_task<LO, HI, AV>(scns, tasks[beg + i0], lds, src, rkA, rkC, last != 0, mark);
+        if (i0 < cnt) stage_task<LO, HI, AV>(scns, tasks[beg + i0], lds, sD, src, rkA, rkC, last != 0, mark);
     } else {
         for (int i = i0; i < cnt; i += stride) {
             if (i != i0) __syncthreads();   // LDS of the previous task is free
-            stage_task<LO, HI, AV>(scns, tasks[beg + i], lds, src, rkA, rkC, last != 0, mark);
+            stage_task<LO, HI, AV>(scns, tasks[beg + i], lds, sD, src, rkA, rkC, last != 0, mark);
         }
     }
 }

@@ -68,7 +68,7 @@ template <int NGL, int MODE>
 __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, const double* __restrict__ Jac,
                                                         const double* __restrict__ Yxi, const double* __restrict__ Yeta,
                                                         const double* __restrict__ Xxi, const double* __restrict__ Xeta,
-                                                        double* q, const double* __restrict__ D,
+                                                        double* q, const double* __restrict__ Dg,
                                                         const double* __restrict__ Bx, const double* __restrict__ By,
                                                         double* __restrict__ Qt, SurfIn si) {
     using C = VolCfg<NGL>;
@@ -81,7 +81,14 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         }
     }
     __shared__ double s[C::EPW * C::ESZ];
+    // D in LDS: its entries are read with wave-uniform addresses (broadcast), which keeps the
+    // 64-entry matrix out of the scalar registers (no SGPR spills into vector registers).  After
+    // the column sweep the same slots take the sensor's VdmInv (the block is one wave, whose LDS
+    // accesses execute in program order).
+    __shared__ double sM[C::NN];
     const int lane = threadIdx.x;
+    for (int t = lane; t < C::NN; t += 64) sM[t] = Dg[t];
+    const double* D = sM;
     const int el = lane / NGL, r = lane % NGL;
     int e = blockIdx.x * C::EPW + el;
     bool act = (el < C::EPW) && (e < Nelem);
@@ -127,8 +134,9 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             S[2 * C::PLANE + r * C::RS + c] = hv[c];
             S[3 * C::PLANE + r * C::RS + c] = inv;
         }
-        line_sweep<NGL, false>(D, g, h, hu, hv, u, v, Y, X, a1, a2, a3);
     }
+    __syncthreads();   // sD written
+    if (act) line_sweep<NGL, false>(D, g, h, hu, hv, u, v, Y, X, a1, a2, a3);
     __syncthreads();
 
     // ---- column phase: xi-column r, nodes (k, r) -------------------------------------------
@@ -253,9 +261,11 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 si.gRK[id] = g1;
                 si.gRK[id + C::NN] = g2;
                 si.gRK[id + 2 * C::NN] = g3;
-                const double n1 = q[id] + cdt * g1;
-                const double n2 = q[id + C::NN] + cdt * g2;
-                const double n3 = q[id + 2 * C::NN] + cdt * g3;
+                // the stage's state at node (k, r) is this lane's column input (h, hu, hv): the same
+                // bits as q there, without a second read of q from HBM
+                const double n1 = h[k] + cdt * g1;
+                const double n2 = hu[k] + cdt * g2;
+                const double n3 = hv[k] + cdt * g3;
                 q[id] = n1;
                 q[id + C::NN] = n2;
                 q[id + 2 * C::NN] = n3;
@@ -283,7 +293,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         }
     }
     if constexpr (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
-        __shared__ double sV[C::NN];
+        double* sV = sM;
         if (si.visc_next)
             for (int t = lane; t < C::NN; t += 64) sV[t] = si.VdmInv[t];
         // side traces of the new state, [e][side][v][k] (read by the next stage's face pass): the
