@@ -15,18 +15,18 @@
 
 namespace esdg {
 
-// Face states as CollectEdgeData stores them into qL / qR, read from the side-trace buffer
-// [e][side][v][k] or straight from the state.
+// Face states of face f as CollectEdgeData stores them into qL / qR, read from the face slots of
+// the side-trace buffer or straight from the state.
 struct TraceFromSides {
     const double* tr;
-    __device__ __forceinline__ FaceState operator()(int ngl, const int* E, bool left, int k, double nx, double ny,
-                                                    const GatherConsts& gc, double bl, double br) const {
-        return gathered_tr(tr, ngl, E, left, k, nx, ny, gc, bl, br);
+    __device__ __forceinline__ FaceState operator()(int ngl, int f, const int* E, bool left, int k, double nx,
+                                                    double ny, const GatherConsts& gc, double bl, double br) const {
+        return gathered_tr(tr, ngl, f, E, left, k, nx, ny, gc, bl, br);
     }
 };
 struct TraceFromState {
     const double* q;
-    __device__ __forceinline__ FaceState operator()(int ngl, const int* E, bool left, int k, double nx, double ny,
+    __device__ __forceinline__ FaceState operator()(int ngl, int, const int* E, bool left, int k, double nx, double ny,
                                                     const GatherConsts& gc, double bl, double br) const {
         return gathered(q, ngl, E, left, k, nx, ny, gc, bl, br);
     }
@@ -40,11 +40,11 @@ struct Br1Flux {
 };
 
 template <class Src>
-__device__ __forceinline__ Br1Flux br1_face_flux(const Src& src, int ngl, const int* E, int i, double nx, double ny,
-                                                 double sc, double bl, double br, const GatherConsts& gc, double tol,
-                                                 double geomFace) {
-    const FaceState L = src(ngl, E, true, i, nx, ny, gc, bl, br);
-    const FaceState Ri = src(ngl, E, false, i, nx, ny, gc, bl, br);
+__device__ __forceinline__ Br1Flux br1_face_flux(const Src& src, int ngl, int f, const int* E, int i, double nx,
+                                                 double ny, double sc, double bl, double br, const GatherConsts& gc,
+                                                 double tol, double geomFace) {
+    const FaceState L = src(ngl, f, E, true, i, nx, ny, gc, bl, br);
+    const FaceState Ri = src(ngl, f, E, false, i, nx, ny, gc, bl, br);
     const FaceState LL = (E[5] == -1) ? Ri : L;
     const FaceState RR = (E[6] == -1) ? L : Ri;
     const double iL = (LL.h > tol) ? 1.0 / LL.h : 0.0;
@@ -126,7 +126,7 @@ __device__ __forceinline__ void av_element_block(const AvMesh& M, const Src& src
             const int ms = M.EMS[e * 4 + side];
             const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
             const int64_t fn = (int64_t)ifa * NGL + inc;
-            const Br1Flux F = br1_face_flux(src, NGL, M.ED + 8 * (int64_t)ifa, inc, M.nx[fn], M.ny[fn], M.scal[fn],
+            const Br1Flux F = br1_face_flux(src, NGL, ifa, M.ED + 8 * (int64_t)ifa, inc, M.nx[fn], M.ny[fn], M.scal[fn],
                                             M.bL[fn], M.bR[fn], M.gc, M.tol, M.geomFace);
             if (ph == 0) gx0 += ms * F.x0 * rJ;
             else gx0 += ms * rJ * F.x0;

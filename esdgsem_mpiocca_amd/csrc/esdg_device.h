@@ -88,14 +88,15 @@ __device__ __forceinline__ FaceState gathered(const double* __restrict__ q, int 
     return left ? L : R;
 }
 
-// Element side traces of the state, [e][side][v][k] with k in the side's node order (side_node):
-// 12 ngl doubles per element, contiguous per side and field.  tr_put stores node n = j*ngl + i of
-// element e into every side it lies on.
-__device__ __forceinline__ void tr_put(double* __restrict__ tr, int ngl, int64_t e, int i, int j, double h, double hu,
-                                       double hv) {
-    double* T = tr + e * 12 * ngl;
+// Side traces of the state in face slots: the traces of the element side that is face f's left
+// (right) side fill slot 2 f (2 f + 1), tr[(slot * 3 + v) * ngl + k] with v = h, hu, hv and k the
+// side node in the element's side order (side_node), so that the face pass reads each face's two
+// traces as one contiguous run.  slot[e * 4 + side] names the slot of each element side
+// (trace_slots).  tr_put stores node n = j*ngl + i of element e into every side it lies on.
+__device__ __forceinline__ void tr_put(double* __restrict__ tr, const int* __restrict__ slot, int ngl, int64_t e,
+                                       int i, int j, double h, double hu, double hv) {
     auto put = [&](int side, int k) {
-        double* t = T + side * 3 * ngl + k;
+        double* t = tr + (int64_t)slot[e * 4 + side] * 3 * ngl + k;
         t[0] = h; t[ngl] = hu; t[2 * ngl] = hv;
     };
     if (j == 0) put(0, i);
@@ -104,17 +105,19 @@ __device__ __forceinline__ void tr_put(double* __restrict__ tr, int ngl, int64_t
     if (i == 0) put(3, j);
 }
 
-// gathered() with the interior traces read from the side-trace buffer instead of the state
-__device__ __forceinline__ FaceState gathered_tr(const double* __restrict__ tr, int ngl, const int* E, bool left, int k,
-                                                 double nx, double ny, const GatherConsts& gc, double bl, double br) {
-    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+// gathered() with the interior traces of face f read from its slots of the side-trace buffer
+// instead of the state
+__device__ __forceinline__ FaceState gathered_tr(const double* __restrict__ tr, int ngl, int f, const int* E, bool left,
+                                                 int k, double nx, double ny, const GatherConsts& gc, double bl,
+                                                 double br) {
+    const int cpuL = E[5], cpuR = E[6];
     FaceState L{0.0, 0.0, 0.0}, R{0.0, 0.0, 0.0};
     if (cpuL == 0) {
-        const double* t = tr + ((int64_t)iel * 4 + isl) * 3 * ngl + k;
+        const double* t = tr + (int64_t)f * 6 * ngl + k;
         L = FaceState{t[0], t[ngl], t[2 * ngl]};
     }
     if (cpuR == 0) {
-        const double* t = tr + ((int64_t)ier * 4 + isr) * 3 * ngl + k;
+        const double* t = tr + ((int64_t)f * 6 + 3) * ngl + k;
         R = FaceState{t[0], t[ngl], t[2 * ngl]};
     }
     if (gc.bc == ESDG_BC_INNER_OUTER) {

@@ -259,7 +259,7 @@ class deviceclass {
         }
         if (rp_.fused == 2) return stage_timeloop(max_steps, steps_taken);
         if (rp_.fused == 1) {
-            rc = fused_.begin(ctx_, o_q);
+            rc = fused_.begin(ctx_, o_q, o_ElemToEdge, o_EdgeData);
             if (rc) return rc;
             // without per-step host work (the analysis buffers take t from the host) dt and t stay
             // on the device and the host runs ahead

@@ -41,8 +41,9 @@ class FusedStage {
     // With AV: face -> AV element pass -> viscous face -> volume + surface + viscous + RK [-> limiter].
     int stage(esdg_ctx* ctx, int s, double rkA, double cdt, const FusedArgs& a);
     // side traces of q (the face pass reads them; each stage's update refreshes them): called
-    // whenever q was set from outside the stages
-    int begin(esdg_ctx* ctx, const double* q);
+    // whenever q was set from outside the stages.  E2E, ED: the mesh's ElemToEdge and EdgeData
+    // (the trace slots of the element sides are derived from them once)
+    int begin(esdg_ctx* ctx, const double* q, const int* E2E, const int* ED);
     void release(esdg_ctx* ctx);
     void* buffer(const std::string& name);
 
@@ -53,7 +54,9 @@ class FusedStage {
     double* grad = nullptr;     // side traces of the scaled velocity gradients [e][side][4][ngl] (AV)
     double* fvisc = nullptr;    // viscous face fluxes [f][2][ngl] (AV)
     double* qvisc = nullptr;    // viscous volume term [e][2][ngl^2] (hu, hv) (AV)
-    double* tr = nullptr;       // side traces of the state [e][side][3][ngl]
+    double* tr = nullptr;       // side traces of the state in face slots [f][L, R][3][ngl]
+    int* slot = nullptr;        // face slot (2 f + 0 / 1) of each element side [e][side]
+    bool slots_built = false;
     int* active = nullptr;      // == seq: the element has viscosity in reach this stage (AV)
     int* alist = nullptr;       // the elements marked this stage, each once, in marking order (AV)
     int* acount = nullptr;      // length of alist for stage seq: acount[seq & 1] (AV)

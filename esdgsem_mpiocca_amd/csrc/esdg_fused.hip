@@ -32,12 +32,13 @@
 namespace esdg {
 
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV, double* tr, const int* active = nullptr, int seq = 0,
-                 const int* list = nullptr, const int* list_count = nullptr);
+                 const double* Qv, const double* SPV, double* tr, const int* slot, const int* active = nullptr,
+                 int seq = 0, const int* list = nullptr, const int* list_count = nullptr);
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
-                    double* tr, double* next_visc, double* next_visc_plot, const double* B, const double* VdmInv,
-                    double* next_lambda, const double* dt_dev);
-int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr);
+                    double* tr, const int* slot, double* next_visc, double* next_visc_plot, const double* B,
+                    const double* VdmInv, double* next_lambda, const double* dt_dev);
+int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr, const int* slot);
+int trace_slots(esdg_ctx* ctx, int Nelem, const int* E2E, const int* ED, int* slot);
 
 // ---------------------------------------------------------------------------------------------
 // fused_face: CollectEdgeData (SolidWalls.okl:16-207) + calcNumFluxes (calcESFlux.okl /
@@ -65,8 +66,8 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
     const double nx = nxa[t], ny = nya[t], sc = scal[t];
     // qL at node i and qR at node ir, exactly as the gather kernel stores them (the element side
     // traces come from the side-trace buffer: contiguous per side, unlike the columns of q)
-    const FaceState L = gathered_tr(tr, ngl, E, true, i, nx, ny, gc, BL[t], BR[t]);
-    const FaceState R = gathered_tr(tr, ngl, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
+    const FaceState L = gathered_tr(tr, ngl, f, E, true, i, nx, ny, gc, BL[t], BR[t]);
+    const FaceState R = gathered_tr(tr, ngl, f, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
     double F0, F1, F2;
     numerical_flux(KIND, c, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
     const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
@@ -219,7 +220,9 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     const int ngl = ctx->p.ngl;
     const size_t fn = (size_t)Nf * ngl;
     int rc = esdg_malloc(ctx, (void**)&fstar, fn * 3 * sizeof(double));
-    if (!rc) rc = esdg_malloc(ctx, (void**)&tr, (size_t)Ne * 12 * ngl * sizeof(double));
+    if (!rc) rc = esdg_malloc(ctx, (void**)&tr, (size_t)Nf * 6 * ngl * sizeof(double));
+    if (!rc) rc = esdg_malloc(ctx, (void**)&slot, (size_t)Ne * 4 * sizeof(int));
+    slots_built = false;
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&active, ((size_t)Ne + 1) / 2 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&alist, ((size_t)Ne + 1) / 2 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&acount, 256);
@@ -230,14 +233,19 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     return rc;
 }
 
-int FusedStage::begin(esdg_ctx* ctx, const double* q) {
+int FusedStage::begin(esdg_ctx* ctx, const double* q, const int* E2E, const int* ED) {
     if (!ready) return set_error(ctx, ESDG_ERR_STATE, "fused stage used before init");
     // an interrupted stage may have left a list behind (both list lengths: 2 ints)
     if (acount) {
         int rc = esdg_fill(ctx, reinterpret_cast<double*>(acount), 0.0, 1);
         if (rc) return rc;
     }
-    return write_traces(ctx, Nelem, q, tr);
+    if (!slots_built) {
+        int rc = trace_slots(ctx, Nelem, E2E, ED, slot);
+        if (rc) return rc;
+        slots_built = true;
+    }
+    return write_traces(ctx, Nelem, q, tr, slot);
 }
 
 int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArgs& A) {
@@ -284,10 +292,10 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         // volume + surface + RK in one element pass; the limiter (if on) follows as its own pass
         // with the limiter on, the limiter writes the traces of the limited state
         double* vtr = A.positivity ? nullptr : tr;
-        int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr, vtr);
+        int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr, vtr, slot);
         if (!rc && A.positivity)
-            rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, A.next_visc, A.next_visc_plot, A.B,
-                                 A.VdmInv, A.next_lambda, A.dt_dev);
+            rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, slot, A.next_visc, A.next_visc_plot,
+                                 A.B, A.VdmInv, A.next_lambda, A.dt_dev);
         return rc;
     }
     int rc = ESDG_OK;
@@ -307,19 +315,20 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         if (rc) return rc;
     }
     // volume + surface + viscous terms + RK in one element pass, then the limiter
-    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, active, seq,
-                      alist, acount + (seq & 1));
+    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, slot, active,
+                      seq, alist, acount + (seq & 1));
     if (!rc && A.positivity)
-        rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, A.next_visc, A.next_visc_plot, A.B,
-                             A.VdmInv, A.next_lambda, A.dt_dev);
+        rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, slot, A.next_visc, A.next_visc_plot,
+                             A.B, A.VdmInv, A.next_lambda, A.dt_dev);
     return rc;
 }
 
 void FusedStage::release(esdg_ctx* ctx) {
     for (double** p : {&fstar, &grad, &fvisc, &qvisc, &tr})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
-    for (int** p : {&active, &alist, &acount})
+    for (int** p : {&active, &alist, &acount, &slot})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
+    slots_built = false;
     seq = 0;
     ready = false;
 }

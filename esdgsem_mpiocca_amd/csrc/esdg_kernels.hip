@@ -353,7 +353,7 @@ template <int NGL>
 __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double zero_tol,
                                                   const double* __restrict__ EleSizes, const double* __restrict__ GLw,
                                                   const double* __restrict__ Jac, double* __restrict__ q,
-                                                  double* __restrict__ tr, NextEval nx) {
+                                                  double* __restrict__ tr, const int* __restrict__ slot, NextEval nx) {
     if (nx.dtp && *nx.dtp == 0.0) return;   // a step issued after T (device clock): nothing changes
     using RB = RowBlock<NGL>;
     constexpr int NN = NGL * NGL;
@@ -437,16 +437,18 @@ __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double 
             }
         }
         __syncthreads();
-        constexpr int TPE = 12 * NGL;
-        // thread (el, r) writes side node r of every (side, v) plane of its element
+        // thread (el, r) writes side node r of every (side, v) plane of its element into the side's
+        // face slot
         if (act) {
-            double* T = tr + (int64_t)e * TPE + r;
 #pragma unroll
-            for (int m = 0; m < 12; ++m) {
-                const int side = m / 3, v = m % 3;
-                const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : r;
-                const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : r;
-                T[m * NGL] = sP[v][el * NN + row * NGL + col];
+            for (int side = 0; side < 4; ++side) {
+                double* T = tr + (int64_t)slot[e * 4 + side] * 3 * NGL + r;
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : r;
+                    const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : r;
+                    T[v * NGL] = sP[v][el * NN + row * NGL + col];
+                }
             }
         }
     }
@@ -518,9 +520,10 @@ __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double 
     }
 }
 
-// side traces of a state (the fused schedule's face pass reads them)
+// side traces of a state in the face slots (the fused schedule's face pass reads them)
 template <int NGL>
-__global__ void __launch_bounds__(256) k_write_traces(int Nelem, const double* __restrict__ q, double* __restrict__ tr) {
+__global__ void __launch_bounds__(256) k_write_traces(int Nelem, const double* __restrict__ q, double* __restrict__ tr,
+                                                      const int* __restrict__ slot) {
     constexpr int NN = NGL * NGL;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)Nelem * NN) return;
@@ -528,7 +531,19 @@ __global__ void __launch_bounds__(256) k_write_traces(int Nelem, const double* _
     const int n = (int)(t % NN);
     const int64_t id = e * NN * ESDG_NEQ + n;
     const int i = n % NGL, j = n / NGL;
-    if (i == 0 || j == 0 || i == NGL - 1 || j == NGL - 1) tr_put(tr, NGL, e, i, j, q[id], q[id + NN], q[id + 2 * NN]);
+    if (i == 0 || j == 0 || i == NGL - 1 || j == NGL - 1) tr_put(tr, slot, NGL, e, i, j, q[id], q[id + NN], q[id + 2 * NN]);
+}
+
+// The face slot of every element side: 2 f for the side that is face f's left side (EdgeData:
+// element E[0], side E[2], present), 2 f + 1 for its right side.
+__global__ void __launch_bounds__(256) k_trace_slots(int Nelem, const int* __restrict__ E2E, const int* __restrict__ ED,
+                                                     int* __restrict__ slot) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nelem * 4) return;
+    const int e = (int)(t / 4), side = (int)(t % 4);
+    const int f = E2E[t] - 1;
+    const int* E = ED + 8 * (int64_t)f;
+    slot[t] = 2 * f + ((E[5] == 0 && E[0] == e && E[2] == side) ? 0 : 1);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -751,21 +766,21 @@ void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, c
 }
 template <int NGL>
 void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q,
-                    double* tr, NextEval nx) {
+                    double* tr, const int* slot, NextEval nx) {
     using RB = RowBlock<NGL>;
     k_pos_pres<NGL><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.PosPresTOL, ctx->p.ZeroTOL,
-                                                                                  sizes, GLw, Jac, q, tr, nx);
+                                                                                  sizes, GLw, Jac, q, tr, slot, nx);
 }
 template <int NGL>
-void launch_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr) {
-    k_write_traces<NGL><<<blocks_for((int64_t)Nelem * NGL * NGL, 256), 256, 0, ctx->stream>>>(Nelem, q, tr);
+void launch_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr, const int* slot) {
+    k_write_traces<NGL><<<blocks_for((int64_t)Nelem * NGL * NGL, 256), 256, 0, ctx->stream>>>(Nelem, q, tr, slot);
 }
 
 // limiter that also refreshes the side traces (fused schedule) and, when asked, evaluates the
 // sensor and the wave speeds of the limited state
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
-                    double* tr, double* next_visc, double* next_visc_plot, const double* B, const double* VdmInv,
-                    double* next_lambda, const double* dt_dev) {
+                    double* tr, const int* slot, double* next_visc, double* next_visc_plot, const double* B,
+                    const double* VdmInv, double* next_lambda, const double* dt_dev) {
     NextEval nx{};
     nx.dtp = dt_dev;
     if (next_visc) {
@@ -778,13 +793,19 @@ int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const doub
     nx.lam = next_lambda;
     nx.g = ctx->p.g;
     KernelScope ks(ctx, "pos_pres");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, tr, nx);
+    if (tr && !slot) return set_error(ctx, ESDG_ERR_STATE, "pos_pres: side traces need the face slots");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, tr, slot, nx);
     return launch_check(ctx, "pos_pres");
 }
-int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr) {
+int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr, const int* slot) {
     KernelScope ks(ctx, "write_traces");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_traces, ctx, Nelem, q, tr);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_traces, ctx, Nelem, q, tr, slot);
     return launch_check(ctx, "write_traces");
+}
+int trace_slots(esdg_ctx* ctx, int Nelem, const int* E2E, const int* ED, int* slot) {
+    KernelScope ks(ctx, "trace_slots");
+    k_trace_slots<<<blocks_for((int64_t)Nelem * 4, 256), 256, 0, ctx->stream>>>(Nelem, E2E, ED, slot);
+    return launch_check(ctx, "trace_slots");
 }
 template <int NGL>
 void launch_gradient(esdg_ctx* ctx, int Nelem, const double* Jac, const double* Yxi, const double* Yeta, const double* Xxi,
@@ -958,7 +979,7 @@ int esdg_pos_pres(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double
     ESDG_REQUIRE(ctx, Nelem >= 0 && EleSizes && GLw && Jac && q, "esdg_pos_pres: bad argument");
     if (Nelem == 0) return ESDG_OK;
     ESDG_ENTRY("pos_pres");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, nullptr, NextEval{});
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, nullptr, nullptr, NextEval{});
     return launch_check(ctx, "pos_pres");
 }
 

@@ -41,7 +41,8 @@ struct SurfIn {
     const double* Qv;      // viscous volume term [e][2][ngl^2] (hu, hv)
     const double* visc;    // element viscosity
     const double* SPV;     // viscous face fluxes [f][2][ngl]
-    double* tr;            // side traces of the updated state (RK modes; may be null)
+    double* tr;            // side traces of the updated state, in face slots (RK modes; may be null)
+    const int* slot;       // face slot of each element side
     // Of the updated state, when tr is given (RK modes; each may be null):
     double* visc_next;     // ShockCapturing's element viscosity (and visc_plot, its copy)
     double* visc_plot;
@@ -296,21 +297,23 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         double* sV = sM;
         if (si.visc_next)
             for (int t = lane; t < C::NN; t += 64) sV[t] = si.VdmInv[t];
-        // side traces of the new state, [e][side][v][k] (read by the next stage's face pass): the
-        // block's elements own one contiguous run of the trace buffer, written lane-consecutively
+        // side traces of the new state, in the face slots of the element sides (read by the next
+        // stage's face pass)
         if (si.tr) {
             __syncthreads();
-            constexpr int TPE = 12 * NGL;
-            // lane (el, r) writes side node r of every (side, v) plane of its element: one unrolled
-            // iteration per plane, so the LDS source of each store is fixed at compile time
-            double* T = si.tr + (int64_t)e * TPE + r;
+            // lane (el, r) writes side node r of every (side, v) plane of its element into the
+            // side's face slot: one unrolled iteration per plane, so the LDS source of each store
+            // is fixed at compile time
             if (act) {
 #pragma unroll
-                for (int m = 0; m < 12; ++m) {
-                    const int side = m / 3, v = m % 3;
-                    const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : r;
-                    const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : r;
-                    T[m * NGL] = S[v * C::PLANE + row * C::RS + col];
+                for (int side = 0; side < 4; ++side) {
+                    double* T = si.tr + (int64_t)si.slot[e * 4 + side] * 3 * NGL + r;
+#pragma unroll
+                    for (int v = 0; v < 3; ++v) {
+                        const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : r;
+                        const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : r;
+                        T[v * NGL] = S[v * C::PLANE + row * C::RS + col];
+                    }
                 }
             }
             // ShockCapturing.okl:63-217 on the updated state, the next stage's viscosity, with the
@@ -444,10 +447,10 @@ void launch_volume_fluxdiff(esdg_ctx* ctx, int Nelem, const double* Jac, const d
 
 template <int NGL>
 void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar,
-                         double* Qt, const double* Qv, const double* SPV, double* tr, const int* active, int seq,
-                         const int* list, const int* list_count) {
+                         double* Qt, const double* Qv, const double* SPV, double* tr, const int* slot,
+                         const int* active, int seq, const int* list, const int* list_count) {
     using C = VolCfg<NGL>;
-    SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV, rk ? tr : nullptr};
+    SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV, rk ? tr : nullptr, slot};
     si.active = active;
     si.seq = seq;
     si.list = list;
@@ -491,13 +494,14 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
 // Fused volume + surface lift (+ viscous terms when Qv is given) (+ RK update in place when rk,
 // with the side traces of the new state written to tr when given): called by the fused stage.
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV, double* tr, const int* active, int seq, const int* list,
-                 const int* list_count) {
+                 const double* Qv, const double* SPV, double* tr, const int* slot, const int* active, int seq,
+                 const int* list, const int* list_count) {
     if (Qv && !(active && list && list_count))
         return set_error(ctx, ESDG_ERR_STATE, "fused_volume: viscous terms need the stage marks and element list");
     KernelScope ks(ctx, "fused_volume");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr, active, seq, list,
-                      list_count);
+    if (tr && !slot) return set_error(ctx, ESDG_ERR_STATE, "fused_volume: side traces need the face slots");
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr, slot, active, seq,
+                      list, list_count);
     return launch_check(ctx, "fused_volume");
 }
 
