@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <tuple>
 #include <vector>
 
 #include "esdg_device_setup.h"
@@ -325,29 +326,49 @@ class deviceclass {
         const double dvc = rp_.DFL / (std::pow(ngl, 2)) * std::pow(rp_.min_ele_size, 2);
         double ck[8] = {0.0, t, 0.0, last_dt, 0.0, 0.0, 0.0, 0.0};
         int rc = esdg_memcpy(ctx_, o_clock, ck, sizeof ck, ESDG_H2D);
+        // one step: the reductions and dt on the device, then the stages
+        auto issue_step = [&]() {
+            int r = lambda_fresh ? ESDG_OK : esdg_find_lambda_max(ctx_, Nelem, o_q, o_LambdaMax);
+            if (!r) r = esdg_reduce_max(ctx_, Nelem, o_LambdaMax, o_red);
+            if (!r && rp_.artificial_viscosity) {
+                if (!visc_fresh) r = viscosity(1);
+                if (!r) r = esdg_reduce_max(ctx_, Nelem, o_ViscPara, o_red + 1);
+            }
+            if (!r) r = fused_clock(ctx_, o_red, o_clock, dti, dvc, rp_.artificial_viscosity, rp_.T);
+            for (int s = 0; s < stages && !r; ++s) {
+                if (rp_.artificial_viscosity && s > 0 && !visc_fresh) r = viscosity(2);
+                if (r) break;
+                FusedArgs fa = fused_args();
+                if (fuse_sensor) { fa.next_visc = o_ViscPara; fa.next_visc_plot = o_ViscForPlot; }
+                if (s == stages - 1) fa.next_lambda = o_LambdaMax;
+                fa.dt_dev = o_clock;
+                fa.C_stage = Cc[s];
+                r = fused_.stage(ctx_, s, A[s], 0.0, fa);
+                visc_fresh = fuse_sensor;
+                lambda_fresh = s == stages - 1;
+            }
+            return r;
+        };
         int issued = 0;
         while (!rc && issued < max_steps && ck[1] < rp_.T) {
             const int chunk = std::min(max_steps - issued, 64);
             for (int k = 0; k < chunk && !rc; ++k) {
-                rc = lambda_fresh ? ESDG_OK : esdg_find_lambda_max(ctx_, Nelem, o_q, o_LambdaMax);
-                if (!rc) rc = esdg_reduce_max(ctx_, Nelem, o_LambdaMax, o_red);
-                if (!rc && rp_.artificial_viscosity) {
-                    if (!visc_fresh) rc = viscosity(1);
-                    if (!rc) rc = esdg_reduce_max(ctx_, Nelem, o_ViscPara, o_red + 1);
+                // From the second step of a run the step's launches and arguments repeat (dt, t
+                // and the stage marks live on the device): that step is captured once as a graph
+                // and replayed.  Profiling runs, and a capture the runtime refuses, stay eager.
+                if (lambda_fresh && !ctx_->profile && graph_ok) {
+                    if (graph_stale()) drop_step_graph();
+                    if (!step_exec) {
+                        rc = capture_step(issue_step);
+                        if (rc) break;
+                    }
+                    if (step_exec) {
+                        rc = fused_.advance(ctx_, stages);
+                        if (!rc) rc = hip_check(ctx_, hipGraphLaunch(step_exec, ctx_->stream), "step graph launch");
+                        continue;
+                    }
                 }
-                if (!rc) rc = fused_clock(ctx_, o_red, o_clock, dti, dvc, rp_.artificial_viscosity, rp_.T);
-                for (int s = 0; s < stages && !rc; ++s) {
-                    if (rp_.artificial_viscosity && s > 0 && !visc_fresh) rc = viscosity(2);
-                    if (rc) break;
-                    FusedArgs fa = fused_args();
-                    if (fuse_sensor) { fa.next_visc = o_ViscPara; fa.next_visc_plot = o_ViscForPlot; }
-                    if (s == stages - 1) fa.next_lambda = o_LambdaMax;
-                    fa.dt_dev = o_clock;
-                    fa.C_stage = Cc[s];
-                    rc = fused_.stage(ctx_, s, A[s], 0.0, fa);
-                    visc_fresh = fuse_sensor;
-                    lambda_fresh = s == stages - 1;
-                }
+                rc = issue_step();
             }
             issued += chunk;
             if (!rc) rc = esdg_memcpy(ctx_, ck, o_clock, sizeof ck, ESDG_D2H);
@@ -359,6 +380,44 @@ class deviceclass {
         last_dt = ck[3];
         if (steps_taken) *steps_taken = (int)ck[2];
         return esdg_finish(ctx_);
+    }
+
+    // Captures one step of the fused schedule into step_exec.  A capture that fails leaves the
+    // solver eager (graph_ok = false) without an error; the step is then issued eagerly.
+    template <class F>
+    int capture_step(F&& issue_step) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(ctx_->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            graph_ok = false;
+            return ESDG_OK;
+        }
+        fused_.capturing = true;
+        const int r = issue_step();
+        fused_.capturing = false;
+        const hipError_t e = hipStreamEndCapture(ctx_->stream, &g);
+        if (r) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        if (e != hipSuccess || !g || hipGraphInstantiate(&step_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            step_exec = nullptr;
+            graph_ok = false;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        // the capture issued nothing: the step is launched from the graph by the caller
+        graph_key = current_graph_key();
+        return ESDG_OK;
+    }
+    // what a captured step depends on beyond its buffers: the learned indicator's settings
+    std::tuple<const void*, float, float, int> current_graph_key() const {
+        return std::make_tuple((const void*)ind_, ind_lo, ind_hi, ind_audit);
+    }
+    bool graph_stale() const { return step_exec && graph_key != current_graph_key(); }
+    void drop_step_graph() {
+        if (step_exec) (void)hipGraphExecDestroy(step_exec);
+        step_exec = nullptr;
     }
 
     // The same time loop with the single-kernel stage: dt and t live on the device, steps run in
@@ -431,6 +490,7 @@ class deviceclass {
     }
 
     int freeOccaVars() {
+        drop_step_graph();
         void* ptrs[] = {o_q, o_Qt, o_gRK, o_Qtmp, o_D, o_Dstrong, o_Dhat, o_VdmInv, o_GLw, o_Jac, o_Yxi, o_Yeta, o_Xxi, o_Xeta,
                         o_B, o_Bx, o_By, o_x, o_y, o_nx, o_ny, o_scal, o_qL, o_qR, o_bL, o_bR, o_SurfaceParts, o_DBSurf1,
                         o_DBSurf2, o_EleSizes, o_LambdaMax, o_ViscPara, o_ViscForPlot, o_red, o_EdgeData, o_EdgeReversed,
@@ -511,6 +571,10 @@ class deviceclass {
     unsigned long long* o_emax = nullptr;
     const esdg_mlp* stage_ind = nullptr;     // the indicator the stage batch was last given
     float stage_lo = 0.0f, stage_hi = 0.0f;
+    // a captured step of the fused schedule (clock_timeloop) and the settings it was captured with
+    hipGraphExec_t step_exec = nullptr;
+    bool graph_ok = true;
+    std::tuple<const void*, float, float, int> graph_key{nullptr, 0.0f, 0.0f, 0};
     double *o_q = nullptr, *o_Qt = nullptr, *o_gRK = nullptr, *o_Qtmp = nullptr;
     double *o_D = nullptr, *o_Dstrong = nullptr, *o_Dhat = nullptr, *o_VdmInv = nullptr, *o_GLw = nullptr;
     double *o_Jac = nullptr, *o_Yxi = nullptr, *o_Yeta = nullptr, *o_Xxi = nullptr, *o_Xeta = nullptr;

@@ -59,8 +59,14 @@ class FusedStage {
     bool slots_built = false;
     int* active = nullptr;      // == seq: the element has viscosity in reach this stage (AV)
     int* alist = nullptr;       // the elements marked this stage, each once, in marking order (AV)
-    int* acount = nullptr;      // length of alist for stage seq: acount[seq & 1] (AV)
-    int seq = 0;                // stage counter for active
+    int* acount = nullptr;      // [0], [1]: length of alist for stage mark m at [m & 1]; [2]: the
+                                // stage mark m, advanced on the device before each stage (AV)
+    int seq = 0;                // stages issued since the marks were last cleared (host count)
+    bool capturing = false;     // a step is being captured: no host-side resets inside it
+    // the marks restart (every element's mark cleared) when the counter nears its range's end
+    int reset_marks(esdg_ctx* ctx);
+    // `stages` stages issued by a graph replay: counted, and the marks restarted first if needed
+    int advance(esdg_ctx* ctx, int stages);
 };
 
 }  // namespace esdg

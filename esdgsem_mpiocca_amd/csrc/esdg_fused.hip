@@ -33,7 +33,7 @@ namespace esdg {
 
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
                  const double* Qv, const double* SPV, double* tr, const int* slot, const int* active = nullptr,
-                 int seq = 0, const int* list = nullptr, const int* list_count = nullptr);
+                 const int* list = nullptr, const int* acount = nullptr);
 int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
                     double* tr, const int* slot, double* next_visc, double* next_visc_plot, const double* B,
                     const double* VdmInv, double* next_lambda, const double* dt_dev);
@@ -54,8 +54,7 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
                                                     const double* __restrict__ scal, const double* __restrict__ BL,
                                                     const double* __restrict__ BR, double* __restrict__ SP,
                                                     const double* __restrict__ visc, int* __restrict__ active,
-                                                    int seq, int Nelem, int* __restrict__ alist,
-                                                    int* __restrict__ acount) {
+                                                    int Nelem, int* __restrict__ alist, int* __restrict__ acount) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)Nfaces * ngl) return;
     const int f = (int)(t / ngl), i = (int)(t % ngl);
@@ -75,8 +74,10 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
     SP[eid + ngl] = sc * c.geomFace * F1;
     SP[eid + 2 * ngl] = sc * c.geomFace * F2;
     if (AV) {
-        // the next stage's list length starts at zero (the previous stage, which used that slot,
-        // has finished)
+        // the stage's mark lives on the device (acount[2], advanced before each stage); the next
+        // stage's list length starts at zero (the previous stage, which used that slot, has
+        // finished)
+        const int seq = acount[2];
         if (t == 0) acount[(seq + 1) & 1] = 0;
         // elements with viscosity in reach (on themselves or a face neighbour) need the viscous
         // work: a face with viscosity on either side marks both its elements for this stage
@@ -121,6 +122,7 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, AvMesh M, const dou
                                                   const int* __restrict__ acount) {
     using EB = ElemBlock<NGL>;
     constexpr int NN = NGL * NGL;
+    const int seq = acount[2];
     __shared__ double sDh[NN], sDs[NN];
     __shared__ double sA[EB::THREADS], sB[EB::THREADS], sC[EB::THREADS], sE[EB::THREADS];
     __shared__ double sEps[EB::EPB];
@@ -129,7 +131,7 @@ __global__ void __launch_bounds__(256) k_fused_av(int Nelem, AvMesh M, const dou
         sDs[threadIdx.x] = Ds[threadIdx.x];
     }
     const int el = threadIdx.x / NN;
-    const int cnt = min(*acount, Nelem);
+    const int cnt = min(acount[seq & 1], Nelem);
     __syncthreads();
     for (int grp = blockIdx.x; grp * EB::EPB < cnt; grp += gridDim.x) {
         const int idx = grp * EB::EPB + el;
@@ -155,10 +157,11 @@ __global__ void __launch_bounds__(256) k_fused_viscous_face(int Nelem, int ngl, 
                                                             const double* __restrict__ nxa,
                                                             const double* __restrict__ nya,
                                                             const double* __restrict__ scal,
-                                                            const int* __restrict__ active, int seq,
+                                                            const int* __restrict__ active,
                                                             const int* __restrict__ alist,
                                                             const int* __restrict__ acount, double* __restrict__ SPV) {
-    const int64_t items = (int64_t)min(*acount, Nelem) * 4 * ngl;
+    const int seq = acount[2];
+    const int64_t items = (int64_t)min(acount[seq & 1], Nelem) * 4 * ngl;
     for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < items;
          it += (int64_t)gridDim.x * blockDim.x) {
         const int li = (int)(it / (4 * ngl)), rem = (int)(it % (4 * ngl));
@@ -185,9 +188,30 @@ void launch_fused_av(esdg_ctx* ctx, const FusedArgs& A, const FusedStage& F) {
                    A.nx, A.ny, A.scal, A.bL, A.bR, GatherConsts{ctx->p.bc_kind, ctx->p.h_zero},
                    ctx->p.geomFace, ctx->p.PosPresTOL, F.grad, F.qvisc};
     k_fused_av<NGL><<<grid, EB::THREADS, 0, ctx->stream>>>(A.Nelem, M, F.tr, A.Dhat, A.Dstrong, A.ViscPara, F.alist,
-                                                          F.acount + (F.seq & 1));
+                                                          F.acount);
 }
 
+
+// The stage mark (acount[2]): one more per stage.
+__global__ void k_fused_mark(int* acount) { acount[2] += 1; }
+
+// Marks restart from 0: every element's mark and the counter cleared (outside any captured step).
+int FusedStage::reset_marks(esdg_ctx* ctx) {
+    int rc = esdg_fill(ctx, reinterpret_cast<double*>(active), 0.0, ((size_t)Nelem + 1) / 2);
+    if (!rc) rc = esdg_fill(ctx, reinterpret_cast<double*>(acount), 0.0, 2);
+    if (!rc) seq = 0;
+    return rc;
+}
+
+int FusedStage::advance(esdg_ctx* ctx, int stages) {
+    if (!av) return ESDG_OK;
+    if (seq >= INT_MAX - 64 - stages) {
+        int rc = reset_marks(ctx);
+        if (rc) return rc;
+    }
+    seq += stages;
+    return ESDG_OK;
+}
 
 __global__ void k_fused_clock(const double* __restrict__ red, double* __restrict__ ck, double dti, double dvc, int av,
                               double T) {
@@ -226,6 +250,7 @@ int FusedStage::init(esdg_ctx* ctx, int Ne, int Nf, bool with_av) {
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&active, ((size_t)Ne + 1) / 2 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&alist, ((size_t)Ne + 1) / 2 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&acount, 256);
+    if (!rc && av) rc = esdg_fill(ctx, reinterpret_cast<double*>(acount), 0.0, 32);
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&grad, (size_t)Ne * 16 * ngl * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&fvisc, fn * 2 * sizeof(double));
     if (!rc && av) rc = esdg_malloc(ctx, (void**)&qvisc, (size_t)Ne * 2 * ngl * ngl * sizeof(double));
@@ -256,13 +281,18 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
     GatherConsts gc{ctx->p.bc_kind, ctx->p.h_zero};
     const unsigned fb = blocks_for((int64_t)A.Nfaces * ngl, 256);
     if (av) {
-        // stage mark of the elements with viscosity in reach; the marks of earlier stages are
-        // other values, so nothing needs clearing (except when the counter wraps)
-        if (++seq == INT_MAX) {
-            int rc = esdg_fill(ctx, reinterpret_cast<double*>(active), 0.0, ((size_t)Nelem + 1) / 2);
-            if (rc) return rc;
-            seq = 1;
+        // stage mark of the elements with viscosity in reach, advanced on the device (replays of a
+        // captured step take fresh marks); the marks of earlier stages are other values, so
+        // nothing needs clearing (except before the counter would wrap)
+        int rc = ESDG_OK;
+        if (!capturing && seq >= INT_MAX - 64) rc = reset_marks(ctx);
+        ++seq;
+        if (!rc) {
+            KernelScope ks(ctx, "fused_mark");
+            k_fused_mark<<<1, 1, 0, ctx->stream>>>(acount);
+            rc = launch_check(ctx, "fused_mark");
         }
+        if (rc) return rc;
     }
     {
         KernelScope ks(ctx, "fused_face");
@@ -271,7 +301,7 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
             constexpr int KK = decltype(kind_tag)::value;
             k_fused_face<AVK, KK><<<fb, 256, 0, ctx->stream>>>(A.Nfaces, ngl, fc, gc, A.EdgeData, tr, A.nx, A.ny,
                                                              A.scal, A.bL, A.bR, fstar, AVK ? A.ViscPara : nullptr,
-                                                             AVK ? active : nullptr, seq, A.Nelem,
+                                                             AVK ? active : nullptr, A.Nelem,
                                                              AVK ? alist : nullptr, AVK ? acount : nullptr);
         };
         auto by_kind = [&](auto av_tag) {
@@ -309,14 +339,14 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         KernelScope ks(ctx, "fused_viscous_face");
         const unsigned vb = (unsigned)std::max<int64_t>(std::min<int64_t>(((int64_t)A.Nelem * 4 * ngl + 255) / 256, 2048), 1);
         k_fused_viscous_face<<<vb, 256, 0, ctx->stream>>>(A.Nelem, ngl, ctx->p.geomFace, A.EdgeData, A.ElemToEdge,
-                                                          grad, A.ViscPara, A.nx, A.ny, A.scal, active, seq, alist,
-                                                          acount + (seq & 1), fvisc);
+                                                          grad, A.ViscPara, A.nx, A.ny, A.scal, active, alist,
+                                                          acount, fvisc);
         rc = launch_check(ctx, "fused_viscous_face");
         if (rc) return rc;
     }
     // volume + surface + viscous terms + RK in one element pass, then the limiter
     rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, slot, active,
-                      seq, alist, acount + (seq & 1));
+                      alist, acount);
     if (!rc && A.positivity)
         rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, slot, A.next_visc, A.next_visc_plot,
                              A.B, A.VdmInv, A.next_lambda, A.dt_dev);

@@ -50,15 +50,14 @@ struct SurfIn {
     SensorConsts sc;
     double* lam;           // FindLambdaMax's element maximum wave speed
     double tol;            // PosPresTOL (velocities are 0 where h <= tol)
-    // VOL_SURF_VISC_RK: elements with active[e] != seq have zero viscosity on themselves and on
-    // every face neighbour; their viscous terms are +-0 and are not read
+    // VOL_SURF_VISC_RK: elements with active[e] != the stage mark have zero viscosity on
+    // themselves and on every face neighbour; their viscous terms are +-0 and are not read
     const int* active;
-    int seq;
-    // the marked elements (list[0 .. *list_count)): with them given, VOL_SURF_RK leaves the marked
-    // elements out and VOL_SURF_VISC_RK runs over the list only, so that the viscous terms run in
-    // wavefronts of marked elements alone
+    // the marked elements (list[0 .. acount[m & 1]), m = acount[2] the stage mark): with them
+    // given, VOL_SURF_RK leaves the marked elements out and VOL_SURF_VISC_RK runs over the list
+    // only, so that the viscous terms run in wavefronts of marked elements alone
     const int* list;
-    const int* list_count;
+    const int* acount;
     // the device clock's dt (may be null): the update uses Cs * dt, and a block with dt = 0 (a step
     // issued after T) does nothing
     const double* dtp;
@@ -93,16 +92,20 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
     const int el = lane / NGL, r = lane % NGL;
     int e = blockIdx.x * C::EPW + el;
     bool act = (el < C::EPW) && (e < Nelem);
+    int seq = 0;
+    if constexpr (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
+        if (si.acount) seq = si.acount[2];
+    }
     if constexpr (MODE == VOL_SURF_VISC_RK) {
         if (si.list) {
             // blocks past the end of the list do nothing (the count is the same for every lane)
-            const int cnt = min(*si.list_count, Nelem);
+            const int cnt = min(si.acount[seq & 1], Nelem);
             if ((int)blockIdx.x * C::EPW >= cnt) return;
             act = (el < C::EPW) && (e < cnt);
             e = act ? si.list[e] : 0;
         }
     } else if constexpr (MODE == VOL_SURF_RK) {
-        if (si.list && act && si.active[e] == si.seq) act = false;
+        if (si.list && act && si.active[e] == seq) act = false;
     }
     double* S = s + (act ? el : 0) * C::ESZ;
     double lam = 0.0;   // this column's maximum wave speed of the updated state (si.lam)
@@ -213,7 +216,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 }
             }
         }
-        if (MODE == VOL_SURF_VISC_RK && si.active[e] == si.seq) {
+        if (MODE == VOL_SURF_VISC_RK && si.active[e] == seq) {
             if (si.visc[e] != 0.0) {
                 const double* qv = si.Qv + (int64_t)e * C::NN * 2 + r;
 #pragma unroll
@@ -448,13 +451,12 @@ void launch_volume_fluxdiff(esdg_ctx* ctx, int Nelem, const double* Jac, const d
 template <int NGL>
 void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar,
                          double* Qt, const double* Qv, const double* SPV, double* tr, const int* slot,
-                         const int* active, int seq, const int* list, const int* list_count) {
+                         const int* active, const int* list, const int* acount) {
     using C = VolCfg<NGL>;
     SurfIn si{a.ElemToEdge, a.EMS, a.EOR, fstar, rkA, cdt, a.gRK, Qv, a.ViscPara, SPV, rk ? tr : nullptr, slot};
     si.active = active;
-    si.seq = seq;
     si.list = list;
-    si.list_count = list_count;
+    si.acount = acount;
     si.dtp = a.dt_dev;
     si.Cs = a.C_stage;
     if (rk && tr) {
@@ -494,14 +496,14 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
 // Fused volume + surface lift (+ viscous terms when Qv is given) (+ RK update in place when rk,
 // with the side traces of the new state written to tr when given): called by the fused stage.
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
-                 const double* Qv, const double* SPV, double* tr, const int* slot, const int* active, int seq,
-                 const int* list, const int* list_count) {
-    if (Qv && !(active && list && list_count))
+                 const double* Qv, const double* SPV, double* tr, const int* slot, const int* active,
+                 const int* list, const int* acount) {
+    if (Qv && !(active && list && acount))
         return set_error(ctx, ESDG_ERR_STATE, "fused_volume: viscous terms need the stage marks and element list");
     KernelScope ks(ctx, "fused_volume");
     if (tr && !slot) return set_error(ctx, ESDG_ERR_STATE, "fused_volume: side traces need the face slots");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr, slot, active, seq,
-                      list, list_count);
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_volume_fused, ctx, a, rk, rkA, cdt, fstar, Qt, Qv, SPV, tr, slot, active,
+                      list, acount);
     return launch_check(ctx, "fused_volume");
 }
 

@@ -364,3 +364,42 @@ def test_ism_mesh_lake_at_rest(dev):
     Q = qg.reshape(case.Nelem, 3, -1)
     assert np.abs(Q[:, 1]).max() <= 1e-9 and np.abs(Q[:, 2]).max() <= 1e-9
     assert np.abs(qg - qo).max() <= 1e-14
+
+
+@pytest.mark.parametrize("learned", [False, True], ids=["sensor", "learned"])
+def test_captured_fused_step_equals_eager(dev, learned):
+    """From the second step of each run the fused schedule replays one captured step (dt, t and the
+    stage marks live on the device; the marked-element update runs on the second stream inside the
+    graph); a run with per-launch events stays eager.  Both give the same bits over two run calls
+    of 40 and 30 steps (a 64-step chunk boundary inside), with the learned indicator's counts too."""
+    case = Case.dam_break(N=7, nel=24, noise=1e-3, seed=7, ratio=2.5, angle=0.4)
+    rp = RunParams(num_flux=3, artificial_viscosity=1, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5, DFL=0.05)
+    outs = []
+    for prof in (False, True):
+        ctx = dev.Context(dev.make_params(case, rp))
+        try:
+            s = dev.Solver(ctx, case, rp, fused=1)
+            net = None
+            if learned:
+                net = dev.MLP(ctx, seed=3)
+                s.set_indicator(net, 0.3, 0.7)
+                s.indicator_audit(True)
+            ctx.profile(prof)
+            n1, _ = s.run(40)
+            n2, t = s.run(30)
+            q = s.get_q()
+            counts = s.indicator_counts() if learned else None
+            visc = s.view("ViscPara", case.Nelem).download()
+            s.close()
+            if net:
+                net.close()
+        finally:
+            ctx.close()
+        outs.append((q, n1 + n2, t, counts, visc))
+    (qa, na, ta, ca, va), (qb, nb, tb, cb, vb) = outs
+    assert (na, ta) == (nb, tb) == (70, tb)
+    assert np.array_equal(qa, qb)
+    assert ca == cb
+    assert np.array_equal(va, vb)
+    if not learned:
+        assert (va > 0).any()
