@@ -26,12 +26,17 @@ namespace esdg {
 // the one already loaded for that degree.
 __constant__ double c_stage_D[9][64];
 
+// Per element: the four planes of the volume sweep (VolCfg::ESZ), reused for the face fluxes
+// [side][v][k] once the sweeps have read them, and 16 doubles for the limiter's element means
+// (keeping consecutive elements 8 doubles apart modulo 16, as ESZ does).  At N = 7 a block of 8
+// elements then fits 8 blocks per CU.
 template <int NGL>
 struct StageCfg {
     using V = VolCfg<NGL>;
     static constexpr int EPW = V::EPW;
     static constexpr int FSZ = 12 * NGL;              // face fluxes [side][v][k] of one element
-    static constexpr int ELDS = V::ESZ + FSZ + ((8 - (V::ESZ + FSZ) % 16) + 16) % 16;   // per element (bank spread as ESZ)
+    static_assert(FSZ <= 4 * V::PLANE, "the face fluxes fit in the element's planes");
+    static constexpr int ELDS = V::ESZ + 16;          // per element
     static constexpr int LDS = EPW * ELDS;            // doubles of LDS per block
 };
 
@@ -46,10 +51,11 @@ struct MaxLds<HI, HI> {
 };
 
 // Volume term, surface lift, RK update, limiter and wave speed of one block of elements; the
-// face fluxes of the elements' sides are in LDS (FS).  Pointer parameters are __restrict__ so
-// that the geometry loads schedule as in a kernel taking them as arguments.
-template <int NGL, bool AV>
-__device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, double* FS, double g, double tol,
+// face fluxes of the elements' sides come from `faces`, into LDS (FS) after the sweeps.  Pointer
+// parameters are __restrict__ so that the geometry loads schedule as in a kernel taking them as
+// arguments.
+template <int NGL, bool AV, class Faces>
+__device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, const Faces& faces, double g, double tol,
                                            double zero_tol, int positivity, double rkA, double rkC, double dt, bool last,
                                            const double* __restrict__ qi, double* __restrict__ qo,
                                            double* __restrict__ gRK, const double* __restrict__ Yeta,
@@ -60,6 +66,8 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
                                            unsigned long long* __restrict__ lmax, const StageScn* SV, int mark,
                                            const double* D) {
     using C = VolCfg<NGL>;
+    double* FS = P;                  // face fluxes, after the sweeps (the planes are free then)
+    double* MS = P + C::ESZ;         // the limiter's element means
     const int lane = threadIdx.x;
     double h[NGL], hu[NGL], hv[NGL], u[NGL], v[NGL], Y[NGL], X[NGL];
     double a1[NGL], a2[NGL], a3[NGL];
@@ -134,6 +142,13 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
             b2[k] = -rJ[k] * b2[k] - g * h[k] * Bx[xb + k * NGL];
             b3[k] = -rJ[k] * b3[k] - g * h[k] * By[xb + k * NGL];
         }
+    }
+    // the face fluxes of the element's sides (from the stage-start state) into the planes, which
+    // every lane has read by now
+    __syncthreads();
+    if (act) faces(FS);
+    __syncthreads();
+    if (act) {
         // surface lift: bottom/top sides first, then right/left (SurfaceKernel.okl).  FS holds
         // -ms F* (ms = +-1, so the sign is exact and -ms F* rJ equals the okl's -ms rJ F* bitwise).
         {
@@ -238,15 +253,15 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, do
             const double size = EleSizes[e];
             ha = ha / size;
             if (fabs(ha) < zero_tol) ha = 0.0;
-            FS[0] = ha;
-            FS[1] = hua / size;
-            FS[2] = hva / size;
-            FS[3] = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
-            FS[4] = hmin;
+            MS[0] = ha;
+            MS[1] = hua / size;
+            MS[2] = hva / size;
+            MS[3] = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
+            MS[4] = hmin;
         }
         __syncthreads();
         if (act) {
-            const double ha = FS[0], hua = FS[1], hva = FS[2], th = FS[3], hmin = FS[4];
+            const double ha = MS[0], hua = MS[1], hva = MS[2], th = MS[3], hmin = MS[4];
 #pragma unroll
             for (int k = 0; k < NGL; ++k) {
                 if (hmin < 0.0) {
@@ -306,15 +321,15 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
     const int e = e0 + el;
     const bool act = (el < C::EPW) && (e < S.Nelem);
     double* P = lds + (act ? el : 0) * SC::ELDS;     // planes: h, hu, hv, 1/h (later: the row sums)
-    double* FS = P + C::ESZ;                         // face fluxes of the element's sides
     const double* qi = S.q[src];
     double* qo = S.q[src ^ 1];
     const double g = S.g;
-    const FluxConsts fc{g, S.geomFace, S.tol, S.ES};
-    const GatherConsts gc{S.bc, S.h_zero};
 
-    // ---- fluxes of the element's four sides: lane r takes side node r of every side ----------
-    if (act) {
+    // ---- fluxes of the element's four sides: lane r takes side node r of every side (called by
+    // stage_core once the sweeps are done, with the face-flux area FS) ----------------------------
+    auto faces = [&](double* FS) {
+        const FluxConsts fc{g, S.geomFace, S.tol, S.ES};
+        const GatherConsts gc{S.bc, S.h_zero};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int f = S.E2E[e * 4 + s] - 1;
@@ -335,11 +350,12 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
             FS[(s * 3 + 1) * NGL + r] = nms * (sc * S.geomFace * F1);
             FS[(s * 3 + 2) * NGL + r] = nms * (sc * S.geomFace * F2);
         }
-    }
+    };
 
     __syncthreads();   // sD written
-    stage_core<NGL, AV>(e, r, act, P, FS, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo, S.gRK,
-                        S.Yeta, S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax, &S, mark, sD);
+    stage_core<NGL, AV>(e, r, act, P, faces, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo,
+                        S.gRK, S.Yeta, S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax, &S, mark,
+                        sD);
 }
 
 template <int LO, int HI, bool AV>
