@@ -144,7 +144,7 @@ def ensemble_bench(dev, args):
     ens.close()
     # per-scenario loops: one context + solver per distinct mesh; only the stepping is timed
     loops = {}
-    for fused, name in ((2, "single_kernel"), (0, "reference_sequence")):
+    for fused, name in ((2, "single_kernel"), (0, "reference_sequence")) if args.ens_loops else ():
         solvers, tsum = {}, 0.0
         try:
             for case, rp, q in members:
@@ -178,7 +178,7 @@ def ensemble_bench(dev, args):
         "scenarios": n, "nodes": nodes, "dofs": 3 * nodes,
         "dof_stage_updates_per_s": 3 * nodes * 5 * K / wall,
         "per_scenario_loop": loops,
-        "speedup_vs_loop": {k: batched / v for k, v in loops.items()},
+        "speedup_vs_loop": {k: batched / v for k, v in loops.items()} or None,
         "schedule": tuned, "autotune_ms_per_step": tuned_ms,
         "stage_kernel": {"us_per_stage": stage_us, "launches_per_stage": launches / 10 if launches else None,
                          "compulsory_bytes_per_launch": stage_bytes,
@@ -530,6 +530,7 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--ens-warmup", type=int, default=2)
     ap.add_argument("--ens-autotune", type=int, default=1)
+    ap.add_argument("--ens-loops", type=int, default=1, help="also time the per-scenario loops (ensemble)")
     ap.add_argument("--ens-av", type=int, default=1,
                     help="also run the ensemble with AV: learned indicator batched vs the plain per-scenario loops")
     ap.add_argument("--ens-av-steps", type=int, default=10)
