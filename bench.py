@@ -120,7 +120,10 @@ def ensemble_bench(dev, args):
     print(f"# ensemble: {n} scenarios, {nodes/1e6:.1f} M nodes, build {t_build:.1f} s, "
           f"create {t_create:.1f} s, {ens.nbytes()/2**30:.2f} GiB", file=sys.stderr, flush=True)
     tuned, tuned_ms = None, None
-    if args.ens_autotune:
+    if args.ens_schedule:
+        ens.set_schedule(**json.loads(args.ens_schedule))
+        tuned = ens.schedule().as_dict()
+    elif args.ens_autotune:
         tuned, tuned_ms = ens.autotune(trial_steps=2)
         print(f"# ensemble autotune: {tuned} ({tuned_ms:.2f} ms/step)", file=sys.stderr, flush=True)
     ens.run(W)
@@ -218,7 +221,12 @@ def ensemble_av_bench(dev, args):
             if learned:
                 net = dev.MLP(ctx, params=wparams)
                 ens.set_indicator(net, args.ind_lo, args.ind_hi)
-            tuned = ens.autotune(trial_steps=2)[0] if args.ens_autotune else None
+            tuned = None
+            if args.ens_schedule:
+                ens.set_schedule(**json.loads(args.ens_schedule))
+                tuned = ens.schedule().as_dict()
+            elif args.ens_autotune:
+                tuned = ens.autotune(trial_steps=2)[0]
             ens.run(W)
             ctx.finish()
             t0 = time.perf_counter()
@@ -530,6 +538,8 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--ens-warmup", type=int, default=2)
     ap.add_argument("--ens-autotune", type=int, default=1)
+    ap.add_argument("--ens-schedule", default="",
+                    help='a fixed ensemble schedule instead of the autotuned one, e.g. \'{"order": 1, "tile": 2}\'')
     ap.add_argument("--ens-loops", type=int, default=1, help="also time the per-scenario loops (ensemble)")
     ap.add_argument("--ens-av", type=int, default=1,
                     help="also run the ensemble with AV: learned indicator batched vs the plain per-scenario loops")

@@ -212,7 +212,7 @@ int esdg_ensemble_set_schedule(esdg_ensemble* E, const esdg_ensemble_schedule* s
     if (!E || !sc) return set_error(E ? E->ctx : nullptr, ESDG_ERR_ARG, "esdg_ensemble_set_schedule: null argument");
     StageSchedule s;
     s.order = sc->order; s.blocks_per_cu = sc->blocks_per_cu; s.per_degree = sc->per_degree != 0;
-    s.xcd = sc->xcd != 0; s.graph = sc->graph != 0;
+    s.xcd = sc->xcd != 0; s.graph = sc->graph != 0; s.tile = sc->tile ? sc->tile : 1;
     return E->batch.set_schedule(E->ctx, s);
 }
 
@@ -220,7 +220,7 @@ int esdg_ensemble_get_schedule(const esdg_ensemble* E, esdg_ensemble_schedule* s
     if (!E || !sc) return set_error(E ? E->ctx : nullptr, ESDG_ERR_ARG, "esdg_ensemble_get_schedule: null argument");
     const StageSchedule& s = E->batch.sched;
     sc->order = s.order; sc->blocks_per_cu = s.blocks_per_cu; sc->per_degree = s.per_degree;
-    sc->xcd = s.xcd; sc->graph = s.graph;
+    sc->xcd = s.xcd; sc->graph = s.graph; sc->tile = s.tile;
     return ESDG_OK;
 }
 
@@ -320,12 +320,11 @@ int esdg_ensemble_autotune(esdg_ensemble* E, int trial_steps, esdg_ensemble_sche
                     s.order = order; s.blocks_per_cu = bpc; s.per_degree = pd; s.xcd = xcd; s.graph = keep.graph;
                     cand.push_back(s);
                 }
-    for (const StageSchedule& s : cand) {
-        if (rc) break;
+    auto trial = [&](const StageSchedule& s) {
         rc = E->batch.set_schedule(ctx, s);
         if (!rc) rc = E->batch.run(ctx, E->A, E->C, E->stages, 2, E->src);   // warm-up, captures the graph
         if (!rc) rc = esdg_finish(ctx);
-        if (rc) break;
+        if (rc) return;
         const auto t0 = std::chrono::steady_clock::now();
         rc = E->batch.run(ctx, E->A, E->C, E->stages, trial_steps, E->src);
         if (!rc) rc = esdg_finish(ctx);
@@ -338,6 +337,18 @@ int esdg_ensemble_autotune(esdg_ensemble* E, int trial_steps, esdg_ensemble_sche
         // back to the snapshot (the state sits in buffer src0 again)
         E->src = src0;
         if (!rc) rc = copy_all(false);
+    };
+    for (const StageSchedule& s : cand) {
+        if (rc) break;
+        trial(s);
+    }
+    // the element-tile size, on the winner's launch shape
+    const StageSchedule shape = best_s;
+    for (int tile : {2, 4}) {
+        if (rc) break;
+        StageSchedule s = shape;
+        s.tile = tile;
+        trial(s);
     }
     esdg_free(ctx, snap);
     if (!rc && E->batch.d_icount) rc = esdg_memcpy(ctx, E->batch.d_icount, icount, sizeof icount, ESDG_H2D);

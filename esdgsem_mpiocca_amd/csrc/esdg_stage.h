@@ -72,6 +72,7 @@ struct StageSchedule {
     int per_degree = 0;     // 1: one launch per degree group instead of one mixed-degree launch
     int xcd = 0;            // 1: each XCD takes a contiguous range of the task list
     int graph = 1;          // replay pairs of steps from a captured hipGraph
+    int tile = 1;           // consecutive tasks (element groups of 64/ngl elements) per block: 1, 2 or 4
 };
 
 // Host side: device copies of the scenario table and the task list, plus the per-step sequence.

@@ -373,15 +373,18 @@ __device__ __forceinline__ void stage_task(const StageScn* __restrict__ scns, co
     }
 }
 
-// One RK stage over tasks[0, n).  Non-persistent: block b takes one task.  Persistent: a grid of
-// gridDim.x blocks loops over the tasks.  With xcd = 1 the task list is cut into 8 contiguous
-// parts and the blocks that the dispatcher places on XCD x (b mod 8 = x) take part x, so tasks
-// that are adjacent in the list (one element tile of several scenarios that share geometry)
-// meet in the same L2.  Every block leaves after a bounded number of tasks.
+// One RK stage over tasks[0, n).  Non-persistent: block b takes one task.  Persistent: the tasks
+// are cut into chunks of `tile` consecutive tasks and a grid of gridDim.x blocks loops over the
+// chunks, one block taking a chunk's tasks one after another (with tile > 1 and a grid of one
+// block per chunk this is the element-tile size: a block advances tile element groups).  With
+// xcd = 1 the task list is cut into 8 contiguous parts and the blocks that the dispatcher places
+// on XCD x (b mod 8 = x) take part x, so tasks that are adjacent in the list (one element tile of
+// several scenarios that share geometry) meet in the same L2.  Every block leaves after a bounded
+// number of tasks.
 template <int LO, int HI, bool PERSIST, bool AV>
 __global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ scns, const StageTask* __restrict__ tasks,
-                                                 int n, int xcd, int src, double rkA, double rkC, int last,
-                                                 const int* __restrict__ markp, int stage) {
+                                                 int n, int xcd, int tile, int src, double rkA, double rkC,
+                                                 int last, const int* __restrict__ markp, int stage) {
     __shared__ double lds[MaxLds<LO, HI>::value];
     __shared__ double sD[HI * HI];
     const int mark = AV ? *markp + stage : 0;
@@ -397,9 +400,13 @@ __global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ sc
     if constexpr (!PERSIST) {
         if (i0 < cnt) stage_task<LO, HI, AV>(scns, tasks[beg + i0], lds, sD, src, rkA, rkC, last != 0, mark);
     } else {
-        for (int i = i0; i < cnt; i += stride) {
-            if (i != i0) __syncthreads();   // LDS of the previous task is free
-            stage_task<LO, HI, AV>(scns, tasks[beg + i], lds, sD, src, rkA, rkC, last != 0, mark);
+        const int nch = (cnt + tile - 1) / tile;
+        for (int c = i0; c < nch; c += stride) {
+            const int i1 = min(cnt, (c + 1) * tile);
+            for (int i = c * tile; i < i1; ++i) {
+                if (i != i0 * tile) __syncthreads();   // LDS of the previous task is free
+                stage_task<LO, HI, AV>(scns, tasks[beg + i], lds, sD, src, rkA, rkC, last != 0, mark);
+            }
         }
     }
 }
@@ -750,8 +757,9 @@ void StageBatch::drop_graphs() {
 }
 
 int StageBatch::set_schedule(esdg_ctx* ctx, const StageSchedule& sc) {
-    if (sc.order < 0 || sc.order > 2 || sc.blocks_per_cu < 0 || sc.blocks_per_cu > 64)
-        return set_error(ctx, ESDG_ERR_ARG, "stage schedule: order must be 0..2 and blocks_per_cu 0..64");
+    if (sc.order < 0 || sc.order > 2 || sc.blocks_per_cu < 0 || sc.blocks_per_cu > 64 ||
+        (sc.tile != 1 && sc.tile != 2 && sc.tile != 4))
+        return set_error(ctx, ESDG_ERR_ARG, "stage schedule: order must be 0..2, blocks_per_cu 0..64, tile 1, 2 or 4");
     const int ns = (int)host.size();
     // scenario visiting order
     std::vector<int> scn(ns);
@@ -760,6 +768,7 @@ int StageBatch::set_schedule(esdg_ctx* ctx, const StageSchedule& sc) {
         std::stable_sort(scn.begin(), scn.end(), [&](int a, int b) { return host[a].ngl > host[b].ngl; });
     std::vector<StageTask> tasks;
     auto tiles = [&](int s) { const int epw = 64 / host[s].ngl; return (host[s].Nelem + epw - 1) / epw; };
+    auto task = [&](int s, int t) { return StageTask{s, t * (64 / host[s].ngl), host[s].ngl, 0}; };
     if (sc.order == 1) {
         // groups of scenarios on the same geometry, in order of first appearance; tile-major inside
         std::vector<int> done(ns, 0);
@@ -775,23 +784,25 @@ int StageBatch::set_schedule(esdg_ctx* ctx, const StageSchedule& sc) {
                     done[s1] = 1;
                 }
             }
-            const int epw = 64 / host[s0].ngl;
             for (int t = 0; t < tiles(s0); ++t)
-                for (int s1 : grp) tasks.push_back(StageTask{s1, t * epw, host[s1].ngl, 0});
+                for (int s1 : grp) tasks.push_back(task(s1, t));
         }
     } else {
-        for (int s : scn) {
-            const int epw = 64 / host[s].ngl;
-            for (int t = 0; t < tiles(s); ++t) tasks.push_back(StageTask{s, t * epw, host[s].ngl, 0});
-        }
+        for (int s : scn)
+            for (int t = 0; t < tiles(s); ++t) tasks.push_back(task(s, t));
     }
     // launch ranges
     std::vector<Range> rg;
     auto add_range = [&](int beg, int n, int lo, int hi) {
         Range r{beg, n, lo, hi, n, 0};
+        // chunks of sc.tile tasks; with xcd every XCD's part is chunked on its own
+        const int nch = sc.xcd ? 8 * (((n + 7) / 8 + sc.tile - 1) / sc.tile) : (n + sc.tile - 1) / sc.tile;
         const int pgrid = sc.blocks_per_cu * ncu;
-        if (sc.blocks_per_cu > 0 && pgrid < n && (!sc.xcd || pgrid >= 8)) {
+        if (sc.blocks_per_cu > 0 && pgrid < nch && (!sc.xcd || pgrid >= 8)) {
             r.grid = sc.xcd ? (pgrid / 8) * 8 : pgrid;
+            r.persistent = 1;
+        } else if (sc.tile > 1) {
+            r.grid = nch;
             r.persistent = 1;
         }
         rg.push_back(r);
@@ -834,11 +845,11 @@ template <int LO, int HI, bool AV>
 static void launch_stage(esdg_ctx* ctx, const StageBatch& b, const StageBatch::Range& r, int src, double rkA,
                          double rkC, int last, int stage) {
     if (r.persistent)
-        k_stage<LO, HI, true, AV><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, src, rkA,
-                                                                   rkC, last, b.d_mark, stage);
+        k_stage<LO, HI, true, AV><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd,
+                                                                   b.sched.tile, src, rkA, rkC, last, b.d_mark, stage);
     else
-        k_stage<LO, HI, false, AV><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, src,
-                                                                    rkA, rkC, last, b.d_mark, stage);
+        k_stage<LO, HI, false, AV><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, 1,
+                                                                    src, rkA, rkC, last, b.d_mark, stage);
 }
 
 template <bool AV>

@@ -399,7 +399,7 @@ class Solver:
 
 class EnsembleSchedule(C.Structure):
     _fields_ = [("order", C.c_int), ("blocks_per_cu", C.c_int), ("per_degree", C.c_int),
-                ("xcd", C.c_int), ("graph", C.c_int)]
+                ("xcd", C.c_int), ("graph", C.c_int), ("tile", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -39,6 +39,8 @@ typedef struct esdg_ensemble_schedule {
     int per_degree;     /* 1: one launch per degree group and stage instead of one mixed launch */
     int xcd;            /* 1: each XCD takes a contiguous part of the task list */
     int graph;          /* 1: replay pairs of steps from a captured hipGraph */
+    int tile;           /* element-tile size: a block of the stage kernel advances 1, 2 or 4 groups of
+                           64/N+1 elements of one scenario, one group after another (0 reads as 1) */
 } esdg_ensemble_schedule;
 
 /* params[i], inputs[i], rp[i] describe scenario i (rp[i].fused is ignored).  Only the inputs the
@@ -49,7 +51,9 @@ int esdg_ensemble_create(esdg_ctx* ctx, int nscn, const esdg_params* params, con
 int esdg_ensemble_set_schedule(esdg_ensemble* ens, const esdg_ensemble_schedule* sc);
 int esdg_ensemble_get_schedule(const esdg_ensemble* ens, esdg_ensemble_schedule* sc);
 /* Times trial_steps steps (after one warm-up step) of every candidate schedule, keeps the fastest
- * and restores the ensemble's state, time and step counts.  best / best_ms_per_step may be null. */
+ * and restores the ensemble's state, time and step counts.  The candidates: order x blocks_per_cu
+ * {0, 4, 8} x per_degree x xcd at tile 1, then tiles 2 and 4 with the winner's other settings.
+ * best / best_ms_per_step may be null. */
 int esdg_ensemble_autotune(esdg_ensemble* ens, int trial_steps, esdg_ensemble_schedule* best,
                            double* best_ms_per_step);
 /* Advances every scenario by `steps` steps; a scenario whose time has reached its T stays put

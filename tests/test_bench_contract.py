@@ -40,8 +40,10 @@ def test_committed_traffic_matches_the_reported_workload():
     read = tr["fetch_size_kib_per_dispatch"] * 1024 * cal["fetch_factor"]
     write = tr["write_size_kib_per_dispatch"] * 1024 * cal["write_factor"]
     assert tr["hbm_bytes_per_launch"] == read + write
-    # HBM traffic at least the compulsory bytes, and not wildly above them
-    assert 1.0 <= tr["hbm_bytes_per_launch"] / alg < 1.5
+    # HBM traffic near the compulsory bytes: a few percent below them is possible, since part of
+    # the face fluxes the face launch has just written is still in the L2 caches (8 x 4 MB) when
+    # the update reads them; well above them would mean re-reads
+    assert 0.95 <= tr["hbm_bytes_per_launch"] / alg < 1.5
 
 
 def test_committed_flops_definition():

@@ -126,8 +126,12 @@ def test_ensemble_matches_oracle(dev, mem, ensemble_default):
     dict(order=1, blocks_per_cu=1, per_degree=0, xcd=1, graph=1),
     dict(order=0, blocks_per_cu=2, per_degree=0, xcd=0, graph=1),
     dict(order=0, blocks_per_cu=1, per_degree=1, xcd=1, graph=1),
+    dict(order=0, blocks_per_cu=0, per_degree=1, xcd=0, graph=1, tile=2),
+    dict(order=1, blocks_per_cu=0, per_degree=0, xcd=1, graph=0, tile=4),
+    dict(order=2, blocks_per_cu=1, per_degree=0, xcd=0, graph=1, tile=4),
 ], ids=["tile_xcd", "degree_persistent_perdeg_eager", "tile_persistent_xcd", "persistent2",
-        "persistent_perdeg_xcd"])
+        "persistent_perdeg_xcd", "elemtile2_perdeg", "elemtile4_interleaved_xcd_eager",
+        "elemtile4_persistent"])
 def test_schedules_do_not_change_results(dev, mem, ensemble_default, schedule):
     res = run_ensemble(dev, mem, STEPS, schedule=schedule)
     for i in range(len(mem)):

@@ -134,7 +134,10 @@ def test_viscosity_changes_the_result(dev, mem, ensemble_default):
     dict(order=1, blocks_per_cu=0, per_degree=0, xcd=1, graph=1),
     dict(order=2, blocks_per_cu=1, per_degree=1, xcd=0, graph=0),
     dict(order=0, blocks_per_cu=2, per_degree=0, xcd=0, graph=1),
-], ids=["tile_xcd", "degree_persistent_perdeg_eager", "persistent2"])
+    dict(order=1, blocks_per_cu=0, per_degree=1, xcd=1, graph=1, tile=2),
+    dict(order=0, blocks_per_cu=0, per_degree=0, xcd=0, graph=0, tile=4),
+], ids=["tile_xcd", "degree_persistent_perdeg_eager", "persistent2", "elemtile2_interleaved_xcd",
+        "elemtile4_eager"])
 def test_av_schedules_do_not_change_results(dev, mem, ensemble_default, schedule):
     res, _ = run_ensemble(dev, mem, STEPS, schedule=schedule)
     for i in range(len(mem)):
