@@ -421,7 +421,9 @@ __device__ __forceinline__ double sensor_rows(int Nelem, int e0, const SensorCon
     }
     __syncthreads();
     if (act) {
-        // row a = r of m, in place of row r of T (only this thread reads and writes that row)
+        // row a = r of m, in place of row r of T (only this thread reads and writes that row); only
+        // the squares of the modes enter the sums, so the row's squares are stored (every lane
+        // squares its own row, and the four sums below only add)
         double t[NGL];
 #pragma unroll
         for (int l = 0; l < NGL; ++l) t[l] = T[r * RB::RS + l];
@@ -430,11 +432,11 @@ __device__ __forceinline__ double sensor_rows(int Nelem, int e0, const SensorCon
             double s = 0.0;
 #pragma unroll
             for (int l = 0; l < NGL; ++l) s += t[l] * sV[c * NGL + l];
-            T[r * RB::RS + c] = s;
+            T[r * RB::RS + c] = s * s;
         }
     }
     __syncthreads();
-    if (act && r < 4) sSum[el][r] = sensor_sum<NGL, RB::RS>(T, r);
+    if (act && r < 4) sSum[el][r] = sensor_sum<NGL, RB::RS, true>(T, r);
     __syncthreads();
     double v = 0.0;
     if (act && r == 0) {

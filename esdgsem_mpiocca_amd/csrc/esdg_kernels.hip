@@ -485,11 +485,11 @@ __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double 
                 double acc = 0.0;
 #pragma unroll
                 for (int l = 0; l < NGL; ++l) acc += t[l] * sV[c * NGL + l];
-                Tm[r * NGL + c] = acc;
+                Tm[r * NGL + c] = acc * acc;   // the sums below take the squares of the modes only
             }
         }
         __syncthreads();
-        if (act && r < 4) sSum[el][r] = sensor_sum<NGL, NGL>(Tm, r);
+        if (act && r < 4) sSum[el][r] = sensor_sum<NGL, NGL, true>(Tm, r);
         __syncthreads();
         if (act && r == 0) {
             const double v = sensor_ramp(sSum[el][0], sSum[el][1], sSum[el][2], sSum[el][3], nx.sc);

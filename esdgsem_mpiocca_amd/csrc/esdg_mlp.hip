@@ -91,7 +91,8 @@ __device__ void row_features(const double* __restrict__ q, const double* __restr
     for (int k = 0; k < 8; ++k) sdev = fmax(sdev, L.red[el][k]);
 #pragma unroll
     for (int i = 0; i < 8; ++i) xn[i] = (sdev > 0.0) ? (float)(dev[i] / sdev) : 0.0f;
-    // row r of m = T V^T (sum over eta), in place of row r of T (only this thread touches it)
+    // row r of m = T V^T (sum over eta), in place of row r of T (only this thread touches it):
+    // only the squares of the modes are used from here on, so the row's squares are stored
     {
         double t[8];
 #pragma unroll
@@ -101,7 +102,7 @@ __device__ void row_features(const double* __restrict__ q, const double* __restr
             double m = 0.0;
 #pragma unroll
             for (int l = 0; l < 8; ++l) m += L.V[c * 8 + l] * t[l];
-            T[r * FRS + c] = m;
+            T[r * FRS + c] = m * m;
         }
     }
     __syncthreads();
@@ -109,10 +110,10 @@ __device__ void row_features(const double* __restrict__ q, const double* __restr
     double Ek = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; ++c)
-        if (c <= r) Ek += T[r * FRS + c] * T[r * FRS + c];
+        if (c <= r) Ek += T[r * FRS + c];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
-        if (a < r) Ek += T[a * FRS + r] * T[a * FRS + r];
+        if (a < r) Ek += T[a * FRS + r];
     L.red[el][r] = Ek;
     __syncthreads();
     double Et = 0.0;
@@ -232,9 +233,10 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, const float* __restric
 // given, the class: 0 smooth (p <= lo), 1 troubled (p >= hi), 2 unsure.  A block takes 32 elements:
 // features by row threads into two 16-row BF16 tiles, then waves 0 and 1 run one tile each.
 // With MS.visc (and estat), the block then finishes ShockCapturing masked by the classes: the
-// feature pass left the modal coefficients m = V (h + b) V^T in LDS, formed with the sensor's
-// arithmetic (ShockCapturing.okl:63-129), so four threads per element form the energy sums from
-// them in the okl's order; the viscosity is the sensor's value, or 0 where the class is smooth.
+// feature pass left the squares of the modal coefficients m = V (h + b) V^T in LDS, formed with
+// the sensor's arithmetic (ShockCapturing.okl:63-129), so four threads per element form the energy
+// sums from them in the okl's order; the viscosity is the sensor's value, or 0 where the class is
+// smooth.
 // counters (may be null): [0] element-stages with viscosity, [1] sensor positives declared smooth
 // and [2] sensor positives (both with audit only), [3] element-stages not declared smooth.
 struct MaskedSensor {
@@ -255,36 +257,44 @@ struct MaskedSensor {
 };
 
 // The indicator over elements e0 .. e0 + FEPB - 1 of one state (all 256 threads of the block).
+// LDS holds the feature pass's tiles (V, the modal coefficients, the row reductions); once the
+// sensor's energy sums are taken from the coefficients, the two 16-row BF16 feature tiles and the
+// hidden activations take their place.  The network's weights are read from global memory (8 KB,
+// cached), not staged per block: about 22 KB of LDS per block, 7 blocks per CU.
 __device__ void mlp_indicator_block(int Nelem, int e0, const double* __restrict__ q, const double* __restrict__ B,
                                     const double* __restrict__ V, const MlpDev& W, float lo, float hi,
                                     float* __restrict__ p, signed char* __restrict__ estat, const MaskedSensor& MS) {
     if (MS.counters && threadIdx.x == 0)
         atomicAdd(MS.counters + 4, (unsigned long long)min(FEPB, Nelem - e0));
-    __shared__ __attribute__((aligned(16))) uint16_t sW1[NH * KP], sW2[NH * NH];
-    __shared__ float sw3[NH], sb[2 * NH + 1];
-    __shared__ __attribute__((aligned(16))) uint16_t sX[2][16 * KP], sH[2][16 * NH];
+    __shared__ __attribute__((aligned(16))) FeatLds L;
     __shared__ float sZ[2][16];
-    __shared__ FeatLds L;
-    load_weights(W, sW1, sW2, sw3, sb);
+    __shared__ signed char sSt[FEPB];
+    __shared__ double sSum[FEPB][4];
+    static_assert(sizeof(L.T) >= (2 * 16 * KP + 2 * 16 * NH) * sizeof(uint16_t), "feature tiles fit the T area");
+    // V is published by the first barrier inside row_features
     if (threadIdx.x < 64) L.V[threadIdx.x] = V[threadIdx.x];
-    for (int t = threadIdx.x; t < 2 * 16 * KP; t += 256) (&sX[0][0])[t] = 0;
-    __syncthreads();
     const int el = threadIdx.x >> 3, r = threadIdx.x & 7;
     const int e = e0 + el;
     const bool act = e < Nelem;
     float xn[8], modal;
     row_features(q, B, L, e, act, xn, modal);
-    if (act) {
-        uint16_t* row = sX[el >> 4] + (el & 15) * KP;
+    // row_features leaves the squares of the modal coefficients m = V (h + b) V^T in L.T: the
+    // sensor's sums
+    if (MS.visc && act && r < 4) sSum[el][r] = sensor_sum<8, FRS, true>(L.T + el * FES, r);
+    __syncthreads();   // L.T is free
+    uint16_t* sX = reinterpret_cast<uint16_t*>(L.T);   // [2][16][KP]
+    uint16_t* sH = sX + 2 * 16 * KP;                   // [2][16][NH]
+    {
+        uint16_t* row = sX + (el >> 4) * 16 * KP + (el & 15) * KP;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) row[r * 8 + i] = f2bf(xn[i]);
-        row[64 + r] = f2bf(modal);
+        for (int i = 0; i < 8; ++i) row[r * 8 + i] = act ? f2bf(xn[i]) : (uint16_t)0;
+        row[64 + r] = act ? f2bf(modal) : (uint16_t)0;
+        for (int k = NIN + r; k < KP; k += 8) row[k] = 0;   // the K padding of layer 1
     }
     __syncthreads();
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    __shared__ signed char sSt[FEPB];
     if (w < 2) {
-        mlp_tile(sX[w], sH[w], sW1, sW2, sw3, sb, sb + NH, sb[2 * NH], sZ[w]);
+        mlp_tile(sX + w * 16 * KP, sH + w * 16 * NH, W.W1T, W.W2T, W.w3, W.bias, W.bias + NH, W.bias[2 * NH], sZ[w]);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         const int ee = e0 + w * 16 + l;
         if (l < 16 && ee < Nelem) {
@@ -296,14 +306,11 @@ __device__ void mlp_indicator_block(int Nelem, int e0, const double* __restrict_
         }
     }
     if (MS.visc) {
-        __shared__ double sSum[FEPB][4];
         // the block's counts and largest viscosity gather in LDS; one global atomic each at the end
         __shared__ unsigned long long sCnt[4], sMax;
         if (threadIdx.x < 4) sCnt[threadIdx.x] = 0ull;
         if (threadIdx.x == 4) sMax = 0ull;
         __syncthreads();   // the classes are in sSt
-        if (act && r < 4) sSum[el][r] = sensor_sum<8, FRS>(L.T + el * FES, r);
-        __syncthreads();
         if (act && r == 0) {
             double v = sensor_ramp(sSum[el][0], sSum[el][1], sSum[el][2], sSum[el][3], MS.sc);
             const int st = sSt[el];

@@ -321,9 +321,9 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             }
             // ShockCapturing.okl:63-217 on the updated state, the next stage's viscosity, with the
             // arithmetic of k_shock_capturing: lane r forms column r of T = V (h + b) from row r of
-            // h + b, then row r of the modes m = T V^T in place; four lanes form the energy sums.
-            // Plane 3 (1/h) is free after the column phase and holds T and then m.  The side
-            // traces above are the last reader of planes 0-2.
+            // h + b, then row r of the squared modes (T V^T)^2 in place; four lanes form the
+            // energy sums.  Plane 3 (1/h) is free after the column phase and holds T and then the
+            // squares.  The side traces above are the last reader of planes 0-2.
             if (si.visc_next) {
                 __shared__ double sSum[C::EPW][4];
                 double* Tm = S + 3 * C::PLANE;
@@ -345,24 +345,18 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                     double t[NGL];
 #pragma unroll
                     for (int l = 0; l < NGL; ++l) t[l] = Tm[r * C::RS + l];
+                    // the squares of the modes, one row per lane (only they enter the sums); then
+                    // the four sums add them in the okl's order
 #pragma unroll
                     for (int c = 0; c < NGL; ++c) {
                         double acc = 0.0;
 #pragma unroll
                         for (int l = 0; l < NGL; ++l) acc += t[l] * V[c * NGL + l];
-                        Tm[r * C::RS + c] = acc;
+                        Tm[r * C::RS + c] = acc * acc;
                     }
                 }
                 __syncthreads();
-                // the squares of the modes, one row per lane, into plane 0 (the new state there is
-                // no longer needed); then the four sums add them in the okl's order
-                double* Sq = S;
-                if (act) {
-#pragma unroll
-                    for (int c = 0; c < NGL; ++c) Sq[r * C::RS + c] = Tm[r * C::RS + c] * Tm[r * C::RS + c];
-                }
-                __syncthreads();
-                if (act && r < 4) sSum[el][r] = sensor_sum<NGL, C::RS, true>(Sq, r);
+                if (act && r < 4) sSum[el][r] = sensor_sum<NGL, C::RS, true>(Tm, r);
                 __syncthreads();
                 if (act && r == 0) {
                     const double v = sensor_ramp(sSum[el][0], sSum[el][1], sSum[el][2], sSum[el][3], si.sc);
