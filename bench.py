@@ -51,19 +51,22 @@ def volume_flops(nelem, ngl):
 # Yxi/Yeta/Xxi/Xeta 4, 1/J, Bx, By, gRK 3 read and gRK 3, q 3 written (19 doubles); per element
 # side node F* 3 [+ viscous face flux 2] read.  (The viscous volume term, read only where the
 # viscosity is non-zero, is not counted.)
-def fused_volume_bytes(nelem, ngl, av, traces=True, sensor=False, stages=5):
+def fused_volume_bytes(nelem, ngl, av, traces=True, sensor=False, stages=5, limiter=False):
     """Average bytes per launch.  Per node: q 3, Yxi/Yeta/Xxi/Xeta 4, 1/J, Bx, By, gRK 3 read, gRK 3
-    and q 3 written; per element side node: F* 3 read, and without the limiter the side traces of
-    the new state (3) written.  Without the limiter the pass also evaluates, on the new state, the
-    sensor when `sensor` (b read per node; the viscosity and its plot copy written per element) and,
-    in the last of the step's `stages`, the element wave speed (one double per element).  The
-    viscous terms (with `av`) are read only for the elements with viscosity in reach, a few percent
-    of them, and are not counted."""
+    and q 3 written; per element side node: F* 3 read, and with `traces` the side traces of the new
+    state (3) written.  With `traces` the pass also evaluates, on the new state, the sensor when
+    `sensor` (b read per node; the viscosity and its plot copy written per element) and, in the
+    last of the step's `stages`, the element wave speed (one double per element).  With `limiter`
+    the new state is limited before it is stored (the element size read per element; the GLL
+    weights are 8 ngl bytes per launch).  The viscous terms (with `av`) are read only for the
+    elements with viscosity in reach, a few percent of them, and are not counted."""
     per_elem = 19 * ngl * ngl + (12 + (12 if traces else 0)) * ngl
     if traces and sensor:
         per_elem += ngl * ngl + 2
     if traces:
         per_elem += 1.0 / stages
+    if limiter:
+        per_elem += 1
     return int(round(8 * nelem * per_elem))
 
 
@@ -325,16 +328,6 @@ def ensemble_av_bench(dev, args):
     return out
 
 
-def pos_pres_bytes(nelem, ngl, sensor=True, stages=5):
-    """Average algorithmic bytes per launch of the fused schedule's limiter pass (PosPres.okl plus
-    the side traces, sensor and wave speed of the limited state): per node q 3, 1/J and (for the
-    sensor) b read; per element the side traces (12 ngl) written, EleSizes read, the viscosity and
-    its plot copy written and, in the last of the step's stages, the wave speed.  The state is
-    rewritten only where the limiter changes it (data dependent, not counted)."""
-    per_elem = (4 + (1 if sensor else 0)) * ngl * ngl + 12 * ngl + 1 + (2 if sensor else 0) + 1.0 / stages
-    return int(round(8 * nelem * per_elem))
-
-
 def indicator_comparison(dev, args, spec, rp_kw, steps, modes):
     """Plain modal sensor against the learned indicator (modal-sensor fallback) on one case built on
     the device (esdg_device_case_create, handed to each solver device to device).  modes: "plain"
@@ -453,13 +446,12 @@ def c4_bench(dev, args, steps):
     out = indicator_comparison(dev, args, spec, {"positivity": 1, "pos_pres_tol": 0.1, "CFL": 0.1}, steps,
                                ("plain", "learned", "audit", "control"))
     nelem, ngl, N, dofs = out.pop("_dcase")
-    # rooflines of the plain run's two dominant kernels (HIP events around every launch of the
-    # profiled chunk): the fused volume pass (no side traces or sensor: the limiter pass writes
-    # them) and the limiter pass
+    # roofline of the plain run's dominant kernel (HIP events around every launch of the profiled
+    # chunk): the fused volume pass, which limits the new state before it stores it and writes the
+    # side traces, the sensor and the wave speed of the limited state
     pk = out["plain"]["kernels"]
     roof = {}
-    for name, alg in (("fused_volume", fused_volume_bytes(nelem, ngl, 1, traces=False)),
-                      ("pos_pres", pos_pres_bytes(nelem, ngl))):
+    for name, alg in (("fused_volume", fused_volume_bytes(nelem, ngl, 1, traces=True, sensor=True, limiter=True)),):
         if name in pk:
             gbs = alg / (pk[name]["avg_us"] * 1e-6) / 1e9
             roof[name] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

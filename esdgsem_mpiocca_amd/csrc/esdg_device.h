@@ -327,19 +327,6 @@ struct SensorConsts {
     double ZeroTOL, eps0, sigmaMin, sigmaMax;
 };
 
-// Outputs a state-writing kernel may add for the next stage / time step (each may be null):
-// ShockCapturing's element viscosity (and its plot copy) and FindLambdaMax's element wave speed.
-struct NextEval {
-    double* visc = nullptr;
-    double* visc_plot = nullptr;
-    const double* B = nullptr;
-    const double* V = nullptr;
-    SensorConsts sc{};
-    double* lam = nullptr;
-    double g = 0.0;
-    const double* dtp = nullptr;   // the device clock's dt: a kernel given dtp does nothing when it is 0
-};
-
 // One of the four modal energy sums of ShockCapturing.okl:145-196, in the okl's serial order:
 // which = 0: zaehler1 (top shell), 1: zaehler2 (second shell), 2: nenner1 (all), 3: nenner2 (below N).
 // RS: row stride of the mode array (m[i][j] at m[i * RS + j]).  SQUARED: m holds the squares of

@@ -17,8 +17,8 @@ struct FusedArgs {
     // element viscosity of the stage: the caller evaluates it (the modal sensor, or the learned
     // indicator with the sensor) on the stage's state before the stage runs
     const double* ViscPara;
-    // Optional: the state update (or, with positivity, the limiter after it) also evaluates, on the
-    // state it writes, the modal sensor into next_visc / next_visc_plot (may be ViscPara itself:
+    // Optional: the state update (with positivity, after limiting the new state) also evaluates, on
+    // the state it writes, the modal sensor into next_visc / next_visc_plot (may be ViscPara itself:
     // each element's value is read before it is replaced) and the element wave speeds into
     // next_lambda.
     double *next_visc = nullptr, *next_visc_plot = nullptr, *next_lambda = nullptr;

@@ -34,9 +34,6 @@ namespace esdg {
 int fused_volume(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA, double cdt, const double* fstar, double* Qt,
                  const double* Qv, const double* SPV, double* tr, const int* slot, const int* active = nullptr,
                  const int* list = nullptr, const int* acount = nullptr);
-int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
-                    double* tr, const int* slot, double* next_visc, double* next_visc_plot, const double* B,
-                    const double* VdmInv, double* next_lambda, const double* dt_dev);
 int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr, const int* slot);
 int trace_slots(esdg_ctx* ctx, int Nelem, const int* E2E, const int* ED, int* slot);
 
@@ -319,14 +316,9 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         if (rc) return rc;
     }
     if (!av) {
-        // volume + surface + RK in one element pass; the limiter (if on) follows as its own pass
-        // with the limiter on, the limiter writes the traces of the limited state
-        double* vtr = A.positivity ? nullptr : tr;
-        int rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr, vtr, slot);
-        if (!rc && A.positivity)
-            rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, slot, A.next_visc, A.next_visc_plot,
-                                 A.B, A.VdmInv, A.next_lambda, A.dt_dev);
-        return rc;
+        // volume + surface + RK (+ the limiter on the new state) in one element pass, which writes
+        // the side traces of the state it stores
+        return fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, nullptr, nullptr, tr, slot);
     }
     int rc = ESDG_OK;
     {
@@ -344,13 +336,8 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         rc = launch_check(ctx, "fused_viscous_face");
         if (rc) return rc;
     }
-    // volume + surface + viscous terms + RK in one element pass, then the limiter
-    rc = fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, A.positivity ? nullptr : tr, slot, active,
-                      alist, acount);
-    if (!rc && A.positivity)
-        rc = pos_pres_traces(ctx, A.Nelem, A.EleSizes, A.GLw, A.Jac, A.q, tr, slot, A.next_visc, A.next_visc_plot,
-                             A.B, A.VdmInv, A.next_lambda, A.dt_dev);
-    return rc;
+    // volume + surface + viscous terms + RK (+ the limiter on the new state) in one element pass
+    return fused_volume(ctx, A, true, rkA, cdt, fstar, nullptr, qvisc, fvisc, tr, slot, active, alist, acount);
 }
 
 void FusedStage::release(esdg_ctx* ctx) {

@@ -344,17 +344,13 @@ __global__ void __launch_bounds__(256) k_shock_capturing(int Nelem, SensorConsts
 // ---------------------------------------------------------------------------------------------
 // One thread per element row (eta = r), 256 / NGL elements per block.  The element means run on
 // three threads (h, hu, hv), each over the node products in the okl's serial order (xi outer, eta
-// inner); the products h w, w = GLw[xi] GLw[eta] / Jinv, are formed by the row threads.  With tr
-// (the fused schedule), the side traces of the limited state are written as well, from LDS and
-// lane-consecutively.  With nx.visc / nx.lam (the fused schedule), ShockCapturing and
-// FindLambdaMax of the limited state follow, with the arithmetic of k_shock_capturing and
-// k_lambda_max, so that the next stage and time step need not run them.
+// inner); the products h w, w = GLw[xi] GLw[eta] / Jinv, are formed by the row threads.  (The
+// fused schedule limits the new state inside its update pass with the same arithmetic,
+// esdg_volume.hip.)
 template <int NGL>
 __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double zero_tol,
                                                   const double* __restrict__ EleSizes, const double* __restrict__ GLw,
-                                                  const double* __restrict__ Jac, double* __restrict__ q,
-                                                  double* __restrict__ tr, const int* __restrict__ slot, NextEval nx) {
-    if (nx.dtp && *nx.dtp == 0.0) return;   // a step issued after T (device clock): nothing changes
+                                                  const double* __restrict__ Jac, double* __restrict__ q) {
     using RB = RowBlock<NGL>;
     constexpr int NN = NGL * NGL;
     __shared__ double sP[3][RB::EPB * NN];
@@ -415,107 +411,13 @@ __global__ void __launch_bounds__(256) k_pos_pres(int Nelem, double tol, double 
                     }
                 }
                 if (ha < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
-                h[i] = hn; hu[i] = hun; hv[i] = hvn;
                 q[qb + i] = hn;
                 q[qb + NN + i] = hun;
                 q[qb + 2 * NN + i] = hvn;
             } else if (h[i] < tol) {
-                hu[i] = 0.0; hv[i] = 0.0;
                 q[qb + NN + i] = 0.0;
                 q[qb + 2 * NN + i] = 0.0;
             }
-        }
-    }
-    if (tr) {
-        __syncthreads();   // the products are no longer read
-        if (act) {
-#pragma unroll
-            for (int i = 0; i < NGL; ++i) {
-                P0[r * NGL + i] = h[i];
-                P1[r * NGL + i] = hu[i];
-                P2[r * NGL + i] = hv[i];
-            }
-        }
-        __syncthreads();
-        // thread (el, r) writes side node r of every (side, v) plane of its element into the side's
-        // face slot
-        if (act) {
-#pragma unroll
-            for (int side = 0; side < 4; ++side) {
-                double* T = tr + (int64_t)slot[e * 4 + side] * 3 * NGL + r;
-#pragma unroll
-                for (int v = 0; v < 3; ++v) {
-                    const int row = (side == 0) ? 0 : (side == 2) ? NGL - 1 : r;
-                    const int col = (side == 1) ? NGL - 1 : (side == 3) ? 0 : r;
-                    T[v * NGL] = sP[v][el * NN + row * NGL + col];
-                }
-            }
-        }
-    }
-    if (nx.visc) {
-        // modal sensor of the limited row values: T = V (h + b) column r, then row r of m = T V^T,
-        // in plane 1 (unpadded rows), as k_shock_capturing does
-        __shared__ double sV[NN];
-        __syncthreads();   // the planes are no longer read
-        if (threadIdx.x < NN) sV[threadIdx.x] = nx.V[threadIdx.x];
-        double* Tm = sP[1] + el * NN;
-        double u[NGL];
-        if (act) {
-            const int64_t xb = (int64_t)e * NN + r * NGL;
-#pragma unroll
-            for (int i = 0; i < NGL; ++i) u[i] = nx.B[xb + i] + h[i];
-        }
-        __syncthreads();
-        if (act) {
-#pragma unroll
-            for (int a = 0; a < NGL; ++a) {
-                double acc = 0.0;
-#pragma unroll
-                for (int l = 0; l < NGL; ++l) acc += sV[a * NGL + l] * u[l];
-                Tm[a * NGL + r] = acc;
-            }
-        }
-        __syncthreads();
-        if (act) {
-            double t[NGL];
-#pragma unroll
-            for (int l = 0; l < NGL; ++l) t[l] = Tm[r * NGL + l];
-#pragma unroll
-            for (int c = 0; c < NGL; ++c) {
-                double acc = 0.0;
-#pragma unroll
-                for (int l = 0; l < NGL; ++l) acc += t[l] * sV[c * NGL + l];
-                Tm[r * NGL + c] = acc * acc;   // the sums below take the squares of the modes only
-            }
-        }
-        __syncthreads();
-        if (act && r < 4) sSum[el][r] = sensor_sum<NGL, NGL, true>(Tm, r);
-        __syncthreads();
-        if (act && r == 0) {
-            const double v = sensor_ramp(sSum[el][0], sSum[el][1], sSum[el][2], sSum[el][3], nx.sc);
-            nx.visc[e] = v;
-            if (nx.visc_plot) nx.visc_plot[e] = v;
-        }
-    }
-    if (nx.lam) {
-        // FindLambdaMax.okl:41-60 over the limited row, then the element maximum (order-free)
-        double lam = 0.0;
-        if (act) {
-#pragma unroll
-            for (int i = 0; i < NGL; ++i) {
-                const double Qinv = (h[i] > tol) ? 1.0 / h[i] : 0.0;
-                const double c = sqrt(nx.g * h[i]);
-                lam = fmax(lam, c + fmax(fabs(hu[i] * Qinv), fabs(hv[i] * Qinv)));
-            }
-        }
-        __syncthreads();   // sMin is no longer read
-        if (act) sMin[el][r] = lam;
-        __syncthreads();
-        if (act && r == 0) {
-            double m = 0.0;
-#pragma unroll
-            for (int k = 0; k < NGL; ++k) m = fmax(m, sMin[el][k]);
-            nx.lam[e] = m;
         }
     }
 }
@@ -765,38 +667,16 @@ void launch_sensor(esdg_ctx* ctx, int Nelem, const double* q, const double* B, c
     k_shock_capturing<NGL><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(Nelem, sc, q, B, V, visc, vp);
 }
 template <int NGL>
-void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q,
-                    double* tr, const int* slot, NextEval nx) {
+void launch_pospres(esdg_ctx* ctx, int Nelem, const double* sizes, const double* GLw, const double* Jac, double* q) {
     using RB = RowBlock<NGL>;
     k_pos_pres<NGL><<<blocks_for(Nelem, RB::EPB), RB::THREADS, 0, ctx->stream>>>(Nelem, ctx->p.PosPresTOL, ctx->p.ZeroTOL,
-                                                                                  sizes, GLw, Jac, q, tr, slot, nx);
+                                                                                  sizes, GLw, Jac, q);
 }
 template <int NGL>
 void launch_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr, const int* slot) {
     k_write_traces<NGL><<<blocks_for((int64_t)Nelem * NGL * NGL, 256), 256, 0, ctx->stream>>>(Nelem, q, tr, slot);
 }
 
-// limiter that also refreshes the side traces (fused schedule) and, when asked, evaluates the
-// sensor and the wave speeds of the limited state
-int pos_pres_traces(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double* GLw, const double* Jac, double* q,
-                    double* tr, const int* slot, double* next_visc, double* next_visc_plot, const double* B,
-                    const double* VdmInv, double* next_lambda, const double* dt_dev) {
-    NextEval nx{};
-    nx.dtp = dt_dev;
-    if (next_visc) {
-        nx.visc = next_visc;
-        nx.visc_plot = next_visc_plot;
-        nx.B = B;
-        nx.V = VdmInv;
-        nx.sc = SensorConsts{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
-    }
-    nx.lam = next_lambda;
-    nx.g = ctx->p.g;
-    KernelScope ks(ctx, "pos_pres");
-    if (tr && !slot) return set_error(ctx, ESDG_ERR_STATE, "pos_pres: side traces need the face slots");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, tr, slot, nx);
-    return launch_check(ctx, "pos_pres");
-}
 int write_traces(esdg_ctx* ctx, int Nelem, const double* q, double* tr, const int* slot) {
     KernelScope ks(ctx, "write_traces");
     ESDG_NGL_DISPATCH(ctx->p.ngl, launch_traces, ctx, Nelem, q, tr, slot);
@@ -979,7 +859,7 @@ int esdg_pos_pres(esdg_ctx* ctx, int Nelem, const double* EleSizes, const double
     ESDG_REQUIRE(ctx, Nelem >= 0 && EleSizes && GLw && Jac && q, "esdg_pos_pres: bad argument");
     if (Nelem == 0) return ESDG_OK;
     ESDG_ENTRY("pos_pres");
-    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q, nullptr, nullptr, NextEval{});
+    ESDG_NGL_DISPATCH(ctx->p.ngl, launch_pospres, ctx, Nelem, EleSizes, GLw, Jac, q);
     return launch_check(ctx, "pos_pres");
 }
 

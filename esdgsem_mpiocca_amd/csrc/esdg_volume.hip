@@ -62,6 +62,13 @@ struct SurfIn {
     // issued after T) does nothing
     const double* dtp;
     double Cs;
+    // RK modes: the positivity limiter (PosPres) on the updated state before it is stored, with
+    // the element sizes, the GLL weights and ZeroTOL; the side traces, sensor and wave speed are
+    // then those of the limited state
+    int limit;
+    const double* EleSizes;
+    const double* GLw;
+    double zero_tol;
 };
 
 template <int NGL, int MODE>
@@ -145,6 +152,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
 
     // ---- column phase: xi-column r, nodes (k, r) -------------------------------------------
     double b1[NGL], b2[NGL], b3[NGL];
+    double rJ[NGL];
     if (act) {
         const int64_t xb = (int64_t)e * C::NN + r;
 #pragma unroll
@@ -180,7 +188,6 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         line_sweep<NGL, true>(D, g, h, hu, hv, u, v, Y, X, b1, b2, b3);
         const int64_t xb = (int64_t)e * C::NN + r;
         const int64_t ob = (int64_t)e * C::NN * ESDG_NEQ + r;
-        double rJ[NGL];
 #pragma unroll
         for (int k = 0; k < NGL; ++k) {
             unscale_sums(b1[k], b2[k], b3[k]);
@@ -255,6 +262,8 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
 #pragma unroll
                 for (int i = 0; i < NGL; ++i) bsen[i] = si.B[bb + i];
             }
+            const double gx = si.limit ? si.GLw[r] : 0.0;
+            double mn = 0.0;
 #pragma unroll
             for (int k = 0; k < NGL; ++k) {
                 const int64_t id = ob + k * NGL;
@@ -270,6 +279,16 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 const double n1 = h[k] + cdt * g1;
                 const double n2 = hu[k] + cdt * g2;
                 const double n3 = hv[k] + cdt * g3;
+                if (si.limit) {
+                    // to the limiter below: the new state to planes 0-2, its weights
+                    // w = GLw[xi] GLw[eta] / Jac to plane 3, the column minimum of h
+                    S[0 * C::PLANE + k * C::RS + r] = n1;
+                    S[1 * C::PLANE + k * C::RS + r] = n2;
+                    S[2 * C::PLANE + k * C::RS + r] = n3;
+                    S[3 * C::PLANE + k * C::RS + r] = gx * si.GLw[k] / rJ[k];
+                    mn = fmin(mn, n1);
+                    continue;
+                }
                 q[id] = n1;
                 q[id + C::NN] = n2;
                 q[id + 2 * C::NN] = n3;
@@ -287,6 +306,8 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                     lam = fmax(lam, c + fmax(fabs(n2 * Qinv), fabs(n3 * Qinv)));
                 }
             }
+            // the column minimum in the padding slot of row r of plane 3
+            if (si.limit) S[3 * C::PLANE + r * C::RS + NGL] = mn;
         } else {
 #pragma unroll
             for (int k = 0; k < NGL; ++k) {
@@ -297,6 +318,66 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         }
     }
     if constexpr (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
+        if (si.limit) {
+            // PosPres (okl/Positivity/PosPres.okl:17-223) on the new state before it is stored,
+            // with the arithmetic of k_pos_pres: lane v < 3 of an element sums the products
+            // h w (hu w, hv w) of plane v and plane 3 in the okl's serial order (xi outer, eta
+            // inner) into the padding slot of row 0 of plane v; then every lane limits its column
+            __syncthreads();
+            if (act && r < 3) {
+                const double* P = S + r * C::PLANE;
+                const double* W = S + 3 * C::PLANE;
+                double a = 0.0;
+                for (int l = 0; l < NGL; ++l)
+                    for (int k = 0; k < NGL; ++k) a += P[k * C::RS + l] * W[k * C::RS + l];
+                S[r * C::PLANE + NGL] = a;
+            }
+            __syncthreads();
+            if (act) {
+                double hmin = 0.0;
+#pragma unroll
+                for (int k = 0; k < NGL; ++k) hmin = fmin(hmin, S[3 * C::PLANE + k * C::RS + NGL]);
+                const double size = si.EleSizes[e];
+                double ha = S[0 * C::PLANE + NGL] / size;
+                if (fabs(ha) < si.zero_tol) ha = 0.0;
+                const double hua = S[1 * C::PLANE + NGL] / size, hva = S[2 * C::PLANE + NGL] / size;
+                const double th = (ha == hmin) ? 1.0 : fmin(1.0, ha / (ha - hmin));
+                const int64_t ob = (int64_t)e * C::NN * ESDG_NEQ + r;
+#pragma unroll
+                for (int k = 0; k < NGL; ++k) {
+                    double n1 = S[0 * C::PLANE + k * C::RS + r];
+                    double n2 = S[1 * C::PLANE + k * C::RS + r];
+                    double n3 = S[2 * C::PLANE + k * C::RS + r];
+                    if (hmin < 0.0) {
+                        double hn = 0.0, hun = 0.0, hvn = 0.0;
+                        if (ha > 0.0) {
+                            hn = fabs(th * (n1 - ha) + ha);
+                            if (hn > si.tol) {
+                                hun = th * (n2 - hua) + hua;
+                                hvn = th * (n3 - hva) + hva;
+                            }
+                        }
+                        if (ha < 0.0) { hn = -INFINITY; hun = 0.0; hvn = 0.0; }
+                        n1 = hn; n2 = hun; n3 = hvn;
+                    } else if (n1 < si.tol) {
+                        n2 = 0.0; n3 = 0.0;
+                    }
+                    const int64_t id = ob + k * NGL;
+                    q[id] = n1;
+                    q[id + C::NN] = n2;
+                    q[id + 2 * C::NN] = n3;
+                    // the limited state in this lane's column of the planes, for the side traces
+                    S[0 * C::PLANE + k * C::RS + r] = n1;
+                    S[1 * C::PLANE + k * C::RS + r] = n2;
+                    S[2 * C::PLANE + k * C::RS + r] = n3;
+                    if (si.lam) {
+                        const double Qinv = (n1 > si.tol) ? 1.0 / n1 : 0.0;
+                        const double c = sqrt(g * n1);
+                        lam = fmax(lam, c + fmax(fabs(n2 * Qinv), fabs(n3 * Qinv)));
+                    }
+                }
+            }
+        }
         double* sV = sM;
         if (si.visc_next)
             for (int t = lane; t < C::NN; t += 64) sV[t] = si.VdmInv[t];
@@ -453,6 +534,14 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
     si.acount = acount;
     si.dtp = a.dt_dev;
     si.Cs = a.C_stage;
+    if (rk) {
+        // PosPres on the new state before it is stored (okl/Positivity/PosPres.okl)
+        si.tol = ctx->p.PosPresTOL;
+        si.limit = a.positivity;
+        si.EleSizes = a.EleSizes;
+        si.GLw = a.GLw;
+        si.zero_tol = ctx->p.ZeroTOL;
+    }
     if (rk && tr) {
         // the updated state's viscosity and wave speed, for the next stage or time step
         si.visc_next = a.next_visc;
@@ -461,7 +550,6 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
         si.VdmInv = a.VdmInv;
         si.sc = SensorConsts{ctx->p.ZeroTOL, ctx->p.eps0, ctx->p.sigmaMin, ctx->p.sigmaMax};
         si.lam = a.next_lambda;
-        si.tol = ctx->p.PosPresTOL;
     }
     if (rk && Qv) {
         // the listed marked elements on the side stream, beside the unmarked ones on the main stream

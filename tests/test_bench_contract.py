@@ -26,8 +26,11 @@ def test_fused_volume_bytes_counts():
     assert bench.fused_volume_bytes(1, 8, 1, True, stages=5) == round(8 * (19 * 64 + 24 * 8 + 0.2))
     # the sensor adds b per node and the viscosity and its plot copy per element
     assert bench.fused_volume_bytes(1, 8, 1, True, sensor=True, stages=5) == round(8 * (20 * 64 + 24 * 8 + 2.2))
-    # with the limiter the update writes no traces and evaluates nothing of the next state
+    # without side traces the update evaluates nothing of the next state
     assert bench.fused_volume_bytes(1, 8, 1, False, sensor=True) == 8 * (19 * 64 + 12 * 8)
+    # the limiter inside the update reads the element size
+    assert bench.fused_volume_bytes(1, 8, 1, True, sensor=True, stages=5, limiter=True) == \
+        round(8 * (20 * 64 + 24 * 8 + 3.2))
 
 
 def test_committed_traffic_matches_the_reported_workload():
