@@ -115,6 +115,28 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         if (si.list && act && si.active[e] == seq) act = false;
     }
     double* S = s + (act ? el : 0) * C::ESZ;
+    // The element's face indices (ElemToEdge, EdgeOrientation, EdgeMasterSlave of its 4 sides) go
+    // to padding slots of its LDS planes (column NGL of every row, which no sweep touches; two ints
+    // per slot, planes 3, 2, ... in turn), loaded beside the row data so that the surface lift
+    // needs no dependent global round trip before its face-flux reads.  The epilogue reuses the
+    // slots only after the lift.
+    auto fidx = [&](int i) -> int& {
+        const int d = i >> 1;
+        return reinterpret_cast<int*>(S + (3 - d / NGL) * C::PLANE + (d % NGL) * C::RS + NGL)[i & 1];
+    };
+    static_assert(4 * NGL * 2 >= 12, "face indices fit the padding slots");
+    constexpr int FPL = (4 + NGL - 1) / NGL;   // sides per lane: lane r loads sides r, r + NGL, ...
+    int fE2E[FPL], fEOR[FPL], fEMS[FPL];
+#pragma unroll
+    for (int j = 0; j < FPL; ++j) {
+        const int sd = r + j * NGL;
+        fE2E[j] = 0; fEOR[j] = 0; fEMS[j] = 0;
+        if (MODE != VOL_ONLY && act && sd < 4) {
+            fE2E[j] = si.E2E[e * 4 + sd];
+            fEOR[j] = si.EOR[e * 4 + sd];
+            fEMS[j] = si.EMS[e * 4 + sd];
+        }
+    }
     double lam = 0.0;   // this column's maximum wave speed of the updated state (si.lam)
     double bsen[NGL];   // b of row r (si.visc_next)
 
@@ -144,6 +166,15 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             S[1 * C::PLANE + r * C::RS + c] = hu[c];
             S[2 * C::PLANE + r * C::RS + c] = hv[c];
             S[3 * C::PLANE + r * C::RS + c] = inv;
+        }
+#pragma unroll
+        for (int j = 0; j < FPL; ++j) {
+            const int sd = r + j * NGL;
+            if (MODE != VOL_ONLY && sd < 4) {
+                fidx(3 * sd) = fE2E[j];
+                fidx(3 * sd + 1) = fEOR[j];
+                fidx(3 * sd + 2) = fEMS[j];
+            }
         }
     }
     __syncthreads();   // sD written
@@ -212,9 +243,9 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                         else if (r == 0) { side = 3; t = k; }
                     }
                     if (side < 0) continue;
-                    const int ifa = si.E2E[e * 4 + side] - 1;
-                    const int rev = si.EOR[e * 4 + side];
-                    const int ms = si.EMS[e * 4 + side];
+                    const int ifa = fidx(3 * side) - 1;
+                    const int rev = fidx(3 * side + 1);
+                    const int ms = fidx(3 * side + 2);
                     const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
                     const int64_t fl = (int64_t)ifa * NGL * ESDG_NEQ + inc;
                     b1[k] += -ms * si.FStar[fl] * rJ[k];
@@ -245,9 +276,9 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                         else if (r == 0) { side = 3; t = k; }
                     }
                     if (side < 0) continue;
-                    const int ifa = si.E2E[e * 4 + side] - 1;
-                    const int rev = si.EOR[e * 4 + side];
-                    const int ms = si.EMS[e * 4 + side];
+                    const int ifa = fidx(3 * side) - 1;
+                    const int rev = fidx(3 * side + 1);
+                    const int ms = fidx(3 * side + 2);
                     const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
                     const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
                     b2[k] += -ms * rJ[k] * si.SPV[fl];
