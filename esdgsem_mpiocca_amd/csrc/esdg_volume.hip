@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             cdt = si.Cs * d;
         }
     }
-    __shared__ double s[C::EPW * C::ESZ];
+    __shared__ __attribute__((aligned(16))) double s[C::EPW * C::ESZ];
     // D in LDS: its entries are read with wave-uniform addresses (broadcast), which keeps the
     // 64-entry matrix out of the scalar registers (no SGPR spills into vector registers).  After
     // the column sweep the same slots take the sensor's VdmInv (the block is one wave, whose LDS
@@ -216,6 +216,34 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             b2[k] = S[1 * C::PLANE + k * C::RS + r];
             b3[k] = S[2 * C::PLANE + k * C::RS + r];
         }
+    }
+    // The RK register gRK of the wave's elements is copied into planes 0-2 (free once the row sums
+    // are read) by direct global-to-LDS loads, issued before the column sweep so that the update
+    // after it finds gRK in LDS instead of waiting for HBM.  An element's gRK is 3 NN contiguous
+    // doubles in HBM and lands contiguously at the start of its LDS area ([v][k NGL + r]); 16-byte
+    // loads need that block 16-byte aligned, and the face indices must stay clear of it (in the
+    // padding slots of plane 3): NGL 8 and 10 (at NGL 6 the copies' addressing costs the instance
+    // 50 VGPRs and a wave per SIMD).
+    constexpr int GBYTES = 3 * C::NN * 8;
+    constexpr bool kDma = (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) && NGL >= 8 && GBYTES % 16 == 0;
+    if constexpr (kDma) {
+        // the row sums' LDS reads complete before the copies overwrite those slots
+        __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+        const unsigned long long am = __ballot(act);
+#pragma unroll
+        for (int sl = 0; sl < C::EPW; ++sl) {
+            if (!((am >> (sl * NGL)) & 1ull)) continue;   // the same for every lane
+            const int es = __builtin_amdgcn_readlane(e, sl * NGL);
+            const char* src = reinterpret_cast<const char*>(si.gRK + (int64_t)es * 3 * C::NN);
+            char* dst = reinterpret_cast<char*>(s + sl * C::ESZ);
+#pragma unroll
+            for (int c = 0; c * 1024 < GBYTES; ++c)
+                if (c * 1024 + lane * 16 < GBYTES)
+                    __builtin_amdgcn_global_load_lds(src + c * 1024 + lane * 16,
+                                                     (__attribute__((address_space(3))) void*)(dst + c * 1024), 16, 0, 0);
+        }
+    }
+    if (act) {
         line_sweep<NGL, true>(D, g, h, hu, hv, u, v, Y, X, b1, b2, b3);
         const int64_t xb = (int64_t)e * C::NN + r;
         const int64_t ob = (int64_t)e * C::NN * ESDG_NEQ + r;
@@ -287,6 +315,18 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             }
         }
         if (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
+            // gRK of this column, from LDS when copied there above (all of it read before the
+            // planes take the new state)
+            double gk[3][NGL];
+            if constexpr (kDma) {
+                __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the copies have landed
+#pragma unroll
+                for (int k = 0; k < NGL; ++k) {
+                    gk[0][k] = S[0 * C::NN + k * NGL + r];
+                    gk[1][k] = S[1 * C::NN + k * NGL + r];
+                    gk[2][k] = S[2 * C::NN + k * NGL + r];
+                }
+            }
             // b of row r for the sensor below, loaded here so that its latency overlaps the update
             if (si.visc_next) {
                 const int64_t bb = (int64_t)e * C::NN + r * NGL;
@@ -298,7 +338,12 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
 #pragma unroll
             for (int k = 0; k < NGL; ++k) {
                 const int64_t id = ob + k * NGL;
-                double g1 = si.gRK[id], g2 = si.gRK[id + C::NN], g3 = si.gRK[id + 2 * C::NN];
+                double g1, g2, g3;
+                if constexpr (kDma) {
+                    g1 = gk[0][k]; g2 = gk[1][k]; g3 = gk[2][k];
+                } else {
+                    g1 = si.gRK[id]; g2 = si.gRK[id + C::NN]; g3 = si.gRK[id + 2 * C::NN];
+                }
                 g1 = si.rkA * g1 + b1[k];
                 g2 = si.rkA * g2 + b2[k];
                 g3 = si.rkA * g3 + b3[k];
