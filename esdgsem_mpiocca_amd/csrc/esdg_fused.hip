@@ -60,12 +60,26 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
     const int ir = (1 - rv) * (ngl - 1 - i) + rv * i;
     const int64_t xidR = (int64_t)f * ngl + ir;
     const double nx = nxa[t], ny = nya[t], sc = scal[t];
+    const double bl = BL[t], br = BR[t];
     // qL at node i and qR at node ir, exactly as the gather kernel stores them (the element side
     // traces come from the side-trace buffer: contiguous per side, unlike the columns of q)
-    const FaceState L = gathered_tr(tr, ngl, f, E, true, i, nx, ny, gc, BL[t], BR[t]);
-    const FaceState R = gathered_tr(tr, ngl, f, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
+    const FaceState L = gathered_tr(tr, ngl, f, E, true, i, nx, ny, gc, bl, br);
+    FaceState R;
+    double brR;
+    if (64 % ngl == 0) {
+        // a face's ngl nodes are consecutive lanes of one wave: each lane gathers qR at its own
+        // node, and node ir's values come from its lane, so no load waits for EdgeData's
+        // orientation
+        const FaceState Ro = gathered_tr(tr, ngl, f, E, false, i, nx, ny, gc, bl, br);
+        const int src = (int)(threadIdx.x & 63) - i + ir;
+        R = FaceState{__shfl(Ro.h, src), __shfl(Ro.hu, src), __shfl(Ro.hv, src)};
+        brR = __shfl(br, src);
+    } else {
+        R = gathered_tr(tr, ngl, f, E, false, ir, nxa[xidR], nya[xidR], gc, BL[xidR], BR[xidR]);
+        brR = BR[xidR];
+    }
     double F0, F1, F2;
-    numerical_flux(KIND, c, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, BL[t], BR[xidR], F0, F1, F2);
+    numerical_flux(KIND, c, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, bl, brR, F0, F1, F2);
     const int64_t eid = (int64_t)f * ngl * ESDG_NEQ + i;
     SP[eid] = sc * c.geomFace * F0;
     SP[eid + ngl] = sc * c.geomFace * F1;
