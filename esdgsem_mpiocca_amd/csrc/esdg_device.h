@@ -111,15 +111,15 @@ __device__ __forceinline__ FaceState gathered_tr(const double* __restrict__ tr, 
                                                  int k, double nx, double ny, const GatherConsts& gc, double bl,
                                                  double br) {
     const int cpuL = E[5], cpuR = E[6];
+    // both slots of the face are read whether or not their side is present (the buffer holds both
+    // slots of every face; an absent side's slot is never written and its values are not used), so
+    // the reads need not wait for EdgeData
+    const double* tl = tr + (int64_t)f * 6 * ngl + k;
+    const double* trr = tr + ((int64_t)f * 6 + 3) * ngl + k;
+    const FaceState Lt{tl[0], tl[ngl], tl[2 * ngl]}, Rt{trr[0], trr[ngl], trr[2 * ngl]};
     FaceState L{0.0, 0.0, 0.0}, R{0.0, 0.0, 0.0};
-    if (cpuL == 0) {
-        const double* t = tr + (int64_t)f * 6 * ngl + k;
-        L = FaceState{t[0], t[ngl], t[2 * ngl]};
-    }
-    if (cpuR == 0) {
-        const double* t = tr + ((int64_t)f * 6 + 3) * ngl + k;
-        R = FaceState{t[0], t[ngl], t[2 * ngl]};
-    }
+    if (cpuL == 0) L = Lt;
+    if (cpuR == 0) R = Rt;
     if (gc.bc == ESDG_BC_INNER_OUTER) {
         if (cpuL == -1) { L.h = R.h; L.hu = 0.0; L.hv = 0.0; }
         if (cpuR == -1) { R.h = L.h; R.hu = 0.0; R.hv = 0.0; }
