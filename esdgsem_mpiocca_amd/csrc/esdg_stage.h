@@ -45,6 +45,12 @@ struct StageScn {
     double* SPV;        // viscous face fluxes [Nfaces][2][ngl]
     int* active;        // == mark: viscosity on the element or a face neighbour this stage
     unsigned long long* emax;   // bit pattern of the step's max viscosity
+    // per element side, what its face flux needs from the connectivity (built by StageBatch::init
+    // from E2E, EOR, EMS and EdgeData; one record per side, shared by the scenarios of a mesh):
+    // x face, y EdgeData[0] (left element), z EdgeData[1] (right element), w bits: [0,2) left
+    // side, [2,4) right side, 4 EdgeData[4] (reversed), 5 EOR, 6 EMS > 0, [8,16) EdgeData[5] + 2,
+    // [16,24) EdgeData[6] + 2
+    const int4* srec;
 };
 
 struct StageTask {
@@ -118,6 +124,7 @@ class StageBatch {
     void drop_graphs();
 
   private:
+    std::vector<int4*> srec_bufs;   // side records, one buffer per distinct mesh
     int build_av_tasks(esdg_ctx* ctx);
     int reset_marks(esdg_ctx* ctx);
     int av_pass(esdg_ctx* ctx, int src, int stage);     // viscosity of the stage's state + marks

@@ -54,8 +54,9 @@ struct MaxLds<HI, HI> {
 // face fluxes of the elements' sides come from `faces`, into LDS (FS) after the sweeps.  Pointer
 // parameters are __restrict__ so that the geometry loads schedule as in a kernel taking them as
 // arguments.
-template <int NGL, bool AV, class Faces>
-__device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, const Faces& faces, double g, double tol,
+template <int NGL, bool AV, class Faces, class Early>
+__device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, const Faces& faces, const Early& early,
+                                           double g, double tol,
                                            double zero_tol, int positivity, double rkA, double rkC, double dt, bool last,
                                            const double* __restrict__ qi, double* __restrict__ qo,
                                            double* __restrict__ gRK, const double* __restrict__ Yeta,
@@ -86,6 +87,7 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, co
             Y[c] = Yeta[xb + c];
             X[c] = Xeta[xb + c];
         }
+        early();   // LDS stores of values loaded beside the row data
 #pragma unroll
         for (int c = 0; c < NGL; ++c) {
             const double inv = (h[c] > 0) ? 1.0 / h[c] : 0.0;
@@ -325,6 +327,24 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
     double* qo = S.q[src ^ 1];
     const double g = S.g;
 
+    // The element's four side records (StageScn::srec) are loaded beside the row data and stored
+    // to the limiter's LDS area (free until the epilogue), so the face-flux reads after the sweeps
+    // need no dependent round trips through ElemToEdge and EdgeData.
+    constexpr int FPL = (4 + NGL - 1) / NGL;   // sides per lane: lane r loads sides r, r + NGL, ...
+    int4 rec[FPL];
+#pragma unroll
+    for (int j = 0; j < FPL; ++j) {
+        const int sd = r + j * NGL;
+        rec[j] = make_int4(0, 0, 0, 0);
+        if (act && sd < 4) rec[j] = S.srec[(int64_t)e * 4 + sd];
+    }
+    int4* RS = reinterpret_cast<int4*>(P + C::ESZ);
+    auto early = [&]() {
+#pragma unroll
+        for (int j = 0; j < FPL; ++j)
+            if (r + j * NGL < 4) RS[r + j * NGL] = rec[j];
+    };
+
     // ---- fluxes of the element's four sides: lane r takes side node r of every side (called by
     // stage_core once the sweeps are done, with the face-flux area FS) ----------------------------
     auto faces = [&](double* FS) {
@@ -332,10 +352,14 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
         const GatherConsts gc{S.bc, S.h_zero};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const int f = S.E2E[e * 4 + s] - 1;
-            const int rev = S.EOR[e * 4 + s];
+            const int4 R4 = RS[s];
+            const int f = R4.x;
+            const unsigned w = (unsigned)R4.w;
+            const int rev = (int)((w >> 5) & 1u);
             const int i = rev * r + (1 - rev) * (NGL - 1 - r);      // face node of side node r
-            const int* E = S.EdgeData + 8 * (int64_t)f;
+            // EdgeData's entries as gathered() reads them
+            const int E[8] = {R4.y, R4.z, (int)(w & 3u), (int)((w >> 2) & 3u), (int)((w >> 4) & 1u),
+                              (int)((w >> 8) & 255u) - 2, (int)((w >> 16) & 255u) - 2, 0};
             const int rv = E[4];
             const int ir = (1 - rv) * (NGL - 1 - i) + rv * i;
             const int64_t t = (int64_t)f * NGL + i, tR = (int64_t)f * NGL + ir;
@@ -345,7 +369,7 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
             double F0, F1, F2;
             numerical_flux(S.num_flux, fc, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, S.bL[t], S.bR[tR], F0, F1,
                            F2);
-            const double nms = -(double)S.EMS[e * 4 + s];
+            const double nms = ((w >> 6) & 1u) ? -1.0 : 1.0;   // -EMS
             FS[(s * 3 + 0) * NGL + r] = nms * (sc * S.geomFace * F0);
             FS[(s * 3 + 1) * NGL + r] = nms * (sc * S.geomFace * F1);
             FS[(s * 3 + 2) * NGL + r] = nms * (sc * S.geomFace * F2);
@@ -353,7 +377,7 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
     };
 
     __syncthreads();   // sD written
-    stage_core<NGL, AV>(e, r, act, P, faces, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo,
+    stage_core<NGL, AV>(e, r, act, P, faces, early, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo,
                         S.gRK, S.Yeta, S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax, &S, mark,
                         sD);
 }
@@ -385,7 +409,7 @@ template <int LO, int HI, bool PERSIST, bool AV>
 __global__ void __launch_bounds__(64, 2) k_stage(const StageScn* __restrict__ scns, const StageTask* __restrict__ tasks,
                                                  int n, int xcd, int tile, int src, double rkA, double rkC,
                                                  int last, const int* __restrict__ markp, int stage) {
-    __shared__ double lds[MaxLds<LO, HI>::value];
+    __shared__ __attribute__((aligned(16))) double lds[MaxLds<LO, HI>::value];
     __shared__ double sD[HI * HI];
     const int mark = AV ? *markp + stage : 0;
     const int b = blockIdx.x;
@@ -638,6 +662,20 @@ static int load_degree_operators(esdg_ctx* ctx, const std::vector<StageScn>& scn
     return ESDG_OK;
 }
 
+// One record per element side (StageScn::srec) from the connectivity arrays.
+__global__ void __launch_bounds__(256) k_side_records(int Nelem, const int* __restrict__ E2E, const int* __restrict__ EOR,
+                                                      const int* __restrict__ EMS, const int* __restrict__ ED,
+                                                      int4* __restrict__ rec) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nelem * 4) return;
+    const int f = E2E[t] - 1;
+    const int* E = ED + 8 * (int64_t)f;
+    const unsigned w = (unsigned)(E[2] & 3) | ((unsigned)(E[3] & 3) << 2) | ((unsigned)(E[4] & 1) << 4) |
+                       ((unsigned)(EOR[t] & 1) << 5) | ((EMS[t] > 0 ? 1u : 0u) << 6) |
+                       (((unsigned)(E[5] + 2) & 255u) << 8) | (((unsigned)(E[6] + 2) & 255u) << 16);
+    rec[t] = make_int4(f, E[0], E[1], (int)w);
+}
+
 int StageBatch::init(esdg_ctx* ctx, const std::vector<StageScn>& scns) {
     host = scns;
     if (scns.empty()) return set_error(ctx, ESDG_ERR_ARG, "the single-kernel stage needs at least one scenario");
@@ -647,12 +685,32 @@ int StageBatch::init(esdg_ctx* ctx, const std::vector<StageScn>& scns) {
     }
     int rc = load_degree_operators(ctx, scns);
     if (rc) return rc;
+    // side records, once per distinct connectivity
+    for (size_t a = 0; a < host.size() && !rc; ++a) {
+        StageScn& S = host[a];
+        S.srec = nullptr;
+        for (size_t b = 0; b < a; ++b)
+            if (host[b].E2E == S.E2E && host[b].EdgeData == S.EdgeData && host[b].Nelem == S.Nelem) {
+                S.srec = host[b].srec;
+                break;
+            }
+        if (S.srec) continue;
+        int4* d = nullptr;
+        rc = esdg_malloc(ctx, (void**)&d, (size_t)S.Nelem * 4 * sizeof(int4));
+        if (rc) break;
+        srec_bufs.push_back(d);
+        k_side_records<<<(unsigned)(((int64_t)S.Nelem * 4 + 255) / 256), 256, 0, ctx->stream>>>(S.Nelem, S.E2E, S.EOR,
+                                                                                              S.EMS, S.EdgeData, d);
+        rc = launch_check(ctx, "side records");
+        S.srec = d;
+    }
+    if (rc) return rc;
     rc = hip_check(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
     if (rc) return rc;
     has_av = false;
     for (const StageScn& S : scns) has_av = has_av || S.av;
-    rc = esdg_malloc(ctx, (void**)&d_scn, scns.size() * sizeof(StageScn));
-    if (!rc) rc = esdg_memcpy(ctx, d_scn, scns.data(), scns.size() * sizeof(StageScn), ESDG_H2D);
+    rc = esdg_malloc(ctx, (void**)&d_scn, host.size() * sizeof(StageScn));
+    if (!rc) rc = esdg_memcpy(ctx, d_scn, host.data(), host.size() * sizeof(StageScn), ESDG_H2D);
     if (!rc && has_av) {
         rc = esdg_malloc(ctx, (void**)&d_mark, 64);
         if (!rc) rc = esdg_malloc(ctx, (void**)&d_icount, 5 * sizeof(unsigned long long));
@@ -949,6 +1007,8 @@ void StageBatch::release(esdg_ctx* ctx) {
     for (StageTask** p : {&d_task, &d_sens, &d_ind, &d_grad, &d_vface})
         if (*p) { esdg_free(ctx, *p); *p = nullptr; }
     if (d_scn) esdg_free(ctx, d_scn);
+    for (int4* p : srec_bufs) esdg_free(ctx, p);
+    srec_bufs.clear();
     if (d_mark) esdg_free(ctx, d_mark);
     if (d_icount) esdg_free(ctx, d_icount);
     d_scn = nullptr;
