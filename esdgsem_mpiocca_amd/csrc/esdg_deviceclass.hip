@@ -166,10 +166,11 @@ class deviceclass {
     // where the network does not settle the element as smooth.
     // clock_mode (the device clock of the fused schedule; 0 without it): the learned indicator skips
     // a step issued after T (1: the step's first evaluation, 2: before a later stage)
-    int viscosity(int clock_mode = 0) {
+    // stream (learned indicator only; null: the context's stream)
+    int viscosity(int clock_mode = 0, hipStream_t stream = nullptr) {
         if (!ind_) return esdg_shock_capturing(ctx_, Nelem, o_q, o_B, o_VdmInv, o_EleSizes, o_ViscPara, o_ViscForPlot);
         return mlp_classify_sensor(ctx_, ind_, Nelem, o_q, o_B, o_VdmInv, ind_lo, ind_hi, o_estat, ind_audit, o_icount,
-                                   o_ViscPara, o_ViscForPlot, clock_mode ? o_clock : nullptr, rp_.T, clock_mode);
+                                   o_ViscPara, o_ViscForPlot, clock_mode ? o_clock : nullptr, rp_.T, clock_mode, stream);
     }
 
     // One time step's dt (src/deviceclass.cpp:899-928), reductions on the device.
@@ -336,9 +337,14 @@ class deviceclass {
             }
             if (!r) r = fused_clock(ctx_, o_red, o_clock, dti, dvc, rp_.artificial_viscosity, rp_.T);
             for (int s = 0; s < stages && !r; ++s) {
-                if (rp_.artificial_viscosity && s > 0 && !visc_fresh) r = viscosity(2);
-                if (r) break;
                 FusedArgs fa = fused_args();
+                if (rp_.artificial_viscosity && s > 0 && !visc_fresh) {
+                    // the learned indicator of this stage's state: issued by the stage beside its
+                    // face fluxes (which do not read the viscosity)
+                    if (ind_) fa.visc_pass = [this](hipStream_t st) { return viscosity(2, st); };
+                    else r = viscosity(2);
+                }
+                if (r) break;
                 if (fuse_sensor) { fa.next_visc = o_ViscPara; fa.next_visc_plot = o_ViscForPlot; }
                 if (s == stages - 1) fa.next_lambda = o_LambdaMax;
                 fa.dt_dev = o_clock;

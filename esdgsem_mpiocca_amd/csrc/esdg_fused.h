@@ -1,6 +1,7 @@
 // esdg_fused.h -- fused per-stage kernels (a faster schedule of the same per-stage arithmetic).
 #pragma once
 
+#include <functional>
 #include <string>
 
 #include "esdg_internal.h"
@@ -26,6 +27,11 @@ struct FusedArgs {
     // issued after T), the update using C_stage * dt; a step with dt = 0 changes nothing.
     const double* dt_dev = nullptr;
     double C_stage = 0.0;
+    // Optional (with AV): the pass that evaluates the stage's element viscosity (ViscPara), issued
+    // by the stage itself on the stream it is given.  The face fluxes do not read the viscosity, so
+    // with the context's second stream (and no per-launch profiling) it runs there beside the
+    // face-flux pass, and the face marks follow once both are done.
+    std::function<int(hipStream_t)> visc_pass;
 };
 
 // The device clock of the fused schedule: clock = {dt, t, steps, last dt, error, bad dt, -, -}.

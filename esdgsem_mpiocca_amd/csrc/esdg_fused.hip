@@ -111,6 +111,30 @@ __global__ void __launch_bounds__(256) k_fused_face(int Nfaces, int ngl, FluxCon
     }
 }
 
+// The face marks of k_fused_face<true, *> on their own (one thread per face), for a stage whose
+// face fluxes were computed while the viscosity was still being evaluated.
+__global__ void __launch_bounds__(256) k_fused_face_marks(int Nfaces, const int* __restrict__ ED,
+                                                          const double* __restrict__ visc, int* __restrict__ active,
+                                                          int Nelem, int* __restrict__ alist, int* __restrict__ acount) {
+    const int f = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (f >= Nfaces) return;
+    const int seq = acount[2];
+    if (f == 0) acount[(seq + 1) & 1] = 0;
+    const int* E = ED + 8 * (int64_t)f;
+    const int cpuL = E[5], cpuR = E[6];
+    const bool hasL = cpuL == 0, hasR = cpuR == 0;
+    if ((hasL && visc[E[0]] != 0.0) || (hasR && visc[E[1]] != 0.0)) {
+        auto mark = [&](int e) {
+            if (atomicExch(&active[e], seq) != seq) {
+                const int k = atomicAdd(acount + (seq & 1), 1);
+                if (k < Nelem) alist[k] = e;
+            }
+        };
+        if (hasL) mark(E[0]);
+        if (hasR) mark(E[1]);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // fused_av: calcGradient.okl:18-156 + SurfaceKernelGradient.okl:16-148 (with the BR1 face fluxes
 // of calcNumFluxesGradient.okl evaluated per element side from the state's side traces) +
@@ -305,6 +329,20 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
         }
         if (rc) return rc;
     }
+    // the stage's viscosity pass (when the stage issues it): beside the face fluxes on the second
+    // stream, or first, in order
+    const bool overlap = av && A.visc_pass && ctx->side && !ctx->profile;
+    if (av && A.visc_pass && !overlap) {
+        const int rc = A.visc_pass(ctx->stream);
+        if (rc) return rc;
+    }
+    if (overlap) {
+        (void)hipEventRecord(ctx->fork, ctx->stream);
+        (void)hipStreamWaitEvent(ctx->side, ctx->fork, 0);
+        const int rc = A.visc_pass(ctx->side);
+        (void)hipEventRecord(ctx->join, ctx->side);
+        if (rc) return rc;
+    }
     {
         KernelScope ks(ctx, "fused_face");
         auto launch = [&](auto av_tag, auto kind_tag) {
@@ -324,9 +362,17 @@ int FusedStage::stage(esdg_ctx* ctx, int, double rkA, double cdt, const FusedArg
                 default: launch(av_tag, std::integral_constant<int, ESDG_FLUX_OCEAN>{}); break;
             }
         };
-        if (av) by_kind(std::true_type{});
+        if (av && !overlap) by_kind(std::true_type{});
         else by_kind(std::false_type{});
         int rc = launch_check(ctx, "fused_face");
+        if (rc) return rc;
+    }
+    if (overlap) {
+        (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
+        KernelScope ks(ctx, "fused_face_marks");
+        k_fused_face_marks<<<blocks_for(A.Nfaces, 256), 256, 0, ctx->stream>>>(A.Nfaces, A.EdgeData, A.ViscPara,
+                                                                                active, A.Nelem, alist, acount);
+        const int rc = launch_check(ctx, "fused_face_marks");
         if (rc) return rc;
     }
     if (!av) {

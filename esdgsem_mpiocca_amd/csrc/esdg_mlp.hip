@@ -560,7 +560,7 @@ int mlp_ensemble_indicator(esdg_ctx* ctx, const esdg_mlp* m, const StageScn* scn
 int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B,
                         const double* VdmInv, float lo, float hi, signed char* estat, int audit,
                         unsigned long long* counters, double* visc, double* visc_plot, const double* clock,
-                        double T, int clock_mode) {
+                        double T, int clock_mode, hipStream_t stream) {
     if (ctx->p.ngl != 8) return set_error(ctx, ESDG_ERR_ARG, "the learned indicator needs N = 7");
     MaskedSensor MS;
     MS.clock = clock;
@@ -572,8 +572,8 @@ int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const doubl
     MS.audit = audit;
     MS.counters = counters;
     KernelScope ks(ctx, "mlp_indicator");
-    k_mlp_indicator<<<blocks_for(Nelem, FEPB), 256, 0, ctx->stream>>>(Nelem, q, B, VdmInv, m->dev(), lo, hi, nullptr,
-                                                                       estat, MS);
+    hipStream_t st = stream ? stream : ctx->stream;
+    k_mlp_indicator<<<blocks_for(Nelem, FEPB), 256, 0, st>>>(Nelem, q, B, VdmInv, m->dev(), lo, hi, nullptr, estat, MS);
     return launch_check(ctx, "mlp_indicator");
 }
 }  // namespace esdg

@@ -23,7 +23,8 @@ int mlp_classify(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, c
 int mlp_classify_sensor(esdg_ctx* ctx, const esdg_mlp* m, int Nelem, const double* q, const double* B,
                         const double* VdmInv, float lo, float hi, signed char* estat, int audit,
                         unsigned long long* counters, double* visc, double* visc_plot,
-                        const double* clock = nullptr, double T = 0.0, int clock_mode = 0);
+                        const double* clock = nullptr, double T = 0.0, int clock_mode = 0,
+                        hipStream_t stream = nullptr);   // null: the context's stream
 // The same pass over the learned scenarios of a batched ensemble (tasks: 32-element tiles of N = 7
 // scenarios, in buffer src of their state): each viscous element marks itself and its face
 // neighbours with *mark + stage; in stage 0 the viscosities enter each scenario's maximum.
