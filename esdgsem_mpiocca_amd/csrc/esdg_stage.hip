@@ -782,18 +782,29 @@ int StageBatch::reset_marks(esdg_ctx* ctx) {
 }
 
 int StageBatch::av_pass(esdg_ctx* ctx, int src, int stage) {
+    if (nind && !mlp) return set_error(ctx, ESDG_ERR_STATE, "learned scenarios without a network");
+    // The modal-sensor and the learned scenarios are disjoint (each pass writes its own scenarios'
+    // viscosities, marks and maxima): with both present the sensor pass runs on the second stream
+    // beside the indicator pass (not under per-launch profiling, whose events are on the main
+    // stream).
+    const bool overlap = nsens && nind && ctx->side && !ctx->profile;
+    if (overlap) {
+        (void)hipEventRecord(ctx->fork, ctx->stream);
+        (void)hipStreamWaitEvent(ctx->side, ctx->fork, 0);
+    }
     if (nsens) {
         KernelScope ks(ctx, "ens_sensor");
-        k_ens_sensor<3, 8><<<nsens, 256, 0, ctx->stream>>>(d_scn, d_sens, src, d_mark, stage);
+        k_ens_sensor<3, 8><<<nsens, 256, 0, overlap ? ctx->side : ctx->stream>>>(d_scn, d_sens, src, d_mark, stage);
         int rc = launch_check(ctx, "ens_sensor");
         if (rc) return rc;
     }
+    if (overlap) (void)hipEventRecord(ctx->join, ctx->side);
     if (nind) {
-        if (!mlp) return set_error(ctx, ESDG_ERR_STATE, "learned scenarios without a network");
         int rc = mlp_ensemble_indicator(ctx, mlp, d_scn, d_ind, nind, src, d_mark, stage, ind_lo, ind_hi, ind_audit,
                                         d_icount);
         if (rc) return rc;
     }
+    if (overlap) (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
     return ESDG_OK;
 }
 
