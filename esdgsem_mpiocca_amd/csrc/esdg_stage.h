@@ -97,6 +97,7 @@ class StageBatch {
 
     struct Range {
         int beg, n, lo, hi, grid, persistent;
+        int learned;   // 1: the tasks of learned-indicator scenarios (see `split`)
     };
     std::vector<StageScn> host;
     StageSchedule sched;
@@ -112,6 +113,13 @@ class StageBatch {
     bool has_av = false;
     StageTask *d_sens = nullptr, *d_ind = nullptr, *d_grad = nullptr, *d_vface = nullptr;
     int nsens = 0, nind = 0, ngrad = 0, nvface = 0;
+    // With both modal-sensor and learned scenarios the stage runs as two streams of work: the
+    // other scenarios (sensor pass, their AV terms, their stage launches) on the second stream and
+    // the learned ones (indicator pass, AV terms, stage launches) on the main stream, so the
+    // indicator overlaps the other scenarios' stage.  The gradient and viscous-face task lists hold
+    // the other scenarios' tasks first (ngrad_o, nvface_o of them).
+    bool split = false;
+    int ngrad_o = 0, nvface_o = 0;
     int* d_mark = nullptr;
     unsigned long long* d_icount = nullptr;        // learned indicator counts (5, as the solver's)
     const esdg_mlp* mlp = nullptr;
@@ -129,6 +137,8 @@ class StageBatch {
     int reset_marks(esdg_ctx* ctx);
     int av_pass(esdg_ctx* ctx, int src, int stage);     // viscosity of the stage's state + marks
     int av_terms(esdg_ctx* ctx, int src, int stage);    // gradients, viscous volume and face terms
+    // the same over the learned (1) or the other (0) scenarios only, on stream st
+    int av_terms_group(esdg_ctx* ctx, int src, int stage, int learned, hipStream_t st);
 };
 
 }  // namespace esdg

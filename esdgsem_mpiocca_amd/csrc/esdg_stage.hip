@@ -727,16 +727,23 @@ int StageBatch::init(esdg_ctx* ctx, const std::vector<StageScn>& scns) {
 // (256 faces).
 int StageBatch::build_av_tasks(esdg_ctx* ctx) {
     std::vector<StageTask> sens, ind, grad, vface;
-    for (int s = 0; s < (int)host.size(); ++s) {
-        const StageScn& S = host[s];
-        if (!S.av) continue;
-        const int ngl = S.ngl;
-        if (S.learned)
-            for (int e0 = 0; e0 < S.Nelem; e0 += 32) ind.push_back(StageTask{s, e0, ngl, 0});
-        else
-            for (int e0 = 0; e0 < S.Nelem; e0 += 256 / ngl) sens.push_back(StageTask{s, e0, ngl, 0});
-        for (int e0 = 0; e0 < S.Nelem; e0 += GRAD_TILE) grad.push_back(StageTask{s, e0, ngl, 0});
-        for (int f0 = 0; f0 < S.Nfaces; f0 += 256) vface.push_back(StageTask{s, f0, ngl, 0});
+    // gradient and viscous-face tasks: the non-learned scenarios' first, then the learned ones'
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int s = 0; s < (int)host.size(); ++s) {
+            const StageScn& S = host[s];
+            if (!S.av || S.learned != pass) continue;
+            const int ngl = S.ngl;
+            if (S.learned)
+                for (int e0 = 0; e0 < S.Nelem; e0 += 32) ind.push_back(StageTask{s, e0, ngl, 0});
+            else
+                for (int e0 = 0; e0 < S.Nelem; e0 += 256 / ngl) sens.push_back(StageTask{s, e0, ngl, 0});
+            for (int e0 = 0; e0 < S.Nelem; e0 += GRAD_TILE) grad.push_back(StageTask{s, e0, ngl, 0});
+            for (int f0 = 0; f0 < S.Nfaces; f0 += 256) vface.push_back(StageTask{s, f0, ngl, 0});
+        }
+        if (pass == 0) {
+            ngrad_o = (int)grad.size();
+            nvface_o = (int)vface.size();
+        }
     }
     auto put = [&](const std::vector<StageTask>& v, StageTask*& d, int& n) -> int {
         if (d) esdg_free(ctx, d);
@@ -769,6 +776,7 @@ int StageBatch::set_indicator(esdg_ctx* ctx, const esdg_mlp* m, float lo, float 
     drop_graphs();
     int rc = esdg_memcpy(ctx, d_scn, host.data(), host.size() * sizeof(StageScn), ESDG_H2D);
     if (!rc && has_av) rc = build_av_tasks(ctx);
+    if (!rc) rc = set_schedule(ctx, sched);   // the stage launches split by group (or not)
     return rc;
 }
 
@@ -805,6 +813,21 @@ int StageBatch::av_pass(esdg_ctx* ctx, int src, int stage) {
         if (rc) return rc;
     }
     if (overlap) (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
+    return ESDG_OK;
+}
+
+int StageBatch::av_terms_group(esdg_ctx* ctx, int src, int stage, int learned, hipStream_t st) {
+    const int gb = learned ? ngrad_o : 0, gn = learned ? ngrad - ngrad_o : ngrad_o;
+    const int vb = learned ? nvface_o : 0, vn = learned ? nvface - nvface_o : nvface_o;
+    if (gn) {
+        k_ens_grad<3, 8><<<gn, 256, 0, st>>>(d_scn, d_grad + gb, src, d_mark, stage);
+        int rc = launch_check(ctx, "ens_grad");
+        if (rc) return rc;
+    }
+    if (vn) {
+        k_ens_vface<<<vn, 256, 0, st>>>(d_scn, d_vface + vb, d_mark, stage);
+        return launch_check(ctx, "ens_viscous_face");
+    }
     return ESDG_OK;
 }
 
@@ -860,10 +883,21 @@ int StageBatch::set_schedule(esdg_ctx* ctx, const StageSchedule& sc) {
         for (int s : scn)
             for (int t = 0; t < tiles(s); ++t) tasks.push_back(task(s, t));
     }
+    // with both modal-sensor and learned scenarios: the other scenarios' tasks first, then the
+    // learned ones', each group in the schedule's order and launched on its own stream
+    split = has_av && nsens > 0 && nind > 0;
+    int nother = (int)tasks.size();
+    if (split) {
+        std::stable_partition(tasks.begin(), tasks.end(), [&](const StageTask& t) { return !host[t.scn].learned; });
+        nother = 0;
+        while (nother < (int)tasks.size() && !host[tasks[nother].scn].learned) ++nother;
+    }
     // launch ranges
     std::vector<Range> rg;
+    int cur_learned = 0;
     auto add_range = [&](int beg, int n, int lo, int hi) {
-        Range r{beg, n, lo, hi, n, 0};
+        if (n <= 0) return;
+        Range r{beg, n, lo, hi, n, 0, cur_learned};
         // chunks of sc.tile tasks; with xcd every XCD's part is chunked on its own
         const int nch = sc.xcd ? 8 * (((n + 7) / 8 + sc.tile - 1) / sc.tile) : (n + sc.tile - 1) / sc.tile;
         const int pgrid = sc.blocks_per_cu * ncu;
@@ -876,19 +910,24 @@ int StageBatch::set_schedule(esdg_ctx* ctx, const StageSchedule& sc) {
         }
         rg.push_back(r);
     };
-    if (sc.per_degree) {
-        int beg = 0;
-        while (beg < (int)tasks.size()) {
-            int end = beg;
-            while (end < (int)tasks.size() && tasks[end].ngl == tasks[beg].ngl) ++end;
-            add_range(beg, end - beg, tasks[beg].ngl, tasks[beg].ngl);
-            beg = end;
+    auto add_group = [&](int g0, int g1) {
+        if (sc.per_degree) {
+            int beg = g0;
+            while (beg < g1) {
+                int end = beg;
+                while (end < g1 && tasks[end].ngl == tasks[beg].ngl) ++end;
+                add_range(beg, end - beg, tasks[beg].ngl, tasks[beg].ngl);
+                beg = end;
+            }
+        } else {
+            int lo = 99, hi = 0;
+            for (int i = g0; i < g1; ++i) { lo = std::min(lo, tasks[i].ngl); hi = std::max(hi, tasks[i].ngl); }
+            add_range(g0, g1 - g0, lo, hi);
         }
-    } else {
-        int lo = 99, hi = 0;
-        for (const StageTask& t : tasks) { lo = std::min(lo, t.ngl); hi = std::max(hi, t.ngl); }
-        add_range(0, (int)tasks.size(), lo, hi);
-    }
+    };
+    add_group(0, nother);
+    cur_learned = 1;
+    add_group(nother, (int)tasks.size());
     if (d_task && (int)tasks.size() != ntask) {
         esdg_free(ctx, d_task);
         d_task = nullptr;
@@ -911,27 +950,27 @@ int StageBatch::lambda(esdg_ctx* ctx, int src) {
 }
 
 template <int LO, int HI, bool AV>
-static void launch_stage(esdg_ctx* ctx, const StageBatch& b, const StageBatch::Range& r, int src, double rkA,
+static void launch_stage(hipStream_t st, const StageBatch& b, const StageBatch::Range& r, int src, double rkA,
                          double rkC, int last, int stage) {
     if (r.persistent)
-        k_stage<LO, HI, true, AV><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd,
+        k_stage<LO, HI, true, AV><<<r.grid, 64, 0, st>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd,
                                                                    b.sched.tile, src, rkA, rkC, last, b.d_mark, stage);
     else
-        k_stage<LO, HI, false, AV><<<r.grid, 64, 0, ctx->stream>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, 1,
+        k_stage<LO, HI, false, AV><<<r.grid, 64, 0, st>>>(b.d_scn, b.d_task + r.beg, r.n, b.sched.xcd, 1,
                                                                     src, rkA, rkC, last, b.d_mark, stage);
 }
 
 template <bool AV>
-static void launch_ranges(esdg_ctx* ctx, const StageBatch& b, const StageBatch::Range& r, int src, double rkA,
+static void launch_ranges(hipStream_t st, const StageBatch& b, const StageBatch::Range& r, int src, double rkA,
                           double rkC, int last, int stage) {
-    if (r.lo != r.hi) launch_stage<3, 8, AV>(ctx, b, r, src, rkA, rkC, last, stage);
+    if (r.lo != r.hi) launch_stage<3, 8, AV>(st, b, r, src, rkA, rkC, last, stage);
     else switch (r.lo) {
-        case 3: launch_stage<3, 3, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
-        case 4: launch_stage<4, 4, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
-        case 5: launch_stage<5, 5, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
-        case 6: launch_stage<6, 6, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
-        case 7: launch_stage<7, 7, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
-        default: launch_stage<8, 8, AV>(ctx, b, r, src, rkA, rkC, last, stage); break;
+        case 3: launch_stage<3, 3, AV>(st, b, r, src, rkA, rkC, last, stage); break;
+        case 4: launch_stage<4, 4, AV>(st, b, r, src, rkA, rkC, last, stage); break;
+        case 5: launch_stage<5, 5, AV>(st, b, r, src, rkA, rkC, last, stage); break;
+        case 6: launch_stage<6, 6, AV>(st, b, r, src, rkA, rkC, last, stage); break;
+        case 7: launch_stage<7, 7, AV>(st, b, r, src, rkA, rkC, last, stage); break;
+        default: launch_stage<8, 8, AV>(st, b, r, src, rkA, rkC, last, stage); break;
     }
 }
 
@@ -953,6 +992,36 @@ int StageBatch::step(esdg_ctx* ctx, const double* A, const double* C, int stages
     }
     for (int s = 0; s < stages; ++s) {
         const int last = (s == stages - 1);
+        if (split && ctx->side && !ctx->profile) {
+            // the other scenarios on the second stream, the learned ones on the main stream
+            (void)hipEventRecord(ctx->fork, ctx->stream);
+            (void)hipStreamWaitEvent(ctx->side, ctx->fork, 0);
+            int rc = ESDG_OK;
+            if (s > 0 && nsens) {
+                k_ens_sensor<3, 8><<<nsens, 256, 0, ctx->side>>>(d_scn, d_sens, src, d_mark, s);
+                rc = launch_check(ctx, "ens_sensor");
+            }
+            if (!rc) rc = av_terms_group(ctx, src, s, 0, ctx->side);
+            for (const Range& r : ranges)
+                if (!rc && !r.learned) {
+                    launch_ranges<true>(ctx->side, *this, r, src, A[s], C[s], last, s);
+                    rc = launch_check(ctx, "stage");
+                }
+            (void)hipEventRecord(ctx->join, ctx->side);
+            if (!rc && s > 0)
+                rc = mlp_ensemble_indicator(ctx, mlp, d_scn, d_ind, nind, src, d_mark, s, ind_lo, ind_hi, ind_audit,
+                                            d_icount);
+            if (!rc) rc = av_terms_group(ctx, src, s, 1, ctx->stream);
+            for (const Range& r : ranges)
+                if (!rc && r.learned) {
+                    launch_ranges<true>(ctx->stream, *this, r, src, A[s], C[s], last, s);
+                    rc = launch_check(ctx, "stage");
+                }
+            (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
+            if (rc) return rc;
+            src ^= 1;
+            continue;
+        }
         if (has_av) {
             int rc = (s > 0) ? av_pass(ctx, src, s) : ESDG_OK;
             if (!rc) rc = av_terms(ctx, src, s);
@@ -960,8 +1029,8 @@ int StageBatch::step(esdg_ctx* ctx, const double* A, const double* C, int stages
         }
         for (const Range& r : ranges) {
             KernelScope ks(ctx, "stage");
-            if (has_av) launch_ranges<true>(ctx, *this, r, src, A[s], C[s], last, s);
-            else launch_ranges<false>(ctx, *this, r, src, A[s], C[s], last, s);
+            if (has_av) launch_ranges<true>(ctx->stream, *this, r, src, A[s], C[s], last, s);
+            else launch_ranges<false>(ctx->stream, *this, r, src, A[s], C[s], last, s);
             int rc = launch_check(ctx, "stage");
             if (rc) return rc;
         }
