@@ -370,8 +370,9 @@ def test_ism_mesh_lake_at_rest(dev):
 def test_captured_fused_step_equals_eager(dev, learned):
     """From the second step of each run the fused schedule replays one captured step (dt, t and the
     stage marks live on the device; the marked-element update runs on the second stream inside the
-    graph); a run with per-launch events stays eager.  Both give the same bits over two run calls
-    of 40 and 30 steps (a 64-step chunk boundary inside), with the learned indicator's counts too."""
+    graph, and so does the learned indicator beside the face fluxes); a run with per-launch events
+    stays eager, on one stream.  Both give the same bits over two run calls of 40 and 30 steps (a
+    64-step chunk boundary inside), with the learned indicator's counts too."""
     case = Case.dam_break(N=7, nel=24, noise=1e-3, seed=7, ratio=2.5, angle=0.4)
     rp = RunParams(num_flux=3, artificial_viscosity=1, eps0=0.005, sigma_min=-4.5, sigma_max=-2.5, DFL=0.05)
     outs = []
