@@ -257,28 +257,29 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         }
         if (MODE != VOL_ONLY) {
             // surface lift: bottom/top sides first (nodes k = 0, N of this column, trace index r),
-            // then right/left (the whole column r = N or 0, trace index k)
-#pragma unroll
-            for (int ph = 0; ph < 2; ++ph) {
+            // then right/left (the whole column r = N or 0, trace index k).  A side's face,
+            // orientation and sign are the same for all its nodes: read once, so the face-flux
+            // loads of a side do not wait for one another.
+            const int fa0 = fidx(0) - 1, rv0 = fidx(1), ms0 = fidx(2);
+            const int fa2 = fidx(6) - 1, rv2 = fidx(7), ms2 = fidx(8);
+            const int sc = (r == NGL - 1) ? 1 : 3;   // the column side of lanes r = N and r = 0
+            const int fac = fidx(3 * sc) - 1, rvc = fidx(3 * sc + 1), msc = fidx(3 * sc + 2);
+            const int64_t l0 = (int64_t)fa0 * NGL * ESDG_NEQ + rv0 * r + (1 - rv0) * (NGL - 1 - r);
+            const int64_t l2 = (int64_t)fa2 * NGL * ESDG_NEQ + rv2 * r + (1 - rv2) * (NGL - 1 - r);
+            b1[0] += -ms0 * si.FStar[l0] * rJ[0];
+            b2[0] += -ms0 * rJ[0] * si.FStar[l0 + NGL];
+            b3[0] += -ms0 * rJ[0] * si.FStar[l0 + 2 * NGL];
+            b1[NGL - 1] += -ms2 * si.FStar[l2] * rJ[NGL - 1];
+            b2[NGL - 1] += -ms2 * rJ[NGL - 1] * si.FStar[l2 + NGL];
+            b3[NGL - 1] += -ms2 * rJ[NGL - 1] * si.FStar[l2 + 2 * NGL];
+            if (r == 0 || r == NGL - 1) {
+                const int64_t fc = (int64_t)fac * NGL * ESDG_NEQ;
 #pragma unroll
                 for (int k = 0; k < NGL; ++k) {
-                    int side = -1, t = 0;
-                    if (ph == 0) {
-                        if (k == 0) { side = 0; t = r; }
-                        else if (k == NGL - 1) { side = 2; t = r; }
-                    } else {
-                        if (r == NGL - 1) { side = 1; t = k; }
-                        else if (r == 0) { side = 3; t = k; }
-                    }
-                    if (side < 0) continue;
-                    const int ifa = fidx(3 * side) - 1;
-                    const int rev = fidx(3 * side + 1);
-                    const int ms = fidx(3 * side + 2);
-                    const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
-                    const int64_t fl = (int64_t)ifa * NGL * ESDG_NEQ + inc;
-                    b1[k] += -ms * si.FStar[fl] * rJ[k];
-                    b2[k] += -ms * rJ[k] * si.FStar[fl + NGL];
-                    b3[k] += -ms * rJ[k] * si.FStar[fl + 2 * NGL];
+                    const int64_t fl = fc + rvc * k + (1 - rvc) * (NGL - 1 - k);
+                    b1[k] += -msc * si.FStar[fl] * rJ[k];
+                    b2[k] += -msc * rJ[k] * si.FStar[fl + NGL];
+                    b3[k] += -msc * rJ[k] * si.FStar[fl + 2 * NGL];
                 }
             }
         }
@@ -291,26 +292,27 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                     b3[k] += qv[C::NN + k * NGL];
                 }
             }
+            // the viscous face fluxes, each side's face, orientation and sign read once (as in the
+            // surface lift above)
+            {
+                const int fa0 = fidx(0) - 1, rv0 = fidx(1), ms0 = fidx(2);
+                const int fa2 = fidx(6) - 1, rv2 = fidx(7), ms2 = fidx(8);
+                const int sc = (r == NGL - 1) ? 1 : 3;
+                const int fac = fidx(3 * sc) - 1, rvc = fidx(3 * sc + 1), msc = fidx(3 * sc + 2);
+                const int64_t l0 = (int64_t)fa0 * NGL * 2 + rv0 * r + (1 - rv0) * (NGL - 1 - r);
+                const int64_t l2 = (int64_t)fa2 * NGL * 2 + rv2 * r + (1 - rv2) * (NGL - 1 - r);
+                b2[0] += -ms0 * rJ[0] * si.SPV[l0];
+                b3[0] += -ms0 * rJ[0] * si.SPV[l0 + NGL];
+                b2[NGL - 1] += -ms2 * rJ[NGL - 1] * si.SPV[l2];
+                b3[NGL - 1] += -ms2 * rJ[NGL - 1] * si.SPV[l2 + NGL];
+                if (r == 0 || r == NGL - 1) {
+                    const int64_t fc = (int64_t)fac * NGL * 2;
 #pragma unroll
-            for (int ph = 0; ph < 2; ++ph) {
-#pragma unroll
-                for (int k = 0; k < NGL; ++k) {
-                    int side = -1, t = 0;
-                    if (ph == 0) {
-                        if (k == 0) { side = 0; t = r; }
-                        else if (k == NGL - 1) { side = 2; t = r; }
-                    } else {
-                        if (r == NGL - 1) { side = 1; t = k; }
-                        else if (r == 0) { side = 3; t = k; }
+                    for (int k = 0; k < NGL; ++k) {
+                        const int64_t fl = fc + rvc * k + (1 - rvc) * (NGL - 1 - k);
+                        b2[k] += -msc * rJ[k] * si.SPV[fl];
+                        b3[k] += -msc * rJ[k] * si.SPV[fl + NGL];
                     }
-                    if (side < 0) continue;
-                    const int ifa = fidx(3 * side) - 1;
-                    const int rev = fidx(3 * side + 1);
-                    const int ms = fidx(3 * side + 2);
-                    const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
-                    const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
-                    b2[k] += -ms * rJ[k] * si.SPV[fl];
-                    b3[k] += -ms * rJ[k] * si.SPV[fl + NGL];
                 }
             }
         }
@@ -333,7 +335,15 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
 #pragma unroll
                 for (int i = 0; i < NGL; ++i) bsen[i] = si.B[bb + i];
             }
-            const double gx = si.limit ? si.GLw[r] : 0.0;
+            // the limiter's GLL weights, all loaded before the loop's stores (which the compiler
+            // cannot tell apart from them)
+            double gw[NGL];
+            double gx = 0.0;
+            if (si.limit) {
+#pragma unroll
+                for (int k = 0; k < NGL; ++k) gw[k] = si.GLw[k];
+                gx = gw[r];
+            }
             double mn = 0.0;
 #pragma unroll
             for (int k = 0; k < NGL; ++k) {
@@ -361,7 +371,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                     S[0 * C::PLANE + k * C::RS + r] = n1;
                     S[1 * C::PLANE + k * C::RS + r] = n2;
                     S[2 * C::PLANE + k * C::RS + r] = n3;
-                    S[3 * C::PLANE + k * C::RS + r] = gx * si.GLw[k] / rJ[k];
+                    S[3 * C::PLANE + k * C::RS + r] = gx * gw[k] / rJ[k];
                     mn = fmin(mn, n1);
                     continue;
                 }
