@@ -9,6 +9,17 @@
 
 namespace esdg {
 
+// A pointer into global memory with its address space in the type.  Pointers that a kernel reads
+// from a record in memory (rather than receives as arguments) are otherwise generic, and loads
+// through them compile to flat instructions, which count against the LDS counter too: every wait
+// for one of them then also waits for all LDS traffic, and no wait can be partial.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> glob(T* p) {
+    return (gptr<T>)p;
+}
+
 // node index of trace point k of element side s (okl/GatherEdgeData/SolidWalls.okl:74-89)
 __device__ __forceinline__ int side_node(int ngl, int side, int k) {
     switch (side) {
@@ -56,7 +67,9 @@ struct FaceState {
 // traces (SolidWalls.okl:74-132) with the exterior state of a boundary face from the wall mirror
 // (SolidWalls.okl:133-175), zero momentum (InnerOuter.okl) or the ocean rule (OceanBoundary.okl:
 // still water h_zero - b where b <= 6.7, wall otherwise).  bl, br: the face's bL, bR at node k.
-__device__ __forceinline__ FaceState gathered(const double* __restrict__ q, int ngl, const int* E, bool left, int k,
+// (QP: a pointer to const double, generic or gptr.)
+template <class QP>
+__device__ __forceinline__ FaceState gathered(QP __restrict__ q, int ngl, const int* E, bool left, int k,
                                               double nx, double ny, const GatherConsts& gc, double bl, double br) {
     const int ngl2 = ngl * ngl;
     const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
@@ -378,10 +391,10 @@ __device__ __forceinline__ double sensor_from_modes(const double* m, const Senso
 // of u, then row r of m from row r of T; four threads form the energy sums in the okl's order.
 // sV holds V (written by the caller before the call; the call's first barrier publishes it), sT
 // EPB * ES doubles, sSum EPB x 4.  Returns the element's viscosity on its row-0 thread.
-template <int NGL>
-__device__ __forceinline__ double sensor_rows(int Nelem, int e0, const SensorConsts& sc, const double* __restrict__ q,
-                                              const double* __restrict__ b, const double* sV, double* sT,
-                                              double (*sSum)[4]) {
+// (QP: a pointer to const double, generic or gptr.)
+template <int NGL, class QP>
+__device__ __forceinline__ double sensor_rows(int Nelem, int e0, const SensorConsts& sc, QP __restrict__ q,
+                                              QP __restrict__ b, const double* sV, double* sT, double (*sSum)[4]) {
     using RB = RowBlock<NGL>;
     constexpr int NN = NGL * NGL;
     const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;

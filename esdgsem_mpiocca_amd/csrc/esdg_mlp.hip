@@ -47,7 +47,7 @@ struct FeatLds {
     double red[FEPB][8];
 };
 
-__device__ void row_features(const double* __restrict__ q, const double* __restrict__ B, FeatLds& L, int e, bool act,
+__device__ void row_features(gptr<const double> __restrict__ q, gptr<const double> __restrict__ B, FeatLds& L, int e, bool act,
                              float* xn, float& modal) {
     const int el = threadIdx.x >> 3, r = threadIdx.x & 7;
     double h[8], u[8];
@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(256) k_mlp_features(int Nelem, const double* _
     const int e = blockIdx.x * FEPB + el;
     const bool act = e < Nelem;
     float xn[8], modal;
-    row_features(q, B, L, e, act, xn, modal);
+    row_features(glob(q), glob(B), L, e, act, xn, modal);
     if (act) {
         float* row = feat + (int64_t)e * NIN;
 #pragma unroll
@@ -261,8 +261,9 @@ struct MaskedSensor {
 // sensor's energy sums are taken from the coefficients, the two 16-row BF16 feature tiles and the
 // hidden activations take their place.  The network's weights are read from global memory (8 KB,
 // cached), not staged per block: about 22 KB of LDS per block, 7 blocks per CU.
-__device__ void mlp_indicator_block(int Nelem, int e0, const double* __restrict__ q, const double* __restrict__ B,
-                                    const double* __restrict__ V, const MlpDev& W, float lo, float hi,
+__device__ void mlp_indicator_block(int Nelem, int e0, gptr<const double> __restrict__ q,
+                                    gptr<const double> __restrict__ B, gptr<const double> __restrict__ V,
+                                    const MlpDev& W, float lo, float hi,
                                     float* __restrict__ p, signed char* __restrict__ estat, const MaskedSensor& MS) {
     if (MS.counters && threadIdx.x == 0)
         atomicAdd(MS.counters + 4, (unsigned long long)min(FEPB, Nelem - e0));
@@ -322,8 +323,8 @@ __device__ void mlp_indicator_block(int Nelem, int e0, const double* __restrict_
                 if (MS.audit && plain > 0.0) atomicAdd(&sCnt[2], 1ull);
                 if (st != 0) atomicAdd(&sCnt[3], 1ull);
             }
-            MS.visc[e] = v;
-            if (MS.visc_plot) MS.visc_plot[e] = v;
+            glob(MS.visc)[e] = v;
+            if (MS.visc_plot) glob(MS.visc_plot)[e] = v;
             if (MS.scn) {
                 if (v != 0.0) mark_reach(*MS.scn, e, MS.mark);
                 // non-negative doubles order as their bit patterns; a NaN stays out, as fmax leaves it
@@ -351,7 +352,7 @@ __global__ void __launch_bounds__(256) k_mlp_indicator(int Nelem, const double* 
         // a step issued after T changes nothing: no evaluation, no counts (uniform per block)
         if ((MS.clock_mode == 1 && MS.clock[1] >= MS.T) || (MS.clock_mode == 2 && MS.clock[0] == 0.0)) return;
     }
-    mlp_indicator_block(Nelem, (int)blockIdx.x * FEPB, q, B, V, W, lo, hi, p, estat, MS);
+    mlp_indicator_block(Nelem, (int)blockIdx.x * FEPB, glob(q), glob(B), glob(V), W, lo, hi, p, estat, MS);
 }
 
 // The indicator over the learned scenarios of a batched ensemble: block b takes tile tasks[b] (32
@@ -373,7 +374,8 @@ __global__ void __launch_bounds__(256) k_ens_indicator(const StageScn* __restric
     MS.scn = &S;
     MS.mark = *mark + stage;
     MS.stage0 = stage == 0;
-    mlp_indicator_block(S.Nelem, tk.e0, S.q[src], S.B, S.VdmInv, W, lo, hi, nullptr, nullptr, MS);
+    mlp_indicator_block(S.Nelem, tk.e0, glob((const double*)S.q[src]), glob(S.B), glob(S.VdmInv), W, lo, hi, nullptr,
+                        nullptr, MS);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -61,12 +61,13 @@ struct StageTask {
 // viscous terms can be non-zero), as the fused face pass does for both elements of every face with
 // viscosity on either side.  Plain stores of one value: concurrent marks agree.
 __device__ __forceinline__ void mark_reach(const StageScn& S, int e, int m) {
-    S.active[e] = m;
+    const gptr<int> active = glob(S.active);
+    active[e] = m;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const int* E = S.EdgeData + 8 * (int64_t)(S.E2E[e * 4 + s] - 1);
-        if (E[5] == 0) S.active[E[0]] = m;
-        if (E[6] == 0) S.active[E[1]] = m;
+        const gptr<const int> E = glob(S.EdgeData) + 8 * (int64_t)(glob(S.E2E)[e * 4 + s] - 1);
+        if (E[5] == 0) active[E[0]] = m;
+        if (E[6] == 0) active[E[1]] = m;
     }
 }
 

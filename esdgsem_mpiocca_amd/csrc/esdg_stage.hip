@@ -53,18 +53,19 @@ struct MaxLds<HI, HI> {
 // Volume term, surface lift, RK update, limiter and wave speed of one block of elements; the
 // face fluxes of the elements' sides come from `faces`, into LDS (FS) after the sweeps.  Pointer
 // parameters are __restrict__ so that the geometry loads schedule as in a kernel taking them as
-// arguments.
+// arguments, and global (gptr) so that they compile to global loads and stores.
 template <int NGL, bool AV, class Faces, class Early>
 __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, const Faces& faces, const Early& early,
                                            double g, double tol,
                                            double zero_tol, int positivity, double rkA, double rkC, double dt, bool last,
-                                           const double* __restrict__ qi, double* __restrict__ qo,
-                                           double* __restrict__ gRK, const double* __restrict__ Yeta,
-                                           const double* __restrict__ Xeta, const double* __restrict__ Yxi,
-                                           const double* __restrict__ Xxi, const double* __restrict__ Jac,
-                                           const double* __restrict__ Bx, const double* __restrict__ By,
-                                           const double* __restrict__ EleSizes, const double* __restrict__ GLw,
-                                           unsigned long long* __restrict__ lmax, const StageScn* SV, int mark,
+                                           gptr<const double> __restrict__ qi, gptr<double> __restrict__ qo,
+                                           gptr<double> __restrict__ gRK, gptr<const double> __restrict__ Yeta,
+                                           gptr<const double> __restrict__ Xeta, gptr<const double> __restrict__ Yxi,
+                                           gptr<const double> __restrict__ Xxi, gptr<const double> __restrict__ Jac,
+                                           gptr<const double> __restrict__ Bx, gptr<const double> __restrict__ By,
+                                           gptr<const double> __restrict__ EleSizes,
+                                           gptr<const double> __restrict__ GLw,
+                                           gptr<unsigned long long> __restrict__ lmax, const StageScn* SV, int mark,
                                            const double* D) {
     using C = VolCfg<NGL>;
     double* FS = P;                  // face fluxes, after the sweeps (the planes are free then)
@@ -177,9 +178,11 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, co
             // after the surface lift: the viscous volume term (where the element's viscosity is not
             // zero), then the lift of the viscous face fluxes (VolumeKernelViscose.okl,
             // SurfaceKernelVisc.okl), as the fused volume pass adds them
-            if (SV->av && SV->active[e] == mark) {
-                if (SV->visc[e] != 0.0) {
-                    const double* qv = SV->Qv + (int64_t)e * C::NN * 2 + r;
+            const gptr<const int> E2E = glob(SV->E2E), EOR = glob(SV->EOR), EMS = glob(SV->EMS);
+            const gptr<const double> SPV = glob((const double*)SV->SPV);
+            if (SV->av && glob(SV->active)[e] == mark) {
+                if (glob(SV->visc)[e] != 0.0) {
+                    const gptr<const double> qv = glob((const double*)SV->Qv) + (int64_t)e * C::NN * 2 + r;
 #pragma unroll
                     for (int k = 0; k < NGL; ++k) {
                         b2[k] += qv[k * NGL];
@@ -199,13 +202,13 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, co
                             else if (r == 0) { side = 3; t = k; }
                         }
                         if (side < 0) continue;
-                        const int ifa = SV->E2E[e * 4 + side] - 1;
-                        const int rev = SV->EOR[e * 4 + side];
-                        const int ms = SV->EMS[e * 4 + side];
+                        const int ifa = E2E[e * 4 + side] - 1;
+                        const int rev = EOR[e * 4 + side];
+                        const int ms = EMS[e * 4 + side];
                         const int inc = rev * t + (1 - rev) * (NGL - 1 - t);
                         const int64_t fl = (int64_t)ifa * NGL * 2 + inc;
-                        b2[k] += -ms * rJ[k] * SV->SPV[fl];
-                        b3[k] += -ms * rJ[k] * SV->SPV[fl + NGL];
+                        b2[k] += -ms * rJ[k] * SPV[fl];
+                        b3[k] += -ms * rJ[k] * SPV[fl + NGL];
                     }
                 }
             }
@@ -306,7 +309,7 @@ __device__ __forceinline__ void stage_core(int e, int r, bool act, double* P, co
     }
     if (last) {
         for (int off = 32; off > 0; off >>= 1) lam = fmax(lam, __shfl_xor(lam, off));
-        if (lane == 0) atomicMax(lmax, (unsigned long long)__double_as_longlong(lam));
+        if (lane == 0) atomicMax((unsigned long long*)lmax, (unsigned long long)__double_as_longlong(lam));
     }
 }
 
@@ -323,8 +326,8 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
     const int e = e0 + el;
     const bool act = (el < C::EPW) && (e < S.Nelem);
     double* P = lds + (act ? el : 0) * SC::ELDS;     // planes: h, hu, hv, 1/h (later: the row sums)
-    const double* qi = S.q[src];
-    double* qo = S.q[src ^ 1];
+    const gptr<const double> qi = glob((const double*)S.q[src]);
+    const gptr<double> qo = glob(S.q[src ^ 1]);
     const double g = S.g;
 
     // The element's four side records (StageScn::srec) are loaded beside the row data and stored
@@ -336,7 +339,12 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
     for (int j = 0; j < FPL; ++j) {
         const int sd = r + j * NGL;
         rec[j] = make_int4(0, 0, 0, 0);
-        if (act && sd < 4) rec[j] = S.srec[(int64_t)e * 4 + sd];
+        if (act && sd < 4) {
+            // one 16-byte load (a vector type: int4's assignment would take a generic reference)
+            using i4v = int __attribute__((ext_vector_type(4)));
+            const i4v v = ((gptr<const i4v>)glob(S.srec))[(int64_t)e * 4 + sd];
+            rec[j] = make_int4(v.x, v.y, v.z, v.w);
+        }
     }
     int4* RS = reinterpret_cast<int4*>(P + C::ESZ);
     auto early = [&]() {
@@ -350,6 +358,8 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
     auto faces = [&](double* FS) {
         const FluxConsts fc{g, S.geomFace, S.tol, S.ES};
         const GatherConsts gc{S.bc, S.h_zero};
+        const gptr<const double> nxp = glob(S.nx), nyp = glob(S.ny), scp = glob(S.scal), bLp = glob(S.bL),
+                                 bRp = glob(S.bR);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int4 R4 = RS[s];
@@ -363,11 +373,11 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
             const int rv = E[4];
             const int ir = (1 - rv) * (NGL - 1 - i) + rv * i;
             const int64_t t = (int64_t)f * NGL + i, tR = (int64_t)f * NGL + ir;
-            const double nx = S.nx[t], ny = S.ny[t], sc = S.scal[t];
-            const FaceState L = gathered(qi, NGL, E, true, i, nx, ny, gc, S.bL[t], S.bR[t]);
-            const FaceState R = gathered(qi, NGL, E, false, ir, S.nx[tR], S.ny[tR], gc, S.bL[tR], S.bR[tR]);
+            const double nx = nxp[t], ny = nyp[t], sc = scp[t];
+            const FaceState L = gathered(qi, NGL, E, true, i, nx, ny, gc, bLp[t], bRp[t]);
+            const FaceState R = gathered(qi, NGL, E, false, ir, nxp[tR], nyp[tR], gc, bLp[tR], bRp[tR]);
             double F0, F1, F2;
-            numerical_flux(S.num_flux, fc, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, S.bL[t], S.bR[tR], F0, F1,
+            numerical_flux(S.num_flux, fc, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, bLp[t], bRp[tR], F0, F1,
                            F2);
             const double nms = ((w >> 6) & 1u) ? -1.0 : 1.0;   // -EMS
             FS[(s * 3 + 0) * NGL + r] = nms * (sc * S.geomFace * F0);
@@ -377,9 +387,9 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
     };
 
     __syncthreads();   // sD written
-    stage_core<NGL, AV>(e, r, act, P, faces, early, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, S.clock[0], last, qi, qo,
-                        S.gRK, S.Yeta, S.Xeta, S.Yxi, S.Xxi, S.Jac, S.Bx, S.By, S.EleSizes, S.GLw, S.lmax, &S, mark,
-                        sD);
+    stage_core<NGL, AV>(e, r, act, P, faces, early, g, S.tol, S.zero_tol, S.positivity, rkA, rkC, glob(S.clock)[0], last,
+                        qi, qo, glob(S.gRK), glob(S.Yeta), glob(S.Xeta), glob(S.Yxi), glob(S.Xxi), glob(S.Jac),
+                        glob(S.Bx), glob(S.By), glob(S.EleSizes), glob(S.GLw), glob(S.lmax), &S, mark, sD);
 }
 
 template <int LO, int HI, bool AV>
@@ -522,12 +532,12 @@ struct MaxRowLds<HI, HI> {
 template <int NGL>
 __device__ __forceinline__ void ens_sensor_body(const StageScn& S, int e0, int src, int m, bool stage0, double* sV,
                                                 double* sT, double (*sSum)[4], unsigned long long* sMax) {
-    if (threadIdx.x < NGL * NGL) sV[threadIdx.x] = S.VdmInv[threadIdx.x];
-    const double v = sensor_rows<NGL>(S.Nelem, e0, S.sc, S.q[src], S.B, sV, sT, sSum);
+    if (threadIdx.x < NGL * NGL) sV[threadIdx.x] = glob(S.VdmInv)[threadIdx.x];
+    const double v = sensor_rows<NGL>(S.Nelem, e0, S.sc, glob((const double*)S.q[src]), glob(S.B), sV, sT, sSum);
     const int el = threadIdx.x / NGL, r = threadIdx.x % NGL;
     const int e = e0 + el;
     if (el < RowBlock<NGL>::EPB && e < S.Nelem && r == 0) {
-        S.visc[e] = v;
+        glob(S.visc)[e] = v;
         if (v != 0.0) mark_reach(S, e, m);
         // non-negative doubles order as their bit patterns; a NaN stays out, as fmax leaves it
         if (stage0 && v > 0.0) atomicMax(sMax, (unsigned long long)__double_as_longlong(v));
