@@ -63,25 +63,36 @@ struct FaceState {
     double h, hu, hv;
 };
 
-// Trace of face node k as CollectEdgeData stores it into qL (left = true) or qR: the element side
-// traces (SolidWalls.okl:74-132) with the exterior state of a boundary face from the wall mirror
-// (SolidWalls.okl:133-175), zero momentum (InnerOuter.okl) or the ocean rule (OceanBoundary.okl:
-// still water h_zero - b where b <= 6.7, wall otherwise).  bl, br: the face's bL, bR at node k.
-// (QP: a pointer to const double, generic or gptr.)
+// The two element side traces of face node k (E: the face's EdgeData), read whether or not the
+// side is an element: an absent side reads the other side's element instead (or element 0 if
+// neither is one), so every address is valid; its values are not used.  The loads need no branch,
+// so they issue together with whatever the caller loads beside them.  (QP: a pointer to const
+// double, generic or gptr; side_node stays inside the element for any side value.)
 template <class QP>
-__device__ __forceinline__ FaceState gathered(QP __restrict__ q, int ngl, const int* E, bool left, int k,
-                                              double nx, double ny, const GatherConsts& gc, double bl, double br) {
+__device__ __forceinline__ void face_side_traces(QP __restrict__ q, int ngl, const int* E, int k, FaceState& Lq,
+                                                 FaceState& Rq) {
     const int ngl2 = ngl * ngl;
-    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3], cpuL = E[5], cpuR = E[6];
+    const int iel = E[0], ier = E[1], isl = E[2], isr = E[3];
+    const bool hl = E[5] == 0, hr = E[6] == 0;
+    const int64_t el = hl ? iel : (hr ? ier : 0), er = hr ? ier : (hl ? iel : 0);
+    const int64_t idl = el * ngl2 * 3 + side_node(ngl, hl ? isl : isr, k);
+    const int64_t idr = er * ngl2 * 3 + side_node(ngl, hr ? isr : isl, k);
+    Lq = FaceState{q[idl], q[idl + ngl2], q[idl + 2 * ngl2]};
+    Rq = FaceState{q[idr], q[idr + ngl2], q[idr + 2 * ngl2]};
+}
+
+// Trace of face node k as CollectEdgeData stores it into qL (left = true) or qR, from the element
+// side traces Lq, Rq at that node (SolidWalls.okl:74-132), with the exterior state of a boundary
+// face from the wall mirror (SolidWalls.okl:133-175), zero momentum (InnerOuter.okl) or the ocean
+// rule (OceanBoundary.okl: still water h_zero - b where b <= 6.7, wall otherwise).  bl, br: the
+// face's bL, bR at node k.
+__device__ __forceinline__ FaceState gathered_from(const FaceState& Lq, const FaceState& Rq, const int* E, bool left,
+                                                   double nx, double ny, const GatherConsts& gc, double bl,
+                                                   double br) {
+    const int cpuL = E[5], cpuR = E[6];
     FaceState L{0.0, 0.0, 0.0}, R{0.0, 0.0, 0.0};
-    if (cpuL == 0) {
-        const int64_t id = (int64_t)iel * ngl2 * 3 + side_node(ngl, isl, k);
-        L = FaceState{q[id], q[id + ngl2], q[id + 2 * ngl2]};
-    }
-    if (cpuR == 0) {
-        const int64_t id = (int64_t)ier * ngl2 * 3 + side_node(ngl, isr, k);
-        R = FaceState{q[id], q[id + ngl2], q[id + 2 * ngl2]};
-    }
+    if (cpuL == 0) L = Lq;
+    if (cpuR == 0) R = Rq;
     if (gc.bc == ESDG_BC_INNER_OUTER) {
         if (cpuL == -1) { L.h = R.h; L.hu = 0.0; L.hv = 0.0; }
         if (cpuR == -1) { R.h = L.h; R.hu = 0.0; R.hv = 0.0; }
@@ -99,6 +110,16 @@ __device__ __forceinline__ FaceState gathered(QP __restrict__ q, int ngl, const 
         if (cpuR == -1) { R.h = L.h; wall_mirror(nx, ny, L.hu, L.hv, R.hu, R.hv); }
     }
     return left ? L : R;
+}
+
+// Trace of face node k as CollectEdgeData stores it into qL (left = true) or qR, read from the
+// state q.
+template <class QP>
+__device__ __forceinline__ FaceState gathered(QP __restrict__ q, int ngl, const int* E, bool left, int k,
+                                              double nx, double ny, const GatherConsts& gc, double bl, double br) {
+    FaceState Lq, Rq;
+    face_side_traces(q, ngl, E, k, Lq, Rq);
+    return gathered_from(Lq, Rq, E, left, nx, ny, gc, bl, br);
 }
 
 // Side traces of the state in face slots: the traces of the element side that is face f's left

@@ -373,12 +373,16 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
             const int rv = E[4];
             const int ir = (1 - rv) * (NGL - 1 - i) + rv * i;
             const int64_t t = (int64_t)f * NGL + i, tR = (int64_t)f * NGL + ir;
-            const double nx = nxp[t], ny = nyp[t], sc = scp[t];
-            const FaceState L = gathered(qi, NGL, E, true, i, nx, ny, gc, bLp[t], bRp[t]);
-            const FaceState R = gathered(qi, NGL, E, false, ir, nxp[tR], nyp[tR], gc, bLp[tR], bRp[tR]);
+            // every load of the side first (one batch), then the trace rules
+            FaceState Li, Ri, Lr, Rr;
+            face_side_traces(qi, NGL, E, i, Li, Ri);
+            face_side_traces(qi, NGL, E, ir, Lr, Rr);
+            const double nx = nxp[t], ny = nyp[t], sc = scp[t], bl = bLp[t], br = bRp[t];
+            const double nxr = nxp[tR], nyr = nyp[tR], blr = bLp[tR], brr = bRp[tR];
+            const FaceState L = gathered_from(Li, Ri, E, true, nx, ny, gc, bl, br);
+            const FaceState R = gathered_from(Lr, Rr, E, false, nxr, nyr, gc, blr, brr);
             double F0, F1, F2;
-            numerical_flux(S.num_flux, fc, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, bLp[t], bRp[tR], F0, F1,
-                           F2);
+            numerical_flux(S.num_flux, fc, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, bl, brr, F0, F1, F2);
             const double nms = ((w >> 6) & 1u) ? -1.0 : 1.0;   // -EMS
             FS[(s * 3 + 0) * NGL + r] = nms * (sc * S.geomFace * F0);
             FS[(s * 3 + 1) * NGL + r] = nms * (sc * S.geomFace * F1);
