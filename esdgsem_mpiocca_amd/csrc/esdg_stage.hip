@@ -372,15 +372,18 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
                               (int)((w >> 8) & 255u) - 2, (int)((w >> 16) & 255u) - 2, 0};
             const int rv = E[4];
             const int ir = (1 - rv) * (NGL - 1 - i) + rv * i;
-            const int64_t t = (int64_t)f * NGL + i, tR = (int64_t)f * NGL + ir;
-            // every load of the side first (one batch), then the trace rules
-            FaceState Li, Ri, Lr, Rr;
+            const int64_t t = (int64_t)f * NGL + i;
+            // every load of the side first (one batch), then the trace rules.  The element's NGL
+            // lanes (one wave) hold the face's NGL nodes: each lane also evaluates the right trace
+            // at its own node, and lane r takes node ir's from the lane holding it
+            FaceState Li, Ri;
             face_side_traces(qi, NGL, E, i, Li, Ri);
-            face_side_traces(qi, NGL, E, ir, Lr, Rr);
             const double nx = nxp[t], ny = nyp[t], sc = scp[t], bl = bLp[t], br = bRp[t];
-            const double nxr = nxp[tR], nyr = nyp[tR], blr = bLp[tR], brr = bRp[tR];
             const FaceState L = gathered_from(Li, Ri, E, true, nx, ny, gc, bl, br);
-            const FaceState R = gathered_from(Lr, Rr, E, false, nxr, nyr, gc, blr, brr);
+            const FaceState Ro = gathered_from(Li, Ri, E, false, nx, ny, gc, bl, br);
+            const int src = lane - r + (rev * ir + (1 - rev) * (NGL - 1 - ir));
+            const FaceState R{__shfl(Ro.h, src), __shfl(Ro.hu, src), __shfl(Ro.hv, src)};
+            const double brr = __shfl(br, src);
             double F0, F1, F2;
             numerical_flux(S.num_flux, fc, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, bl, brr, F0, F1, F2);
             const double nms = ((w >> 6) & 1u) ? -1.0 : 1.0;   // -EMS
