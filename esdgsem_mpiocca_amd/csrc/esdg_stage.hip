@@ -360,37 +360,56 @@ __device__ __forceinline__ void stage_body(const StageScn& S, int e0, double* ld
         const GatherConsts gc{S.bc, S.h_zero};
         const gptr<const double> nxp = glob(S.nx), nyp = glob(S.ny), scp = glob(S.scal), bLp = glob(S.bL),
                                  bRp = glob(S.bR);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        // one side's decoded record and loads: side s+1's loads are issued before side s's flux
+        // is evaluated, so a wave waits for memory once per element rather than once per side
+        struct Side {
+            int E[8];
+            int ir, rev;     // the reversed face node (EdgeData[4]) of the lane's node; EOR
+            unsigned w;
+            FaceState Li, Ri;
+            double nx, ny, sc, bl, br;
+        };
+        auto load = [&](int s, Side& d) {
             const int4 R4 = RS[s];
             const int f = R4.x;
-            const unsigned w = (unsigned)R4.w;
-            const int rev = (int)((w >> 5) & 1u);
-            const int i = rev * r + (1 - rev) * (NGL - 1 - r);      // face node of side node r
+            d.w = (unsigned)R4.w;
+            d.rev = (int)((d.w >> 5) & 1u);
+            const int i = d.rev * r + (1 - d.rev) * (NGL - 1 - r);      // face node of side node r
             // EdgeData's entries as gathered() reads them
-            const int E[8] = {R4.y, R4.z, (int)(w & 3u), (int)((w >> 2) & 3u), (int)((w >> 4) & 1u),
-                              (int)((w >> 8) & 255u) - 2, (int)((w >> 16) & 255u) - 2, 0};
-            const int rv = E[4];
-            const int ir = (1 - rv) * (NGL - 1 - i) + rv * i;
+            d.E[0] = R4.y; d.E[1] = R4.z; d.E[2] = (int)(d.w & 3u); d.E[3] = (int)((d.w >> 2) & 3u);
+            d.E[4] = (int)((d.w >> 4) & 1u); d.E[5] = (int)((d.w >> 8) & 255u) - 2;
+            d.E[6] = (int)((d.w >> 16) & 255u) - 2; d.E[7] = 0;
+            const int rv = d.E[4];
+            d.ir = (1 - rv) * (NGL - 1 - i) + rv * i;
             const int64_t t = (int64_t)f * NGL + i;
-            // every load of the side first (one batch), then the trace rules.  The element's NGL
-            // lanes (one wave) hold the face's NGL nodes: each lane also evaluates the right trace
-            // at its own node, and lane r takes node ir's from the lane holding it
-            FaceState Li, Ri;
-            face_side_traces(qi, NGL, E, i, Li, Ri);
-            const double nx = nxp[t], ny = nyp[t], sc = scp[t], bl = bLp[t], br = bRp[t];
-            const FaceState L = gathered_from(Li, Ri, E, true, nx, ny, gc, bl, br);
-            const FaceState Ro = gathered_from(Li, Ri, E, false, nx, ny, gc, bl, br);
-            const int src = lane - r + (rev * ir + (1 - rev) * (NGL - 1 - ir));
+            face_side_traces(qi, NGL, d.E, i, d.Li, d.Ri);
+            d.nx = nxp[t]; d.ny = nyp[t]; d.sc = scp[t]; d.bl = bLp[t]; d.br = bRp[t];
+        };
+        // The element's NGL lanes (one wave) hold the face's NGL nodes: each lane also evaluates
+        // the right trace at its own node, and lane r takes node ir's from the lane holding it
+        auto flux = [&](int s, const Side& d) {
+            const FaceState L = gathered_from(d.Li, d.Ri, d.E, true, d.nx, d.ny, gc, d.bl, d.br);
+            const FaceState Ro = gathered_from(d.Li, d.Ri, d.E, false, d.nx, d.ny, gc, d.bl, d.br);
+            const int src = lane - r + (d.rev * d.ir + (1 - d.rev) * (NGL - 1 - d.ir));
             const FaceState R{__shfl(Ro.h, src), __shfl(Ro.hu, src), __shfl(Ro.hv, src)};
-            const double brr = __shfl(br, src);
+            const double brr = __shfl(d.br, src);
             double F0, F1, F2;
-            numerical_flux(S.num_flux, fc, E[6], nx, ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, bl, brr, F0, F1, F2);
-            const double nms = ((w >> 6) & 1u) ? -1.0 : 1.0;   // -EMS
-            FS[(s * 3 + 0) * NGL + r] = nms * (sc * S.geomFace * F0);
-            FS[(s * 3 + 1) * NGL + r] = nms * (sc * S.geomFace * F1);
-            FS[(s * 3 + 2) * NGL + r] = nms * (sc * S.geomFace * F2);
-        }
+            numerical_flux(S.num_flux, fc, d.E[6], d.nx, d.ny, L.h, L.hu, L.hv, R.h, R.hu, R.hv, d.bl, brr, F0, F1,
+                           F2);
+            const double nms = ((d.w >> 6) & 1u) ? -1.0 : 1.0;   // -EMS
+            FS[(s * 3 + 0) * NGL + r] = nms * (d.sc * S.geomFace * F0);
+            FS[(s * 3 + 1) * NGL + r] = nms * (d.sc * S.geomFace * F1);
+            FS[(s * 3 + 2) * NGL + r] = nms * (d.sc * S.geomFace * F2);
+        };
+        Side a, b;
+        load(0, a);
+        load(1, b);
+        flux(0, a);
+        load(2, a);
+        flux(1, b);
+        load(3, b);
+        flux(2, a);
+        flux(3, b);
     };
 
     __syncthreads();   // sD written
