@@ -352,6 +352,20 @@ def test_ism_mesh_bump_matches_oracle(dev, fused):
     assert h <= 1e-12 and hu <= 1e-12 and hv <= 1e-12, (h, hu, hv)
 
 
+@pytest.mark.parametrize("N", [4, 6])
+def test_ism_mesh_single_kernel_equals_fused_bitwise(dev, N):
+    """On the curved ISM-V2 mesh (83 faces whose two sides run in opposite directions), at degrees
+    whose line of nodes does not divide the wave (5 and 7 nodes), the single-kernel stage, which
+    takes the reversed node's right trace from the lane holding it, equals the fused schedule,
+    which evaluates each face once, bit for bit."""
+    case = Case.from_ism(ISM_MESH, N=N, case_kind=S.CASE_GAUSSIAN_BUMP, amp=0.05)
+    rp = RunParams(num_flux=3)
+    qa, na, ta = run_gpu(dev, case, rp, 15, fused=1)
+    qb, nb, tb = run_gpu(dev, case, rp, 15, fused=2)
+    assert na == nb == 15 and ta == tb
+    assert np.array_equal(qa, qb)
+
+
 def test_ism_mesh_lake_at_rest(dev):
     """Lake at rest on the ISM mesh: the same state as the oracle's, and momentum bounded by what
     this mesh's geometry allows.  Its curved Coons patches preserve even a flat free stream only to
