@@ -71,8 +71,19 @@ struct SurfIn {
     double zero_tol;
 };
 
+// The LDS of one block of the update pass (one wave): the element planes, D (later the sensor's
+// VdmInv), and the sensor's and wave speed's reductions.
+template <int NGL>
+struct VolLds {
+    __attribute__((aligned(16))) double s[VolCfg<NGL>::EPW * VolCfg<NGL>::ESZ];
+    double sM[VolCfg<NGL>::NN];
+    double sSum[VolCfg<NGL>::EPW][4];
+    double sLam[64];
+};
+
+// The update pass over block blk of its grid (a kernel's whole body: a block that returns is done).
 template <int NGL, int MODE>
-__global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, const double* __restrict__ Jac,
+__device__ __forceinline__ void volume_body(int blk, VolLds<NGL>& L, int Nelem, double g, const double* __restrict__ Jac,
                                                         const double* __restrict__ Yxi, const double* __restrict__ Yeta,
                                                         const double* __restrict__ Xxi, const double* __restrict__ Xeta,
                                                         double* q, const double* __restrict__ Dg,
@@ -87,17 +98,17 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             cdt = si.Cs * d;
         }
     }
-    __shared__ __attribute__((aligned(16))) double s[C::EPW * C::ESZ];
+    double* s = L.s;
     // D in LDS: its entries are read with wave-uniform addresses (broadcast), which keeps the
     // 64-entry matrix out of the scalar registers (no SGPR spills into vector registers).  After
     // the column sweep the same slots take the sensor's VdmInv (the block is one wave, whose LDS
     // accesses execute in program order).
-    __shared__ double sM[C::NN];
+    double* sM = L.sM;
     const int lane = threadIdx.x;
     for (int t = lane; t < C::NN; t += 64) sM[t] = Dg[t];
     const double* D = sM;
     const int el = lane / NGL, r = lane % NGL;
-    int e = blockIdx.x * C::EPW + el;
+    int e = blk * C::EPW + el;
     bool act = (el < C::EPW) && (e < Nelem);
     int seq = 0;
     if constexpr (MODE == VOL_SURF_RK || MODE == VOL_SURF_VISC_RK) {
@@ -107,7 +118,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
         if (si.list) {
             // blocks past the end of the list do nothing (the count is the same for every lane)
             const int cnt = min(si.acount[seq & 1], Nelem);
-            if ((int)blockIdx.x * C::EPW >= cnt) return;
+            if (blk * C::EPW >= cnt) return;
             act = (el < C::EPW) && (e < cnt);
             e = act ? si.list[e] : 0;
         }
@@ -492,7 +503,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             // energy sums.  Plane 3 (1/h) is free after the column phase and holds T and then the
             // squares.  The side traces above are the last reader of planes 0-2.
             if (si.visc_next) {
-                __shared__ double sSum[C::EPW][4];
+                double (*sSum)[4] = L.sSum;
                 double* Tm = S + 3 * C::PLANE;
                 const double* V = sV;
                 if (act) {
@@ -532,7 +543,7 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
                 }
             }
             if (si.lam) {
-                __shared__ double sLam[64];
+                double* sLam = L.sLam;
                 sLam[lane] = lam;
                 __syncthreads();
                 if (act && r == 0) {
@@ -544,6 +555,35 @@ __global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, 
             }
         }
     }
+}
+
+template <int NGL, int MODE>
+__global__ void __launch_bounds__(64, 2) k_volume_fluxdiff(int Nelem, double g, const double* __restrict__ Jac,
+                                                        const double* __restrict__ Yxi, const double* __restrict__ Yeta,
+                                                        const double* __restrict__ Xxi, const double* __restrict__ Xeta,
+                                                        double* q, const double* __restrict__ Dg,
+                                                        const double* __restrict__ Bx, const double* __restrict__ By,
+                                                        double* __restrict__ Qt, SurfIn si) {
+    __shared__ VolLds<NGL> L;
+    volume_body<NGL, MODE>(blockIdx.x, L, Nelem, g, Jac, Yxi, Yeta, Xxi, Xeta, q, Dg, Bx, By, Qt, si);
+}
+
+// Both update launches of a stage with artificial viscosity as one: blocks [0, nl) take the listed
+// marked elements (VOL_SURF_VISC_RK), the others the unmarked elements (VOL_SURF_RK).  The list's
+// blocks come first, so its few working blocks start with the launch; its blocks past the list's
+// end exit at once.  One launch instead of two on two streams: no fork and join per stage.
+template <int NGL>
+__global__ void __launch_bounds__(64, 2) k_volume_split(int nl, int Nelem, double g, const double* __restrict__ Jac,
+                                                        const double* __restrict__ Yxi, const double* __restrict__ Yeta,
+                                                        const double* __restrict__ Xxi, const double* __restrict__ Xeta,
+                                                        double* q, const double* __restrict__ Dg,
+                                                        const double* __restrict__ Bx, const double* __restrict__ By,
+                                                        double* __restrict__ Qt, SurfIn si) {
+    __shared__ VolLds<NGL> L;
+    if ((int)blockIdx.x < nl)
+        volume_body<NGL, VOL_SURF_VISC_RK>(blockIdx.x, L, Nelem, g, Jac, Yxi, Yeta, Xxi, Xeta, q, Dg, Bx, By, Qt, si);
+    else
+        volume_body<NGL, VOL_SURF_RK>((int)blockIdx.x - nl, L, Nelem, g, Jac, Yxi, Yeta, Xxi, Xeta, q, Dg, Bx, By, Qt, si);
 }
 
 // Standard strong-form volume term (okl/DG/VolumeKernelV99.okl:17-167), FluxDifferencing = 0.
@@ -638,21 +678,20 @@ void launch_volume_fused(esdg_ctx* ctx, const FusedArgs& a, bool rk, double rkA,
         si.lam = a.next_lambda;
     }
     if (rk && Qv) {
-        // the listed marked elements on the side stream, beside the unmarked ones on the main stream
-        // (disjoint sets, each updated once; nothing else is read or written by both), joined back
-        // before anything that follows on the main stream (without a second stream, created with
-        // the context, both launches stay on the main stream, in order)
-        hipStream_t ls = ctx->side ? ctx->side : ctx->stream;
-        if (ctx->side) {
-            (void)hipEventRecord(ctx->fork, ctx->stream);
-            (void)hipStreamWaitEvent(ctx->side, ctx->fork, 0);
+        // the listed marked elements and the unmarked ones (disjoint sets, each updated once;
+        // nothing else is read or written by both) in one launch
+        // (at ngl = 10 the two bodies in one kernel spill far more than each alone: two launches
+        // there, in order)
+        const unsigned nb = blocks_for(a.Nelem, C::EPW);
+        if constexpr (NGL <= 9) {
+            k_volume_split<NGL><<<2 * nb, 64, 0, ctx->stream>>>(
+                (int)nb, a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
+        } else {
+            k_volume_fluxdiff<NGL, VOL_SURF_VISC_RK><<<nb, 64, 0, ctx->stream>>>(
+                a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
+            k_volume_fluxdiff<NGL, VOL_SURF_RK><<<nb, 64, 0, ctx->stream>>>(
+                a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
         }
-        k_volume_fluxdiff<NGL, VOL_SURF_VISC_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ls>>>(
-            a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
-        if (ctx->side) (void)hipEventRecord(ctx->join, ctx->side);
-        k_volume_fluxdiff<NGL, VOL_SURF_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
-            a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
-        if (ctx->side) (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
     } else if (rk)
         k_volume_fluxdiff<NGL, VOL_SURF_RK><<<blocks_for(a.Nelem, C::EPW), 64, 0, ctx->stream>>>(
             a.Nelem, ctx->p.g, a.Jac, a.Yxi, a.Yeta, a.Xxi, a.Xeta, a.q, a.D, a.Bx, a.By, nullptr, si);
