@@ -15,10 +15,9 @@
  *
  * Thread-safety: one context is used from one host thread at a time (the reference's one rank
  * per device).  All of a context's work is ordered on its stream, esdg_stream(ctx).  Some work
- * runs on a second stream that the context owns: with artificial viscosity the fused update of
- * the marked elements, with the learned indicator its pass beside the face fluxes, and in a
- * batched ensemble with both learned and modal-sensor scenarios the latter's passes and stage
- * launches.  That work is forked from and joined back into the main stream inside the same call,
+ * runs on a second stream that the context owns: with the learned indicator its pass beside the
+ * face fluxes, and in a batched ensemble with both learned and modal-sensor scenarios the latter's
+ * passes and stage launches.  That work is forked from and joined back into the main stream inside the same call,
  * so synchronising esdg_stream(ctx) (or esdg_finish) covers it.
  */
 #ifndef ESDG_H
